@@ -1,0 +1,163 @@
+"""Benchmark: Poisson (self-gravity) solves per second at 258 x 130 x 512.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--grid R Z P] [--warm]
+
+A step is one potential_solver call (cold start: 20 ADI steps from zero, the
+reference's setting for a solve without a previous potential) on a seeded
+binary of n = 1 polytropes already resident in HBM, enqueued through the C
+ABI on one stream.  Prints one JSON line.  One GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Poisson solves/s @258×130×512 mixed vs FP64 at relL2≤1e-8; p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def rell2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def cpu_baseline(numr, numz, numphi, rho, threads):
+    """The oracle (C restatement, OpenMP) timed on one cold solve of the same
+    grid; Green's tables zero-filled because their generation is one-time
+    setup (potsetup), which the reference's timer excludes too."""
+    from oracle import oracle as O
+    O.set_threads(threads)
+    plan = O.OraclePlan(numr, numz, numphi, with_tables="zero")
+    t0 = time.perf_counter()
+    plan.solve(rho, tstep=0)
+    dt = time.perf_counter() - t0
+    del plan
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"1 cold solve {numr}x{numz}x{numphi} (oracle/bh_oracle.c, OpenMP, "
+                      f"tables zero-filled), {dt:.2f} s"}
+
+
+def reference_slices_error(pot):
+    path = os.path.join(ROOT, "tests", "golden", "c2_258x130x512.npz")
+    if not os.path.exists(path) or pot.shape != (512, 130, 258):
+        return None
+    g = np.load(path)
+    parts = [(pot[:, 64:66, :], g["mid1"]), (pot[[0, 256], :, :], g["mer1"]), (pot[:, -1, :], g["top1"]),
+             (pot[:, 0, :], g["bot1"]), (pot[:, :, -1], g["side1"])]
+    num = sum(float(np.sum((a - b) ** 2)) for a, b in parts)
+    den = sum(float(np.sum(b ** 2)) for _, b in parts)
+    return (num / den) ** 0.5
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", type=int, nargs=3, default=[258, 130, 512])
+    ap.add_argument("--warm", action="store_true", help="time warm solves (tstep > 1, 5 ADI steps)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from bipoly_hydro_amd import density
+    from bipoly_hydro_amd import potential as P
+
+    numr, numz, numphi = args.grid
+    dev = torch.device("cuda:0")
+    if (numr, numz, numphi) == (258, 130, 512):
+        rho = density.config2_density(seed=1234)
+    else:
+        rho = density.scaled_binary(numr, numz, numphi, seed=1234)
+    d_rho = torch.from_numpy(rho).to(dev)
+    d_pot = torch.empty_like(d_rho)
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    tstep = 2 if args.warm else 0
+
+    plan = P.PotentialPlan(numr, numz, numphi, precision="fp64")
+    # accuracy of the timed configuration against the reference's own output
+    plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
+    torch.cuda.synchronize()
+    pot_cold = d_pot.cpu().numpy()
+    ref_err = reference_slices_error(pot_cold)
+
+    # stage timings of one solve (HIP events on the solve's stream)
+    plan.set_timing(True)
+    plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
+    stages = plan.last_timings()
+    plan.set_timing(False)
+
+    for _ in range(args.warmup):
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(b) for a, b in ev])
+    ms_per_step = wall * 1e3 / args.steps
+
+    # ADI half-step kernel: algorithmic bytes = 3 streams (read phi, read
+    # rho-hat, write phi) x 8 B x interior points x modes
+    nsteps = 5 if args.warm else 20
+    interior = (numr - 2) * (numz - 2) * numphi
+    bytes_per_launch = 3 * 8 * interior
+    launch_ms = stages["adi"] / (2 * nsteps)
+    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(numr, numz, numphi, rho, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": 1e3 / ms_per_step,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"potential_solver {'warm (5 ADI steps)' if args.warm else 'cold (20 ADI steps)'}"
+                               f" {numr}x{numz}x{numphi}, isym=1, mmax=30, binary n=1 polytropes, "
+                               "inputs resident in HBM",
+                   "grid": [numr, numz, numphi], "precision": "fp64", "tstep": tstep},
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "relL2_vs_reference_slices": ref_err,
+        "stage_ms": stages,
+        "roofline": {"bound": "hbm", "kernel": "line_sweep_kernel (ADI half-step)",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms},
+        "cpu_baseline": cpu,
+        "mixed": None,
+    }
+    plan.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,87 @@
+// bh_internal.h -- declarations shared by the plan code and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+
+namespace bh {
+
+// Geometry on the device side.  J-layout: [mode/phi][K][J], J fastest, pitch
+// numr (identical to the reference's (numr, numz, numphi) column-major array).
+// K-layout: [mode][J][K], K fastest, pitch numz.
+struct Geo {
+    int numr, numz, numphi, mmax;
+    int nri, nzi;  // interior counts numr-2, numz-2
+    double dr, dz, dphi, gamma, four_pi, piinv;
+};
+
+// Line-sweep arguments for one ADI half-step (helmadi.F90:195-353).
+struct LineArgs {
+    const double *xin;   // potential, input layout
+    double *xout;        // potential, output (transposed) layout
+    const double *rh;    // -> rho-hat, input layout
+    long SA, PA;         // input: plane stride, row pitch
+    long SB, PB;         // output: plane stride, row pitch
+    int nlines, len;     // interior lines / elements per line
+    const double *alpha, *ibet, *gback;  // Thomas factors, per element
+    long coef_stride;    // per-mode stride of the factors (0: shared by all modes)
+    int dir;             // 0 radial (lines = K, elements = J), 1 vertical
+    double dt, gamma, four_pi;
+    const double *alphar, *betar;  // [numr + 1], 1-based
+    const double *elzb;            // [numphi][numr + 1], 1-based J
+    int numr;
+};
+
+// GEMM epilogue: rows [0, rsplit) -> c0 (optional masking of the layers the
+// reference zeroes in rhop), rows [rsplit, R) -> c1.
+struct GemmOut {
+    double *c0; long ldc0;
+    double *c1; long ldc1;
+    int rsplit;
+    int mask_numr, mask_numz;  // 0 = no mask
+};
+
+void launch_gemm_f64(const double *A, long lda, const double *B, long ldb, int R, int M, int Kd,
+                     const GemmOut &out, hipStream_t s);
+
+void launch_fill(double *p, size_t n, double v, hipStream_t s);
+
+void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int nplanes,
+                         hipStream_t s);
+
+void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s);
+
+void launch_axis(double *pot, int numr, int numz, int numphi, hipStream_t s);
+
+// Green's tables ------------------------------------------------------------
+struct TableArgs {
+    int numr, numz, mmax, nser;
+    double hypr_cutoff, piinv, dz;
+    const double *rhf_g;    // [numr + 1] 1-based
+    const double *rhfinv_g; // [numr + 1]
+    const double *zhf_g;    // [numz + 1]
+    const double *coefh;    // [mmax]
+    const double *cser;     // [mmax][nser] = exp(dcoefh)
+    const double *nu;       // [mmax + 1] 1-based
+};
+// top table T[m][jg-2][kz][jr], jg = 2..numr-1
+void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s);
+// side table in Toeplitz form S[m][d][jr], d = kg - kl + nzi - 1
+void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s);
+
+// boundary contraction and face rebuild (bessel.F90:176-296, 396-488)
+// proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
+// Stb: [mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
+// Ss:  [mmax][2][nzi]
+void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz,
+                            int mmax, hipStream_t s);
+void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz,
+                          int mmax, hipStream_t s);
+// faces F[L][fi]: fi in [0,nri) top J=fi+2, [nri,2nri) bottom, [2nri,2nri+nzi) side
+void launch_face_rebuild(const double *Stb, const double *Ss, const double *besc, const double *bess,
+                         double *F, int numr, int numz, int numphi, int mmax, double factor,
+                         hipStream_t s);
+// scatter Fourier-space faces FF[mode][fi] into both layouts
+void launch_face_scatter(const double *FF, double *XJ, double *XK, int numr, int numz, int numphi,
+                         hipStream_t s);
+
+}  // namespace bh

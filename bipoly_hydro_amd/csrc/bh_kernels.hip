@@ -1,0 +1,601 @@
+// bh_kernels.hip -- CDNA4 (gfx950) device code of the self-gravity solve, FP64.
+//
+//   gemm_f64      dense phi-DFT / inverse DFT / boundary projection as one
+//                 matrix product on v_mfma_f64_16x16x4f64 (realft.F90 and the
+//                 projection loops of bessel.F90:179-185, 228-237)
+//   line_sweep    one ADI half-step: RHS stencil + Thomas solve with one
+//                 wavefront per tridiagonal line (helmadi.F90:201-343,
+//                 tridagr.F90, tridagz.F90), written transposed so the next
+//                 half-step reads its lines contiguously (the on-device form
+//                 of swap_r_z / swap_z_r)
+//   gen_*_table   Green's-function tables (tm.F90 / sm.F90 / bm.F90)
+//   contract_*    boundary contraction (bessel.F90:186-262)
+//   face_*        boundary face rebuild and placement (bessel.F90:396-488)
+#include "bh_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace bh {
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM on MFMA:  C[R x M] = A[R x Kd] * B[Kd x M]
+// Tile 64 x 64 per workgroup of 256 threads; each wave owns a 32 x 32 block
+// = 2 x 2 MFMA 16x16 tiles; K staged through LDS in chunks of 16.
+// v_mfma_f64_16x16x4f64 operand layout (lane l): A[l%16][l/16],
+// B[l/16][l%16]; D element v of lane l is D[4*v + l/16][l%16] (measured on
+// gfx950 with a one-hot operand probe).
+// ---------------------------------------------------------------------------
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+constexpr int GBM = 64, GBN = 64, GBK = 16;
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict__ A, long lda,
+                                                        const double *__restrict__ B, long ldb, int R,
+                                                        int M, int Kd, GemmOut out)
+{
+    __shared__ double As[GBK][GBM + 1];
+    __shared__ double Bs[GBK][GBN + 1];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int row0 = blockIdx.y * GBM, col0 = blockIdx.x * GBN;
+
+    double4_t acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
+
+    for (int k0 = 0; k0 < Kd; k0 += GBK) {
+        // A tile: 64 rows x 16 k -> As[k][row]
+        for (int t = tid; t < GBM * GBK; t += 256) {
+            int r = t / GBK, k = t % GBK;
+            int gr = row0 + r, gk = k0 + k;
+            As[k][r] = (gr < R && gk < Kd) ? A[(long)gr * lda + gk] : 0.0;
+        }
+        // B tile: 16 k x 64 cols -> Bs[k][col]
+        for (int t = tid; t < GBK * GBN; t += 256) {
+            int k = t / GBN, c = t % GBN;
+            int gk = k0 + k, gc = col0 + c;
+            Bs[k][c] = (gk < Kd && gc < M) ? B[(long)gk * ldb + gc] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            double a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wr * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // epilogue
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = row0 + wr * 32 + i * 16 + 4 * v + (lane >> 4);
+                int c = col0 + wc * 32 + j * 16 + (lane & 15);
+                if (r >= R || c >= M) continue;
+                double val = acc[i][j][v];
+                if (r < out.rsplit) {
+                    if (out.mask_numr) {
+                        // rhop zeroing (potential_solver.F90:70-78): K in
+                        // {1,2,numz-1,numz}, J in {numr-1,numr} (1-based)
+                        int J = c % out.mask_numr + 1, K = c / out.mask_numr + 1;
+                        if (J >= out.mask_numr - 1 || K <= 2 || K >= out.mask_numz - 1) val = 0.0;
+                    }
+                    out.c0[(long)r * out.ldc0 + c] = val;
+                } else {
+                    out.c1[(long)(r - out.rsplit) * out.ldc1 + c] = val;
+                }
+            }
+}
+
+void launch_gemm_f64(const double *A, long lda, const double *B, long ldb, int R, int M, int Kd,
+                     const GemmOut &out, hipStream_t s)
+{
+    dim3 grid((M + GBN - 1) / GBN, (R + GBM - 1) / GBM);
+    hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, s, A, lda, B, ldb, R, M, Kd, out);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void fill_kernel(double *p, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+void launch_fill(double *p, size_t n, double v, hipStream_t s)
+{
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+// [plane][K][J] -> [plane][J][K]
+__global__ __launch_bounds__(256) void transpose_jk_kernel(const double *__restrict__ src,
+                                                           double *__restrict__ dst, int numr, int numz)
+{
+    __shared__ double tile[32][33];
+    const long plane = (long)numr * numz;
+    const double *s = src + blockIdx.z * plane;
+    double *d = dst + blockIdx.z * plane;
+    int j0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        int k = k0 + r, j = j0 + tx;
+        if (k < numz && j < numr) tile[r][tx] = s[(long)k * numr + j];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        int j = j0 + r, k = k0 + tx;
+        if (k < numz && j < numr) d[(long)j * numz + k] = tile[tx][r];
+    }
+}
+
+void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int nplanes, hipStream_t s)
+{
+    dim3 grid((numr + 31) / 32, (numz + 31) / 32, nplanes);
+    hipLaunchKernelGGL(transpose_jk_kernel, grid, dim3(256), 0, s, src, dst, numr, numz);
+}
+
+// ---------------------------------------------------------------------------
+// ADI half-step, one wavefront per tridiagonal line.
+// Element e of a line sits in lane e % 64, register slot e / 64 (segment), so
+// every load and store of a segment is 64 consecutive doubles.  The Thomas
+// forward sweep y_e = alpha_e y_{e-1} + ibet_e rhs_e and back substitution
+// x_e = y_e + gback_e x_{e+1} are first-order linear recurrences, evaluated
+// per segment as an inclusive wavefront scan of affine maps, with the value
+// carried between segments.
+// ---------------------------------------------------------------------------
+
+__device__ inline double shfl_up_d(double v, int d) { return __shfl_up(v, d, 64); }
+__device__ inline double shfl_down_d(double v, int d) { return __shfl_down(v, d, 64); }
+__device__ inline double bcast_d(double v, int src) { return __shfl(v, src, 64); }
+
+// LS_LINES lines per workgroup (4 waves); LDS tile [LS_LINES][64*C + 1] doubles
+template <int C, int LS_LINES>
+__global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
+{
+    extern __shared__ double sres[];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int L = blockIdx.y;  // mode plane
+    const int line0 = blockIdx.x * LS_LINES;
+    const int lenp = 64 * C + 1;
+    const double *xin = a.xin + (long)L * a.SA;
+    const double *rh = a.rh + (long)L * a.SA;
+    const double *alpha = a.alpha + (long)L * a.coef_stride;
+    const double *ibet = a.ibet + (long)L * a.coef_stride;
+    const double *gback = a.gback + (long)L * a.coef_stride;
+    const int len = a.len;
+    const double dtt_mtwogamma = a.dt - 2.0 * a.gamma;
+
+    for (int jl = wave; jl < LS_LINES; jl += 4) {
+        const int i = line0 + jl;  // interior line index, row i+1 of the input layout
+        if (i >= a.nlines) break;
+        const double *xp = xin + (long)i * a.PA;        // row i   (previous line)
+        const double *x0 = xin + (long)(i + 1) * a.PA;  // row i+1 (this line)
+        const double *xn = xin + (long)(i + 2) * a.PA;  // row i+2 (next line)
+        const double *r0 = rh + (long)(i + 1) * a.PA;
+        double rhs[C];
+        // line-constant RHS coefficients
+        double cd, cp, cn;
+        if (a.dir == 0) {
+            cd = dtt_mtwogamma; cp = a.gamma; cn = a.gamma;
+        } else {
+            const int J = i + 2;
+            cd = a.elzb[(long)L * (a.numr + 1) + J] + a.dt;
+            cn = -a.alphar[J];
+            cp = (J > 2) ? -a.betar[J] : 0.0;
+        }
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            const int e = 64 * s + lane;
+            double v = 0.0;
+            if (e < len) {
+                const int p = e + 1;
+                if (a.dir == 0) {
+                    // knownr (helmadi.F90:240-257)
+                    v = -a.four_pi * r0[p] + cd * x0[p] + cp * (xn[p] + xp[p]);
+                    if (e == len - 1) v = v - a.alphar[a.numr - 1] * x0[len + 1];
+                } else {
+                    // knownz (helmadi.F90:284-340)
+                    v = -a.four_pi * r0[p] + cd * x0[p] + cn * xn[p];
+                    if (cp != 0.0) v = v + cp * xp[p];
+                    if (e == len - 1) v = v + a.gamma * x0[len + 1];
+                    if (e == 0) v = v + a.gamma * x0[0];
+                }
+            }
+            rhs[s] = v;
+        }
+        // forward sweep
+        double carry = 0.0;
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            const int e = 64 * s + lane;
+            double A = 0.0, B = 0.0;
+            if (e < len) { A = alpha[e]; B = ibet[e] * rhs[s]; }
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                double Ap = shfl_up_d(A, d), Bp = shfl_up_d(B, d);
+                if (lane >= d) { B = A * Bp + B; A = A * Ap; }
+            }
+            double y = A * carry + B;
+            rhs[s] = y;
+            carry = bcast_d(y, 63);
+        }
+        // back substitution
+        carry = 0.0;
+#pragma unroll
+        for (int s = C - 1; s >= 0; --s) {
+            const int e = 64 * s + lane;
+            double A = 0.0, B = 0.0;
+            if (e < len) { A = gback[e]; B = rhs[s]; }
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                double An = shfl_down_d(A, d), Bn = shfl_down_d(B, d);
+                if (lane + d < 64) { B = A * Bn + B; A = A * An; }
+            }
+            double x = A * carry + B;
+            rhs[s] = x;
+            carry = bcast_d(x, 0);
+        }
+#pragma unroll
+        for (int s = 0; s < C; ++s) sres[jl * lenp + 64 * s + lane] = rhs[s];
+    }
+    __syncthreads();
+    // transposed, coalesced write: element e of line i -> out[(e+1)*PB + i+1]
+    const int nl = min(LS_LINES, a.nlines - line0);
+    double *xout = a.xout + (long)L * a.SB;
+    for (int t = threadIdx.x; t < LS_LINES * len; t += 256) {
+        int j = t % LS_LINES, e = t / LS_LINES;
+        if (j < nl) xout[(long)(e + 1) * a.PB + line0 + j + 1] = sres[j * lenp + e];
+    }
+}
+
+void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s)
+{
+    int c = (a.len + 63) / 64;
+    if (c <= 4) {
+        constexpr int NL = 16;
+        dim3 grid((a.nlines + NL - 1) / NL, nplanes);
+        if (c <= 1)
+            hipLaunchKernelGGL((line_sweep_kernel<1, NL>), grid, dim3(256), NL * (64 * 1 + 1) * sizeof(double), s, a);
+        else if (c <= 2)
+            hipLaunchKernelGGL((line_sweep_kernel<2, NL>), grid, dim3(256), NL * (64 * 2 + 1) * sizeof(double), s, a);
+        else
+            hipLaunchKernelGGL((line_sweep_kernel<4, NL>), grid, dim3(256), NL * (64 * 4 + 1) * sizeof(double), s, a);
+    } else {
+        constexpr int NL = 8;  // 8 x 513 doubles = 32.8 KB
+        dim3 grid((a.nlines + NL - 1) / NL, nplanes);
+        hipLaunchKernelGGL((line_sweep_kernel<8, NL>), grid, dim3(256), NL * (64 * 8 + 1) * sizeof(double), s, a);
+    }
+}
+
+// axis ghost by a pi rotation: pot(1,K,L) = pot(2,K,L +- numphi/2)
+// (helmadi.F90:396-405)
+__global__ void axis_kernel(double *pot, int numr, int numz, int numphi)
+{
+    int K = blockIdx.x * blockDim.x + threadIdx.x;
+    int L = blockIdx.y;
+    if (K >= numz) return;
+    int h = numphi / 2;
+    int Ls = (L < h) ? L + h : L - h;
+    long plane = (long)numr * numz;
+    pot[(long)L * plane + (long)K * numr] = pot[(long)Ls * plane + (long)K * numr + 1];
+}
+
+void launch_axis(double *pot, int numr, int numz, int numphi, hipStream_t s)
+{
+    dim3 grid((numz + 127) / 128, numphi);
+    hipLaunchKernelGGL(axis_kernel, grid, dim3(128), 0, s, pot, numr, numz, numphi);
+}
+
+// ---------------------------------------------------------------------------
+// Green's functions.  G_m = sqrt(R'/R) / pi * Q_{m-1/2}(chi) with
+// chi = ((z_B - z')^2 + R^2 + R'^2) / (2 R R')  (tm.F90:162-167, sm.F90:178-182)
+// Q evaluated as the reference does (tm.F90:168-194): below hypr_cutoff from
+// K, E and the upward Legendre recurrence; from the cutoff on by the
+// hypergeometric series truncated after nser terms, here summed by Horner's
+// rule in 1/chi^2 with coefficients exp(dcoefh) prepared on the host.
+// ---------------------------------------------------------------------------
+__device__ double d_rf(double x, double y, double z)
+{
+    const double errtol = 0.0025;
+    double xt = x, yt = y, zt = z, ave, dx, dy, dz;
+    for (int it = 0; it < 64; ++it) {
+        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
+        double lam = sx * (sy + sz) + sy * sz;
+        xt = 0.25 * (xt + lam);
+        yt = 0.25 * (yt + lam);
+        zt = 0.25 * (zt + lam);
+        ave = (1.0 / 3.0) * (xt + yt + zt);
+        dx = (ave - xt) / ave;
+        dy = (ave - yt) / ave;
+        dz = (ave - zt) / ave;
+        if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) <= errtol) break;
+    }
+    double e2 = dx * dy - dz * dz, e3 = dx * dy * dz;
+    return (1.0 + ((1.0 / 24.0) * e2 - 0.1 - (3.0 / 44.0) * e3) * e2 + (1.0 / 14.0) * e3) / sqrt(ave);
+}
+
+__device__ double d_rd(double x, double y, double z)
+{
+    const double errtol = 0.0015;
+    double xt = x, yt = y, zt = z, sum = 0.0, fac = 1.0, ave, dx, dy, dz;
+    for (int it = 0; it < 64; ++it) {
+        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
+        double lam = sx * (sy + sz) + sy * sz;
+        sum += fac / (sz * (zt + lam));
+        fac *= 0.25;
+        xt = 0.25 * (xt + lam);
+        yt = 0.25 * (yt + lam);
+        zt = 0.25 * (zt + lam);
+        ave = 0.2 * (xt + yt + 3.0 * zt);
+        dx = (ave - xt) / ave;
+        dy = (ave - yt) / ave;
+        dz = (ave - zt) / ave;
+        if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) <= errtol) break;
+    }
+    const double c1 = 3.0 / 14.0, c2 = 1.0 / 6.0, c3 = 9.0 / 22.0, c4 = 3.0 / 26.0;
+    double ea = dx * dy, eb = dz * dz, ec = ea - eb, ed = ea - 6.0 * eb, ee = ed + ec + ec;
+    double c5 = 0.25 * c3, c6 = 1.5 * c4;
+    return 3.0 * sum + fac * (1.0 + ed * (-c1 + c5 * ed - c6 * dz * ee) +
+                              dz * (c2 * ee + dz * (-c3 * ec + dz * c4 * ea))) /
+                           (ave * sqrt(ave));
+}
+
+// Writes coef * Q_{m-1/2}(xm) to out[m * mstride], m = 0..mmax-1.
+__device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, double *out, long mstride)
+{
+    const int mmax = a.mmax;
+    if (xm < a.hypr_cutoff) {
+        double mum = sqrt(2.0 / (1.0 + xm));
+        double lam = sqrt(2.0 * (1.0 + xm));
+        double kq = (1.0 - mum) * (1.0 + mum);
+        double kk = d_rf(0.0, kq, 1.0);
+        double ee = kk - mum * mum * d_rd(0.0, kq, 1.0) * (1.0 / 3.0);
+        double q0 = kk * mum;
+        out[0] = coef * q0;
+        if (mmax < 2) return;
+        double q1 = xm * mum * kk - lam * ee;
+        out[mstride] = coef * q1;
+        for (int m = 3; m <= mmax; ++m) {
+            double nu = a.nu[m];
+            double q2 = (2.0 * nu + 1.0) / (nu + 1.0) * xm * q1 - nu / (nu + 1.0) * q0;
+            out[(long)(m - 1) * mstride] = coef * q2;
+            q0 = q1;
+            q1 = q2;
+        }
+    } else {
+        double l2x = log(2.0 * xm);
+        double y = 1.0 / (xm * xm);
+        for (int m = 0; m < mmax; ++m) {
+            const double *c = a.cser + (long)m * a.nser;
+            double s = c[a.nser - 1];
+            for (int ir = a.nser - 2; ir >= 0; --ir) s = s * y + c[ir];
+            out[(long)m * mstride] = coef * (exp(a.coefh[m] - ((double)m + 0.5) * l2x) * s);
+        }
+    }
+}
+
+__global__ __launch_bounds__(128) void gen_top_table_kernel(TableArgs a, double *T)
+{
+    const int nri = a.numr - 2, nzi = a.numz - 2;
+    const long npts = (long)nri * nzi;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long total = npts * nri;
+    if (t >= total) return;
+    int jr = (int)(t % nri);
+    int kz = (int)((t / nri) % nzi);
+    int jo = (int)(t / npts);  // target jg = jo + 2
+    int jl = jr + 2, kl = kz + 2, jg = jo + 2;
+    double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
+    double zT = a.zhf_g[a.numz], zhf = a.zhf_g[kl];
+    double coef = sqrt(rhf * a.rhfinv_g[jg]) * a.piinv;
+    double xm = 0.5 * rhfinv * a.rhfinv_g[jg] *
+                ((zT - zhf) * (zT - zhf) + a.rhf_g[jg] * a.rhf_g[jg] + rhf * rhf);
+    d_qhalf_store(a, xm, coef, T + (long)jo * npts + (long)kz * nri + jr, (long)nri * npts);
+}
+
+__global__ __launch_bounds__(128) void gen_side_table_kernel(TableArgs a, double *S)
+{
+    const int nri = a.numr - 2, nzi = a.numz - 2;
+    const int nd = 2 * nzi - 1;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nd * nri) return;
+    int jr = (int)(t % nri);
+    int d = (int)(t / nri);
+    int jl = jr + 2;
+    double dz = (double)(d - (nzi - 1)) * a.dz;
+    double rB = a.rhf_g[a.numr], rBinv = 1.0 / rB;
+    double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
+    double coef = sqrt(rhf * rBinv) * a.piinv;
+    double xm = 0.5 * rhfinv * rBinv * (dz * dz + rB * rB + rhf * rhf);
+    d_qhalf_store(a, xm, coef, S + (long)d * nri + jr, (long)nd * nri);
+}
+
+void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s)
+{
+    long total = (long)(a.numr - 2) * (a.numr - 2) * (a.numz - 2);
+    hipLaunchKernelGGL(gen_top_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, T);
+}
+
+void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
+{
+    long total = (long)(2 * (a.numz - 2) - 1) * (a.numr - 2);
+    hipLaunchKernelGGL(gen_side_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, S);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary contraction.  Top and bottom share one table: the bottom Green's
+// function at source height z' equals the top one at -z' on the symmetric
+// isym = 1 grid (bm.F90 vs tm.F90), so each table row is contracted with the
+// projections and with their z-mirror.  Deterministic block reduction.
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ void block_reduce_store(double (&v)[NV], double *out, long ostride)
+{
+    __shared__ double red[NV][256];
+    for (int k = 0; k < NV; ++k) red[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int k = 0; k < NV; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NV; ++k) out[k * ostride] = red[k][0];
+}
+
+__global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__restrict__ T,
+                                                              const double *__restrict__ proj,
+                                                              double *__restrict__ Stb, int numr,
+                                                              int numz, int mmax)
+{
+    const int nri = numr - 2, nzi = numz - 2;
+    const long npts = (long)nri * nzi, plane = (long)numr * numz;
+    const int jo = blockIdx.x, m = blockIdx.y;
+    const double *t = T + ((long)m * nri + jo) * npts;
+    const double *pc = proj + (long)m * plane;
+    const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long idx = threadIdx.x; idx < npts; idx += 256) {
+        int kz = (int)(idx / nri), jr = (int)(idx % nri);
+        double g = t[idx];
+        long pa = (long)(kz + 1) * numr + jr + 1;
+        long pm = (long)(nzi - kz) * numr + jr + 1;
+        v[0] += g * pc[pa];
+        v[2] += g * pc[pm];
+        if (m > 0) {
+            v[1] += g * ps[pa];
+            v[3] += g * ps[pm];
+        }
+    }
+    block_reduce_store<4>(v, Stb + (long)m * 4 * nri + jo, nri);
+}
+
+__global__ __launch_bounds__(256) void contract_side_kernel(const double *__restrict__ S,
+                                                            const double *__restrict__ proj,
+                                                            double *__restrict__ Ss, int numr, int numz,
+                                                            int mmax)
+{
+    const int nri = numr - 2, nzi = numz - 2;
+    const long plane = (long)numr * numz;
+    const int ko = blockIdx.x, m = blockIdx.y;  // target kg = ko + 2
+    const int nd = 2 * nzi - 1;
+    const double *sm = S + (long)m * nd * nri;
+    const double *pc = proj + (long)m * plane;
+    const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
+    double v[2] = {0.0, 0.0};
+    const long npts = (long)nri * nzi;
+    for (long idx = threadIdx.x; idx < npts; idx += 256) {
+        int kz = (int)(idx / nri), jr = (int)(idx % nri);
+        double g = sm[(long)(ko - kz + nzi - 1) * nri + jr];
+        long pa = (long)(kz + 1) * numr + jr + 1;
+        v[0] += g * pc[pa];
+        if (m > 0) v[1] += g * ps[pa];
+    }
+    block_reduce_store<2>(v, Ss + (long)m * 2 * nzi + ko, nzi);
+}
+
+void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(contract_topbot_kernel, dim3(numr - 2, mmax), dim3(256), 0, s, T, proj, Stb, numr,
+                       numz, mmax);
+}
+
+void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(contract_side_kernel, dim3(numz - 2, mmax), dim3(256), 0, s, S, proj, Ss, numr, numz,
+                       mmax);
+}
+
+// physical faces: factor * (S_C(0) + sum_m 2 S_C(m) cos + 2 S_S(m) sin)
+__global__ void face_rebuild_kernel(const double *__restrict__ Stb, const double *__restrict__ Ss,
+                                    const double *__restrict__ besc, const double *__restrict__ bess,
+                                    double *__restrict__ F, int numr, int numz, int numphi, int mmax,
+                                    double factor)
+{
+    const int nri = numr - 2, nzi = numz - 2;
+    const int nf = 2 * nri + nzi;
+    int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    int L = blockIdx.y;
+    if (fi >= nf) return;
+    const double *sc, *ss;
+    long st;
+    int idx;
+    if (fi < nri) { idx = fi; sc = Stb; ss = Stb + nri; st = 4 * (long)nri; }
+    else if (fi < 2 * nri) { idx = fi - nri; sc = Stb + 2 * nri; ss = Stb + 3 * nri; st = 4 * (long)nri; }
+    else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; }
+    double acc = sc[idx];
+    for (int m = 1; m < mmax; ++m) {
+        double c = besc[(long)m * numphi + L], s = bess[(long)m * numphi + L];
+        acc = acc + 2.0 * (sc[m * st + idx] * c) + 2.0 * (ss[m * st + idx] * s);
+    }
+    F[(long)L * nf + fi] = factor * acc;
+}
+
+void launch_face_rebuild(const double *Stb, const double *Ss, const double *besc, const double *bess, double *F,
+                         int numr, int numz, int numphi, int mmax, double factor, hipStream_t s)
+{
+    int nf = 2 * (numr - 2) + (numz - 2);
+    dim3 grid((nf + 127) / 128, numphi);
+    hipLaunchKernelGGL(face_rebuild_kernel, grid, dim3(128), 0, s, Stb, Ss, besc, bess, F, numr, numz, numphi,
+                       mmax, factor);
+}
+
+// Fourier faces into the J- and K-layout potentials; the outer corners of the
+// side face are zero (bessel.F90:428, 456, 487).
+__global__ void face_scatter_kernel(const double *__restrict__ FF, double *__restrict__ XJ,
+                                    double *__restrict__ XK, int numr, int numz)
+{
+    const int nri = numr - 2, nzi = numz - 2;
+    const int nf = 2 * nri + nzi;
+    const int L = blockIdx.y;
+    const long plane = (long)numr * numz;
+    double *xj = XJ + L * plane, *xk = XK + L * plane;
+    const double *ff = FF + (long)L * nf;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nri) {  // top, J = t + 2
+        int J = t + 2;
+        double v = ff[t];
+        xj[(long)(numz - 1) * numr + (J - 1)] = v;
+        xk[(long)(J - 1) * numz + (numz - 1)] = v;
+    } else if (t < 2 * nri) {  // bottom
+        int J = t - nri + 2;
+        double v = ff[t];
+        xj[(J - 1)] = v;
+        xk[(long)(J - 1) * numz] = v;
+    } else if (t < nf) {  // side, K = t - 2 nri + 2
+        int K = t - 2 * nri + 2;
+        double v = ff[t];
+        xj[(long)(K - 1) * numr + (numr - 1)] = v;
+        xk[(long)(numr - 1) * numz + (K - 1)] = v;
+    } else if (t == nf) {  // corners
+        xj[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
+        xj[(numr - 1)] = 0.0;
+        xk[(long)(numr - 1) * numz + (numz - 1)] = 0.0;
+        xk[(long)(numr - 1) * numz] = 0.0;
+    }
+}
+
+void launch_face_scatter(const double *FF, double *XJ, double *XK, int numr, int numz, int numphi, hipStream_t s)
+{
+    int nf = 2 * (numr - 2) + (numz - 2) + 1;
+    dim3 grid((nf + 127) / 128, numphi);
+    hipLaunchKernelGGL(face_scatter_kernel, grid, dim3(128), 0, s, FF, XJ, XK, numr, numz);
+}
+
+}  // namespace bh

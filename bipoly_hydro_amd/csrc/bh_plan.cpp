@@ -1,0 +1,591 @@
+// bh_plan.cpp -- plan construction (the potsetup counterpart) and the solve
+// sequence (the potential_solver counterpart) behind the C ABI of
+// include/bh_potential.h.
+#include "../../include/bh_potential.h"
+#include "bh_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace bh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(BH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+    } while (0)
+
+// ln Gamma in the 6-term Lanczos form (gamma = 5) published in Numerical
+// Recipes, 2nd ed., sec. 6.1.  The Green's-function series coefficients are
+// built from it as in tm.F90:125-152; its own error (|eps| < 2e-10) reaches
+// the tables, so the same approximation is used here (gammln.F90:14-26).
+double lanczos_lngamma(double xx)
+{
+    static const double c[6] = {76.18009172947146, -86.50532032941677, 24.01409824083091,
+                                -1.231739572450155, 0.1208650973866179e-2, -0.5395239384953e-5};
+    double x = xx, y = xx;
+    double tmp = x + 5.5;
+    tmp = (x + 0.5) * std::log(tmp) - tmp;
+    double ser = 1.000000000190015;
+    for (int j = 0; j < 6; ++j) {
+        y = y + 1.0;
+        ser = ser + c[j] / y;
+    }
+    return tmp + std::log(2.5066282746310005 * ser / x);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct bh_plan {
+    bh_grid_desc d;
+    int precision;
+    size_t budget;
+    Geo g;
+    int nsteps_cold, nsteps_warm;
+    // device memory
+    std::vector<DevBuf> bufs;
+    size_t bytes = 0;
+    double *RJ = nullptr, *RK = nullptr, *XJ = nullptr, *XK = nullptr, *proj = nullptr;
+    double *T = nullptr, *S = nullptr, *Stb = nullptr, *Ss = nullptr, *F = nullptr, *FF = nullptr;
+    double *Dfwd = nullptr, *Dinv = nullptr, *besc = nullptr, *bess = nullptr;
+    double *alphar = nullptr, *betar = nullptr, *elzb = nullptr;
+    double *rc_alpha[2] = {nullptr, nullptr}, *rc_ibet[2] = {nullptr, nullptr}, *rc_gback[2] = {nullptr, nullptr};
+    double *zc_alpha[2] = {nullptr, nullptr}, *zc_ibet[2] = {nullptr, nullptr}, *zc_gback[2] = {nullptr, nullptr};
+    double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
+    std::vector<double> dt[2];                     // 0 cold, 1 warm
+    bool has_solution = false;
+    bool timing = false;
+    hipEvent_t ev[5] = {};
+    double last_ms[5] = {0, 0, 0, 0, 0};
+    bool have_timing = false;
+    int device = 0;
+};
+
+namespace {
+
+int dalloc(bh_plan *p, double **out, size_t n)
+{
+    size_t bytes = n * sizeof(double);
+    if (p->budget && p->bytes + bytes > p->budget)
+        return fail(BH_E_NOMEM, "plan exceeds the HBM budget (" + std::to_string(p->budget) + " bytes)");
+    void *ptr = nullptr;
+    hipError_t e = hipMalloc(&ptr, bytes ? bytes : 8);
+    if (e != hipSuccess) return fail(BH_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    p->bufs.push_back({ptr, bytes});
+    p->bytes += bytes;
+    *out = (double *)ptr;
+    return BH_OK;
+}
+
+int upload(double *dst, const std::vector<double> &src)
+{
+    HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    return BH_OK;
+}
+
+void free_all(bh_plan *p)
+{
+    for (auto &b : p->bufs) (void)hipFree(b.p);
+    p->bufs.clear();
+    for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+}  // namespace
+
+extern "C" const char *bh_last_error(void) { return g_err.c_str(); }
+
+extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hbm_budget, bh_plan **out)
+{
+    if (!desc || !out) return fail(BH_E_INVALID, "null argument");
+    *out = nullptr;
+    bh_grid_desc d = *desc;
+    if (d.isym == 0) d.isym = 1;
+    if (d.mmax == 0) d.mmax = 30;
+    if (d.dr == 0.0) d.dr = 1.0 / (130.0 - 3.0);
+    if (d.nsteps_cold == 0) d.nsteps_cold = 20;
+    if (d.nsteps_warm == 0) d.nsteps_warm = 5;
+    if (d.isym != 1) return fail(BH_E_INVALID, "isym: only 1 (no symmetry) is supported");
+    if (d.numr < 5 || d.numz < 5) return fail(BH_E_INVALID, "numr and numz must be >= 5");
+    if (d.numphi < 4 || (d.numphi & (d.numphi - 1))) return fail(BH_E_INVALID, "numphi must be a power of two >= 4");
+    if (d.numz % 2) return fail(BH_E_INVALID, "numz must be even for isym = 1 (symmetric z grid)");
+    if (d.mmax < 1 || d.mmax > 250) return fail(BH_E_INVALID, "mmax must be in [1, 250]");
+    if (d.numr - 2 > 512 || d.numz - 2 > 512) return fail(BH_E_INVALID, "numr-2 and numz-2 must be <= 512");
+    if (d.nsteps_cold < 2 || d.nsteps_warm < 2) return fail(BH_E_INVALID, "nsteps must be >= 2");
+    if (!(d.dr > 0.0) || !std::isfinite(d.dr)) return fail(BH_E_INVALID, "dr must be positive");
+    if (precision != BH_PREC_FP64) return fail(BH_E_INVALID, "precision: only BH_PREC_FP64 is available");
+
+    bh_plan *p = new bh_plan();
+    p->d = d;
+    p->precision = precision;
+    p->budget = hbm_budget;
+    (void)hipGetDevice(&p->device);
+    const int numr = d.numr, numz = d.numz, N = d.numphi, mmax = d.mmax;
+    const int nri = numr - 2, nzi = numz - 2;
+    const long M = (long)numr * numz;
+    Geo &g = p->g;
+    g.numr = numr; g.numz = numz; g.numphi = N; g.mmax = mmax; g.nri = nri; g.nzi = nzi;
+    // constants and grid: setup_frac.F90:345, 374-379, 428-463; potsetup.F90:113-114, 151-153
+    const double pi = std::acos(-1.0);
+    g.dr = d.dr; g.dz = d.dr;
+    g.dphi = 2.0 * pi * (1.0 / (double)N);
+    g.piinv = 1.0 / pi;
+    g.four_pi = 4.0 * pi;
+    const double drinv = 1.0 / g.dr, dzinv = 1.0 / g.dz, dphiinv = 1.0 / g.dphi;
+    g.gamma = dzinv * dzinv;
+    const double drinv2 = drinv * drinv, dphiinv2 = dphiinv * dphiinv;
+    std::vector<double> r_g(numr + 1), rhf_g(numr + 1), rinv_g(numr + 1), rhfinv_g(numr + 1), zhf_g(numz + 1);
+    for (int J = 1; J <= numr; ++J) {
+        double x = (double)J;
+        r_g[J] = (x - 2.0) * g.dr;
+        rhf_g[J] = r_g[J] + 0.5 * g.dr;
+        rinv_g[J] = (r_g[J] != 0.0) ? 1.0 / r_g[J] : 0.0;
+        rhfinv_g[J] = 1.0 / rhf_g[J];
+    }
+    {
+        double offset = -((double)(numz / 2) * g.dz);
+        for (int K = 1; K <= numz; ++K) zhf_g[K] = offset + ((double)K - 0.5) * g.dz;
+    }
+    // ADI coefficients: potsetup.F90:155-252 (one block: global radial index)
+    std::vector<double> alphar(numr + 1, 0.0), betar(numr + 1, 0.0);
+    for (int J = 2; J <= numr - 1; ++J) {
+        alphar[J] = -r_g[J + 1] * rhfinv_g[J] * drinv2;
+        betar[J] = -r_g[J] * rhfinv_g[J] * drinv2;
+    }
+    const std::vector<double> &ar = betar, &cr = alphar;
+    std::vector<double> brb((size_t)N * (numr + 1), 0.0), elzb((size_t)N * (numr + 1), 0.0);
+    for (int L = 1; L <= N; ++L) {
+        int lstop = N / 2 + 1;
+        int mode = (L <= lstop) ? L - 1 : L - lstop;
+        double m1mode = (mode % 2 == 0) ? 1.0 : -1.0;
+        double elm = std::cos((double)mode * g.dphi);
+        double *b = &brb[(size_t)(L - 1) * (numr + 1)], *e = &elzb[(size_t)(L - 1) * (numr + 1)];
+        for (int J = 3; J <= numr - 1; ++J)
+            b[J] = 2.0 * drinv2 - 2.0 * (elm - 1.0) * dphiinv2 * rhfinv_g[J] * rhfinv_g[J];
+        b[2] = -alphar[2] + (m1mode - 1.0) * betar[2] - 2.0 * (elm - 1.0) * dphiinv2 * rhfinv_g[2] * rhfinv_g[2];
+        for (int J = 2; J <= numr - 1; ++J)
+            e[J] = -2.0 * drinv2 + 2.0 * (elm - 1.0) * dphiinv2 * rhfinv_g[J] * rhfinv_g[J];
+        e[2] = alphar[2] - (m1mode - 1.0) * betar[2] + 2.0 * (elm - 1.0) * dphiinv2 * rhfinv_g[2] * rhfinv_g[2];
+    }
+    // pseudo-timesteps: helmadi.F90:124-129
+    p->nsteps_cold = d.nsteps_cold;
+    p->nsteps_warm = d.nsteps_warm;
+    for (int w = 0; w < 2; ++w) {
+        int ns = w ? d.nsteps_warm : d.nsteps_cold;
+        std::vector<double> &dt = p->dt[w];
+        dt.assign(ns, 0.0);
+        dt[ns - 1] = 4.0 * (rinv_g[numr] * rinv_g[numr]);
+        double alph = std::pow(r_g[numr] * rinv_g[3], 2.0 / ((double)ns - 1.0));
+        for (int i = ns - 2; i >= 0; --i) dt[i] = alph * dt[i + 1];
+    }
+
+    int rc;
+#define ALLOC(ptr, n)                                     \
+    do {                                                  \
+        if ((rc = dalloc(p, &(ptr), (size_t)(n))) != BH_OK) { \
+            free_all(p);                                  \
+            delete p;                                     \
+            return rc;                                    \
+        }                                                 \
+    } while (0)
+    ALLOC(p->RJ, M * N);
+    ALLOC(p->RK, M * N);
+    ALLOC(p->XJ, M * N);
+    ALLOC(p->XK, M * N);
+    ALLOC(p->proj, M * (2 * mmax - 1));
+    ALLOC(p->T, (size_t)mmax * nri * nri * nzi);
+    ALLOC(p->S, (size_t)mmax * (2 * nzi - 1) * nri);
+    ALLOC(p->Stb, (size_t)mmax * 4 * nri);
+    ALLOC(p->Ss, (size_t)mmax * 2 * nzi);
+    const int nf = 2 * nri + nzi;
+    ALLOC(p->F, (size_t)N * nf);
+    ALLOC(p->FF, (size_t)N * nf);
+    ALLOC(p->Dfwd, (size_t)(N + 2 * mmax - 1) * N);
+    ALLOC(p->Dinv, (size_t)N * N);
+    ALLOC(p->besc, (size_t)mmax * N);
+    ALLOC(p->bess, (size_t)mmax * N);
+    ALLOC(p->alphar, numr + 1);
+    ALLOC(p->betar, numr + 1);
+    ALLOC(p->elzb, (size_t)N * (numr + 1));
+    for (int w = 0; w < 2; ++w) {
+        int ns = (int)p->dt[w].size();
+        ALLOC(p->rc_alpha[w], (size_t)ns * N * nri);
+        ALLOC(p->rc_ibet[w], (size_t)ns * N * nri);
+        ALLOC(p->rc_gback[w], (size_t)ns * N * nri);
+        ALLOC(p->zc_alpha[w], (size_t)ns * nzi);
+        ALLOC(p->zc_ibet[w], (size_t)ns * nzi);
+        ALLOC(p->zc_gback[w], (size_t)ns * nzi);
+    }
+    double *d_rhf = nullptr, *d_rhfinv = nullptr, *d_zhf = nullptr, *d_coefh = nullptr, *d_cser = nullptr, *d_nu = nullptr;
+    const int nser = 111;  // hypr_upr_bnd + 1 (pot.h:7)
+    ALLOC(d_rhf, numr + 1);
+    ALLOC(d_rhfinv, numr + 1);
+    ALLOC(d_zhf, numz + 1);
+    ALLOC(d_coefh, mmax);
+    ALLOC(d_cser, (size_t)mmax * nser);
+    ALLOC(d_nu, mmax + 1);
+#undef ALLOC
+
+#define UPLOAD(dst, vec)                              \
+    do {                                              \
+        if ((rc = upload((dst), (vec))) != BH_OK) {   \
+            free_all(p);                              \
+            delete p;                                 \
+            return rc;                                \
+        }                                             \
+    } while (0)
+
+    // DFT matrices.  Forward rows in helmadi's mode order (helmadi.F90:131-156):
+    // k = 0..N/2 -> sum x cos(2 pi k n/N); N/2+k -> -sum x sin(2 pi k n/N).
+    // Then the projection rows of bessel (potsetup.F90:131-139): m = 0 (sum),
+    // cos(m (L-1/2) dphi) for m = 1..mmax-1, then sin for m = 1..mmax-1.
+    {
+        std::vector<double> D((size_t)(N + 2 * mmax - 1) * N), Di((size_t)N * N);
+        for (int n = 0; n < N; ++n) {
+            for (int k = 0; k <= N / 2; ++k) {
+                long kn = ((long)k * n) % N;
+                D[(size_t)k * N + n] = std::cos(2.0 * pi * (double)kn / (double)N);
+            }
+            for (int k = 1; k < N / 2; ++k) {
+                long kn = ((long)k * n) % N;
+                D[(size_t)(N / 2 + k) * N + n] = -std::sin(2.0 * pi * (double)kn / (double)N);
+            }
+        }
+        std::vector<double> bc((size_t)mmax * N), bs((size_t)mmax * N);
+        for (int M1 = 1; M1 <= mmax; ++M1)
+            for (int L = 1; L <= N; ++L) {
+                double a = 0.5 * g.dphi * ((double)M1 - 1.0) * (2.0 * (double)L - 1.0);
+                bc[(size_t)(M1 - 1) * N + (L - 1)] = std::cos(a);
+                bs[(size_t)(M1 - 1) * N + (L - 1)] = std::sin(a);
+            }
+        for (int n = 0; n < N; ++n) {
+            D[(size_t)N * N + n] = 1.0;
+            for (int m = 1; m < mmax; ++m) {
+                D[(size_t)(N + m) * N + n] = bc[(size_t)m * N + n];
+                D[(size_t)(N + mmax - 1 + m) * N + n] = bs[(size_t)m * N + n];
+            }
+        }
+        // inverse (realft isign = -1 with the 2/N of helmadi.F90:386-392)
+        for (int n = 0; n < N; ++n) {
+            Di[(size_t)n * N + 0] = 1.0 / N;
+            Di[(size_t)n * N + N / 2] = ((n % 2) ? -1.0 : 1.0) / N;
+            for (int k = 1; k < N / 2; ++k) {
+                long kn = ((long)k * n) % N;
+                double ang = 2.0 * pi * (double)kn / (double)N;
+                Di[(size_t)n * N + k] = 2.0 * std::cos(ang) / N;
+                Di[(size_t)n * N + N / 2 + k] = -2.0 * std::sin(ang) / N;
+            }
+        }
+        UPLOAD(p->Dfwd, D);
+        UPLOAD(p->Dinv, Di);
+        UPLOAD(p->besc, bc);
+        UPLOAD(p->bess, bs);
+    }
+    UPLOAD(p->alphar, alphar);
+    UPLOAD(p->betar, betar);
+    UPLOAD(p->elzb, elzb);
+    // Thomas factors per pseudo-timestep (tridagr.F90:41-60, tridagz.F90:115-132)
+    for (int w = 0; w < 2; ++w) {
+        const std::vector<double> &dt = p->dt[w];
+        int ns = (int)dt.size();
+        std::vector<double> ra((size_t)ns * N * nri), ri((size_t)ns * N * nri), rg((size_t)ns * N * nri);
+        std::vector<double> za((size_t)ns * nzi), zi((size_t)ns * nzi), zg((size_t)ns * nzi);
+        std::vector<double> bet(numr + 1), gam(numr + 2);
+        for (int s = 0; s < ns; ++s) {
+            for (int L = 1; L <= N; ++L) {
+                const double *b = &brb[(size_t)(L - 1) * (numr + 1)];
+                bet[2] = b[2] + dt[s];
+                for (int J = 3; J <= numr - 1; ++J) {
+                    gam[J] = cr[J - 1] / bet[J - 1];
+                    bet[J] = (b[J] + dt[s]) - ar[J] * gam[J];
+                }
+                size_t off = ((size_t)s * N + (L - 1)) * nri;
+                for (int J = 2; J <= numr - 1; ++J) {
+                    double ib = 1.0 / bet[J];
+                    ra[off + J - 2] = (J == 2) ? 0.0 : -ar[J] * ib;
+                    ri[off + J - 2] = ib;
+                    rg[off + J - 2] = (J <= numr - 2) ? -gam[J + 1] : 0.0;
+                }
+            }
+            std::vector<double> bz(numz + 1), betz(numz + 1), gamz(numz + 2);
+            const double az = -g.gamma, cz = -g.gamma;
+            for (int K = 2; K <= numz - 1; ++K) bz[K] = 2.0 * g.gamma + dt[s];
+            betz[2] = bz[2];
+            for (int K = 3; K <= numz - 1; ++K) {
+                gamz[K] = cz / betz[K - 1];
+                betz[K] = bz[K] - az * gamz[K];
+            }
+            for (int K = 2; K <= numz - 1; ++K) {
+                double ib = 1.0 / betz[K];
+                za[(size_t)s * nzi + K - 2] = (K == 2) ? 0.0 : -az * ib;
+                zi[(size_t)s * nzi + K - 2] = ib;
+                zg[(size_t)s * nzi + K - 2] = (K <= numz - 2) ? -gamz[K + 1] : 0.0;
+            }
+        }
+        UPLOAD(p->rc_alpha[w], ra);
+        UPLOAD(p->rc_ibet[w], ri);
+        UPLOAD(p->rc_gback[w], rg);
+        UPLOAD(p->zc_alpha[w], za);
+        UPLOAD(p->zc_ibet[w], zi);
+        UPLOAD(p->zc_gback[w], zg);
+    }
+    // Green's tables on the device (tm.F90:103-200, sm.F90:118-215)
+    {
+        std::vector<double> nu(mmax + 1, 0.0), coefh(mmax), cser((size_t)mmax * nser);
+        for (int m = 3; m <= mmax; ++m) nu[m] = (double)m - 2.5;
+        double glh = lanczos_lngamma(0.5);
+        for (int m = 1; m <= mmax; ++m) {
+            double mm = (double)m - 1.0;
+            double aa = 0.5 * (mm + 1.5), bb = 0.5 * (mm + 0.5), cc = mm + 1.0;
+            coefh[m - 1] = glh + lanczos_lngamma(mm + 0.5) - lanczos_lngamma(aa) - lanczos_lngamma(bb);
+            for (int ir = 0; ir < nser; ++ir) {
+                double ri = (double)ir;
+                double dc = lanczos_lngamma(aa + ri) + lanczos_lngamma(bb + ri) - lanczos_lngamma(cc + ri) -
+                            lanczos_lngamma(ri + 1.0);
+                cser[(size_t)(m - 1) * nser + ir] = std::exp(dc);
+            }
+        }
+        UPLOAD(d_rhf, rhf_g);
+        UPLOAD(d_rhfinv, rhfinv_g);
+        UPLOAD(d_zhf, zhf_g);
+        UPLOAD(d_coefh, coefh);
+        UPLOAD(d_cser, cser);
+        UPLOAD(d_nu, nu);
+        TableArgs ta;
+        ta.numr = numr; ta.numz = numz; ta.mmax = mmax; ta.nser = nser;
+        ta.hypr_cutoff = 1.025;  // pot.h:5
+        ta.piinv = g.piinv; ta.dz = g.dz;
+        ta.rhf_g = d_rhf; ta.rhfinv_g = d_rhfinv; ta.zhf_g = d_zhf;
+        ta.coefh = d_coefh; ta.cser = d_cser; ta.nu = d_nu;
+        launch_gen_top_table(ta, p->T, 0);
+        launch_gen_side_table(ta, p->S, 0);
+    }
+#undef UPLOAD
+    launch_fill(p->XJ, (size_t)M * N, 0.0, 0);
+    launch_fill(p->XK, (size_t)M * N, 0.0, 0);
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e != hipSuccess) {
+        free_all(p);
+        delete p;
+        return fail(BH_E_DEVICE, std::string("plan setup: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return BH_OK;
+}
+
+extern "C" int bh_plan_destroy(bh_plan *p)
+{
+    if (!p) return BH_OK;
+    (void)hipDeviceSynchronize();
+    free_all(p);
+    delete p;
+    return BH_OK;
+}
+
+extern "C" size_t bh_plan_device_bytes(const bh_plan *p) { return p ? p->bytes : 0; }
+
+extern "C" int bh_plan_set_timing(bh_plan *p, int on)
+{
+    if (!p) return fail(BH_E_INVALID, "null plan");
+    if (on && !p->ev[0])
+        for (auto &e : p->ev) HIPCHK(hipEventCreate(&e));
+    p->timing = on != 0;
+    return BH_OK;
+}
+
+extern "C" int bh_last_timings(const bh_plan *p, double *ms, int n)
+{
+    if (!p || !ms) return fail(BH_E_INVALID, "null argument");
+    if (!p->have_timing) return fail(BH_E_STATE, "no timed solve yet");
+    for (int i = 0; i < n && i < 5; ++i) ms[i] = p->last_ms[i];
+    return BH_OK;
+}
+
+namespace {
+
+// The solve sequence (potential_solver.F90:58-106 with bessel and helmadi).
+int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hipStream_t s)
+{
+    const Geo &g = p->g;
+    const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax, nri = g.nri, nzi = g.nzi;
+    const long M = (long)numr * numz;
+    const bool warm = (tstep > 1) && true;
+    const int w = warm ? 1 : 0;
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[0], s));
+    // 1. one product gives rho-hat (with rhop's zeroed layers) and the
+    //    boundary projections of rho (bessel.F90:179-185, 228-237)
+    GemmOut o{p->RJ, M, p->proj, M, N, numr, numz};
+    launch_gemm_f64(p->Dfwd, N, d_rho, M, N + 2 * mmax - 1, (int)M, N, o, s);
+    launch_transpose_jk(p->RJ, p->RK, numr, numz, N, s);
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
+    // 2. boundary potential (bessel.F90:186-488) -> Fourier-space faces
+    launch_contract_topbot(p->T, p->proj, p->Stb, numr, numz, mmax, s);
+    launch_contract_side(p->S, p->proj, p->Ss, numr, numz, mmax, s);
+    const double factor = -g.dr * g.dz * g.dphi;
+    launch_face_rebuild(p->Stb, p->Ss, p->besc, p->bess, p->F, numr, numz, N, mmax, factor, s);
+    const int nf = 2 * nri + nzi;
+    GemmOut of{p->FF, nf, nullptr, 0, N, 0, 0};
+    launch_gemm_f64(p->Dfwd, N, p->F, nf, N, nf, N, of, s);
+    // 3. starting potential: zero (cold, potential_solver.F90:88-93) or the
+    //    previous solution kept in Fourier space (warm)
+    if (!warm) launch_fill(p->XJ, (size_t)M * N, 0.0, s);
+    launch_face_scatter(p->FF, p->XJ, p->XK, numr, numz, N, s);
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
+    // 4. Peaceman-Rachford iterations (helmadi.F90:195-355)
+    const std::vector<double> &dt = p->dt[w];
+    for (int st = 0; st < (int)dt.size(); ++st) {
+        LineArgs r{};
+        r.xin = p->XJ; r.xout = p->XK; r.rh = p->RJ;
+        r.SA = M; r.PA = numr; r.SB = M; r.PB = numz;
+        r.nlines = nzi; r.len = nri;
+        r.alpha = p->rc_alpha[w] + (size_t)st * N * nri;
+        r.ibet = p->rc_ibet[w] + (size_t)st * N * nri;
+        r.gback = p->rc_gback[w] + (size_t)st * N * nri;
+        r.coef_stride = nri;
+        r.dir = 0; r.dt = dt[st]; r.gamma = g.gamma; r.four_pi = g.four_pi;
+        r.alphar = p->alphar; r.betar = p->betar; r.elzb = p->elzb; r.numr = numr;
+        launch_line_sweep(r, N, s);
+        LineArgs z = r;
+        z.xin = p->XK; z.xout = p->XJ; z.rh = p->RK;
+        z.PA = numz; z.PB = numr;
+        z.nlines = nri; z.len = nzi;
+        z.alpha = p->zc_alpha[w] + (size_t)st * nzi;
+        z.ibet = p->zc_ibet[w] + (size_t)st * nzi;
+        z.gback = p->zc_gback[w] + (size_t)st * nzi;
+        z.coef_stride = 0;
+        z.dir = 1;
+        launch_line_sweep(z, N, s);
+    }
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
+    // 5. inverse DFT with the 2/N normalisation, then the axis ghost
+    GemmOut oi{d_pot, M, nullptr, 0, N, 0, 0};
+    launch_gemm_f64(p->Dinv, N, p->XJ, M, N, (int)M, N, oi, s);
+    launch_axis(d_pot, numr, numz, N, s);
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
+    HIPCHK(hipGetLastError());
+    p->has_solution = true;
+    return BH_OK;
+}
+
+int collect_timing(bh_plan *p)
+{
+    if (!p->timing) return BH_OK;
+    HIPCHK(hipEventSynchronize(p->ev[4]));
+    float t[4];
+    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&t[i], p->ev[i], p->ev[i + 1]));
+    // order: boundary, forward DFT, ADI, inverse DFT, total
+    p->last_ms[0] = t[1];
+    p->last_ms[1] = t[0];
+    p->last_ms[2] = t[2];
+    p->last_ms[3] = t[3];
+    float tot;
+    HIPCHK(hipEventElapsedTime(&tot, p->ev[0], p->ev[4]));
+    p->last_ms[4] = tot;
+    p->have_timing = true;
+    return BH_OK;
+}
+
+}  // namespace
+
+extern "C" int bh_solve_device(bh_plan *p, const double *d_rho, double *d_pot, int tstep, void *stream)
+{
+    if (!p || !d_rho || !d_pot) return fail(BH_E_INVALID, "null argument");
+    int rc = enqueue_solve(p, d_rho, d_pot, tstep, (hipStream_t)stream);
+    if (rc != BH_OK) return rc;
+    return BH_OK;
+}
+
+extern "C" int bh_solve(bh_plan *p, const double *rho, double *pot, int tstep)
+{
+    if (!p || !rho || !pot) return fail(BH_E_INVALID, "null argument");
+    const size_t n = (size_t)p->g.numr * p->g.numz * p->g.numphi;
+    for (size_t i = 0; i < n; ++i)
+        if (!std::isfinite(rho[i])) return fail(BH_E_NONFINITE, "rho holds a non-finite value at index " + std::to_string(i));
+    if (!p->rho_in) {
+        int rc;
+        if ((rc = dalloc(p, &p->rho_in, n)) != BH_OK) return rc;
+        if ((rc = dalloc(p, &p->pot_out, n)) != BH_OK) return rc;
+    }
+    HIPCHK(hipMemcpy(p->rho_in, rho, n * sizeof(double), hipMemcpyHostToDevice));
+    int rc = enqueue_solve(p, p->rho_in, p->pot_out, tstep, 0);
+    if (rc != BH_OK) return rc;
+    HIPCHK(hipMemcpy(pot, p->pot_out, n * sizeof(double), hipMemcpyDeviceToHost));
+    return collect_timing(p);
+}
+
+extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
+{
+    if (!p || !pot) return fail(BH_E_INVALID, "null argument");
+    const Geo &g = p->g;
+    const long M = (long)g.numr * g.numz;
+    const size_t n = (size_t)M * g.numphi;
+    const double *src = pot;
+    if (!on_device) {
+        if (!p->rho_in) {
+            int rc;
+            if ((rc = dalloc(p, &p->rho_in, n)) != BH_OK) return rc;
+            if ((rc = dalloc(p, &p->pot_out, n)) != BH_OK) return rc;
+        }
+        HIPCHK(hipMemcpy(p->pot_out, pot, n * sizeof(double), hipMemcpyHostToDevice));
+        src = p->pot_out;
+    }
+    GemmOut o{p->XJ, M, nullptr, 0, g.numphi, 0, 0};
+    launch_gemm_f64(p->Dfwd, g.numphi, src, M, g.numphi, (int)M, g.numphi, o, 0);
+    HIPCHK(hipDeviceSynchronize());
+    p->has_solution = true;
+    return BH_OK;
+}
+
+extern "C" int bh_sync_timing(bh_plan *p) { return p ? collect_timing(p) : fail(BH_E_INVALID, "null plan"); }
+
+// ---------------------------------------------------------------------------
+// Test hooks (not part of the public ABI): run the GEMM on device buffers and
+// copy a plan's internal buffers to the host, for stage-by-stage checks.
+// ---------------------------------------------------------------------------
+extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb, int R, int M, int Kd, double *C,
+                            long ldc)
+{
+    GemmOut o{C, ldc, nullptr, 0, R, 0, 0};
+    launch_gemm_f64(A, lda, B, ldb, R, M, Kd, o, 0);
+    HIPCHK(hipDeviceSynchronize());
+    return BH_OK;
+}
+
+// which: 0 RJ, 1 RK, 2 XJ, 3 XK, 4 proj, 5 T, 6 S, 7 Stb, 8 Ss, 9 F, 10 FF
+extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
+{
+    if (!p || !host) return fail(BH_E_INVALID, "null argument");
+    const Geo &g = p->g;
+    const size_t M = (size_t)g.numr * g.numz, N = g.numphi, mm = g.mmax, nri = g.nri, nzi = g.nzi;
+    const size_t nf = 2 * nri + nzi;
+    const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->T, p->S, p->Stb, p->Ss, p->F, p->FF};
+    const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), mm * nri * nri * nzi,
+                          mm * (2 * nzi - 1) * nri, mm * 4 * nri, mm * 2 * nzi, N * nf, N * nf};
+    if (which < 0 || which > 10) return fail(BH_E_INVALID, "which");
+    if (n < cnt[which]) return fail(BH_E_INVALID, "buffer too small: need " + std::to_string(cnt[which]));
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpy(host, src[which], cnt[which] * sizeof(double), hipMemcpyDeviceToHost));
+    return BH_OK;
+}

@@ -222,15 +222,17 @@ static int bho_mode_of(int L, int numphi)
 static void bho_build_tables(bho_plan *p);
 
 /* Create a plan: grid, ADI coefficients, trig and Green's tables.
-   with_tables = 0 skips the (expensive) Green's tables; bessel then needs
-   them set by the caller.  Returns NULL on invalid arguments. */
+   with_tables = 1 computes the Green's tables; 0 skips them (bessel must
+   not be called); 2 allocates them zero-filled, for timing the solve
+   arithmetic only (table generation is one-time setup, potsetup).
+   Returns NULL on invalid arguments. */
 bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, int mmax,
                           int with_tables)
 {
     if (isym != 1) return NULL;
     if (numr < 4 || numz < 4 || numphi < 4) return NULL;
     if (numphi & (numphi - 1)) return NULL;
-    if (mmax < 1 || mmax > numphi / 2) return NULL;
+    if (mmax < 1 || mmax > 250) return NULL;
     bho_plan *p = (bho_plan *)xcalloc(1, sizeof(bho_plan));
     p->numr = numr; p->numz = numz; p->numphi = numphi; p->isym = isym; p->mmax = mmax;
     p->nser = BHO_HYPR_UPR_BND + 1;
@@ -321,7 +323,13 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
                 bho_gammln(aa + ri) + bho_gammln(bb + ri) - bho_gammln(cc + ri) - bho_gammln(ri + 1.0);
         }
     }
-    if (with_tables) bho_build_tables(p);
+    if (with_tables == 1) bho_build_tables(p);
+    if (with_tables == 2) {
+        size_t nri = numr - 2, nzi = numz - 2;
+        p->tmr = (double *)xcalloc((size_t)mmax * numr * nzi * nri, sizeof(double));
+        p->bmr = (double *)xcalloc((size_t)mmax * numr * nzi * nri, sizeof(double));
+        p->smz = (double *)xcalloc((size_t)mmax * numz * nzi * nri, sizeof(double));
+    }
     return p;
 }
 

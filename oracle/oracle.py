@@ -70,16 +70,22 @@ class OraclePlan:
     """Grid + coefficients (+ Green's tables) for one geometry, isym = 1."""
 
     def __init__(self, numr, numz, numphi, dr=1.0 / 127.0, mmax=30, with_tables=True):
+        """with_tables: True computes the Green's tables, False skips them,
+        "zero" allocates them zero-filled (timing of the solve arithmetic)."""
         self.shape = (numphi, numz, numr)
         self.numr, self.numz, self.numphi, self.mmax = numr, numz, numphi, mmax
-        self._h = lib().bho_plan_create(numr, numz, numphi, 1, dr, mmax, 1 if with_tables else 0)
+        mode = 2 if with_tables == "zero" else (1 if with_tables else 0)
+        self._h = lib().bho_plan_create(numr, numz, numphi, 1, dr, mmax, mode)
         if not self._h:
             raise ValueError("oracle: invalid geometry")
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            lib().bho_plan_destroy(h)
+            try:
+                lib().bho_plan_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
             self._h = None
 
     def solve(self, rho: np.ndarray, potp: np.ndarray | None = None, tstep: int = 0) -> np.ndarray:

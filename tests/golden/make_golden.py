@@ -1,0 +1,87 @@
+"""Writes the golden vectors of tests/golden/ from the reference's own
+Fortran/MPI potential_solver, built by oracle/ref/Makefile from the sources
+under /root/reference (build container only).
+
+    python tests/golden/make_golden.py [--c2]
+
+Each case: a seeded density (bipoly_hydro_amd.density), a cold solve
+(tstep = 0) and a warm solve (tstep = 2) of a perturbed density starting
+from the cold result.  Output: <case>.npz with rho1, pot1, rho2, pot2 (small
+grids) or slices and norms (258 x 130 x 512).
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from bipoly_hydro_amd import density  # noqa: E402
+
+# name, numr, numz, numphi, numr_procs, numz_procs, density builder
+CASES = [
+    ("c1_66x66x64", 66, 66, 64, 2, 2, lambda s: density.config1_density(seed=s)),
+    ("g34x34x32", 34, 34, 32, 2, 2,
+     lambda s: density.lane_emden_n1_binary(34, 34, 32, radius_cells=(6.0, 4.0), centre_cells=(8.0, -8.0),
+                                            noise=1e-6, seed=s)),
+    ("g42x26x16", 42, 26, 16, 2, 2,
+     lambda s: density.lane_emden_n1_binary(42, 26, 16, radius_cells=(7.0, 5.0), centre_cells=(12.0, -12.0),
+                                            noise=1e-6, seed=s)),
+]
+
+
+def run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, workdir):
+    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle/ref/Makefile"), f"NAME={name}",
+                    f"NUMR={numr}", f"NUMZ={numz}", f"NUMPHI={numphi}", f"NRP={nrp}", f"NZP={nzp}", "-j8"],
+                   check=True)
+    exe = os.path.join(ROOT, "oracle/_ref", name, "bh_ref")
+    f = {k: os.path.join(workdir, f"{name}_{k}.bin") for k in ("rho1", "pot1", "rho2", "pot2")}
+    rho1.tofile(f["rho1"])
+    rho2.tofile(f["rho2"])
+    cmd = f"ulimit -s unlimited; /opt/conda/bin/mpiexec -n {nrp * nzp} {exe} {f['rho1']} {f['pot1']} {f['rho2']} {f['pot2']}"
+    out = subprocess.run(cmd, shell=True, check=True, capture_output=True, text=True).stdout
+    shape = rho1.shape
+    return (np.fromfile(f["pot1"]).reshape(shape), np.fromfile(f["pot2"]).reshape(shape), out)
+
+
+def c2_case(wd):
+    """258 x 130 x 512 on 4 x 2 ranks: slices and norms (the full field is
+    137 MB).  Slices: the two mid-plane rows K = 65, 66, the meridional planes
+    L = 1 and L = 257, and the outer faces."""
+    name, numr, numz, numphi, nrp, nzp = "c2_258x130x512", 258, 130, 512, 4, 2
+    rho1 = density.config2_density(seed=1234)
+    rho2 = rho1 * 1.01 + 1e-3 * density.config2_density(seed=99)
+    pot1, pot2, log = run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, wd)
+    out = {"rho_sums": np.array([rho1.sum(), rho2.sum(), np.abs(rho1).max()]),
+           "seeds": np.array([1234, 99]), "procs": np.array([nrp, nzp])}
+    for tag, pot in (("1", pot1), ("2", pot2)):
+        out["mid" + tag] = pot[:, 64:66, :].copy()
+        out["mer" + tag] = pot[[0, 256], :, :].copy()
+        out["top" + tag] = pot[:, -1, :].copy()
+        out["bot" + tag] = pot[:, 0, :].copy()
+        out["side" + tag] = pot[:, :, -1].copy()
+        out["norm" + tag] = np.array([np.linalg.norm(pot), pot.sum(), np.abs(pot).max()])
+    np.savez(os.path.join(HERE, f"{name}.npz"), **out)
+    print(name, log.strip().replace("\n", " | "))
+
+
+def main():
+    with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref") if os.path.isdir(
+            os.path.join(ROOT, "oracle", "_ref")) else None) as wd:
+        for name, numr, numz, numphi, nrp, nzp, build in CASES:
+            rho1 = build(1234)
+            rho2 = rho1 * 1.01 + 1e-3 * build(99)
+            pot1, pot2, log = run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, wd)
+            np.savez(os.path.join(HERE, f"{name}.npz"), pot1=pot1, pot2=pot2,
+                     rho_sums=np.array([rho1.sum(), rho2.sum(), np.abs(rho1).max()]),
+                     seeds=np.array([1234, 99]), procs=np.array([nrp, nzp]))
+            print(name, log.strip().replace("\n", " | "))
+        if "--c2" in sys.argv:
+            c2_case(wd)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,29 @@
+"""Seeded inputs of the golden cases in tests/golden/ (see make_golden.py)."""
+import os
+
+import numpy as np
+
+from bipoly_hydro_amd import density
+
+SMALL_CASES = {
+    "c1_66x66x64": (66, 66, 64, lambda s: density.config1_density(seed=s)),
+    "g34x34x32": (34, 34, 32, lambda s: density.lane_emden_n1_binary(
+        34, 34, 32, radius_cells=(6.0, 4.0), centre_cells=(8.0, -8.0), noise=1e-6, seed=s)),
+    "g42x26x16": (42, 26, 16, lambda s: density.lane_emden_n1_binary(
+        42, 26, 16, radius_cells=(7.0, 5.0), centre_cells=(12.0, -12.0), noise=1e-6, seed=s)),
+}
+
+
+def rell2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def load_case(golden_dir, name):
+    numr, numz, numphi, build = SMALL_CASES[name]
+    g = np.load(os.path.join(golden_dir, name + ".npz"))
+    s1, s2 = (int(x) for x in g["seeds"])
+    rho1 = build(s1)
+    rho2 = rho1 * 1.01 + 1e-3 * build(s2)
+    # the density generator must still produce the inputs the reference saw
+    np.testing.assert_allclose([rho1.sum(), rho2.sum(), np.abs(rho1).max()], g["rho_sums"], rtol=1e-14)
+    return (numr, numz, numphi), rho1, rho2, g["pot1"], g["pot2"]

@@ -1,0 +1,115 @@
+"""GPU parity of the HIP path (through the C ABI) against the reference's own
+Fortran/MPI output (tests/golden/) and against the oracle on the same seeded
+inputs.  Plain FP64 path bar: relative L2 <= 1e-11 on the full potential,
+faces, axis ghost and corners included."""
+import os
+
+import numpy as np
+import pytest
+
+from bipoly_hydro_amd import density
+from bipoly_hydro_amd import potential as P
+from golden_cases import load_case
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-11
+
+
+def rell2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("name", ["g42x26x16", "g34x34x32", "c1_66x66x64"])
+def test_fp64_matches_reference_cold_then_warm(golden_dir, name):
+    (numr, numz, numphi), rho1, rho2, ref1, ref2 = load_case(golden_dir, name)
+    with P.PotentialPlan(numr, numz, numphi) as plan:
+        pot1 = plan.potential_solver(rho1, tstep=0)
+        pot2 = plan.potential_solver(rho2, tstep=2)
+    e1, e2 = rell2(pot1, ref1), rell2(pot2, ref2)
+    print(f"{name}: cold relL2 {e1:.3e} warm relL2 {e2:.3e}", flush=True)
+    assert e1 <= TOL_FP64 and e2 <= TOL_FP64
+    # the outer corners stay zero and the axis ghost is the pi rotation
+    assert np.all(pot1[:, 0, -1] == 0.0) and np.all(pot1[:, -1, -1] == 0.0)
+    h = numphi // 2
+    np.testing.assert_array_equal(pot1[:h, :, 0], pot1[h:, :, 1])
+
+
+@pytest.mark.parametrize("shape,seed", [((130, 66, 128), 5), ((98, 34, 64), 11), ((194, 130, 32), 2)])
+def test_fp64_matches_oracle_on_other_grids(shape, seed):
+    from oracle import oracle as O
+    numr, numz, numphi = shape
+    rho1 = density.scaled_binary(numr, numz, numphi, seed)
+    rho2 = density.scaled_binary(numr, numz, numphi, seed, mass_ratio=0.5, separation=0.6)
+    oplan = O.OraclePlan(numr, numz, numphi)
+    o1 = oplan.solve(rho1, tstep=0)
+    o2 = oplan.solve(rho2, potp=o1, tstep=3)
+    with P.PotentialPlan(numr, numz, numphi) as plan:
+        g1 = plan.potential_solver(rho1, tstep=0)
+        g2 = plan.potential_solver(rho2, tstep=3)
+    e1, e2 = rell2(g1, o1), rell2(g2, o2)
+    print(f"{shape}: cold {e1:.3e} warm {e2:.3e}", flush=True)
+    assert e1 <= TOL_FP64 and e2 <= TOL_FP64
+
+
+def test_fp64_production_grid_against_reference_slices(golden_dir):
+    path = os.path.join(golden_dir, "c2_258x130x512.npz")
+    if not os.path.exists(path):
+        pytest.skip("258x130x512 reference slices not generated")
+    g = np.load(path)
+    rho1 = density.config2_density(seed=int(g["seeds"][0]))
+    rho2 = rho1 * 1.01 + 1e-3 * density.config2_density(seed=int(g["seeds"][1]))
+    np.testing.assert_allclose([rho1.sum(), rho2.sum(), np.abs(rho1).max()], g["rho_sums"], rtol=1e-13)
+    with P.PotentialPlan(258, 130, 512) as plan:
+        pots = [plan.potential_solver(rho1, tstep=0), plan.potential_solver(rho2, tstep=2)]
+    for tag, pot in zip(("1", "2"), pots):
+        parts = {"mid": pot[:, 64:66, :], "mer": pot[[0, 256], :, :], "top": pot[:, -1, :],
+                 "bot": pot[:, 0, :], "side": pot[:, :, -1]}
+        for k, v in parts.items():
+            e = rell2(v, g[k + tag]) if np.linalg.norm(g[k + tag]) > 0 else float(np.abs(v).max())
+            print(f"c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
+            assert e <= TOL_FP64, (tag, k, e)
+        assert abs(np.linalg.norm(pot) / g["norm" + tag][0] - 1.0) < TOL_FP64
+
+
+def test_solves_are_deterministic_and_warm_start_matches_import():
+    numr, numz, numphi = 66, 66, 64
+    rho1 = density.config1_density(seed=7)
+    rho2 = rho1 * 0.99
+    with P.PotentialPlan(numr, numz, numphi) as a, P.PotentialPlan(numr, numz, numphi) as b:
+        p1 = a.potential_solver(rho1, tstep=0)
+        q1 = b.potential_solver(rho1, tstep=0)
+        np.testing.assert_array_equal(p1, q1)
+        p2 = a.potential_solver(rho2, tstep=2)
+        # a potential imported from a "continuation file" gives the same warm start
+        with P.PotentialPlan(numr, numz, numphi) as c:
+            c.set_potential(p1)
+            r2 = c.potential_solver(rho2, tstep=2)
+    assert rell2(r2, p2) < 1e-13
+
+
+def test_device_entry_point_matches_host_entry_point():
+    torch = pytest.importorskip("torch")
+    numr, numz, numphi = 66, 66, 64
+    rho = density.config1_density(seed=3)
+    with P.PotentialPlan(numr, numz, numphi) as plan:
+        host = plan.potential_solver(rho, tstep=0)
+        d_rho = torch.from_numpy(rho).cuda()
+        d_pot = torch.empty_like(d_rho)
+        stream = torch.cuda.current_stream().cuda_stream
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep=0, stream=stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(d_pot.cpu().numpy(), host)
+
+
+def test_error_paths():
+    rho = density.config1_density(seed=3)
+    with P.PotentialPlan(66, 66, 64) as plan:
+        bad = rho.copy()
+        bad[3, 4, 5] = np.nan
+        with pytest.raises(P.BHError) as ei:
+            plan.potential_solver(bad)
+        assert ei.value.code == -4
+    with pytest.raises(P.BHError) as ei:
+        P.PotentialPlan(66, 66, 64, hbm_budget=1 << 20)
+    assert ei.value.code == -2

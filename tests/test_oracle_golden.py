@@ -1,0 +1,118 @@
+"""The oracle (oracle/bh_oracle.c) against the reference's own Fortran/MPI
+potential_solver output (tests/golden/, written by make_golden.py) and
+against what mathematics fixes for each of its parts."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from bipoly_hydro_amd import density
+from oracle import oracle as O
+from golden_cases import SMALL_CASES, load_case, rell2
+
+@pytest.mark.parametrize("name", list(SMALL_CASES))
+def test_oracle_matches_reference_cold_and_warm(golden_dir, name):
+    (numr, numz, numphi), rho1, rho2, ref1, ref2 = load_case(golden_dir, name)
+    plan = O.OraclePlan(numr, numz, numphi)
+    pot1 = plan.solve(rho1, tstep=0)
+    pot2 = plan.solve(rho2, potp=pot1, tstep=2)
+    assert rell2(pot1, ref1) < 1e-13
+    assert rell2(pot2, ref2) < 1e-13
+
+
+def test_gammln_is_the_lanczos_approximation():
+    # |eps| < 2e-10 against exact ln Gamma, and not exact: the tables inherit it
+    xs = [0.25 * k for k in range(1, 600)]
+    err = [abs(O.gammln(x) - math.lgamma(x)) for x in xs]
+    assert max(err) < 2e-10
+    assert max(err) > 1e-12
+
+
+def test_carlson_known_values_and_complete_integrals():
+    from scipy import special
+    # Carlson (1995) test values
+    assert abs(O.rf(1.0, 2.0, 0.0) - 1.3110287771461) < 1e-12
+    assert abs(O.rd(0.0, 2.0, 1.0) - 1.7972103521034) < 1e-12
+    for k in np.linspace(0.05, 0.999, 40):
+        assert abs(O.ellk(k) / special.ellipk(k * k) - 1.0) < 1e-14
+        assert abs(O.elle(k) / special.ellipe(k * k) - 1.0) < 1e-14
+
+
+def test_dft_against_numpy_and_roundtrip():
+    rng = np.random.default_rng(3)
+    for n in (4, 16, 64, 512):
+        x = rng.standard_normal(n)
+        f = O.dft_forward(x)
+        X = np.fft.rfft(x)
+        np.testing.assert_allclose(f[: n // 2 + 1], X.real, atol=1e-12 * n)
+        np.testing.assert_allclose(f[n // 2 + 1:], X.imag[1: n // 2], atol=1e-12 * n)
+        np.testing.assert_allclose(O.dft_inverse(f), x, atol=1e-13 * n)
+
+
+def test_qhalf_against_mpmath():
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 30
+    plan = O.OraclePlan(10, 10, 64, with_tables=False)
+    # series branch, far from the cutoff the 111-term truncation is negligible;
+    # what remains is the Lanczos ln Gamma error (~1e-10 relative)
+    for x in (2.0, 5.0, 40.0):
+        q = plan.qhalf(x)
+        for m in range(30):
+            ex = float(mp.legenq(m - 0.5, 0, x, type=3).real)
+            assert abs(q[m] / ex - 1.0) < 2e-9, (x, m)
+    # recurrence branch: Q_{-1/2}, Q_{1/2} from K and E are exact to rounding
+    for x in (1.001, 1.01, 1.02):
+        q = plan.qhalf(x)
+        for m in (0, 1):
+            ex = float(mp.legenq(m - 0.5, 0, x, type=3).real)
+            assert abs(q[m] / ex - 1.0) < 1e-13
+        ex = float(mp.legenq(9.5, 0, x, type=3).real)
+        assert abs(q[10] / ex - 1.0) < 1e-9
+
+
+def test_dt_schedule_closed_form():
+    # dt(1) = 4/dr^2, dt(n) = 4/r_B^2, geometric in between (helmadi.F90:124-129)
+    plan = O.OraclePlan(66, 66, 64, with_tables=False)
+    dr = 1.0 / 127.0
+    dt = plan.dt_schedule(20)
+    assert abs(dt[0] * dr * dr / 4.0 - 1.0) < 1e-12
+    assert abs(dt[-1] * (64 * dr) ** 2 / 4.0 - 1.0) < 1e-12
+    ratios = dt[:-1] / dt[1:]
+    assert np.allclose(ratios, ratios[0], rtol=1e-12)
+
+
+def test_single_polytrope_against_analytic_potential():
+    # one n = 1 polytrope on the axis: phi_in = -(4 rho_c R^2/pi)(1 + sin(u)/u),
+    # u = pi d/R, phi_out = -M/d, M = 4 rho_c R^3/pi.  Agreement is at the
+    # level of the grid and of the 20-step ADI truncation, not rounding.
+    numr, numz, numphi = 34, 34, 16
+    dr = 1.0 / 127.0
+    R = 10.0
+    rho = density.lane_emden_n1_binary(numr, numz, numphi, rho_c=(1.0,), radius_cells=(R,),
+                                       centre_cells=(0.0,), noise=0.0)
+    plan = O.OraclePlan(numr, numz, numphi)
+    pot = plan.solve(rho, tstep=0)
+    r, z, _ = density.grid_coords(numr, numz, numphi)
+    d = np.sqrt(r[None, :] ** 2 + z[:, None] ** 2)
+    Rd = R * dr
+    u = np.pi * d / Rd
+    inside = d < Rd
+    exact = np.where(inside, -(4 * Rd * Rd / np.pi) * (1 + np.sin(u) / np.where(u > 0, u, 1)),
+                     -(4 * Rd ** 3 / np.pi) / d)
+    num = pot[0, 1:-1, 1:-1]
+    ex = exact[1:-1, 1:-1]
+    assert np.max(np.abs(num - ex)) / np.max(np.abs(ex)) < 2e-2
+    # the boundary values come from the Green's-function contraction alone
+    assert np.max(np.abs(pot[0, -1, 1:-1] - exact[-1, 1:-1])) / np.max(np.abs(ex)) < 5e-3
+
+
+def test_planted_error_is_caught(golden_dir):
+    # a dropped term in the oracle (here: the top boundary forced to zero)
+    # must fail the reference comparison by orders of magnitude
+    (numr, numz, numphi), rho1, _, ref1, _ = load_case(golden_dir, "g42x26x16")
+    plan = O.OraclePlan(numr, numz, numphi)
+    pot1 = plan.solve(rho1, tstep=0)
+    bad = pot1.copy()
+    bad[:, -1, :] = 0.0
+    assert rell2(bad, ref1) > 1e-3
