@@ -49,8 +49,9 @@ def reference_slices_error(pot):
     if not os.path.exists(path) or pot.shape != (512, 130, 258):
         return None
     g = np.load(path)
-    parts = [(pot[:, 64:66, :], g["mid1"]), (pot[[0, 256], :, :], g["mer1"]), (pot[:, -1, :], g["top1"]),
-             (pot[:, 0, :], g["bot1"]), (pot[:, :, -1], g["side1"])]
+    # the cold-solve slices kept by tests/golden/make_golden.py
+    parts = [(pot[:, 64, :], g["mid1"]), (pot[0], g["mer1"]), (pot[::8, -1, :], g["top1"]),
+             (pot[::8, 0, :], g["bot1"]), (pot[::8, :, -1], g["side1"])]
     num = sum(float(np.sum((a - b) ** 2)) for a, b in parts)
     den = sum(float(np.sum(b ** 2)) for _, b in parts)
     return (num / den) ** 0.5

@@ -154,40 +154,125 @@ void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int
 // every load and store of a segment is 64 consecutive doubles.  The Thomas
 // forward sweep y_e = alpha_e y_{e-1} + ibet_e rhs_e and back substitution
 // x_e = y_e + gback_e x_{e+1} are first-order linear recurrences, evaluated
-// per segment as an inclusive wavefront scan of affine maps, with the value
-// carried between segments.
+// per segment as an inclusive wavefront scan of affine maps (DPP row shifts
+// inside rows of 16 lanes, readlane across rows), with the value carried
+// between segments.  A wave owns consecutive lines and rolls the neighbour
+// rows in registers, loading the row two lines ahead while it solves.
 // ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ inline double dpp_d(double old, double v)
+{
+    long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, 0xf, 0xf, false);
+    int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
-__device__ inline double shfl_up_d(double v, int d) { return __shfl_up(v, d, 64); }
-__device__ inline double shfl_down_d(double v, int d) { return __shfl_down(v, d, 64); }
-__device__ inline double bcast_d(double v, int src) { return __shfl(v, src, 64); }
+__device__ inline double readlane_d(double v, int l)
+{
+    long long x = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_readlane((int)x, l);
+    int hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
-// LS_LINES lines per workgroup (4 waves); LDS tile [LS_LINES][64*C + 1] doubles
+// compose (A,B) <- (A,B) o (Ap,Bp):  y = A (Ap y0 + Bp) + B
+__device__ inline void compose(double &A, double &B, double Ap, double Bp)
+{
+    B = A * Bp + B;
+    A = A * Ap;
+}
+
+// inclusive scan from lane 0 upwards
+__device__ inline void scan_up(double &A, double &B, int lane)
+{
+    compose(A, B, dpp_d<0x111>(1.0, A), dpp_d<0x111>(0.0, B));  // row_shr:1
+    compose(A, B, dpp_d<0x112>(1.0, A), dpp_d<0x112>(0.0, B));  // row_shr:2
+    compose(A, B, dpp_d<0x114>(1.0, A), dpp_d<0x114>(0.0, B));  // row_shr:4
+    compose(A, B, dpp_d<0x118>(1.0, A), dpp_d<0x118>(0.0, B));  // row_shr:8
+    const int row = lane >> 4;
+    double A15 = readlane_d(A, 15), B15 = readlane_d(B, 15);
+    double A47 = readlane_d(A, 47), B47 = readlane_d(B, 47);
+    if (row == 1) compose(A, B, A15, B15);
+    if (row == 3) compose(A, B, A47, B47);
+    double A31 = readlane_d(A, 31), B31 = readlane_d(B, 31);
+    if (row >= 2) compose(A, B, A31, B31);
+}
+
+// inclusive scan from lane 63 downwards
+__device__ inline void scan_down(double &A, double &B, int lane)
+{
+    compose(A, B, dpp_d<0x101>(1.0, A), dpp_d<0x101>(0.0, B));  // row_shl:1
+    compose(A, B, dpp_d<0x102>(1.0, A), dpp_d<0x102>(0.0, B));  // row_shl:2
+    compose(A, B, dpp_d<0x104>(1.0, A), dpp_d<0x104>(0.0, B));  // row_shl:4
+    compose(A, B, dpp_d<0x108>(1.0, A), dpp_d<0x108>(0.0, B));  // row_shl:8
+    const int row = lane >> 4;
+    double A16 = readlane_d(A, 16), B16 = readlane_d(B, 16);
+    double A48 = readlane_d(A, 48), B48 = readlane_d(B, 48);
+    if (row == 0) compose(A, B, A16, B16);
+    if (row == 2) compose(A, B, A48, B48);
+    double A32 = readlane_d(A, 32), B32 = readlane_d(B, 32);
+    if (row <= 1) compose(A, B, A32, B32);
+}
+
+// LS_LINES lines per workgroup (4 waves, LS_LINES/4 consecutive lines each);
+// LDS tile [LS_LINES][64*C + 1] doubles for the transposed write.
 template <int C, int LS_LINES>
 __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
 {
     extern __shared__ double sres[];
+    constexpr int LPW = LS_LINES / 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int L = blockIdx.y;  // mode plane
     const int line0 = blockIdx.x * LS_LINES;
     const int lenp = 64 * C + 1;
     const double *xin = a.xin + (long)L * a.SA;
     const double *rh = a.rh + (long)L * a.SA;
-    const double *alpha = a.alpha + (long)L * a.coef_stride;
-    const double *ibet = a.ibet + (long)L * a.coef_stride;
-    const double *gback = a.gback + (long)L * a.coef_stride;
     const int len = a.len;
     const double dtt_mtwogamma = a.dt - 2.0 * a.gamma;
 
-    for (int jl = wave; jl < LS_LINES; jl += 4) {
-        const int i = line0 + jl;  // interior line index, row i+1 of the input layout
-        if (i >= a.nlines) break;
-        const double *xp = xin + (long)i * a.PA;        // row i   (previous line)
-        const double *x0 = xin + (long)(i + 1) * a.PA;  // row i+1 (this line)
-        const double *xn = xin + (long)(i + 2) * a.PA;  // row i+2 (next line)
-        const double *r0 = rh + (long)(i + 1) * a.PA;
-        double rhs[C];
-        // line-constant RHS coefficients
+    // Thomas factors: the same for every line of this mode plane
+    double fa[C], fi[C], fg[C];
+    {
+        const double *alpha = a.alpha + (long)L * a.coef_stride;
+        const double *ibet = a.ibet + (long)L * a.coef_stride;
+        const double *gback = a.gback + (long)L * a.coef_stride;
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            const int e = 64 * s + lane;
+            const bool ok = e < len;
+            fa[s] = ok ? alpha[e] : 0.0;
+            fi[s] = ok ? ibet[e] : 0.0;
+            fg[s] = ok ? gback[e] : 0.0;
+        }
+    }
+    const int first = line0 + wave * LPW;          // first line of this wave
+    const int nmine = min(LPW, a.nlines - first);  // may be <= 0
+    double xm[C], xc[C], xn[C], rc[C], rn[C];
+    auto load_row = [&](const double *row, double *dst) {
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            const int e = 64 * s + lane;
+            dst[s] = (e < len) ? row[e + 1] : 0.0;
+        }
+    };
+    if (nmine > 0) {
+        load_row(xin + (long)first * a.PA, xm);
+        load_row(xin + (long)(first + 1) * a.PA, xc);
+        load_row(xin + (long)(first + 2) * a.PA, xn);
+        load_row(rh + (long)(first + 1) * a.PA, rc);
+    }
+    for (int k = 0; k < nmine; ++k) {
+        const int i = first + k;  // interior line, row i+1 of the input layout
+        const double *x0 = xin + (long)(i + 1) * a.PA;
+        // boundary values of this line (positions 0 and len+1)
+        const double b0 = x0[0], b1 = x0[len + 1];
+        double xnn[C];
+        const bool more = (k + 1 < nmine);
+        if (more) {
+            load_row(xin + (long)(i + 3) * a.PA, xnn);
+            load_row(rh + (long)(i + 2) * a.PA, rn);
+        }
         double cd, cp, cn;
         if (a.dir == 0) {
             cd = dtt_mtwogamma; cp = a.gamma; cn = a.gamma;
@@ -197,60 +282,59 @@ __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
             cn = -a.alphar[J];
             cp = (J > 2) ? -a.betar[J] : 0.0;
         }
+        double v[C];
 #pragma unroll
         for (int s = 0; s < C; ++s) {
             const int e = 64 * s + lane;
-            double v = 0.0;
+            double t = 0.0;
             if (e < len) {
-                const int p = e + 1;
                 if (a.dir == 0) {
                     // knownr (helmadi.F90:240-257)
-                    v = -a.four_pi * r0[p] + cd * x0[p] + cp * (xn[p] + xp[p]);
-                    if (e == len - 1) v = v - a.alphar[a.numr - 1] * x0[len + 1];
+                    t = -a.four_pi * rc[s] + cd * xc[s] + cp * (xn[s] + xm[s]);
+                    if (e == len - 1) t = t - a.alphar[a.numr - 1] * b1;
                 } else {
                     // knownz (helmadi.F90:284-340)
-                    v = -a.four_pi * r0[p] + cd * x0[p] + cn * xn[p];
-                    if (cp != 0.0) v = v + cp * xp[p];
-                    if (e == len - 1) v = v + a.gamma * x0[len + 1];
-                    if (e == 0) v = v + a.gamma * x0[0];
+                    t = -a.four_pi * rc[s] + cd * xc[s] + cn * xn[s];
+                    if (cp != 0.0) t = t + cp * xm[s];
+                    if (e == len - 1) t = t + a.gamma * b1;
+                    if (e == 0) t = t + a.gamma * b0;
                 }
             }
-            rhs[s] = v;
+            v[s] = t;
         }
         // forward sweep
         double carry = 0.0;
 #pragma unroll
         for (int s = 0; s < C; ++s) {
-            const int e = 64 * s + lane;
-            double A = 0.0, B = 0.0;
-            if (e < len) { A = alpha[e]; B = ibet[e] * rhs[s]; }
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                double Ap = shfl_up_d(A, d), Bp = shfl_up_d(B, d);
-                if (lane >= d) { B = A * Bp + B; A = A * Ap; }
-            }
+            double A = fa[s], B = fi[s] * v[s];
+            scan_up(A, B, lane);
             double y = A * carry + B;
-            rhs[s] = y;
-            carry = bcast_d(y, 63);
+            v[s] = y;
+            carry = readlane_d(y, 63);
         }
         // back substitution
         carry = 0.0;
 #pragma unroll
         for (int s = C - 1; s >= 0; --s) {
-            const int e = 64 * s + lane;
-            double A = 0.0, B = 0.0;
-            if (e < len) { A = gback[e]; B = rhs[s]; }
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                double An = shfl_down_d(A, d), Bn = shfl_down_d(B, d);
-                if (lane + d < 64) { B = A * Bn + B; A = A * An; }
-            }
+            double A = fg[s], B = v[s];
+            scan_down(A, B, lane);
             double x = A * carry + B;
-            rhs[s] = x;
-            carry = bcast_d(x, 0);
+            v[s] = x;
+            carry = readlane_d(x, 0);
         }
+        const int jl = wave * LPW + k;
 #pragma unroll
-        for (int s = 0; s < C; ++s) sres[jl * lenp + 64 * s + lane] = rhs[s];
+        for (int s = 0; s < C; ++s) sres[jl * lenp + 64 * s + lane] = v[s];
+        // roll the neighbour rows
+#pragma unroll
+        for (int s = 0; s < C; ++s) {
+            xm[s] = xc[s];
+            xc[s] = xn[s];
+            if (more) {
+                xn[s] = xnn[s];
+                rc[s] = rn[s];
+            }
+        }
     }
     __syncthreads();
     // transposed, coalesced write: element e of line i -> out[(e+1)*PB + i+1]
@@ -442,21 +526,9 @@ void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
 // isym = 1 grid (bm.F90 vs tm.F90), so each table row is contracted with the
 // projections and with their z-mirror.  Deterministic block reduction.
 // ---------------------------------------------------------------------------
-template <int NV>
-__device__ void block_reduce_store(double (&v)[NV], double *out, long ostride)
-{
-    __shared__ double red[NV][256];
-    for (int k = 0; k < NV; ++k) red[k][threadIdx.x] = v[k];
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w)
-            for (int k = 0; k < NV; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < NV; ++k) out[k * ostride] = red[k][0];
-}
+constexpr int CT_TGT = 8;  // contraction targets per workgroup
 
+// Top and bottom: Stb[m][q][jo], q = top cos, top sin, bottom cos, bottom sin
 __global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__restrict__ T,
                                                               const double *__restrict__ proj,
                                                               double *__restrict__ Stb, int numr,
@@ -464,26 +536,49 @@ __global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__re
 {
     const int nri = numr - 2, nzi = numz - 2;
     const long npts = (long)nri * nzi, plane = (long)numr * numz;
-    const int jo = blockIdx.x, m = blockIdx.y;
-    const double *t = T + ((long)m * nri + jo) * npts;
+    const int jo0 = blockIdx.x * CT_TGT, m = blockIdx.y;
+    const int nt = min(CT_TGT, nri - jo0);
+    const double *t = T + ((long)m * nri + jo0) * npts;
     const double *pc = proj + (long)m * plane;
     const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
-    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    double v[4 * CT_TGT];
+#pragma unroll
+    for (int k = 0; k < 4 * CT_TGT; ++k) v[k] = 0.0;
     for (long idx = threadIdx.x; idx < npts; idx += 256) {
         int kz = (int)(idx / nri), jr = (int)(idx % nri);
-        double g = t[idx];
         long pa = (long)(kz + 1) * numr + jr + 1;
         long pm = (long)(nzi - kz) * numr + jr + 1;
-        v[0] += g * pc[pa];
-        v[2] += g * pc[pm];
-        if (m > 0) {
-            v[1] += g * ps[pa];
-            v[3] += g * ps[pm];
+        double c0 = pc[pa], c1 = pc[pm];
+        double s0 = (m > 0) ? ps[pa] : 0.0, s1 = (m > 0) ? ps[pm] : 0.0;
+#pragma unroll
+        for (int q = 0; q < CT_TGT; ++q) {
+            if (q < nt) {
+                double g = t[q * npts + idx];
+                v[4 * q + 0] += g * c0;
+                v[4 * q + 1] += g * s0;
+                v[4 * q + 2] += g * c1;
+                v[4 * q + 3] += g * s1;
+            }
         }
     }
-    block_reduce_store<4>(v, Stb + (long)m * 4 * nri + jo, nri);
+    __shared__ double red[4][4 * CT_TGT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 4 * CT_TGT; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (lane == 0) red[wave][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4 * nt) {
+        int q = threadIdx.x / 4, c = threadIdx.x % 4;
+        int k = threadIdx.x;
+        Stb[((long)m * 4 + c) * nri + jo0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+    }
 }
 
+// Side: Ss[m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
 __global__ __launch_bounds__(256) void contract_side_kernel(const double *__restrict__ S,
                                                             const double *__restrict__ proj,
                                                             double *__restrict__ Ss, int numr, int numz,
@@ -491,35 +586,59 @@ __global__ __launch_bounds__(256) void contract_side_kernel(const double *__rest
 {
     const int nri = numr - 2, nzi = numz - 2;
     const long plane = (long)numr * numz;
-    const int ko = blockIdx.x, m = blockIdx.y;  // target kg = ko + 2
+    const int ko0 = blockIdx.x * CT_TGT, m = blockIdx.y;
+    const int nt = min(CT_TGT, nzi - ko0);
     const int nd = 2 * nzi - 1;
     const double *sm = S + (long)m * nd * nri;
     const double *pc = proj + (long)m * plane;
     const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
-    double v[2] = {0.0, 0.0};
+    double v[2 * CT_TGT];
+#pragma unroll
+    for (int k = 0; k < 2 * CT_TGT; ++k) v[k] = 0.0;
     const long npts = (long)nri * nzi;
     for (long idx = threadIdx.x; idx < npts; idx += 256) {
         int kz = (int)(idx / nri), jr = (int)(idx % nri);
-        double g = sm[(long)(ko - kz + nzi - 1) * nri + jr];
         long pa = (long)(kz + 1) * numr + jr + 1;
-        v[0] += g * pc[pa];
-        if (m > 0) v[1] += g * ps[pa];
+        double c0 = pc[pa], s0 = (m > 0) ? ps[pa] : 0.0;
+        const double *g0 = sm + (long)(ko0 - kz + nzi - 1) * nri + jr;
+#pragma unroll
+        for (int q = 0; q < CT_TGT; ++q) {
+            if (q < nt) {
+                double g = g0[(long)q * nri];
+                v[2 * q + 0] += g * c0;
+                v[2 * q + 1] += g * s0;
+            }
+        }
     }
-    block_reduce_store<2>(v, Ss + (long)m * 2 * nzi + ko, nzi);
+    __shared__ double red[4][2 * CT_TGT];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 2 * CT_TGT; ++k) {
+        double x = v[k];
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+        if (lane == 0) red[wave][k] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 * nt) {
+        int q = threadIdx.x / 2, c = threadIdx.x % 2;
+        int k = threadIdx.x;
+        Ss[((long)m * 2 + c) * nzi + ko0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+    }
 }
 
 void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL(contract_topbot_kernel, dim3(numr - 2, mmax), dim3(256), 0, s, T, proj, Stb, numr,
-                       numz, mmax);
+    hipLaunchKernelGGL(contract_topbot_kernel, dim3((numr - 2 + CT_TGT - 1) / CT_TGT, mmax), dim3(256), 0, s, T,
+                       proj, Stb, numr, numz, mmax);
 }
 
 void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax,
                           hipStream_t s)
 {
-    hipLaunchKernelGGL(contract_side_kernel, dim3(numz - 2, mmax), dim3(256), 0, s, S, proj, Ss, numr, numz,
-                       mmax);
+    hipLaunchKernelGGL(contract_side_kernel, dim3((numz - 2 + CT_TGT - 1) / CT_TGT, mmax), dim3(256), 0, s, S,
+                       proj, Ss, numr, numz, mmax);
 }
 
 // physical faces: factor * (S_C(0) + sum_m 2 S_C(m) cos + 2 S_S(m) sin)

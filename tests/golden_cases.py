@@ -27,3 +27,12 @@ def load_case(golden_dir, name):
     # the density generator must still produce the inputs the reference saw
     np.testing.assert_allclose([rho1.sum(), rho2.sum(), np.abs(rho1).max()], g["rho_sums"], rtol=1e-14)
     return (numr, numz, numphi), rho1, rho2, g["pot1"], g["pot2"]
+
+
+def c2_slices(pot, tag):
+    """The parts of a 258 x 130 x 512 potential kept in c2_258x130x512.npz
+    for solve `tag` ("1" cold, "2" warm); see make_golden.c2_case."""
+    if tag == "1":
+        return {"mid1": pot[:, 64, :], "mer1": pot[0], "top1": pot[::8, -1, :], "bot1": pot[::8, 0, :],
+                "side1": pot[::8, :, -1]}
+    return {"mid2": pot[::4, 64, :], "mer2": pot[0]}
