@@ -1,0 +1,327 @@
+// bh_mixed.hip -- device code of the mixed-precision solve.
+//
+// The reference's Peaceman-Rachford iteration (helmadi.F90:195-355) uses a
+// radial operator B_r that acts along J only and a vertical operator B_z that
+// acts along K only, so the two commute and N steps from x0 give exactly
+//     x_N = x* + M (x0 - x*),   A x* = g,   A = B_r + B_z,
+//     M = prod_s (dt_s - B_r)(dt_s + B_r)^-1 (dt_s - B_z)(dt_s + B_z)^-1,
+// i.e. x_N solves A x_N = g - M r0 with r0 = g - A x0.  B_z = gamma
+// tridiag(-1, 2, -1) is diagonalised by the sine transform S, so M is
+// mu_z(k) x M_r in (sine mode k, J) coordinates, with M_r a dense nri x nri
+// matrix per azimuthal mode built once per plan.  The mixed solve evaluates
+// this with FP32 MFMA products (sine transforms, the M_r product) and FP64
+// residuals of A x = g - w in physical (J, K) coordinates, then refines once.
+//
+//   gemm_f32        batched FP32 GEMM on v_mfma_f32_16x16x4f32, optional
+//                   transposed operands, epilogues: plain, rhop mask, filter
+//   residual64      r = g - A x (- w), optionally after x += delta; FP64
+//   tridiag_shift   (B_r + lambda_z(k)) y = c for every (mode, k), FP64
+//   build_mr        M_r for one pseudo-timestep schedule (plan time), FP64
+#include "bh_internal.h"
+#include "bh_mixed.h"
+
+#include <hip/hip_runtime.h>
+
+namespace bh {
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// v_mfma_f32_16x16x4f32 operand layout (lane l): A[l%16][l/16],
+// B[l/16][l%16]; D element v of lane l is D[4*(l/16)+v][l%16] (measured on
+// gfx950 with a one-hot operand probe).
+constexpr int FBM = 64, FBN = 64, FBK = 16;
+
+template <bool TA, bool TB, typename BT, int EPI>
+__global__ __launch_bounds__(256) void gemm_f32_kernel(GemmF32Args g)
+{
+    __shared__ float As[FBK][FBM + 4];
+    __shared__ float Bs[FBK][FBN + 4];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int row0 = blockIdx.y * FBM, col0 = blockIdx.x * FBN;
+    const long z = blockIdx.z;
+    const float *A = g.A + z * g.sA;
+    const BT *B = (const BT *)g.B + z * g.sB;
+    const int R = g.R, M = g.M, Kd = g.Kd;
+
+    float4_t acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = 0; k0 < Kd; k0 += FBK) {
+        for (int t = tid; t < FBM * FBK; t += 256) {
+            int r, k;
+            if (TA) { k = t / FBM; r = t % FBM; }
+            else { r = t / FBK; k = t % FBK; }
+            int gr = row0 + r, gk = k0 + k;
+            float v = 0.f;
+            if (gr < R && gk < Kd) v = TA ? A[(long)gk * g.lda + gr] : A[(long)gr * g.lda + gk];
+            As[k][r] = v;
+        }
+        for (int t = tid; t < FBK * FBN; t += 256) {
+            int k, c;
+            if (TB) { c = t / FBK; k = t % FBK; }
+            else { k = t / FBN; c = t % FBN; }
+            int gk = k0 + k, gc = col0 + c;
+            float v = 0.f;
+            if (gk < Kd && gc < M) v = (float)(TB ? B[(long)gc * g.ldb + gk] : B[(long)gk * g.ldb + gc]);
+            Bs[k][c] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < FBK; kk += 4) {
+            float a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wr * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    float *C = g.C + z * g.sC;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                int r = row0 + wr * 32 + i * 16 + 4 * (lane >> 4) + v;
+                int c = col0 + wc * 32 + j * 16 + (lane & 15);
+                if (r >= R || c >= M) continue;
+                float val = acc[i][j][v];
+                const long o = (long)r * g.ldc + c;
+                if (EPI == EPI_MASK) {
+                    // rhop zeroing (potential_solver.F90:70-78), c = (K-1)*numr + J-1
+                    int J = c % g.mask_numr + 1, K = c / g.mask_numr + 1;
+                    if (J >= g.mask_numr - 1 || K <= 2 || K >= g.mask_numz - 1) val = 0.f;
+                    C[o] = val;
+                } else if (EPI == EPI_FILTER) {
+                    // W = mu_z(c) * acc;  C2 = Rin - W
+                    float w = g.mu[c] * val;
+                    C[o] = w;
+                    g.C2[z * g.sC + o] = g.Rin[z * g.sC + o] - w;
+                } else {
+                    C[o] = val;
+                }
+            }
+}
+
+template <bool TA, bool TB, typename BT, int EPI>
+static void launch_f32(const GemmF32Args &g, hipStream_t s)
+{
+    dim3 grid((g.M + FBN - 1) / FBN, (g.R + FBM - 1) / FBM, g.batch);
+    hipLaunchKernelGGL((gemm_f32_kernel<TA, TB, BT, EPI>), grid, dim3(256), 0, s, g);
+}
+
+void launch_gemm_f32(const GemmF32Args &g, bool ta, bool tb, bool b_double, int epi, hipStream_t s)
+{
+    if (b_double) {
+        if (!ta && !tb && epi == EPI_MASK) launch_f32<false, false, double, EPI_MASK>(g, s);
+        else if (!ta && !tb) launch_f32<false, false, double, EPI_PLAIN>(g, s);
+        return;
+    }
+    if (ta && !tb) launch_f32<true, false, float, EPI_PLAIN>(g, s);
+    else if (!ta && tb) launch_f32<false, true, float, EPI_PLAIN>(g, s);
+    else if (!ta && !tb && epi == EPI_FILTER) launch_f32<false, false, float, EPI_FILTER>(g, s);
+    else launch_f32<false, false, float, EPI_PLAIN>(g, s);
+}
+
+// ---------------------------------------------------------------------------
+// r = -4 pi rho-hat - A x - w on the interior, with x = X + dlt when dlt is
+// given (then also written to Xo).  A x = B_r x + B_z x as the ADI defines
+// them (potsetup.F90:175-228): B_r x = brb x + betar x(J-1) + alphar x(J+1)
+// (no J-1 term at J = 2), B_z x = 2 gamma x - gamma (x(K-1) + x(K+1)); face
+// values enter through the neighbours.  All of X, dlt, w, rh are full
+// (numr x numz) planes in J-layout; dlt and w have zero faces.  r is written
+// compact: r[(K-2)*nri + (J-2)].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void residual64_kernel(ResidualArgs a)
+{
+    const int nri = a.numr - 2, nzi = a.numz - 2;
+    const int L = blockIdx.y;
+    const long plane = (long)a.numr * a.numz;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nri * nzi) return;
+    int jr = t % nri, kz = t / nri;
+    int J = jr + 2, K = kz + 2;
+    const double *X = a.X + L * plane;
+    const float *dl = a.dlt ? a.dlt + L * plane : nullptr;
+    long c = (long)(K - 1) * a.numr + (J - 1);
+    auto val = [&](long i) { return dl ? X[i] + (double)dl[i] : X[i]; };
+    double xc = val(c), xe = val(c + 1), xs = val(c - a.numr), xn = val(c + a.numr);
+    double Ax = a.brb[(long)L * (a.numr + 1) + J] * xc + a.alphar[J] * xe;
+    if (J > 2) Ax += a.betar[J] * val(c - 1);
+    Ax += 2.0 * a.gamma * xc - a.gamma * (xs + xn);
+    double r = -a.four_pi * (double)a.rh[L * plane + c] - Ax;
+    if (a.w) r -= (double)a.w[L * plane + c];
+    if (a.Xo) a.Xo[L * plane + c] = xc;
+    if (a.r) a.r[(long)L * nri * nzi + (long)kz * nri + jr] = (float)r;
+}
+
+void launch_residual64(const ResidualArgs &a, int nplanes, hipStream_t s)
+{
+    int n = (a.numr - 2) * (a.numz - 2);
+    dim3 grid((n + 255) / 256, nplanes);
+    hipLaunchKernelGGL(residual64_kernel, grid, dim3(256), 0, s, a);
+}
+
+// Xo = X + dlt on the interior (faces of Xo are kept)
+__global__ void add_update_kernel(const double *X, const float *dlt, double *Xo, int numr, int numz)
+{
+    const long plane = (long)numr * numz;
+    const int L = blockIdx.y;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int nri = numr - 2, nzi = numz - 2;
+    if (t >= nri * nzi) return;
+    int jr = t % nri, kz = t / nri;
+    long c = L * plane + (long)(kz + 1) * numr + (jr + 1);
+    Xo[c] = X[c] + (double)dlt[c];
+}
+
+void launch_add_update(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes,
+                       hipStream_t s)
+{
+    int n = (numr - 2) * (numz - 2);
+    dim3 grid((n + 255) / 256, nplanes);
+    hipLaunchKernelGGL(add_update_kernel, grid, dim3(256), 0, s, X, dlt, Xo, numr, numz);
+}
+
+// ---------------------------------------------------------------------------
+// (B_r(mode L) + lambda_z(k)) y = c along J for every (L, k); data layout
+// [L][J'][k] (k fastest, one thread per k: coalesced).  LU factors of the
+// shifted operator come precomputed per (L, J', k): ibet and gam (FP64).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void tridiag_shift_kernel(const float *__restrict__ cin, float *__restrict__ yout,
+                                                            const double *__restrict__ ibet,
+                                                            const double *__restrict__ gam,
+                                                            const double *__restrict__ betar, int nri, int nzi)
+{
+    const int L = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nzi) return;
+    const long base = (long)L * nri * nzi + k;
+    const double *ib = ibet + base, *gm = gam + base;
+    // forward: y_J = (c_J - a_J y_{J-1}) / bet_J, a_J = betar(J) for J > 2
+    double y = (double)cin[base] * ib[0];
+    yout[base] = (float)y;
+    for (int j = 1; j < nri; ++j) {
+        long o = (long)j * nzi;
+        y = ((double)cin[base + o] - betar[j + 2] * y) * ib[o];
+        yout[base + o] = (float)y;
+    }
+    // back: x_J = y_J - gam_{J+1} x_{J+1}
+    double x = y;
+    for (int j = nri - 2; j >= 0; --j) {
+        long o = (long)j * nzi;
+        x = (double)yout[base + o] - gm[o + nzi] * x;
+        yout[base + o] = (float)x;
+    }
+}
+
+void launch_tridiag_shift(const float *cin, float *yout, const double *ibet, const double *gam,
+                          const double *betar, int nri, int nzi, int nplanes, hipStream_t s)
+{
+    dim3 grid((nzi + 255) / 256, nplanes);
+    hipLaunchKernelGGL(tridiag_shift_kernel, grid, dim3(256), 0, s, cin, yout, ibet, gam, betar, nri, nzi);
+}
+
+// ---------------------------------------------------------------------------
+// Plan time: M_r(L) = prod_s (dt_s - B_r)(dt_s + B_r)^-1 column by column,
+// one thread per (mode, column); FP64 work vectors in scratch laid out
+// [J][thread] for coalescing.  Output Mr[L][J][j'] as FP32.
+// ---------------------------------------------------------------------------
+__global__ void build_mr_kernel(MrArgs a)
+{
+    const int nri = a.nri;
+    const long nthr = (long)a.nmodes * nri;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nthr) return;
+    const int Lloc = (int)(t / nri), jc = (int)(t % nri);
+    const int L = a.mode0 + Lloc;
+    double *v = a.scratch_v + t, *u = a.scratch_u + t, *gm = a.scratch_g + t;
+    const long st = nthr;  // stride between J entries
+    for (int j = 0; j < nri; ++j) v[j * st] = (j == jc) ? 1.0 : 0.0;
+    const double *brb = a.brb + (long)L * (a.numr + 1);
+    for (int s = 0; s < a.nsteps; ++s) {
+        const double dt = a.dt[s];
+        // u = (dt - B_r) v
+        for (int j = 0; j < nri; ++j) {
+            const int J = j + 2;
+            double bv = brb[J] * v[j * st];
+            if (j > 0) bv += a.betar[J] * v[(j - 1) * st];
+            if (j < nri - 1) bv += a.alphar[J] * v[(j + 1) * st];
+            u[j * st] = dt * v[j * st] - bv;
+        }
+        // (dt + B_r) v = u  (Thomas, tridagr.F90 form)
+        double bet = brb[2] + dt;
+        double y = u[0] / bet;
+        v[0] = y;
+        for (int j = 1; j < nri; ++j) {
+            const int J = j + 2;
+            double g = a.alphar[J - 1] / bet;
+            gm[j * st] = g;
+            bet = (brb[J] + dt) - a.betar[J] * g;
+            y = (u[j * st] - a.betar[J] * y) / bet;
+            v[j * st] = y;
+        }
+        for (int j = nri - 2; j >= 0; --j) v[j * st] = v[j * st] - gm[(j + 1) * st] * v[(j + 1) * st];
+    }
+    float *out = a.Mr + (long)L * nri * nri;
+    for (int j = 0; j < nri; ++j) out[(long)j * nri + jc] = (float)v[j * st];
+}
+
+void launch_build_mr(const MrArgs &a, hipStream_t s)
+{
+    long nthr = (long)a.nmodes * a.nri;
+    hipLaunchKernelGGL(build_mr_kernel, dim3((unsigned)((nthr + 127) / 128)), dim3(128), 0, s, a);
+}
+
+__global__ void fillf_kernel(float *p, size_t n, float v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+void launch_fillf(float *p, size_t n, float v, hipStream_t s)
+{
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fillf_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+// Face values into a J-layout potential (top K = numz, bottom K = 1, side
+// J = numr; outer corners zero, bessel.F90:428, 456, 487)
+__global__ void face_scatter_j_kernel(const double *__restrict__ FF, double *__restrict__ X, int numr, int numz)
+{
+    const int nri = numr - 2, nzi = numz - 2;
+    const int nf = 2 * nri + nzi;
+    const int L = blockIdx.y;
+    const long plane = (long)numr * numz;
+    double *x = X + L * plane;
+    const double *ff = FF + (long)L * nf;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nri) x[(long)(numz - 1) * numr + t + 1] = ff[t];
+    else if (t < 2 * nri) x[t - nri + 1] = ff[t];
+    else if (t < nf) x[(long)(t - 2 * nri + 1) * numr + (numr - 1)] = ff[t];
+    else if (t == nf) {
+        x[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
+        x[numr - 1] = 0.0;
+    }
+}
+
+void launch_face_scatter_j(const double *FF, double *X, int numr, int numz, int numphi, hipStream_t s)
+{
+    int nf = 2 * (numr - 2) + (numz - 2) + 1;
+    dim3 grid((nf + 127) / 128, numphi);
+    hipLaunchKernelGGL(face_scatter_j_kernel, grid, dim3(128), 0, s, FF, X, numr, numz);
+}
+
+}  // namespace bh

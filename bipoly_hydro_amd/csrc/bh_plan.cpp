@@ -3,11 +3,13 @@
 // include/bh_potential.h.
 #include "../../include/bh_potential.h"
 #include "bh_internal.h"
+#include "bh_mixed.h"
 
 #include <hip/hip_runtime.h>
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -73,6 +75,13 @@ struct bh_plan {
     double *rc_alpha[2] = {nullptr, nullptr}, *rc_ibet[2] = {nullptr, nullptr}, *rc_gback[2] = {nullptr, nullptr};
     double *zc_alpha[2] = {nullptr, nullptr}, *zc_ibet[2] = {nullptr, nullptr}, *zc_gback[2] = {nullptr, nullptr};
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
+    // mixed precision (BH_PREC_MIXED)
+    int passes = 1;                                // FP64 refinement passes
+    float *Dfwd32 = nullptr, *S32 = nullptr, *mu32[2] = {nullptr, nullptr}, *Mr[2] = {nullptr, nullptr};
+    double *tri_ibet = nullptr, *tri_gam = nullptr;
+    float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
+    float *D1 = nullptr, *Wf = nullptr;
+    double *brb_d = nullptr;
     std::vector<double> dt[2];                     // 0 cold, 1 warm
     bool has_solution = false;
     bool timing = false;
@@ -101,6 +110,12 @@ int dalloc(bh_plan *p, double **out, size_t n)
 int upload(double *dst, const std::vector<double> &src)
 {
     HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(double), hipMemcpyHostToDevice));
+    return BH_OK;
+}
+
+int upload_f(float *dst, const std::vector<float> &src)
+{
+    HIPCHK(hipMemcpy(dst, src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice));
     return BH_OK;
 }
 
@@ -134,7 +149,8 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
     if (d.numr - 2 > 512 || d.numz - 2 > 512) return fail(BH_E_INVALID, "numr-2 and numz-2 must be <= 512");
     if (d.nsteps_cold < 2 || d.nsteps_warm < 2) return fail(BH_E_INVALID, "nsteps must be >= 2");
     if (!(d.dr > 0.0) || !std::isfinite(d.dr)) return fail(BH_E_INVALID, "dr must be positive");
-    if (precision != BH_PREC_FP64) return fail(BH_E_INVALID, "precision: only BH_PREC_FP64 is available");
+    if (precision != BH_PREC_FP64 && precision != BH_PREC_MIXED)
+        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64 or BH_PREC_MIXED");
 
     bh_plan *p = new bh_plan();
     p->d = d;
@@ -209,8 +225,11 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
             return rc;                                    \
         }                                                 \
     } while (0)
-    ALLOC(p->RJ, M * N);
-    ALLOC(p->RK, M * N);
+    const bool mixed = (precision == BH_PREC_MIXED);
+    if (!mixed) {
+        ALLOC(p->RJ, M * N);
+        ALLOC(p->RK, M * N);
+    }
     ALLOC(p->XJ, M * N);
     ALLOC(p->XK, M * N);
     ALLOC(p->proj, M * (2 * mmax - 1));
@@ -228,7 +247,7 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
     ALLOC(p->alphar, numr + 1);
     ALLOC(p->betar, numr + 1);
     ALLOC(p->elzb, (size_t)N * (numr + 1));
-    for (int w = 0; w < 2; ++w) {
+    for (int w = 0; w < 2 && !mixed; ++w) {
         int ns = (int)p->dt[w].size();
         ALLOC(p->rc_alpha[w], (size_t)ns * N * nri);
         ALLOC(p->rc_ibet[w], (size_t)ns * N * nri);
@@ -306,7 +325,7 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
     UPLOAD(p->betar, betar);
     UPLOAD(p->elzb, elzb);
     // Thomas factors per pseudo-timestep (tridagr.F90:41-60, tridagz.F90:115-132)
-    for (int w = 0; w < 2; ++w) {
+    for (int w = 0; w < 2 && !mixed; ++w) {
         const std::vector<double> &dt = p->dt[w];
         int ns = (int)dt.size();
         std::vector<double> ra((size_t)ns * N * nri), ri((size_t)ns * N * nri), rg((size_t)ns * N * nri);
@@ -381,6 +400,137 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
         launch_gen_top_table(ta, p->T, 0);
         launch_gen_side_table(ta, p->S, 0);
     }
+
+    if (mixed) {
+#define MALLOCF(ptr, n)                                              \
+    do {                                                             \
+        double *tmp_ = nullptr;                                      \
+        if ((rc = dalloc(p, &tmp_, ((size_t)(n) + 1) / 2)) != BH_OK) { \
+            free_all(p);                                             \
+            delete p;                                                \
+            return rc;                                               \
+        }                                                            \
+        (ptr) = (float *)tmp_;                                       \
+    } while (0)
+#define MALLOCD(ptr, n)                                      \
+    do {                                                     \
+        if ((rc = dalloc(p, &(ptr), (size_t)(n))) != BH_OK) { \
+            free_all(p);                                     \
+            delete p;                                        \
+            return rc;                                       \
+        }                                                    \
+    } while (0)
+        const size_t PI = (size_t)N * nri * nzi;  // interior values, all modes
+        MALLOCF(p->Dfwd32, (size_t)N * N);
+        MALLOCF(p->S32, (size_t)nzi * nzi);
+        MALLOCF(p->mu32[0], nzi);
+        MALLOCF(p->mu32[1], nzi);
+        MALLOCF(p->Mr[0], (size_t)N * nri * nri);
+        MALLOCF(p->Mr[1], (size_t)N * nri * nri);
+        MALLOCD(p->tri_ibet, PI);
+        MALLOCD(p->tri_gam, PI);
+        MALLOCF(p->RH32, (size_t)M * N);
+        MALLOCF(p->r32, PI);
+        MALLOCF(p->RT, PI);
+        MALLOCF(p->WT, PI);
+        MALLOCF(p->CT, PI);
+        MALLOCF(p->DT, PI);
+        MALLOCF(p->D1, (size_t)M * N);
+        MALLOCF(p->Wf, (size_t)M * N);
+        MALLOCD(p->brb_d, (size_t)N * (numr + 1));
+        UPLOAD(p->brb_d, brb);
+        // FP32 forward DFT rows (helmadi's mode order), sine transform,
+        // lambda_z, mu_z for both schedules
+        {
+            std::vector<float> D32((size_t)N * N), S((size_t)nzi * nzi);
+            for (int k = 0; k < N; ++k)
+                for (int n = 0; n < N; ++n) {
+                    long kn;
+                    double v;
+                    if (k <= N / 2) { kn = ((long)k * n) % N; v = std::cos(2.0 * pi * (double)kn / N); }
+                    else { kn = ((long)(k - N / 2) * n) % N; v = -std::sin(2.0 * pi * (double)kn / N); }
+                    D32[(size_t)k * N + n] = (float)v;
+                }
+            const double sn = std::sqrt(2.0 / (nzi + 1));
+            for (int a = 0; a < nzi; ++a)
+                for (int b = 0; b < nzi; ++b) {
+                    long ab = ((long)(a + 1) * (b + 1)) % (2 * (nzi + 1));
+                    S[(size_t)a * nzi + b] = (float)(sn * std::sin(pi * (double)ab / (nzi + 1)));
+                }
+            if ((rc = upload_f(p->Dfwd32, D32)) != BH_OK || (rc = upload_f(p->S32, S)) != BH_OK) {
+                free_all(p);
+                delete p;
+                return rc;
+            }
+            std::vector<double> lamz(nzi);
+            for (int k = 0; k < nzi; ++k) lamz[k] = 2.0 * g.gamma * (1.0 - std::cos(pi * (k + 1) / (nzi + 1)));
+            for (int w = 0; w < 2; ++w) {
+                std::vector<float> mu(nzi);
+                for (int k = 0; k < nzi; ++k) {
+                    double m = 1.0;
+                    for (double d : p->dt[w]) m *= (d - lamz[k]) / (d + lamz[k]);
+                    mu[k] = (float)m;
+                }
+                if ((rc = upload_f(p->mu32[w], mu)) != BH_OK) {
+                    free_all(p);
+                    delete p;
+                    return rc;
+                }
+            }
+            // LU factors of B_r(L) + lambda_z(k), layout [L][J'][k]
+            std::vector<double> ib(PI), gm(PI);
+            for (int L = 0; L < N; ++L) {
+                const double *b = &brb[(size_t)L * (numr + 1)];
+                for (int k = 0; k < nzi; ++k) {
+                    size_t base = (size_t)L * nri * nzi + k;
+                    double bet = b[2] + lamz[k];
+                    ib[base] = 1.0 / bet;
+                    gm[base] = 0.0;
+                    for (int j = 1; j < nri; ++j) {
+                        int J = j + 2;
+                        double gg = alphar[J - 1] / bet;
+                        bet = (b[J] + lamz[k]) - betar[J] * gg;
+                        ib[base + (size_t)j * nzi] = 1.0 / bet;
+                        gm[base + (size_t)j * nzi] = gg;
+                    }
+                }
+            }
+            UPLOAD(p->tri_ibet, ib);
+            UPLOAD(p->tri_gam, gm);
+        }
+        // M_r(L) for both schedules, on the device in chunks of modes
+        {
+            const int chunk = 64;
+            double *sv = nullptr, *su = nullptr, *sg = nullptr, *ddt = nullptr;
+            size_t nsc = (size_t)chunk * nri * nri;
+            const size_t ndt = std::max(p->dt[0].size(), p->dt[1].size());
+            hipError_t e1 = hipMalloc(&sv, nsc * 8), e2 = hipMalloc(&su, nsc * 8), e3 = hipMalloc(&sg, nsc * 8),
+                       e4 = hipMalloc(&ddt, ndt * 8);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+                (void)hipFree(sv); (void)hipFree(su); (void)hipFree(sg); (void)hipFree(ddt);
+                free_all(p);
+                delete p;
+                return fail(BH_E_NOMEM, "plan setup: scratch for M_r");
+            }
+            for (int w = 0; w < 2; ++w) {
+                (void)hipMemcpy(ddt, p->dt[w].data(), p->dt[w].size() * 8, hipMemcpyHostToDevice);
+                for (int m0 = 0; m0 < N; m0 += chunk) {
+                    MrArgs ma;
+                    ma.brb = p->brb_d; ma.alphar = p->alphar; ma.betar = p->betar; ma.dt = ddt;
+                    ma.numr = numr; ma.nri = nri; ma.nsteps = (int)p->dt[w].size();
+                    ma.mode0 = m0; ma.nmodes = std::min(chunk, N - m0);
+                    ma.scratch_v = sv; ma.scratch_u = su; ma.scratch_g = sg; ma.Mr = p->Mr[w];
+                    launch_build_mr(ma, 0);
+                }
+                (void)hipDeviceSynchronize();
+            }
+            (void)hipFree(sv); (void)hipFree(su); (void)hipFree(sg); (void)hipFree(ddt);
+        }
+        launch_fillf(p->D1, (size_t)M * N, 0.f, 0);
+        launch_fillf(p->Wf, (size_t)M * N, 0.f, 0);
+#undef MALLOCF
+#undef MALLOCD
+    }
 #undef UPLOAD
     launch_fill(p->XJ, (size_t)M * N, 0.0, 0);
     launch_fill(p->XK, (size_t)M * N, 0.0, 0);
@@ -425,9 +575,110 @@ extern "C" int bh_last_timings(const bh_plan *p, double *ms, int n)
 
 namespace {
 
+
+// The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
+// iteration as the FP64 path (helmadi.F90:195-355), as A x = g - M r0 (see
+// bh_mixed.hip), with FP32 MFMA transforms and FP64 residuals.
+int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hipStream_t s)
+{
+    const Geo &g = p->g;
+    const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax, nri = g.nri, nzi = g.nzi;
+    const long M = (long)numr * numz;
+    const long PIm = (long)nri * nzi;  // interior values per mode
+    const int w = (tstep > 1) ? 1 : 0;
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[0], s));
+    // 1. rho-hat in FP32 (masked as rhop), projections in FP64
+    {
+        GemmF32Args a{};
+        a.A = p->Dfwd32; a.lda = N; a.sA = 0;
+        a.B = d_rho; a.ldb = M; a.sB = 0;
+        a.C = p->RH32; a.ldc = M; a.sC = 0;
+        a.R = N; a.M = (int)M; a.Kd = N; a.batch = 1;
+        a.mask_numr = numr; a.mask_numz = numz;
+        launch_gemm_f32(a, false, false, true, EPI_MASK, s);
+        GemmOut o{p->proj, M, nullptr, 0, 2 * mmax - 1, 0, 0};
+        launch_gemm_f64(p->Dfwd + (size_t)N * N, N, d_rho, M, 2 * mmax - 1, (int)M, N, o, s);
+    }
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
+    // 2. boundary faces (FP64), into both potential buffers
+    launch_contract_topbot(p->T, p->proj, p->Stb, numr, numz, mmax, s);
+    launch_contract_side(p->S, p->proj, p->Ss, numr, numz, mmax, s);
+    launch_face_rebuild(p->Stb, p->Ss, p->besc, p->bess, p->F, numr, numz, N, mmax, -g.dr * g.dz * g.dphi, s);
+    const int nf = 2 * nri + nzi;
+    GemmOut of{p->FF, nf, nullptr, 0, N, 0, 0};
+    launch_gemm_f64(p->Dfwd, N, p->F, nf, N, nf, N, of, s);
+    double *Xa = p->XJ, *Xb = p->XK;
+    if (tstep <= 1) launch_fill(Xa, (size_t)M * N, 0.0, s);
+    launch_face_scatter_j(p->FF, Xa, numr, numz, N, s);
+    launch_face_scatter_j(p->FF, Xb, numr, numz, N, s);
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[2], s));
+    // 3. r0 = g - A x0
+    ResidualArgs ra{};
+    ra.brb = p->brb_d; ra.alphar = p->alphar; ra.betar = p->betar;
+    ra.gamma = g.gamma; ra.four_pi = g.four_pi; ra.numr = numr; ra.numz = numz;
+    ra.rh = p->RH32;
+    ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
+    launch_residual64(ra, N, s);
+    // sine transform: RT[L] (nri x nzi) = r[L]^T S
+    auto dst_fwd = [&](const float *r, float *out) {
+        GemmF32Args a{};
+        a.A = r; a.lda = nri; a.sA = PIm;
+        a.B = p->S32; a.ldb = nzi; a.sB = 0;
+        a.C = out; a.ldc = nzi; a.sC = PIm;
+        a.R = nri; a.M = nzi; a.Kd = nzi; a.batch = N;
+        launch_gemm_f32(a, true, false, false, EPI_PLAIN, s);
+    };
+    // inverse: full-plane out[L] interior (nzi x nri) = S * in[L]^T
+    auto dst_inv = [&](const float *in, float *out) {
+        GemmF32Args a{};
+        a.A = p->S32; a.lda = nzi; a.sA = 0;
+        a.B = in; a.ldb = nzi; a.sB = PIm;
+        a.C = out + numr + 1; a.ldc = numr; a.sC = M;
+        a.R = nzi; a.M = nri; a.Kd = nzi; a.batch = N;
+        launch_gemm_f32(a, false, true, false, EPI_PLAIN, s);
+    };
+    dst_fwd(p->r32, p->RT);
+    // 4. W = mu_z . (M_r RT), C = RT - W
+    {
+        GemmF32Args a{};
+        a.A = p->Mr[w]; a.lda = nri; a.sA = (long)nri * nri;
+        a.B = p->RT; a.ldb = nzi; a.sB = PIm;
+        a.C = p->WT; a.ldc = nzi; a.sC = PIm;
+        a.R = nri; a.M = nzi; a.Kd = nri; a.batch = N;
+        a.mu = p->mu32[w]; a.Rin = p->RT; a.C2 = p->CT;
+        launch_gemm_f32(a, false, false, false, EPI_FILTER, s);
+    }
+    // 5. (B_r + lambda_z) dhat = C, back to physical space
+    launch_tridiag_shift(p->CT, p->DT, p->tri_ibet, p->tri_gam, p->betar, nri, nzi, N, s);
+    dst_inv(p->DT, p->D1);
+    dst_inv(p->WT, p->Wf);
+    // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
+    double *Xcur = Xa, *Xnext = Xb;
+    for (int it = 0; it < p->passes; ++it) {
+        ra.X = Xcur; ra.dlt = p->D1; ra.w = p->Wf; ra.Xo = Xnext; ra.r = p->r32;
+        launch_residual64(ra, N, s);
+        dst_fwd(p->r32, p->RT);
+        launch_tridiag_shift(p->RT, p->DT, p->tri_ibet, p->tri_gam, p->betar, nri, nzi, N, s);
+        dst_inv(p->DT, p->D1);
+        std::swap(Xcur, Xnext);
+    }
+    // final update into Xa, the warm-start state (pointwise, so in place is safe)
+    launch_add_update(Xcur, p->D1, Xa, numr, numz, N, s);
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
+    // 7. inverse DFT (FP64) and axis ghost
+    GemmOut oi{d_pot, M, nullptr, 0, N, 0, 0};
+    launch_gemm_f64(p->Dinv, N, Xa, M, N, (int)M, N, oi, s);
+    launch_axis(d_pot, numr, numz, N, s);
+    if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
+    HIPCHK(hipGetLastError());
+    p->has_solution = true;
+    return BH_OK;
+}
+
 // The solve sequence (potential_solver.F90:58-106 with bessel and helmadi).
 int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hipStream_t s)
 {
+    if (p->precision == BH_PREC_MIXED) return enqueue_solve_mixed(p, d_rho, d_pot, tstep, s);
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax, nri = g.nri, nzi = g.nzi;
     const long M = (long)numr * numz;

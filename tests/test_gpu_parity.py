@@ -111,3 +111,35 @@ def test_error_paths():
     with pytest.raises(P.BHError) as ei:
         P.PotentialPlan(66, 66, 64, hbm_budget=1 << 20)
     assert ei.value.code == -2
+
+
+# ---------------------------------------------------------------------------
+# Mixed precision: FP32 MFMA transforms, FP64 residual and refinement.  Bar:
+# relative L2 <= 1e-8 against the reference's FP64 output.
+TOL_MIXED = 1e-8
+
+
+@pytest.mark.parametrize("name", ["g42x26x16", "g34x34x32", "c1_66x66x64"])
+def test_mixed_matches_reference_cold_then_warm(golden_dir, name):
+    (numr, numz, numphi), rho1, rho2, ref1, ref2 = load_case(golden_dir, name)
+    with P.PotentialPlan(numr, numz, numphi, precision="mixed") as plan:
+        pot1 = plan.potential_solver(rho1, tstep=0)
+        pot2 = plan.potential_solver(rho2, tstep=2)
+    e1, e2 = rell2(pot1, ref1), rell2(pot2, ref2)
+    print(f"mixed {name}: cold relL2 {e1:.3e} warm relL2 {e2:.3e}", flush=True)
+    assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
+    assert np.all(pot1[:, 0, -1] == 0.0) and np.all(pot1[:, -1, -1] == 0.0)
+
+
+def test_mixed_production_grid_against_reference_slices(golden_dir):
+    path = os.path.join(golden_dir, "c2_258x130x512.npz")
+    g = np.load(path)
+    rho1 = density.config2_density(seed=int(g["seeds"][0]))
+    rho2 = rho1 * 1.01 + 1e-3 * density.config2_density(seed=int(g["seeds"][1]))
+    with P.PotentialPlan(258, 130, 512, precision="mixed") as plan:
+        pots = [plan.potential_solver(rho1, tstep=0), plan.potential_solver(rho2, tstep=2)]
+    for tag, pot in zip(("1", "2"), pots):
+        for k, v in c2_slices(pot, tag).items():
+            e = rell2(v, g[k])
+            print(f"mixed c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
+            assert e <= TOL_MIXED, (tag, k, e)
