@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbh_potential.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["bh_kernels.hip", "bh_mixed.hip", "bh_plan.cpp"]
+SOURCES = ["bh_kernels.hip", "bh_gemm.hip", "bh_mixed.hip", "bh_plan.cpp"]
 
 # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -31,18 +31,6 @@ struct LineArgs {
     int numr;
 };
 
-// GEMM epilogue: rows [0, rsplit) -> c0 (optional masking of the layers the
-// reference zeroes in rhop), rows [rsplit, R) -> c1.
-struct GemmOut {
-    double *c0; long ldc0;
-    double *c1; long ldc1;
-    int rsplit;
-    int mask_numr, mask_numz;  // 0 = no mask
-};
-
-void launch_gemm_f64(const double *A, long lda, const double *B, long ldb, int R, int M, int Kd,
-                     const GemmOut &out, hipStream_t s);
-
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
 
 void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int nplanes,
@@ -68,10 +56,12 @@ void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s);
 // side table in Toeplitz form S[m][d][jr], d = kg - kl + nzi - 1
 void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s);
 
-// boundary contraction and face rebuild (bessel.F90:176-296, 396-488)
+// boundary contraction and face rebuild (bessel.F90:176-296, 396-488);
+// the contractions write CT_SPLIT partial sums that the rebuild adds
+constexpr int CT_SPLIT = 4;
 // proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
-// Stb: [mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
-// Ss:  [mmax][2][nzi]
+// Stb: [CT_SPLIT][mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
+// Ss:  [CT_SPLIT][mmax][2][nzi]
 void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz,
                             int mmax, hipStream_t s);
 void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz,

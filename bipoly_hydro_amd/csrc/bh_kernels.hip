@@ -1,8 +1,5 @@
 // bh_kernels.hip -- CDNA4 (gfx950) device code of the self-gravity solve, FP64.
 //
-//   gemm_f64      dense phi-DFT / inverse DFT / boundary projection as one
-//                 matrix product on v_mfma_f64_16x16x4f64 (realft.F90 and the
-//                 projection loops of bessel.F90:179-185, 228-237)
 //   line_sweep    one ADI half-step: RHS stencil + Thomas solve with one
 //                 wavefront per tridiagonal line (helmadi.F90:201-343,
 //                 tridagr.F90, tridagz.F90), written transposed so the next
@@ -16,94 +13,6 @@
 #include <hip/hip_runtime.h>
 
 namespace bh {
-
-// ---------------------------------------------------------------------------
-// FP64 GEMM on MFMA:  C[R x M] = A[R x Kd] * B[Kd x M]
-// Tile 64 x 64 per workgroup of 256 threads; each wave owns a 32 x 32 block
-// = 2 x 2 MFMA 16x16 tiles; K staged through LDS in chunks of 16.
-// v_mfma_f64_16x16x4f64 operand layout (lane l): A[l%16][l/16],
-// B[l/16][l%16]; D element v of lane l is D[4*v + l/16][l%16] (measured on
-// gfx950 with a one-hot operand probe).
-// ---------------------------------------------------------------------------
-typedef double double4_t __attribute__((ext_vector_type(4)));
-
-constexpr int GBM = 64, GBN = 64, GBK = 16;
-
-__global__ __launch_bounds__(256) void gemm_f64_kernel(const double *__restrict__ A, long lda,
-                                                        const double *__restrict__ B, long ldb, int R,
-                                                        int M, int Kd, GemmOut out)
-{
-    __shared__ double As[GBK][GBM + 1];
-    __shared__ double Bs[GBK][GBN + 1];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int row0 = blockIdx.y * GBM, col0 = blockIdx.x * GBN;
-
-    double4_t acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = (double4_t){0.0, 0.0, 0.0, 0.0};
-
-    for (int k0 = 0; k0 < Kd; k0 += GBK) {
-        // A tile: 64 rows x 16 k -> As[k][row]
-        for (int t = tid; t < GBM * GBK; t += 256) {
-            int r = t / GBK, k = t % GBK;
-            int gr = row0 + r, gk = k0 + k;
-            As[k][r] = (gr < R && gk < Kd) ? A[(long)gr * lda + gk] : 0.0;
-        }
-        // B tile: 16 k x 64 cols -> Bs[k][col]
-        for (int t = tid; t < GBK * GBN; t += 256) {
-            int k = t / GBN, c = t % GBN;
-            int gk = k0 + k, gc = col0 + c;
-            Bs[k][c] = (gk < Kd && gc < M) ? B[(long)gk * ldb + gc] : 0.0;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < GBK; kk += 4) {
-            double a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wr * 32 + i * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 32 + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    // epilogue
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int r = row0 + wr * 32 + i * 16 + 4 * v + (lane >> 4);
-                int c = col0 + wc * 32 + j * 16 + (lane & 15);
-                if (r >= R || c >= M) continue;
-                double val = acc[i][j][v];
-                if (r < out.rsplit) {
-                    if (out.mask_numr) {
-                        // rhop zeroing (potential_solver.F90:70-78): K in
-                        // {1,2,numz-1,numz}, J in {numr-1,numr} (1-based)
-                        int J = c % out.mask_numr + 1, K = c / out.mask_numr + 1;
-                        if (J >= out.mask_numr - 1 || K <= 2 || K >= out.mask_numz - 1) val = 0.0;
-                    }
-                    out.c0[(long)r * out.ldc0 + c] = val;
-                } else {
-                    out.c1[(long)(r - out.rsplit) * out.ldc1 + c] = val;
-                }
-            }
-}
-
-void launch_gemm_f64(const double *A, long lda, const double *B, long ldb, int R, int M, int Kd,
-                     const GemmOut &out, hipStream_t s)
-{
-    dim3 grid((M + GBN - 1) / GBN, (R + GBM - 1) / GBM);
-    hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, s, A, lda, B, ldb, R, M, Kd, out);
-}
 
 // ---------------------------------------------------------------------------
 __global__ void fill_kernel(double *p, size_t n, double v)
@@ -527,8 +436,11 @@ void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
 // projections and with their z-mirror.  Deterministic block reduction.
 // ---------------------------------------------------------------------------
 constexpr int CT_TGT = 8;  // contraction targets per workgroup
+// The source points of each contraction are split into CT_SPLIT contiguous
+// ranges handled by separate workgroups; the face rebuild adds the partial
+// sums in a fixed order, so results are reproducible bit for bit.
 
-// Top and bottom: Stb[m][q][jo], q = top cos, top sin, bottom cos, bottom sin
+// Top and bottom: Stb[split][m][q][jo], q = top cos, top sin, bottom cos, bottom sin
 __global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__restrict__ T,
                                                               const double *__restrict__ proj,
                                                               double *__restrict__ Stb, int numr,
@@ -536,28 +448,45 @@ __global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__re
 {
     const int nri = numr - 2, nzi = numz - 2;
     const long npts = (long)nri * nzi, plane = (long)numr * numz;
-    const int jo0 = blockIdx.x * CT_TGT, m = blockIdx.y;
+    const int jo0 = blockIdx.x * CT_TGT, m = blockIdx.y, sp = blockIdx.z;
     const int nt = min(CT_TGT, nri - jo0);
+    const long chunk = (npts + CT_SPLIT - 1) / CT_SPLIT;
+    const long i0 = sp * chunk, i1 = min(npts, i0 + chunk);
     const double *t = T + ((long)m * nri + jo0) * npts;
     const double *pc = proj + (long)m * plane;
     const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
     double v[4 * CT_TGT];
 #pragma unroll
     for (int k = 0; k < 4 * CT_TGT; ++k) v[k] = 0.0;
-    for (long idx = threadIdx.x; idx < npts; idx += 256) {
-        int kz = (int)(idx / nri), jr = (int)(idx % nri);
-        long pa = (long)(kz + 1) * numr + jr + 1;
-        long pm = (long)(nzi - kz) * numr + jr + 1;
-        double c0 = pc[pa], c1 = pc[pm];
-        double s0 = (m > 0) ? ps[pa] : 0.0, s1 = (m > 0) ? ps[pm] : 0.0;
+    constexpr int U = 4;  // source points in flight per thread
+    for (long base = i0 + threadIdx.x; base < i1; base += 256 * U) {
+        double c0[U], c1[U], s0[U], s1[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            long idx = base + (long)u * 256;
+            ok[u] = idx < i1;
+            long id = ok[u] ? idx : i0;
+            int kz = (int)(id / nri), jr = (int)(id % nri);
+            long pa = (long)(kz + 1) * numr + jr + 1;
+            long pm = (long)(nzi - kz) * numr + jr + 1;
+            c0[u] = pc[pa];
+            c1[u] = pc[pm];
+            s0[u] = (m > 0) ? ps[pa] : 0.0;
+            s1[u] = (m > 0) ? ps[pm] : 0.0;
+        }
 #pragma unroll
         for (int q = 0; q < CT_TGT; ++q) {
             if (q < nt) {
-                double g = t[q * npts + idx];
-                v[4 * q + 0] += g * c0;
-                v[4 * q + 1] += g * s0;
-                v[4 * q + 2] += g * c1;
-                v[4 * q + 3] += g * s1;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    long idx = base + (long)u * 256;
+                    double g = ok[u] ? t[q * npts + idx] : 0.0;
+                    v[4 * q + 0] += g * c0[u];
+                    v[4 * q + 1] += g * s0[u];
+                    v[4 * q + 2] += g * c1[u];
+                    v[4 * q + 3] += g * s1[u];
+                }
             }
         }
     }
@@ -574,11 +503,11 @@ __global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__re
     if (threadIdx.x < 4 * nt) {
         int q = threadIdx.x / 4, c = threadIdx.x % 4;
         int k = threadIdx.x;
-        Stb[((long)m * 4 + c) * nri + jo0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        Stb[(((long)sp * mmax + m) * 4 + c) * nri + jo0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
     }
 }
 
-// Side: Ss[m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
+// Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
 __global__ __launch_bounds__(256) void contract_side_kernel(const double *__restrict__ S,
                                                             const double *__restrict__ proj,
                                                             double *__restrict__ Ss, int numr, int numz,
@@ -586,7 +515,7 @@ __global__ __launch_bounds__(256) void contract_side_kernel(const double *__rest
 {
     const int nri = numr - 2, nzi = numz - 2;
     const long plane = (long)numr * numz;
-    const int ko0 = blockIdx.x * CT_TGT, m = blockIdx.y;
+    const int ko0 = blockIdx.x * CT_TGT, m = blockIdx.y, sp = blockIdx.z;
     const int nt = min(CT_TGT, nzi - ko0);
     const int nd = 2 * nzi - 1;
     const double *sm = S + (long)m * nd * nri;
@@ -596,17 +525,33 @@ __global__ __launch_bounds__(256) void contract_side_kernel(const double *__rest
 #pragma unroll
     for (int k = 0; k < 2 * CT_TGT; ++k) v[k] = 0.0;
     const long npts = (long)nri * nzi;
-    for (long idx = threadIdx.x; idx < npts; idx += 256) {
-        int kz = (int)(idx / nri), jr = (int)(idx % nri);
-        long pa = (long)(kz + 1) * numr + jr + 1;
-        double c0 = pc[pa], s0 = (m > 0) ? ps[pa] : 0.0;
-        const double *g0 = sm + (long)(ko0 - kz + nzi - 1) * nri + jr;
+    const long chunk = (npts + CT_SPLIT - 1) / CT_SPLIT;
+    const long i0 = sp * chunk, i1 = min(npts, i0 + chunk);
+    constexpr int U = 4;
+    for (long base = i0 + threadIdx.x; base < i1; base += 256 * U) {
+        double c0[U], s0[U];
+        long gi[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            long idx = base + (long)u * 256;
+            ok[u] = idx < i1;
+            long id = ok[u] ? idx : i0;
+            int kz = (int)(id / nri), jr = (int)(id % nri);
+            long pa = (long)(kz + 1) * numr + jr + 1;
+            c0[u] = pc[pa];
+            s0[u] = (m > 0) ? ps[pa] : 0.0;
+            gi[u] = (long)(ko0 - kz + nzi - 1) * nri + jr;
+        }
 #pragma unroll
         for (int q = 0; q < CT_TGT; ++q) {
             if (q < nt) {
-                double g = g0[(long)q * nri];
-                v[2 * q + 0] += g * c0;
-                v[2 * q + 1] += g * s0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    double g = ok[u] ? sm[gi[u] + (long)q * nri] : 0.0;
+                    v[2 * q + 0] += g * c0[u];
+                    v[2 * q + 1] += g * s0[u];
+                }
             }
         }
     }
@@ -623,22 +568,22 @@ __global__ __launch_bounds__(256) void contract_side_kernel(const double *__rest
     if (threadIdx.x < 2 * nt) {
         int q = threadIdx.x / 2, c = threadIdx.x % 2;
         int k = threadIdx.x;
-        Ss[((long)m * 2 + c) * nzi + ko0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+        Ss[(((long)sp * mmax + m) * 2 + c) * nzi + ko0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
     }
 }
 
 void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax,
                             hipStream_t s)
 {
-    hipLaunchKernelGGL(contract_topbot_kernel, dim3((numr - 2 + CT_TGT - 1) / CT_TGT, mmax), dim3(256), 0, s, T,
-                       proj, Stb, numr, numz, mmax);
+    hipLaunchKernelGGL(contract_topbot_kernel, dim3((numr - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT), dim3(256), 0,
+                       s, T, proj, Stb, numr, numz, mmax);
 }
 
 void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax,
                           hipStream_t s)
 {
-    hipLaunchKernelGGL(contract_side_kernel, dim3((numz - 2 + CT_TGT - 1) / CT_TGT, mmax), dim3(256), 0, s, S,
-                       proj, Ss, numr, numz, mmax);
+    hipLaunchKernelGGL(contract_side_kernel, dim3((numz - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT), dim3(256), 0,
+                       s, S, proj, Ss, numr, numz, mmax);
 }
 
 // physical faces: factor * (S_C(0) + sum_m 2 S_C(m) cos + 2 S_S(m) sin)
@@ -653,15 +598,27 @@ __global__ void face_rebuild_kernel(const double *__restrict__ Stb, const double
     int L = blockIdx.y;
     if (fi >= nf) return;
     const double *sc, *ss;
-    long st;
+    long st, sst;  // per-m stride, per-split stride
     int idx;
     if (fi < nri) { idx = fi; sc = Stb; ss = Stb + nri; st = 4 * (long)nri; }
     else if (fi < 2 * nri) { idx = fi - nri; sc = Stb + 2 * nri; ss = Stb + 3 * nri; st = 4 * (long)nri; }
     else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; }
-    double acc = sc[idx];
+    sst = st * mmax;
+    // partial sums of the CT_SPLIT source ranges, added in a fixed order
+    auto S_c = [&](int m) {
+        double v = sc[m * st + idx];
+        for (int p = 1; p < CT_SPLIT; ++p) v += sc[p * sst + m * st + idx];
+        return v;
+    };
+    auto S_s = [&](int m) {
+        double v = ss[m * st + idx];
+        for (int p = 1; p < CT_SPLIT; ++p) v += ss[p * sst + m * st + idx];
+        return v;
+    };
+    double acc = S_c(0);
     for (int m = 1; m < mmax; ++m) {
         double c = besc[(long)m * numphi + L], s = bess[(long)m * numphi + L];
-        acc = acc + 2.0 * (sc[m * st + idx] * c) + 2.0 * (ss[m * st + idx] * s);
+        acc = acc + 2.0 * (S_c(m) * c) + 2.0 * (S_s(m) * s);
     }
     F[(long)L * nf + fi] = factor * acc;
 }

@@ -12,8 +12,6 @@
 // this with FP32 MFMA products (sine transforms, the M_r product) and FP64
 // residuals of A x = g - w in physical (J, K) coordinates, then refines once.
 //
-//   gemm_f32        batched FP32 GEMM on v_mfma_f32_16x16x4f32, optional
-//                   transposed operands, epilogues: plain, rhop mask, filter
 //   residual64      r = g - A x (- w), optionally after x += delta; FP64
 //   tridiag_shift   (B_r + lambda_z(k)) y = c for every (mode, k), FP64
 //   build_mr        M_r for one pseudo-timestep schedule (plan time), FP64
@@ -23,114 +21,6 @@
 #include <hip/hip_runtime.h>
 
 namespace bh {
-
-typedef float float4_t __attribute__((ext_vector_type(4)));
-
-// v_mfma_f32_16x16x4f32 operand layout (lane l): A[l%16][l/16],
-// B[l/16][l%16]; D element v of lane l is D[4*(l/16)+v][l%16] (measured on
-// gfx950 with a one-hot operand probe).
-constexpr int FBM = 64, FBN = 64, FBK = 16;
-
-template <bool TA, bool TB, typename BT, int EPI>
-__global__ __launch_bounds__(256) void gemm_f32_kernel(GemmF32Args g)
-{
-    __shared__ float As[FBK][FBM + 4];
-    __shared__ float Bs[FBK][FBN + 4];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int row0 = blockIdx.y * FBM, col0 = blockIdx.x * FBN;
-    const long z = blockIdx.z;
-    const float *A = g.A + z * g.sA;
-    const BT *B = (const BT *)g.B + z * g.sB;
-    const int R = g.R, M = g.M, Kd = g.Kd;
-
-    float4_t acc[2][2];
-    for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) acc[i][j] = (float4_t){0.f, 0.f, 0.f, 0.f};
-
-    for (int k0 = 0; k0 < Kd; k0 += FBK) {
-        for (int t = tid; t < FBM * FBK; t += 256) {
-            int r, k;
-            if (TA) { k = t / FBM; r = t % FBM; }
-            else { r = t / FBK; k = t % FBK; }
-            int gr = row0 + r, gk = k0 + k;
-            float v = 0.f;
-            if (gr < R && gk < Kd) v = TA ? A[(long)gk * g.lda + gr] : A[(long)gr * g.lda + gk];
-            As[k][r] = v;
-        }
-        for (int t = tid; t < FBK * FBN; t += 256) {
-            int k, c;
-            if (TB) { c = t / FBK; k = t % FBK; }
-            else { k = t / FBN; c = t % FBN; }
-            int gk = k0 + k, gc = col0 + c;
-            float v = 0.f;
-            if (gk < Kd && gc < M) v = (float)(TB ? B[(long)gc * g.ldb + gk] : B[(long)gk * g.ldb + gc]);
-            Bs[k][c] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < FBK; kk += 4) {
-            float a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wr * 32 + i * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 32 + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    float *C = g.C + z * g.sC;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                int r = row0 + wr * 32 + i * 16 + 4 * (lane >> 4) + v;
-                int c = col0 + wc * 32 + j * 16 + (lane & 15);
-                if (r >= R || c >= M) continue;
-                float val = acc[i][j][v];
-                const long o = (long)r * g.ldc + c;
-                if (EPI == EPI_MASK) {
-                    // rhop zeroing (potential_solver.F90:70-78), c = (K-1)*numr + J-1
-                    int J = c % g.mask_numr + 1, K = c / g.mask_numr + 1;
-                    if (J >= g.mask_numr - 1 || K <= 2 || K >= g.mask_numz - 1) val = 0.f;
-                    C[o] = val;
-                } else if (EPI == EPI_FILTER) {
-                    // W = mu_z(c) * acc;  C2 = Rin - W
-                    float w = g.mu[c] * val;
-                    C[o] = w;
-                    g.C2[z * g.sC + o] = g.Rin[z * g.sC + o] - w;
-                } else {
-                    C[o] = val;
-                }
-            }
-}
-
-template <bool TA, bool TB, typename BT, int EPI>
-static void launch_f32(const GemmF32Args &g, hipStream_t s)
-{
-    dim3 grid((g.M + FBN - 1) / FBN, (g.R + FBM - 1) / FBM, g.batch);
-    hipLaunchKernelGGL((gemm_f32_kernel<TA, TB, BT, EPI>), grid, dim3(256), 0, s, g);
-}
-
-void launch_gemm_f32(const GemmF32Args &g, bool ta, bool tb, bool b_double, int epi, hipStream_t s)
-{
-    if (b_double) {
-        if (!ta && !tb && epi == EPI_MASK) launch_f32<false, false, double, EPI_MASK>(g, s);
-        else if (!ta && !tb) launch_f32<false, false, double, EPI_PLAIN>(g, s);
-        return;
-    }
-    if (ta && !tb) launch_f32<true, false, float, EPI_PLAIN>(g, s);
-    else if (!ta && tb) launch_f32<false, true, float, EPI_PLAIN>(g, s);
-    else if (!ta && !tb && epi == EPI_FILTER) launch_f32<false, false, float, EPI_FILTER>(g, s);
-    else launch_f32<false, false, float, EPI_PLAIN>(g, s);
-}
 
 // ---------------------------------------------------------------------------
 // r = -4 pi rho-hat - A x - w on the interior, with x = X + dlt when dlt is
@@ -195,36 +85,65 @@ void launch_add_update(const double *X, const float *dlt, double *Xo, int numr, 
 // ---------------------------------------------------------------------------
 // (B_r(mode L) + lambda_z(k)) y = c along J for every (L, k); data layout
 // [L][J'][k] (k fastest, one thread per k: coalesced).  LU factors of the
-// shifted operator come precomputed per (L, J', k): ibet and gam (FP64).
+// shifted operator come precomputed per (L, J', k): ibet and gam (FP32).
+// The sequential sweeps work in chunks of TS elements whose loads are all
+// issued before the recurrence consumes them.
 // ---------------------------------------------------------------------------
+constexpr int TS = 16;
+
 __global__ __launch_bounds__(256) void tridiag_shift_kernel(const float *__restrict__ cin, float *__restrict__ yout,
-                                                            const double *__restrict__ ibet,
-                                                            const double *__restrict__ gam,
+                                                            const float *__restrict__ ibet,
+                                                            const float *__restrict__ gam,
                                                             const double *__restrict__ betar, int nri, int nzi)
 {
     const int L = blockIdx.y;
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nzi) return;
     const long base = (long)L * nri * nzi + k;
-    const double *ib = ibet + base, *gm = gam + base;
-    // forward: y_J = (c_J - a_J y_{J-1}) / bet_J, a_J = betar(J) for J > 2
-    double y = (double)cin[base] * ib[0];
-    yout[base] = (float)y;
-    for (int j = 1; j < nri; ++j) {
-        long o = (long)j * nzi;
-        y = ((double)cin[base + o] - betar[j + 2] * y) * ib[o];
-        yout[base + o] = (float)y;
+    // forward: y_J = (c_J - a_J y_{J-1}) / bet_J, a_J = betar(J); a_2 y_1 absent
+    float y = 0.f;
+    for (int j0 = 0; j0 < nri; j0 += TS) {
+        float cv[TS], iv[TS];
+#pragma unroll
+        for (int u = 0; u < TS; ++u) {
+            int j = j0 + u;
+            long o = base + (long)j * nzi;
+            cv[u] = (j < nri) ? cin[o] : 0.f;
+            iv[u] = (j < nri) ? ibet[o] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < TS; ++u) {
+            int j = j0 + u;
+            if (j < nri) {
+                float a = (j > 0) ? (float)betar[j + 2] : 0.f;
+                y = (cv[u] - a * y) * iv[u];
+                yout[base + (long)j * nzi] = y;
+            }
+        }
     }
     // back: x_J = y_J - gam_{J+1} x_{J+1}
-    double x = y;
-    for (int j = nri - 2; j >= 0; --j) {
-        long o = (long)j * nzi;
-        x = (double)yout[base + o] - gm[o + nzi] * x;
-        yout[base + o] = (float)x;
+    float x = y;
+    for (int j1 = nri - 2; j1 >= 0; j1 -= TS) {
+        float yv[TS], gv[TS];
+#pragma unroll
+        for (int u = 0; u < TS; ++u) {
+            int j = j1 - u;
+            long o = base + (long)j * nzi;
+            yv[u] = (j >= 0) ? yout[o] : 0.f;
+            gv[u] = (j >= 0) ? gam[o + nzi] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < TS; ++u) {
+            int j = j1 - u;
+            if (j >= 0) {
+                x = yv[u] - gv[u] * x;
+                yout[base + (long)j * nzi] = x;
+            }
+        }
     }
 }
 
-void launch_tridiag_shift(const float *cin, float *yout, const double *ibet, const double *gam,
+void launch_tridiag_shift(const float *cin, float *yout, const float *ibet, const float *gam,
                           const double *betar, int nri, int nzi, int nplanes, hipStream_t s)
 {
     dim3 grid((nzi + 255) / 256, nplanes);

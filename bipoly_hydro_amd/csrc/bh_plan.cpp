@@ -3,6 +3,7 @@
 // include/bh_potential.h.
 #include "../../include/bh_potential.h"
 #include "bh_internal.h"
+#include "bh_gemm.h"
 #include "bh_mixed.h"
 
 #include <hip/hip_runtime.h>
@@ -70,15 +71,16 @@ struct bh_plan {
     size_t bytes = 0;
     double *RJ = nullptr, *RK = nullptr, *XJ = nullptr, *XK = nullptr, *proj = nullptr;
     double *T = nullptr, *S = nullptr, *Stb = nullptr, *Ss = nullptr, *F = nullptr, *FF = nullptr;
-    double *Dfwd = nullptr, *Dinv = nullptr, *besc = nullptr, *bess = nullptr;
+    // real DFT in halves: Dc/Ds forward (cosine/sine), Ec/Es inverse; Pm projections
+    double *Dc = nullptr, *Ds = nullptr, *Ec = nullptr, *Es = nullptr, *Pm = nullptr, *besc = nullptr, *bess = nullptr;
     double *alphar = nullptr, *betar = nullptr, *elzb = nullptr;
     double *rc_alpha[2] = {nullptr, nullptr}, *rc_ibet[2] = {nullptr, nullptr}, *rc_gback[2] = {nullptr, nullptr};
     double *zc_alpha[2] = {nullptr, nullptr}, *zc_ibet[2] = {nullptr, nullptr}, *zc_gback[2] = {nullptr, nullptr};
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
     // mixed precision (BH_PREC_MIXED)
     int passes = 1;                                // FP64 refinement passes
-    float *Dfwd32 = nullptr, *S32 = nullptr, *mu32[2] = {nullptr, nullptr}, *Mr[2] = {nullptr, nullptr};
-    double *tri_ibet = nullptr, *tri_gam = nullptr;
+    float *Dc32 = nullptr, *Ds32 = nullptr, *S32 = nullptr, *mu32[2] = {nullptr, nullptr}, *Mr[2] = {nullptr, nullptr};
+    float *tri_ibet = nullptr, *tri_gam = nullptr;
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
     float *D1 = nullptr, *Wf = nullptr;
     double *brb_d = nullptr;
@@ -235,13 +237,17 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
     ALLOC(p->proj, M * (2 * mmax - 1));
     ALLOC(p->T, (size_t)mmax * nri * nri * nzi);
     ALLOC(p->S, (size_t)mmax * (2 * nzi - 1) * nri);
-    ALLOC(p->Stb, (size_t)mmax * 4 * nri);
-    ALLOC(p->Ss, (size_t)mmax * 2 * nzi);
+    ALLOC(p->Stb, (size_t)CT_SPLIT * mmax * 4 * nri);
+    ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
     const int nf = 2 * nri + nzi;
     ALLOC(p->F, (size_t)N * nf);
     ALLOC(p->FF, (size_t)N * nf);
-    ALLOC(p->Dfwd, (size_t)(N + 2 * mmax - 1) * N);
-    ALLOC(p->Dinv, (size_t)N * N);
+    const int nh = N / 2 + 1, ns_ = N / 2 - 1;  // sizes of the cosine and sine halves
+    ALLOC(p->Dc, (size_t)nh * nh);
+    ALLOC(p->Ds, (size_t)ns_ * ns_);
+    ALLOC(p->Ec, (size_t)nh * nh);
+    ALLOC(p->Es, (size_t)ns_ * ns_);
+    ALLOC(p->Pm, (size_t)(2 * mmax - 1) * N);
     ALLOC(p->besc, (size_t)mmax * N);
     ALLOC(p->bess, (size_t)mmax * N);
     ALLOC(p->alphar, numr + 1);
@@ -280,18 +286,30 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
     // Then the projection rows of bessel (potsetup.F90:131-139): m = 0 (sum),
     // cos(m (L-1/2) dphi) for m = 1..mmax-1, then sin for m = 1..mmax-1.
     {
-        std::vector<double> D((size_t)(N + 2 * mmax - 1) * N), Di((size_t)N * N);
-        for (int n = 0; n < N; ++n) {
-            for (int k = 0; k <= N / 2; ++k) {
-                long kn = ((long)k * n) % N;
-                D[(size_t)k * N + n] = std::cos(2.0 * pi * (double)kn / (double)N);
+        // a_k = sum_n x_n cos(2 pi k n/N) = x_0 + (-1)^k x_{N/2} + sum_{n<N/2} (x_n + x_{N-n}) cos(...)
+        // -b_k = -sum_n x_n sin(2 pi k n/N) = -sum_{n<N/2} (x_n - x_{N-n}) sin(...)
+        std::vector<double> Dc((size_t)nh * nh), Ds((size_t)ns_ * ns_), Ec((size_t)nh * nh), Es((size_t)ns_ * ns_);
+        auto cosr = [&](long a) { return std::cos(2.0 * pi * (double)(a % N) / (double)N); };
+        auto sinr = [&](long a) { return std::sin(2.0 * pi * (double)(a % N) / (double)N); };
+        for (int k = 0; k < nh; ++k)
+            for (int n = 0; n < nh; ++n) Dc[(size_t)k * nh + n] = cosr((long)k * n);
+        for (int k = 0; k < ns_; ++k)
+            for (int n = 0; n < ns_; ++n) Ds[(size_t)k * ns_ + n] = -sinr((long)(k + 1) * (n + 1));
+        // inverse (realft isign = -1 with the 2/N of helmadi.F90:386-392):
+        // x_n = P_n - Q_n, x_{N-n} = P_n + Q_n
+        for (int n = 0; n < nh; ++n)
+            for (int k = 0; k < nh; ++k) {
+                double v;
+                if (k == 0) v = 1.0 / N;
+                else if (k == N / 2) v = ((n % 2) ? -1.0 : 1.0) / N;
+                else v = 2.0 * cosr((long)k * n) / N;
+                Ec[(size_t)n * nh + k] = v;
             }
-            for (int k = 1; k < N / 2; ++k) {
-                long kn = ((long)k * n) % N;
-                D[(size_t)(N / 2 + k) * N + n] = -std::sin(2.0 * pi * (double)kn / (double)N);
-            }
-        }
-        std::vector<double> bc((size_t)mmax * N), bs((size_t)mmax * N);
+        for (int n = 0; n < ns_; ++n)
+            for (int k = 0; k < ns_; ++k) Es[(size_t)n * ns_ + k] = 2.0 * sinr((long)(k + 1) * (n + 1)) / N;
+        // projection rows of bessel (potsetup.F90:131-139): m = 0 (sum),
+        // cos(m (L-1/2) dphi) for m = 1..mmax-1, then sin for m = 1..mmax-1
+        std::vector<double> bc((size_t)mmax * N), bs((size_t)mmax * N), Pm((size_t)(2 * mmax - 1) * N);
         for (int M1 = 1; M1 <= mmax; ++M1)
             for (int L = 1; L <= N; ++L) {
                 double a = 0.5 * g.dphi * ((double)M1 - 1.0) * (2.0 * (double)L - 1.0);
@@ -299,25 +317,17 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
                 bs[(size_t)(M1 - 1) * N + (L - 1)] = std::sin(a);
             }
         for (int n = 0; n < N; ++n) {
-            D[(size_t)N * N + n] = 1.0;
+            Pm[n] = 1.0;
             for (int m = 1; m < mmax; ++m) {
-                D[(size_t)(N + m) * N + n] = bc[(size_t)m * N + n];
-                D[(size_t)(N + mmax - 1 + m) * N + n] = bs[(size_t)m * N + n];
+                Pm[(size_t)m * N + n] = bc[(size_t)m * N + n];
+                Pm[(size_t)(mmax - 1 + m) * N + n] = bs[(size_t)m * N + n];
             }
         }
-        // inverse (realft isign = -1 with the 2/N of helmadi.F90:386-392)
-        for (int n = 0; n < N; ++n) {
-            Di[(size_t)n * N + 0] = 1.0 / N;
-            Di[(size_t)n * N + N / 2] = ((n % 2) ? -1.0 : 1.0) / N;
-            for (int k = 1; k < N / 2; ++k) {
-                long kn = ((long)k * n) % N;
-                double ang = 2.0 * pi * (double)kn / (double)N;
-                Di[(size_t)n * N + k] = 2.0 * std::cos(ang) / N;
-                Di[(size_t)n * N + N / 2 + k] = -2.0 * std::sin(ang) / N;
-            }
-        }
-        UPLOAD(p->Dfwd, D);
-        UPLOAD(p->Dinv, Di);
+        UPLOAD(p->Dc, Dc);
+        UPLOAD(p->Ds, Ds);
+        UPLOAD(p->Ec, Ec);
+        UPLOAD(p->Es, Es);
+        UPLOAD(p->Pm, Pm);
         UPLOAD(p->besc, bc);
         UPLOAD(p->bess, bs);
     }
@@ -421,14 +431,15 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
         }                                                    \
     } while (0)
         const size_t PI = (size_t)N * nri * nzi;  // interior values, all modes
-        MALLOCF(p->Dfwd32, (size_t)N * N);
+        MALLOCF(p->Dc32, (size_t)nh * nh);
+        MALLOCF(p->Ds32, (size_t)ns_ * ns_);
         MALLOCF(p->S32, (size_t)nzi * nzi);
         MALLOCF(p->mu32[0], nzi);
         MALLOCF(p->mu32[1], nzi);
         MALLOCF(p->Mr[0], (size_t)N * nri * nri);
         MALLOCF(p->Mr[1], (size_t)N * nri * nri);
-        MALLOCD(p->tri_ibet, PI);
-        MALLOCD(p->tri_gam, PI);
+        MALLOCF(p->tri_ibet, PI);
+        MALLOCF(p->tri_gam, PI);
         MALLOCF(p->RH32, (size_t)M * N);
         MALLOCF(p->r32, PI);
         MALLOCF(p->RT, PI);
@@ -442,22 +453,21 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
         // FP32 forward DFT rows (helmadi's mode order), sine transform,
         // lambda_z, mu_z for both schedules
         {
-            std::vector<float> D32((size_t)N * N), S((size_t)nzi * nzi);
-            for (int k = 0; k < N; ++k)
-                for (int n = 0; n < N; ++n) {
-                    long kn;
-                    double v;
-                    if (k <= N / 2) { kn = ((long)k * n) % N; v = std::cos(2.0 * pi * (double)kn / N); }
-                    else { kn = ((long)(k - N / 2) * n) % N; v = -std::sin(2.0 * pi * (double)kn / N); }
-                    D32[(size_t)k * N + n] = (float)v;
-                }
+            std::vector<float> Dc32((size_t)nh * nh), Ds32((size_t)ns_ * ns_), S((size_t)nzi * nzi);
+            for (int k = 0; k < nh; ++k)
+                for (int n = 0; n < nh; ++n)
+                    Dc32[(size_t)k * nh + n] = (float)std::cos(2.0 * pi * (double)(((long)k * n) % N) / N);
+            for (int k = 0; k < ns_; ++k)
+                for (int n = 0; n < ns_; ++n)
+                    Ds32[(size_t)k * ns_ + n] = (float)-std::sin(2.0 * pi * (double)(((long)(k + 1) * (n + 1)) % N) / N);
             const double sn = std::sqrt(2.0 / (nzi + 1));
             for (int a = 0; a < nzi; ++a)
                 for (int b = 0; b < nzi; ++b) {
                     long ab = ((long)(a + 1) * (b + 1)) % (2 * (nzi + 1));
                     S[(size_t)a * nzi + b] = (float)(sn * std::sin(pi * (double)ab / (nzi + 1)));
                 }
-            if ((rc = upload_f(p->Dfwd32, D32)) != BH_OK || (rc = upload_f(p->S32, S)) != BH_OK) {
+            if ((rc = upload_f(p->Dc32, Dc32)) != BH_OK || (rc = upload_f(p->Ds32, Ds32)) != BH_OK ||
+                (rc = upload_f(p->S32, S)) != BH_OK) {
                 free_all(p);
                 delete p;
                 return rc;
@@ -478,25 +488,28 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
                 }
             }
             // LU factors of B_r(L) + lambda_z(k), layout [L][J'][k]
-            std::vector<double> ib(PI), gm(PI);
+            std::vector<float> ib(PI), gm(PI);
             for (int L = 0; L < N; ++L) {
                 const double *b = &brb[(size_t)L * (numr + 1)];
                 for (int k = 0; k < nzi; ++k) {
                     size_t base = (size_t)L * nri * nzi + k;
                     double bet = b[2] + lamz[k];
-                    ib[base] = 1.0 / bet;
-                    gm[base] = 0.0;
+                    ib[base] = (float)(1.0 / bet);
+                    gm[base] = 0.f;
                     for (int j = 1; j < nri; ++j) {
                         int J = j + 2;
                         double gg = alphar[J - 1] / bet;
                         bet = (b[J] + lamz[k]) - betar[J] * gg;
-                        ib[base + (size_t)j * nzi] = 1.0 / bet;
-                        gm[base + (size_t)j * nzi] = gg;
+                        ib[base + (size_t)j * nzi] = (float)(1.0 / bet);
+                        gm[base + (size_t)j * nzi] = (float)gg;
                     }
                 }
             }
-            UPLOAD(p->tri_ibet, ib);
-            UPLOAD(p->tri_gam, gm);
+            if ((rc = upload_f(p->tri_ibet, ib)) != BH_OK || (rc = upload_f(p->tri_gam, gm)) != BH_OK) {
+                free_all(p);
+                delete p;
+                return rc;
+            }
         }
         // M_r(L) for both schedules, on the device in chunks of modes
         {
@@ -576,6 +589,46 @@ extern "C" int bh_last_timings(const bh_plan *p, double *ms, int n)
 namespace {
 
 
+// Forward real DFT along phi of the N rows of B (columns Mcols), written in
+// helmadi's mode order (helmadi.F90:131-156) to C; cosine and sine halves.
+void dft_fwd64(const bh_plan *p, const double *B, long ldb, int Mcols, double *C, long ldc, bool mask, hipStream_t s)
+{
+    const int N = p->g.numphi, nh = N / 2 + 1, nsn = N / 2 - 1;
+    GemmArgs a{};
+    a.A = p->Dc; a.lda = nh; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc;
+    a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
+    a.mask_numr = p->g.numr; a.mask_numz = p->g.numz;
+    gemm64(a, B_FOLD_PLUS, mask ? E_MASK : E_PLAIN, s);
+    a.A = p->Ds; a.lda = nsn; a.C = C + (long)nh * ldc; a.R = nsn; a.Kd = nsn;
+    gemm64(a, B_FOLD_MINUS, mask ? E_MASK : E_PLAIN, s);
+}
+
+// Inverse real DFT with the 2/N normalisation (helmadi.F90:357-392) of the
+// N mode rows of X (+ Xadd in FP32 when given) into N phi rows of out.
+void dft_inv64(const bh_plan *p, const double *X, const float *Xadd, long ldx, int Mcols, double *out, long ldo,
+               hipStream_t s)
+{
+    const int N = p->g.numphi, nh = N / 2 + 1, nsn = N / 2 - 1;
+    GemmArgs a{};
+    a.A = p->Ec; a.lda = nh; a.B = X; a.ldb = ldx; a.Badd = Xadd; a.C = out; a.ldc = ldo;
+    a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
+    gemm64(a, Xadd ? B_ADD_F32 : B_PLAIN, E_PLAIN, s);
+    a.A = p->Es; a.lda = nsn; a.B = X + (long)nh * ldx; a.Badd = Xadd ? Xadd + (long)nh * ldx : nullptr;
+    a.R = nsn; a.Kd = nsn;
+    gemm64(a, Xadd ? B_ADD_F32 : B_PLAIN, E_INV_COMBINE, s);
+}
+
+// Boundary projections of rho onto the mmax azimuthal modes (bessel.F90:179-185, 228-237)
+void proj64(const bh_plan *p, const double *rho, hipStream_t s)
+{
+    const int N = p->g.numphi;
+    const long M = (long)p->g.numr * p->g.numz;
+    GemmArgs a{};
+    a.A = p->Pm; a.lda = N; a.B = rho; a.ldb = M; a.C = p->proj; a.ldc = M;
+    a.R = 2 * p->g.mmax - 1; a.M = (int)M; a.Kd = N; a.batch = 1;
+    gemm64(a, B_PLAIN, E_PLAIN, s);
+}
+
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
 // iteration as the FP64 path (helmadi.F90:195-355), as A x = g - M r0 (see
 // bh_mixed.hip), with FP32 MFMA transforms and FP64 residuals.
@@ -589,15 +642,15 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
     if (p->timing) HIPCHK(hipEventRecord(p->ev[0], s));
     // 1. rho-hat in FP32 (masked as rhop), projections in FP64
     {
-        GemmF32Args a{};
-        a.A = p->Dfwd32; a.lda = N; a.sA = 0;
-        a.B = d_rho; a.ldb = M; a.sB = 0;
-        a.C = p->RH32; a.ldc = M; a.sC = 0;
-        a.R = N; a.M = (int)M; a.Kd = N; a.batch = 1;
+        const int nh = N / 2 + 1, nsn = N / 2 - 1;
+        GemmArgs a{};
+        a.A = p->Dc32; a.lda = nh; a.B = d_rho; a.ldb = M; a.C = p->RH32; a.ldc = M;
+        a.R = nh; a.M = (int)M; a.Kd = nh; a.batch = 1; a.nfold = N;
         a.mask_numr = numr; a.mask_numz = numz;
-        launch_gemm_f32(a, false, false, true, EPI_MASK, s);
-        GemmOut o{p->proj, M, nullptr, 0, 2 * mmax - 1, 0, 0};
-        launch_gemm_f64(p->Dfwd + (size_t)N * N, N, d_rho, M, 2 * mmax - 1, (int)M, N, o, s);
+        gemm32(a, B_FOLD_PLUS, E_MASK, false, false, true, s);
+        a.A = p->Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * M; a.R = nsn; a.Kd = nsn;
+        gemm32(a, B_FOLD_MINUS, E_MASK, false, false, true, s);
+        proj64(p, d_rho, s);
     }
     if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
     // 2. boundary faces (FP64), into both potential buffers
@@ -605,8 +658,7 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
     launch_contract_side(p->S, p->proj, p->Ss, numr, numz, mmax, s);
     launch_face_rebuild(p->Stb, p->Ss, p->besc, p->bess, p->F, numr, numz, N, mmax, -g.dr * g.dz * g.dphi, s);
     const int nf = 2 * nri + nzi;
-    GemmOut of{p->FF, nf, nullptr, 0, N, 0, 0};
-    launch_gemm_f64(p->Dfwd, N, p->F, nf, N, nf, N, of, s);
+    dft_fwd64(p, p->F, nf, nf, p->FF, nf, false, s);
     double *Xa = p->XJ, *Xb = p->XK;
     if (tstep <= 1) launch_fill(Xa, (size_t)M * N, 0.0, s);
     launch_face_scatter_j(p->FF, Xa, numr, numz, N, s);
@@ -621,32 +673,32 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
     launch_residual64(ra, N, s);
     // sine transform: RT[L] (nri x nzi) = r[L]^T S
     auto dst_fwd = [&](const float *r, float *out) {
-        GemmF32Args a{};
+        GemmArgs a{};
         a.A = r; a.lda = nri; a.sA = PIm;
         a.B = p->S32; a.ldb = nzi; a.sB = 0;
         a.C = out; a.ldc = nzi; a.sC = PIm;
         a.R = nri; a.M = nzi; a.Kd = nzi; a.batch = N;
-        launch_gemm_f32(a, true, false, false, EPI_PLAIN, s);
+        gemm32(a, B_PLAIN, E_PLAIN, true, false, false, s);
     };
     // inverse: full-plane out[L] interior (nzi x nri) = S * in[L]^T
     auto dst_inv = [&](const float *in, float *out) {
-        GemmF32Args a{};
+        GemmArgs a{};
         a.A = p->S32; a.lda = nzi; a.sA = 0;
         a.B = in; a.ldb = nzi; a.sB = PIm;
         a.C = out + numr + 1; a.ldc = numr; a.sC = M;
         a.R = nzi; a.M = nri; a.Kd = nzi; a.batch = N;
-        launch_gemm_f32(a, false, true, false, EPI_PLAIN, s);
+        gemm32(a, B_PLAIN, E_PLAIN, false, true, false, s);
     };
     dst_fwd(p->r32, p->RT);
     // 4. W = mu_z . (M_r RT), C = RT - W
     {
-        GemmF32Args a{};
+        GemmArgs a{};
         a.A = p->Mr[w]; a.lda = nri; a.sA = (long)nri * nri;
         a.B = p->RT; a.ldb = nzi; a.sB = PIm;
         a.C = p->WT; a.ldc = nzi; a.sC = PIm;
         a.R = nri; a.M = nzi; a.Kd = nri; a.batch = N;
         a.mu = p->mu32[w]; a.Rin = p->RT; a.C2 = p->CT;
-        launch_gemm_f32(a, false, false, false, EPI_FILTER, s);
+        gemm32(a, B_PLAIN, E_FILTER, false, false, false, s);
     }
     // 5. (B_r + lambda_z) dhat = C, back to physical space
     launch_tridiag_shift(p->CT, p->DT, p->tri_ibet, p->tri_gam, p->betar, nri, nzi, N, s);
@@ -666,8 +718,7 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
     launch_add_update(Xcur, p->D1, Xa, numr, numz, N, s);
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     // 7. inverse DFT (FP64) and axis ghost
-    GemmOut oi{d_pot, M, nullptr, 0, N, 0, 0};
-    launch_gemm_f64(p->Dinv, N, Xa, M, N, (int)M, N, oi, s);
+    dft_inv64(p, Xa, nullptr, M, (int)M, d_pot, M, s);
     launch_axis(d_pot, numr, numz, N, s);
     if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
     HIPCHK(hipGetLastError());
@@ -687,8 +738,8 @@ int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hip
     if (p->timing) HIPCHK(hipEventRecord(p->ev[0], s));
     // 1. one product gives rho-hat (with rhop's zeroed layers) and the
     //    boundary projections of rho (bessel.F90:179-185, 228-237)
-    GemmOut o{p->RJ, M, p->proj, M, N, numr, numz};
-    launch_gemm_f64(p->Dfwd, N, d_rho, M, N + 2 * mmax - 1, (int)M, N, o, s);
+    dft_fwd64(p, d_rho, M, (int)M, p->RJ, M, true, s);
+    proj64(p, d_rho, s);
     launch_transpose_jk(p->RJ, p->RK, numr, numz, N, s);
     if (p->timing) HIPCHK(hipEventRecord(p->ev[1], s));
     // 2. boundary potential (bessel.F90:186-488) -> Fourier-space faces
@@ -697,8 +748,7 @@ int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hip
     const double factor = -g.dr * g.dz * g.dphi;
     launch_face_rebuild(p->Stb, p->Ss, p->besc, p->bess, p->F, numr, numz, N, mmax, factor, s);
     const int nf = 2 * nri + nzi;
-    GemmOut of{p->FF, nf, nullptr, 0, N, 0, 0};
-    launch_gemm_f64(p->Dfwd, N, p->F, nf, N, nf, N, of, s);
+    dft_fwd64(p, p->F, nf, nf, p->FF, nf, false, s);
     // 3. starting potential: zero (cold, potential_solver.F90:88-93) or the
     //    previous solution kept in Fourier space (warm)
     if (!warm) launch_fill(p->XJ, (size_t)M * N, 0.0, s);
@@ -731,8 +781,7 @@ int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hip
     }
     if (p->timing) HIPCHK(hipEventRecord(p->ev[3], s));
     // 5. inverse DFT with the 2/N normalisation, then the axis ghost
-    GemmOut oi{d_pot, M, nullptr, 0, N, 0, 0};
-    launch_gemm_f64(p->Dinv, N, p->XJ, M, N, (int)M, N, oi, s);
+    dft_inv64(p, p->XJ, nullptr, M, (int)M, d_pot, M, s);
     launch_axis(d_pot, numr, numz, N, s);
     if (p->timing) HIPCHK(hipEventRecord(p->ev[4], s));
     HIPCHK(hipGetLastError());
@@ -802,8 +851,7 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
         HIPCHK(hipMemcpy(p->pot_out, pot, n * sizeof(double), hipMemcpyHostToDevice));
         src = p->pot_out;
     }
-    GemmOut o{p->XJ, M, nullptr, 0, g.numphi, 0, 0};
-    launch_gemm_f64(p->Dfwd, g.numphi, src, M, g.numphi, (int)M, g.numphi, o, 0);
+    dft_fwd64(p, src, M, (int)M, p->XJ, M, false, 0);
     HIPCHK(hipDeviceSynchronize());
     p->has_solution = true;
     return BH_OK;
@@ -818,8 +866,9 @@ extern "C" int bh_sync_timing(bh_plan *p) { return p ? collect_timing(p) : fail(
 extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb, int R, int M, int Kd, double *C,
                             long ldc)
 {
-    GemmOut o{C, ldc, nullptr, 0, R, 0, 0};
-    launch_gemm_f64(A, lda, B, ldb, R, M, Kd, o, 0);
+    GemmArgs a{};
+    a.A = A; a.lda = lda; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc; a.R = R; a.M = M; a.Kd = Kd; a.batch = 1;
+    gemm64(a, B_PLAIN, E_PLAIN, 0);
     HIPCHK(hipDeviceSynchronize());
     return BH_OK;
 }
@@ -833,10 +882,21 @@ extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
     const size_t nf = 2 * nri + nzi;
     const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->T, p->S, p->Stb, p->Ss, p->F, p->FF};
     const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), mm * nri * nri * nzi,
-                          mm * (2 * nzi - 1) * nri, mm * 4 * nri, mm * 2 * nzi, N * nf, N * nf};
+                          mm * (2 * nzi - 1) * nri, CT_SPLIT * mm * 4 * nri, CT_SPLIT * mm * 2 * nzi, N * nf, N * nf};
     if (which < 0 || which > 10) return fail(BH_E_INVALID, "which");
     if (n < cnt[which]) return fail(BH_E_INVALID, "buffer too small: need " + std::to_string(cnt[which]));
     HIPCHK(hipDeviceSynchronize());
     HIPCHK(hipMemcpy(host, src[which], cnt[which] * sizeof(double), hipMemcpyDeviceToHost));
+    return BH_OK;
+}
+
+// forward (inverse = 0) or inverse (inverse = 1) real DFT of nplanes x ncols
+// device data with the plan's matrices; test hook for the folded DFT GEMMs.
+extern "C" int bhx_dft64(const bh_plan *p, const double *in, double *out, int ncols, int inverse)
+{
+    if (!p || !in || !out) return fail(BH_E_INVALID, "null argument");
+    if (inverse) dft_inv64(p, in, nullptr, ncols, ncols, out, ncols, 0);
+    else dft_fwd64(p, in, ncols, ncols, out, ncols, false, 0);
+    HIPCHK(hipDeviceSynchronize());
     return BH_OK;
 }

@@ -1,0 +1,50 @@
+"""GPU unit checks of the MFMA GEMM and the folded real DFT against numpy
+(test hooks bhx_* of libbh_potential.so)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from bipoly_hydro_amd import potential as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _lib():
+    L = P.load_library()
+    vp = ctypes.c_void_p
+    L.bhx_gemm_f64.argtypes = [vp, ctypes.c_long, vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp,
+                               ctypes.c_long]
+    L.bhx_dft64.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int]
+    return L
+
+
+@pytest.mark.parametrize("R,M,K", [(16, 16, 4), (64, 64, 16), (37, 101, 23), (80, 200, 64), (257, 333, 257)])
+def test_gemm_f64_matches_numpy(R, M, K):
+    torch = pytest.importorskip("torch")
+    L = _lib()
+    rng = np.random.default_rng(R * 1000 + M)
+    A, B = rng.standard_normal((R, K)), rng.standard_normal((K, M))
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    dC = torch.zeros((R, M), dtype=torch.float64, device="cuda")
+    assert L.bhx_gemm_f64(dA.data_ptr(), K, dB.data_ptr(), M, R, M, K, dC.data_ptr(), M) == 0
+    np.testing.assert_allclose(dC.cpu().numpy(), A @ B, rtol=0, atol=1e-13 * K)
+
+
+@pytest.mark.parametrize("numphi", [4, 16, 64, 512])
+def test_folded_dft_matches_numpy(numphi):
+    torch = pytest.importorskip("torch")
+    L = _lib()
+    ncols = 97
+    rng = np.random.default_rng(numphi)
+    x = rng.standard_normal((numphi, ncols))
+    with P.PotentialPlan(10, 10, numphi) as plan:
+        dx = torch.from_numpy(x).cuda()
+        dy = torch.zeros_like(dx)
+        assert L.bhx_dft64(plan.handle, dx.data_ptr(), dy.data_ptr(), ncols, 0) == 0
+        X = np.fft.rfft(x, axis=0)
+        ref = np.concatenate([X.real[: numphi // 2 + 1], X.imag[1: numphi // 2]], axis=0)
+        np.testing.assert_allclose(dy.cpu().numpy(), ref, rtol=0, atol=1e-12 * numphi)
+        dz = torch.zeros_like(dx)
+        assert L.bhx_dft64(plan.handle, dy.data_ptr(), dz.data_ptr(), ncols, 1) == 0
+        np.testing.assert_allclose(dz.cpu().numpy(), x, rtol=0, atol=1e-13 * numphi)
