@@ -20,6 +20,7 @@ struct GemmArgs {
     int mask_numr, mask_numz;     // E_MASK
     const float *mu;              // E_FILTER: per-column factor
     const void *Rin; void *C2;    // E_FILTER: C2 = Rin - mu * acc, same indexing as C
+    int tile;                     // 0 wide 64x256, 1 square 128x128, 2 small 64x64
 };
 
 void gemm64(const GemmArgs &g, int bmode, int epi, hipStream_t s);

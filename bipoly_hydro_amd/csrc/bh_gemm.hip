@@ -44,19 +44,24 @@ struct Mfma<float> {
     static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-constexpr int BM = 64, BN = 64, BK = 16;
-constexpr int LDA_T = 4, LDB_T = 4;  // per-thread staged elements of each tile (64*16/256)
+// Workgroup of 4 waves arranged WR x WC; each wave computes TM x TN MFMA
+// tiles of 16 x 16, so the workgroup tile is (WR*TM*16) x (WC*TN*16).  K is
+// staged through LDS in chunks of BK, with the next chunk's global loads
+// issued before the current chunk is multiplied.
+constexpr int BK = 16;
 
-template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB>
+template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB, int WR, int WC, int TM, int TN>
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
 {
     typedef typename Mfma<T>::acc_t acc_t;
-    __shared__ T As[BK][BM + 2];
-    __shared__ T Bs[BK][BN + 2];
+    constexpr int BMt = WR * TM * 16, BNt = WC * TN * 16;
+    constexpr int NA = BMt * BK / 256, NB = BK * BNt / 256;  // staged elements per thread
+    __shared__ T As[BK][BMt + 2];
+    __shared__ T Bs[BK][BNt + 2];
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
-    const int row0 = blockIdx.y * BM, col0 = blockIdx.x * BN;
+    const int wr = wave / WC, wc = wave % WC;
+    const int row0 = blockIdx.y * BMt, col0 = blockIdx.x * BNt;
     const long z = blockIdx.z;
     const T *A = (const T *)g.A + z * g.sA;
     const BT *B = (const BT *)g.B + z * g.sB;
@@ -65,10 +70,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
 
     auto loadA = [&](int k0, T *st) {
 #pragma unroll
-        for (int i = 0; i < LDA_T; ++i) {
+        for (int i = 0; i < NA; ++i) {
             int t = tid + 256 * i;
             int r, k;
-            if (TA) { k = t / BM; r = t % BM; }
+            if (TA) { k = t / BMt; r = t % BMt; }
             else { r = t / BK; k = t % BK; }
             int gr = row0 + r, gk = k0 + k;
             T v = (T)0;
@@ -78,11 +83,11 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
     };
     auto loadB = [&](int k0, T *st) {
 #pragma unroll
-        for (int i = 0; i < LDB_T; ++i) {
+        for (int i = 0; i < NB; ++i) {
             int t = tid + 256 * i;
             int k, c;
             if (TB) { c = t / BK; k = t % BK; }
-            else { k = t / BN; c = t % BN; }
+            else { k = t / BNt; c = t % BNt; }
             int gk = k0 + k, gc = col0 + c;
             T v = (T)0;
             if (gk < Kd && gc < M) {
@@ -103,32 +108,32 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
     };
     auto storeA = [&](const T *st) {
 #pragma unroll
-        for (int i = 0; i < LDA_T; ++i) {
+        for (int i = 0; i < NA; ++i) {
             int t = tid + 256 * i;
             int r, k;
-            if (TA) { k = t / BM; r = t % BM; }
+            if (TA) { k = t / BMt; r = t % BMt; }
             else { r = t / BK; k = t % BK; }
             As[k][r] = st[i];
         }
     };
     auto storeB = [&](const T *st) {
 #pragma unroll
-        for (int i = 0; i < LDB_T; ++i) {
+        for (int i = 0; i < NB; ++i) {
             int t = tid + 256 * i;
             int k, c;
             if (TB) { c = t / BK; k = t % BK; }
-            else { k = t / BN; c = t % BN; }
+            else { k = t / BNt; c = t % BNt; }
             Bs[k][c] = st[i];
         }
     };
 
-    acc_t acc[2][2];
+    acc_t acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = (acc_t){(T)0, (T)0, (T)0, (T)0};
+        for (int j = 0; j < TN; ++j) acc[i][j] = (acc_t){(T)0, (T)0, (T)0, (T)0};
 
-    T sa[LDA_T], sb[LDB_T];
+    T sa[NA], sb[NB];
     loadA(0, sa);
     loadB(0, sb);
     for (int k0 = 0; k0 < Kd; k0 += BK) {
@@ -141,28 +146,28 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
-            T a[2], b[2];
+            T a[TM], b[TN];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kk + (lane >> 4)][wr * 32 + i * 16 + (lane & 15)];
+            for (int i = 0; i < TM; ++i) a[i] = As[kk + (lane >> 4)][(wr * TM + i) * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kk + (lane >> 4)][wc * 32 + j * 16 + (lane & 15)];
+            for (int j = 0; j < TN; ++j) b[j] = Bs[kk + (lane >> 4)][(wc * TN + j) * 16 + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
         }
         __syncthreads();
     }
 
     T *C = (T *)g.C + z * g.sC;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < TN; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int r = row0 + wr * 32 + i * 16 + Mfma<T>::row(lane, v);
-                const int c = col0 + wc * 32 + j * 16 + (lane & 15);
+                const int r = row0 + (wr * TM + i) * 16 + Mfma<T>::row(lane, v);
+                const int c = col0 + (wc * TN + j) * 16 + (lane & 15);
                 if (r >= R || c >= M) continue;
                 T val = acc[i][j][v];
                 const long o = (long)r * g.ldc + c;
@@ -188,11 +193,30 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
             }
 }
 
+// tile shapes (rows x columns): 0: 64 x 256, 1: 128 x 128, 2: 64 x 64,
+// 3: 64 x 128, 4: 128 x 64
+
 template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB>
 static void launch(const GemmArgs &g, hipStream_t s)
 {
-    dim3 grid((g.M + BN - 1) / BN, (g.R + BM - 1) / BM, g.batch > 0 ? g.batch : 1);
-    hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB>), grid, dim3(256), 0, s, g);
+    int shape = g.tile;
+    const unsigned nb = g.batch > 0 ? g.batch : 1;
+    if (shape == 0) {
+        dim3 grid((g.M + 255) / 256, (g.R + 63) / 64, nb);
+        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 1, 4, 4, 4>), grid, dim3(256), 0, s, g);
+    } else if (shape == 1) {
+        dim3 grid((g.M + 127) / 128, (g.R + 127) / 128, nb);
+        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 4, 4>), grid, dim3(256), 0, s, g);
+    } else if (shape == 3) {
+        dim3 grid((g.M + 127) / 128, (g.R + 63) / 64, nb);
+        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 2, 4>), grid, dim3(256), 0, s, g);
+    } else if (shape == 4) {
+        dim3 grid((g.M + 63) / 64, (g.R + 127) / 128, nb);
+        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 4, 2>), grid, dim3(256), 0, s, g);
+    } else {
+        dim3 grid((g.M + 63) / 64, (g.R + 63) / 64, g.batch > 0 ? g.batch : 1);
+        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 2, 2>), grid, dim3(256), 0, s, g);
+    }
 }
 
 void gemm64(const GemmArgs &g, int bmode, int epi, hipStream_t s)

@@ -59,14 +59,14 @@ void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int
 
 // ---------------------------------------------------------------------------
 // ADI half-step, one wavefront per tridiagonal line.
-// Element e of a line sits in lane e % 64, register slot e / 64 (segment), so
-// every load and store of a segment is 64 consecutive doubles.  The Thomas
-// forward sweep y_e = alpha_e y_{e-1} + ibet_e rhs_e and back substitution
-// x_e = y_e + gback_e x_{e+1} are first-order linear recurrences, evaluated
-// per segment as an inclusive wavefront scan of affine maps (DPP row shifts
-// inside rows of 16 lanes, readlane across rows), with the value carried
-// between segments.  A wave owns consecutive lines and rolls the neighbour
-// rows in registers, loading the row two lines ahead while it solves.
+// The Thomas forward sweep y_e = alpha_e y_{e-1} + ibet_e rhs_e and back
+// substitution x_e = y_e + gback_e x_{e+1} are first-order linear
+// recurrences: each lane composes the affine maps of its own consecutive
+// elements, an inclusive wavefront scan of those maps (DPP row shifts inside
+// rows of 16 lanes, readlane across rows) joins the lanes, and each lane then
+// finishes its elements from the value entering it.  A wave owns consecutive
+// lines and rolls the neighbour rows in registers, loading the row two lines
+// ahead while it solves.
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ inline double dpp_d(double old, double v)
@@ -125,8 +125,11 @@ __device__ inline void scan_down(double &A, double &B, int lane)
 }
 
 // LS_LINES lines per workgroup (4 waves, LS_LINES/4 consecutive lines each);
-// LDS tile [LS_LINES][64*C + 1] doubles for the transposed write.
-template <int C, int LS_LINES>
+// LDS tile [LS_LINES][64*V + 1] doubles for the transposed write.  Blocked
+// layout: lane l owns the V consecutive elements e = l*V .. l*V+V-1, composes
+// their affine maps locally, and one wavefront scan per direction joins the
+// lanes.
+template <int V, int LS_LINES>
 __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
 {
     extern __shared__ double sres[];
@@ -134,35 +137,36 @@ __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int L = blockIdx.y;  // mode plane
     const int line0 = blockIdx.x * LS_LINES;
-    const int lenp = 64 * C + 1;
+    const int lenp = 64 * V + 1;
     const double *xin = a.xin + (long)L * a.SA;
     const double *rh = a.rh + (long)L * a.SA;
     const int len = a.len;
+    const int e0 = lane * V;  // first element of this lane
     const double dtt_mtwogamma = a.dt - 2.0 * a.gamma;
 
     // Thomas factors: the same for every line of this mode plane
-    double fa[C], fi[C], fg[C];
+    double fa[V], fi[V], fg[V];
     {
         const double *alpha = a.alpha + (long)L * a.coef_stride;
         const double *ibet = a.ibet + (long)L * a.coef_stride;
         const double *gback = a.gback + (long)L * a.coef_stride;
 #pragma unroll
-        for (int s = 0; s < C; ++s) {
-            const int e = 64 * s + lane;
+        for (int v = 0; v < V; ++v) {
+            const int e = e0 + v;
             const bool ok = e < len;
-            fa[s] = ok ? alpha[e] : 0.0;
-            fi[s] = ok ? ibet[e] : 0.0;
-            fg[s] = ok ? gback[e] : 0.0;
+            fa[v] = ok ? alpha[e] : 0.0;
+            fi[v] = ok ? ibet[e] : 0.0;
+            fg[v] = ok ? gback[e] : 0.0;
         }
     }
     const int first = line0 + wave * LPW;          // first line of this wave
     const int nmine = min(LPW, a.nlines - first);  // may be <= 0
-    double xm[C], xc[C], xn[C], rc[C], rn[C];
+    double xm[V], xc[V], xn[V], rc[V], rn[V];
     auto load_row = [&](const double *row, double *dst) {
 #pragma unroll
-        for (int s = 0; s < C; ++s) {
-            const int e = 64 * s + lane;
-            dst[s] = (e < len) ? row[e + 1] : 0.0;
+        for (int v = 0; v < V; ++v) {
+            const int e = e0 + v;
+            dst[v] = (e < len) ? row[e + 1] : 0.0;
         }
     };
     if (nmine > 0) {
@@ -174,9 +178,8 @@ __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
     for (int k = 0; k < nmine; ++k) {
         const int i = first + k;  // interior line, row i+1 of the input layout
         const double *x0 = xin + (long)(i + 1) * a.PA;
-        // boundary values of this line (positions 0 and len+1)
-        const double b0 = x0[0], b1 = x0[len + 1];
-        double xnn[C];
+        const double b0 = x0[0], b1 = x0[len + 1];  // boundary values of this line
+        double xnn[V];
         const bool more = (k + 1 < nmine);
         if (more) {
             load_row(xin + (long)(i + 3) * a.PA, xnn);
@@ -191,57 +194,73 @@ __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
             cn = -a.alphar[J];
             cp = (J > 2) ? -a.betar[J] : 0.0;
         }
-        double v[C];
+        double val[V];
 #pragma unroll
-        for (int s = 0; s < C; ++s) {
-            const int e = 64 * s + lane;
+        for (int v = 0; v < V; ++v) {
+            const int e = e0 + v;
             double t = 0.0;
             if (e < len) {
                 if (a.dir == 0) {
                     // knownr (helmadi.F90:240-257)
-                    t = -a.four_pi * rc[s] + cd * xc[s] + cp * (xn[s] + xm[s]);
+                    t = -a.four_pi * rc[v] + cd * xc[v] + cp * (xn[v] + xm[v]);
                     if (e == len - 1) t = t - a.alphar[a.numr - 1] * b1;
                 } else {
                     // knownz (helmadi.F90:284-340)
-                    t = -a.four_pi * rc[s] + cd * xc[s] + cn * xn[s];
-                    if (cp != 0.0) t = t + cp * xm[s];
+                    t = -a.four_pi * rc[v] + cd * xc[v] + cn * xn[v];
+                    if (cp != 0.0) t = t + cp * xm[v];
                     if (e == len - 1) t = t + a.gamma * b1;
                     if (e == 0) t = t + a.gamma * b0;
                 }
             }
-            v[s] = t;
+            val[v] = t;
         }
-        // forward sweep
-        double carry = 0.0;
+        // forward sweep y_e = fa_e y_{e-1} + fi_e rhs_e: local composition,
+        // wavefront scan, then the local values from the incoming y
+        {
+            double A = 1.0, B = 0.0;
 #pragma unroll
-        for (int s = 0; s < C; ++s) {
-            double A = fa[s], B = fi[s] * v[s];
+            for (int v = 0; v < V; ++v) {
+                const double b = fi[v] * val[v];
+                B = fa[v] * B + b;
+                A = fa[v] * A;
+            }
             scan_up(A, B, lane);
-            double y = A * carry + B;
-            v[s] = y;
-            carry = readlane_d(y, 63);
-        }
-        // back substitution
-        carry = 0.0;
+            double yin = __shfl_up(B, 1, 64);  // y of the element before this lane's first
+            if (lane == 0) yin = 0.0;
 #pragma unroll
-        for (int s = C - 1; s >= 0; --s) {
-            double A = fg[s], B = v[s];
+            for (int v = 0; v < V; ++v) {
+                yin = fa[v] * yin + fi[v] * val[v];
+                val[v] = yin;
+            }
+        }
+        // back substitution x_e = y_e + fg_e x_{e+1}
+        {
+            double A = 1.0, B = 0.0;
+#pragma unroll
+            for (int v = V - 1; v >= 0; --v) {
+                B = fg[v] * B + val[v];
+                A = fg[v] * A;
+            }
             scan_down(A, B, lane);
-            double x = A * carry + B;
-            v[s] = x;
-            carry = readlane_d(x, 0);
+            double xin_ = __shfl_down(B, 1, 64);  // x of the element after this lane's last
+            if (lane == 63) xin_ = 0.0;
+#pragma unroll
+            for (int v = V - 1; v >= 0; --v) {
+                xin_ = fg[v] * xin_ + val[v];
+                val[v] = xin_;
+            }
         }
         const int jl = wave * LPW + k;
 #pragma unroll
-        for (int s = 0; s < C; ++s) sres[jl * lenp + 64 * s + lane] = v[s];
+        for (int v = 0; v < V; ++v) sres[jl * lenp + e0 + v] = val[v];
         // roll the neighbour rows
 #pragma unroll
-        for (int s = 0; s < C; ++s) {
-            xm[s] = xc[s];
-            xc[s] = xn[s];
+        for (int v = 0; v < V; ++v) {
+            xm[v] = xc[v];
+            xc[v] = xn[v];
             if (more) {
-                xn[s] = xnn[s];
-                rc[s] = rn[s];
+                xn[v] = xnn[v];
+                rc[v] = rn[v];
             }
         }
     }

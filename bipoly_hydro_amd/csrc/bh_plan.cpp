@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -79,6 +80,7 @@ struct bh_plan {
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
     // mixed precision (BH_PREC_MIXED)
     int passes = 1;                                // FP64 refinement passes
+    int tile_dft = 2, tile_spec = 3;               // GEMM tile shapes (bh_gemm.hip), measured best
     float *Dc32 = nullptr, *Ds32 = nullptr, *S32 = nullptr, *mu32[2] = {nullptr, nullptr}, *Mr[2] = {nullptr, nullptr};
     float *tri_ibet = nullptr, *tri_gam = nullptr;
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
@@ -156,6 +158,9 @@ extern "C" int bh_plan_create(const bh_grid_desc *desc, int precision, size_t hb
 
     bh_plan *p = new bh_plan();
     p->d = d;
+    // GEMM tile shapes, overridable for tuning sweeps
+    if (const char *e = std::getenv("BH_GEMM_TILE_DFT")) p->tile_dft = std::atoi(e);
+    if (const char *e = std::getenv("BH_GEMM_TILE_SPEC")) p->tile_spec = std::atoi(e);
     p->precision = precision;
     p->budget = hbm_budget;
     (void)hipGetDevice(&p->device);
@@ -598,6 +603,7 @@ void dft_fwd64(const bh_plan *p, const double *B, long ldb, int Mcols, double *C
     a.A = p->Dc; a.lda = nh; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc;
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
     a.mask_numr = p->g.numr; a.mask_numz = p->g.numz;
+    a.tile = Mcols >= 4096 ? p->tile_dft : 2;
     gemm64(a, B_FOLD_PLUS, mask ? E_MASK : E_PLAIN, s);
     a.A = p->Ds; a.lda = nsn; a.C = C + (long)nh * ldc; a.R = nsn; a.Kd = nsn;
     gemm64(a, B_FOLD_MINUS, mask ? E_MASK : E_PLAIN, s);
@@ -612,6 +618,7 @@ void dft_inv64(const bh_plan *p, const double *X, const float *Xadd, long ldx, i
     GemmArgs a{};
     a.A = p->Ec; a.lda = nh; a.B = X; a.ldb = ldx; a.Badd = Xadd; a.C = out; a.ldc = ldo;
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
+    a.tile = Mcols >= 4096 ? p->tile_dft : 2;
     gemm64(a, Xadd ? B_ADD_F32 : B_PLAIN, E_PLAIN, s);
     a.A = p->Es; a.lda = nsn; a.B = X + (long)nh * ldx; a.Badd = Xadd ? Xadd + (long)nh * ldx : nullptr;
     a.R = nsn; a.Kd = nsn;
@@ -626,6 +633,7 @@ void proj64(const bh_plan *p, const double *rho, hipStream_t s)
     GemmArgs a{};
     a.A = p->Pm; a.lda = N; a.B = rho; a.ldb = M; a.C = p->proj; a.ldc = M;
     a.R = 2 * p->g.mmax - 1; a.M = (int)M; a.Kd = N; a.batch = 1;
+    a.tile = M >= 4096 ? p->tile_dft : 2;
     gemm64(a, B_PLAIN, E_PLAIN, s);
 }
 
@@ -647,6 +655,7 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
         a.A = p->Dc32; a.lda = nh; a.B = d_rho; a.ldb = M; a.C = p->RH32; a.ldc = M;
         a.R = nh; a.M = (int)M; a.Kd = nh; a.batch = 1; a.nfold = N;
         a.mask_numr = numr; a.mask_numz = numz;
+        a.tile = M >= 4096 ? p->tile_dft : 2;
         gemm32(a, B_FOLD_PLUS, E_MASK, false, false, true, s);
         a.A = p->Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * M; a.R = nsn; a.Kd = nsn;
         gemm32(a, B_FOLD_MINUS, E_MASK, false, false, true, s);
@@ -678,6 +687,7 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
         a.B = p->S32; a.ldb = nzi; a.sB = 0;
         a.C = out; a.ldc = nzi; a.sC = PIm;
         a.R = nri; a.M = nzi; a.Kd = nzi; a.batch = N;
+        a.tile = p->tile_spec;
         gemm32(a, B_PLAIN, E_PLAIN, true, false, false, s);
     };
     // inverse: full-plane out[L] interior (nzi x nri) = S * in[L]^T
@@ -687,6 +697,7 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
         a.B = in; a.ldb = nzi; a.sB = PIm;
         a.C = out + numr + 1; a.ldc = numr; a.sC = M;
         a.R = nzi; a.M = nri; a.Kd = nzi; a.batch = N;
+        a.tile = p->tile_spec;
         gemm32(a, B_PLAIN, E_PLAIN, false, true, false, s);
     };
     dst_fwd(p->r32, p->RT);
@@ -698,6 +709,7 @@ int enqueue_solve_mixed(bh_plan *p, const double *d_rho, double *d_pot, int tste
         a.C = p->WT; a.ldc = nzi; a.sC = PIm;
         a.R = nri; a.M = nzi; a.Kd = nri; a.batch = N;
         a.mu = p->mu32[w]; a.Rin = p->RT; a.C2 = p->CT;
+        a.tile = p->tile_spec;
         gemm32(a, B_PLAIN, E_FILTER, false, false, false, s);
     }
     // 5. (B_r + lambda_z) dhat = C, back to physical space
@@ -868,6 +880,7 @@ extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb
 {
     GemmArgs a{};
     a.A = A; a.lda = lda; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc; a.R = R; a.M = M; a.Kd = Kd; a.batch = 1;
+    a.tile = (M >= 256 && R >= 64) ? 0 : 2;
     gemm64(a, B_PLAIN, E_PLAIN, 0);
     HIPCHK(hipDeviceSynchronize());
     return BH_OK;
@@ -897,6 +910,20 @@ extern "C" int bhx_dft64(const bh_plan *p, const double *in, double *out, int nc
     if (!p || !in || !out) return fail(BH_E_INVALID, "null argument");
     if (inverse) dft_inv64(p, in, nullptr, ncols, ncols, out, ncols, 0);
     else dft_fwd64(p, in, ncols, ncols, out, ncols, false, 0);
+    HIPCHK(hipDeviceSynchronize());
+    return BH_OK;
+}
+
+// FP64 / FP32 GEMM with an explicit tile shape (0 wide, 1 square, 2 small)
+// and transposes (FP32 only); test hook.
+extern "C" int bhx_gemm_tile(int fp32, const void *A, long lda, const void *B, long ldb, int R, int M, int Kd, void *C,
+                             long ldc, int tile, int ta, int tb)
+{
+    GemmArgs a{};
+    a.A = A; a.lda = lda; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc; a.R = R; a.M = M; a.Kd = Kd; a.batch = 1;
+    a.tile = tile;
+    if (fp32) gemm32(a, B_PLAIN, E_PLAIN, ta != 0, tb != 0, false, 0);
+    else gemm64(a, B_PLAIN, E_PLAIN, 0);
     HIPCHK(hipDeviceSynchronize());
     return BH_OK;
 }

@@ -48,3 +48,27 @@ def test_folded_dft_matches_numpy(numphi):
         dz = torch.zeros_like(dx)
         assert L.bhx_dft64(plan.handle, dy.data_ptr(), dz.data_ptr(), ncols, 1) == 0
         np.testing.assert_allclose(dz.cpu().numpy(), x, rtol=0, atol=1e-13 * numphi)
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("fp32,ta,tb", [(0, 0, 0), (1, 0, 0), (1, 1, 0), (1, 0, 1)])
+def test_gemm_tiles_and_transposes(tile, fp32, ta, tb):
+    torch = pytest.importorskip("torch")
+    L = _lib()
+    vp = ctypes.c_void_p
+    L.bhx_gemm_tile.argtypes = [ctypes.c_int, vp, ctypes.c_long, vp, ctypes.c_long, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, vp, ctypes.c_long, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    R, M, K = 200, 300, 130
+    dt = np.float32 if fp32 else np.float64
+    rng = np.random.default_rng(tile * 10 + fp32 + 2 * ta + 4 * tb)
+    A, B = rng.standard_normal((R, K)).astype(dt), rng.standard_normal((K, M)).astype(dt)
+    As = np.ascontiguousarray(A.T) if ta else A
+    Bs = np.ascontiguousarray(B.T) if tb else B
+    dA, dB = torch.from_numpy(As).cuda(), torch.from_numpy(Bs).cuda()
+    dC = torch.zeros((R, M), dtype=torch.float32 if fp32 else torch.float64, device="cuda")
+    lda = R if ta else K
+    ldb = K if tb else M
+    assert L.bhx_gemm_tile(fp32, dA.data_ptr(), lda, dB.data_ptr(), ldb, R, M, K, dC.data_ptr(), M, tile, ta, tb) == 0
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    tol = 1e-4 * np.sqrt(K) if fp32 else 1e-12 * K
+    np.testing.assert_allclose(dC.cpu().numpy(), ref, rtol=0, atol=tol)
