@@ -66,6 +66,9 @@ def main():
     ap.add_argument("--warm", action="store_true", help="time warm solves (tstep > 1, 5 ADI steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first copy 256 MiB device-to-device three times (a known-size kernel for "
+                         "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; --gpus must be 1", file=sys.stderr)
@@ -77,6 +80,13 @@ def main():
 
     numr, numz, numphi = args.grid
     dev = torch.device("cuda:0")
+    if args.calib_copy:
+        src = torch.ones(1 << 25, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if (numr, numz, numphi) == (258, 130, 512):
         rho = density.config2_density(seed=1234)
     else:

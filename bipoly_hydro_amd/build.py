@@ -17,7 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbh_potential.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["bh_kernels.hip", "bh_gemm.hip", "bh_mixed.hip", "bh_plan.cpp"]
+SOURCES = ["bh_kernels.hip", "bh_gemm.hip", "bh_mixed.hip", "bh_ops.cpp", "bh_resources.cpp", "bh_plan.cpp",
+           "bh_exec.cpp", "bh_batch.cpp"]
 
 # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -25,17 +26,24 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
              "-I" + os.path.join(ROOT, "include")]
 
 
-def build(verbose: bool = False) -> str:
-    objs = []
+def build(verbose: bool = False, jobs: int = 0) -> str:
+    from concurrent.futures import ThreadPoolExecutor
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
+    cmds, objs = [], []
     for src in SOURCES:
         obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+
+    def run(cmd):
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
-        objs.append(obj)
+
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(run, cmds))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:

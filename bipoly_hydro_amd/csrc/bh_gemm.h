@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "bh_ops.h"
+
 namespace bh {
 
 enum { B_PLAIN = 0, B_FOLD_PLUS = 1, B_FOLD_MINUS = 2, B_ADD_F32 = 3 };
@@ -20,8 +22,19 @@ struct GemmArgs {
     int mask_numr, mask_numz;     // E_MASK
     const float *mu;              // E_FILTER: per-column factor
     const void *Rin; void *C2;    // E_FILTER: C2 = Rin - mu * acc, same indexing as C
-    int tile;                     // 0 wide 64x256, 1 square 128x128, 2 small 64x64
+    int tile;                     // 0: 64x256, 1: 128x128, 2: 64x64, 3: 64x128, 4: 128x64 (rows x cols)
 };
+
+// The instantiated (element types, B mode, epilogue, transposes) combinations.
+enum GemmInst {
+    G64_PLAIN = 0, G64_FOLDP_MASK, G64_FOLDM_MASK, G64_FOLDP, G64_FOLDM, G64_INV, G64_ADD, G64_ADD_INV,
+    G32D_FOLDP_MASK, G32D_FOLDM_MASK, G32_TA, G32_TB, G32_FILTER, G32_PLAIN, G_NINST
+};
+int gemm_inst64(int bmode, int epi);
+int gemm_inst32(int bmode, int epi, bool ta, bool tb, bool b_double);
+// The Op of one product (variant = inst * 8 + tile; grid from the tile shape).
+Op gemm_op(const GemmArgs &g, int inst);
+void gemm_tile_rows_cols(int tile, int *rows, int *cols);
 
 void gemm64(const GemmArgs &g, int bmode, int epi, hipStream_t s);
 void gemm32(const GemmArgs &g, int bmode, int epi, bool ta, bool tb, bool b_double, hipStream_t s);

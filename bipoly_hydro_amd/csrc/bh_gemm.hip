@@ -18,8 +18,11 @@
 // zero mask of potential_solver.F90:70-78, applies the sine-mode filter of
 // the mixed solve, or combines the two halves of an inverse real DFT.
 #include "bh_gemm.h"
+#include "bh_ops.h"
 
 #include <hip/hip_runtime.h>
+
+#include <utility>
 
 namespace bh {
 
@@ -51,7 +54,7 @@ struct Mfma<float> {
 constexpr int BK = 16;
 
 template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB, int WR, int WC, int TM, int TN>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int bx, int by, int bz)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int BMt = WR * TM * 16, BNt = WC * TN * 16;
@@ -61,8 +64,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave / WC, wc = wave % WC;
-    const int row0 = blockIdx.y * BMt, col0 = blockIdx.x * BNt;
-    const long z = blockIdx.z;
+    const int row0 = by * BMt, col0 = bx * BNt;
+    const long z = bz;
     const T *A = (const T *)g.A + z * g.sA;
     const BT *B = (const BT *)g.B + z * g.sB;
     const float *Badd = g.Badd ? g.Badd + z * g.sB : nullptr;
@@ -193,57 +196,164 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
             }
 }
 
+// Template instances: element types, B mode and epilogue per GemmInst, and
+// the workgroup arrangement per tile shape.
+template <int I>
+struct Inst;
+#define BH_INST(I, T_, BT_, BM_, EP_, TA_, TB_)           \
+    template <>                                          \
+    struct Inst<I> {                                     \
+        typedef T_ T;                                    \
+        typedef BT_ BT;                                  \
+        static constexpr int BMODE = BM_, EPI = EP_;     \
+        static constexpr bool TA = TA_, TB = TB_;        \
+    };
+BH_INST(G64_PLAIN, double, double, B_PLAIN, E_PLAIN, false, false)
+BH_INST(G64_FOLDP_MASK, double, double, B_FOLD_PLUS, E_MASK, false, false)
+BH_INST(G64_FOLDM_MASK, double, double, B_FOLD_MINUS, E_MASK, false, false)
+BH_INST(G64_FOLDP, double, double, B_FOLD_PLUS, E_PLAIN, false, false)
+BH_INST(G64_FOLDM, double, double, B_FOLD_MINUS, E_PLAIN, false, false)
+BH_INST(G64_INV, double, double, B_PLAIN, E_INV_COMBINE, false, false)
+BH_INST(G64_ADD, double, double, B_ADD_F32, E_PLAIN, false, false)
+BH_INST(G64_ADD_INV, double, double, B_ADD_F32, E_INV_COMBINE, false, false)
+BH_INST(G32D_FOLDP_MASK, float, double, B_FOLD_PLUS, E_MASK, false, false)
+BH_INST(G32D_FOLDM_MASK, float, double, B_FOLD_MINUS, E_MASK, false, false)
+BH_INST(G32_TA, float, float, B_PLAIN, E_PLAIN, true, false)
+BH_INST(G32_TB, float, float, B_PLAIN, E_PLAIN, false, true)
+BH_INST(G32_FILTER, float, float, B_PLAIN, E_FILTER, false, false)
+BH_INST(G32_PLAIN, float, float, B_PLAIN, E_PLAIN, false, false)
+#undef BH_INST
+
 // tile shapes (rows x columns): 0: 64 x 256, 1: 128 x 128, 2: 64 x 64,
 // 3: 64 x 128, 4: 128 x 64
+template <int S>
+struct Shape;
+template <> struct Shape<0> { static constexpr int WR = 1, WC = 4, TM = 4, TN = 4; };
+template <> struct Shape<1> { static constexpr int WR = 2, WC = 2, TM = 4, TN = 4; };
+template <> struct Shape<2> { static constexpr int WR = 2, WC = 2, TM = 2, TN = 2; };
+template <> struct Shape<3> { static constexpr int WR = 2, WC = 2, TM = 2, TN = 4; };
+template <> struct Shape<4> { static constexpr int WR = 2, WC = 2, TM = 4, TN = 2; };
 
-template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB>
-static void launch(const GemmArgs &g, hipStream_t s)
+template <int I, int S>
+__device__ __forceinline__ void gemm_tile_is(const GemmArgs &g, int bx, int by, int bz)
 {
-    int shape = g.tile;
-    const unsigned nb = g.batch > 0 ? g.batch : 1;
-    if (shape == 0) {
-        dim3 grid((g.M + 255) / 256, (g.R + 63) / 64, nb);
-        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 1, 4, 4, 4>), grid, dim3(256), 0, s, g);
-    } else if (shape == 1) {
-        dim3 grid((g.M + 127) / 128, (g.R + 127) / 128, nb);
-        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 4, 4>), grid, dim3(256), 0, s, g);
-    } else if (shape == 3) {
-        dim3 grid((g.M + 127) / 128, (g.R + 63) / 64, nb);
-        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 2, 4>), grid, dim3(256), 0, s, g);
-    } else if (shape == 4) {
-        dim3 grid((g.M + 63) / 64, (g.R + 127) / 128, nb);
-        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 4, 2>), grid, dim3(256), 0, s, g);
-    } else {
-        dim3 grid((g.M + 63) / 64, (g.R + 63) / 64, g.batch > 0 ? g.batch : 1);
-        hipLaunchKernelGGL((gemm_kernel<T, BT, BMODE, EPI, TA, TB, 2, 2, 2, 2>), grid, dim3(256), 0, s, g);
-    }
+    typedef Inst<I> In;
+    typedef Shape<S> Sh;
+    gemm_tile<typename In::T, typename In::BT, In::BMODE, In::EPI, In::TA, In::TB, Sh::WR, Sh::WC, Sh::TM, Sh::TN>(
+        g, bx, by, bz);
+}
+
+template <int I, int S>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
+{
+    gemm_tile_is<I, S>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <int I, int S>
+__global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmArgs *__restrict__ args, GroupSched gs)
+{
+    BH_GROUP_LOOP(GemmArgs, args, gs, (gemm_tile_is<I, S>(ga, bx, by, bz)));
+}
+
+typedef void (*GemmSingleFn)(GemmArgs);
+typedef void (*GemmGroupedFn)(const GemmArgs *, GroupSched);
+constexpr int G_NSHAPE = 5;
+
+template <int... V>
+struct GemmTable {
+    static constexpr GemmSingleFn single[] = {&gemm_kernel<V / G_NSHAPE, V % G_NSHAPE>...};
+    static constexpr GemmGroupedFn grouped[] = {&gemm_grouped_kernel<V / G_NSHAPE, V % G_NSHAPE>...};
+};
+template <int... V>
+constexpr GemmSingleFn GemmTable<V...>::single[];
+template <int... V>
+constexpr GemmGroupedFn GemmTable<V...>::grouped[];
+
+template <int... V>
+GemmTable<V...> make_table(std::integer_sequence<int, V...>);
+typedef decltype(make_table(std::make_integer_sequence<int, G_NINST * G_NSHAPE>())) Table;
+
+void gemm_tile_rows_cols(int tile, int *rows, int *cols)
+{
+    static const int r[G_NSHAPE] = {64, 128, 64, 64, 128}, c[G_NSHAPE] = {256, 128, 64, 128, 64};
+    if (tile < 0 || tile >= G_NSHAPE) tile = 2;
+    *rows = r[tile];
+    *cols = c[tile];
+}
+
+Op gemm_op(const GemmArgs &g, int inst)
+{
+    int rows, cols;
+    const int tile = (g.tile >= 0 && g.tile < G_NSHAPE) ? g.tile : 2;
+    gemm_tile_rows_cols(tile, &rows, &cols);
+    GemmArgs a = g;
+    a.tile = tile;
+    return Op::make(OP_GEMM, inst * 8 + tile, a, (g.M + cols - 1) / cols, (g.R + rows - 1) / rows,
+                    g.batch > 0 ? g.batch : 1);
+}
+
+static int variant_index(int variant)
+{
+    const int inst = variant / 8, tile = variant % 8;
+    if (inst < 0 || inst >= G_NINST || tile >= G_NSHAPE) return -1;
+    return inst * G_NSHAPE + tile;
+}
+
+bool gemm_launch(const Op &op, hipStream_t s)
+{
+    if (op.kind != OP_GEMM) return false;
+    const int v = variant_index(op.variant);
+    if (v < 0) return true;
+    hipLaunchKernelGGL(Table::single[v], dim3(op.nx, op.ny, op.nz), dim3(256), 0, s, op.as<GemmArgs>());
+    return true;
+}
+
+bool gemm_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
+{
+    if (kind != OP_GEMM) return false;
+    const int v = variant_index(variant);
+    if (v < 0) return true;
+    hipLaunchKernelGGL(Table::grouped[v], dim3(nblocks), dim3(256), 0, s, (const GemmArgs *)d_args, gs);
+    return true;
+}
+
+bool gemm_occupancy(int kind, int variant, int *blocks)
+{
+    if (kind != OP_GEMM) return false;
+    const int v = variant_index(variant);
+    *blocks = 0;
+    if (v >= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, (const void *)Table::grouped[v], 256, 0) !=
+                      hipSuccess)
+        *blocks = 0;
+    return true;
 }
 
 void gemm64(const GemmArgs &g, int bmode, int epi, hipStream_t s)
 {
-    if (bmode == B_PLAIN && epi == E_PLAIN) launch<double, double, B_PLAIN, E_PLAIN, false, false>(g, s);
-    else if (bmode == B_FOLD_PLUS && epi == E_MASK) launch<double, double, B_FOLD_PLUS, E_MASK, false, false>(g, s);
-    else if (bmode == B_FOLD_MINUS && epi == E_MASK) launch<double, double, B_FOLD_MINUS, E_MASK, false, false>(g, s);
-    else if (bmode == B_FOLD_PLUS && epi == E_PLAIN) launch<double, double, B_FOLD_PLUS, E_PLAIN, false, false>(g, s);
-    else if (bmode == B_FOLD_MINUS && epi == E_PLAIN) launch<double, double, B_FOLD_MINUS, E_PLAIN, false, false>(g, s);
-    else if (bmode == B_PLAIN && epi == E_INV_COMBINE)
-        launch<double, double, B_PLAIN, E_INV_COMBINE, false, false>(g, s);
-    else if (bmode == B_ADD_F32 && epi == E_PLAIN) launch<double, double, B_ADD_F32, E_PLAIN, false, false>(g, s);
-    else if (bmode == B_ADD_F32 && epi == E_INV_COMBINE)
-        launch<double, double, B_ADD_F32, E_INV_COMBINE, false, false>(g, s);
+    launch_op(gemm_op(g, gemm_inst64(bmode, epi)), s);
 }
 
 void gemm32(const GemmArgs &g, int bmode, int epi, bool ta, bool tb, bool b_double, hipStream_t s)
 {
-    if (b_double) {
-        if (bmode == B_FOLD_PLUS && epi == E_MASK) launch<float, double, B_FOLD_PLUS, E_MASK, false, false>(g, s);
-        else if (bmode == B_FOLD_MINUS && epi == E_MASK) launch<float, double, B_FOLD_MINUS, E_MASK, false, false>(g, s);
-        return;
-    }
-    if (ta && !tb) launch<float, float, B_PLAIN, E_PLAIN, true, false>(g, s);
-    else if (!ta && tb) launch<float, float, B_PLAIN, E_PLAIN, false, true>(g, s);
-    else if (epi == E_FILTER) launch<float, float, B_PLAIN, E_FILTER, false, false>(g, s);
-    else launch<float, float, B_PLAIN, E_PLAIN, false, false>(g, s);
+    launch_op(gemm_op(g, gemm_inst32(bmode, epi, ta, tb, b_double)), s);
+}
+
+int gemm_inst64(int bmode, int epi)
+{
+    if (bmode == B_FOLD_PLUS) return epi == E_MASK ? G64_FOLDP_MASK : G64_FOLDP;
+    if (bmode == B_FOLD_MINUS) return epi == E_MASK ? G64_FOLDM_MASK : G64_FOLDM;
+    if (bmode == B_ADD_F32) return epi == E_INV_COMBINE ? G64_ADD_INV : G64_ADD;
+    return epi == E_INV_COMBINE ? G64_INV : G64_PLAIN;
+}
+
+int gemm_inst32(int bmode, int epi, bool ta, bool tb, bool b_double)
+{
+    if (b_double) return bmode == B_FOLD_MINUS ? G32D_FOLDM_MASK : G32D_FOLDP_MASK;
+    if (ta) return G32_TA;
+    if (tb) return G32_TB;
+    if (epi == E_FILTER) return G32_FILTER;
+    (void)bmode;
+    return G32_PLAIN;
 }
 
 }  // namespace bh

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 
+#include "bh_ops.h"
+
 namespace bh {
 
 // Geometry on the device side.  J-layout: [mode/phi][K][J], J fastest, pitch
@@ -31,14 +33,49 @@ struct LineArgs {
     int numr;
 };
 
+struct FillArgs {
+    double *p;
+    long n;
+    double v;
+};
+struct TransposeArgs {
+    const double *src;
+    double *dst;
+    int numr, numz;
+};
+struct AxisArgs {
+    double *pot;
+    int numr, numz, numphi;
+};
+struct ContractArgs {
+    const double *table, *proj;
+    double *out;
+    int numr, numz, mmax;
+};
+struct RebuildArgs {
+    const double *Stb, *Ss, *besc, *bess;
+    double *F;
+    int numr, numz, numphi, mmax;
+    double factor;
+};
+struct ScatterArgs {
+    const double *FF;
+    double *X1, *X2;
+    int numr, numz, x2_klayout;
+};
+
+// grid-stride fill for plan set-up
 void launch_fill(double *p, size_t n, double v, hipStream_t s);
-
-void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int nplanes,
-                         hipStream_t s);
-
+// Ops of the solve sequence.  fill_op writes n values but its launch grid
+// covers `capacity` (n may be 0: same grid, no work).
+Op fill_op(double *p, size_t n, size_t capacity, double v);
+// [plane][K][J] -> [plane][J][K]
+Op transpose_op(const double *src, double *dst, int numr, int numz, int nplanes);
+// one ADI half-step; nl = lines per workgroup (8 or 16, 0: default)
+Op sweep_op(const LineArgs &a, int nplanes, int nl);
 void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s);
-
-void launch_axis(double *pot, int numr, int numz, int numphi, hipStream_t s);
+// axis ghost by a pi rotation (helmadi.F90:396-405)
+Op axis_op(double *pot, int numr, int numz, int numphi);
 
 // Green's tables ------------------------------------------------------------
 struct TableArgs {
@@ -62,16 +99,13 @@ constexpr int CT_SPLIT = 4;
 // proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
 // Stb: [CT_SPLIT][mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
 // Ss:  [CT_SPLIT][mmax][2][nzi]
-void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz,
-                            int mmax, hipStream_t s);
-void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz,
-                          int mmax, hipStream_t s);
+Op contract_topbot_op(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax);
+Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
 // faces F[L][fi]: fi in [0,nri) top J=fi+2, [nri,2nri) bottom, [2nri,2nri+nzi) side
-void launch_face_rebuild(const double *Stb, const double *Ss, const double *besc, const double *bess,
-                         double *F, int numr, int numz, int numphi, int mmax, double factor,
-                         hipStream_t s);
-// scatter Fourier-space faces FF[mode][fi] into both layouts
-void launch_face_scatter(const double *FF, double *XJ, double *XK, int numr, int numz, int numphi,
-                         hipStream_t s);
+Op face_rebuild_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *F, int numr,
+                   int numz, int numphi, int mmax, double factor);
+// scatter Fourier-space faces FF[mode][fi] into X1 (J-layout) and X2
+// (K-layout when x2_klayout, else J-layout; may be null)
+Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int numr, int numz, int numphi);
 
 }  // namespace bh

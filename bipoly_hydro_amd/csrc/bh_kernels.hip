@@ -9,13 +9,14 @@
 //   contract_*    boundary contraction (bessel.F90:186-262)
 //   face_*        boundary face rebuild and placement (bessel.F90:396-488)
 #include "bh_internal.h"
+#include "bh_ops.h"
 
 #include <hip/hip_runtime.h>
 
 namespace bh {
 
 // ---------------------------------------------------------------------------
-__global__ void fill_kernel(double *p, size_t n, double v)
+__global__ void setup_fill_kernel(double *p, size_t n, double v)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -27,18 +28,30 @@ void launch_fill(double *p, size_t n, double v, hipStream_t s)
     if (!n) return;
     size_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+    hipLaunchKernelGGL(setup_fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+}
+
+// Tiled fill for solve sequences: tile bx covers FILL_TILE elements.
+constexpr int FILL_TILE = 2048;
+__device__ __forceinline__ void fill_tile(const FillArgs &a, int bx, int, int)
+{
+    const long i0 = (long)bx * FILL_TILE;
+#pragma unroll
+    for (int u = 0; u < FILL_TILE / 256; ++u) {
+        const long i = i0 + u * 256 + threadIdx.x;
+        if (i < a.n) a.p[i] = a.v;
+    }
 }
 
 // [plane][K][J] -> [plane][J][K]
-__global__ __launch_bounds__(256) void transpose_jk_kernel(const double *__restrict__ src,
-                                                           double *__restrict__ dst, int numr, int numz)
+__device__ __forceinline__ void transpose_tile(const TransposeArgs &a, int bx, int by, int bz)
 {
     __shared__ double tile[32][33];
+    const int numr = a.numr, numz = a.numz;
     const long plane = (long)numr * numz;
-    const double *s = src + blockIdx.z * plane;
-    double *d = dst + blockIdx.z * plane;
-    int j0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+    const double *s = a.src + (long)bz * plane;
+    double *d = a.dst + (long)bz * plane;
+    int j0 = bx * 32, k0 = by * 32;
     int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (int r = ty; r < 32; r += 8) {
         int k = k0 + r, j = j0 + tx;
@@ -51,10 +64,16 @@ __global__ __launch_bounds__(256) void transpose_jk_kernel(const double *__restr
     }
 }
 
-void launch_transpose_jk(const double *src, double *dst, int numr, int numz, int nplanes, hipStream_t s)
+Op transpose_op(const double *src, double *dst, int numr, int numz, int nplanes)
 {
-    dim3 grid((numr + 31) / 32, (numz + 31) / 32, nplanes);
-    hipLaunchKernelGGL(transpose_jk_kernel, grid, dim3(256), 0, s, src, dst, numr, numz);
+    TransposeArgs a{src, dst, numr, numz};
+    return Op::make(OP_TRANSPOSE, 0, a, (numr + 31) / 32, (numz + 31) / 32, nplanes);
+}
+
+Op fill_op(double *p, size_t n, size_t capacity, double v)
+{
+    FillArgs a{p, (long)n, v};
+    return Op::make(OP_FILL, 0, a, (int)((capacity + FILL_TILE - 1) / FILL_TILE), 1, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -130,13 +149,12 @@ __device__ inline void scan_down(double &A, double &B, int lane)
 // their affine maps locally, and one wavefront scan per direction joins the
 // lanes.
 template <int V, int LS_LINES>
-__global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
+__device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int by, double *sres)
 {
-    extern __shared__ double sres[];
     constexpr int LPW = LS_LINES / 4;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int L = blockIdx.y;  // mode plane
-    const int line0 = blockIdx.x * LS_LINES;
+    const int L = by;  // mode plane
+    const int line0 = bx * LS_LINES;
     const int lenp = 64 * V + 1;
     const double *xin = a.xin + (long)L * a.SA;
     const double *rh = a.rh + (long)L * a.SA;
@@ -274,31 +292,42 @@ __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
     }
 }
 
-void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s)
+template <int V, int NL>
+__global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
 {
-    int c = (a.len + 63) / 64;
-    if (c <= 4) {
-        constexpr int NL = 16;
-        dim3 grid((a.nlines + NL - 1) / NL, nplanes);
-        if (c <= 1)
-            hipLaunchKernelGGL((line_sweep_kernel<1, NL>), grid, dim3(256), NL * (64 * 1 + 1) * sizeof(double), s, a);
-        else if (c <= 2)
-            hipLaunchKernelGGL((line_sweep_kernel<2, NL>), grid, dim3(256), NL * (64 * 2 + 1) * sizeof(double), s, a);
-        else
-            hipLaunchKernelGGL((line_sweep_kernel<4, NL>), grid, dim3(256), NL * (64 * 4 + 1) * sizeof(double), s, a);
-    } else {
-        constexpr int NL = 8;  // 8 x 513 doubles = 32.8 KB
-        dim3 grid((a.nlines + NL - 1) / NL, nplanes);
-        hipLaunchKernelGGL((line_sweep_kernel<8, NL>), grid, dim3(256), NL * (64 * 8 + 1) * sizeof(double), s, a);
-    }
+    extern __shared__ double sres[];
+    line_sweep_tile<V, NL>(a, blockIdx.x, blockIdx.y, sres);
 }
+
+template <int V, int NL>
+__global__ __launch_bounds__(256) void line_sweep_grouped(const LineArgs *__restrict__ args, GroupSched gs)
+{
+    extern __shared__ double sres[];
+    BH_GROUP_LOOP(LineArgs, args, gs, (line_sweep_tile<V, NL>(ga, bx, by, sres)));
+}
+
+// variant = V * 100 + NL: V elements per lane (line length <= 64 V), NL
+// lines per workgroup
+static int sweep_lds(int variant) { return (variant % 100) * (64 * (variant / 100) + 1) * (int)sizeof(double); }
+
+Op sweep_op(const LineArgs &a, int nplanes, int nl)
+{
+    const int c = (a.len + 63) / 64;
+    const int V = c <= 1 ? 1 : c <= 2 ? 2 : c <= 4 ? 4 : 8;
+    if (nl != 8 && nl != 16) nl = (V <= 4) ? 16 : 8;
+    return Op::make(OP_SWEEP, V * 100 + nl, a, (a.nlines + nl - 1) / nl, nplanes, 1);
+}
+
+void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s) { launch_op(sweep_op(a, nplanes, 0), s); }
 
 // axis ghost by a pi rotation: pot(1,K,L) = pot(2,K,L +- numphi/2)
 // (helmadi.F90:396-405)
-__global__ void axis_kernel(double *pot, int numr, int numz, int numphi)
+__device__ __forceinline__ void axis_tile(const AxisArgs &a, int bx, int by, int)
 {
-    int K = blockIdx.x * blockDim.x + threadIdx.x;
-    int L = blockIdx.y;
+    const int numr = a.numr, numz = a.numz, numphi = a.numphi;
+    double *pot = a.pot;
+    int K = bx * 128 + threadIdx.x;
+    int L = by;
     if (K >= numz) return;
     int h = numphi / 2;
     int Ls = (L < h) ? L + h : L - h;
@@ -306,10 +335,10 @@ __global__ void axis_kernel(double *pot, int numr, int numz, int numphi)
     pot[(long)L * plane + (long)K * numr] = pot[(long)Ls * plane + (long)K * numr + 1];
 }
 
-void launch_axis(double *pot, int numr, int numz, int numphi, hipStream_t s)
+Op axis_op(double *pot, int numr, int numz, int numphi)
 {
-    dim3 grid((numz + 127) / 128, numphi);
-    hipLaunchKernelGGL(axis_kernel, grid, dim3(128), 0, s, pot, numr, numz, numphi);
+    AxisArgs a{pot, numr, numz, numphi};
+    return Op::make(OP_AXIS, 0, a, (numz + 127) / 128, numphi, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,14 +489,15 @@ constexpr int CT_TGT = 8;  // contraction targets per workgroup
 // sums in a fixed order, so results are reproducible bit for bit.
 
 // Top and bottom: Stb[split][m][q][jo], q = top cos, top sin, bottom cos, bottom sin
-__global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__restrict__ T,
-                                                              const double *__restrict__ proj,
-                                                              double *__restrict__ Stb, int numr,
-                                                              int numz, int mmax)
+__device__ __forceinline__ void contract_topbot_tile(const ContractArgs &a, int bx, int by, int bz)
 {
+    const double *__restrict__ T = a.table;
+    const double *__restrict__ proj = a.proj;
+    double *__restrict__ Stb = a.out;
+    const int numr = a.numr, numz = a.numz, mmax = a.mmax;
     const int nri = numr - 2, nzi = numz - 2;
     const long npts = (long)nri * nzi, plane = (long)numr * numz;
-    const int jo0 = blockIdx.x * CT_TGT, m = blockIdx.y, sp = blockIdx.z;
+    const int jo0 = bx * CT_TGT, m = by, sp = bz;
     const int nt = min(CT_TGT, nri - jo0);
     const long chunk = (npts + CT_SPLIT - 1) / CT_SPLIT;
     const long i0 = sp * chunk, i1 = min(npts, i0 + chunk);
@@ -527,14 +557,15 @@ __global__ __launch_bounds__(256) void contract_topbot_kernel(const double *__re
 }
 
 // Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
-__global__ __launch_bounds__(256) void contract_side_kernel(const double *__restrict__ S,
-                                                            const double *__restrict__ proj,
-                                                            double *__restrict__ Ss, int numr, int numz,
-                                                            int mmax)
+__device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx, int by, int bz)
 {
+    const double *__restrict__ S = a.table;
+    const double *__restrict__ proj = a.proj;
+    double *__restrict__ Ss = a.out;
+    const int numr = a.numr, numz = a.numz, mmax = a.mmax;
     const int nri = numr - 2, nzi = numz - 2;
     const long plane = (long)numr * numz;
-    const int ko0 = blockIdx.x * CT_TGT, m = blockIdx.y, sp = blockIdx.z;
+    const int ko0 = bx * CT_TGT, m = by, sp = bz;
     const int nt = min(CT_TGT, nzi - ko0);
     const int nd = 2 * nzi - 1;
     const double *sm = S + (long)m * nd * nri;
@@ -591,30 +622,30 @@ __global__ __launch_bounds__(256) void contract_side_kernel(const double *__rest
     }
 }
 
-void launch_contract_topbot(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax,
-                            hipStream_t s)
+Op contract_topbot_op(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax)
 {
-    hipLaunchKernelGGL(contract_topbot_kernel, dim3((numr - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT), dim3(256), 0,
-                       s, T, proj, Stb, numr, numz, mmax);
+    ContractArgs a{T, proj, Stb, numr, numz, mmax};
+    return Op::make(OP_CONTRACT_TB, 0, a, (numr - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT);
 }
 
-void launch_contract_side(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax,
-                          hipStream_t s)
+Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax)
 {
-    hipLaunchKernelGGL(contract_side_kernel, dim3((numz - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT), dim3(256), 0,
-                       s, S, proj, Ss, numr, numz, mmax);
+    ContractArgs a{S, proj, Ss, numr, numz, mmax};
+    return Op::make(OP_CONTRACT_SIDE, 0, a, (numz - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT);
 }
 
 // physical faces: factor * (S_C(0) + sum_m 2 S_C(m) cos + 2 S_S(m) sin)
-__global__ void face_rebuild_kernel(const double *__restrict__ Stb, const double *__restrict__ Ss,
-                                    const double *__restrict__ besc, const double *__restrict__ bess,
-                                    double *__restrict__ F, int numr, int numz, int numphi, int mmax,
-                                    double factor)
+__device__ __forceinline__ void face_rebuild_tile(const RebuildArgs &a, int bx, int by, int)
 {
+    const double *__restrict__ Stb = a.Stb;
+    const double *__restrict__ Ss = a.Ss;
+    const double *__restrict__ besc = a.besc;
+    const double *__restrict__ bess = a.bess;
+    const int numr = a.numr, numz = a.numz, numphi = a.numphi, mmax = a.mmax;
     const int nri = numr - 2, nzi = numz - 2;
     const int nf = 2 * nri + nzi;
-    int fi = blockIdx.x * blockDim.x + threadIdx.x;
-    int L = blockIdx.y;
+    int fi = bx * 128 + threadIdx.x;
+    int L = by;
     if (fi >= nf) return;
     const double *sc, *ss;
     long st, sst;  // per-m stride, per-split stride
@@ -639,58 +670,148 @@ __global__ void face_rebuild_kernel(const double *__restrict__ Stb, const double
         double c = besc[(long)m * numphi + L], s = bess[(long)m * numphi + L];
         acc = acc + 2.0 * (S_c(m) * c) + 2.0 * (S_s(m) * s);
     }
-    F[(long)L * nf + fi] = factor * acc;
+    a.F[(long)L * nf + fi] = a.factor * acc;
 }
 
-void launch_face_rebuild(const double *Stb, const double *Ss, const double *besc, const double *bess, double *F,
-                         int numr, int numz, int numphi, int mmax, double factor, hipStream_t s)
+Op face_rebuild_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *F, int numr,
+                   int numz, int numphi, int mmax, double factor)
 {
+    RebuildArgs a{Stb, Ss, besc, bess, F, numr, numz, numphi, mmax, factor};
     int nf = 2 * (numr - 2) + (numz - 2);
-    dim3 grid((nf + 127) / 128, numphi);
-    hipLaunchKernelGGL(face_rebuild_kernel, grid, dim3(128), 0, s, Stb, Ss, besc, bess, F, numr, numz, numphi,
-                       mmax, factor);
+    return Op::make(OP_REBUILD, 0, a, (nf + 127) / 128, numphi, 1);
 }
 
-// Fourier faces into the J- and K-layout potentials; the outer corners of the
-// side face are zero (bessel.F90:428, 456, 487).
-__global__ void face_scatter_kernel(const double *__restrict__ FF, double *__restrict__ XJ,
-                                    double *__restrict__ XK, int numr, int numz)
+// Fourier faces into potential planes: X1 in J-layout, X2 (when given) in
+// K-layout or J-layout.  The outer corners of the side face are zero
+// (bessel.F90:428, 456, 487).
+__device__ __forceinline__ void face_scatter_tile(const ScatterArgs &a, int bx, int by, int)
 {
+    const int numr = a.numr, numz = a.numz;
     const int nri = numr - 2, nzi = numz - 2;
     const int nf = 2 * nri + nzi;
-    const int L = blockIdx.y;
+    const int L = by;
     const long plane = (long)numr * numz;
-    double *xj = XJ + L * plane, *xk = XK + L * plane;
-    const double *ff = FF + (long)L * nf;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nri) {  // top, J = t + 2
-        int J = t + 2;
-        double v = ff[t];
-        xj[(long)(numz - 1) * numr + (J - 1)] = v;
-        xk[(long)(J - 1) * numz + (numz - 1)] = v;
-    } else if (t < 2 * nri) {  // bottom
-        int J = t - nri + 2;
-        double v = ff[t];
-        xj[(J - 1)] = v;
-        xk[(long)(J - 1) * numz] = v;
-    } else if (t < nf) {  // side, K = t - 2 nri + 2
-        int K = t - 2 * nri + 2;
-        double v = ff[t];
-        xj[(long)(K - 1) * numr + (numr - 1)] = v;
-        xk[(long)(numr - 1) * numz + (K - 1)] = v;
+    double *x1 = a.X1 + L * plane;
+    double *x2 = a.X2 ? a.X2 + L * plane : nullptr;
+    const bool k2 = a.x2_klayout != 0;
+    // position of (J, K) (1-based) in a plane of the second target
+    auto at2 = [&](int J, int K) { return k2 ? (long)(J - 1) * numz + (K - 1) : (long)(K - 1) * numr + (J - 1); };
+    const double *ff = a.FF + (long)L * nf;
+    int t = bx * 128 + threadIdx.x;
+    int J = 0, K = 0;
+    double v = 0.0;
+    if (t < nri) { J = t + 2; K = numz; v = ff[t]; }                      // top
+    else if (t < 2 * nri) { J = t - nri + 2; K = 1; v = ff[t]; }          // bottom
+    else if (t < nf) { J = numr; K = t - 2 * nri + 2; v = ff[t]; }        // side
+    if (J) {
+        x1[(long)(K - 1) * numr + (J - 1)] = v;
+        if (x2) x2[at2(J, K)] = v;
     } else if (t == nf) {  // corners
-        xj[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
-        xj[(numr - 1)] = 0.0;
-        xk[(long)(numr - 1) * numz + (numz - 1)] = 0.0;
-        xk[(long)(numr - 1) * numz] = 0.0;
+        x1[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
+        x1[numr - 1] = 0.0;
+        if (x2) {
+            x2[at2(numr, numz)] = 0.0;
+            x2[at2(numr, 1)] = 0.0;
+        }
     }
 }
 
-void launch_face_scatter(const double *FF, double *XJ, double *XK, int numr, int numz, int numphi, hipStream_t s)
+Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int numr, int numz, int numphi)
 {
+    ScatterArgs a{FF, X1, X2, numr, numz, x2_klayout};
     int nf = 2 * (numr - 2) + (numz - 2) + 1;
-    dim3 grid((nf + 127) / 128, numphi);
-    hipLaunchKernelGGL(face_scatter_kernel, grid, dim3(128), 0, s, FF, XJ, XK, numr, numz);
+    return Op::make(OP_SCATTER, 0, a, (nf + 127) / 128, numphi, 1);
+}
+
+// ---------------------------------------------------------------------------
+// single and grouped entry points of the tile functions above
+#define BH_KERNEL_PAIR(NAME, ARGS_T, NT)                                                          \
+    __global__ __launch_bounds__(NT) void NAME##_kernel(ARGS_T a)                                 \
+    {                                                                                             \
+        NAME##_tile(a, blockIdx.x, blockIdx.y, blockIdx.z);                                       \
+    }                                                                                             \
+    __global__ __launch_bounds__(NT) void NAME##_grouped(const ARGS_T *__restrict__ args, GroupSched gs) \
+    {                                                                                             \
+        BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
+    }
+BH_KERNEL_PAIR(fill, FillArgs, 256)
+BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
+BH_KERNEL_PAIR(axis, AxisArgs, 128)
+BH_KERNEL_PAIR(contract_topbot, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
+BH_KERNEL_PAIR(face_rebuild, RebuildArgs, 128)
+BH_KERNEL_PAIR(face_scatter, ScatterArgs, 128)
+#undef BH_KERNEL_PAIR
+
+namespace {
+struct KernelRef {
+    const void *single, *grouped;
+    int threads;
+};
+
+bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
+{
+    *lds = 0;
+#define BH_REF(K_, NAME, NT)                                                                      \
+    case K_:                                                                                      \
+        *k = {(const void *)&NAME##_kernel, (const void *)&NAME##_grouped, NT};                   \
+        return true;
+    switch (kind) {
+        BH_REF(OP_FILL, fill, 256)
+        BH_REF(OP_TRANSPOSE, transpose, 256)
+        BH_REF(OP_AXIS, axis, 128)
+        BH_REF(OP_CONTRACT_TB, contract_topbot, 256)
+        BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
+        BH_REF(OP_REBUILD, face_rebuild, 128)
+        BH_REF(OP_SCATTER, face_scatter, 128)
+    case OP_SWEEP:
+        *lds = sweep_lds(variant);
+        switch (variant) {
+#define BH_SW(V, NL)                                                                              \
+    case V * 100 + NL:                                                                            \
+        *k = {(const void *)&line_sweep_kernel<V, NL>, (const void *)&line_sweep_grouped<V, NL>, 256}; \
+        return true;
+            BH_SW(1, 8) BH_SW(1, 16) BH_SW(2, 8) BH_SW(2, 16) BH_SW(4, 8) BH_SW(4, 16) BH_SW(8, 8) BH_SW(8, 16)
+#undef BH_SW
+        default:
+            return false;
+        }
+    default:
+        return false;
+    }
+#undef BH_REF
+}
+}  // namespace
+
+bool kernels_launch(const Op &op, hipStream_t s)
+{
+    KernelRef k;
+    int lds;
+    if (!kernel_ref(op.kind, op.variant, &k, &lds)) return false;
+    void *argv[] = {(void *)op.args};
+    (void)hipLaunchKernel(k.single, dim3(op.nx, op.ny, op.nz), dim3(k.threads), argv, lds, s);
+    return true;
+}
+
+bool kernels_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
+{
+    KernelRef k;
+    int lds;
+    if (!kernel_ref(kind, variant, &k, &lds)) return false;
+    GroupSched g = gs;
+    const void *a = d_args;
+    void *argv[] = {(void *)&a, (void *)&g};
+    (void)hipLaunchKernel(k.grouped, dim3(nblocks), dim3(k.threads), argv, lds, s);
+    return true;
+}
+
+bool kernels_occupancy(int kind, int variant, int *blocks)
+{
+    KernelRef k;
+    int lds;
+    if (!kernel_ref(kind, variant, &k, &lds)) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k.grouped, k.threads, lds) != hipSuccess) *blocks = 0;
+    return true;
 }
 
 }  // namespace bh

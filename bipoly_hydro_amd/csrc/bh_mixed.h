@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "bh_ops.h"
+
 namespace bh {
 
 struct ResidualArgs {
@@ -11,11 +13,23 @@ struct ResidualArgs {
     double gamma, four_pi;
     int numr, numz;
 };
-void launch_residual64(const ResidualArgs &a, int nplanes, hipStream_t s);
-void launch_add_update(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes,
-                       hipStream_t s);
-void launch_tridiag_shift(const float *cin, float *yout, const float *ibet, const float *gam,
-                          const double *betar, int nri, int nzi, int nplanes, hipStream_t s);
+struct UpdateArgs {
+    const double *X;
+    const float *dlt;
+    double *Xo;
+    int numr, numz;
+};
+struct TridiagArgs {
+    const float *cin;
+    float *yout;
+    const float *ibet, *gam;
+    const double *betar;
+    int nri, nzi;
+};
+Op residual_op(const ResidualArgs &a, int nplanes);
+Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes);
+Op tridiag_shift_op(const float *cin, float *yout, const float *ibet, const float *gam, const double *betar, int nri,
+                    int nzi, int nplanes);
 
 struct MrArgs {
     const double *brb, *alphar, *betar, *dt;
@@ -25,6 +39,5 @@ struct MrArgs {
 };
 void launch_build_mr(const MrArgs &a, hipStream_t s);
 void launch_fillf(float *p, size_t n, float v, hipStream_t s);
-void launch_face_scatter_j(const double *FF, double *X, int numr, int numz, int numphi, hipStream_t s);
 
 }  // namespace bh

@@ -17,6 +17,7 @@
 //   build_mr        M_r for one pseudo-timestep schedule (plan time), FP64
 #include "bh_internal.h"
 #include "bh_mixed.h"
+#include "bh_ops.h"
 
 #include <hip/hip_runtime.h>
 
@@ -31,12 +32,12 @@ namespace bh {
 // (numr x numz) planes in J-layout; dlt and w have zero faces.  r is written
 // compact: r[(K-2)*nri + (J-2)].
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void residual64_kernel(ResidualArgs a)
+__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int)
 {
     const int nri = a.numr - 2, nzi = a.numz - 2;
-    const int L = blockIdx.y;
+    const int L = by;
     const long plane = (long)a.numr * a.numz;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    int t = bx * 256 + threadIdx.x;
     if (t >= nri * nzi) return;
     int jr = t % nri, kz = t / nri;
     int J = jr + 2, K = kz + 2;
@@ -54,32 +55,31 @@ __global__ __launch_bounds__(256) void residual64_kernel(ResidualArgs a)
     if (a.r) a.r[(long)L * nri * nzi + (long)kz * nri + jr] = (float)r;
 }
 
-void launch_residual64(const ResidualArgs &a, int nplanes, hipStream_t s)
+Op residual_op(const ResidualArgs &a, int nplanes)
 {
     int n = (a.numr - 2) * (a.numz - 2);
-    dim3 grid((n + 255) / 256, nplanes);
-    hipLaunchKernelGGL(residual64_kernel, grid, dim3(256), 0, s, a);
+    return Op::make(OP_RESIDUAL, 0, a, (n + 255) / 256, nplanes, 1);
 }
 
 // Xo = X + dlt on the interior (faces of Xo are kept)
-__global__ void add_update_kernel(const double *X, const float *dlt, double *Xo, int numr, int numz)
+__device__ __forceinline__ void add_update_tile(const UpdateArgs &a, int bx, int by, int)
 {
+    const int numr = a.numr, numz = a.numz;
     const long plane = (long)numr * numz;
-    const int L = blockIdx.y;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int L = by;
+    int t = bx * 256 + threadIdx.x;
     int nri = numr - 2, nzi = numz - 2;
     if (t >= nri * nzi) return;
     int jr = t % nri, kz = t / nri;
     long c = L * plane + (long)(kz + 1) * numr + (jr + 1);
-    Xo[c] = X[c] + (double)dlt[c];
+    a.Xo[c] = a.X[c] + (double)a.dlt[c];
 }
 
-void launch_add_update(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes,
-                       hipStream_t s)
+Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes)
 {
+    UpdateArgs a{X, dlt, Xo, numr, numz};
     int n = (numr - 2) * (numz - 2);
-    dim3 grid((n + 255) / 256, nplanes);
-    hipLaunchKernelGGL(add_update_kernel, grid, dim3(256), 0, s, X, dlt, Xo, numr, numz);
+    return Op::make(OP_UPDATE, 0, a, (n + 255) / 256, nplanes, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -91,13 +91,16 @@ void launch_add_update(const double *X, const float *dlt, double *Xo, int numr, 
 // ---------------------------------------------------------------------------
 constexpr int TS = 16;
 
-__global__ __launch_bounds__(256) void tridiag_shift_kernel(const float *__restrict__ cin, float *__restrict__ yout,
-                                                            const float *__restrict__ ibet,
-                                                            const float *__restrict__ gam,
-                                                            const double *__restrict__ betar, int nri, int nzi)
+__device__ __forceinline__ void tridiag_shift_tile(const TridiagArgs &a, int bx, int by, int)
 {
-    const int L = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const float *__restrict__ cin = a.cin;
+    float *__restrict__ yout = a.yout;
+    const float *__restrict__ ibet = a.ibet;
+    const float *__restrict__ gam = a.gam;
+    const double *__restrict__ betar = a.betar;
+    const int nri = a.nri, nzi = a.nzi;
+    const int L = by;
+    const int k = bx * 256 + threadIdx.x;
     if (k >= nzi) return;
     const long base = (long)L * nri * nzi + k;
     // forward: y_J = (c_J - a_J y_{J-1}) / bet_J, a_J = betar(J); a_2 y_1 absent
@@ -143,11 +146,11 @@ __global__ __launch_bounds__(256) void tridiag_shift_kernel(const float *__restr
     }
 }
 
-void launch_tridiag_shift(const float *cin, float *yout, const float *ibet, const float *gam,
-                          const double *betar, int nri, int nzi, int nplanes, hipStream_t s)
+Op tridiag_shift_op(const float *cin, float *yout, const float *ibet, const float *gam, const double *betar, int nri,
+                    int nzi, int nplanes)
 {
-    dim3 grid((nzi + 255) / 256, nplanes);
-    hipLaunchKernelGGL(tridiag_shift_kernel, grid, dim3(256), 0, s, cin, yout, ibet, gam, betar, nri, nzi);
+    TridiagArgs a{cin, yout, ibet, gam, betar, nri, nzi};
+    return Op::make(OP_TRIDIAG, 0, a, (nzi + 255) / 256, nplanes, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -216,31 +219,63 @@ void launch_fillf(float *p, size_t n, float v, hipStream_t s)
     hipLaunchKernelGGL(fillf_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
 }
 
-// Face values into a J-layout potential (top K = numz, bottom K = 1, side
-// J = numr; outer corners zero, bessel.F90:428, 456, 487)
-__global__ void face_scatter_j_kernel(const double *__restrict__ FF, double *__restrict__ X, int numr, int numz)
+// single and grouped entry points of the tile functions above
+#define BH_KERNEL_PAIR(NAME, ARGS_T, NT)                                                          \
+    __global__ __launch_bounds__(NT) void NAME##_kernel(ARGS_T a)                                 \
+    {                                                                                             \
+        NAME##_tile(a, blockIdx.x, blockIdx.y, blockIdx.z);                                       \
+    }                                                                                             \
+    __global__ __launch_bounds__(NT) void NAME##_grouped(const ARGS_T *__restrict__ args, GroupSched gs) \
+    {                                                                                             \
+        BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
+    }
+BH_KERNEL_PAIR(residual, ResidualArgs, 256)
+BH_KERNEL_PAIR(add_update, UpdateArgs, 256)
+BH_KERNEL_PAIR(tridiag_shift, TridiagArgs, 256)
+#undef BH_KERNEL_PAIR
+
+namespace {
+bool mixed_ref(int kind, const void **single, const void **grouped)
 {
-    const int nri = numr - 2, nzi = numz - 2;
-    const int nf = 2 * nri + nzi;
-    const int L = blockIdx.y;
-    const long plane = (long)numr * numz;
-    double *x = X + L * plane;
-    const double *ff = FF + (long)L * nf;
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nri) x[(long)(numz - 1) * numr + t + 1] = ff[t];
-    else if (t < 2 * nri) x[t - nri + 1] = ff[t];
-    else if (t < nf) x[(long)(t - 2 * nri + 1) * numr + (numr - 1)] = ff[t];
-    else if (t == nf) {
-        x[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
-        x[numr - 1] = 0.0;
+    switch (kind) {
+    case OP_RESIDUAL: *single = (const void *)&residual_kernel; *grouped = (const void *)&residual_grouped; return true;
+    case OP_UPDATE: *single = (const void *)&add_update_kernel; *grouped = (const void *)&add_update_grouped; return true;
+    case OP_TRIDIAG:
+        *single = (const void *)&tridiag_shift_kernel;
+        *grouped = (const void *)&tridiag_shift_grouped;
+        return true;
+    default:
+        return false;
     }
 }
+}  // namespace
 
-void launch_face_scatter_j(const double *FF, double *X, int numr, int numz, int numphi, hipStream_t s)
+bool mixed_launch(const Op &op, hipStream_t s)
 {
-    int nf = 2 * (numr - 2) + (numz - 2) + 1;
-    dim3 grid((nf + 127) / 128, numphi);
-    hipLaunchKernelGGL(face_scatter_j_kernel, grid, dim3(128), 0, s, FF, X, numr, numz);
+    const void *k1, *k2;
+    if (!mixed_ref(op.kind, &k1, &k2)) return false;
+    void *argv[] = {(void *)op.args};
+    (void)hipLaunchKernel(k1, dim3(op.nx, op.ny, op.nz), dim3(256), argv, 0, s);
+    return true;
+}
+
+bool mixed_launch_grouped(int kind, int, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
+{
+    const void *k1, *k2;
+    if (!mixed_ref(kind, &k1, &k2)) return false;
+    GroupSched g = gs;
+    const void *a = d_args;
+    void *argv[] = {(void *)&a, (void *)&g};
+    (void)hipLaunchKernel(k2, dim3(nblocks), dim3(256), argv, 0, s);
+    return true;
+}
+
+bool mixed_occupancy(int kind, int, int *blocks)
+{
+    const void *k1, *k2;
+    if (!mixed_ref(kind, &k1, &k2)) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k2, 256, 0) != hipSuccess) *blocks = 0;
+    return true;
 }
 
 }  // namespace bh

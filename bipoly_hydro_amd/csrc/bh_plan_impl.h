@@ -1,0 +1,137 @@
+// bh_plan_impl.h -- plan internals shared by the single-grid solve, the
+// batch executor and the serving loop.
+#pragma once
+#include "../../include/bh_potential.h"
+#include "bh_internal.h"
+#include "bh_ops.h"
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace bh {
+
+// Error reporting for the C ABI: the message of the last failure on this thread.
+int fail(int code, const std::string &msg);
+const char *last_error();
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::bh::fail(BH_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device allocations owned by one object, freed with it.
+struct DevMem {
+    std::vector<void *> bufs;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem &) = delete;
+    DevMem &operator=(const DevMem &) = delete;
+    ~DevMem();
+    int alloc_bytes(void **out, size_t nbytes);
+    template <class T>
+    int alloc(T **out, size_t n)
+    {
+        void *p = nullptr;
+        int rc = alloc_bytes(&p, n * sizeof(T));
+        *out = (T *)p;
+        return rc;
+    }
+};
+
+// Host-side geometry and coefficients of one grid (setup_frac.F90:374-463,
+// potsetup.F90:113-252, helmadi.F90:124-129).
+struct HostGeo {
+    bh_grid_desc d;
+    Geo g;
+    std::vector<double> r_g, rhf_g, rinv_g, rhfinv_g, zhf_g;  // 1-based
+    std::vector<double> alphar, betar;                         // [numr + 1]
+    std::vector<double> brb, elzb;                             // [numphi][numr + 1]
+    std::vector<double> dt[2];                                 // cold, warm schedules
+};
+// Fills defaults (zeros take the reference's values) and validates.
+int normalize_desc(const bh_grid_desc *in, int precision, bh_grid_desc *out);
+void host_geometry(const bh_grid_desc &d, HostGeo *h);
+
+// Green's tables of one (numr, numz, dr, mmax) geometry on one device
+// (tm.F90, sm.F90, bm.F90): shared by every plan of that geometry.
+struct GeoTables {
+    DevMem mem;
+    double *T = nullptr;  // top (and mirrored bottom) table [m][jo][kz][jr]
+    double *S = nullptr;  // side table, Toeplitz in z [m][d][jr]
+};
+
+// Transform matrices and ADI operator data of one (grid, schedule,
+// precision) on one device: shared by every plan of that configuration.
+struct ModeOps {
+    DevMem mem;
+    Geo g;
+    std::vector<double> dt[2];
+    double *Dc = nullptr, *Ds = nullptr, *Ec = nullptr, *Es = nullptr, *Pm = nullptr, *besc = nullptr, *bess = nullptr;
+    double *alphar = nullptr, *betar = nullptr, *elzb = nullptr;
+    // FP64 Thomas factors per pseudo-timestep
+    double *rc_alpha[2] = {}, *rc_ibet[2] = {}, *rc_gback[2] = {};
+    double *zc_alpha[2] = {}, *zc_ibet[2] = {}, *zc_gback[2] = {};
+    // mixed precision
+    float *Dc32 = nullptr, *Ds32 = nullptr, *S32 = nullptr, *mu32[2] = {}, *Mr[2] = {};
+    float *tri_ibet = nullptr, *tri_gam = nullptr;
+    double *brb_d = nullptr;
+};
+
+// Shared resources from the per-process cache (created on first use; kept
+// while any plan holds them).  `created` reports whether this call built them.
+int get_tables(const HostGeo &h, int device, std::shared_ptr<const GeoTables> *out, bool *created);
+int get_mode_ops(const HostGeo &h, int precision, int device, std::shared_ptr<const ModeOps> *out, bool *created);
+// Bytes of device memory a plan would add: its own buffers, and the shared
+// resources not yet in the cache.
+size_t plan_private_bytes(const bh_grid_desc &d, int precision);
+size_t plan_shared_bytes(const bh_grid_desc &d, int precision);
+size_t plan_new_shared_bytes(const bh_grid_desc &d, int precision, int device);
+
+struct TimingSet {
+    hipEvent_t ev[7];
+};
+
+}  // namespace bh
+
+struct bh_plan {
+    bh_grid_desc d;
+    int precision = BH_PREC_FP64;
+    size_t budget = 0;
+    bh::Geo g;
+    int device = 0;
+    std::shared_ptr<const bh::GeoTables> tab;
+    std::shared_ptr<const bh::ModeOps> ops;
+    bh::DevMem mem;  // this plan's own buffers
+    size_t bytes = 0;  // own + shared
+    // per-solve state
+    double *RJ = nullptr, *RK = nullptr, *XJ = nullptr, *XK = nullptr, *proj = nullptr;
+    double *Stb = nullptr, *Ss = nullptr, *F = nullptr, *FF = nullptr;
+    float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
+    float *D1 = nullptr, *Wf = nullptr;
+    double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
+    int passes = 1;                                // FP64 refinement passes (mixed)
+    int tile_dft = 2, tile_spec = 3;               // GEMM tile shapes, measured best
+    bool has_solution = false;
+    // timing (HIP events at the stage marks of each timed solve)
+    bool timing = false;
+    std::vector<bh::TimingSet> tsets;
+    int tcount = 0;
+    double last_ms[6] = {0, 0, 0, 0, 0, 0};
+    bool have_timing = false;
+};
+
+namespace bh {
+// Create a plan (validated descriptor).  Set-up work runs on its own stream.
+int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **out);
+// The solve of one grid as Ops, with stage marks 0-4 (start, after the
+// forward DFT, after the boundary, after the ADI, end) and 5-6 around the
+// top/bottom boundary contraction.
+void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out);
+// Launch the Ops of one solve on s (records timing events when enabled).
+int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s);
+}  // namespace bh

@@ -150,8 +150,10 @@ class PotentialPlan:
         _check(load_library().bh_plan_set_timing(self._h, 1 if on else 0))
 
     def last_timings(self) -> dict:
+        """Mean stage times (ms, HIP events on the solve stream) over the
+        solves timed since timing was switched on or last read."""
         L = load_library()
         _check(L.bh_sync_timing(self._h))
-        ms = (ctypes.c_double * 5)()
-        _check(L.bh_last_timings(self._h, ms, 5))
-        return dict(zip(("boundary", "forward_dft", "adi", "inverse_dft", "total"), list(ms)))
+        ms = (ctypes.c_double * 6)()
+        _check(L.bh_last_timings(self._h, ms, 6))
+        return dict(zip(("boundary", "forward_dft", "adi", "inverse_dft", "total", "contract_topbot"), list(ms)))

@@ -84,17 +84,67 @@ int bh_set_potential(bh_plan *plan, const double *pot, int on_device);
 /* Device memory the plan holds, in bytes. */
 size_t bh_plan_device_bytes(const bh_plan *plan);
 
-/* Stage timings of the last solve in milliseconds (HIP events), written to
- * ms[0..n-1] in the order: boundary, forward DFT, ADI, inverse DFT, total.
- * Timing is recorded only after bh_plan_set_timing(plan, 1). */
+/* Stage timings in milliseconds (HIP events on the solve's stream), written
+ * to ms[0..n-1] in the order: boundary, forward DFT, ADI, inverse DFT,
+ * total, and the top/bottom boundary contraction kernel alone.  They are the
+ * mean over the solves timed since bh_plan_set_timing(plan, 1) or since the
+ * last bh_sync_timing; timing is recorded only while switched on. */
 int bh_plan_set_timing(bh_plan *plan, int on);
 int bh_last_timings(const bh_plan *plan, double *ms, int n);
-/* Wait for the last timed solve enqueued by bh_solve_device and read its
- * stage timings (bh_solve does this by itself). */
+/* Wait for the timed solves enqueued by bh_solve_device and average their
+ * stage timings (bh_solve does this by itself after each solve). */
 int bh_sync_timing(bh_plan *plan);
 
 /* Message of the last failure on this thread ("" if none). */
 const char *bh_last_error(void);
+
+/* ---- Batches of independent grids --------------------------------------
+ * No counterpart in the reference, which solves one grid per run.  A batch
+ * holds one plan per grid (grids may differ in size; plans of equal size
+ * share their Green's tables and transform matrices) and solves all grids
+ * together: step k of every grid's solve runs in one persistent grouped
+ * launch per kernel kind, over the tiles of all grids. */
+typedef struct bh_batch bh_batch;
+
+#define BH_BATCH_GRAPH 1    /* capture the launch sequence as a hipGraph and replay it */
+#define BH_BATCH_AUTOTUNE 2 /* time the tile and pencil shape candidates over the first
+                               5 solves (one candidate per solve), then keep the fastest
+                               per step */
+
+/* Create a batch of ngrids grids (descs[i] as for bh_plan_create).
+ * hbm_budget = 0 means no limit; otherwise creation fails with BH_E_NOMEM
+ * when the batch would need more device memory than that. */
+int bh_batch_create(const bh_grid_desc *descs, int ngrids, int precision, size_t hbm_budget, int flags,
+                    bh_batch **out);
+int bh_batch_destroy(bh_batch *batch);
+
+/* One solve of every grid: d_rho[i], d_pot[i] are grid i's device arrays,
+ * tstep[i] its cold/warm choice as in bh_solve (tstep = NULL: all cold).
+ * Enqueued on `stream`; returns once enqueued (autotuning solves wait for
+ * their own timing).  Each grid keeps its potential for warm starts. */
+int bh_batch_solve_device(bh_batch *batch, const double *const *d_rho, double *const *d_pot, const int *tstep,
+                          void *stream);
+/* The same from host arrays; synchronous. */
+int bh_batch_solve(bh_batch *batch, const double *const *rho, double *const *pot, const int *tstep);
+
+size_t bh_batch_device_bytes(const bh_batch *batch);
+
+typedef struct bh_batch_stats {
+    int ngrids;
+    int launches;    /* grouped launches per solve */
+    int graph;       /* 1 when solves replay a captured hipGraph */
+    int tuning_left; /* autotuning solves still to run */
+    double last_ms;  /* device time of the last timed solve (ms) */
+} bh_batch_stats;
+/* Statistics; waits for the last timed solve. */
+int bh_batch_get_stats(bh_batch *batch, bh_batch_stats *stats);
+int bh_batch_set_timing(bh_batch *batch, int on);
+/* Tile shape and sweep lines per workgroup chosen for each step of the
+ * solve (-1: default); writes up to n entries, returns the step count. */
+int bh_batch_tuning(const bh_batch *batch, int *tile, int *nl, int n);
+/* Grid i's plan (owned by the batch: do not destroy), e.g. for
+ * bh_set_potential. */
+int bh_batch_plan(bh_batch *batch, int i, bh_plan **plan);
 
 #ifdef __cplusplus
 }
