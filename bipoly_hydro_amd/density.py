@@ -103,3 +103,16 @@ def scaled_binary(numr: int, numz: int, numphi: int, seed: int, *, mass_ratio=No
     x2 = max(x2, -(lim - r2))
     return lane_emden_n1_binary(numr, numz, numphi, rho_c=(1.0, 1.0), radius_cells=(r1, r2),
                                 centre_cells=(x1, x2), noise=1e-6, seed=seed, dr=dr)
+
+
+def batched_grids(n: int = 64, seed: int = 4242):
+    """Grid sizes of the batched workload: numr in {66, 130, 194, 258}, numz in
+    {34, 66, 130}, numphi in {64, 128, 256, 512}, drawn by seed; returns
+    [(numr, numz, numphi, density_seed)]."""
+    u = splitmix64_uniform(seed, 3 * n)
+    nr, nz, npf = (66, 130, 194, 258), (34, 66, 130), (64, 128, 256, 512)
+    out = []
+    for i in range(n):
+        out.append((nr[min(3, int(u[3 * i] * 4))], nz[min(2, int(u[3 * i + 1] * 3))],
+                    npf[min(3, int(u[3 * i + 2] * 4))], seed * 1000 + i))
+    return out

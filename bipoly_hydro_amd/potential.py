@@ -31,7 +31,13 @@ _PREC = {"fp64": BH_PREC_FP64, "mixed": BH_PREC_MIXED}
 
 EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_set_potential", "bh_plan_device_bytes", "bh_plan_set_timing",
-           "bh_last_timings", "bh_sync_timing", "bh_last_error")
+           "bh_last_timings", "bh_sync_timing", "bh_last_error",
+           "bh_batch_create", "bh_batch_destroy", "bh_batch_solve_device", "bh_batch_solve",
+           "bh_batch_device_bytes", "bh_batch_get_stats", "bh_batch_set_timing", "bh_batch_tuning",
+           "bh_batch_plan")
+
+BH_BATCH_GRAPH = 1
+BH_BATCH_AUTOTUNE = 2
 
 
 class BHError(RuntimeError):
@@ -44,6 +50,11 @@ class GridDesc(ctypes.Structure):
     _fields_ = [("numr", ctypes.c_int), ("numz", ctypes.c_int), ("numphi", ctypes.c_int),
                 ("isym", ctypes.c_int), ("dr", ctypes.c_double), ("mmax", ctypes.c_int),
                 ("nsteps_cold", ctypes.c_int), ("nsteps_warm", ctypes.c_int)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("ngrids", ctypes.c_int), ("launches", ctypes.c_int), ("graph", ctypes.c_int),
+                ("tuning_left", ctypes.c_int), ("last_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -78,6 +89,19 @@ def load_library(path: str = LIB_PATH):
     L.bh_last_timings.argtypes = [vp, dp, ctypes.c_int]
     L.bh_sync_timing.argtypes = [vp]
     L.bh_last_error.restype = ctypes.c_char_p
+    pp = ctypes.POINTER(vp)
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.bh_batch_create.argtypes = [ctypes.POINTER(GridDesc), ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                  ctypes.c_int, pp]
+    L.bh_batch_destroy.argtypes = [vp]
+    L.bh_batch_solve_device.argtypes = [vp, pp, pp, ip, vp]
+    L.bh_batch_solve.argtypes = [vp, pp, pp, ip]
+    L.bh_batch_device_bytes.argtypes = [vp]
+    L.bh_batch_device_bytes.restype = ctypes.c_size_t
+    L.bh_batch_get_stats.argtypes = [vp, ctypes.POINTER(BatchStats)]
+    L.bh_batch_set_timing.argtypes = [vp, ctypes.c_int]
+    L.bh_batch_tuning.argtypes = [vp, ip, ip, ctypes.c_int]
+    L.bh_batch_plan.argtypes = [vp, ctypes.c_int, pp]
     _lib = L
     return L
 
@@ -157,3 +181,86 @@ class PotentialPlan:
         ms = (ctypes.c_double * 6)()
         _check(L.bh_last_timings(self._h, ms, 6))
         return dict(zip(("boundary", "forward_dft", "adi", "inverse_dft", "total", "contract_topbot"), list(ms)))
+
+
+class PotentialBatch:
+    """Independent grids of different sizes solved together (one plan per
+    grid; step k of every grid's solve runs as one grouped launch per kernel
+    kind).  ``graph`` replays the launch sequence from a captured hipGraph;
+    ``autotune`` times tile and pencil candidates over the first five solves
+    and keeps the fastest per step."""
+
+    def __init__(self, grids, *, precision: str = "mixed", hbm_budget: int = 0, graph: bool = False,
+                 autotune: bool = False, dr: float = 1.0 / 127.0, mmax: int = 30,
+                 nsteps_cold: int = 20, nsteps_warm: int = 5):
+        L = load_library()
+        self.shapes = [(p, z, r) for (r, z, p) in grids]
+        n = len(grids)
+        descs = (GridDesc * n)(*[GridDesc(r, z, p, 1, dr, mmax, nsteps_cold, nsteps_warm) for (r, z, p) in grids])
+        flags = (BH_BATCH_GRAPH if graph else 0) | (BH_BATCH_AUTOTUNE if autotune else 0)
+        h = ctypes.c_void_p()
+        _check(L.bh_batch_create(descs, n, _PREC[precision], hbm_budget, flags, ctypes.byref(h)))
+        self._h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load_library().bh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def device_bytes(self) -> int:
+        return load_library().bh_batch_device_bytes(self._h)
+
+    @staticmethod
+    def _ptrs(ptrs):
+        return (ctypes.c_void_p * len(ptrs))(*[ctypes.c_void_p(int(x)) for x in ptrs])
+
+    def _tsteps(self, tstep):
+        if isinstance(tstep, int):
+            tstep = [tstep] * self.n
+        return (ctypes.c_int * self.n)(*[int(t) for t in tstep])
+
+    def solve(self, rhos, tstep=0):
+        """Host arrays in, host potentials out (synchronous)."""
+        rhos = [np.ascontiguousarray(r, dtype=np.float64) for r in rhos]
+        for r, shp in zip(rhos, self.shapes):
+            if r.shape != shp:
+                raise ValueError(f"rho shape {r.shape} != {shp}")
+        pots = [np.empty(s, dtype=np.float64) for s in self.shapes]
+        _check(load_library().bh_batch_solve(self._h, self._ptrs([r.ctypes.data for r in rhos]),
+                                             self._ptrs([p.ctypes.data for p in pots]), self._tsteps(tstep)))
+        return pots
+
+    def solve_device(self, rho_ptrs, pot_ptrs, tstep=0, stream: int = 0):
+        _check(load_library().bh_batch_solve_device(self._h, self._ptrs(rho_ptrs), self._ptrs(pot_ptrs),
+                                                    self._tsteps(tstep), ctypes.c_void_p(stream)))
+
+    def set_timing(self, on: bool = True):
+        _check(load_library().bh_batch_set_timing(self._h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        st = BatchStats()
+        _check(load_library().bh_batch_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+    def tuning(self):
+        L = load_library()
+        m = L.bh_batch_tuning(self._h, None, None, 0)
+        if m < 0:
+            _check(m)
+        tile = (ctypes.c_int * max(m, 1))()
+        nl = (ctypes.c_int * max(m, 1))()
+        L.bh_batch_tuning(self._h, tile, nl, m)
+        return list(tile)[:m], list(nl)[:m]
