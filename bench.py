@@ -66,6 +66,11 @@ def main():
     ap.add_argument("--warm", action="store_true", help="time warm solves (tstep > 1, 5 ADI steps)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batched", action="store_true",
+                    help="instead of the headline: 64 uneven grids solved as one batch (grouped "
+                         "persistent launches, hipGraph, autotuned tiles), against the same grids "
+                         "solved one plan at a time")
+    ap.add_argument("--ngrids", type=int, default=64)
     ap.add_argument("--calib-copy", action="store_true",
                     help="first copy 256 MiB device-to-device three times (a known-size kernel for "
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
@@ -78,6 +83,8 @@ def main():
     from bipoly_hydro_amd import density
     from bipoly_hydro_amd import potential as P
 
+    if args.batched:
+        return run_batched(args)
     numr, numz, numphi = args.grid
     dev = torch.device("cuda:0")
     if args.calib_copy:
@@ -102,16 +109,14 @@ def main():
         plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
         torch.cuda.synchronize()
         pot_cold = d_pot.cpu().numpy()
-        # stage timings of one solve (HIP events on the solve's stream)
-        plan.set_timing(True)
-        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
-        stages = plan.last_timings()
-        plan.set_timing(False)
         for _ in range(args.warmup):
             plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
         torch.cuda.synchronize()
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
               for _ in range(args.steps)]
+        # stage and kernel times: HIP events recorded by the library on the
+        # solve's stream during the timed solves, averaged over them
+        plan.set_timing(True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for a, b in ev:
@@ -120,6 +125,8 @@ def main():
             b.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        stages = plan.last_timings()
+        plan.set_timing(False)
         lat = np.array([a.elapsed_time(b) for a, b in ev])
         plan.close()
         return {"ms_per_step": wall * 1e3 / args.steps, "solves_per_s": args.steps / wall,
@@ -131,15 +138,24 @@ def main():
     mixed["relL2_vs_fp64_gpu"] = rell2(potmx, pot64)
     ms_per_step = mixed["ms_per_step"]
     lat_p50, lat_p99 = mixed["p50_ms"], mixed["p99_ms"]
-    stages = fp64["stage_ms"]
 
-    # ADI half-step kernel of the FP64 path: algorithmic bytes = 3 streams
-    # (read phi, read rho-hat, write phi) x 8 B x interior points x modes
+    # Dominant kernel of the measured (mixed) solve: the top/bottom boundary
+    # contraction, bound by reading the FP64 Green's table once:
+    # mmax x (numr-2)^2 x (numz-2) x 8 B per launch (one launch per solve).
+    nri, nzi, mmax = numr - 2, numz - 2, 30
+    ct_bytes = mmax * nri * nri * nzi * 8
+    ct_ms = mixed["stage_ms"]["contract_topbot"]
+    # HBM traffic of that launch at 258x130x512 from the FETCH_SIZE / WRITE_SIZE
+    # counters (profiles/potential_solver_258x130x512_hbm_traffic.md):
+    # FETCH 1118997 KiB x 2.000 (copy-kernel calibration) + WRITE 960 KiB
+    ct_traffic = (1118997 * 2.0 + 960) * 1024 if (numr, numz, numphi) == (258, 130, 512) else None
+    # ADI half-step kernel of the FP64 path: 3 streams (read phi, read
+    # rho-hat, write phi) x 8 B x interior points x modes per launch
     nsteps = 5 if args.warm else 20
     interior = (numr - 2) * (numz - 2) * numphi
-    bytes_per_launch = 3 * 8 * interior
-    launch_ms = stages["adi"] / (2 * nsteps)
-    achieved = bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    sw_bytes = 3 * 8 * interior
+    sw_ms = fp64["stage_ms"]["adi"] / (2 * nsteps)
+    sw_traffic = (178004 * 2.0 + 163840) * 1024 if (numr, numz, numphi) == (258, 130, 512) else None
 
     cpu = None
     if not args.no_cpu:
@@ -168,14 +184,92 @@ def main():
         "p99_ms": lat_p99,
         "relL2_vs_reference_slices": mixed["relL2_vs_reference_slices"],
         "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
-        "roofline": {"bound": "hbm", "kernel": "line_sweep_kernel (ADI half-step)",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bytes_per_launch, "launch_ms": launch_ms},
+        "roofline": {"bound": "hbm", "kernel": "contract_topbot (boundary Green's-table contraction)",
+                     "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,
+                     "bytes_per_launch": ct_bytes, "launch_ms": ct_ms},
+        "roofline_fp64_sweep": {"bound": "hbm", "kernel": "line_sweep (FP64 ADI half-step)",
+                                "achieved": sw_bytes / (sw_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": sw_bytes / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                "traffic": sw_traffic, "bytes_per_launch": sw_bytes, "launch_ms": sw_ms},
         "cpu_baseline": cpu,
         "mixed": mixed,
         "fp64": fp64,
     }
+    print(json.dumps(out), flush=True)
+
+
+def run_batched(args):
+    """64 independent grids of seeded uneven sizes, mixed precision, solved
+    as one batch; a step is one solve of every grid (cold, 20 ADI steps)."""
+    import ctypes
+    import torch
+    from bipoly_hydro_amd import density
+    from bipoly_hydro_amd import potential as P
+
+    dev = torch.device("cuda:0")
+    grids = density.batched_grids(args.ngrids)
+    print(f"batched: {len(grids)} grids, {sum(r * z * p for r, z, p, _ in grids) / 1e6:.1f} M points",
+          file=sys.stderr, flush=True)
+    d_rho = [torch.from_numpy(density.scaled_binary(r, z, p, seed=s)).to(dev) for r, z, p, s in grids]
+    d_pot = [torch.empty_like(x) for x in d_rho]
+    rp, pp = [x.data_ptr() for x in d_rho], [x.data_ptr() for x in d_pot]
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    t0 = time.perf_counter()
+    b = P.PotentialBatch([(r, z, p) for r, z, p, _ in grids], precision="mixed", graph=True, autotune=True)
+    setup_s = time.perf_counter() - t0
+    for _ in range(5 + args.warmup):  # five autotuning solves, then warm-up of the tuned graph
+        b.solve_device(rp, pp, 0, sp)
+    torch.cuda.synchronize()
+    ref = [x.cpu() for x in d_pot]
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for a, c in ev:
+        a.record(stream)
+        b.solve_device(rp, pp, 0, sp)
+        c.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    lat = np.array([a.elapsed_time(c) for a, c in ev])
+    st = b.stats()
+    same = all(torch.equal(x.cpu(), y) for x, y in zip(d_pot, ref))
+    # the same grids one plan at a time, each launch on its own (no grouping)
+    L = P.load_library()
+    plans = []
+    for i in range(len(grids)):
+        h = ctypes.c_void_p()
+        P._check(L.bh_batch_plan(b._h, i, ctypes.byref(h)))
+        plans.append(h)
+    for h, r, q in zip(plans, rp, pp):
+        P._check(L.bh_solve_device(h, ctypes.c_void_p(r), ctypes.c_void_p(q), 0, ctypes.c_void_p(sp)))
+    torch.cuda.synchronize()
+    same_single = all(torch.equal(x.cpu(), y) for x, y in zip(d_pot, ref))
+    nseq = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(nseq):
+        for h, r, q in zip(plans, rp, pp):
+            L.bh_solve_device(h, ctypes.c_void_p(r), ctypes.c_void_p(q), 0, ctypes.c_void_p(sp))
+    torch.cuda.synchronize()
+    seq_ms = (time.perf_counter() - t0) * 1e3 / nseq
+    ms = wall * 1e3 / args.steps
+    out = {
+        "metric": "batched uneven Poisson solves/s (grids solved per second), mixed precision",
+        "value": len(grids) * 1e3 / ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"{len(grids)} independent grids, sizes drawn by seed from numr in "
+                               "{66,130,194,258}, numz in {34,66,130}, numphi in {64,128,256,512}; "
+                               "n=1 polytrope binaries, q in [0.3,1], separation in [0.4,0.8] x grid "
+                               "radius; cold solves", "grids": [g[:3] for g in grids]},
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "grouped_launches_per_step": st["launches"], "graph": st["graph"],
+        "tuned_tiles": b.tuning()[0], "setup_s": setup_s,
+        "one_plan_at_a_time_ms": seq_ms, "batch_speedup": seq_ms / ms,
+        "batch_equals_single_plans_bitwise": bool(same and same_single),
+        "device_bytes": b.device_bytes(),
+    }
+    b.close()
     print(json.dumps(out), flush=True)
 
 

@@ -9,7 +9,8 @@ void launch_op(const Op &op, hipStream_t s)
     if (op.kind == OP_MARK) return;
     if (gemm_launch(op, s)) return;
     if (kernels_launch(op, s)) return;
-    (void)mixed_launch(op, s);
+    if (mixed_launch(op, s)) return;
+    (void)density_launch(op, s);
 }
 
 void launch_op_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
@@ -17,7 +18,8 @@ void launch_op_grouped(int kind, int variant, const void *d_args, const GroupSch
     if (kind == OP_MARK || nblocks <= 0 || gs.total <= 0) return;
     if (gemm_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (kernels_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
-    (void)mixed_launch_grouped(kind, variant, d_args, gs, nblocks, s);
+    if (mixed_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
+    (void)density_launch_grouped(kind, variant, d_args, gs, nblocks, s);
 }
 
 int op_blocks_per_cu(int kind, int variant)
@@ -26,6 +28,7 @@ int op_blocks_per_cu(int kind, int variant)
     if (gemm_occupancy(kind, variant, &b)) return b;
     if (kernels_occupancy(kind, variant, &b)) return b;
     if (mixed_occupancy(kind, variant, &b)) return b;
+    if (density_occupancy(kind, variant, &b)) return b;
     return 0;
 }
 

@@ -63,6 +63,7 @@ enum OpKind {
     OP_RESIDUAL,      // bh_mixed.hip
     OP_UPDATE,
     OP_TRIDIAG,
+    OP_DENSITY,       // bh_density.hip
     OP_NKINDS
 };
 
@@ -122,5 +123,9 @@ bool mixed_launch(const Op &op, hipStream_t s);
 bool mixed_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
                           hipStream_t s);
 bool mixed_occupancy(int kind, int variant, int *blocks);
+bool density_launch(const Op &op, hipStream_t s);
+bool density_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
+                            hipStream_t s);
+bool density_occupancy(int kind, int variant, int *blocks);
 
 }  // namespace bh

@@ -116,3 +116,50 @@ def batched_grids(n: int = 64, seed: int = 4242):
         out.append((nr[min(3, int(u[3 * i] * 4))], nz[min(2, int(u[3 * i + 1] * 3))],
                     npf[min(3, int(u[3 * i + 2] * 4))], seed * 1000 + i))
     return out
+
+
+def _splitmix_hash_uniform(x: np.ndarray) -> np.ndarray:
+    """splitmix64 finaliser of x + golden gamma, as a double in [0, 1)."""
+    with np.errstate(over="ignore"):
+        z = x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def bipolytrope_binary(numr: int, numz: int, numphi: int, model: dict, solve_index: int, profile: np.ndarray,
+                       dr: float = 1.0 / 127.0) -> np.ndarray:
+    """Host statement of the served workload's density (the device generator
+    of bh_density.hip): two stars with rho_c f(d/R), f the tabulated
+    bipolytrope profile interpolated linearly, centred about the centre of
+    mass at separation * (1 + drift * solve_index), plus hashed noise."""
+    rho_c = model.get("rho_c", (1.0, 1.0))
+    rad = model["radius"]
+    sep = model["separation"] * (1.0 + model.get("drift", 0.0) * solve_index)
+    m1, m2 = rho_c[0] * rad[0] ** 3, rho_c[1] * rad[1] ** 3
+    cx = (sep * m2 / (m1 + m2) * dr, -sep * m1 / (m1 + m2) * dr)
+    noise, seed = model.get("noise", 0.0), int(model.get("seed", 0))
+    nprof = len(profile)
+    J = np.arange(1, numr + 1, dtype=np.float64)
+    K = np.arange(1, numz + 1, dtype=np.float64)
+    r = (J - 1.5) * dr
+    z = (K - 0.5 - float(numz // 2)) * dr
+    phi = np.arange(numphi, dtype=np.float64) * (2.0 * math.pi * (1.0 / numphi))
+    x = r[None, None, :] * np.cos(phi)[:, None, None]
+    y = r[None, None, :] * np.sin(phi)[:, None, None]
+    zz = z[None, :, None]
+    idx = np.arange(numphi * numz * numr, dtype=np.uint64).reshape(numphi, numz, numr)
+    rho = np.zeros((numphi, numz, numr))
+    for s in range(2):
+        R = rad[s] * dr
+        d = np.sqrt((x - cx[s]) ** 2 + y ** 2 + zz ** 2)
+        inside = d < R
+        u = d[inside] / R * (nprof - 1)
+        i = np.minimum(u.astype(np.int64), nprof - 2)
+        w = u - i
+        rho[inside] += rho_c[s] * (profile[i] + w * (profile[i + 1] - profile[i]))
+        if noise:
+            h = _splitmix_hash_uniform(np.uint64(seed * 1000003 + s * (1 << 32)) + idx[inside])
+            rho[inside] += (2.0 * h - 1.0) * noise * rho_c[s]
+    return rho

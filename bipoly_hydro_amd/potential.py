@@ -34,7 +34,8 @@ EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_last_timings", "bh_sync_timing", "bh_last_error",
            "bh_batch_create", "bh_batch_destroy", "bh_batch_solve_device", "bh_batch_solve",
            "bh_batch_device_bytes", "bh_batch_get_stats", "bh_batch_set_timing", "bh_batch_tuning",
-           "bh_batch_plan")
+           "bh_batch_plan", "bh_server_create", "bh_server_submit", "bh_server_wait", "bh_server_drain",
+           "bh_server_get_stats", "bh_server_destroy")
 
 BH_BATCH_GRAPH = 1
 BH_BATCH_AUTOTUNE = 2
@@ -55,6 +56,34 @@ class GridDesc(ctypes.Structure):
 class BatchStats(ctypes.Structure):
     _fields_ = [("ngrids", ctypes.c_int), ("launches", ctypes.c_int), ("graph", ctypes.c_int),
                 ("tuning_left", ctypes.c_int), ("last_ms", ctypes.c_double)]
+
+
+class BinaryModel(ctypes.Structure):
+    _fields_ = [("rho_c", ctypes.c_double * 2), ("radius", ctypes.c_double * 2), ("separation", ctypes.c_double),
+                ("drift", ctypes.c_double), ("n_core", ctypes.c_double), ("n_env", ctypes.c_double),
+                ("core_frac", ctypes.c_double), ("noise", ctypes.c_double), ("seed", ctypes.c_ulonglong)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("grid", GridDesc), ("nsolves", ctypes.c_int), ("model", BinaryModel),
+                ("pot_out", ctypes.c_void_p), ("solve_ms", ctypes.c_void_p)]
+
+
+class RequestResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_longlong), ("status", ctypes.c_int), ("nsolves", ctypes.c_int),
+                ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double), ("done_s", ctypes.c_double)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("max_active", ctypes.c_int), ("precision", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_longlong), ("completed", ctypes.c_longlong), ("solves", ctypes.c_longlong),
+                ("iterations", ctypes.c_longlong), ("active", ctypes.c_int), ("waiting", ctypes.c_int),
+                ("peak_active", ctypes.c_int), ("bytes_in_use", ctypes.c_size_t), ("peak_bytes", ctypes.c_size_t),
+                ("elapsed_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("mean_ms", ctypes.c_double)]
 
 
 _lib = None
@@ -102,6 +131,12 @@ def load_library(path: str = LIB_PATH):
     L.bh_batch_set_timing.argtypes = [vp, ctypes.c_int]
     L.bh_batch_tuning.argtypes = [vp, ip, ip, ctypes.c_int]
     L.bh_batch_plan.argtypes = [vp, ctypes.c_int, pp]
+    L.bh_server_create.argtypes = [ctypes.POINTER(ServerConfig), pp]
+    L.bh_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
+    L.bh_server_wait.argtypes = [vp, ctypes.c_longlong, ctypes.POINTER(RequestResult)]
+    L.bh_server_drain.argtypes = [vp]
+    L.bh_server_get_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.bh_server_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -264,3 +299,83 @@ class PotentialBatch:
         nl = (ctypes.c_int * max(m, 1))()
         L.bh_batch_tuning(self._h, tile, nl, m)
         return list(tile)[:m], list(nl)[:m]
+
+
+class PotentialServer:
+    """Continuous-batching server of SCF solve sequences (bh_server_*)."""
+
+    def __init__(self, *, hbm_budget: int = 0, max_active: int = 64, precision: str = "mixed"):
+        L = load_library()
+        cfg = ServerConfig(hbm_budget, max_active, _PREC[precision])
+        h = ctypes.c_void_p()
+        _check(L.bh_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            load_library().bh_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # interpreter shutdown
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, grid, nsolves: int, model: dict, *, want_potential: bool = False,
+               want_latencies: bool = True, dr: float = 1.0 / 127.0) -> int:
+        """grid = (numr, numz, numphi); model keys as bh_binary_model."""
+        numr, numz, numphi = grid
+        m = BinaryModel()
+        m.rho_c[:] = list(model.get("rho_c", (0.0, 0.0)))
+        m.radius[:] = list(model["radius"])
+        m.separation = model["separation"]
+        m.drift = model.get("drift", 0.0)
+        m.n_core, m.n_env, m.core_frac = model.get("n_core", 0.0), model.get("n_env", 0.0), model.get("core_frac", 0.0)
+        m.noise = model.get("noise", 0.0)
+        m.seed = int(model.get("seed", 0))
+        pot = np.empty((numphi, numz, numr)) if want_potential else None
+        lat = np.zeros(nsolves) if want_latencies else None
+        r = Request(GridDesc(numr, numz, numphi, 1, dr, 30, 20, 5), nsolves, m,
+                    pot.ctypes.data if pot is not None else None, lat.ctypes.data if lat is not None else None)
+        rid = ctypes.c_longlong()
+        _check(load_library().bh_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
+        self._keep[rid.value] = (pot, lat)
+        return rid.value
+
+    def wait(self, rid: int):
+        """Block until request rid finishes: (result dict, last potential or
+        None, per-solve latencies in ms or None)."""
+        res = RequestResult()
+        rc = load_library().bh_server_wait(self._h, rid, ctypes.byref(res))
+        pot, lat = self._keep.pop(rid, (None, None))
+        _check(rc)
+        return {k: getattr(res, k) for k, _ in RequestResult._fields_}, pot, lat
+
+    def drain(self):
+        _check(load_library().bh_server_drain(self._h))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(load_library().bh_server_get_stats(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+
+def bipolytrope_profile(n_core: float = 3.0, n_env: float = 1.5, core_frac: float = 0.3, ntab: int = 2049):
+    """The served workload's normalised bipolytrope profile (host code of the
+    library): (table on u = 0..1, surface radius, interface radius) in the
+    core's Lane-Emden units."""
+    L = load_library()
+    f = L.bhx_bipolytrope_profile
+    f.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    t = np.zeros(ntab)
+    info = np.zeros(2)
+    _check(f(n_core, n_env, core_frac, t.ctypes.data, ntab, info.ctypes.data))
+    return t, float(info[0]), float(info[1])

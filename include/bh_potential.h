@@ -146,6 +146,79 @@ int bh_batch_tuning(const bh_batch *batch, int *tile, int *nl, int n);
  * bh_set_potential. */
 int bh_batch_plan(bh_batch *batch, int i, bh_plan **plan);
 
+/* ---- Serving --------------------------------------------------------------
+ * No counterpart in the reference.  A server runs requests of different grid
+ * sizes, each a self-consistent-field (SCF) sequence of dependent solves: a
+ * cold first solve, then warm solves from the potential the request's plan
+ * keeps on the device, on a density that changes slowly from solve to solve
+ * and is generated on the device.  A worker thread runs continuous batching:
+ * every loop iteration takes the next solve of each active request and runs
+ * them together as grouped launches; requests that arrive meanwhile are
+ * admitted, first come first served, as soon as a slot and device memory
+ * (hbm_budget) are free.  Plans of finished requests are kept per grid size
+ * for reuse and released when memory is needed. */
+typedef struct bh_server bh_server;
+
+/* Binary of two bipolytropes (core of index n_core out to core_frac of the
+ * radius, envelope of index n_env; density and pressure continuous), centred
+ * on the x axis in the equatorial plane about their centre of mass (masses
+ * taken as rho_c R^3).  Solve i uses the separation
+ * separation * (1 + drift * i).  Zero fields take: rho_c 1, n_core 3,
+ * n_env 1.5, core_frac 0.3. */
+typedef struct bh_binary_model {
+    double rho_c[2];      /* central densities */
+    double radius[2];     /* radii, in cells */
+    double separation;    /* centre distance at the first solve, in cells */
+    double drift;         /* relative change of the separation per solve */
+    double n_core, n_env, core_frac;
+    double noise;         /* noise amplitude inside the stars, relative to rho_c */
+    unsigned long long seed;
+} bh_binary_model;
+
+typedef struct bh_request {
+    bh_grid_desc grid;
+    int nsolves;            /* length of the SCF sequence */
+    bh_binary_model model;
+    double *pot_out;        /* optional host (numr, numz, numphi) array: the last potential */
+    double *solve_ms;       /* optional host array [nsolves]: per-solve latencies (ms) */
+} bh_request;
+
+typedef struct bh_request_result {
+    long long id;
+    int status;             /* BH_OK or a BH_E_* code */
+    int nsolves;            /* solves completed */
+    double submit_s, admit_s, done_s;  /* seconds since the server started */
+} bh_request_result;
+
+typedef struct bh_server_config {
+    size_t hbm_budget;  /* device memory the server may hold; 0 -> 90% of what is free */
+    int max_active;     /* requests on the device at once; 0 -> 64 */
+    int precision;      /* BH_PREC_MIXED or BH_PREC_FP64 */
+} bh_server_config;
+
+typedef struct bh_server_stats {
+    long long submitted, completed, solves, iterations;
+    int active, waiting, peak_active;
+    size_t bytes_in_use, peak_bytes;
+    double elapsed_s;        /* first submission to last completion */
+    double p50_ms, p99_ms, mean_ms;  /* per-solve latency: completion minus the
+                                        completion of the request's previous solve
+                                        (minus submission for the first solve) */
+} bh_server_stats;
+
+int bh_server_create(const bh_server_config *config, bh_server **out);
+/* Queue a request (thread-safe); *id identifies it.  Requests that could
+ * not fit in hbm_budget even alone are refused with BH_E_NOMEM. */
+int bh_server_submit(bh_server *server, const bh_request *request, long long *id);
+/* Wait for request id to finish; fills *result (may be NULL).  Each id can
+ * be waited for once. */
+int bh_server_wait(bh_server *server, long long id, bh_request_result *result);
+/* Wait until every submitted request has finished. */
+int bh_server_drain(bh_server *server);
+int bh_server_get_stats(bh_server *server, bh_server_stats *stats);
+/* Finish the queued requests, stop the worker, release everything. */
+int bh_server_destroy(bh_server *server);
+
 #ifdef __cplusplus
 }
 #endif
