@@ -66,7 +66,10 @@ Op density_op(const DensityArgs &a)
     return Op::make(OP_DENSITY, 0, a, (int)((plane + 255) / 256), a.numphi, 1);
 }
 
-__global__ __launch_bounds__(256) void density_kernel(DensityArgs a) { density_tile(a, blockIdx.x, blockIdx.y, 0); }
+__global__ __launch_bounds__(256) void density_kernel(DensityArgs a, int nx, int ny, int total)
+{
+    BH_SINGLE_LOOP(a, nx, ny, total, (density_tile(a, bx, by, bz)));
+}
 
 __global__ __launch_bounds__(256) void density_grouped(const DensityArgs *__restrict__ args, GroupSched gs)
 {
@@ -76,7 +79,9 @@ __global__ __launch_bounds__(256) void density_grouped(const DensityArgs *__rest
 bool density_launch(const Op &op, hipStream_t s)
 {
     if (op.kind != OP_DENSITY) return false;
-    hipLaunchKernelGGL(density_kernel, dim3(op.nx, op.ny, op.nz), dim3(256), 0, s, op.as<DensityArgs>());
+    const int nb = persistent_blocks(op.tiles(), kernel_occupancy((const void *)density_kernel, 256, 0));
+    hipLaunchKernelGGL(density_kernel, dim3(nb), dim3(256), 0, s, op.as<DensityArgs>(), op.nx, op.ny,
+                       (int)op.tiles());
     return true;
 }
 
@@ -90,8 +95,7 @@ bool density_launch_grouped(int kind, int, const void *d_args, const GroupSched 
 bool density_occupancy(int kind, int, int *blocks)
 {
     if (kind != OP_DENSITY) return false;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, (const void *)density_grouped, 256, 0) != hipSuccess)
-        *blocks = 0;
+    *blocks = kernel_occupancy((const void *)density_grouped, 256, 0);
     return true;
 }
 

@@ -121,9 +121,7 @@ int GroupExec::prepare(const std::vector<std::vector<Op>> &lists, bool *same_lay
             }
             if (total > 0x7fffffffL) return fail(BH_E_INVALID, "grouped launch with more than 2^31 tiles");
             L.total = (int)total;
-            const int occ = occupancy(L.kind, L.variant);
-            const long cap = occ > 0 ? (long)occ * cus_ : total;
-            L.nblocks = (int)std::max(1L, std::min(total, cap));
+            L.nblocks = persistent_blocks(total, occupancy(L.kind, L.variant));
             launches_.push_back(L);
         }
     }

@@ -23,6 +23,7 @@ struct GemmArgs {
     const float *mu;              // E_FILTER: per-column factor
     const void *Rin; void *C2;    // E_FILTER: C2 = Rin - mu * acc, same indexing as C
     int tile;                     // 0: 64x256, 1: 128x128, 2: 64x64, 3: 64x128, 4: 128x64 (rows x cols)
+    int col_fastest;              // tile numbering: 0 row tiles fastest, 1 column tiles fastest
 };
 
 // The instantiated (element types, B mode, epilogue, transposes) combinations.

@@ -54,7 +54,7 @@ struct Mfma<float> {
 constexpr int BK = 16;
 
 template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB, int WR, int WC, int TM, int TN>
-__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int bx, int by, int bz)
+__device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int bz)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     constexpr int BMt = WR * TM * 16, BNt = WC * TN * 16;
@@ -64,7 +64,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int bx, int by, int
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave / WC, wc = wave % WC;
-    const int row0 = by * BMt, col0 = bx * BNt;
+    const int row0 = rt * BMt, col0 = ct * BNt;
     const long z = bz;
     const T *A = (const T *)g.A + z * g.sA;
     const BT *B = (const BT *)g.B + z * g.sB;
@@ -243,19 +243,24 @@ __device__ __forceinline__ void gemm_tile_is(const GemmArgs &g, int bx, int by, 
         g, bx, by, bz);
 }
 
+// Tiles are numbered row tile fastest (the row tiles that share a column
+// panel of B run next to each other, on one XCD) or column tile fastest
+// (consecutive tiles share a row block of A), per GemmArgs::col_fastest.
 template <int I, int S>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g)
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g, int nx, int ny, int total)
 {
-    gemm_tile_is<I, S>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+    BH_SINGLE_LOOP(g, nx, ny, total,
+                   (gemm_tile_is<I, S>(g, g.col_fastest ? by : bx, g.col_fastest ? bx : by, bz)));
 }
 
 template <int I, int S>
 __global__ __launch_bounds__(256) void gemm_grouped_kernel(const GemmArgs *__restrict__ args, GroupSched gs)
 {
-    BH_GROUP_LOOP(GemmArgs, args, gs, (gemm_tile_is<I, S>(ga, bx, by, bz)));
+    BH_GROUP_LOOP(GemmArgs, args, gs,
+                  (gemm_tile_is<I, S>(ga, ga.col_fastest ? by : bx, ga.col_fastest ? bx : by, bz)));
 }
 
-typedef void (*GemmSingleFn)(GemmArgs);
+typedef void (*GemmSingleFn)(GemmArgs, int, int, int);
 typedef void (*GemmGroupedFn)(const GemmArgs *, GroupSched);
 constexpr int G_NSHAPE = 5;
 
@@ -288,7 +293,8 @@ Op gemm_op(const GemmArgs &g, int inst)
     gemm_tile_rows_cols(tile, &rows, &cols);
     GemmArgs a = g;
     a.tile = tile;
-    return Op::make(OP_GEMM, inst * 8 + tile, a, (g.M + cols - 1) / cols, (g.R + rows - 1) / rows,
+    const int nr = (g.R + rows - 1) / rows, nc = (g.M + cols - 1) / cols;
+    return Op::make(OP_GEMM, inst * 8 + tile, a, a.col_fastest ? nc : nr, a.col_fastest ? nr : nc,
                     g.batch > 0 ? g.batch : 1);
 }
 
@@ -304,7 +310,9 @@ bool gemm_launch(const Op &op, hipStream_t s)
     if (op.kind != OP_GEMM) return false;
     const int v = variant_index(op.variant);
     if (v < 0) return true;
-    hipLaunchKernelGGL(Table::single[v], dim3(op.nx, op.ny, op.nz), dim3(256), 0, s, op.as<GemmArgs>());
+    const int nb = persistent_blocks(op.tiles(), kernel_occupancy((const void *)Table::single[v], 256, 0));
+    hipLaunchKernelGGL(Table::single[v], dim3(nb), dim3(256), 0, s, op.as<GemmArgs>(), op.nx, op.ny,
+                       (int)op.tiles());
     return true;
 }
 
@@ -321,10 +329,7 @@ bool gemm_occupancy(int kind, int variant, int *blocks)
 {
     if (kind != OP_GEMM) return false;
     const int v = variant_index(variant);
-    *blocks = 0;
-    if (v >= 0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, (const void *)Table::grouped[v], 256, 0) !=
-                      hipSuccess)
-        *blocks = 0;
+    *blocks = v >= 0 ? kernel_occupancy((const void *)Table::grouped[v], 256, 0) : 0;
     return true;
 }
 

@@ -54,7 +54,7 @@ struct ContractArgs {
 };
 struct RebuildArgs {
     const double *Stb, *Ss, *besc, *bess;
-    double *F;
+    double *F;  // faces in Fourier mode order
     int numr, numz, numphi, mmax;
     double factor;
 };
@@ -101,9 +101,10 @@ constexpr int CT_SPLIT = 4;
 // Ss:  [CT_SPLIT][mmax][2][nzi]
 Op contract_topbot_op(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax);
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
-// faces F[L][fi]: fi in [0,nri) top J=fi+2, [nri,2nri) bottom, [2nri,2nri+nzi) side
-Op face_rebuild_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *F, int numr,
-                   int numz, int numphi, int mmax, double factor);
+// faces in Fourier mode order FF[mode row][fi]: fi in [0,nri) top J=fi+2,
+// [nri,2nri) bottom, [2nri,2nri+nzi) side, straight from the contraction sums
+Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
+                 int numz, int numphi, int mmax, double factor);
 // scatter Fourier-space faces FF[mode][fi] into X1 (J-layout) and X2
 // (K-layout when x2_klayout, else J-layout; may be null)
 Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int numr, int numz, int numphi);

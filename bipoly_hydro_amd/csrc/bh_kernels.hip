@@ -293,10 +293,10 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
 }
 
 template <int V, int NL>
-__global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a)
+__global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a, int nx, int ny, int total)
 {
     extern __shared__ double sres[];
-    line_sweep_tile<V, NL>(a, blockIdx.x, blockIdx.y, sres);
+    BH_SINGLE_LOOP(a, nx, ny, total, (line_sweep_tile<V, NL>(a, bx, by, sres)));
 }
 
 template <int V, int NL>
@@ -634,18 +634,30 @@ Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, i
     return Op::make(OP_CONTRACT_SIDE, 0, a, (numz - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT);
 }
 
-// physical faces: factor * (S_C(0) + sum_m 2 S_C(m) cos + 2 S_S(m) sin)
-__device__ __forceinline__ void face_rebuild_tile(const RebuildArgs &a, int bx, int by, int)
+// Boundary faces in helmadi's Fourier mode order.  bessel rebuilds each
+// physical face value as a trigonometric sum over the azimuthal modes
+// (bessel.F90:396-488),
+//   F(theta) = f (S_C(0) + sum_{m>=1} 2 S_C(m) cos(m theta) + 2 S_S(m) sin(m theta)),
+//   theta_L = (L - 1/2) dphi,  f = -dr dz dphi,
+// and helmadi transforms it along phi (helmadi.F90:131-156).  The discrete
+// Fourier transform of such a sum is exact in closed form: with
+// X_k = sum_L F_L e^{-2 pi i k (L-1)/N} and alpha = m dphi / 2,
+//   X_k = f N [ S_C(0) [k = 0 mod N]
+//             + sum_{m = k mod N} ((S_C c + S_S s) + i (S_C s - S_S c))
+//             + sum_{m = -k mod N} ((S_C c + S_S s) - i (S_C s - S_S c)) ],
+// c = cos alpha, s = sin alpha, and the mode rows hold a_k = Re X_k (rows
+// 0..N/2) and -b_k = Im X_k (rows N/2+1..N-1, k = 1..N/2-1).  So the faces
+// come straight from the contraction sums, with no phi-space face array and
+// no transform (aliasing included when mmax > N/2).
+__device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, int by, int)
 {
     const double *__restrict__ Stb = a.Stb;
     const double *__restrict__ Ss = a.Ss;
-    const double *__restrict__ besc = a.besc;
-    const double *__restrict__ bess = a.bess;
-    const int numr = a.numr, numz = a.numz, numphi = a.numphi, mmax = a.mmax;
+    const int numr = a.numr, numz = a.numz, N = a.numphi, mmax = a.mmax;
     const int nri = numr - 2, nzi = numz - 2;
     const int nf = 2 * nri + nzi;
-    int fi = bx * 128 + threadIdx.x;
-    int L = by;
+    const int fi = bx * 128 + threadIdx.x;
+    const int row = by;
     if (fi >= nf) return;
     const double *sc, *ss;
     long st, sst;  // per-m stride, per-split stride
@@ -654,7 +666,7 @@ __device__ __forceinline__ void face_rebuild_tile(const RebuildArgs &a, int bx, 
     else if (fi < 2 * nri) { idx = fi - nri; sc = Stb + 2 * nri; ss = Stb + 3 * nri; st = 4 * (long)nri; }
     else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; }
     sst = st * mmax;
-    // partial sums of the CT_SPLIT source ranges, added in a fixed order
+    // partial sums of the source ranges of the contraction, in a fixed order
     auto S_c = [&](int m) {
         double v = sc[m * st + idx];
         for (int p = 1; p < CT_SPLIT; ++p) v += sc[p * sst + m * st + idx];
@@ -665,20 +677,28 @@ __device__ __forceinline__ void face_rebuild_tile(const RebuildArgs &a, int bx, 
         for (int p = 1; p < CT_SPLIT; ++p) v += ss[p * sst + m * st + idx];
         return v;
     };
-    double acc = S_c(0);
+    const bool imag = row > N / 2;
+    const int k = imag ? row - N / 2 : row;
+    double re = 0.0, im = 0.0;
+    if (k % N == 0) re = S_c(0);
     for (int m = 1; m < mmax; ++m) {
-        double c = besc[(long)m * numphi + L], s = bess[(long)m * numphi + L];
-        acc = acc + 2.0 * (S_c(m) * c) + 2.0 * (S_s(m) * s);
+        const bool plus = (m - k) % N == 0, minus = (m + k) % N == 0;
+        if (!plus && !minus) continue;
+        const double c = a.besc[(long)m * N], s = a.bess[(long)m * N];
+        const double C = S_c(m), S = S_s(m);
+        const double r = C * c + S * s, q = C * s - S * c;
+        if (plus) { re += r; im += q; }
+        if (minus) { re += r; im -= q; }
     }
-    a.F[(long)L * nf + fi] = a.factor * acc;
+    a.F[(long)row * nf + fi] = a.factor * (double)N * (imag ? im : re);
 }
 
-Op face_rebuild_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *F, int numr,
-                   int numz, int numphi, int mmax, double factor)
+Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
+                 int numz, int numphi, int mmax, double factor)
 {
-    RebuildArgs a{Stb, Ss, besc, bess, F, numr, numz, numphi, mmax, factor};
+    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor};
     int nf = 2 * (numr - 2) + (numz - 2);
-    return Op::make(OP_REBUILD, 0, a, (nf + 127) / 128, numphi, 1);
+    return Op::make(OP_FACE_MODES, 0, a, (nf + 127) / 128, numphi, 1);
 }
 
 // Fourier faces into potential planes: X1 in J-layout, X2 (when given) in
@@ -726,9 +746,9 @@ Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int
 // ---------------------------------------------------------------------------
 // single and grouped entry points of the tile functions above
 #define BH_KERNEL_PAIR(NAME, ARGS_T, NT)                                                          \
-    __global__ __launch_bounds__(NT) void NAME##_kernel(ARGS_T a)                                 \
+    __global__ __launch_bounds__(NT) void NAME##_kernel(ARGS_T a, int nx, int ny, int total)      \
     {                                                                                             \
-        NAME##_tile(a, blockIdx.x, blockIdx.y, blockIdx.z);                                       \
+        BH_SINGLE_LOOP(a, nx, ny, total, (NAME##_tile(a, bx, by, bz)));                           \
     }                                                                                             \
     __global__ __launch_bounds__(NT) void NAME##_grouped(const ARGS_T *__restrict__ args, GroupSched gs) \
     {                                                                                             \
@@ -739,7 +759,7 @@ BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
 BH_KERNEL_PAIR(axis, AxisArgs, 128)
 BH_KERNEL_PAIR(contract_topbot, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
-BH_KERNEL_PAIR(face_rebuild, RebuildArgs, 128)
+BH_KERNEL_PAIR(face_modes, RebuildArgs, 128)
 BH_KERNEL_PAIR(face_scatter, ScatterArgs, 128)
 #undef BH_KERNEL_PAIR
 
@@ -762,7 +782,7 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         BH_REF(OP_AXIS, axis, 128)
         BH_REF(OP_CONTRACT_TB, contract_topbot, 256)
         BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
-        BH_REF(OP_REBUILD, face_rebuild, 128)
+        BH_REF(OP_FACE_MODES, face_modes, 128)
         BH_REF(OP_SCATTER, face_scatter, 128)
     case OP_SWEEP:
         *lds = sweep_lds(variant);
@@ -788,8 +808,10 @@ bool kernels_launch(const Op &op, hipStream_t s)
     KernelRef k;
     int lds;
     if (!kernel_ref(op.kind, op.variant, &k, &lds)) return false;
-    void *argv[] = {(void *)op.args};
-    (void)hipLaunchKernel(k.single, dim3(op.nx, op.ny, op.nz), dim3(k.threads), argv, lds, s);
+    int nx = op.nx, ny = op.ny, total = (int)op.tiles();
+    void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
+    const int nb = persistent_blocks(total, kernel_occupancy(k.single, k.threads, lds));
+    (void)hipLaunchKernel(k.single, dim3(nb), dim3(k.threads), argv, lds, s);
     return true;
 }
 
@@ -810,7 +832,7 @@ bool kernels_occupancy(int kind, int variant, int *blocks)
     KernelRef k;
     int lds;
     if (!kernel_ref(kind, variant, &k, &lds)) return false;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k.grouped, k.threads, lds) != hipSuccess) *blocks = 0;
+    *blocks = kernel_occupancy(k.grouped, k.threads, lds);
     return true;
 }
 

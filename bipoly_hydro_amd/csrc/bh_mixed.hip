@@ -221,9 +221,9 @@ void launch_fillf(float *p, size_t n, float v, hipStream_t s)
 
 // single and grouped entry points of the tile functions above
 #define BH_KERNEL_PAIR(NAME, ARGS_T, NT)                                                          \
-    __global__ __launch_bounds__(NT) void NAME##_kernel(ARGS_T a)                                 \
+    __global__ __launch_bounds__(NT) void NAME##_kernel(ARGS_T a, int nx, int ny, int total)      \
     {                                                                                             \
-        NAME##_tile(a, blockIdx.x, blockIdx.y, blockIdx.z);                                       \
+        BH_SINGLE_LOOP(a, nx, ny, total, (NAME##_tile(a, bx, by, bz)));                           \
     }                                                                                             \
     __global__ __launch_bounds__(NT) void NAME##_grouped(const ARGS_T *__restrict__ args, GroupSched gs) \
     {                                                                                             \
@@ -254,8 +254,10 @@ bool mixed_launch(const Op &op, hipStream_t s)
 {
     const void *k1, *k2;
     if (!mixed_ref(op.kind, &k1, &k2)) return false;
-    void *argv[] = {(void *)op.args};
-    (void)hipLaunchKernel(k1, dim3(op.nx, op.ny, op.nz), dim3(256), argv, 0, s);
+    int nx = op.nx, ny = op.ny, total = (int)op.tiles();
+    void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
+    const int nb = persistent_blocks(total, kernel_occupancy(k1, 256, 0));
+    (void)hipLaunchKernel(k1, dim3(nb), dim3(256), argv, 0, s);
     return true;
 }
 
@@ -274,7 +276,7 @@ bool mixed_occupancy(int kind, int, int *blocks)
 {
     const void *k1, *k2;
     if (!mixed_ref(kind, &k1, &k2)) return false;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, k2, 256, 0) != hipSuccess) *blocks = 0;
+    *blocks = kernel_occupancy(k2, 256, 0);
     return true;
 }
 

@@ -35,18 +35,54 @@ __device__ inline int bh_find_group(const GroupTile *grp, int ng, int t)
     return lo;
 }
 
-// Persistent loop over the tiles of every group; BODY sees `ga` (this
-// group's arguments, a register copy) and bx, by, bz.
-#define BH_GROUP_LOOP(ARGS_T, args, gs, BODY)                                       \
-    for (int t_ = blockIdx.x; t_ < (gs).total; t_ += gridDim.x) {                   \
-        const int g_ = bh_find_group((gs).grp, (gs).ngroups, t_);                   \
-        const GroupTile e_ = (gs).grp[g_];                                          \
-        const int l_ = t_ - e_.tile0;                                               \
-        const int bx = l_ % e_.nx, by = (l_ / e_.nx) % e_.ny, bz = l_ / (e_.nx * e_.ny); \
-        const ARGS_T ga = (args)[g_];                                               \
-        BODY;                                                                       \
-        __syncthreads();                                                            \
+// Persistent loop over tiles 0..total-1.  Workgroups are dispatched to the
+// 8 XCDs round-robin (workgroup b runs on XCD b % 8), and each XCD has its
+// own L2.  When the grid is a multiple of 8, XCD x takes the contiguous
+// chunk [x c, (x+1) c) of tiles, c = ceil(total / 8), so that neighbouring
+// tiles, which share operands (a GEMM's column panel, a mode's factors, a
+// sweep's halo rows), meet in one L2.
+#define BH_TILE_LOOP(total, T_VAR, ...)                                                         \
+    {                                                                                           \
+        const int n_ = gridDim.x;                                                               \
+        const int b_ = blockIdx.x;                                                              \
+        int t0_ = b_;                                                                           \
+        int t1_ = (total);                                                                      \
+        int st_ = n_;                                                                           \
+        if (n_ >= 8 && (n_ & 7) == 0) {                                                         \
+            const int c_ = ((total) + 7) >> 3;                                                  \
+            const int x_ = b_ & 7;                                                              \
+            t0_ = x_ * c_ + (b_ >> 3);                                                          \
+            t1_ = min((total), (x_ + 1) * c_);                                                  \
+            st_ = n_ >> 3;                                                                      \
+        }                                                                                       \
+        for (int T_VAR = t0_; T_VAR < t1_; T_VAR += st_) {                                      \
+            __VA_ARGS__;                                                                        \
+            __syncthreads();                                                                    \
+        }                                                                                       \
     }
+
+// Single-problem form: tile t of an (nx, ny, nz) grid.
+#define BH_SINGLE_LOOP(a, nx, ny, total, ...)                                                   \
+    BH_TILE_LOOP(total, t_, {                                                                   \
+        const int bx = t_ % (nx);                                                               \
+        const int by = (t_ / (nx)) % (ny);                                                      \
+        const int bz = t_ / ((nx) * (ny));                                                      \
+        __VA_ARGS__;                                                                            \
+    })
+
+// Grouped form: the body sees `ga` (this group's arguments, a register
+// copy) and bx, by, bz.
+#define BH_GROUP_LOOP(ARGS_T, args, gs, ...)                                                    \
+    BH_TILE_LOOP((gs).total, t_, {                                                              \
+        const int g_ = bh_find_group((gs).grp, (gs).ngroups, t_);                               \
+        const GroupTile e_ = (gs).grp[g_];                                                      \
+        const int l_ = t_ - e_.tile0;                                                           \
+        const int bx = l_ % e_.nx;                                                              \
+        const int by = (l_ / e_.nx) % e_.ny;                                                    \
+        const int bz = l_ / (e_.nx * e_.ny);                                                    \
+        const ARGS_T ga = (args)[g_];                                                           \
+        __VA_ARGS__;                                                                            \
+    })
 #endif
 
 enum OpKind {
@@ -58,7 +94,7 @@ enum OpKind {
     OP_AXIS,
     OP_CONTRACT_TB,
     OP_CONTRACT_SIDE,
-    OP_REBUILD,
+    OP_FACE_MODES,
     OP_SCATTER,
     OP_RESIDUAL,      // bh_mixed.hip
     OP_UPDATE,
@@ -102,6 +138,10 @@ struct Op {
     long tiles() const { return (long)nx * ny * nz; }
 };
 
+// Workgroups for a launch of `tiles` tiles through BH_TILE_LOOP.
+int persistent_blocks(long tiles, int blocks_per_cu);
+// Resident workgroups per CU of a kernel (cached per kernel and LDS size).
+int kernel_occupancy(const void *fn, int threads, int lds);
 // One Op, alone, on stream s.
 void launch_op(const Op &op, hipStream_t s);
 // The Ops of one (kind, variant) for several problems: d_args holds their

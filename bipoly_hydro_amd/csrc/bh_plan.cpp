@@ -41,6 +41,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     // GEMM tile shapes, overridable for tuning sweeps
     if (const char *e = std::getenv("BH_GEMM_TILE_DFT")) p->tile_dft = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_TILE_SPEC")) p->tile_spec = std::atoi(e);
+    if (const char *e = std::getenv("BH_GEMM_ORDER_DFT")) p->order_dft = std::atoi(e);
+    if (const char *e = std::getenv("BH_GEMM_ORDER_SPEC")) p->order_spec = std::atoi(e);
     bool created;
     int rc;
     if ((rc = get_tables(h, device, &p->tab, &created)) != BH_OK) return rc;
@@ -58,7 +60,6 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     ALLOC(p->proj, M * (2 * mmax - 1));
     ALLOC(p->Stb, (size_t)CT_SPLIT * mmax * 4 * nri);
     ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
-    ALLOC(p->F, (size_t)N * nf);
     ALLOC(p->FF, (size_t)N * nf);
     if (precision == BH_PREC_MIXED) {
         const size_t PI = (size_t)N * nri * nzi;
@@ -109,6 +110,7 @@ void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, doubl
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
     a.mask_numr = p->g.numr; a.mask_numz = p->g.numz;
     a.tile = Mcols >= 4096 ? p->tile_dft : 2;
+    a.col_fastest = p->order_dft;
     out->push_back(gemm_op(a, gemm_inst64(B_FOLD_PLUS, mask ? E_MASK : E_PLAIN)));
     a.A = o.Ds; a.lda = nsn; a.C = C + (long)nh * ldc; a.R = nsn; a.Kd = nsn;
     out->push_back(gemm_op(a, gemm_inst64(B_FOLD_MINUS, mask ? E_MASK : E_PLAIN)));
@@ -125,6 +127,7 @@ void dft_inv64_ops(const bh_plan *p, const double *X, long ldx, int Mcols, doubl
     a.A = o.Ec; a.lda = nh; a.B = X; a.ldb = ldx; a.C = out; a.ldc = ldo;
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
     a.tile = Mcols >= 4096 ? p->tile_dft : 2;
+    a.col_fastest = p->order_dft;
     ops->push_back(gemm_op(a, G64_PLAIN));
     a.A = o.Es; a.lda = nsn; a.B = X + (long)nh * ldx;
     a.R = nsn; a.Kd = nsn;
@@ -140,6 +143,7 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
     a.A = p->ops->Pm; a.lda = N; a.B = rho; a.ldb = M; a.C = p->proj; a.ldc = M;
     a.R = 2 * p->g.mmax - 1; a.M = (int)M; a.Kd = N; a.batch = 1;
     a.tile = M >= 4096 ? p->tile_dft : 2;
+    a.col_fastest = p->order_dft;
     out->push_back(gemm_op(a, G64_PLAIN));
 }
 
@@ -152,10 +156,8 @@ void boundary_ops(const bh_plan *p, std::vector<Op> *out)
     out->push_back(contract_topbot_op(p->tab->T, p->proj, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
     out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax));
-    out->push_back(face_rebuild_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->F, numr, numz, N, mmax,
-                                   -g.dr * g.dz * g.dphi));
-    const int nf = 2 * g.nri + g.nzi;
-    dft_fwd64_ops(p, p->F, nf, nf, p->FF, nf, false, out);
+    out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax,
+                                 -g.dr * g.dz * g.dphi));
 }
 
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
@@ -178,6 +180,8 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.R = nh; a.M = (int)M; a.Kd = nh; a.batch = 1; a.nfold = N;
         a.mask_numr = numr; a.mask_numz = numz;
         a.tile = M >= 4096 ? p->tile_dft : 2;
+        a.col_fastest = p->order_dft;
+    a.col_fastest = p->order_dft;
         out->push_back(gemm_op(a, G32D_FOLDP_MASK));
         a.A = o.Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * M; a.R = nsn; a.Kd = nsn;
         out->push_back(gemm_op(a, G32D_FOLDM_MASK));
@@ -206,6 +210,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.C = dst; a.ldc = nzi; a.sC = PIm;
         a.R = nri; a.M = nzi; a.Kd = nzi; a.batch = N;
         a.tile = p->tile_spec;
+        a.col_fastest = p->order_spec;
         out->push_back(gemm_op(a, G32_TA));
     };
     // inverse: full-plane dst[L] interior (nzi x nri) = S * in[L]^T
@@ -216,6 +221,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.C = dst + numr + 1; a.ldc = numr; a.sC = M;
         a.R = nzi; a.M = nri; a.Kd = nzi; a.batch = N;
         a.tile = p->tile_spec;
+        a.col_fastest = p->order_spec;
         out->push_back(gemm_op(a, G32_TB));
     };
     dst_fwd(p->r32, p->RT);
@@ -228,6 +234,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.R = nri; a.M = nzi; a.Kd = nri; a.batch = N;
         a.mu = o.mu32[w]; a.Rin = p->RT; a.C2 = p->CT;
         a.tile = p->tile_spec;
+        a.col_fastest = p->order_spec;
         out->push_back(gemm_op(a, G32_FILTER));
     }
     // 5. (B_r + lambda_z) dhat = C, back to physical space
@@ -485,14 +492,14 @@ extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb
     return BH_OK;
 }
 
-// which: 0 RJ, 1 RK, 2 XJ, 3 XK, 4 proj, 5 T, 6 S, 7 Stb, 8 Ss, 9 F, 10 FF
+// which: 0 RJ, 1 RK, 2 XJ, 3 XK, 4 proj, 5 T, 6 S, 7 Stb, 8 Ss, 9 (unused), 10 FF
 extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
 {
     if (!p || !host) return fail(BH_E_INVALID, "null argument");
     const Geo &g = p->g;
     const size_t M = (size_t)g.numr * g.numz, N = g.numphi, mm = g.mmax, nri = g.nri, nzi = g.nzi;
     const size_t nf = 2 * nri + nzi;
-    const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->tab->T, p->tab->S, p->Stb, p->Ss, p->F, p->FF};
+    const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->tab->T, p->tab->S, p->Stb, p->Ss, nullptr, p->FF};
     const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), mm * nri * nri * nzi,
                           mm * (2 * nzi - 1) * nri, CT_SPLIT * mm * 4 * nri, CT_SPLIT * mm * 2 * nzi, N * nf, N * nf};
     if (which < 0 || which > 10) return fail(BH_E_INVALID, "which");

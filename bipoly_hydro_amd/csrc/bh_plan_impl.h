@@ -110,12 +110,13 @@ struct bh_plan {
     size_t bytes = 0;  // own + shared
     // per-solve state
     double *RJ = nullptr, *RK = nullptr, *XJ = nullptr, *XK = nullptr, *proj = nullptr;
-    double *Stb = nullptr, *Ss = nullptr, *F = nullptr, *FF = nullptr;
+    double *Stb = nullptr, *Ss = nullptr, *FF = nullptr;
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
     float *D1 = nullptr, *Wf = nullptr;
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
     int passes = 1;                                // FP64 refinement passes (mixed)
     int tile_dft = 2, tile_spec = 3;               // GEMM tile shapes, measured best
+    int order_dft = 0, order_spec = 0;             // GEMM tile numbering (GemmArgs::col_fastest)
     bool has_solution = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;

@@ -547,7 +547,7 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
 {
     const size_t M = (size_t)d.numr * d.numz, N = d.numphi, mm = d.mmax, nri = d.numr - 2, nzi = d.numz - 2;
     const size_t nf = 2 * nri + nzi;
-    size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + CT_SPLIT * mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + 2 * N * nf);
+    size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + CT_SPLIT * mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf);
     if (precision == BH_PREC_MIXED) b += 4 * (M * N + 5 * N * nri * nzi + 2 * M * N);
     else b += 8 * 2 * M * N;
     return b;
