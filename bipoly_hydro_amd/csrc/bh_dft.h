@@ -1,0 +1,33 @@
+// bh_dft.h -- real DFT along phi as panel-wise MFMA products (bh_dft.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <vector>
+
+#include "bh_ops.h"
+
+namespace bh {
+
+struct DftArgs {
+    const double *x;  // input: N rows (phi, or modes for the inverse) x M columns, row pitch ldx
+    long ldx;
+    void *out;        // output: N rows x M columns, row pitch ldo (float for the FP32 forward)
+    long ldo;
+    const void *A;    // matrix operand in MFMA fragment order (dft_*_fragments)
+    int N, M;
+    int mask_numr, mask_numz;  // forward: zero the rhop layers (potential_solver.F90:70-78); 0 = no mask
+};
+
+enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };
+// N <= 512 (power of two >= 4)
+constexpr int DFT_MAX_N = 512;
+Op dft_op(const DftArgs &a, int kind);
+size_t dft_lds_bytes(int kind, int N);
+// Matrix operands in fragment order: forward [row tile][k step][lane] with
+// cosine rows k = 0..N/2-1 then sine rows k = 1..N/2-1; inverse P rows then
+// Q rows, n = 0..N/2-1.
+void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *f64);
+void dft_inverse_fragments(int N, std::vector<double> *f64);
+
+}  // namespace bh

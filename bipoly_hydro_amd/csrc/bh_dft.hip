@@ -1,0 +1,984 @@
+// bh_dft.hip -- the real DFT along phi (realft.F90 with helmadi's reorder,
+// helmadi.F90:131-183 and 357-392) as dense MFMA products, one column panel
+// per workgroup.
+//
+// Forward, in helmadi's mode order (rows 0..N/2: a_k; rows N/2+1..N-1: -b_k):
+//   u_n = x_n + x_{N-n}, v_n = x_n - x_{N-n}   (n = 1..H, H = N/2 - 1)
+//   a_k  = x_0 + (-1)^k x_{N/2} + sum_n cos(2 pi k n / N) u_n
+//   -b_k = -sum_n sin(2 pi k n / N) v_n
+// Inverse, with the 2/N normalisation:
+//   P_n = a_0/N + (-1)^n a_{N/2}/N + sum_k (2/N) cos(2 pi k n / N) a_k
+//   Q_n = sum_k (2/N) sin(2 pi k n / N) (-b_k)        (k = 1..H)
+//   x_n = P_n - Q_n,  x_{N-n} = P_n + Q_n
+//
+// A workgroup of 8 waves loads its panel of CB columns once, forms both
+// halves (u, v or a, -b) in LDS, and computes every output row: the
+// K = H products run on MFMA over row tiles of 16 (rows k or n = 0..N/2-1
+// of each half), the terms in x_0, x_{N/2} (a_0, a_{N/2}) are added in the
+// epilogue, and the one remaining row (k = N/2, or n = N/2) is a reduction
+// over the panel in FP64.  The matrix operand is pre-arranged on the host in
+// MFMA fragment order, [row tile][k step][lane], so every fragment load is
+// one contiguous 64-lane read from L2.  Row tile t of the forward product
+// belongs to wave t % 8; the inverse pairs the P and Q tiles of the same n
+// in one wave so that x_n and x_{N-n} are formed in registers.
+#include "bh_dft.h"
+#include "bh_ops.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <functional>
+#include <vector>
+
+namespace bh {
+
+namespace {
+
+template <typename T>
+struct Mma;
+template <>
+struct Mma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    static __device__ inline acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <>
+struct Mma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    static __device__ inline acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+// Products run over K in groups of 16 (four MFMA k-steps).  Within a group
+// the K index of (lane l, step g) is 4 (l / 16) + g rather than the MFMA's
+// own 4 g + l / 16: any one-to-one assignment gives the same sum when A and
+// B agree, and this one makes each lane's four values of a group contiguous,
+// so a group costs one 16-byte fragment load of A (from L2) and one of B
+// (from LDS, stored column-major with a pitch that spreads the 16 columns
+// of a fragment over distinct banks) per tile.
+template <typename T>
+struct Pad;
+template <>
+struct Pad<float> {
+    static constexpr int v = 4;  // pitch = K + 4 floats: 16 columns x 16 B hit distinct bank quads
+};
+template <>
+struct Pad<double> {
+    static constexpr int v = 2;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+template <typename T, int CB, bool CONTIG>
+__device__ __forceinline__ void dft_fwd_tile(const DftArgs &a, int bx, unsigned char *lds)
+{
+    typedef typename Mma<T>::acc_t acc_t;
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    constexpr int TN = CB / 16;
+    const int N = a.N, H = N / 2 - 1, KG = (H + 15) / 16, KP = 16 * KG, KPp = KP + Pad<T>::v;
+    const int RTc = (N / 2 + 15) / 16, RTs = (H + 15) / 16, RT = RTc + RTs;
+    T *U = (T *)lds;                                 // [CB][KPp]: u_{j+1} at [c][j]
+    T *Vv = U + (size_t)CB * KPp;                    // [CB][KPp]: v_{j+1}
+    double *x0 = (double *)(Vv + (size_t)CB * KPp);  // [CB] x_0
+    double *xh = x0 + CB;                            // [CB] x_{N/2}
+    double *part = xh + CB;                          // [16][CB] partial sums of row N/2
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col0 = bx * CB;
+    const double *x = a.x;
+    const long ldx = a.ldx;
+
+    // 1. panel: u, v (padded to KP with zeros), x_0, x_{N/2}; each thread
+    //    issues all the loads of a batch before it uses any
+    constexpr int BATCH = 4;
+    for (int e0 = 0; e0 < KP * CB; e0 += 512 * BATCH) {
+        double x1[BATCH], x2[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int e = e0 + 512 * b + tid, j = e / CB, gc = col0 + e % CB;
+            const bool ok = e < KP * CB && j < H && gc < a.M;
+            x1[b] = ok ? x[(long)(j + 1) * ldx + gc] : 0.0;
+            x2[b] = ok ? x[(long)(N - 1 - j) * ldx + gc] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int e = e0 + 512 * b + tid, j = e / CB, c = e % CB;
+            if (e < KP * CB) {
+                U[c * KPp + j] = (T)(x1[b] + x2[b]);
+                Vv[c * KPp + j] = (T)(x1[b] - x2[b]);
+            }
+        }
+    }
+    if (tid < CB) {
+        const int gc = col0 + tid;
+        x0[tid] = gc < a.M ? x[gc] : 0.0;
+        xh[tid] = gc < a.M ? x[(long)(N / 2) * ldx + gc] : 0.0;
+    }
+    __syncthreads();
+
+    // 2. row N/2: a_{N/2} = x_0 + (-1)^{N/2} x_{N/2} + sum_n (-1)^n u_n, in FP64
+    {
+        const int c = tid % CB, q = tid / CB;  // q in [0, 512 / CB)
+        constexpr int NQ = 512 / CB;
+        double s = 0.0;
+        for (int j = q; j < H; j += NQ) {
+            const double u = (double)U[c * KPp + j];
+            s += ((j + 1) & 1) ? -u : u;
+        }
+        if (q < 16) part[q * CB + c] = s;
+        __syncthreads();
+        if (NQ > 16 && q >= 16) part[(q & 15) * CB + c] += s;  // NQ = 32: two passes, fixed order
+        __syncthreads();
+        if (tid < CB) {
+            double t = 0.0;
+            for (int i = 0; i < 16; ++i) t += part[i * CB + tid];
+            const int gc = col0 + tid;
+            if (gc < a.M) {
+                double val = x0[tid] + (((N / 2) & 1) ? -xh[tid] : xh[tid]) + t;
+                if (a.mask_numr) {
+                    const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
+                    if (J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1) val = 0.0;
+                }
+                ((T *)a.out)[(long)(N / 2) * a.ldo + gc] = (T)val;
+            }
+        }
+    }
+
+    // 3. the MFMA products.  With 32 row tiles (N = 512) wave w owns tiles
+    //    4w..4w+3, so that it reads only u (cosine rows) or only v (sine
+    //    rows); otherwise tiles w, w + 8, w + 16, w + 24.
+    const bool contiguous = CONTIG;  // launched with CONTIG exactly when RT == 32 (N = 512)
+    int rt[4], ntile = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        rt[i] = contiguous ? 4 * wave + i : wave + 8 * i;
+        if (rt[i] < RT) ntile = i + 1;
+    }
+    bool need_u = false, need_v = false;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+        if (i < ntile) {
+            need_u |= rt[i] < RTc;
+            need_v |= rt[i] >= RTc;
+        }
+    acc_t acc[4][TN];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) acc[i][t] = (acc_t){(T)0, (T)0, (T)0, (T)0};
+    const v4 *Af = (const v4 *)a.A;  // [row tile][group][lane] of 4
+    const int jl = 4 * (lane >> 4);
+    if (contiguous) {
+        // four tiles of one half: one B source, no per-tile selects; A
+        // fragments alternate between two register sets, one group ahead
+        const T *Bp = (rt[0] < RTc) ? U : Vv;
+        const v4 *A0 = Af + ((long)rt[0] * KG) * 64 + lane;
+        const T *B0 = Bp + (lane & 15) * KPp + jl;
+        v4 p0[4], p1[4];
+        auto loadA = [&](int grp, v4 (&f)[4]) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) f[i] = A0[((long)i * KG + grp) * 64];
+        };
+        auto step = [&](int grp, const v4 (&f)[4]) {
+            v4 b[TN];
+#pragma unroll
+            for (int t = 0; t < TN; ++t) b[t] = *(const v4 *)(B0 + 16 * t * KPp + 16 * grp);
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int t = 0; t < TN; ++t) acc[i][t] = Mma<T>::mma(f[i][g], b[t][g], acc[i][t]);
+        };
+        loadA(0, p0);
+        int grp = 0;
+        for (; grp + 1 < KG; grp += 2) {
+            loadA(grp + 1, p1);
+            step(grp, p0);
+            if (grp + 2 < KG) loadA(grp + 2, p0);
+            step(grp + 1, p1);
+        }
+        if (grp < KG) step(grp, p0);
+    } else if (ntile > 0) {
+        v4 an[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            an[i] = (i < ntile) ? Af[((long)rt[i] * KG) * 64 + lane] : (v4){(T)0, (T)0, (T)0, (T)0};
+        for (int grp = 0; grp < KG; ++grp) {
+            v4 ac[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) ac[i] = an[i];
+            if (grp + 1 < KG) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if (i < ntile) an[i] = Af[((long)rt[i] * KG + grp + 1) * 64 + lane];
+            }
+            v4 bu[TN], bv[TN];
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+                const int off = (16 * t + (lane & 15)) * KPp + 16 * grp + jl;
+                bu[t] = need_u ? *(const v4 *)(U + off) : (v4){(T)0, (T)0, (T)0, (T)0};
+                bv[t] = need_v ? *(const v4 *)(Vv + off) : (v4){(T)0, (T)0, (T)0, (T)0};
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    if (i < ntile) {
+                        const bool cosrow = rt[i] < RTc;
+#pragma unroll
+                        for (int t = 0; t < TN; ++t)
+                            acc[i][t] = Mma<T>::mma(ac[i][g], cosrow ? bu[t][g] : bv[t][g], acc[i][t]);
+                    }
+                }
+        }
+    }
+
+    // 4. epilogue: a_k (+ x_0, x_{N/2} terms) in rows 0..N/2-1, -b_k in rows N/2+1..N-1
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= ntile) continue;
+        const bool cosrow = rt[i] < RTc;
+#pragma unroll
+        for (int t = 0; t < TN; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * (cosrow ? rt[i] : rt[i] - RTc) + Mma<T>::row(lane, v);
+                const int c = 16 * t + (lane & 15), gc = col0 + c;
+                if (gc >= a.M) continue;
+                long orow;
+                double val = (double)acc[i][t][v];
+                if (cosrow) {
+                    if (r >= N / 2) continue;
+                    orow = r;
+                    val = (x0[c] + ((r & 1) ? -xh[c] : xh[c])) + val;
+                } else {
+                    if (r >= H) continue;
+                    orow = N / 2 + 1 + r;
+                }
+                if (a.mask_numr) {
+                    const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
+                    if (J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1) val = 0.0;
+                }
+                ((T *)a.out)[orow * a.ldo + gc] = (T)val;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverse (FP64)
+// ---------------------------------------------------------------------------
+template <int CB>
+__device__ __forceinline__ void dft_inv_tile(const DftArgs &a, int bx, unsigned char *lds)
+{
+    typedef double T;
+    typedef Mma<double>::acc_t acc_t;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    constexpr int TN = CB / 16;
+    const int N = a.N, H = N / 2 - 1, KG = (H + 15) / 16, KP = 16 * KG, KPp = KP + Pad<T>::v;
+    const int NT = (N / 2 + 15) / 16;  // n tiles (n = 0..N/2-1) of P and of Q
+    T *Ca = (T *)lds;                    // [CB][KPp] a_k, k = j + 1
+    T *Sb = Ca + (size_t)CB * KPp;       // [CB][KPp] -b_k
+    double *a0 = Sb + (size_t)CB * KPp;  // [CB]
+    double *ah = a0 + CB;                // [CB]
+    double *part = ah + CB;              // [16][CB]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col0 = bx * CB;
+    const double *X = a.x;
+    const long ldx = a.ldx;
+    const double invN = 1.0 / (double)N;
+
+    constexpr int BATCH = 4;
+    for (int e0 = 0; e0 < KP * CB; e0 += 512 * BATCH) {
+        double p[BATCH], m[BATCH];
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int e = e0 + 512 * b + tid, j = e / CB, gc = col0 + e % CB;
+            const bool ok = e < KP * CB && j < H && gc < a.M;
+            p[b] = ok ? X[(long)(j + 1) * ldx + gc] : 0.0;
+            m[b] = ok ? X[(long)(N / 2 + 1 + j) * ldx + gc] : 0.0;
+        }
+#pragma unroll
+        for (int b = 0; b < BATCH; ++b) {
+            const int e = e0 + 512 * b + tid, j = e / CB, c = e % CB;
+            if (e < KP * CB) {
+                Ca[c * KPp + j] = p[b];
+                Sb[c * KPp + j] = m[b];
+            }
+        }
+    }
+    if (tid < CB) {
+        const int gc = col0 + tid;
+        a0[tid] = gc < a.M ? X[gc] : 0.0;
+        ah[tid] = gc < a.M ? X[(long)(N / 2) * ldx + gc] : 0.0;
+    }
+    __syncthreads();
+
+    // x_{N/2} = P_{N/2} = a_0/N + (-1)^{N/2} a_{N/2}/N + sum_k 2 (-1)^k a_k / N
+    {
+        const int c = tid % CB, q = tid / CB;
+        constexpr int NQ = 512 / CB;
+        double s = 0.0;
+        for (int j = q; j < H; j += NQ) {
+            const double v = Ca[c * KPp + j];
+            s += ((j + 1) & 1) ? -v : v;
+        }
+        if (q < 16) part[q * CB + c] = s;
+        __syncthreads();
+        if (NQ > 16 && q >= 16) part[(q & 15) * CB + c] += s;
+        __syncthreads();
+        if (tid < CB) {
+            double t = 0.0;
+            for (int i = 0; i < 16; ++i) t += part[i * CB + tid];
+            const int gc = col0 + tid;
+            if (gc < a.M)
+                ((double *)a.out)[(long)(N / 2) * a.ldo + gc] =
+                    a0[tid] * invN + (((N / 2) & 1) ? -ah[tid] : ah[tid]) * invN + 2.0 * invN * t;
+        }
+    }
+
+    // wave w: n tiles w, w + 8 of P (A rows 0..NT-1) and of Q (A rows NT..2NT-1)
+    const v4 *Af = (const v4 *)a.A;
+    acc_t accP[2][TN], accQ[2][TN];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            accP[i][t] = (acc_t){0.0, 0.0, 0.0, 0.0};
+            accQ[i][t] = (acc_t){0.0, 0.0, 0.0, 0.0};
+        }
+    int ntile = 0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+        if (wave + 8 * i < NT) ntile = i + 1;
+    if (ntile > 0) {
+        const v4 zero = {0.0, 0.0, 0.0, 0.0};
+        const int jl = 4 * (lane >> 4);
+        auto loadA = [&](int grp, v4 (&fp)[2], v4 (&fq)[2]) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                fp[i] = (i < ntile) ? Af[((long)(wave + 8 * i) * KG + grp) * 64 + lane] : zero;
+                fq[i] = (i < ntile) ? Af[((long)(NT + wave + 8 * i) * KG + grp) * 64 + lane] : zero;
+            }
+        };
+        auto step = [&](int grp, const v4 (&fp)[2], const v4 (&fq)[2]) {
+            v4 bc[TN], bs[TN];
+#pragma unroll
+            for (int t = 0; t < TN; ++t) {
+                const int off = (16 * t + (lane & 15)) * KPp + 16 * grp + jl;
+                bc[t] = *(const v4 *)(Ca + off);
+                bs[t] = *(const v4 *)(Sb + off);
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    if (i < ntile) {
+#pragma unroll
+                        for (int t = 0; t < TN; ++t) {
+                            accP[i][t] = Mma<double>::mma(fp[i][g], bc[t][g], accP[i][t]);
+                            accQ[i][t] = Mma<double>::mma(fq[i][g], bs[t][g], accQ[i][t]);
+                        }
+                    }
+        };
+        v4 p0[2], q0[2], p1[2], q1[2];
+        loadA(0, p0, q0);
+        int grp = 0;
+        for (; grp + 1 < KG; grp += 2) {
+            loadA(grp + 1, p1, q1);
+            step(grp, p0, q0);
+            if (grp + 2 < KG) loadA(grp + 2, p0, q0);
+            step(grp + 1, p1, q1);
+        }
+        if (grp < KG) step(grp, p0, q0);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (i >= ntile) continue;
+        const int nt = wave + 8 * i;
+#pragma unroll
+        for (int t = 0; t < TN; ++t)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = 16 * nt + Mma<double>::row(lane, v);
+                const int c = 16 * t + (lane & 15), gc = col0 + c;
+                if (n >= N / 2 || gc >= a.M) continue;
+                const double p = (a0[c] * invN + ((n & 1) ? -ah[c] : ah[c]) * invN) + accP[i][t][v];
+                const double q = accQ[i][t][v];
+                double *out = (double *)a.out;
+                out[(long)n * a.ldo + gc] = p - q;
+                if (n > 0) out[(long)(N - n) * a.ldo + gc] = p + q;
+            }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Pipelined form for N = 512 (the production grids): a workgroup per CU
+// loops over its panels with two panel buffers in LDS.  While the MFMA
+// products of panel p run from one buffer, the loads of panel p + 1 are
+// spread over the k-groups (each thread loads one element pair per group,
+// two groups before storing it) into the other buffer, so HBM and the
+// matrix cores stay busy together.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int PN = 512, PH = PN / 2 - 1, PKG = (PH + 15) / 16, PKP = 16 * PKG;  // 255 -> 16 groups, 256
+
+template <typename T, int CB>
+struct PipeLayout {
+    static constexpr int KPp = PKP + Pad<T>::v;
+    static constexpr size_t half = (size_t)CB * KPp * sizeof(T);        // one of U / V
+    static constexpr size_t buf = 2 * half + 2 * CB * sizeof(double);    // U, V, x0, xh
+    static constexpr size_t bytes = 2 * buf + 16 * CB * sizeof(double);  // two buffers + part
+};
+
+// Loads of one staging slice (e = slice * 512 + tid) of panel bx of a.
+template <int CB>
+__device__ __forceinline__ void slice_load(const DftArgs &a, int bx, int slice, bool inverse, double &v1, double &v2)
+{
+    const int e = slice * 512 + threadIdx.x, j = e / CB, gc = bx * CB + e % CB;
+    v1 = v2 = 0.0;
+    if (e < PKP * CB && j < PH && gc < a.M) {
+        v1 = a.x[(long)(j + 1) * a.ldx + gc];
+        v2 = inverse ? a.x[(long)(PN / 2 + 1 + j) * a.ldx + gc] : a.x[(long)(PN - 1 - j) * a.ldx + gc];
+    }
+}
+
+template <typename T, int CB>
+__device__ __forceinline__ void slice_store(unsigned char *buf, int slice, bool inverse, double v1, double v2)
+{
+    typedef PipeLayout<T, CB> Ly;
+    const int e = slice * 512 + threadIdx.x, j = e / CB, c = e % CB;
+    if (e >= PKP * CB) return;
+    T *P = (T *)buf, *Q = (T *)(buf + Ly::half);
+    P[c * Ly::KPp + j] = inverse ? (T)v1 : (T)(v1 + v2);
+    Q[c * Ly::KPp + j] = inverse ? (T)v2 : (T)(v1 - v2);
+}
+
+template <int CB>
+__device__ __forceinline__ void ends_load(const DftArgs &a, int bx, double &e0, double &eh)
+{
+    const int gc = bx * CB + threadIdx.x;
+    e0 = eh = 0.0;
+    if (threadIdx.x < CB && gc < a.M) {
+        e0 = a.x[gc];
+        eh = a.x[(long)(PN / 2) * a.ldx + gc];
+    }
+}
+
+template <typename T, int CB>
+__device__ __forceinline__ void ends_store(unsigned char *buf, double e0, double eh)
+{
+    typedef PipeLayout<T, CB> Ly;
+    double *d = (double *)(buf + 2 * Ly::half);
+    if (threadIdx.x < CB) {
+        d[threadIdx.x] = e0;
+        d[CB + threadIdx.x] = eh;
+    }
+}
+
+// Stage a whole panel (the first of a workgroup), no overlap.
+template <typename T, int CB>
+__device__ __forceinline__ void stage_all(const DftArgs &a, int bx, bool inverse, unsigned char *buf)
+{
+    constexpr int NS = (PKP * CB + 511) / 512;
+    double v1[NS], v2[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl) slice_load<CB>(a, bx, sl, inverse, v1[sl], v2[sl]);
+    double e0, eh;
+    ends_load<CB>(a, bx, e0, eh);
+#pragma unroll
+    for (int sl = 0; sl < NS; ++sl) slice_store<T, CB>(buf, sl, inverse, v1[sl], v2[sl]);
+    ends_store<T, CB>(buf, e0, eh);
+}
+
+// Row N/2 of the panel in buffer `buf` (forward: a_{N/2}; inverse: x_{N/2}).
+template <typename T, int CB>
+__device__ __forceinline__ void middle_row(const DftArgs &a, int bx, bool inverse, const unsigned char *buf,
+                                           double *part)
+{
+    typedef PipeLayout<T, CB> Ly;
+    const T *P = (const T *)buf;
+    const double *d = (const double *)(buf + 2 * Ly::half);
+    const int tid = threadIdx.x, c = tid % CB, q = tid / CB;
+    constexpr int NQ = 512 / CB;
+    double s = 0.0;
+    for (int j = q; j < PH; j += NQ) {
+        const double u = (double)P[c * Ly::KPp + j];
+        s += ((j + 1) & 1) ? -u : u;
+    }
+    if (q < 16) part[q * CB + c] = s;
+    __syncthreads();
+    if (NQ > 16 && q >= 16) part[(q & 15) * CB + c] += s;
+    __syncthreads();
+    if (tid < CB) {
+        double t = 0.0;
+        for (int i = 0; i < 16; ++i) t += part[i * CB + tid];
+        const int gc = bx * CB + tid;
+        if (gc < a.M) {
+            const double e0 = d[tid], eh = d[CB + tid];
+            double val;
+            if (inverse) {
+                const double invN = 1.0 / (double)PN;
+                val = e0 * invN + (((PN / 2) & 1) ? -eh : eh) * invN + 2.0 * invN * t;
+            } else {
+                val = e0 + (((PN / 2) & 1) ? -eh : eh) + t;
+                if (a.mask_numr) {
+                    const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
+                    if (J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1) val = 0.0;
+                }
+            }
+            ((T *)a.out)[(long)(PN / 2) * a.ldo + gc] = (T)val;
+        }
+    }
+}
+
+// The products of one panel from `cur`, staging the next panel (when
+// `nxt` is given) into `nbuf` along the way.
+template <typename T, int CB>
+__device__ __forceinline__ void fwd_panel(const DftArgs &a, int bx, const unsigned char *cur, bool more,
+                                          const DftArgs &nxt, int nbx, unsigned char *nbuf)
+{
+    typedef typename Mma<T>::acc_t acc_t;
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    typedef PipeLayout<T, CB> Ly;
+    constexpr int TN = CB / 16, RTc = PN / 2 / 16;  // 16 cosine row tiles, 16 sine
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int rt0 = 4 * wave;  // four tiles of one half
+    const bool cosw = rt0 < RTc;
+    const T *Bp = (const T *)(cur + (cosw ? 0 : Ly::half));
+    const double *d = (const double *)(cur + 2 * Ly::half);
+    const v4 *A0 = (const v4 *)a.A + ((long)rt0 * PKG) * 64 + lane;
+    const T *B0 = Bp + (lane & 15) * Ly::KPp + 4 * (lane >> 4);
+    acc_t acc[4][TN];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) acc[i][t] = (acc_t){(T)0, (T)0, (T)0, (T)0};
+    v4 p0[4], p1[4];
+    double s1[2], s2[2];  // staging loads in flight, two groups deep
+    auto loadA = [&](int grp, v4 (&f)[4]) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) f[i] = A0[((long)i * PKG + grp) * 64];
+    };
+    auto step = [&](int grp, const v4 (&f)[4]) {
+        if (more) {
+            if (grp >= 2) slice_store<T, CB>(nbuf, grp - 2, false, s1[grp & 1], s2[grp & 1]);
+            slice_load<CB>(nxt, nbx, grp, false, s1[grp & 1], s2[grp & 1]);
+        }
+        v4 b[TN];
+#pragma unroll
+        for (int t = 0; t < TN; ++t) b[t] = *(const v4 *)(B0 + 16 * t * Ly::KPp + 16 * grp);
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int t = 0; t < TN; ++t) acc[i][t] = Mma<T>::mma(f[i][g], b[t][g], acc[i][t]);
+    };
+    loadA(0, p0);
+    for (int grp = 0; grp < PKG; grp += 2) {  // PKG = 16: even
+        loadA(grp + 1, p1);
+        step(grp, p0);
+        if (grp + 2 < PKG) loadA(grp + 2, p0);
+        step(grp + 1, p1);
+    }
+    if (more) {
+        slice_store<T, CB>(nbuf, PKG - 2, false, s1[0], s2[0]);
+        slice_store<T, CB>(nbuf, PKG - 1, false, s1[1], s2[1]);
+        double e0, eh;
+        ends_load<CB>(nxt, nbx, e0, eh);
+        ends_store<T, CB>(nbuf, e0, eh);
+    }
+    // epilogue
+    const int col0 = bx * CB;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            const int c = 16 * t + (lane & 15), gc = col0 + c;
+            bool masked = false;
+            if (a.mask_numr) {
+                const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
+                masked = J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1;
+            }
+            const double e0 = d[c], eh = d[CB + c];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 16 * ((rt0 + i) % RTc) + Mma<T>::row(lane, v);
+                if (gc >= a.M) continue;
+                double val = (double)acc[i][t][v];
+                long orow;
+                if (cosw) {
+                    orow = r;
+                    val = (e0 + ((r & 1) ? -eh : eh)) + val;
+                } else {
+                    if (r >= PH) continue;
+                    orow = PN / 2 + 1 + r;
+                }
+                ((T *)a.out)[orow * a.ldo + gc] = (T)(masked ? 0.0 : val);
+            }
+        }
+}
+
+template <int CB>
+__device__ __forceinline__ void inv_panel(const DftArgs &a, int bx, const unsigned char *cur, bool more,
+                                          const DftArgs &nxt, int nbx, unsigned char *nbuf)
+{
+    typedef double T;
+    typedef Mma<double>::acc_t acc_t;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    typedef PipeLayout<T, CB> Ly;
+    constexpr int TN = CB / 16, NT = PN / 2 / 16;  // 16 n tiles of P and of Q
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const T *Ca = (const T *)cur, *Sb = (const T *)(cur + Ly::half);
+    const double *d = (const double *)(cur + 2 * Ly::half);
+    const v4 *Af = (const v4 *)a.A;
+    const int jl = 4 * (lane >> 4);
+    acc_t accP[2][TN], accQ[2][TN];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            accP[i][t] = (acc_t){0.0, 0.0, 0.0, 0.0};
+            accQ[i][t] = (acc_t){0.0, 0.0, 0.0, 0.0};
+        }
+    double s1[2], s2[2];
+    auto loadA = [&](int grp, v4 (&fp)[2], v4 (&fq)[2]) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            fp[i] = Af[((long)(wave + 8 * i) * PKG + grp) * 64 + lane];
+            fq[i] = Af[((long)(NT + wave + 8 * i) * PKG + grp) * 64 + lane];
+        }
+    };
+    auto step = [&](int grp, const v4 (&fp)[2], const v4 (&fq)[2]) {
+        if (more) {
+            if (grp >= 2) slice_store<T, CB>(nbuf, grp - 2, true, s1[grp & 1], s2[grp & 1]);
+            slice_load<CB>(nxt, nbx, grp, true, s1[grp & 1], s2[grp & 1]);
+        }
+        v4 bc[TN], bs[TN];
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            const int off = (16 * t + (lane & 15)) * Ly::KPp + 16 * grp + jl;
+            bc[t] = *(const v4 *)(Ca + off);
+            bs[t] = *(const v4 *)(Sb + off);
+        }
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int t = 0; t < TN; ++t) {
+                    accP[i][t] = Mma<double>::mma(fp[i][g], bc[t][g], accP[i][t]);
+                    accQ[i][t] = Mma<double>::mma(fq[i][g], bs[t][g], accQ[i][t]);
+                }
+    };
+    v4 p0[2], q0[2], p1[2], q1[2];
+    loadA(0, p0, q0);
+    for (int grp = 0; grp < PKG; grp += 2) {
+        loadA(grp + 1, p1, q1);
+        step(grp, p0, q0);
+        if (grp + 2 < PKG) loadA(grp + 2, p0, q0);
+        step(grp + 1, p1, q1);
+    }
+    if (more) {
+        slice_store<T, CB>(nbuf, PKG - 2, true, s1[0], s2[0]);
+        slice_store<T, CB>(nbuf, PKG - 1, true, s1[1], s2[1]);
+        double e0, eh;
+        ends_load<CB>(nxt, nbx, e0, eh);
+        ends_store<T, CB>(nbuf, e0, eh);
+    }
+    const double invN = 1.0 / (double)PN;
+    const int col0 = bx * CB;
+    double *out = (double *)a.out;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+            const int c = 16 * t + (lane & 15), gc = col0 + c;
+            const double e0 = d[c] * invN, eh = d[CB + c] * invN;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int n = 16 * (wave + 8 * i) + Mma<double>::row(lane, v);
+                if (gc >= a.M) continue;
+                const double p = (e0 + ((n & 1) ? -eh : eh)) + accP[i][t][v];
+                const double q = accQ[i][t][v];
+                out[(long)n * a.ldo + gc] = p - q;
+                if (n > 0) out[(long)(PN - n) * a.ldo + gc] = p + q;
+            }
+        }
+}
+
+// Workgroup loop over panels t0, t0 + st, ... < t1 (XCD-chunked as in
+// BH_TILE_LOOP); `resolve(t, args, bx)` names a tile's arguments and panel.
+template <typename T, int CB, bool INV, class Resolve>
+__device__ __forceinline__ void pipe_loop(int total, Resolve resolve, unsigned char *lds)
+{
+    typedef PipeLayout<T, CB> Ly;
+    const int n_ = gridDim.x, b_ = blockIdx.x;
+    int t0 = b_, t1 = total, st = n_;
+    if (n_ >= 8 && (n_ & 7) == 0) {
+        const int c_ = (total + 7) >> 3, x_ = b_ & 7;
+        t0 = x_ * c_ + (b_ >> 3);
+        t1 = min(total, (x_ + 1) * c_);
+        st = n_ >> 3;
+    }
+    if (t0 >= t1) return;
+    double *part = (double *)(lds + 2 * Ly::buf);
+    DftArgs a, an;
+    int bx, nbx = 0;
+    resolve(t0, a, bx);
+    stage_all<T, CB>(a, bx, INV, lds);
+    __syncthreads();
+    int k = 0;
+    for (int t = t0; t < t1; t += st, k ^= 1) {
+        const bool more = t + st < t1;
+        if (more) resolve(t + st, an, nbx);
+        unsigned char *cur = lds + (size_t)k * Ly::buf, *nb = lds + (size_t)(k ^ 1) * Ly::buf;
+        middle_row<T, CB>(a, bx, INV, cur, part);
+        if (INV) inv_panel<CB>(a, bx, cur, more, an, nbx, nb);
+        else fwd_panel<T, CB>(a, bx, cur, more, an, nbx, nb);
+        __syncthreads();
+        a = an;
+        bx = nbx;
+    }
+}
+
+template <int V>
+struct PipeCfg;
+template <> struct PipeCfg<0> { typedef float T; static constexpr int CB = 32; static constexpr bool INV = false; };
+template <> struct PipeCfg<1> { typedef double T; static constexpr int CB = 16; static constexpr bool INV = false; };
+template <> struct PipeCfg<2> { typedef double T; static constexpr int CB = 16; static constexpr bool INV = true; };
+
+template <int V>
+__global__ __launch_bounds__(512) void dft_pipe_kernel(DftArgs a, int nx, int ny, int total)
+{
+    extern __shared__ __align__(16) unsigned char pipe_lds[];
+    typedef PipeCfg<V> C;
+    pipe_loop<typename C::T, C::CB, C::INV>(
+        total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, pipe_lds);
+}
+
+template <int V>
+__global__ __launch_bounds__(512) void dft_pipe_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+{
+    extern __shared__ __align__(16) unsigned char pipe_lds[];
+    typedef PipeCfg<V> C;
+    pipe_loop<typename C::T, C::CB, C::INV>(
+        gs.total,
+        [&](int t, DftArgs &o, int &bx) {
+            const int g = bh_find_group(gs.grp, gs.ngroups, t);
+            o = args[g];
+            bx = t - gs.grp[g].tile0;
+        },
+        pipe_lds);
+}
+
+template <int V>
+constexpr size_t pipe_lds_bytes()
+{
+    return PipeLayout<typename PipeCfg<V>::T, PipeCfg<V>::CB>::bytes;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// entry points.  variant = kind + 4 log2(N); kind 0: forward FP32 (32
+// columns per panel), 1: forward FP64 (16 columns), 2: inverse FP64 (16)
+// ---------------------------------------------------------------------------
+static int dft_cols(int kind) { return kind == DFT_FWD_F32 ? 32 : 16; }
+
+size_t dft_lds_bytes(int kind, int N)
+{
+    const int H = N / 2 - 1, KP = 16 * ((H + 15) / 16), CB = dft_cols(kind);
+    const size_t es = kind == DFT_FWD_F32 ? 4 : 8;
+    const int KPp = KP + (kind == DFT_FWD_F32 ? 4 : 2);
+    return 2 * (size_t)KPp * CB * es + (2 + 16) * (size_t)CB * 8;
+}
+
+static int log2i(int n)
+{
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+
+template <int V>
+__device__ __forceinline__ void dft_tile_v(const DftArgs &a, int bx, unsigned char *lds)
+{
+    // kinds 0-2 at N = 512 (32 row tiles: contiguous per wave), 4-5: the
+    // forward products at smaller N
+    if (V == 0) dft_fwd_tile<float, 32, true>(a, bx, lds);
+    else if (V == 1) dft_fwd_tile<double, 16, true>(a, bx, lds);
+    else if (V == 2) dft_inv_tile<16>(a, bx, lds);
+    else if (V == 4) dft_fwd_tile<float, 32, false>(a, bx, lds);
+    else dft_fwd_tile<double, 16, false>(a, bx, lds);
+}
+
+template <int V>
+__global__ __launch_bounds__(512, 4) void dft_kernel(DftArgs a, int nx, int ny, int total)
+{
+    extern __shared__ __align__(16) unsigned char dft_lds[];
+    BH_SINGLE_LOOP(a, nx, ny, total, (dft_tile_v<V>(a, bx, dft_lds)));
+}
+
+template <int V>
+__global__ __launch_bounds__(512, 4) void dft_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+{
+    extern __shared__ __align__(16) unsigned char dft_lds[];
+    BH_GROUP_LOOP(DftArgs, args, gs, (dft_tile_v<V>(ga, bx, dft_lds)));
+}
+
+Op dft_op(const DftArgs &a, int kind)
+{
+    const int CB = dft_cols(kind);
+    return Op::make(OP_DFT, kind + 4 * log2i(a.N), a, (a.M + CB - 1) / CB, 1, 1);
+}
+
+namespace {
+const void *dft_fn(int variant, bool grouped)
+{
+    const int kind = variant % 4, N = 1 << (variant / 4);
+    const bool contig = N == DFT_MAX_N;  // 32 row tiles: four per wave
+    if (kind == DFT_INV_F64) return grouped ? (const void *)&dft_grouped<2> : (const void *)&dft_kernel<2>;
+    if (kind == DFT_FWD_F32) {
+        if (contig) return grouped ? (const void *)&dft_grouped<0> : (const void *)&dft_kernel<0>;
+        return grouped ? (const void *)&dft_grouped<4> : (const void *)&dft_kernel<4>;
+    }
+    if (contig) return grouped ? (const void *)&dft_grouped<1> : (const void *)&dft_kernel<1>;
+    return grouped ? (const void *)&dft_grouped<5> : (const void *)&dft_kernel<5>;
+}
+}  // namespace
+
+static size_t pipe_bytes(int kind)
+{
+    return kind == 0 ? pipe_lds_bytes<0>() : kind == 1 ? pipe_lds_bytes<1>() : pipe_lds_bytes<2>();
+}
+
+static const void *pipe_fn(int kind, bool grouped)
+{
+    switch (kind) {
+    case 0: return grouped ? (const void *)&dft_pipe_grouped<0> : (const void *)&dft_pipe_kernel<0>;
+    case 1: return grouped ? (const void *)&dft_pipe_grouped<1> : (const void *)&dft_pipe_kernel<1>;
+    default: return grouped ? (const void *)&dft_pipe_grouped<2> : (const void *)&dft_pipe_kernel<2>;
+    }
+}
+
+// one workgroup per CU (two panel buffers fill the LDS)
+static int pipe_blocks(int total)
+{
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    const int n = total < cus ? total : cus;
+    return n >= 8 ? n / 8 * 8 : n;
+}
+
+bool dft_launch(const Op &op, hipStream_t s)
+{
+    if (op.kind != OP_DFT) return false;
+    if ((1 << (op.variant / 4)) == DFT_MAX_N) {
+        const int kind = op.variant % 4;
+        int nx = op.nx, ny = op.ny, total = (int)op.tiles();
+        void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
+        (void)hipLaunchKernel(pipe_fn(kind, false), dim3(pipe_blocks(total)), dim3(512), argv, (int)pipe_bytes(kind), s);
+        return true;
+    }
+    const int lds = (int)dft_lds_bytes(op.variant % 4, 1 << (op.variant / 4));
+    int nx = op.nx, ny = op.ny, total = (int)op.tiles();
+    void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
+    const void *fn = dft_fn(op.variant, false);
+    (void)hipLaunchKernel(fn, dim3(persistent_blocks(total, 0)), dim3(512), argv, lds, s);
+    return true;
+}
+
+bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
+{
+    if (kind != OP_DFT) return false;
+    if ((1 << (variant / 4)) == DFT_MAX_N) {
+        GroupSched g = gs;
+        const void *ap = d_args;
+        void *argv[] = {(void *)&ap, (void *)&g};
+        (void)hipLaunchKernel(pipe_fn(variant % 4, true), dim3(pipe_blocks(gs.total)), dim3(512), argv,
+                              (int)pipe_bytes(variant % 4), s);
+        return true;
+    }
+    const int lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4));
+    GroupSched g = gs;
+    const void *ap = d_args;
+    void *argv[] = {(void *)&ap, (void *)&g};
+    (void)hipLaunchKernel(dft_fn(variant, true), dim3(nblocks), dim3(512), argv, lds, s);
+    return true;
+}
+
+bool dft_occupancy(int kind, int variant, int *blocks)
+{
+    if (kind != OP_DFT) return false;
+    *blocks = kernel_occupancy(dft_fn(variant, true), 512, (int)dft_lds_bytes(variant % 4, 1 << (variant / 4)));
+    return true;
+}
+
+// MFMA fragment order of the matrix operands (host): [row tile][group of
+// 16 k][lane][4], element g of lane l holding row 16 rt + l % 16 and
+// k = 16 group + 4 (l / 16) + g (the order the kernels read B in).
+template <typename T>
+static void frag_fill(std::vector<T> *out, int RT, int KG, const std::function<double(int, int)> &A)
+{
+    out->assign((size_t)RT * KG * 64 * 4, (T)0);
+    for (int rt = 0; rt < RT; ++rt)
+        for (int grp = 0; grp < KG; ++grp)
+            for (int l = 0; l < 64; ++l)
+                for (int g = 0; g < 4; ++g)
+                    (*out)[(((size_t)rt * KG + grp) * 64 + l) * 4 + g] =
+                        (T)A(rt * 16 + (l & 15), 16 * grp + 4 * (l >> 4) + g);
+}
+
+void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *f64)
+{
+    const int H = N / 2 - 1, KS = (H + 15) / 16;
+    const int RTc = (N / 2 + 15) / 16, RTs = (H + 15) / 16;
+    const double pi = std::acos(-1.0);
+    auto A = [&](int row, int j) -> double {
+        if (j >= H) return 0.0;
+        const long n = j + 1;
+        if (row < 16 * RTc) {  // cosine rows k = row (k < N/2)
+            const long k = row;
+            if (k >= N / 2) return 0.0;
+            return std::cos(2.0 * pi * (double)((k * n) % N) / (double)N);
+        }
+        const long k = row - 16 * RTc + 1;  // sine rows k = 1..H
+        if (k > H) return 0.0;
+        return -std::sin(2.0 * pi * (double)((k * n) % N) / (double)N);
+    };
+    if (f32) frag_fill<float>(f32, RTc + RTs, KS, A);
+    if (f64) frag_fill<double>(f64, RTc + RTs, KS, A);
+}
+
+void dft_inverse_fragments(int N, std::vector<double> *f64)
+{
+    const int H = N / 2 - 1, KS = (H + 15) / 16;
+    const int NT = (N / 2 + 15) / 16;
+    const double pi = std::acos(-1.0);
+    auto A = [&](int row, int j) -> double {
+        if (j >= H) return 0.0;
+        const long k = j + 1;
+        const bool q = row >= 16 * NT;
+        const long n = q ? row - 16 * NT : row;
+        if (n >= N / 2) return 0.0;
+        const double ang = 2.0 * pi * (double)((k * n) % N) / (double)N;
+        return 2.0 / (double)N * (q ? std::sin(ang) : std::cos(ang));
+    };
+    frag_fill<double>(f64, 2 * NT, KS, A);
+}
+
+}  // namespace bh

@@ -48,7 +48,8 @@ void launch_op(const Op &op, hipStream_t s)
     if (gemm_launch(op, s)) return;
     if (kernels_launch(op, s)) return;
     if (mixed_launch(op, s)) return;
-    (void)density_launch(op, s);
+    if (density_launch(op, s)) return;
+    (void)dft_launch(op, s);
 }
 
 void launch_op_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
@@ -57,7 +58,8 @@ void launch_op_grouped(int kind, int variant, const void *d_args, const GroupSch
     if (gemm_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (kernels_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (mixed_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
-    (void)density_launch_grouped(kind, variant, d_args, gs, nblocks, s);
+    if (density_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
+    (void)dft_launch_grouped(kind, variant, d_args, gs, nblocks, s);
 }
 
 int op_blocks_per_cu(int kind, int variant)
@@ -67,6 +69,7 @@ int op_blocks_per_cu(int kind, int variant)
     if (kernels_occupancy(kind, variant, &b)) return b;
     if (mixed_occupancy(kind, variant, &b)) return b;
     if (density_occupancy(kind, variant, &b)) return b;
+    if (dft_occupancy(kind, variant, &b)) return b;
     return 0;
 }
 

@@ -100,6 +100,7 @@ enum OpKind {
     OP_UPDATE,
     OP_TRIDIAG,
     OP_DENSITY,       // bh_density.hip
+    OP_DFT,           // bh_dft.hip
     OP_NKINDS
 };
 
@@ -167,5 +168,8 @@ bool density_launch(const Op &op, hipStream_t s);
 bool density_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
                             hipStream_t s);
 bool density_occupancy(int kind, int variant, int *blocks);
+bool dft_launch(const Op &op, hipStream_t s);
+bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s);
+bool dft_occupancy(int kind, int variant, int *blocks);
 
 }  // namespace bh

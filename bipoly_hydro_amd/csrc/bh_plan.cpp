@@ -2,6 +2,7 @@
 // sequence (the potential_solver counterpart) behind the C ABI of
 // include/bh_potential.h.
 #include "bh_plan_impl.h"
+#include "bh_dft.h"
 #include "bh_gemm.h"
 #include "bh_mixed.h"
 
@@ -43,6 +44,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     if (const char *e = std::getenv("BH_GEMM_TILE_SPEC")) p->tile_spec = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_ORDER_DFT")) p->order_dft = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_ORDER_SPEC")) p->order_spec = std::atoi(e);
+    if (const char *e = std::getenv("BH_DFT_KERNEL")) p->use_dft = std::atoi(e) != 0;
     bool created;
     int rc;
     if ((rc = get_tables(h, device, &p->tab, &created)) != BH_OK) return rc;
@@ -105,6 +107,11 @@ void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, doubl
 {
     const ModeOps &o = *p->ops;
     const int N = p->g.numphi, nh = N / 2 + 1, nsn = N / 2 - 1;
+    if (o.dfwd64 && p->use_dft) {
+        DftArgs d{B, ldb, C, ldc, o.dfwd64, N, Mcols, mask ? p->g.numr : 0, mask ? p->g.numz : 0};
+        out->push_back(dft_op(d, DFT_FWD_F64));
+        return;
+    }
     GemmArgs a{};
     a.A = o.Dc; a.lda = nh; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc;
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
@@ -123,6 +130,11 @@ void dft_inv64_ops(const bh_plan *p, const double *X, long ldx, int Mcols, doubl
 {
     const ModeOps &o = *p->ops;
     const int N = p->g.numphi, nh = N / 2 + 1, nsn = N / 2 - 1;
+    if (o.dinv64 && p->use_dft) {
+        DftArgs d{X, ldx, out, ldo, o.dinv64, N, Mcols, 0, 0};
+        ops->push_back(dft_op(d, DFT_INV_F64));
+        return;
+    }
     GemmArgs a{};
     a.A = o.Ec; a.lda = nh; a.B = X; a.ldb = ldx; a.C = out; a.ldc = ldo;
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
@@ -173,7 +185,11 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     const int w = (tstep > 1) ? 1 : 0;
     out->push_back(Op::mark(0));
     // 1. rho-hat in FP32 (masked as rhop), projections in FP64
-    {
+    if (o.dfwd32 && p->use_dft) {
+        DftArgs d{d_rho, M, p->RH32, M, o.dfwd32, N, (int)M, numr, numz};
+        out->push_back(dft_op(d, DFT_FWD_F32));
+        proj64_ops(p, d_rho, out);
+    } else {
         const int nh = N / 2 + 1, nsn = N / 2 - 1;
         GemmArgs a{};
         a.A = o.Dc32; a.lda = nh; a.B = d_rho; a.ldb = M; a.C = p->RH32; a.ldc = M;
@@ -181,7 +197,6 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.mask_numr = numr; a.mask_numz = numz;
         a.tile = M >= 4096 ? p->tile_dft : 2;
         a.col_fastest = p->order_dft;
-    a.col_fastest = p->order_dft;
         out->push_back(gemm_op(a, G32D_FOLDP_MASK));
         a.A = o.Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * M; a.R = nsn; a.Kd = nsn;
         out->push_back(gemm_op(a, G32D_FOLDM_MASK));

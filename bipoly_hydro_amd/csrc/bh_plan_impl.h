@@ -73,6 +73,9 @@ struct ModeOps {
     std::vector<double> dt[2];
     double *Dc = nullptr, *Ds = nullptr, *Ec = nullptr, *Es = nullptr, *Pm = nullptr, *besc = nullptr, *bess = nullptr;
     double *alphar = nullptr, *betar = nullptr, *elzb = nullptr;
+    // real DFT operands in MFMA fragment order (numphi <= DFT_MAX_N)
+    float *dfwd32 = nullptr;
+    double *dfwd64 = nullptr, *dinv64 = nullptr;
     // FP64 Thomas factors per pseudo-timestep
     double *rc_alpha[2] = {}, *rc_ibet[2] = {}, *rc_gback[2] = {};
     double *zc_alpha[2] = {}, *zc_ibet[2] = {}, *zc_gback[2] = {};
@@ -117,6 +120,7 @@ struct bh_plan {
     int passes = 1;                                // FP64 refinement passes (mixed)
     int tile_dft = 2, tile_spec = 3;               // GEMM tile shapes, measured best
     int order_dft = 0, order_spec = 0;             // GEMM tile numbering (GemmArgs::col_fastest)
+    bool use_dft = true;                           // panel DFT kernel (bh_dft.hip) when numphi allows
     bool has_solution = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;

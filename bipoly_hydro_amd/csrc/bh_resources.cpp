@@ -3,6 +3,7 @@
 // operator data) kept in a per-process cache so that plans of the same grid
 // share them.
 #include "bh_plan_impl.h"
+#include "bh_dft.h"
 #include "bh_mixed.h"
 
 #include <hip/hip_runtime.h>
@@ -204,6 +205,10 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     const size_t numr = d.numr, N = d.numphi, mm = d.mmax, nri = numr - 2, nzi = d.numz - 2;
     const size_t nh = N / 2 + 1, ns = N / 2 - 1;
     size_t b = 8 * (2 * nh * nh + 2 * ns * ns + (2 * mm - 1) * N + 2 * mm * N + 2 * (numr + 1) + N * (numr + 1));
+    if (N <= 512) {
+        const size_t kg = (N / 2 + 14) / 16, rt = (N / 2 + 15) / 16 + (N / 2 + 14) / 16;
+        b += 8 * 2 * rt * kg * 256 + (precision == BH_PREC_MIXED ? 4 * rt * kg * 256 : 0);
+    }
     if (precision == BH_PREC_MIXED) {
         b += 4 * (nh * nh + ns * ns + nzi * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
         b += 8 * N * (numr + 1);
@@ -347,6 +352,19 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
             (rc = up(o->Pm, std::move(Pm))) != BH_OK || (rc = up(o->besc, std::move(bc))) != BH_OK ||
             (rc = up(o->bess, std::move(bs))) != BH_OK)
             return rc;
+    }
+    if (N <= DFT_MAX_N) {
+        std::vector<float> f32;
+        std::vector<double> f64, i64;
+        dft_forward_fragments(N, mixed ? &f32 : nullptr, &f64);
+        dft_inverse_fragments(N, &i64);
+        ALLOC(o->dfwd64, f64.size());
+        ALLOC(o->dinv64, i64.size());
+        if ((rc = up(o->dfwd64, std::move(f64))) != BH_OK || (rc = up(o->dinv64, std::move(i64))) != BH_OK) return rc;
+        if (mixed) {
+            ALLOC(o->dfwd32, f32.size());
+            if ((rc = upf(o->dfwd32, std::move(f32))) != BH_OK) return rc;
+        }
     }
     if ((rc = up(o->alphar, std::vector<double>(h.alphar))) != BH_OK ||
         (rc = up(o->betar, std::vector<double>(h.betar))) != BH_OK ||
