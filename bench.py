@@ -22,6 +22,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solves/s @258×130×512 mixed vs FP64 at relL2≤1e-8; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per solve of the boundary contraction sequence (scale, contract,
+# reduce) and of one FP64 ADI half-step at 258 x 130 x 512, from rocprofv3
+# FETCH_SIZE (x 2.000, the copy-kernel calibration) + WRITE_SIZE;
+# profiles/potential_solver_258x130x512_hbm_traffic.md
+CONTRACT_TRAFFIC = None
+SWEEP_TRAFFIC = (178004 * 2.0 + 163840) * 1024
 
 
 def rell2(a, b):
@@ -153,22 +159,23 @@ def main():
     lat_p50, lat_p99 = mixed["p50_ms"], mixed["p99_ms"]
 
     # Dominant kernel of the measured (mixed) solve: the top/bottom boundary
-    # contraction, bound by reading the FP64 Green's table once:
-    # mmax x (numr-2)^2 x (numz-2) x 8 B per launch (one launch per solve).
+    # contraction, bound by reading the FP64 Green's table once.  The table
+    # is stored in blocked symmetric form (16 x 16 blocks (I, J), J <= I, of
+    # targets x sources per source plane and mode): mmax x NB(NB+1)/2 x
+    # (numz-2) x 256 x 8 B, NB = ceil((numr-2)/16).  The timed span (events
+    # around it) also holds the projection scaling and the final reduction.
     nri, nzi, mmax = numr - 2, numz - 2, 30
-    ct_bytes = mmax * nri * nri * nzi * 8
+    nb = (nri + 15) // 16
+    ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 8
     ct_ms = mixed["stage_ms"]["contract_topbot"]
-    # HBM traffic of that launch at 258x130x512 from the FETCH_SIZE / WRITE_SIZE
-    # counters (profiles/potential_solver_258x130x512_hbm_traffic.md):
-    # FETCH 1118997 KiB x 2.000 (copy-kernel calibration) + WRITE 960 KiB
-    ct_traffic = (1118997 * 2.0 + 960) * 1024 if (numr, numz, numphi) == (258, 130, 512) else None
+    ct_traffic = CONTRACT_TRAFFIC if (numr, numz, numphi) == (258, 130, 512) else None
     # ADI half-step kernel of the FP64 path: 3 streams (read phi, read
     # rho-hat, write phi) x 8 B x interior points x modes per launch
     nsteps = 5 if args.warm else 20
     interior = (numr - 2) * (numz - 2) * numphi
     sw_bytes = 3 * 8 * interior
     sw_ms = fp64["stage_ms"]["adi"] / (2 * nsteps)
-    sw_traffic = (178004 * 2.0 + 163840) * 1024 if (numr, numz, numphi) == (258, 130, 512) else None
+    sw_traffic = SWEEP_TRAFFIC if (numr, numz, numphi) == (258, 130, 512) else None
 
     cpu = None
     if not args.no_cpu:
@@ -197,7 +204,7 @@ def main():
         "p99_ms": lat_p99,
         "relL2_vs_reference_slices": mixed["relL2_vs_reference_slices"],
         "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
-        "roofline": {"bound": "hbm", "kernel": "contract_topbot (boundary Green's-table contraction)",
+        "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (topsym scale + contract + reduce)",
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,
                      "bytes_per_launch": ct_bytes, "launch_ms": ct_ms},

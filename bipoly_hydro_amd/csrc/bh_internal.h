@@ -51,12 +51,15 @@ struct ContractArgs {
     const double *table, *proj;
     double *out;
     int numr, numz, mmax;
+    const double *aux;  // sqrt(rhf) [numr + 1], 1-based J (symmetric top table)
+    const double *unused;
 };
 struct RebuildArgs {
     const double *Stb, *Ss, *besc, *bess;
     double *F;  // faces in Fourier mode order
     int numr, numz, numphi, mmax;
     double factor;
+    int nsplit_tb;  // partial sums per top/bottom value in Stb (1 or CT_SPLIT)
 };
 struct ScatterArgs {
     const double *FF;
@@ -90,21 +93,36 @@ struct TableArgs {
 };
 // top table T[m][jg-2][kz][jr], jg = 2..numr-1
 void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s);
+// blocked symmetric top table TS[m][pair][kz][16][16] (see bh_kernels.hip)
+void launch_gen_topsym_table(const TableArgs &a, double *T, hipStream_t s);
+inline size_t topsym_table_elems(int numr, int numz, int mmax)
+{
+    const size_t NB = (numr - 2 + 15) / 16;
+    return (size_t)mmax * NB * (NB + 1) / 2 * (numz - 2) * 256;
+}
 // side table in Toeplitz form S[m][d][jr], d = kg - kl + nzi - 1
 void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s);
 
 // boundary contraction and face rebuild (bessel.F90:176-296, 396-488);
 // the contractions write CT_SPLIT partial sums that the rebuild adds
 constexpr int CT_SPLIT = 4;
+// the symmetric top/bottom contraction takes 16 source planes per workgroup
+constexpr int TS_PLANES = 16;
+__host__ __device__ inline int topsym_splits(int numz) { return (numz - 2 + TS_PLANES - 1) / TS_PLANES; }
 // proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
 // Stb: [CT_SPLIT][mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
 // Ss:  [CT_SPLIT][mmax][2][nzi]
 Op contract_topbot_op(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax);
+// symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4),
+// contract (Ptb, mmax*topsym_splits*ceil(NB/2)*NB16*4), reduce into Stb (one slot)
+Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int numr, int numz, int mmax);
+Op contract_topsym_op(const double *TS, const double *Pt, double *Ptb, int numr, int numz, int mmax);
+Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax);
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
 // faces in Fourier mode order FF[mode row][fi]: fi in [0,nri) top J=fi+2,
 // [nri,2nri) bottom, [2nri,2nri+nzi) side, straight from the contraction sums
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
-                 int numz, int numphi, int mmax, double factor);
+                 int numz, int numphi, int mmax, double factor, int nsplit_tb);
 // scatter Fourier-space faces FF[mode][fi] into X1 (J-layout) and X2
 // (K-layout when x2_klayout, else J-layout; may be null)
 Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int numr, int numz, int numphi);

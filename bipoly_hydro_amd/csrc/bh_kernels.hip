@@ -448,6 +448,39 @@ __global__ __launch_bounds__(128) void gen_top_table_kernel(TableArgs a, double 
     d_qhalf_store(a, xm, coef, T + (long)jo * npts + (long)kz * nri + jr, (long)nri * npts);
 }
 
+// Top table in blocked symmetric form.  Q_{m-1/2}(chi) depends on the
+// source radius R' and the target radius R only through chi, which is
+// symmetric in them, so only the blocks (I, J), J <= I, of 16 x 16 targets
+// x sources are stored: TS[m][pair][kz][i][j], pair = I (I + 1) / 2 + J,
+// target jo = 16 I + i, source jr = 16 J + j, value Q / pi (the sqrt(R'/R)
+// factor of tm.F90:162 is applied to the projections and the result).
+__global__ __launch_bounds__(128) void gen_topsym_table_kernel(TableArgs a, double *T)
+{
+    const int nri = a.numr - 2, nzi = a.numz - 2, NB = (nri + 15) / 16, NP = NB * (NB + 1) / 2;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long total = (long)NP * nzi * 256;
+    if (t >= total) return;
+    const int e = (int)(t & 255), i = e >> 4, j = e & 15;
+    const int kz = (int)((t >> 8) % nzi);
+    const int pair = (int)((t >> 8) / nzi);
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= pair) ++I;
+    const int J = pair - I * (I + 1) / 2;
+    const int jo = 16 * I + i, jr = 16 * J + j;
+    const long mstride = (long)NP * nzi * 256;
+    double *out = T + ((long)pair * nzi + kz) * 256 + e;
+    if (jo >= nri || jr >= nri) {
+        for (int m = 0; m < a.mmax; ++m) out[(long)m * mstride] = 0.0;
+        return;
+    }
+    const int jl = jr + 2, kl = kz + 2, jg = jo + 2;
+    const double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
+    const double zT = a.zhf_g[a.numz], zhf = a.zhf_g[kl];
+    const double xm = 0.5 * rhfinv * a.rhfinv_g[jg] *
+                      ((zT - zhf) * (zT - zhf) + a.rhf_g[jg] * a.rhf_g[jg] + rhf * rhf);
+    d_qhalf_store(a, xm, a.piinv, out, mstride);
+}
+
 __global__ __launch_bounds__(128) void gen_side_table_kernel(TableArgs a, double *S)
 {
     const int nri = a.numr - 2, nzi = a.numz - 2;
@@ -469,6 +502,13 @@ void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s)
 {
     long total = (long)(a.numr - 2) * (a.numr - 2) * (a.numz - 2);
     hipLaunchKernelGGL(gen_top_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, T);
+}
+
+void launch_gen_topsym_table(const TableArgs &a, double *T, hipStream_t s)
+{
+    const long NB = (a.numr - 2 + 15) / 16;
+    const long total = NB * (NB + 1) / 2 * (a.numz - 2) * 256;
+    hipLaunchKernelGGL(gen_topsym_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, T);
 }
 
 void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
@@ -554,6 +594,151 @@ __device__ __forceinline__ void contract_topbot_tile(const ContractArgs &a, int 
         int k = threadIdx.x;
         Stb[(((long)sp * mmax + m) * 4 + c) * nri + jo0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
     }
+}
+
+// Top and bottom with the symmetric table.  The projections are first
+// scaled by sqrt(R'): Pt[m][kz][jr][q] = sqrt(rhf(jr + 2)) proj, q = top
+// cos, top sin (source row kz), bottom cos, bottom sin (mirrored row
+// nzi - 1 - kz); rows jr >= nri are zero.
+__device__ __forceinline__ void topsym_scale_tile(const ContractArgs &a, int bx, int by, int)
+{
+    const int numr = a.numr, numz = a.numz, mmax = a.mmax, nri = numr - 2, nzi = numz - 2;
+    const int NB16 = 16 * ((nri + 15) / 16);
+    const int m = by;
+    const int t = bx * 256 + threadIdx.x;  // kz * NB16 + jr
+    if (t >= nzi * NB16) return;
+    const int kz = t / NB16, jr = t % NB16;
+    const long plane = (long)numr * numz;
+    double4 v = {0.0, 0.0, 0.0, 0.0};
+    if (jr < nri) {
+        const double *pc = a.proj + (long)m * plane;
+        const double *ps = a.proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
+        const long pa = (long)(kz + 1) * numr + jr + 1, pm = (long)(nzi - kz) * numr + jr + 1;
+        const double sq = a.aux[jr + 2];
+        v.x = sq * pc[pa];
+        v.y = m > 0 ? sq * ps[pa] : 0.0;
+        v.z = sq * pc[pm];
+        v.w = m > 0 ? sq * ps[pm] : 0.0;
+    }
+    ((double4 *)a.out)[((long)m * nzi + kz) * NB16 + jr] = v;
+}
+
+Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int numr, int numz, int mmax)
+{
+    ContractArgs a{nullptr, proj, Pt, numr, numz, mmax, sqrt_rhf, nullptr};
+    const int NB16 = 16 * ((numr - 2 + 15) / 16);
+    return Op::make(OP_TOPSYM_SCALE, 0, a, ((numz - 2) * NB16 + 255) / 256, mmax, 1);
+}
+
+// Workgroup (w, m, split): row blocks I = w and NB-1-w (equal work: NB+1
+// blocks in all), source planes kz of its split.  Thread (i, j) of 256
+// takes element (i, j) of every block: the row part adds T_ij Pt(jr) to
+// target jo, the transposed part (off the diagonal) T_ij Pt(jo) to target
+// jr.  Per-target partial sums of the workgroup go to
+// Ptb[m][split][w][jo][q], reduced in a fixed order by topsym_reduce.
+__device__ __forceinline__ void contract_topsym_tile(const ContractArgs &a, int bx, int by, int bz)
+{
+    const int numr = a.numr, numz = a.numz, nri = numr - 2, nzi = numz - 2;
+    const int NB = (nri + 15) / 16, NP = NB * (NB + 1) / 2, NB16 = 16 * NB, W = (NB + 1) / 2;
+    const int w = bx, m = by, sp = bz;
+    constexpr int KZC = TS_PLANES;  // source planes per workgroup
+    const int kz0 = sp * KZC, nkz = max(0, min(nzi, kz0 + KZC) - kz0);
+    __shared__ double4 y[512];       // per-target sums (NB16 <= 512)
+    __shared__ double4 red[4][16];   // column partials of the 4 waves
+    __shared__ double4 sI[KZC][16];  // scaled projections of the row block, per plane
+    __shared__ double4 sJ[KZC][16];  // ... of the column block
+    const int tid = threadIdx.x, i = tid >> 4, j = tid & 15, lane = tid & 63, wave = tid >> 6;
+    for (int k = tid; k < NB16; k += 256) y[k] = (double4){0.0, 0.0, 0.0, 0.0};
+    const double4 *Pt = (const double4 *)a.proj + ((long)m * nzi + kz0) * NB16;
+    const double4 zero4 = {0.0, 0.0, 0.0, 0.0};
+    for (int pass = 0; pass < 2; ++pass) {
+        const int I = pass == 0 ? w : NB - 1 - w;
+        if (pass == 1 && I == w) break;
+        __syncthreads();
+        sI[i][j] = i < nkz ? Pt[(long)i * NB16 + 16 * I + j] : zero4;  // 256 threads = KZC x 16
+        double4 row = zero4;
+        for (int J = 0; J <= I; ++J) {
+            const bool off = J != I;
+            __syncthreads();  // sJ free (and sI written)
+            sJ[i][j] = i < nkz ? Pt[(long)i * NB16 + 16 * J + j] : zero4;
+            const double *blk = a.table + (((long)m * NP + I * (I + 1) / 2 + J) * nzi + kz0) * 256 + tid;
+            double q[KZC];
+#pragma unroll
+            for (int u = 0; u < KZC; ++u) q[u] = u < nkz ? blk[(long)u * 256] : 0.0;
+            __syncthreads();  // sJ written
+            double4 col = zero4;
+#pragma unroll
+            for (int u = 0; u < KZC; ++u) {
+                const double4 pj = sJ[u][j];
+                row.x += q[u] * pj.x; row.y += q[u] * pj.y; row.z += q[u] * pj.z; row.w += q[u] * pj.w;
+                const double4 pi = sI[u][i];
+                col.x += q[u] * pi.x; col.y += q[u] * pi.y; col.z += q[u] * pi.z; col.w += q[u] * pi.w;
+            }
+            if (off) {
+                // sum over i: lanes j, j+16, j+32, j+48 of each wave, then the 4 waves
+#pragma unroll
+                for (int d = 16; d < 64; d <<= 1) {
+                    col.x += __shfl_xor(col.x, d, 64); col.y += __shfl_xor(col.y, d, 64);
+                    col.z += __shfl_xor(col.z, d, 64); col.w += __shfl_xor(col.w, d, 64);
+                }
+                if (lane < 16) red[wave][lane] = col;
+                __syncthreads();
+                if (tid < 16) {
+                    const double4 s0 = red[0][tid], s1 = red[1][tid], s2 = red[2][tid], s3 = red[3][tid];
+                    double4 &t = y[16 * J + tid];
+                    t.x += (s0.x + s1.x) + (s2.x + s3.x); t.y += (s0.y + s1.y) + (s2.y + s3.y);
+                    t.z += (s0.z + s1.z) + (s2.z + s3.z); t.w += (s0.w + s1.w) + (s2.w + s3.w);
+                }
+            }
+        }
+        // sum over j: 16 consecutive lanes
+#pragma unroll
+        for (int d = 1; d < 16; d <<= 1) {
+            row.x += __shfl_xor(row.x, d, 64); row.y += __shfl_xor(row.y, d, 64);
+            row.z += __shfl_xor(row.z, d, 64); row.w += __shfl_xor(row.w, d, 64);
+        }
+        __syncthreads();
+        if (j == 0) {
+            double4 &t = y[16 * I + i];
+            t.x += row.x; t.y += row.y; t.z += row.z; t.w += row.w;
+        }
+    }
+    __syncthreads();
+    const int NS = (nzi + KZC - 1) / KZC;
+    double4 *out = (double4 *)a.out + (((long)m * NS + sp) * W + w) * NB16;
+    for (int k = tid; k < NB16; k += 256) out[k] = y[k];
+}
+
+Op contract_topsym_op(const double *TS, const double *Pt, double *Ptb, int numr, int numz, int mmax)
+{
+    ContractArgs a{TS, Pt, Ptb, numr, numz, mmax, nullptr, nullptr};
+    const int NB = (numr - 2 + 15) / 16;
+    return Op::make(OP_CONTRACT_TOPSYM, 0, a, (NB + 1) / 2, mmax, topsym_splits(numz));
+}
+
+// Stb[m][q][jo] = sum over (split, w) of Ptb / sqrt(rhf(jo + 2)), in a fixed order
+__device__ __forceinline__ void topsym_reduce_tile(const ContractArgs &a, int bx, int by, int)
+{
+    const int nri = a.numr - 2, NB = (nri + 15) / 16, NB16 = 16 * NB, W = (NB + 1) / 2;
+    const int m = by, e = bx * 256 + threadIdx.x, q = e / nri, jo = e % nri;
+    if (q >= 4) return;
+    const int nk = topsym_splits(a.numz) * W;
+    const double *P = a.proj + ((long)m * nk * NB16 + jo) * 4 + q;
+    double s = 0.0;
+    for (int k0 = 0; k0 < nk; k0 += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = k0 + u < nk ? P[(long)(k0 + u) * NB16 * 4] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+    }
+    a.out[((long)m * 4 + q) * nri + jo] = s / a.aux[jo + 2];
+}
+
+Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax)
+{
+    ContractArgs a{nullptr, Ptb, Stb, numr, numz, mmax, sqrt_rhf, nullptr};
+    return Op::make(OP_TOPSYM_REDUCE, 0, a, (4 * (numr - 2) + 255) / 256, mmax, 1);
 }
 
 // Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
@@ -662,19 +847,20 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     const double *sc, *ss;
     long st, sst;  // per-m stride, per-split stride
     int idx;
-    if (fi < nri) { idx = fi; sc = Stb; ss = Stb + nri; st = 4 * (long)nri; }
-    else if (fi < 2 * nri) { idx = fi - nri; sc = Stb + 2 * nri; ss = Stb + 3 * nri; st = 4 * (long)nri; }
-    else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; }
+    int nsp;  // partial sums stored per value
+    if (fi < nri) { idx = fi; sc = Stb; ss = Stb + nri; st = 4 * (long)nri; nsp = a.nsplit_tb; }
+    else if (fi < 2 * nri) { idx = fi - nri; sc = Stb + 2 * nri; ss = Stb + 3 * nri; st = 4 * (long)nri; nsp = a.nsplit_tb; }
+    else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; nsp = CT_SPLIT; }
     sst = st * mmax;
     // partial sums of the source ranges of the contraction, in a fixed order
     auto S_c = [&](int m) {
         double v = sc[m * st + idx];
-        for (int p = 1; p < CT_SPLIT; ++p) v += sc[p * sst + m * st + idx];
+        for (int p = 1; p < nsp; ++p) v += sc[p * sst + m * st + idx];
         return v;
     };
     auto S_s = [&](int m) {
         double v = ss[m * st + idx];
-        for (int p = 1; p < CT_SPLIT; ++p) v += ss[p * sst + m * st + idx];
+        for (int p = 1; p < nsp; ++p) v += ss[p * sst + m * st + idx];
         return v;
     };
     const bool imag = row > N / 2;
@@ -694,9 +880,9 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
 }
 
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
-                 int numz, int numphi, int mmax, double factor)
+                 int numz, int numphi, int mmax, double factor, int nsplit_tb)
 {
-    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor};
+    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor, nsplit_tb};
     int nf = 2 * (numr - 2) + (numz - 2);
     return Op::make(OP_FACE_MODES, 0, a, (nf + 127) / 128, numphi, 1);
 }
@@ -759,6 +945,9 @@ BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
 BH_KERNEL_PAIR(axis, AxisArgs, 128)
 BH_KERNEL_PAIR(contract_topbot, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
+BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
+BH_KERNEL_PAIR(topsym_reduce, ContractArgs, 256)
 BH_KERNEL_PAIR(face_modes, RebuildArgs, 128)
 BH_KERNEL_PAIR(face_scatter, ScatterArgs, 128)
 #undef BH_KERNEL_PAIR
@@ -782,6 +971,9 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         BH_REF(OP_AXIS, axis, 128)
         BH_REF(OP_CONTRACT_TB, contract_topbot, 256)
         BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
+        BH_REF(OP_TOPSYM_SCALE, topsym_scale, 256)
+        BH_REF(OP_CONTRACT_TOPSYM, contract_topsym, 256)
+        BH_REF(OP_TOPSYM_REDUCE, topsym_reduce, 256)
         BH_REF(OP_FACE_MODES, face_modes, 128)
         BH_REF(OP_SCATTER, face_scatter, 128)
     case OP_SWEEP:

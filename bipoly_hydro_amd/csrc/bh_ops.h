@@ -94,6 +94,9 @@ enum OpKind {
     OP_AXIS,
     OP_CONTRACT_TB,
     OP_CONTRACT_SIDE,
+    OP_TOPSYM_SCALE,
+    OP_CONTRACT_TOPSYM,
+    OP_TOPSYM_REDUCE,
     OP_FACE_MODES,
     OP_SCATTER,
     OP_RESIDUAL,      // bh_mixed.hip

@@ -60,7 +60,12 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     ALLOC(p->XJ, M * N);
     ALLOC(p->XK, M * N);
     ALLOC(p->proj, M * (2 * mmax - 1));
-    ALLOC(p->Stb, (size_t)CT_SPLIT * mmax * 4 * nri);
+    ALLOC(p->Stb, (size_t)mmax * 4 * nri);
+    {
+        const size_t NB = (nri + 15) / 16;
+        ALLOC(p->Pt, (size_t)mmax * nzi * 16 * NB * 4);
+        ALLOC(p->Ptb, (size_t)mmax * topsym_splits(numz) * ((NB + 1) / 2) * 16 * NB * 4);
+    }
     ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
     ALLOC(p->FF, (size_t)N * nf);
     if (precision == BH_PREC_MIXED) {
@@ -165,11 +170,13 @@ void boundary_ops(const bh_plan *p, std::vector<Op> *out)
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
     out->push_back(Op::mark(5));
-    out->push_back(contract_topbot_op(p->tab->T, p->proj, p->Stb, numr, numz, mmax));
+    out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
+    out->push_back(contract_topsym_op(p->tab->T, p->Pt, p->Ptb, numr, numz, mmax));
+    out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
     out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax));
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax,
-                                 -g.dr * g.dz * g.dphi));
+                                 -g.dr * g.dz * g.dphi, 1));
 }
 
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
@@ -507,7 +514,7 @@ extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb
     return BH_OK;
 }
 
-// which: 0 RJ, 1 RK, 2 XJ, 3 XK, 4 proj, 5 T, 6 S, 7 Stb, 8 Ss, 9 (unused), 10 FF
+// which: 0 RJ, 1 RK, 2 XJ, 3 XK, 4 proj, 5 T (blocked symmetric), 6 S, 7 Stb, 8 Ss, 9 (unused), 10 FF
 extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
 {
     if (!p || !host) return fail(BH_E_INVALID, "null argument");
@@ -515,8 +522,8 @@ extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
     const size_t M = (size_t)g.numr * g.numz, N = g.numphi, mm = g.mmax, nri = g.nri, nzi = g.nzi;
     const size_t nf = 2 * nri + nzi;
     const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->tab->T, p->tab->S, p->Stb, p->Ss, nullptr, p->FF};
-    const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), mm * nri * nri * nzi,
-                          mm * (2 * nzi - 1) * nri, CT_SPLIT * mm * 4 * nri, CT_SPLIT * mm * 2 * nzi, N * nf, N * nf};
+    const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), topsym_table_elems(g.numr, g.numz, g.mmax),
+                          mm * (2 * nzi - 1) * nri, mm * 4 * nri, CT_SPLIT * mm * 2 * nzi, N * nf, N * nf};
     if (which < 0 || which > 10) return fail(BH_E_INVALID, "which");
     if (!src[which]) return fail(BH_E_STATE, "buffer not used by this plan's precision");
     if (n < cnt[which]) return fail(BH_E_INVALID, "buffer too small: need " + std::to_string(cnt[which]));

@@ -61,8 +61,9 @@ void host_geometry(const bh_grid_desc &d, HostGeo *h);
 // (tm.F90, sm.F90, bm.F90): shared by every plan of that geometry.
 struct GeoTables {
     DevMem mem;
-    double *T = nullptr;  // top (and mirrored bottom) table [m][jo][kz][jr]
-    double *S = nullptr;  // side table, Toeplitz in z [m][d][jr]
+    double *T = nullptr;         // top (and mirrored bottom) table, blocked symmetric [m][pair][kz][16][16]
+    double *S = nullptr;         // side table, Toeplitz in z [m][d][jr]
+    double *sqrt_rhf = nullptr;  // sqrt(rhf) [numr + 1], 1-based J
 };
 
 // Transform matrices and ADI operator data of one (grid, schedule,
@@ -114,6 +115,7 @@ struct bh_plan {
     // per-solve state
     double *RJ = nullptr, *RK = nullptr, *XJ = nullptr, *XK = nullptr, *proj = nullptr;
     double *Stb = nullptr, *Ss = nullptr, *FF = nullptr;
+    double *Pt = nullptr, *Ptb = nullptr;  // scaled projections, contraction partials (top/bottom)
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
     float *D1 = nullptr, *Wf = nullptr;
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging

@@ -197,7 +197,7 @@ std::string modeops_key(const bh_grid_desc &d, int precision, int device)
 size_t tables_bytes(const bh_grid_desc &d)
 {
     const size_t nri = d.numr - 2, nzi = d.numz - 2, mm = d.mmax;
-    return 8 * (mm * nri * nri * nzi + mm * (2 * nzi - 1) * nri);
+    return 8 * (topsym_table_elems(d.numr, d.numz, d.mmax) + mm * (2 * nzi - 1) * nri + d.numr + 1);
 }
 
 size_t modeops_bytes(const bh_grid_desc &d, int precision)
@@ -222,8 +222,9 @@ int build_tables(const HostGeo &h, GeoTables *t)
 {
     const int numr = h.g.numr, numz = h.g.numz, mmax = h.g.mmax, nri = h.g.nri, nzi = h.g.nzi;
     int rc;
-    if ((rc = t->mem.alloc(&t->T, (size_t)mmax * nri * nri * nzi)) != BH_OK) return rc;
+    if ((rc = t->mem.alloc(&t->T, topsym_table_elems(numr, numz, mmax))) != BH_OK) return rc;
     if ((rc = t->mem.alloc(&t->S, (size_t)mmax * (2 * nzi - 1) * nri)) != BH_OK) return rc;
+    if ((rc = t->mem.alloc(&t->sqrt_rhf, numr + 1)) != BH_OK) return rc;
     // Green's tables on the device (tm.F90:103-200, sm.F90:118-215)
     const int nser = 111;  // hypr_upr_bnd + 1 (pot.h:7)
     std::vector<double> nu(mmax + 1, 0.0), coefh(mmax), cser((size_t)mmax * nser);
@@ -258,8 +259,11 @@ int build_tables(const HostGeo &h, GeoTables *t)
     ta.piinv = h.g.piinv; ta.dz = h.g.dz;
     ta.rhf_g = d_rhf; ta.rhfinv_g = d_rhfinv; ta.zhf_g = d_zhf;
     ta.coefh = d_coefh; ta.cser = d_cser; ta.nu = d_nu;
-    launch_gen_top_table(ta, t->T, ss.s);
+    launch_gen_topsym_table(ta, t->T, ss.s);
     launch_gen_side_table(ta, t->S, ss.s);
+    std::vector<double> sq(numr + 1, 0.0);
+    for (int J = 1; J <= numr; ++J) sq[J] = std::sqrt(h.rhf_g[J]);
+    if ((rc = upload(t->sqrt_rhf, sq, ss.s)) != BH_OK) return rc;
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ss.s));
     return BH_OK;
@@ -565,7 +569,9 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
 {
     const size_t M = (size_t)d.numr * d.numz, N = d.numphi, mm = d.mmax, nri = d.numr - 2, nzi = d.numz - 2;
     const size_t nf = 2 * nri + nzi;
-    size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + CT_SPLIT * mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf);
+    const size_t NB16 = 16 * ((nri + 15) / 16), W = (nri + 31) / 32;
+    size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf +
+                    mm * nzi * NB16 * 4 + mm * topsym_splits(d.numz) * W * NB16 * 4);
     if (precision == BH_PREC_MIXED) b += 4 * (M * N + 5 * N * nri * nzi + 2 * M * N);
     else b += 8 * 2 * M * N;
     return b;
