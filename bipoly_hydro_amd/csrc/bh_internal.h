@@ -59,7 +59,7 @@ struct RebuildArgs {
     double *F;  // faces in Fourier mode order
     int numr, numz, numphi, mmax;
     double factor;
-    int nsplit_tb;  // partial sums per top/bottom value in Stb (1 or CT_SPLIT)
+    int nsplit_tb;  // partial sums per top/bottom value in Stb
 };
 struct ScatterArgs {
     const double *FF;
@@ -91,8 +91,6 @@ struct TableArgs {
     const double *cser;     // [mmax][nser] = exp(dcoefh)
     const double *nu;       // [mmax + 1] 1-based
 };
-// top table T[m][jg-2][kz][jr], jg = 2..numr-1
-void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s);
 // blocked symmetric top table TS[m][pair][kz][16][16] (see bh_kernels.hip)
 void launch_gen_topsym_table(const TableArgs &a, double *T, hipStream_t s);
 inline size_t topsym_table_elems(int numr, int numz, int mmax)
@@ -104,15 +102,14 @@ inline size_t topsym_table_elems(int numr, int numz, int mmax)
 void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s);
 
 // boundary contraction and face rebuild (bessel.F90:176-296, 396-488);
-// the contractions write CT_SPLIT partial sums that the rebuild adds
+// the side contraction writes CT_SPLIT partial sums that the face kernel adds
 constexpr int CT_SPLIT = 4;
 // the symmetric top/bottom contraction takes 16 source planes per workgroup
 constexpr int TS_PLANES = 16;
 __host__ __device__ inline int topsym_splits(int numz) { return (numz - 2 + TS_PLANES - 1) / TS_PLANES; }
 // proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
-// Stb: [CT_SPLIT][mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
+// Stb: [mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
 // Ss:  [CT_SPLIT][mmax][2][nzi]
-Op contract_topbot_op(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax);
 // symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4),
 // contract (Ptb, mmax*topsym_splits*ceil(NB/2)*NB16*4), reduce into Stb (one slot)
 Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int numr, int numz, int mmax);

@@ -429,25 +429,6 @@ __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, double
     }
 }
 
-__global__ __launch_bounds__(128) void gen_top_table_kernel(TableArgs a, double *T)
-{
-    const int nri = a.numr - 2, nzi = a.numz - 2;
-    const long npts = (long)nri * nzi;
-    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    long total = npts * nri;
-    if (t >= total) return;
-    int jr = (int)(t % nri);
-    int kz = (int)((t / nri) % nzi);
-    int jo = (int)(t / npts);  // target jg = jo + 2
-    int jl = jr + 2, kl = kz + 2, jg = jo + 2;
-    double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
-    double zT = a.zhf_g[a.numz], zhf = a.zhf_g[kl];
-    double coef = sqrt(rhf * a.rhfinv_g[jg]) * a.piinv;
-    double xm = 0.5 * rhfinv * a.rhfinv_g[jg] *
-                ((zT - zhf) * (zT - zhf) + a.rhf_g[jg] * a.rhf_g[jg] + rhf * rhf);
-    d_qhalf_store(a, xm, coef, T + (long)jo * npts + (long)kz * nri + jr, (long)nri * npts);
-}
-
 // Top table in blocked symmetric form.  Q_{m-1/2}(chi) depends on the
 // source radius R' and the target radius R only through chi, which is
 // symmetric in them, so only the blocks (I, J), J <= I, of 16 x 16 targets
@@ -498,12 +479,6 @@ __global__ __launch_bounds__(128) void gen_side_table_kernel(TableArgs a, double
     d_qhalf_store(a, xm, coef, S + (long)d * nri + jr, (long)nd * nri);
 }
 
-void launch_gen_top_table(const TableArgs &a, double *T, hipStream_t s)
-{
-    long total = (long)(a.numr - 2) * (a.numr - 2) * (a.numz - 2);
-    hipLaunchKernelGGL(gen_top_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, T);
-}
-
 void launch_gen_topsym_table(const TableArgs &a, double *T, hipStream_t s)
 {
     const long NB = (a.numr - 2 + 15) / 16;
@@ -520,81 +495,13 @@ void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
 // ---------------------------------------------------------------------------
 // Boundary contraction.  Top and bottom share one table: the bottom Green's
 // function at source height z' equals the top one at -z' on the symmetric
-// isym = 1 grid (bm.F90 vs tm.F90), so each table row is contracted with the
-// projections and with their z-mirror.  Deterministic block reduction.
+// isym = 1 grid (bm.F90 vs tm.F90), so each table entry is contracted with
+// the projections and with their z-mirror.  Deterministic reductions.
 // ---------------------------------------------------------------------------
 constexpr int CT_TGT = 8;  // contraction targets per workgroup
 // The source points of each contraction are split into CT_SPLIT contiguous
 // ranges handled by separate workgroups; the face rebuild adds the partial
 // sums in a fixed order, so results are reproducible bit for bit.
-
-// Top and bottom: Stb[split][m][q][jo], q = top cos, top sin, bottom cos, bottom sin
-__device__ __forceinline__ void contract_topbot_tile(const ContractArgs &a, int bx, int by, int bz)
-{
-    const double *__restrict__ T = a.table;
-    const double *__restrict__ proj = a.proj;
-    double *__restrict__ Stb = a.out;
-    const int numr = a.numr, numz = a.numz, mmax = a.mmax;
-    const int nri = numr - 2, nzi = numz - 2;
-    const long npts = (long)nri * nzi, plane = (long)numr * numz;
-    const int jo0 = bx * CT_TGT, m = by, sp = bz;
-    const int nt = min(CT_TGT, nri - jo0);
-    const long chunk = (npts + CT_SPLIT - 1) / CT_SPLIT;
-    const long i0 = sp * chunk, i1 = min(npts, i0 + chunk);
-    const double *t = T + ((long)m * nri + jo0) * npts;
-    const double *pc = proj + (long)m * plane;
-    const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
-    double v[4 * CT_TGT];
-#pragma unroll
-    for (int k = 0; k < 4 * CT_TGT; ++k) v[k] = 0.0;
-    constexpr int U = 4;  // source points in flight per thread
-    for (long base = i0 + threadIdx.x; base < i1; base += 256 * U) {
-        double c0[U], c1[U], s0[U], s1[U];
-        bool ok[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            long idx = base + (long)u * 256;
-            ok[u] = idx < i1;
-            long id = ok[u] ? idx : i0;
-            int kz = (int)(id / nri), jr = (int)(id % nri);
-            long pa = (long)(kz + 1) * numr + jr + 1;
-            long pm = (long)(nzi - kz) * numr + jr + 1;
-            c0[u] = pc[pa];
-            c1[u] = pc[pm];
-            s0[u] = (m > 0) ? ps[pa] : 0.0;
-            s1[u] = (m > 0) ? ps[pm] : 0.0;
-        }
-#pragma unroll
-        for (int q = 0; q < CT_TGT; ++q) {
-            if (q < nt) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    long idx = base + (long)u * 256;
-                    double g = ok[u] ? t[q * npts + idx] : 0.0;
-                    v[4 * q + 0] += g * c0[u];
-                    v[4 * q + 1] += g * s0[u];
-                    v[4 * q + 2] += g * c1[u];
-                    v[4 * q + 3] += g * s1[u];
-                }
-            }
-        }
-    }
-    __shared__ double red[4][4 * CT_TGT];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 4 * CT_TGT; ++k) {
-        double x = v[k];
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-        if (lane == 0) red[wave][k] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4 * nt) {
-        int q = threadIdx.x / 4, c = threadIdx.x % 4;
-        int k = threadIdx.x;
-        Stb[(((long)sp * mmax + m) * 4 + c) * nri + jo0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
-    }
-}
 
 // Top and bottom with the symmetric table.  The projections are first
 // scaled by sqrt(R'): Pt[m][kz][jr][q] = sqrt(rhf(jr + 2)) proj, q = top
@@ -807,12 +714,6 @@ __device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx
     }
 }
 
-Op contract_topbot_op(const double *T, const double *proj, double *Stb, int numr, int numz, int mmax)
-{
-    ContractArgs a{T, proj, Stb, numr, numz, mmax};
-    return Op::make(OP_CONTRACT_TB, 0, a, (numr - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT);
-}
-
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax)
 {
     ContractArgs a{S, proj, Ss, numr, numz, mmax};
@@ -943,7 +844,6 @@ Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int
 BH_KERNEL_PAIR(fill, FillArgs, 256)
 BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
 BH_KERNEL_PAIR(axis, AxisArgs, 128)
-BH_KERNEL_PAIR(contract_topbot, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
@@ -969,7 +869,6 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         BH_REF(OP_FILL, fill, 256)
         BH_REF(OP_TRANSPOSE, transpose, 256)
         BH_REF(OP_AXIS, axis, 128)
-        BH_REF(OP_CONTRACT_TB, contract_topbot, 256)
         BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
         BH_REF(OP_TOPSYM_SCALE, topsym_scale, 256)
         BH_REF(OP_CONTRACT_TOPSYM, contract_topsym, 256)

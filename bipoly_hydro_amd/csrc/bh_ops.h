@@ -92,7 +92,6 @@ enum OpKind {
     OP_TRANSPOSE,
     OP_SWEEP,
     OP_AXIS,
-    OP_CONTRACT_TB,
     OP_CONTRACT_SIDE,
     OP_TOPSYM_SCALE,
     OP_CONTRACT_TOPSYM,

@@ -348,7 +348,7 @@ int collect_timing(bh_plan *p)
         for (int k = 0; k < 4; ++k) HIPCHK(hipEventElapsedTime(&d[k], t.ev[k], t.ev[k + 1]));
         HIPCHK(hipEventElapsedTime(&tot, t.ev[0], t.ev[4]));
         HIPCHK(hipEventElapsedTime(&ct, t.ev[5], t.ev[6]));
-        // order: boundary, forward DFT, ADI, inverse DFT, total, contract_topbot
+        // order: boundary, forward DFT, ADI, inverse DFT, total, top/bottom contraction
         sum[0] += d[1]; sum[1] += d[0]; sum[2] += d[2]; sum[3] += d[3]; sum[4] += tot; sum[5] += ct;
     }
     for (int k = 0; k < 6; ++k) p->last_ms[k] = sum[k] / p->tcount;
