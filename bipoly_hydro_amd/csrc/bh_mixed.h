@@ -12,6 +12,7 @@ struct ResidualArgs {
     const double *brb, *alphar, *betar;
     double gamma, four_pi;
     int numr, numz;
+    int cold;  // X is zero on the interior (a cold start): read it only on the faces
 };
 struct UpdateArgs {
     const double *X;

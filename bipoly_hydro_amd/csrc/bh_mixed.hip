@@ -32,33 +32,53 @@ namespace bh {
 // (numr x numz) planes in J-layout; dlt and w have zero faces.  r is written
 // compact: r[(K-2)*nri + (J-2)].
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int)
+// Tile: 256 consecutive J of one plane, walking RES_ROWS rows of K with the
+// rows K-1, K, K+1 of the column kept in registers (one new value of x per
+// row), the J coefficients loaded once.
+constexpr int RES_ROWS = 16;
+
+__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
 {
-    const int nri = a.numr - 2, nzi = a.numz - 2;
-    const int L = by;
-    const long plane = (long)a.numr * a.numz;
-    int t = bx * 256 + threadIdx.x;
-    if (t >= nri * nzi) return;
-    int jr = t % nri, kz = t / nri;
-    int J = jr + 2, K = kz + 2;
+    const int numr = a.numr, nri = numr - 2, nzi = a.numz - 2;
+    const int L = bz;
+    const long plane = (long)numr * a.numz;
+    const int jr = bx * 256 + threadIdx.x;
+    if (jr >= nri) return;
+    const int J = jr + 2;
+    const int K0 = by * RES_ROWS + 2, K1 = min(K0 + RES_ROWS, nzi + 2);
     const double *X = a.X + L * plane;
     const float *dl = a.dlt ? a.dlt + L * plane : nullptr;
-    long c = (long)(K - 1) * a.numr + (J - 1);
     auto val = [&](long i) { return dl ? X[i] + (double)dl[i] : X[i]; };
-    double xc = val(c), xe = val(c + 1), xs = val(c - a.numr), xn = val(c + a.numr);
-    double Ax = a.brb[(long)L * (a.numr + 1) + J] * xc + a.alphar[J] * xe;
-    if (J > 2) Ax += a.betar[J] * val(c - 1);
-    Ax += 2.0 * a.gamma * xc - a.gamma * (xs + xn);
-    double r = -a.four_pi * (double)a.rh[L * plane + c] - Ax;
-    if (a.w) r -= (double)a.w[L * plane + c];
-    if (a.Xo) a.Xo[L * plane + c] = xc;
-    if (a.r) a.r[(long)L * nri * nzi + (long)kz * nri + jr] = (float)r;
+    const double brbJ = a.brb[(long)L * (numr + 1) + J], aJ = a.alphar[J], bJ = a.betar[J];
+    const float *rh = a.rh + L * plane;
+    const float *w = a.w ? a.w + L * plane : nullptr;
+    double *Xo = a.Xo ? a.Xo + L * plane : nullptr;
+    float *r32 = a.r ? a.r + (long)L * nri * nzi : nullptr;
+    // cold start: the interior of X is zero and dlt is zero on the faces, so
+    // interior values are dlt alone and face values X alone
+    const bool cold = a.cold != 0;
+    auto inner = [&](long i) { return cold ? (dl ? (double)dl[i] : 0.0) : val(i); };
+    auto face = [&](long i) { return cold ? X[i] : val(i); };
+    long c = (long)(K0 - 1) * numr + (J - 1);
+    double xs = (K0 - 1 == 1) ? face(c - numr) : inner(c - numr), xc = inner(c);
+    for (int K = K0; K < K1; ++K, c += numr) {
+        const double xn = (K + 1 == a.numz) ? face(c + numr) : inner(c + numr);
+        const double xe = (J + 1 == numr) ? face(c + 1) : inner(c + 1);
+        double Ax = brbJ * xc + aJ * xe;
+        if (J > 2) Ax += bJ * inner(c - 1);
+        Ax += 2.0 * a.gamma * xc - a.gamma * (xs + xn);
+        double r = -a.four_pi * (double)rh[c] - Ax;
+        if (w) r -= (double)w[c];
+        if (Xo) Xo[c] = xc;
+        if (r32) r32[(long)(K - 2) * nri + jr] = (float)r;
+        xs = xc;
+        xc = xn;
+    }
 }
 
 Op residual_op(const ResidualArgs &a, int nplanes)
 {
-    int n = (a.numr - 2) * (a.numz - 2);
-    return Op::make(OP_RESIDUAL, 0, a, (n + 255) / 256, nplanes, 1);
+    return Op::make(OP_RESIDUAL, 0, a, (a.numr - 2 + 255) / 256, (a.numz - 2 + RES_ROWS - 1) / RES_ROWS, nplanes);
 }
 
 // Xo = X + dlt on the interior (faces of Xo are kept)

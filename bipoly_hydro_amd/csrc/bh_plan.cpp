@@ -214,7 +214,8 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     //    potential is zero (cold) or the one kept from the last solve (warm)
     boundary_ops(p, out);
     double *Xa = p->XJ, *Xb = p->XK;
-    out->push_back(fill_op(Xa, tstep <= 1 ? (size_t)M * N : 0, (size_t)M * N, 0.0));
+    // a cold start needs no zeroing: the residuals below read only the faces
+    // of Xa, and Xa's interior is rewritten by the final update
     out->push_back(face_scatter_op(p->FF, Xa, Xb, 0, numr, numz, N));
     out->push_back(Op::mark(2));
     // 3. r0 = g - A x0
@@ -223,6 +224,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.gamma = g.gamma; ra.four_pi = g.four_pi; ra.numr = numr; ra.numz = numz;
     ra.rh = p->RH32;
     ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
+    ra.cold = tstep <= 1;
     out->push_back(residual_op(ra, N));
     // sine transform: RT[L] (nri x nzi) = r[L]^T S
     auto dst_fwd = [&](const float *r, float *dst) {
@@ -267,6 +269,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     double *Xcur = Xa, *Xnext = Xb;
     for (int it = 0; it < p->passes; ++it) {
         ra.X = Xcur; ra.dlt = p->D1; ra.w = p->Wf; ra.Xo = Xnext; ra.r = p->r32;
+        ra.cold = (tstep <= 1 && it == 0);  // Xcur = Xa until the first pass writes Xnext
         out->push_back(residual_op(ra, N));
         dst_fwd(p->r32, p->RT);
         out->push_back(tridiag_shift_op(p->RT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
