@@ -17,6 +17,8 @@ struct DftArgs {
     const void *A;    // matrix operand in MFMA fragment order (dft_*_fragments)
     int N, M;
     int mask_numr, mask_numz;  // forward: zero the rhop layers (potential_solver.F90:70-78); 0 = no mask
+    const float *xadd;  // inverse at N = 512: input is x + xadd (FP32), ...
+    double *xstore;     // ... and x + xadd is also stored here (both null: plain input)
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };

@@ -446,8 +446,16 @@ __device__ __forceinline__ void slice_load(const DftArgs &a, int bx, int slice, 
     const int e = slice * 512 + threadIdx.x, j = e / CB, gc = bx * CB + e % CB;
     v1 = v2 = 0.0;
     if (e < PKP * CB && j < PH && gc < a.M) {
-        v1 = a.x[(long)(j + 1) * a.ldx + gc];
-        v2 = inverse ? a.x[(long)(PN / 2 + 1 + j) * a.ldx + gc] : a.x[(long)(PN - 1 - j) * a.ldx + gc];
+        const long i1 = (long)(j + 1) * a.ldx + gc;
+        const long i2 = inverse ? (long)(PN / 2 + 1 + j) * a.ldx + gc : (long)(PN - 1 - j) * a.ldx + gc;
+        v1 = a.x[i1];
+        v2 = a.x[i2];
+        if (inverse && a.xadd) {  // x + xadd, kept as the new x
+            v1 += (double)a.xadd[i1];
+            v2 += (double)a.xadd[i2];
+            a.xstore[i1] = v1;
+            a.xstore[i2] = v2;
+        }
     }
 }
 
@@ -468,8 +476,15 @@ __device__ __forceinline__ void ends_load(const DftArgs &a, int bx, double &e0, 
     const int gc = bx * CB + threadIdx.x;
     e0 = eh = 0.0;
     if (threadIdx.x < CB && gc < a.M) {
+        const long ih = (long)(PN / 2) * a.ldx + gc;
         e0 = a.x[gc];
-        eh = a.x[(long)(PN / 2) * a.ldx + gc];
+        eh = a.x[ih];
+        if (a.xadd) {
+            e0 += (double)a.xadd[gc];
+            eh += (double)a.xadd[ih];
+            a.xstore[gc] = e0;
+            a.xstore[ih] = eh;
+        }
     }
 }
 

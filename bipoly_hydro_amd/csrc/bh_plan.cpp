@@ -276,11 +276,17 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         dst_inv(p->DT, p->D1);
         std::swap(Xcur, Xnext);
     }
-    // final update into Xa, the warm-start state (pointwise, so in place is safe)
-    out->push_back(add_update_op(Xcur, p->D1, Xa, numr, numz, N));
-    out->push_back(Op::mark(3));
-    // 7. inverse DFT (FP64) and axis ghost
-    dft_inv64_ops(p, Xa, M, (int)M, d_pot, M, out);
+    // 7. x = Xcur + D1 into Xa, the warm-start state, then the inverse DFT
+    //    (FP64) and the axis ghost; the panel DFT does both in one pass
+    if (o.dinv64 && p->use_dft && N == DFT_MAX_N) {
+        out->push_back(Op::mark(3));
+        DftArgs d{Xcur, M, d_pot, M, o.dinv64, N, (int)M, 0, 0, p->D1, Xa};
+        out->push_back(dft_op(d, DFT_INV_F64));
+    } else {
+        out->push_back(add_update_op(Xcur, p->D1, Xa, numr, numz, N));  // pointwise: in place is safe
+        out->push_back(Op::mark(3));
+        dft_inv64_ops(p, Xa, M, (int)M, d_pot, M, out);
+    }
     out->push_back(axis_op(d_pot, numr, numz, N));
     out->push_back(Op::mark(4));
 }
