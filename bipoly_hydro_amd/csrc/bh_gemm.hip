@@ -53,7 +53,7 @@ struct Mfma<float> {
 // issued before the current chunk is multiplied.
 constexpr int BK = 16;
 
-template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB, int WR, int WC, int TM, int TN>
+template <typename T, typename BT, int BMODE, int EPI, bool TA, bool TB, int DST, int WR, int WC, int TM, int TN>
 __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int bz)
 {
     typedef typename Mfma<T>::acc_t acc_t;
@@ -64,14 +64,27 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wr = wave / WC, wc = wave % WC;
-    const int row0 = rt * BMt, col0 = ct * BNt;
     const long z = bz;
     const T *A = (const T *)g.A + z * g.sA;
     const BT *B = (const BT *)g.B + z * g.sB;
     const float *Badd = g.Badd ? g.Badd + z * g.sB : nullptr;
-    const int R = g.R, M = g.M, Kd = g.Kd, N = g.nfold;
+    T *C = (T *)g.C + z * g.sC;
+    const int N = g.nfold;
+    int R = g.R, M = g.M, Kd = g.Kd;
+    T fsgn = (T)1;
+    if constexpr (DST == DST_FWD) {
+        // column tiles [0, ncth) cover the even sine modes, the rest the odd
+        const int he = (N + 1) / 2, ncth = (he + BNt - 1) / BNt;
+        const int half = ct >= ncth ? 1 : 0;
+        ct -= half * ncth;
+        Kd = M = half ? N / 2 : he;
+        B += half * he;
+        C += half * he;
+        fsgn = half ? (T)-1 : (T)1;
+    }
+    const int row0 = rt * BMt, col0 = ct * BNt;
 
-    auto loadA = [&](int k0, T *st) {
+    auto loadA = [&](const T *Ap, int k0, int Klen, T *st) {
 #pragma unroll
         for (int i = 0; i < NA; ++i) {
             int t = tid + 256 * i;
@@ -80,11 +93,21 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int
             else { r = t / BK; k = t % BK; }
             int gr = row0 + r, gk = k0 + k;
             T v = (T)0;
-            if (gr < R && gk < Kd) v = TA ? A[(long)gk * g.lda + gr] : A[(long)gr * g.lda + gk];
+            if (gr < R && gk < Klen) {
+                if constexpr (DST == DST_FWD) {
+                    // planes a and n-1-a folded; the middle plane of an odd n once
+                    v = Ap[(long)gk * g.lda + gr];
+                    const int mk = N - 1 - gk;
+                    if (mk != gk) v += fsgn * Ap[(long)mk * g.lda + gr];
+                    else if (fsgn < (T)0) v = (T)0;
+                } else {
+                    v = TA ? Ap[(long)gk * g.lda + gr] : Ap[(long)gr * g.lda + gk];
+                }
+            }
             st[i] = v;
         }
     };
-    auto loadB = [&](int k0, T *st) {
+    auto loadB = [&](const BT *Bp, int k0, int Klen, T *st) {
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
             int t = tid + 256 * i;
@@ -93,17 +116,17 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int
             else { k = t / BNt; c = t % BNt; }
             int gk = k0 + k, gc = col0 + c;
             T v = (T)0;
-            if (gk < Kd && gc < M) {
+            if (gk < Klen && gc < M) {
                 if (BMODE == B_PLAIN) {
-                    v = (T)(TB ? B[(long)gc * g.ldb + gk] : B[(long)gk * g.ldb + gc]);
+                    v = (T)(TB ? Bp[(long)gc * g.ldb + gk] : Bp[(long)gk * g.ldb + gc]);
                 } else if (BMODE == B_FOLD_PLUS) {
-                    double x = (double)B[(long)gk * g.ldb + gc];
-                    if (gk > 0 && gk < N / 2) x += (double)B[(long)(N - gk) * g.ldb + gc];
+                    double x = (double)Bp[(long)gk * g.ldb + gc];
+                    if (gk > 0 && gk < N / 2) x += (double)Bp[(long)(N - gk) * g.ldb + gc];
                     v = (T)x;
                 } else if (BMODE == B_FOLD_MINUS) {
-                    v = (T)((double)B[(long)(gk + 1) * g.ldb + gc] - (double)B[(long)(N - gk - 1) * g.ldb + gc]);
+                    v = (T)((double)Bp[(long)(gk + 1) * g.ldb + gc] - (double)Bp[(long)(N - gk - 1) * g.ldb + gc]);
                 } else {  // B_ADD_F32
-                    v = (T)((double)B[(long)gk * g.ldb + gc] + (double)Badd[(long)gk * g.ldb + gc]);
+                    v = (T)((double)Bp[(long)gk * g.ldb + gc] + (double)Badd[(long)gk * g.ldb + gc]);
                 }
             }
             st[i] = v;
@@ -129,40 +152,66 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int
             Bs[k][c] = st[i];
         }
     };
+    // acc += A[:, 0:Klen] B[0:Klen, :] from the operands at Ap, Bp
+    auto kloop = [&](const T *Ap, const BT *Bp, int Klen, acc_t(&acc)[TM][TN]) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) acc[i][j] = (acc_t){(T)0, (T)0, (T)0, (T)0};
+        T sa[NA], sb[NB];
+        loadA(Ap, 0, Klen, sa);
+        loadB(Bp, 0, Klen, sb);
+        for (int k0 = 0; k0 < Klen; k0 += BK) {
+            storeA(sa);
+            storeB(sb);
+            __syncthreads();
+            if (k0 + BK < Klen) {  // the next tile's loads fly while this one is multiplied
+                loadA(Ap, k0 + BK, Klen, sa);
+                loadB(Bp, k0 + BK, Klen, sb);
+            }
+#pragma unroll
+            for (int kk = 0; kk < BK; kk += 4) {
+                T a[TM], b[TN];
+#pragma unroll
+                for (int i = 0; i < TM; ++i) a[i] = As[kk + (lane >> 4)][(wr * TM + i) * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < TN; ++j) b[j] = Bs[kk + (lane >> 4)][(wc * TN + j) * 16 + (lane & 15)];
+#pragma unroll
+                for (int i = 0; i < TM; ++i)
+#pragma unroll
+                    for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+            }
+            __syncthreads();
+        }
+    };
 
     acc_t acc[TM][TN];
+    if constexpr (DST == DST_INV) {
+        // rows z < ceil(n/2): even modes (table columns [0, he)) and odd
+        // modes (columns [he, n)); the pair gives rows z and n-1-z
+        const int he = (N + 1) / 2;
+        acc_t acco[TM][TN];
+        kloop(A, B, he, acc);
+        kloop(A + he, B + he, N / 2, acco);
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) acc[i][j] = (acc_t){(T)0, (T)0, (T)0, (T)0};
-
-    T sa[NA], sb[NB];
-    loadA(0, sa);
-    loadB(0, sb);
-    for (int k0 = 0; k0 < Kd; k0 += BK) {
-        storeA(sa);
-        storeB(sb);
-        __syncthreads();
-        if (k0 + BK < Kd) {  // the next tile's loads fly while this one is multiplied
-            loadA(k0 + BK, sa);
-            loadB(k0 + BK, sb);
-        }
+            for (int j = 0; j < TN; ++j)
 #pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            T a[TM], b[TN];
-#pragma unroll
-            for (int i = 0; i < TM; ++i) a[i] = As[kk + (lane >> 4)][(wr * TM + i) * 16 + (lane & 15)];
-#pragma unroll
-            for (int j = 0; j < TN; ++j) b[j] = Bs[kk + (lane >> 4)][(wc * TN + j) * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < TM; ++i)
-#pragma unroll
-                for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
-        }
-        __syncthreads();
+                for (int v = 0; v < 4; ++v) {
+                    const int r = row0 + (wr * TM + i) * 16 + Mfma<T>::row(lane, v);
+                    const int c = col0 + (wc * TN + j) * 16 + (lane & 15);
+                    if (r >= he || c >= M) continue;
+                    const T e = acc[i][j][v], o = acco[i][j][v];
+                    C[(long)r * g.ldc + c] = e + o;
+                    const int rm = N - 1 - r;
+                    if (rm != r) C[(long)rm * g.ldc + c] = e - o;
+                }
+        return;
+    } else {
+        kloop(A, B, Kd, acc);
     }
 
-    T *C = (T *)g.C + z * g.sC;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -200,28 +249,30 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int
 // the workgroup arrangement per tile shape.
 template <int I>
 struct Inst;
-#define BH_INST(I, T_, BT_, BM_, EP_, TA_, TB_)           \
-    template <>                                          \
-    struct Inst<I> {                                     \
-        typedef T_ T;                                    \
-        typedef BT_ BT;                                  \
-        static constexpr int BMODE = BM_, EPI = EP_;     \
-        static constexpr bool TA = TA_, TB = TB_;        \
+#define BH_INST(I, T_, BT_, BM_, EP_, TA_, TB_, DST_)          \
+    template <>                                                \
+    struct Inst<I> {                                           \
+        typedef T_ T;                                          \
+        typedef BT_ BT;                                        \
+        static constexpr int BMODE = BM_, EPI = EP_, DST = DST_; \
+        static constexpr bool TA = TA_, TB = TB_;              \
     };
-BH_INST(G64_PLAIN, double, double, B_PLAIN, E_PLAIN, false, false)
-BH_INST(G64_FOLDP_MASK, double, double, B_FOLD_PLUS, E_MASK, false, false)
-BH_INST(G64_FOLDM_MASK, double, double, B_FOLD_MINUS, E_MASK, false, false)
-BH_INST(G64_FOLDP, double, double, B_FOLD_PLUS, E_PLAIN, false, false)
-BH_INST(G64_FOLDM, double, double, B_FOLD_MINUS, E_PLAIN, false, false)
-BH_INST(G64_INV, double, double, B_PLAIN, E_INV_COMBINE, false, false)
-BH_INST(G64_ADD, double, double, B_ADD_F32, E_PLAIN, false, false)
-BH_INST(G64_ADD_INV, double, double, B_ADD_F32, E_INV_COMBINE, false, false)
-BH_INST(G32D_FOLDP_MASK, float, double, B_FOLD_PLUS, E_MASK, false, false)
-BH_INST(G32D_FOLDM_MASK, float, double, B_FOLD_MINUS, E_MASK, false, false)
-BH_INST(G32_TA, float, float, B_PLAIN, E_PLAIN, true, false)
-BH_INST(G32_TB, float, float, B_PLAIN, E_PLAIN, false, true)
-BH_INST(G32_FILTER, float, float, B_PLAIN, E_FILTER, false, false)
-BH_INST(G32_PLAIN, float, float, B_PLAIN, E_PLAIN, false, false)
+BH_INST(G64_PLAIN, double, double, B_PLAIN, E_PLAIN, false, false, DST_NONE)
+BH_INST(G64_FOLDP_MASK, double, double, B_FOLD_PLUS, E_MASK, false, false, DST_NONE)
+BH_INST(G64_FOLDM_MASK, double, double, B_FOLD_MINUS, E_MASK, false, false, DST_NONE)
+BH_INST(G64_FOLDP, double, double, B_FOLD_PLUS, E_PLAIN, false, false, DST_NONE)
+BH_INST(G64_FOLDM, double, double, B_FOLD_MINUS, E_PLAIN, false, false, DST_NONE)
+BH_INST(G64_INV, double, double, B_PLAIN, E_INV_COMBINE, false, false, DST_NONE)
+BH_INST(G64_ADD, double, double, B_ADD_F32, E_PLAIN, false, false, DST_NONE)
+BH_INST(G64_ADD_INV, double, double, B_ADD_F32, E_INV_COMBINE, false, false, DST_NONE)
+BH_INST(G32D_FOLDP_MASK, float, double, B_FOLD_PLUS, E_MASK, false, false, DST_NONE)
+BH_INST(G32D_FOLDM_MASK, float, double, B_FOLD_MINUS, E_MASK, false, false, DST_NONE)
+BH_INST(G32_TA, float, float, B_PLAIN, E_PLAIN, true, false, DST_NONE)
+BH_INST(G32_TB, float, float, B_PLAIN, E_PLAIN, false, true, DST_NONE)
+BH_INST(G32_FILTER, float, float, B_PLAIN, E_FILTER, false, false, DST_NONE)
+BH_INST(G32_PLAIN, float, float, B_PLAIN, E_PLAIN, false, false, DST_NONE)
+BH_INST(G32_DST_FWD, float, float, B_PLAIN, E_PLAIN, true, false, DST_FWD)
+BH_INST(G32_DST_INV, float, float, B_PLAIN, E_PLAIN, false, true, DST_INV)
 #undef BH_INST
 
 // tile shapes (rows x columns): 0: 64 x 256, 1: 128 x 128, 2: 64 x 64,
@@ -239,8 +290,8 @@ __device__ __forceinline__ void gemm_tile_is(const GemmArgs &g, int bx, int by, 
 {
     typedef Inst<I> In;
     typedef Shape<S> Sh;
-    gemm_tile<typename In::T, typename In::BT, In::BMODE, In::EPI, In::TA, In::TB, Sh::WR, Sh::WC, Sh::TM, Sh::TN>(
-        g, bx, by, bz);
+    gemm_tile<typename In::T, typename In::BT, In::BMODE, In::EPI, In::TA, In::TB, In::DST, Sh::WR, Sh::WC, Sh::TM,
+              Sh::TN>(g, bx, by, bz);
 }
 
 // Tiles are numbered row tile fastest (the row tiles that share a column
@@ -293,7 +344,9 @@ Op gemm_op(const GemmArgs &g, int inst)
     gemm_tile_rows_cols(tile, &rows, &cols);
     GemmArgs a = g;
     a.tile = tile;
-    const int nr = (g.R + rows - 1) / rows, nc = (g.M + cols - 1) / cols;
+    int nr = (g.R + rows - 1) / rows, nc = (g.M + cols - 1) / cols;
+    if (inst == G32_DST_FWD) nc = 2 * (((g.nfold + 1) / 2 + cols - 1) / cols);  // even and odd modes
+    if (inst == G32_DST_INV) nr = ((g.nfold + 1) / 2 + rows - 1) / rows;        // mirrored row pairs
     return Op::make(OP_GEMM, inst * 8 + tile, a, a.col_fastest ? nc : nr, a.col_fastest ? nr : nc,
                     g.batch > 0 ? g.batch : 1);
 }

@@ -6,6 +6,15 @@
 
 namespace bh {
 
+// Sine mode k (0-based) held in spectral slot m of the mixed solve's
+// transforms: the ceil(n/2) even modes first, then the odd ones, so that
+// the folded sine transform writes and reads each half as one block.
+inline int spec_mode(int m, int n)
+{
+    const int he = (n + 1) / 2;
+    return m < he ? 2 * m : 2 * (m - he) + 1;
+}
+
 struct ResidualArgs {
     const double *X; const float *dlt; const float *w; const float *rh;
     double *Xo; float *r;

@@ -42,6 +42,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     // GEMM tile shapes, overridable for tuning sweeps
     if (const char *e = std::getenv("BH_GEMM_TILE_DFT")) p->tile_dft = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_TILE_SPEC")) p->tile_spec = std::atoi(e);
+    if (const char *e = std::getenv("BH_GEMM_TILE_DSTF")) p->tile_dstf = std::atoi(e);
+    if (const char *e = std::getenv("BH_GEMM_TILE_DSTI")) p->tile_dsti = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_ORDER_DFT")) p->order_dft = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_ORDER_SPEC")) p->order_spec = std::atoi(e);
     if (const char *e = std::getenv("BH_DFT_KERNEL")) p->use_dft = std::atoi(e) != 0;
@@ -73,11 +75,11 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
         ALLOC(p->RH32, M * N);
         ALLOC(p->r32, PI);
         ALLOC(p->RT, PI);
-        ALLOC(p->WT, PI);
         ALLOC(p->CT, PI);
-        ALLOC(p->DT, PI);
-        ALLOC(p->D1, M * N);
-        ALLOC(p->Wf, M * N);
+        ALLOC(p->DT, 2 * PI);  // DT then WT, so that one product maps both back
+        p->WT = p->DT + PI;
+        ALLOC(p->D1, 2 * M * N);  // D1 then Wf
+        p->Wf = p->D1 + M * N;
     } else {
         ALLOC(p->RJ, M * N);
         ALLOC(p->RK, M * N);
@@ -89,8 +91,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     launch_fill(p->XJ, M * N, 0.0, s);
     launch_fill(p->XK, M * N, 0.0, s);
     if (precision == BH_PREC_MIXED) {
-        launch_fillf(p->D1, M * N, 0.f, s);
-        launch_fillf(p->Wf, M * N, 0.f, s);
+        launch_fillf(p->D1, 2 * M * N, 0.f, s);
     }
     hipError_t e = hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
@@ -226,27 +227,28 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
     ra.cold = tstep <= 1;
     out->push_back(residual_op(ra, N));
-    // sine transform: RT[L] (nri x nzi) = r[L]^T S
+    // folded sine transforms (spectral slots ordered by spec_mode):
+    // forward RT[L] (nri x nzi) = r[L]^T S, inverse dst[L] interior
+    // (nzi x nri) = S in[L]^T for nb consecutive modes of in and dst
     auto dst_fwd = [&](const float *r, float *dst) {
         GemmArgs a{};
         a.A = r; a.lda = nri; a.sA = PIm;
         a.B = o.S32; a.ldb = nzi; a.sB = 0;
         a.C = dst; a.ldc = nzi; a.sC = PIm;
-        a.R = nri; a.M = nzi; a.Kd = nzi; a.batch = N;
-        a.tile = p->tile_spec;
+        a.R = nri; a.M = nzi; a.Kd = nzi; a.batch = N; a.nfold = nzi;
+        a.tile = p->tile_dstf;
         a.col_fastest = p->order_spec;
-        out->push_back(gemm_op(a, G32_TA));
+        out->push_back(gemm_op(a, G32_DST_FWD));
     };
-    // inverse: full-plane dst[L] interior (nzi x nri) = S * in[L]^T
-    auto dst_inv = [&](const float *in, float *dst) {
+    auto dst_inv = [&](const float *in, float *dst, int nb) {
         GemmArgs a{};
         a.A = o.S32; a.lda = nzi; a.sA = 0;
         a.B = in; a.ldb = nzi; a.sB = PIm;
         a.C = dst + numr + 1; a.ldc = numr; a.sC = M;
-        a.R = nzi; a.M = nri; a.Kd = nzi; a.batch = N;
-        a.tile = p->tile_spec;
+        a.R = (nzi + 1) / 2; a.M = nri; a.Kd = nzi; a.batch = nb; a.nfold = nzi;
+        a.tile = p->tile_dsti;
         a.col_fastest = p->order_spec;
-        out->push_back(gemm_op(a, G32_TB));
+        out->push_back(gemm_op(a, G32_DST_INV));
     };
     dst_fwd(p->r32, p->RT);
     // 4. W = mu_z . (M_r RT), C = RT - W
@@ -263,8 +265,8 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     }
     // 5. (B_r + lambda_z) dhat = C, back to physical space
     out->push_back(tridiag_shift_op(p->CT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
-    dst_inv(p->DT, p->D1);
-    dst_inv(p->WT, p->Wf);
+    // delta = S dhat and w = S W in one launch: WT follows DT and Wf follows D1
+    dst_inv(p->DT, p->D1, 2 * N);
     // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
     double *Xcur = Xa, *Xnext = Xb;
     for (int it = 0; it < p->passes; ++it) {
@@ -273,7 +275,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         out->push_back(residual_op(ra, N));
         dst_fwd(p->r32, p->RT);
         out->push_back(tridiag_shift_op(p->RT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
-        dst_inv(p->DT, p->D1);
+        dst_inv(p->DT, p->D1, N);
         std::swap(Xcur, Xnext);
     }
     // 7. x = Xcur + D1 into Xa, the warm-start state, then the inverse DFT
@@ -564,6 +566,30 @@ extern "C" int bhx_gemm_tile(int fp32, const void *A, long lda, const void *B, l
     a.tile = tile;
     if (fp32) gemm32(a, B_PLAIN, E_PLAIN, ta != 0, tb != 0, false, 0);
     else gemm64(a, B_PLAIN, E_PLAIN, 0);
+    HIPCHK(hipDeviceSynchronize());
+    return BH_OK;
+}
+
+// Folded sine transform of length n over batch blocks with the folded table
+// Sf (ceil(n/2) x n, columns in spectral slot order); test hook.  Forward:
+// out[b] (nrows x n) = in[b]^T S with in[b] n x nrows; inverse: out[b]
+// (n x nrows) = S in[b]^T with in[b] nrows x n.
+extern "C" int bhx_dst32(const float *Sf, int n, const float *in, float *out, int nrows, int batch, int inverse,
+                         int tile)
+{
+    GemmArgs a{};
+    const long blk = (long)n * nrows;
+    a.nfold = n; a.batch = batch; a.tile = tile; a.Kd = n;
+    if (inverse) {
+        a.A = Sf; a.lda = n; a.B = in; a.ldb = n; a.sB = blk; a.C = out; a.ldc = nrows; a.sC = blk;
+        a.R = (n + 1) / 2; a.M = nrows;
+        launch_op(gemm_op(a, G32_DST_INV), 0);
+    } else {
+        a.A = in; a.lda = nrows; a.sA = blk; a.B = Sf; a.ldb = n; a.C = out; a.ldc = n; a.sC = blk;
+        a.R = nrows; a.M = n;
+        launch_op(gemm_op(a, G32_DST_FWD), 0);
+    }
+    HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     return BH_OK;
 }

@@ -120,7 +120,7 @@ struct bh_plan {
     float *D1 = nullptr, *Wf = nullptr;
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
     int passes = 1;                                // FP64 refinement passes (mixed)
-    int tile_dft = 2, tile_spec = 3;               // GEMM tile shapes, measured best
+    int tile_dft = 2, tile_spec = 2, tile_dstf = 2, tile_dsti = 2;  // GEMM tile shapes, measured best
     int order_dft = 0, order_spec = 0;             // GEMM tile numbering (GemmArgs::col_fastest)
     bool use_dft = true;                           // panel DFT kernel (bh_dft.hip) when numphi allows
     bool has_solution = false;

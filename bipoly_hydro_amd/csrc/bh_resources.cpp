@@ -210,7 +210,7 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
         b += 8 * 2 * rt * kg * 256 + (precision == BH_PREC_MIXED ? 4 * rt * kg * 256 : 0);
     }
     if (precision == BH_PREC_MIXED) {
-        b += 4 * (nh * nh + ns * ns + nzi * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
+        b += 4 * (nh * nh + ns * ns + (nzi + 1) / 2 * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
         b += 8 * N * (numr + 1);
     } else {
         b += 8 * 3 * ((size_t)d.nsteps_cold + d.nsteps_warm) * (N * nri + nzi);
@@ -434,7 +434,7 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     const size_t PI = (size_t)N * nri * nzi;  // interior values, all modes
     ALLOC(o->Dc32, (size_t)nh * nh);
     ALLOC(o->Ds32, (size_t)ns_ * ns_);
-    ALLOC(o->S32, (size_t)nzi * nzi);
+    ALLOC(o->S32, (size_t)((nzi + 1) / 2) * nzi);
     ALLOC(o->mu32[0], nzi);
     ALLOC(o->mu32[1], nzi);
     ALLOC(o->Mr[0], (size_t)N * nri * nri);
@@ -445,18 +445,24 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
 #undef ALLOC
     if ((rc = up(o->brb_d, std::vector<double>(h.brb))) != BH_OK) return rc;
     {
-        std::vector<float> Dc32((size_t)nh * nh), Ds32((size_t)ns_ * ns_), S((size_t)nzi * nzi);
+        std::vector<float> Dc32((size_t)nh * nh), Ds32((size_t)ns_ * ns_), S((size_t)((nzi + 1) / 2) * nzi);
         for (int k = 0; k < nh; ++k)
             for (int n = 0; n < nh; ++n)
                 Dc32[(size_t)k * nh + n] = (float)std::cos(2.0 * pi * (double)(((long)k * n) % N) / N);
         for (int k = 0; k < ns_; ++k)
             for (int n = 0; n < ns_; ++n)
                 Ds32[(size_t)k * ns_ + n] = (float)-std::sin(2.0 * pi * (double)(((long)(k + 1) * (n + 1)) % N) / N);
+        // The sine basis of B_z, S[a][k] = sqrt(2/(n+1)) sin(pi (a+1)(k+1)/(n+1)),
+        // n = nzi, kept folded: S[n-1-a][k] = (-1)^k S[a][k], so the rows
+        // a < ceil(n/2) hold it all.  Spectral storage orders the modes even
+        // k first: slot m holds k = spec_mode(m) (zeros of the table exact).
+        const int he = (nzi + 1) / 2;
         const double sn = std::sqrt(2.0 / (nzi + 1));
-        for (int a = 0; a < nzi; ++a)
-            for (int b = 0; b < nzi; ++b) {
-                long ab = ((long)(a + 1) * (b + 1)) % (2 * (nzi + 1));
-                S[(size_t)a * nzi + b] = (float)(sn * std::sin(pi * (double)ab / (nzi + 1)));
+        for (int a = 0; a < he; ++a)
+            for (int m = 0; m < nzi; ++m) {
+                const int k = spec_mode(m, nzi);
+                long ab = ((long)(a + 1) * (k + 1)) % (2 * (nzi + 1));
+                S[(size_t)a * nzi + m] = ab % (nzi + 1) == 0 ? 0.f : (float)(sn * std::sin(pi * (double)ab / (nzi + 1)));
             }
         if ((rc = upf(o->Dc32, std::move(Dc32))) != BH_OK || (rc = upf(o->Ds32, std::move(Ds32))) != BH_OK ||
             (rc = upf(o->S32, std::move(S))) != BH_OK)
@@ -465,26 +471,28 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
         for (int k = 0; k < nzi; ++k) lamz[k] = 2.0 * g.gamma * (1.0 - std::cos(pi * (k + 1) / (nzi + 1)));
         for (int w = 0; w < 2; ++w) {
             std::vector<float> mu(nzi);
-            for (int k = 0; k < nzi; ++k) {
+            for (int m = 0; m < nzi; ++m) {
+                const double lk = lamz[spec_mode(m, nzi)];
                 double mm = 1.0;
-                for (double d : h.dt[w]) mm *= (d - lamz[k]) / (d + lamz[k]);
-                mu[k] = (float)mm;
+                for (double d : h.dt[w]) mm *= (d - lk) / (d + lk);
+                mu[m] = (float)mm;
             }
             if ((rc = upf(o->mu32[w], std::move(mu))) != BH_OK) return rc;
         }
-        // LU factors of B_r(L) + lambda_z(k), layout [L][J'][k]
+        // LU factors of B_r(L) + lambda_z(k), layout [L][J'][m], k = spec_mode(m)
         std::vector<float> ib(PI), gm(PI);
         for (int L = 0; L < N; ++L) {
             const double *b = &h.brb[(size_t)L * (numr + 1)];
-            for (int k = 0; k < nzi; ++k) {
-                size_t base = (size_t)L * nri * nzi + k;
-                double bet = b[2] + lamz[k];
+            for (int m = 0; m < nzi; ++m) {
+                const double lk = lamz[spec_mode(m, nzi)];
+                size_t base = (size_t)L * nri * nzi + m;
+                double bet = b[2] + lk;
                 ib[base] = (float)(1.0 / bet);
                 gm[base] = 0.f;
                 for (int j = 1; j < nri; ++j) {
                     int J = j + 2;
                     double gg = h.alphar[J - 1] / bet;
-                    bet = (b[J] + lamz[k]) - h.betar[J] * gg;
+                    bet = (b[J] + lk) - h.betar[J] * gg;
                     ib[base + (size_t)j * nzi] = (float)(1.0 / bet);
                     gm[base + (size_t)j * nzi] = (float)gg;
                 }

@@ -72,3 +72,42 @@ def test_gemm_tiles_and_transposes(tile, fp32, ta, tb):
     ref = A.astype(np.float64) @ B.astype(np.float64)
     tol = 1e-4 * np.sqrt(K) if fp32 else 1e-12 * K
     np.testing.assert_allclose(dC.cpu().numpy(), ref, rtol=0, atol=tol)
+
+
+def _sine_basis(n):
+    a = np.arange(1, n + 1, dtype=np.float64)
+    return np.sqrt(2.0 / (n + 1)) * np.sin(np.pi * np.outer(a, a) / (n + 1))
+
+
+def _spec_modes(n):
+    he = (n + 1) // 2
+    return np.array([2 * m if m < he else 2 * (m - he) + 1 for m in range(n)])
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("n,nrows", [(128, 256), (64, 100), (33, 70), (7, 20)])
+def test_folded_sine_transform_matches_dense(tile, n, nrows):
+    """The folded DST-I products (even/odd halves of half the length) against
+    the dense orthonormal sine matrix, both directions, and their round trip."""
+    torch = pytest.importorskip("torch")
+    L = _lib()
+    vp = ctypes.c_void_p
+    L.bhx_dst32.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    S = _sine_basis(n)
+    perm = _spec_modes(n)
+    Sf = np.ascontiguousarray(S[: (n + 1) // 2][:, perm]).astype(np.float32)
+    batch = 3
+    rng = np.random.default_rng(1000 * tile + n)
+    x = rng.standard_normal((batch, n, nrows)).astype(np.float32)
+    dS, dx = torch.from_numpy(Sf).cuda(), torch.from_numpy(x).cuda()
+    dy = torch.full((batch, nrows, n), np.nan, dtype=torch.float32, device="cuda")
+    assert L.bhx_dst32(dS.data_ptr(), n, dx.data_ptr(), dy.data_ptr(), nrows, batch, 0, tile) == 0
+    ref = np.einsum("bar,am->brm", x.astype(np.float64), S[:, perm])
+    tol = 2e-6 * np.sqrt(n) * np.abs(ref).max()
+    np.testing.assert_allclose(dy.cpu().numpy(), ref, rtol=0, atol=tol)
+    dz = torch.full((batch, n, nrows), np.nan, dtype=torch.float32, device="cuda")
+    assert L.bhx_dst32(dS.data_ptr(), n, dy.data_ptr(), dz.data_ptr(), nrows, batch, 1, tile) == 0
+    y = dy.cpu().numpy().astype(np.float64)
+    ref_inv = np.einsum("zm,brm->bzr", S[:, perm], y)
+    np.testing.assert_allclose(dz.cpu().numpy(), ref_inv, rtol=0, atol=tol)
+    np.testing.assert_allclose(dz.cpu().numpy(), x, rtol=0, atol=2 * tol)  # S is its own inverse
