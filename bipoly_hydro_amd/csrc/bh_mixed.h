@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include "bh_ops.h"
 
 namespace bh {
@@ -13,6 +15,15 @@ inline int spec_mode(int m, int n)
 {
     const int he = (n + 1) / 2;
     return m < he ? 2 * m : 2 * (m - he) + 1;
+}
+
+// Entry (a, k) of the orthonormal sine basis of tridiag(-1, 2, -1) of order
+// n: sqrt(2/(n+1)) sin(pi (a+1)(k+1)/(n+1)), exactly zero where the sine is.
+inline double sine_basis(int a, int k, int n)
+{
+    const long ab = ((long)(a + 1) * (k + 1)) % (2 * (n + 1));
+    if (ab % (n + 1) == 0) return 0.0;
+    return std::sqrt(2.0 / (n + 1)) * std::sin(3.141592653589793238 * (double)ab / (n + 1));
 }
 
 struct ResidualArgs {
@@ -46,6 +57,7 @@ struct MrArgs {
     int numr, nri, nsteps, mode0, nmodes;
     double *scratch_v, *scratch_u, *scratch_g;  // nmodes*nri*nri each
     float *Mr;                                   // [numphi][nri][nri]
+    int frag;  // store M_r(L) in MFMA A-fragment order: [row tile][k group][lane][4]
 };
 void launch_build_mr(const MrArgs &a, hipStream_t s);
 void launch_fillf(float *p, size_t n, float v, hipStream_t s);

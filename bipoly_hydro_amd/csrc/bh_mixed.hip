@@ -215,7 +215,14 @@ __global__ void build_mr_kernel(MrArgs a)
         for (int j = nri - 2; j >= 0; --j) v[j * st] = v[j * st] - gm[(j + 1) * st] * v[(j + 1) * st];
     }
     float *out = a.Mr + (long)L * nri * nri;
-    for (int j = 0; j < nri; ++j) out[(long)j * nri + jc] = (float)v[j * st];
+    if (a.frag) {
+        // element (row j, k = jc): lane (j % 16) + 16 ((jc % 16) / 4), slot jc % 4
+        const long kgrp = jc / 16, lk = ((jc % 16) / 4) * 16, g = jc % 4, nkg = nri / 16;
+        for (int j = 0; j < nri; ++j)
+            out[(((long)(j / 16) * nkg + kgrp) * 64 + (j % 16) + lk) * 4 + g] = (float)v[j * st];
+    } else {
+        for (int j = 0; j < nri; ++j) out[(long)j * nri + jc] = (float)v[j * st];
+    }
 }
 
 void launch_build_mr(const MrArgs &a, hipStream_t s)

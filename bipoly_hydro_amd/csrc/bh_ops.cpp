@@ -49,6 +49,7 @@ void launch_op(const Op &op, hipStream_t s)
     if (kernels_launch(op, s)) return;
     if (mixed_launch(op, s)) return;
     if (density_launch(op, s)) return;
+    if (spectral_launch(op, s)) return;
     (void)dft_launch(op, s);
 }
 
@@ -59,6 +60,7 @@ void launch_op_grouped(int kind, int variant, const void *d_args, const GroupSch
     if (kernels_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (mixed_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (density_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
+    if (spectral_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     (void)dft_launch_grouped(kind, variant, d_args, gs, nblocks, s);
 }
 
@@ -70,6 +72,7 @@ int op_blocks_per_cu(int kind, int variant)
     if (mixed_occupancy(kind, variant, &b)) return b;
     if (density_occupancy(kind, variant, &b)) return b;
     if (dft_occupancy(kind, variant, &b)) return b;
+    if (spectral_occupancy(kind, variant, &b)) return b;
     return 0;
 }
 

@@ -103,6 +103,7 @@ enum OpKind {
     OP_TRIDIAG,
     OP_DENSITY,       // bh_density.hip
     OP_DFT,           // bh_dft.hip
+    OP_SPECTRAL,      // bh_spectral.hip
     OP_NKINDS
 };
 
@@ -173,5 +174,9 @@ bool density_occupancy(int kind, int variant, int *blocks);
 bool dft_launch(const Op &op, hipStream_t s);
 bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s);
 bool dft_occupancy(int kind, int variant, int *blocks);
+bool spectral_launch(const Op &op, hipStream_t s);
+bool spectral_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
+                             hipStream_t s);
+bool spectral_occupancy(int kind, int variant, int *blocks);
 
 }  // namespace bh

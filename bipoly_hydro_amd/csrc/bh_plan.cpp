@@ -5,6 +5,7 @@
 #include "bh_dft.h"
 #include "bh_gemm.h"
 #include "bh_mixed.h"
+#include "bh_spectral.h"
 
 #include <hip/hip_runtime.h>
 
@@ -74,10 +75,12 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
         const size_t PI = (size_t)N * nri * nzi;
         ALLOC(p->RH32, M * N);
         ALLOC(p->r32, PI);
-        ALLOC(p->RT, PI);
-        ALLOC(p->CT, PI);
-        ALLOC(p->DT, 2 * PI);  // DT then WT, so that one product maps both back
-        p->WT = p->DT + PI;
+        if (!p->ops->spec_fused) {  // spectral blocks between the separate kernels
+            ALLOC(p->RT, PI);
+            ALLOC(p->CT, PI);
+            ALLOC(p->DT, 2 * PI);  // DT then WT, so that one product maps both back
+            p->WT = p->DT + PI;
+        }
         ALLOC(p->D1, 2 * M * N);  // D1 then Wf
         p->Wf = p->D1 + M * N;
     } else {
@@ -250,32 +253,45 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.col_fastest = p->order_spec;
         out->push_back(gemm_op(a, G32_DST_INV));
     };
-    dst_fwd(p->r32, p->RT);
-    // 4. W = mu_z . (M_r RT), C = RT - W
-    {
-        GemmArgs a{};
-        a.A = o.Mr[w]; a.lda = nri; a.sA = (long)nri * nri;
-        a.B = p->RT; a.ldb = nzi; a.sB = PIm;
-        a.C = p->WT; a.ldc = nzi; a.sC = PIm;
-        a.R = nri; a.M = nzi; a.Kd = nri; a.batch = N;
-        a.mu = o.mu32[w]; a.Rin = p->RT; a.C2 = p->CT;
-        a.tile = p->tile_spec;
-        a.col_fastest = p->order_spec;
-        out->push_back(gemm_op(a, G32_FILTER));
-    }
-    // 5. (B_r + lambda_z) dhat = C, back to physical space
-    out->push_back(tridiag_shift_op(p->CT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
-    // delta = S dhat and w = S W in one launch: WT follows DT and Wf follows D1
-    dst_inv(p->DT, p->D1, 2 * N);
-    // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
     double *Xcur = Xa, *Xnext = Xb;
+    if (o.spec_fused) {
+        // 4-5 in one kernel per mode, the block kept in LDS
+        SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.tri_gam, o.betar, p->D1, p->Wf,
+                    numr, numz};
+        out->push_back(spectral_op(sa, N, nzi));
+    } else {
+        dst_fwd(p->r32, p->RT);
+        // 4. W = mu_z . (M_r RT), C = RT - W
+        {
+            GemmArgs a{};
+            a.A = o.Mr[w]; a.lda = nri; a.sA = (long)nri * nri;
+            a.B = p->RT; a.ldb = nzi; a.sB = PIm;
+            a.C = p->WT; a.ldc = nzi; a.sC = PIm;
+            a.R = nri; a.M = nzi; a.Kd = nri; a.batch = N;
+            a.mu = o.mu32[w]; a.Rin = p->RT; a.C2 = p->CT;
+            a.tile = p->tile_spec;
+            a.col_fastest = p->order_spec;
+            out->push_back(gemm_op(a, G32_FILTER));
+        }
+        // 5. (B_r + lambda_z) dhat = C, back to physical space; delta = S dhat
+        //    and w = S W in one launch: WT follows DT and Wf follows D1
+        out->push_back(tridiag_shift_op(p->CT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
+        dst_inv(p->DT, p->D1, 2 * N);
+    }
+    // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
     for (int it = 0; it < p->passes; ++it) {
         ra.X = Xcur; ra.dlt = p->D1; ra.w = p->Wf; ra.Xo = Xnext; ra.r = p->r32;
         ra.cold = (tstep <= 1 && it == 0);  // Xcur = Xa until the first pass writes Xnext
         out->push_back(residual_op(ra, N));
-        dst_fwd(p->r32, p->RT);
-        out->push_back(tridiag_shift_op(p->RT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
-        dst_inv(p->DT, p->D1, N);
+        if (o.spec_fused) {
+            SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.tri_gam, o.betar, p->D1, nullptr,
+                        numr, numz};
+            out->push_back(spectral_op(sa, N, nzi));
+        } else {
+            dst_fwd(p->r32, p->RT);
+            out->push_back(tridiag_shift_op(p->RT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
+            dst_inv(p->DT, p->D1, N);
+        }
         std::swap(Xcur, Xnext);
     }
     // 7. x = Xcur + D1 into Xa, the warm-start state, then the inverse DFT

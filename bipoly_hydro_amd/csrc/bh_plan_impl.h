@@ -84,6 +84,9 @@ struct ModeOps {
     float *Dc32 = nullptr, *Ds32 = nullptr, *S32 = nullptr, *mu32[2] = {}, *Mr[2] = {};
     float *tri_ibet = nullptr, *tri_gam = nullptr;
     double *brb_d = nullptr;
+    // fused per-mode spectral step (then Mr is in A-fragment order)
+    bool spec_fused = false;
+    float *sfb = nullptr, *sfa = nullptr;
 };
 
 // Shared resources from the per-process cache (created on first use; kept

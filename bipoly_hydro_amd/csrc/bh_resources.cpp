@@ -5,6 +5,7 @@
 #include "bh_plan_impl.h"
 #include "bh_dft.h"
 #include "bh_mixed.h"
+#include "bh_spectral.h"
 
 #include <hip/hip_runtime.h>
 
@@ -189,8 +190,9 @@ std::string tables_key(const bh_grid_desc &d, int device)
 std::string modeops_key(const bh_grid_desc &d, int precision, int device)
 {
     char b[200];
-    std::snprintf(b, sizeof b, "%d|%d|%d|%d|%.17g|%d|%d|%d|%d", device, d.numr, d.numz, d.numphi, d.dr, d.mmax,
-                  d.nsteps_cold, d.nsteps_warm, precision);
+    std::snprintf(b, sizeof b, "%d|%d|%d|%d|%.17g|%d|%d|%d|%d|%d", device, d.numr, d.numz, d.numphi, d.dr, d.mmax,
+                  d.nsteps_cold, d.nsteps_warm, precision,
+                  precision == BH_PREC_MIXED && spectral_fused_ok(d.numr - 2, d.numz - 2) ? 1 : 0);
     return b;
 }
 
@@ -211,6 +213,7 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     }
     if (precision == BH_PREC_MIXED) {
         b += 4 * (nh * nh + ns * ns + (nzi + 1) / 2 * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
+        if (spectral_fused_ok((int)nri, (int)nzi)) b += 4 * 2 * (nzi / 2) * nzi;
         b += 8 * N * (numr + 1);
     } else {
         b += 8 * 3 * ((size_t)d.nsteps_cold + d.nsteps_warm) * (N * nri + nzi);
@@ -442,8 +445,18 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     ALLOC(o->tri_ibet, PI);
     ALLOC(o->tri_gam, PI);
     ALLOC(o->brb_d, (size_t)N * (numr + 1));
+    o->spec_fused = spectral_fused_ok(nri, nzi);
+    if (o->spec_fused) {
+        ALLOC(o->sfb, (size_t)(nzi / 2) * nzi);
+        ALLOC(o->sfa, (size_t)(nzi / 2) * nzi);
+    }
 #undef ALLOC
     if ((rc = up(o->brb_d, std::vector<double>(h.brb))) != BH_OK) return rc;
+    if (o->spec_fused) {
+        std::vector<float> fb, fa;
+        spectral_fragments(nzi, &fb, &fa);
+        if ((rc = upf(o->sfb, std::move(fb))) != BH_OK || (rc = upf(o->sfa, std::move(fa))) != BH_OK) return rc;
+    }
     {
         std::vector<float> Dc32((size_t)nh * nh), Ds32((size_t)ns_ * ns_), S((size_t)((nzi + 1) / 2) * nzi);
         for (int k = 0; k < nh; ++k)
@@ -457,13 +470,8 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
         // a < ceil(n/2) hold it all.  Spectral storage orders the modes even
         // k first: slot m holds k = spec_mode(m) (zeros of the table exact).
         const int he = (nzi + 1) / 2;
-        const double sn = std::sqrt(2.0 / (nzi + 1));
         for (int a = 0; a < he; ++a)
-            for (int m = 0; m < nzi; ++m) {
-                const int k = spec_mode(m, nzi);
-                long ab = ((long)(a + 1) * (k + 1)) % (2 * (nzi + 1));
-                S[(size_t)a * nzi + m] = ab % (nzi + 1) == 0 ? 0.f : (float)(sn * std::sin(pi * (double)ab / (nzi + 1)));
-            }
+            for (int m = 0; m < nzi; ++m) S[(size_t)a * nzi + m] = (float)sine_basis(a, spec_mode(m, nzi), nzi);
         if ((rc = upf(o->Dc32, std::move(Dc32))) != BH_OK || (rc = upf(o->Ds32, std::move(Ds32))) != BH_OK ||
             (rc = upf(o->S32, std::move(S))) != BH_OK)
             return rc;
@@ -519,6 +527,7 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
                 ma.numr = numr; ma.nri = nri; ma.nsteps = (int)h.dt[w].size();
                 ma.mode0 = m0; ma.nmodes = std::min(chunk, N - m0);
                 ma.scratch_v = sv; ma.scratch_u = su; ma.scratch_g = sg; ma.Mr = o->Mr[w];
+                ma.frag = o->spec_fused ? 1 : 0;
                 launch_build_mr(ma, s);
             }
         }
@@ -580,7 +589,11 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
     const size_t NB16 = 16 * ((nri + 15) / 16), W = (nri + 31) / 32;
     size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf +
                     mm * nzi * NB16 * 4 + mm * topsym_splits(d.numz) * W * NB16 * 4);
-    if (precision == BH_PREC_MIXED) b += 4 * (M * N + 5 * N * nri * nzi + 2 * M * N);
+    if (precision == BH_PREC_MIXED) {
+        // RH32, r32, D1 + Wf; RT, CT, DT + WT only for the separate-kernel step
+        const bool fused = spectral_fused_ok((int)nri, (int)nzi);
+        b += 4 * (M * N + (fused ? 1 : 5) * N * nri * nzi + 2 * M * N);
+    }
     else b += 8 * 2 * M * N;
     return b;
 }

@@ -1,0 +1,38 @@
+// bh_spectral.h -- the fused per-mode spectral step of the mixed solve
+// (bh_spectral.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "bh_ops.h"
+
+namespace bh {
+
+// One workgroup per azimuthal mode L does, with the mode's data in LDS:
+//   RT = r^T S                     folded sine transform (FP32 MFMA)
+//   W = mu . (M_r RT), C = RT - W  when mr is given (FP32 MFMA)
+//   (B_r + lambda) D = C           Thomas sweeps per spectral column (FP32)
+//   delta = S D^T, w = S W^T       folded inverse transforms into planes
+// r is compact [L][nzi][nri]; delta and w are full numr x numz planes whose
+// interior is written (faces untouched).
+struct SpecArgs {
+    const float *r;
+    const float *sfb, *sfa;       // folded sine table in B- and A-fragment order
+    const float *mr;              // [L] M_r in A-fragment order, or null
+    const float *mu;              // [nzi] filter factors, spectral slot order
+    const float *ibet, *gam;      // [L][nri][nzi] LU factors of B_r + lambda
+    const double *betar;
+    float *dlt, *w;
+    int numr, numz;
+};
+
+// Sizes the fused kernel takes: nri a multiple of 32 up to 256 and nzi one
+// of 32, 64, 128; the environment variable BH_SPEC_KERNEL=0 turns it off.
+bool spectral_fused_ok(int nri, int nzi);
+Op spectral_op(const SpecArgs &a, int nplanes, int nzi);
+// The folded sine table (ceil(n/2) x n, spectral slot columns) in the two
+// fragment orders the kernel reads.
+void spectral_fragments(int nzi, std::vector<float> *sfb, std::vector<float> *sfa);
+
+}  // namespace bh
