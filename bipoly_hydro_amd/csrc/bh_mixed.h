@@ -26,6 +26,7 @@ inline double sine_basis(int a, int k, int n)
     return std::sqrt(2.0 / (n + 1)) * std::sin(3.141592653589793238 * (double)ab / (n + 1));
 }
 
+// dlt, w and Xo are given together (the refinement pass) or all null
 struct ResidualArgs {
     const double *X; const float *dlt; const float *w; const float *rh;
     double *Xo; float *r;

@@ -35,9 +35,13 @@ namespace bh {
 // Tile: 256 consecutive J of one plane, walking RES_ROWS rows of K with the
 // rows K-1, K, K+1 of the column kept in registers (one new value of x per
 // row), the J coefficients loaded once.
-constexpr int RES_ROWS = 16;
+// The variant fixes at compile time whether delta, w and Xo are present
+// (UPD: the refinement pass) and whether X is a cold start (COLD), so that
+// the row loop carries no pointer tests.
+constexpr int RES_ROWS = 8;
 
-__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
+template <bool UPD, bool COLD>
+__device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz)
 {
     const int numr = a.numr, nri = numr - 2, nzi = a.numz - 2;
     const int L = bz;
@@ -46,21 +50,21 @@ __device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int
     if (jr >= nri) return;
     const int J = jr + 2;
     const int K0 = by * RES_ROWS + 2, K1 = min(K0 + RES_ROWS, nzi + 2);
-    const double *X = a.X + L * plane;
-    const float *dl = a.dlt ? a.dlt + L * plane : nullptr;
-    auto val = [&](long i) { return dl ? X[i] + (double)dl[i] : X[i]; };
+    const double *__restrict__ X = a.X + L * plane;
+    const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
+    const float *__restrict__ rh = a.rh + L * plane;
+    const float *__restrict__ w = UPD ? a.w + L * plane : nullptr;
+    double *__restrict__ Xo = UPD ? a.Xo + L * plane : nullptr;
+    float *__restrict__ r32 = a.r + (long)L * nri * nzi;
     const double brbJ = a.brb[(long)L * (numr + 1) + J], aJ = a.alphar[J], bJ = a.betar[J];
-    const float *rh = a.rh + L * plane;
-    const float *w = a.w ? a.w + L * plane : nullptr;
-    double *Xo = a.Xo ? a.Xo + L * plane : nullptr;
-    float *r32 = a.r ? a.r + (long)L * nri * nzi : nullptr;
     // cold start: the interior of X is zero and dlt is zero on the faces, so
     // interior values are dlt alone and face values X alone
-    const bool cold = a.cold != 0;
-    auto inner = [&](long i) { return cold ? (dl ? (double)dl[i] : 0.0) : val(i); };
-    auto face = [&](long i) { return cold ? X[i] : val(i); };
+    auto val = [&](long i) { return UPD ? X[i] + (double)dl[i] : X[i]; };
+    auto inner = [&](long i) { return COLD ? (UPD ? (double)dl[i] : 0.0) : val(i); };
+    auto face = [&](long i) { return COLD ? X[i] : val(i); };
     long c = (long)(K0 - 1) * numr + (J - 1);
     double xs = (K0 - 1 == 1) ? face(c - numr) : inner(c - numr), xc = inner(c);
+#pragma unroll 4
     for (int K = K0; K < K1; ++K, c += numr) {
         const double xn = (K + 1 == a.numz) ? face(c + numr) : inner(c + numr);
         const double xe = (J + 1 == numr) ? face(c + 1) : inner(c + 1);
@@ -68,11 +72,24 @@ __device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int
         if (J > 2) Ax += bJ * inner(c - 1);
         Ax += 2.0 * a.gamma * xc - a.gamma * (xs + xn);
         double r = -a.four_pi * (double)rh[c] - Ax;
-        if (w) r -= (double)w[c];
-        if (Xo) Xo[c] = xc;
-        if (r32) r32[(long)(K - 2) * nri + jr] = (float)r;
+        if (UPD) {
+            r -= (double)w[c];
+            Xo[c] = xc;
+        }
+        r32[(long)(K - 2) * nri + jr] = (float)r;
         xs = xc;
         xc = xn;
+    }
+}
+
+// the refinement form (delta, w and Xo given) or the first residual, cold or warm
+__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
+{
+    switch ((a.dlt ? 2 : 0) + (a.cold ? 1 : 0)) {
+    case 0: residual_tile_v<false, false>(a, bx, by, bz); break;
+    case 1: residual_tile_v<false, true>(a, bx, by, bz); break;
+    case 2: residual_tile_v<true, false>(a, bx, by, bz); break;
+    default: residual_tile_v<true, true>(a, bx, by, bz); break;
     }
 }
 

@@ -27,7 +27,7 @@ struct SpecArgs {
     int numr, numz;
 };
 
-// Sizes the fused kernel takes: nri a multiple of 32 up to 256 and nzi one
+// Sizes the fused kernel takes: nri a multiple of 64 up to 256 and nzi one
 // of 32, 64, 128; the environment variable BH_SPEC_KERNEL=0 turns it off.
 bool spectral_fused_ok(int nri, int nzi);
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi);

@@ -44,47 +44,87 @@ struct Shape {
     static constexpr int P = NZ + 4;      // LDS row pitch (floats)
 };
 
+// Global-memory views of the argument pointers: loads through them are
+// vector-memory loads with their own wait counter (generic pointers become
+// flat loads, whose waits also drain the LDS reads).
+#define BH_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const BH_GLOBAL T *gld(const void *p)
+{
+    return (const BH_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ BH_GLOBAL T *gst(void *p)
+{
+    return (BH_GLOBAL T *)p;
+}
+
 template <int NZ>
 __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int L)
 {
     typedef Shape<NZ> S;
     constexpr int P = S::P;
-    const int numr = a.numr, nri = numr - 2;
+    // numr passes through an empty asm so that the compiler treats the sizes
+    // as new in every tile of the persistent loop: otherwise it hoists
+    // hundreds of tile-invariant address offsets out of the loop and spills
+    int numr = a.numr;
+    asm volatile("" : "+s"(numr));
+    const int nri = numr - 2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int t = lane & 15, q = lane >> 4;
     const int j0 = wave * 32;
     const bool active = j0 < nri;
+    float *pq = buf + SPEC_ROWS * P;  // [2][4][NZ] segment maps of the Thomas sweeps
+    float *bl = pq + 8 * NZ;          // [SPEC_ROWS] sub-diagonal a_j = betar(j + 2), a_0 = 0
+    // its load flies during the forward transform; LDS is written after it
+    const double blv = tid > 0 && tid < nri ? gld<double>(a.betar)[tid + 2] : 0.0;
 
     // ---- 1. RT = r^T S, folded: even columns from u = r_a + r_{n-1-a}, odd
     //         from v = r_a - r_{n-1-a}.  Row t of row block b is j0 + 2t + b,
-    //         so one 8-byte load gives a lane both blocks' A values.
+    //         so one 8-byte load gives a lane both blocks' A values.  The r
+    //         planes and table fragments of k group kg + 1 load while group
+    //         kg is multiplied.
     if (active) {
-        const float *r = a.r + (long)L * NZ * nri + j0 + 2 * t;
+        const BH_GLOBAL float *r = gld<float>(a.r) + (long)L * NZ * nri + j0 + 2 * t;
+        const BH_GLOBAL f4 *sb = gld<f4>(a.sfb) + lane;
         f4 acc[2][S::CT];
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int c = 0; c < S::CT; ++c) acc[b][c] = (f4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll 1
-        for (int kg = 0; kg < S::KG; ++kg) {
-            f2 u[4], v[4];
+        f2 xr[2][4], yr[2][4];
+        f4 bc[2][S::CT];
+        auto load = [&](int kg, int slot) {
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int pa = 16 * kg + 4 * q + g;
-                const f2 x = *(const f2 *)(r + (long)pa * nri);
-                const f2 y = *(const f2 *)(r + (long)(NZ - 1 - pa) * nri);
-                u[g] = x + y;
-                v[g] = x - y;
+                xr[slot][g] = *(const BH_GLOBAL f2 *)(r + (long)pa * nri);
+                yr[slot][g] = *(const BH_GLOBAL f2 *)(r + (long)(NZ - 1 - pa) * nri);
             }
-            const f4 *sb = (const f4 *)a.sfb + kg * 64 + lane;
 #pragma unroll
-            for (int c = 0; c < S::CT; ++c) {
-                const f4 bf = sb[c * S::KG * 64];
+            for (int c = 0; c < S::CT; ++c) bc[slot][c] = sb[(c * S::KG + kg) * 64];
+        };
+        load(0, 0);
+#pragma unroll
+        for (int kg = 0; kg < S::KG; ++kg) {
+            const int cur = kg & 1;
+            if (kg + 1 < S::KG) load(kg + 1, cur ^ 1);
+            // the loads above stay ahead of this group's products (the
+            // scheduler would otherwise sink them next to their use)
+            __builtin_amdgcn_sched_barrier(0);
+            f2 u[4], v[4];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                u[g] = xr[cur][g] + yr[cur][g];
+                v[g] = xr[cur][g] - yr[cur][g];
+            }
+#pragma unroll
+            for (int c = 0; c < S::CT; ++c)
 #pragma unroll
                 for (int g = 0; g < 4; ++g)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) acc[b][c] = mma(c < S::CT / 2 ? u[g][b] : v[g][b], bf[g], acc[b][c]);
-            }
+                    for (int b = 0; b < 2; ++b)
+                        acc[b][c] = mma(c < S::CT / 2 ? u[g][b] : v[g][b], bc[cur][c][g], acc[b][c]);
         }
 #pragma unroll
         for (int b = 0; b < 2; ++b)
@@ -93,6 +133,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
 #pragma unroll
                 for (int e = 0; e < 4; ++e) buf[(j0 + 2 * (4 * q + e) + b) * P + 16 * c + t] = acc[b][c][e];
     }
+    if (tid < SPEC_ROWS) bl[tid] = (float)blv;
     __syncthreads();
 
     // ---- 2. W = mu . (M_r RT), C = RT - W in place; W (rows j0 + 16b + 4q + e)
@@ -105,17 +146,24 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     if (a.mr) {
         if (active) {
             const int nkg = nri / 16;
-            const f4 *mr = (const f4 *)(a.mr + (long)L * nri * nri) + lane;
+            const BH_GLOBAL f4 *mr = gld<f4>(a.mr) + (long)L * nri * nri / 4 + lane;
             const long rstride = (long)nkg * 64;  // f4 per row tile
-            f4 an[2];
+            // M_r fragments two k groups ahead of the products
+            f4 an[2], an2[2];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) an[b] = mr[(2 * wave + b) * rstride];
+            for (int b = 0; b < 2; ++b) {
+                an[b] = mr[(2 * wave + b) * rstride];
+                an2[b] = mr[(2 * wave + b) * rstride + 64];
+            }
             for (int kg = 0; kg < nkg; ++kg) {
                 const f4 ac[2] = {an[0], an[1]};
-                if (kg + 1 < nkg) {
+                an[0] = an2[0];
+                an[1] = an2[1];
+                if (kg + 2 < nkg) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) an[b] = mr[(2 * wave + b) * rstride + (kg + 1) * 64];
+                    for (int b = 0; b < 2; ++b) an2[b] = mr[(2 * wave + b) * rstride + (kg + 2) * 64];
                 }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int g = 0; g < 4; ++g) {
                     const float *row = buf + (16 * kg + 4 * q + g) * P + t;
@@ -131,15 +179,16 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         }
         __syncthreads();
         if (active) {
+            const BH_GLOBAL float *mu = gld<float>(a.mu);
 #pragma unroll
             for (int c = 0; c < S::CT; ++c) {
-                const float mu = a.mu[16 * c + t];
+                const float m = mu[16 * c + t];
 #pragma unroll
                 for (int b = 0; b < 2; ++b)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
                         const int idx = (j0 + 16 * b + 4 * q + e) * P + 16 * c + t;
-                        const float wv = mu * wacc[b][c][e];
+                        const float wv = m * wacc[b][c][e];
                         wacc[b][c][e] = wv;
                         buf[idx] = buf[idx] - wv;
                     }
@@ -148,85 +197,135 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         __syncthreads();
     }
 
-    // ---- 3. (B_r(L) + lambda_m') D = C along j, one thread per column m'
-    //         (the recurrences of the separate Thomas kernel)
-    if (tid < NZ) {
-        const long base = (long)L * nri * NZ + tid;
-        const float *ib = a.ibet + base, *gm = a.gam + base;
-        float *col = buf + tid;
-        float y = 0.f;
-        for (int jb = 0; jb < nri; jb += 16) {
-            float iv[16];
+    // ---- 3. (B_r(L) + lambda_m') D = C along j for every column m', as the
+    //         separate Thomas kernel's recurrences (forward y_j = (c_j -
+    //         a_j y_{j-1}) ib_j, back x_j = y_j - g_{j+1} x_{j+1}), split into
+    //         nri/64 segments of 64 rows: a thread composes its segment's
+    //         affine map, the maps are chained through LDS, and the segment is
+    //         swept again from its true entry value.  Each thread issues all
+    //         64 factor loads of a sweep at once.
+    {
+        const int nseg = nri / 64;
+        const bool tri = tid < nseg * NZ;  // whole waves: nzi is a multiple of 32
+        const int m = tid % NZ, sg = tid / NZ, jlo = sg * 64;
+        const long fb = (long)L * nri * NZ + (long)jlo * NZ + m;
+        float *col = buf + jlo * P + m;
+        const float *al = bl + jlo;
+        float fv[64];
+        if (tri) {
+            const BH_GLOBAL float *ib = gld<float>(a.ibet) + fb;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) iv[u] = ib[(long)(jb + u) * NZ];
+            for (int u = 0; u < 64; ++u) fv[u] = ib[u * NZ];
+            __builtin_amdgcn_sched_barrier(0);  // all 64 loads in flight before the first use
+            float Pf = 0.f, Qf = 1.f;  // y_{jlo + 63} = Pf + Qf y_{jlo - 1}
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int j = jb + u;
-                const float aj = j > 0 ? (float)a.betar[j + 2] : 0.f;
-                y = (col[j * P] - aj * y) * iv[u];
-                col[j * P] = y;
+            for (int u = 0; u < 64; ++u) {
+                const float aj = al[u];
+                Pf = (col[u * P] - aj * Pf) * fv[u];
+                Qf = -aj * Qf * fv[u];
             }
+            pq[sg * NZ + m] = Pf;
+            pq[(4 + sg) * NZ + m] = Qf;
         }
-        float x = y;
-        for (int jb = nri - 2; jb >= 0; jb -= 16) {
-            float gv[16];
+        __syncthreads();
+        if (tri) {
+            float y = 0.f;  // y_{jlo - 1} (a_0 = 0 in the first segment)
+            for (int k = 0; k < sg; ++k) y = pq[k * NZ + m] + pq[(4 + k) * NZ + m] * y;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) gv[u] = jb - u >= 0 ? gm[(long)(jb - u + 1) * NZ] : 0.f;
+            for (int u = 0; u < 64; ++u) {
+                y = (col[u * P] - al[u] * y) * fv[u];
+                col[u * P] = y;
+            }
+            // back over the same rows, high j to low: fv[u] = g_{j+1} for
+            // j = jlo + 63 - u, none past the last row
+            const BH_GLOBAL float *gm = gld<float>(a.gam) + fb;
+            fv[0] = jlo + 64 < nri ? gm[64 * NZ] : 0.f;
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int j = jb - u;
-                if (j >= 0) {
-                    x = col[j * P] - gv[u] * x;
-                    col[j * P] = x;
-                }
+            for (int u = 1; u < 64; ++u) fv[u] = gm[(64 - u) * NZ];
+        }
+        __syncthreads();  // the forward maps are read before they are replaced
+        if (tri) {
+            float Pb = 0.f, Qb = 1.f;  // x_{jlo} = Pb + Qb x_{jlo + 64}
+#pragma unroll
+            for (int u = 0; u < 64; ++u) {
+                Pb = col[(63 - u) * P] - fv[u] * Pb;
+                Qb = -fv[u] * Qb;
+            }
+            pq[sg * NZ + m] = Pb;
+            pq[(4 + sg) * NZ + m] = Qb;
+        }
+        __syncthreads();
+        if (tri) {
+            float x = 0.f;  // x_{jlo + 64}; the last segment does not use it
+            for (int k = nseg - 1; k > sg; --k) x = pq[k * NZ + m] + pq[(4 + k) * NZ + m] * x;
+#pragma unroll
+            for (int u = 0; u < 64; ++u) {
+                x = col[(63 - u) * P] - fv[u] * x;
+                col[(63 - u) * P] = x;
             }
         }
     }
     __syncthreads();
 
     // ---- 4. out[z][j] = E + O, out[n-1-z][j] = E - O for z < n/2, with E
-    //         over the even spectral slots of the LDS block and O the odd
-    auto inverse = [&](float *out) {
+    //         over the even spectral slots of the LDS block and O the odd.
+    //         The z tiles go in passes of ZH so that the accumulators stay
+    //         small while W waits in registers.
+    auto inverse = [&](float *outp) {
         if (!active) return;
-        f4 ev[S::ZT][2], od[S::ZT][2];
-#pragma unroll
-        for (int i = 0; i < S::ZT; ++i)
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) {
-                ev[i][cc] = (f4){0.f, 0.f, 0.f, 0.f};
-                od[i][cc] = (f4){0.f, 0.f, 0.f, 0.f};
-            }
-        // k groups [0, CT/2) are the even slots, [CT/2, CT) the odd
-        auto kstep = [&](int kg, f4(&acc)[S::ZT][2]) {
-            f4 bf[2];
-#pragma unroll
-            for (int cc = 0; cc < 2; ++cc) bf[cc] = *(const f4 *)(buf + (j0 + 16 * cc + t) * P + 16 * kg + 4 * q);
-            const f4 *sa = (const f4 *)a.sfa + kg * 64 + lane;
-#pragma unroll
-            for (int i = 0; i < S::ZT; ++i) {
-                const f4 af = sa[i * S::CT * 64];
-#pragma unroll
-                for (int g = 0; g < 4; ++g)
-#pragma unroll
-                    for (int cc = 0; cc < 2; ++cc) acc[i][cc] = mma(af[g], bf[cc][g], acc[i][cc]);
-            }
-        };
+        constexpr int ZH = S::ZT > 1 ? S::ZT / 2 : 1;
+        BH_GLOBAL float *pl = gst<float>(outp) + (long)L * numr * a.numz + numr + 1 + j0 + t;
 #pragma unroll 1
-        for (int kg = 0; kg < S::CT / 2; ++kg) kstep(kg, ev);
-#pragma unroll 1
-        for (int kg = S::CT / 2; kg < S::CT; ++kg) kstep(kg, od);
-        float *pl = out + (long)L * numr * a.numz + numr + 1 + j0 + t;
+        for (int z0 = 0; z0 < S::ZT; z0 += ZH) {
+            f4 ev[ZH][2], od[ZH][2];
 #pragma unroll
-        for (int i = 0; i < S::ZT; ++i)
+            for (int i = 0; i < ZH; ++i)
 #pragma unroll
-            for (int cc = 0; cc < 2; ++cc)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int z = 16 * i + 4 * q + e;
-                    const float x = ev[i][cc][e], y = od[i][cc][e];
-                    pl[(long)z * numr + 16 * cc] = x + y;
-                    pl[(long)(NZ - 1 - z) * numr + 16 * cc] = x - y;
+                for (int cc = 0; cc < 2; ++cc) {
+                    ev[i][cc] = (f4){0.f, 0.f, 0.f, 0.f};
+                    od[i][cc] = (f4){0.f, 0.f, 0.f, 0.f};
                 }
+            // k groups [0, CT/2) are the even slots, [CT/2, CT) the odd; the
+            // table fragments of the next group load during this group's products
+            auto kstep = [&](const f4(&af)[ZH], int kg, f4(&acc)[ZH][2]) {
+                f4 bf[2];
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) bf[cc] = *(const f4 *)(buf + (j0 + 16 * cc + t) * P + 16 * kg + 4 * q);
+#pragma unroll
+                for (int i = 0; i < ZH; ++i)
+#pragma unroll
+                    for (int g = 0; g < 4; ++g)
+#pragma unroll
+                        for (int cc = 0; cc < 2; ++cc) acc[i][cc] = mma(af[i][g], bf[cc][g], acc[i][cc]);
+            };
+            const BH_GLOBAL f4 *sa = gld<f4>(a.sfa) + (long)z0 * S::CT * 64 + lane;
+            f4 afc[ZH], afn[ZH];
+#pragma unroll
+            for (int i = 0; i < ZH; ++i) afc[i] = sa[i * S::CT * 64];
+#pragma unroll 1
+            for (int kg = 0; kg < S::CT; ++kg) {
+                if (kg + 1 < S::CT) {
+#pragma unroll
+                    for (int i = 0; i < ZH; ++i) afn[i] = sa[(i * S::CT + kg + 1) * 64];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (kg < S::CT / 2) kstep(afc, kg, ev);
+                else kstep(afc, kg, od);
+#pragma unroll
+                for (int i = 0; i < ZH; ++i) afc[i] = afn[i];
+            }
+#pragma unroll
+            for (int i = 0; i < ZH; ++i)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int z = 16 * (z0 + i) + 4 * q + e;
+                        const float x = ev[i][cc][e], y = od[i][cc][e];
+                        pl[(long)z * numr + 16 * cc] = x + y;
+                        pl[(long)(NZ - 1 - z) * numr + 16 * cc] = x - y;
+                    }
+        }
     };
     inverse(a.dlt);
     if (a.mr) {
@@ -270,7 +369,12 @@ const void *spec_fn(int variant, bool grouped)
     }
 }
 
-int spec_lds_bytes(int variant) { return SPEC_ROWS * ((32 << variant) + 4) * (int)sizeof(float); }
+// the block, the segment maps of the Thomas sweeps and the sub-diagonal
+int spec_lds_bytes(int variant)
+{
+    const int nz = 32 << variant;
+    return (SPEC_ROWS * (nz + 4) + 8 * nz + SPEC_ROWS) * (int)sizeof(float);
+}
 
 int spec_variant(int nzi) { return nzi == 32 ? 0 : nzi == 64 ? 1 : nzi == 128 ? 2 : -1; }
 
@@ -280,7 +384,7 @@ bool spectral_fused_ok(int nri, int nzi)
 {
     const char *e = std::getenv("BH_SPEC_KERNEL");
     if (e && std::strcmp(e, "0") == 0) return false;
-    return nri > 0 && nri <= SPEC_ROWS && nri % 32 == 0 && spec_variant(nzi) >= 0;
+    return nri > 0 && nri <= SPEC_ROWS && nri % 64 == 0 && spec_variant(nzi) >= 0;
 }
 
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi)

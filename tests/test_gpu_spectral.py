@@ -14,8 +14,8 @@ from bipoly_hydro_amd import potential as P
 
 pytestmark = pytest.mark.gpu
 
-# nri = numr - 2 in {32, 64, 128, 192, 256}, nzi = numz - 2 in {32, 64, 128}
-GRIDS = [(34, 34, 16), (66, 66, 64), (130, 34, 32), (194, 66, 16), (258, 130, 16)]
+# nri = numr - 2 in {64, 128, 192, 256}, nzi = numz - 2 in {32, 64, 128}
+GRIDS = [(66, 34, 16), (66, 66, 64), (130, 34, 32), (194, 66, 16), (258, 130, 16)]
 
 
 def rel(a, b):
