@@ -62,14 +62,25 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     auto val = [&](long i) { return UPD ? X[i] + (double)dl[i] : X[i]; };
     auto inner = [&](long i) { return COLD ? (UPD ? (double)dl[i] : 0.0) : val(i); };
     auto face = [&](long i) { return COLD ? X[i] : val(i); };
+    // the lane-dependent cases (the east face at J = numr - 1, no west term
+    // at J = 2) are selects, not branches, so that the loads of a row issue
+    // together; every address read is inside the plane
+    const bool east_face = J + 1 == numr, has_west = J > 2;
     long c = (long)(K0 - 1) * numr + (J - 1);
     double xs = (K0 - 1 == 1) ? face(c - numr) : inner(c - numr), xc = inner(c);
 #pragma unroll 4
     for (int K = K0; K < K1; ++K, c += numr) {
         const double xn = (K + 1 == a.numz) ? face(c + numr) : inner(c + numr);
-        const double xe = (J + 1 == numr) ? face(c + 1) : inner(c + 1);
+        double xe;
+        if (COLD) {
+            const double xf = X[c - (J - 1) + numr - 1];  // the row's face value, one address per row
+            xe = east_face ? xf : inner(c + 1);
+        } else {
+            xe = val(c + 1);
+        }
+        const double xw = inner(c - 1);
         double Ax = brbJ * xc + aJ * xe;
-        if (J > 2) Ax += bJ * inner(c - 1);
+        Ax += has_west ? bJ * xw : 0.0;
         Ax += 2.0 * a.gamma * xc - a.gamma * (xs + xn);
         double r = -a.four_pi * (double)rh[c] - Ax;
         if (UPD) {
