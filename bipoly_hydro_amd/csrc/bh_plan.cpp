@@ -256,7 +256,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     double *Xcur = Xa, *Xnext = Xb;
     if (o.spec_fused) {
         // 4-5 in one kernel per mode, the block kept in LDS
-        SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.tri_gam, o.betar, p->D1, p->Wf,
+        SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.betar, o.alphar, p->D1, p->Wf,
                     numr, numz};
         out->push_back(spectral_op(sa, N, nzi));
     } else {
@@ -284,7 +284,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         ra.cold = (tstep <= 1 && it == 0);  // Xcur = Xa until the first pass writes Xnext
         out->push_back(residual_op(ra, N));
         if (o.spec_fused) {
-            SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.tri_gam, o.betar, p->D1, nullptr,
+            SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, p->D1, nullptr,
                         numr, numz};
             out->push_back(spectral_op(sa, N, nzi));
         } else {

@@ -213,7 +213,7 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     }
     if (precision == BH_PREC_MIXED) {
         b += 4 * (nh * nh + ns * ns + (nzi + 1) / 2 * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
-        if (spectral_fused_ok((int)nri, (int)nzi)) b += 4 * 2 * (nzi / 2) * nzi;
+        if (spectral_fused_ok((int)nri, (int)nzi)) b += 4 * 2 * (nzi / 2) * nzi - 4 * N * nri * nzi;  // no g table
         b += 8 * N * (numr + 1);
     } else {
         b += 8 * 3 * ((size_t)d.nsteps_cold + d.nsteps_warm) * (N * nri + nzi);
@@ -443,9 +443,9 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     ALLOC(o->Mr[0], (size_t)N * nri * nri);
     ALLOC(o->Mr[1], (size_t)N * nri * nri);
     ALLOC(o->tri_ibet, PI);
-    ALLOC(o->tri_gam, PI);
-    ALLOC(o->brb_d, (size_t)N * (numr + 1));
     o->spec_fused = spectral_fused_ok(nri, nzi);
+    if (!o->spec_fused) ALLOC(o->tri_gam, PI);  // the fused step forms g from the pivots
+    ALLOC(o->brb_d, (size_t)N * (numr + 1));
     if (o->spec_fused) {
         ALLOC(o->sfb, (size_t)(nzi / 2) * nzi);
         ALLOC(o->sfa, (size_t)(nzi / 2) * nzi);
@@ -506,8 +506,8 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
                 }
             }
         }
-        if ((rc = upf(o->tri_ibet, std::move(ib))) != BH_OK || (rc = upf(o->tri_gam, std::move(gm))) != BH_OK)
-            return rc;
+        if ((rc = upf(o->tri_ibet, std::move(ib))) != BH_OK) return rc;
+        if (o->tri_gam && (rc = upf(o->tri_gam, std::move(gm))) != BH_OK) return rc;
     }
     // M_r(L) for both schedules, on the device in chunks of modes
     {

@@ -21,8 +21,8 @@ struct SpecArgs {
     const float *sfb, *sfa;       // folded sine table in B- and A-fragment order
     const float *mr;              // [L] M_r in A-fragment order, or null
     const float *mu;              // [nzi] filter factors, spectral slot order
-    const float *ibet, *gam;      // [L][nri][nzi] LU factors of B_r + lambda
-    const double *betar;
+    const float *ibet;            // [L][nri][nzi] pivots 1/bet of B_r + lambda
+    const double *betar, *alphar;
     float *dlt, *w;
     int numr, numz;
 };

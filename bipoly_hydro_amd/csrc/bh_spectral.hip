@@ -76,8 +76,10 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     const bool active = j0 < nri;
     float *pq = buf + SPEC_ROWS * P;  // [2][4][NZ] segment maps of the Thomas sweeps
     float *bl = pq + 8 * NZ;          // [SPEC_ROWS] sub-diagonal a_j = betar(j + 2), a_0 = 0
-    // its load flies during the forward transform; LDS is written after it
+    float *cl = bl + SPEC_ROWS;       // [SPEC_ROWS] super-diagonal c_j = alphar(j + 2), c_{nri-1} = 0
+    // their loads fly during the forward transform; LDS is written after it
     const double blv = tid > 0 && tid < nri ? gld<double>(a.betar)[tid + 2] : 0.0;
+    const double clv = tid < nri - 1 ? gld<double>(a.alphar)[tid + 2] : 0.0;
 
     // ---- 1. RT = r^T S, folded: even columns from u = r_a + r_{n-1-a}, odd
     //         from v = r_a - r_{n-1-a}.  Row t of row block b is j0 + 2t + b,
@@ -133,7 +135,10 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
 #pragma unroll
                 for (int e = 0; e < 4; ++e) buf[(j0 + 2 * (4 * q + e) + b) * P + 16 * c + t] = acc[b][c][e];
     }
-    if (tid < SPEC_ROWS) bl[tid] = (float)blv;
+    if (tid < SPEC_ROWS) {
+        bl[tid] = (float)blv;
+        cl[tid] = (float)clv;
+    }
     __syncthreads();
 
     // ---- 2. W = mu . (M_r RT), C = RT - W in place; W (rows j0 + 16b + 4q + e)
@@ -202,8 +207,9 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     //         a_j y_{j-1}) ib_j, back x_j = y_j - g_{j+1} x_{j+1}), split into
     //         nri/64 segments of 64 rows: a thread composes its segment's
     //         affine map, the maps are chained through LDS, and the segment is
-    //         swept again from its true entry value.  Each thread issues all
-    //         64 factor loads of a sweep at once.
+    //         swept again from its true entry value.  The back sweep's
+    //         g_{j+1} = alphar(j + 2) ib_j comes from the pivots already in
+    //         registers, so only ib is read, all 64 loads at once.
     {
         const int nseg = nri / 64;
         const bool tri = tid < nseg * NZ;  // whole waves: nzi is a multiple of 32
@@ -237,11 +243,14 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                 col[u * P] = y;
             }
             // back over the same rows, high j to low: fv[u] = g_{j+1} for
-            // j = jlo + 63 - u, none past the last row
-            const BH_GLOBAL float *gm = gld<float>(a.gam) + fb;
-            fv[0] = jlo + 64 < nri ? gm[64 * NZ] : 0.f;
+            // j = jlo + 63 - u (zero past the last row)
+            const float *cu = cl + jlo;
 #pragma unroll
-            for (int u = 1; u < 64; ++u) fv[u] = gm[(64 - u) * NZ];
+            for (int u = 0; u < 32; ++u) {  // reverse in place: fv[u] <- c_{63-u} ib_{63-u}
+                const float lo = fv[u], hi = fv[63 - u];
+                fv[u] = cu[63 - u] * hi;
+                fv[63 - u] = cu[u] * lo;
+            }
         }
         __syncthreads();  // the forward maps are read before they are replaced
         if (tri) {
@@ -369,11 +378,11 @@ const void *spec_fn(int variant, bool grouped)
     }
 }
 
-// the block, the segment maps of the Thomas sweeps and the sub-diagonal
+// the block, the segment maps of the Thomas sweeps, the sub- and super-diagonal
 int spec_lds_bytes(int variant)
 {
     const int nz = 32 << variant;
-    return (SPEC_ROWS * (nz + 4) + 8 * nz + SPEC_ROWS) * (int)sizeof(float);
+    return (SPEC_ROWS * (nz + 4) + 8 * nz + 2 * SPEC_ROWS) * (int)sizeof(float);
 }
 
 int spec_variant(int nzi) { return nzi == 32 ? 0 : nzi == 64 ? 1 : nzi == 128 ? 2 : -1; }
