@@ -58,7 +58,7 @@ struct MrArgs {
     int numr, nri, nsteps, mode0, nmodes;
     double *scratch_v, *scratch_u, *scratch_g;  // nmodes*nri*nri each
     float *Mr;                                   // [numphi][nri][nri]
-    int frag;  // store M_r(L) in MFMA A-fragment order: [row tile][k group][lane][4]
+    int frag;  // store M_r(L) as bf16 hi/lo pairs in the bf16 MFMA A-fragment order
 };
 void launch_build_mr(const MrArgs &a, hipStream_t s);
 void launch_fillf(float *p, size_t n, float v, hipStream_t s);

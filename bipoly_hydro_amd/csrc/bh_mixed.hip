@@ -206,6 +206,14 @@ Op tridiag_shift_op(const float *cin, float *yout, const float *ibet, const floa
 // one thread per (mode, column); FP64 work vectors in scratch laid out
 // [J][thread] for coalescing.  Output Mr[L][J][j'] as FP32.
 // ---------------------------------------------------------------------------
+// bf16 bits of x, rounded to nearest even (through FP32), and their value
+__device__ inline unsigned short bf16_bits(double x)
+{
+    const unsigned u = __float_as_uint((float)x);
+    return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline double bf16_value(unsigned short b) { return (double)__uint_as_float((unsigned)b << 16); }
+
 __global__ void build_mr_kernel(MrArgs a)
 {
     const int nri = a.nri;
@@ -244,10 +252,19 @@ __global__ void build_mr_kernel(MrArgs a)
     }
     float *out = a.Mr + (long)L * nri * nri;
     if (a.frag) {
-        // element (row j, k = jc): lane (j % 16) + 16 ((jc % 16) / 4), slot jc % 4
-        const long kgrp = jc / 16, lk = ((jc % 16) / 4) * 16, g = jc % 4, nkg = nri / 16;
-        for (int j = 0; j < nri; ++j)
-            out[(((long)(j / 16) * nkg + kgrp) * 64 + (j % 16) + lk) * 4 + g] = (float)v[j * st];
+        // bf16 pairs x = hi + lo (each rounded to nearest) in the A-fragment
+        // order of v_mfma_f32_16x16x32_bf16: [row tile][k chunk of 32][hi, lo]
+        // [lane][8]; element (row j, k = jc) at lane (j % 16) + 16 ((jc % 32) / 8),
+        // slot jc % 8.  The same bytes as the FP32 matrix.
+        unsigned short *ob = (unsigned short *)out;
+        const long kc = jc / 32, lk = ((jc % 32) / 8) * 16, e = jc % 8, nkc = nri / 32;
+        for (int j = 0; j < nri; ++j) {
+            const double x = v[j * st];
+            const unsigned short hi = bf16_bits(x), lo = bf16_bits(x - bf16_value(hi));
+            const long f = ((long)(j / 16) * nkc + kc) * 2 * 64 + (j % 16) + lk;
+            ob[f * 8 + e] = hi;
+            ob[(f + 64) * 8 + e] = lo;
+        }
     } else {
         for (int j = 0; j < nri; ++j) out[(long)j * nri + jc] = (float)v[j * st];
     }

@@ -84,7 +84,7 @@ struct ModeOps {
     float *Dc32 = nullptr, *Ds32 = nullptr, *S32 = nullptr, *mu32[2] = {}, *Mr[2] = {};
     float *tri_ibet = nullptr, *tri_gam = nullptr;
     double *brb_d = nullptr;
-    // fused per-mode spectral step (then Mr is in A-fragment order)
+    // fused per-mode spectral step (then Mr holds split bf16 fragments)
     bool spec_fused = false;
     float *sfb = nullptr, *sfa = nullptr;
 };

@@ -33,7 +33,15 @@ typedef float f2 __attribute__((ext_vector_type(2)));
 constexpr int SPEC_THREADS = 512;
 constexpr int SPEC_ROWS = 256;  // largest nri
 
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
 __device__ __forceinline__ f4 mma(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+// lane l: A row l % 16, B column l % 16, eight k values per lane taken in the
+// same order from both (any consistent assignment; here k = 8 (l / 16) + e)
+__device__ __forceinline__ f4 mma_bf16(bf8 a, bf8 b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 template <int NZ>
 struct Shape {
@@ -141,60 +149,79 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     }
     __syncthreads();
 
-    // ---- 2. W = mu . (M_r RT), C = RT - W in place; W (rows j0 + 16b + 4q + e)
-    //         stays in registers for its inverse transform
-    f4 wacc[2][S::CT];
+    // ---- 2. W = mu . (M_r RT), C = RT - W in place, as split-bf16 products
+    //         (x = hi + lo; hi.hi + hi.lo + lo.hi in FP32, v_mfma_f32_16x16x32_bf16):
+    //         M_r comes split from the table, RT is split as it is read.  Wave
+    //         (rg, cg) = (wave / 2, wave % 2) owns rows 64 rg + [0, 64) and columns
+    //         (nzi/2) cg + [0, nzi/2); W stays in registers for its inverse.
+    constexpr int CTW = S::CT / 2;  // 16-wide column tiles per wave
+    const int rg = wave >> 1, cg = wave & 1;
+    const bool fil = rg < nri / 64;
+    f4 wacc[4][CTW];
 #pragma unroll
-    for (int b = 0; b < 2; ++b)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int c = 0; c < S::CT; ++c) wacc[b][c] = (f4){0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < CTW; ++c) wacc[i][c] = (f4){0.f, 0.f, 0.f, 0.f};
     if (a.mr) {
-        if (active) {
-            const int nkg = nri / 16;
-            const BH_GLOBAL f4 *mr = gld<f4>(a.mr) + (long)L * nri * nri / 4 + lane;
-            const long rstride = (long)nkg * 64;  // f4 per row tile
-            // M_r fragments two k groups ahead of the products
-            f4 an[2], an2[2];
+        if (fil) {
+            const int nkc = nri / 32;
+            // fragment (row tile 4 rg + i, chunk kc, part): 64 lanes x 8 bf16
+            const BH_GLOBAL bf8 *mrb = gld<bf8>(a.mr) + (long)L * nri * nri / 4 + (long)4 * rg * nkc * 128 + lane;
+            bf8 ah[4], al_[4], nh[4], nl[4];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) {
-                an[b] = mr[(2 * wave + b) * rstride];
-                an2[b] = mr[(2 * wave + b) * rstride + 64];
+            for (int i = 0; i < 4; ++i) {
+                ah[i] = mrb[(i * nkc) * 128];
+                al_[i] = mrb[(i * nkc) * 128 + 64];
             }
-            for (int kg = 0; kg < nkg; ++kg) {
-                const f4 ac[2] = {an[0], an[1]};
-                an[0] = an2[0];
-                an[1] = an2[1];
-                if (kg + 2 < nkg) {
+            const float *bcol = buf + 8 * q * P + (NZ / 2) * cg + t;
+            for (int kc = 0; kc < nkc; ++kc) {
+                if (kc + 1 < nkc) {
 #pragma unroll
-                    for (int b = 0; b < 2; ++b) an2[b] = mr[(2 * wave + b) * rstride + (kg + 2) * 64];
+                    for (int i = 0; i < 4; ++i) {
+                        nh[i] = mrb[(i * nkc + kc + 1) * 128];
+                        nl[i] = mrb[(i * nkc + kc + 1) * 128 + 64];
+                    }
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const float *row = buf + (16 * kg + 4 * q + g) * P + t;
-                    float bv[S::CT];
+                for (int c = 0; c < CTW; ++c) {
+                    // B[k = 32 kc + 8 q + e][col]: rows of the LDS block, split on the fly
+                    bf8 bh, bo;
 #pragma unroll
-                    for (int c = 0; c < S::CT; ++c) bv[c] = row[16 * c];
+                    for (int e = 0; e < 8; ++e) {
+                        const float x = bcol[(32 * kc + e) * P + 16 * c];
+                        const __bf16 h = (__bf16)x;
+                        bh[e] = h;
+                        bo[e] = (__bf16)(x - (float)h);
+                    }
 #pragma unroll
-                    for (int c = 0; c < S::CT; ++c)
+                    for (int i = 0; i < 4; ++i) {
+                        wacc[i][c] = mma_bf16(al_[i], bh, wacc[i][c]);
+                        wacc[i][c] = mma_bf16(ah[i], bo, wacc[i][c]);
+                        wacc[i][c] = mma_bf16(ah[i], bh, wacc[i][c]);
+                    }
+                }
 #pragma unroll
-                        for (int b = 0; b < 2; ++b) wacc[b][c] = mma(ac[b][g], bv[c], wacc[b][c]);
+                for (int i = 0; i < 4; ++i) {
+                    ah[i] = nh[i];
+                    al_[i] = nl[i];
                 }
             }
         }
         __syncthreads();
-        if (active) {
+        if (fil) {
             const BH_GLOBAL float *mu = gld<float>(a.mu);
 #pragma unroll
-            for (int c = 0; c < S::CT; ++c) {
-                const float m = mu[16 * c + t];
+            for (int c = 0; c < CTW; ++c) {
+                const int col = (NZ / 2) * cg + 16 * c + t;
+                const float m = mu[col];
 #pragma unroll
-                for (int b = 0; b < 2; ++b)
+                for (int i = 0; i < 4; ++i)
 #pragma unroll
                     for (int e = 0; e < 4; ++e) {
-                        const int idx = (j0 + 16 * b + 4 * q + e) * P + 16 * c + t;
-                        const float wv = m * wacc[b][c][e];
-                        wacc[b][c][e] = wv;
+                        const int idx = (64 * rg + 16 * i + 4 * q + e) * P + col;
+                        const float wv = m * wacc[i][c][e];
+                        wacc[i][c][e] = wv;
                         buf[idx] = buf[idx] - wv;
                     }
             }
@@ -339,13 +366,14 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     inverse(a.dlt);
     if (a.mr) {
         __syncthreads();  // every wave has read D
-        if (active) {
+        if (fil) {
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
+            for (int i = 0; i < 4; ++i)
 #pragma unroll
-                for (int c = 0; c < S::CT; ++c)
+                for (int c = 0; c < CTW; ++c)
 #pragma unroll
-                    for (int e = 0; e < 4; ++e) buf[(j0 + 16 * b + 4 * q + e) * P + 16 * c + t] = wacc[b][c][e];
+                    for (int e = 0; e < 4; ++e)
+                        buf[(64 * rg + 16 * i + 4 * q + e) * P + (NZ / 2) * cg + 16 * c + t] = wacc[i][c][e];
         }
         __syncthreads();
         inverse(a.w);

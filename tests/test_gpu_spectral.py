@@ -1,9 +1,13 @@
 """The fused per-mode spectral step of the mixed solve (one workgroup per
 azimuthal mode: folded sine transform, M_r filter, Thomas sweeps and the two
 inverse transforms with the block in LDS) against the same step done by the
-separate GEMM and Thomas kernels (BH_SPEC_KERNEL=0).  Both evaluate the same
-FP32 arithmetic in different summation orders, so the potentials agree to
-FP32 rounding carried through one FP64 refinement pass."""
+separate GEMM and Thomas kernels (BH_SPEC_KERNEL=0).  The transforms and
+sweeps are the same FP32 arithmetic in other summation orders; the fused
+filter product is split bf16 (x = hi + lo, three products, about 2^-17
+relative) where the separate one is FP32.  Its error enters the potential
+only through w = M r0, which is small where the iteration has converged:
+measured differences are 1e-13 (cold, 20 steps) to 3e-10 (warm, 5 steps),
+against 2e-9 here and the 1e-8 bar of the mixed path."""
 import os
 
 import numpy as np
@@ -47,4 +51,4 @@ def test_fused_spectral_step_matches_separate_kernels(grid):
         e = rel(g, f)
         print(f"{grid} solve {k}: fused vs separate relL2 {e:.3e}", flush=True)
         assert np.all(np.isfinite(g))
-        assert e < 1e-9
+        assert e < 2e-9
