@@ -257,7 +257,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     if (o.spec_fused) {
         // 4-5 in one kernel per mode, the block kept in LDS
         SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.betar, o.alphar, p->D1, p->Wf,
-                    numr, numz};
+                    numr, numz, N};
         out->push_back(spectral_op(sa, N, nzi));
     } else {
         dst_fwd(p->r32, p->RT);
@@ -285,7 +285,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         out->push_back(residual_op(ra, N));
         if (o.spec_fused) {
             SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, p->D1, nullptr,
-                        numr, numz};
+                        numr, numz, N};
             out->push_back(spectral_op(sa, N, nzi));
         } else {
             dst_fwd(p->r32, p->RT);

@@ -213,7 +213,8 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     }
     if (precision == BH_PREC_MIXED) {
         b += 4 * (nh * nh + ns * ns + (nzi + 1) / 2 * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
-        if (spectral_fused_ok((int)nri, (int)nzi)) b += 4 * 2 * (nzi / 2) * nzi - 4 * N * nri * nzi;  // no g table
+        if (spectral_fused_ok((int)nri, (int)nzi))  // tables; no g; M_r and pivots per mode number
+            b += 4 * 2 * (nzi / 2) * nzi - 4 * N * nri * nzi - 4 * (N / 2 - 1) * (2 * nri * nri + nri * nzi);
         b += 8 * N * (numr + 1);
     } else {
         b += 8 * 3 * ((size_t)d.nsteps_cold + d.nsteps_warm) * (N * nri + nzi);
@@ -440,10 +441,13 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     ALLOC(o->S32, (size_t)((nzi + 1) / 2) * nzi);
     ALLOC(o->mu32[0], nzi);
     ALLOC(o->mu32[1], nzi);
-    ALLOC(o->Mr[0], (size_t)N * nri * nri);
-    ALLOC(o->Mr[1], (size_t)N * nri * nri);
-    ALLOC(o->tri_ibet, PI);
+    // the fused step keeps M_r and the pivots per azimuthal mode number
+    // (planes m and m + N/2 share them): N/2 + 1 of each
     o->spec_fused = spectral_fused_ok(nri, nzi);
+    const int NM = o->spec_fused ? N / 2 + 1 : N;
+    ALLOC(o->Mr[0], (size_t)NM * nri * nri);
+    ALLOC(o->Mr[1], (size_t)NM * nri * nri);
+    ALLOC(o->tri_ibet, (size_t)NM * nri * nzi);
     if (!o->spec_fused) ALLOC(o->tri_gam, PI);  // the fused step forms g from the pivots
     ALLOC(o->brb_d, (size_t)N * (numr + 1));
     if (o->spec_fused) {
@@ -488,21 +492,22 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
             if ((rc = upf(o->mu32[w], std::move(mu))) != BH_OK) return rc;
         }
         // LU factors of B_r(L) + lambda_z(k), layout [L][J'][m], k = spec_mode(m)
-        std::vector<float> ib(PI), gm(PI);
-        for (int L = 0; L < N; ++L) {
+        std::vector<float> ib((size_t)NM * nri * nzi), gm(o->spec_fused ? 0 : PI);
+        for (int L = 0; L < NM; ++L) {
             const double *b = &h.brb[(size_t)L * (numr + 1)];
             for (int m = 0; m < nzi; ++m) {
                 const double lk = lamz[spec_mode(m, nzi)];
                 size_t base = (size_t)L * nri * nzi + m;
                 double bet = b[2] + lk;
+                const bool keep_g = !gm.empty();
                 ib[base] = (float)(1.0 / bet);
-                gm[base] = 0.f;
+                if (keep_g) gm[base] = 0.f;
                 for (int j = 1; j < nri; ++j) {
                     int J = j + 2;
                     double gg = h.alphar[J - 1] / bet;
                     bet = (b[J] + lk) - h.betar[J] * gg;
                     ib[base + (size_t)j * nzi] = (float)(1.0 / bet);
-                    gm[base + (size_t)j * nzi] = (float)gg;
+                    if (keep_g) gm[base + (size_t)j * nzi] = (float)gg;
                 }
             }
         }
@@ -521,11 +526,11 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
             return rc;
         for (int w = 0; w < 2; ++w) {
             if ((rc = upload(ddt[w], h.dt[w], s)) != BH_OK) return rc;
-            for (int m0 = 0; m0 < N; m0 += chunk) {
+            for (int m0 = 0; m0 < NM; m0 += chunk) {
                 MrArgs ma;
                 ma.brb = o->brb_d; ma.alphar = o->alphar; ma.betar = o->betar; ma.dt = ddt[w];
                 ma.numr = numr; ma.nri = nri; ma.nsteps = (int)h.dt[w].size();
-                ma.mode0 = m0; ma.nmodes = std::min(chunk, N - m0);
+                ma.mode0 = m0; ma.nmodes = std::min(chunk, NM - m0);
                 ma.scratch_v = sv; ma.scratch_u = su; ma.scratch_g = sg; ma.Mr = o->Mr[w];
                 ma.frag = o->spec_fused ? 1 : 0;
                 launch_build_mr(ma, s);

@@ -19,12 +19,12 @@ namespace bh {
 struct SpecArgs {
     const float *r;
     const float *sfb, *sfa;       // folded sine table in B- and A-fragment order
-    const float *mr;              // [L] M_r in A-fragment order, or null
+    const float *mr;              // [mode] M_r as split-bf16 A fragments, or null
     const float *mu;              // [nzi] filter factors, spectral slot order
-    const float *ibet;            // [L][nri][nzi] pivots 1/bet of B_r + lambda
+    const float *ibet;            // [mode][nri][nzi] pivots 1/bet of B_r + lambda
     const double *betar, *alphar;
     float *dlt, *w;
-    int numr, numz;
+    int numr, numz, nphi;         // mode number of plane L: L, or L - nphi/2 past nphi/2
 };
 
 // Sizes the fused kernel takes: nri a multiple of 64 up to 256 and nzi one

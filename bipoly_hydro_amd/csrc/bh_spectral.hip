@@ -67,11 +67,16 @@ __device__ __forceinline__ BH_GLOBAL T *gst(void *p)
     return (BH_GLOBAL T *)p;
 }
 
+// Tile t takes plane t/2 + (t%2) N/2: planes m and m + N/2 (the cosine and
+// sine rows of azimuthal mode m) run side by side on one XCD, and share M_r
+// and the pivots, stored per mode number mi = L or L - N/2.
 template <int NZ>
-__device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int L)
+__device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int tile)
 {
     typedef Shape<NZ> S;
     constexpr int P = S::P;
+    const int hn = a.nphi / 2;
+    const int L = (tile >> 1) + (tile & 1) * hn, mi = L <= hn ? L : L - hn;
     // numr passes through an empty asm so that the compiler treats the sizes
     // as new in every tile of the persistent loop: otherwise it hoists
     // hundreds of tile-invariant address offsets out of the loop and spills
@@ -91,30 +96,35 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
 
     // ---- 1. RT = r^T S, folded: even columns from u = r_a + r_{n-1-a}, odd
     //         from v = r_a - r_{n-1-a}.  Row t of row block b is j0 + 2t + b,
-    //         so one 8-byte load gives a lane both blocks' A values.  The r
-    //         planes and table fragments of k group kg + 1 load while group
-    //         kg is multiplied.
+    //         so one 8-byte load gives a lane both blocks' A values.  The
+    //         folded table is staged in LDS (in the block's space, free until
+    //         the products are written): read once per workgroup, not once
+    //         per wave.  The r planes of k group kg + 1 load while group kg is
+    //         multiplied.
+    const BH_GLOBAL float *r = gld<float>(a.r) + (long)L * NZ * nri + j0 + 2 * t;
+    f2 xr[2][4], yr[2][4];
+    auto load = [&](int kg, int slot) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const int pa = 16 * kg + 4 * q + g;
+            xr[slot][g] = *(const BH_GLOBAL f2 *)(r + (long)pa * nri);
+            yr[slot][g] = *(const BH_GLOBAL f2 *)(r + (long)(NZ - 1 - pa) * nri);
+        }
+    };
+    if (active) load(0, 0);
+    {
+        const BH_GLOBAL f4 *sbg = gld<f4>(a.sfb);
+        f4 *sbl = (f4 *)buf;
+        for (int i = tid; i < S::HE * NZ / 4; i += SPEC_THREADS) sbl[i] = sbg[i];
+    }
+    __syncthreads();
+    f4 acc[2][S::CT];
     if (active) {
-        const BH_GLOBAL float *r = gld<float>(a.r) + (long)L * NZ * nri + j0 + 2 * t;
-        const BH_GLOBAL f4 *sb = gld<f4>(a.sfb) + lane;
-        f4 acc[2][S::CT];
+        const f4 *sb = (const f4 *)buf + lane;
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int c = 0; c < S::CT; ++c) acc[b][c] = (f4){0.f, 0.f, 0.f, 0.f};
-        f2 xr[2][4], yr[2][4];
-        f4 bc[2][S::CT];
-        auto load = [&](int kg, int slot) {
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const int pa = 16 * kg + 4 * q + g;
-                xr[slot][g] = *(const BH_GLOBAL f2 *)(r + (long)pa * nri);
-                yr[slot][g] = *(const BH_GLOBAL f2 *)(r + (long)(NZ - 1 - pa) * nri);
-            }
-#pragma unroll
-            for (int c = 0; c < S::CT; ++c) bc[slot][c] = sb[(c * S::KG + kg) * 64];
-        };
-        load(0, 0);
 #pragma unroll
         for (int kg = 0; kg < S::KG; ++kg) {
             const int cur = kg & 1;
@@ -129,13 +139,17 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                 v[g] = xr[cur][g] - yr[cur][g];
             }
 #pragma unroll
-            for (int c = 0; c < S::CT; ++c)
+            for (int c = 0; c < S::CT; ++c) {
+                const f4 bc = sb[(c * S::KG + kg) * 64];
 #pragma unroll
                 for (int g = 0; g < 4; ++g)
 #pragma unroll
-                    for (int b = 0; b < 2; ++b)
-                        acc[b][c] = mma(c < S::CT / 2 ? u[g][b] : v[g][b], bc[cur][c][g], acc[b][c]);
+                    for (int b = 0; b < 2; ++b) acc[b][c] = mma(c < S::CT / 2 ? u[g][b] : v[g][b], bc[g], acc[b][c]);
+            }
         }
+    }
+    __syncthreads();  // every wave is done with the table
+    if (active) {
 #pragma unroll
         for (int b = 0; b < 2; ++b)
 #pragma unroll
@@ -166,7 +180,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         if (fil) {
             const int nkc = nri / 32;
             // fragment (row tile 4 rg + i, chunk kc, part): 64 lanes x 8 bf16
-            const BH_GLOBAL bf8 *mrb = gld<bf8>(a.mr) + (long)L * nri * nri / 4 + (long)4 * rg * nkc * 128 + lane;
+            const BH_GLOBAL bf8 *mrb = gld<bf8>(a.mr) + (long)mi * nri * nri / 4 + (long)4 * rg * nkc * 128 + lane;
             bf8 ah[4], al_[4], nh[4], nl[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -241,7 +255,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         const int nseg = nri / 64;
         const bool tri = tid < nseg * NZ;  // whole waves: nzi is a multiple of 32
         const int m = tid % NZ, sg = tid / NZ, jlo = sg * 64;
-        const long fb = (long)L * nri * NZ + (long)jlo * NZ + m;
+        const long fb = (long)mi * nri * NZ + (long)jlo * NZ + m;
         float *col = buf + jlo * P + m;
         const float *al = bl + jlo;
         float fv[64];
@@ -305,62 +319,49 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
 
     // ---- 4. out[z][j] = E + O, out[n-1-z][j] = E - O for z < n/2, with E
     //         over the even spectral slots of the LDS block and O the odd.
-    //         The z tiles go in passes of ZH so that the accumulators stay
-    //         small while W waits in registers.
+    //         Wave w takes z tile w % ZT and 32 ZT columns j from (w / ZT) 32 ZT,
+    //         so that it reads 1/ZT of the table (the block comes from LDS).
     auto inverse = [&](float *outp) {
-        if (!active) return;
-        constexpr int ZH = S::ZT > 1 ? S::ZT / 2 : 1;
-        BH_GLOBAL float *pl = gst<float>(outp) + (long)L * numr * a.numz + numr + 1 + j0 + t;
-#pragma unroll 1
-        for (int z0 = 0; z0 < S::ZT; z0 += ZH) {
-            f4 ev[ZH][2], od[ZH][2];
+        constexpr int JW = 32 * S::ZT, JT = JW / 16;
+        const int zt = wave % S::ZT, jb = (wave / S::ZT) * JW;
+        if (jb >= nri) return;
+        f4 ev[JT], od[JT];
 #pragma unroll
-            for (int i = 0; i < ZH; ++i)
+        for (int i = 0; i < JT; ++i) {
+            ev[i] = (f4){0.f, 0.f, 0.f, 0.f};
+            od[i] = (f4){0.f, 0.f, 0.f, 0.f};
+        }
+        // k groups [0, CT/2) are the even slots, [CT/2, CT) the odd; the
+        // table fragment of the next group loads during this group's products
+        auto kstep = [&](const f4 &af, int kg, f4(&acc)[JT]) {
 #pragma unroll
-                for (int cc = 0; cc < 2; ++cc) {
-                    ev[i][cc] = (f4){0.f, 0.f, 0.f, 0.f};
-                    od[i][cc] = (f4){0.f, 0.f, 0.f, 0.f};
-                }
-            // k groups [0, CT/2) are the even slots, [CT/2, CT) the odd; the
-            // table fragments of the next group load during this group's products
-            auto kstep = [&](const f4(&af)[ZH], int kg, f4(&acc)[ZH][2]) {
-                f4 bf[2];
+            for (int i = 0; i < JT; ++i) {
+                const f4 bf = *(const f4 *)(buf + (jb + 16 * i + t) * P + 16 * kg + 4 * q);
 #pragma unroll
-                for (int cc = 0; cc < 2; ++cc) bf[cc] = *(const f4 *)(buf + (j0 + 16 * cc + t) * P + 16 * kg + 4 * q);
-#pragma unroll
-                for (int i = 0; i < ZH; ++i)
-#pragma unroll
-                    for (int g = 0; g < 4; ++g)
-#pragma unroll
-                        for (int cc = 0; cc < 2; ++cc) acc[i][cc] = mma(af[i][g], bf[cc][g], acc[i][cc]);
-            };
-            const BH_GLOBAL f4 *sa = gld<f4>(a.sfa) + (long)z0 * S::CT * 64 + lane;
-            f4 afc[ZH], afn[ZH];
-#pragma unroll
-            for (int i = 0; i < ZH; ++i) afc[i] = sa[i * S::CT * 64];
-#pragma unroll 1
-            for (int kg = 0; kg < S::CT; ++kg) {
-                if (kg + 1 < S::CT) {
-#pragma unroll
-                    for (int i = 0; i < ZH; ++i) afn[i] = sa[(i * S::CT + kg + 1) * 64];
-                }
-                __builtin_amdgcn_sched_barrier(0);
-                if (kg < S::CT / 2) kstep(afc, kg, ev);
-                else kstep(afc, kg, od);
-#pragma unroll
-                for (int i = 0; i < ZH; ++i) afc[i] = afn[i];
+                for (int g = 0; g < 4; ++g) acc[i] = mma(af[g], bf[g], acc[i]);
             }
+        };
+        const BH_GLOBAL f4 *sa = gld<f4>(a.sfa) + (long)zt * S::CT * 64 + lane;
+        f4 afc = sa[0], afn = afc;
+#pragma unroll 1
+        for (int kg = 0; kg < S::CT; ++kg) {
+            if (kg + 1 < S::CT) afn = sa[(kg + 1) * 64];
+            __builtin_amdgcn_sched_barrier(0);
+            if (kg < S::CT / 2) kstep(afc, kg, ev);
+            else kstep(afc, kg, od);
+            afc = afn;
+        }
+        BH_GLOBAL float *pl = gst<float>(outp) + (long)L * numr * a.numz + numr + 1 + jb + t;
 #pragma unroll
-            for (int i = 0; i < ZH; ++i)
+        for (int i = 0; i < JT; ++i) {
+            if (jb + 16 * i >= nri) break;
 #pragma unroll
-                for (int cc = 0; cc < 2; ++cc)
-#pragma unroll
-                    for (int e = 0; e < 4; ++e) {
-                        const int z = 16 * (z0 + i) + 4 * q + e;
-                        const float x = ev[i][cc][e], y = od[i][cc][e];
-                        pl[(long)z * numr + 16 * cc] = x + y;
-                        pl[(long)(NZ - 1 - z) * numr + 16 * cc] = x - y;
-                    }
+            for (int e = 0; e < 4; ++e) {
+                const int z = 16 * zt + 4 * q + e;
+                const float x = ev[i][e], y = od[i][e];
+                pl[(long)z * numr + 16 * i] = x + y;
+                pl[(long)(NZ - 1 - z) * numr + 16 * i] = x - y;
+            }
         }
     };
     inverse(a.dlt);
