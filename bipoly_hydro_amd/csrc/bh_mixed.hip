@@ -32,15 +32,14 @@ namespace bh {
 // (numr x numz) planes in J-layout; dlt and w have zero faces.  r is written
 // compact: r[(K-2)*nri + (J-2)].
 // ---------------------------------------------------------------------------
-// Tile: 256 consecutive J of one plane, walking RES_ROWS rows of K with the
-// rows K-1, K, K+1 of the column kept in registers (one new value of x per
-// row), the J coefficients loaded once.
+// Tile: 256 consecutive J of one plane and RES_ROWS rows of K, the column's
+// rows K0-1 .. K0+RES_ROWS and the J coefficients loaded once.
 // The variant fixes at compile time whether delta, w and Xo are present
 // (UPD: the refinement pass) and whether X is a cold start (COLD), so that
 // the row loop carries no pointer tests.
 constexpr int RES_ROWS = 8;
 
-template <bool UPD, bool COLD>
+template <bool UPD, bool COLD, bool EDGE>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz)
 {
     const int numr = a.numr, nri = numr - 2, nzi = a.numz - 2;
@@ -63,45 +62,84 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     auto inner = [&](long i) { return COLD ? (UPD ? (double)dl[i] : 0.0) : val(i); };
     auto face = [&](long i) { return COLD ? X[i] : val(i); };
     // the lane-dependent cases (the east face at J = numr - 1, no west term
-    // at J = 2) are selects, not branches, so that the loads of a row issue
-    // together; every address read is inside the plane
+    // at J = 2) are selects, not branches; every address read is inside the
+    // plane.  All of the tile's loads are issued before the first use and
+    // the stores follow the arithmetic (stores share the load counter, so
+    // interleaving them would serialise the rows).
     const bool east_face = J + 1 == numr, has_west = J > 2;
-    long c = (long)(K0 - 1) * numr + (J - 1);
-    double xs = (K0 - 1 == 1) ? face(c - numr) : inner(c - numr), xc = inner(c);
-#pragma unroll 4
-    for (int K = K0; K < K1; ++K, c += numr) {
-        const double xn = (K + 1 == a.numz) ? face(c + numr) : inner(c + numr);
-        double xe;
-        if (COLD) {
-            const double xf = X[c - (J - 1) + numr - 1];  // the row's face value, one address per row
-            xe = east_face ? xf : inner(c + 1);
+    // interior tiles (not EDGE) have RES_ROWS rows and no face row: their
+    // loads are straight-line code (a branch around a load makes the
+    // compiler wait for every load in flight where the paths join)
+    const int nk = EDGE ? K1 - K0 : RES_ROWS;
+    const long c0 = (long)(K0 - 1) * numr + (J - 1);  // (J, K0)
+    double xcv[RES_ROWS + 2], xev[RES_ROWS], xwv[RES_ROWS];  // xcv[i]: row K0 - 1 + i
+    float rv[RES_ROWS], wv[RES_ROWS];
+#pragma unroll
+    for (int i = 0; i < RES_ROWS + 2; ++i) {
+        const int K = K0 - 1 + i;
+        const long ci = c0 + (long)(i - 1) * numr;
+        if (!EDGE) {
+            xcv[i] = inner(ci);
         } else {
-            xe = val(c + 1);
+            xcv[i] = 0.0;
+            if (i <= nk + 1) xcv[i] = (K == 1 || K == a.numz) ? face(ci) : inner(ci);
         }
-        const double xw = inner(c - 1);
-        double Ax = brbJ * xc + aJ * xe;
-        Ax += has_west ? bJ * xw : 0.0;
-        Ax += 2.0 * a.gamma * xc - a.gamma * (xs + xn);
-        double r = -a.four_pi * (double)rh[c] - Ax;
-        if (UPD) {
-            r -= (double)w[c];
-            Xo[c] = xc;
+    }
+#pragma unroll
+    for (int i = 0; i < RES_ROWS; ++i) {
+        const long ci = c0 + (long)i * numr;
+        xev[i] = 0.0;
+        xwv[i] = 0.0;
+        rv[i] = 0.f;
+        wv[i] = 0.f;
+        if (i < nk) {
+            if (COLD) {
+                const double xf = X[ci - (J - 1) + numr - 1];  // the row's face value, one address per row
+                const double xi = inner(ci + 1);
+                xev[i] = east_face ? xf : xi;
+            } else {
+                xev[i] = val(ci + 1);
+            }
+            xwv[i] = inner(ci - 1);
+            rv[i] = rh[ci];
+            if (UPD) wv[i] = w[ci];
         }
-        r32[(long)(K - 2) * nri + jr] = (float)r;
-        xs = xc;
-        xc = xn;
+    }
+#pragma unroll
+    for (int i = 0; i < RES_ROWS; ++i) {
+        if (i < nk) {
+            const long ci = c0 + (long)i * numr;
+            const double xc = xcv[i + 1];
+            double Ax = brbJ * xc + aJ * xev[i];
+            Ax += has_west ? bJ * xwv[i] : 0.0;
+            Ax += 2.0 * a.gamma * xc - a.gamma * (xcv[i] + xcv[i + 2]);
+            double r = -a.four_pi * (double)rv[i] - Ax;
+            if (UPD) {
+                r -= (double)wv[i];
+                Xo[ci] = xc;
+            }
+            r32[(long)(K0 - 2 + i) * nri + jr] = (float)r;
+        }
     }
 }
 
-// the refinement form (delta, w and Xo given) or the first residual, cold or warm
-__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
+// the refinement form (delta, w and Xo given) or the first residual, cold or
+// warm; the first and last tiles of a plane (face rows K = 1, numz) apart
+template <bool EDGE>
+__device__ __forceinline__ void residual_tile_e(const ResidualArgs &a, int bx, int by, int bz)
 {
     switch ((a.dlt ? 2 : 0) + (a.cold ? 1 : 0)) {
-    case 0: residual_tile_v<false, false>(a, bx, by, bz); break;
-    case 1: residual_tile_v<false, true>(a, bx, by, bz); break;
-    case 2: residual_tile_v<true, false>(a, bx, by, bz); break;
-    default: residual_tile_v<true, true>(a, bx, by, bz); break;
+    case 0: residual_tile_v<false, false, EDGE>(a, bx, by, bz); break;
+    case 1: residual_tile_v<false, true, EDGE>(a, bx, by, bz); break;
+    case 2: residual_tile_v<true, false, EDGE>(a, bx, by, bz); break;
+    default: residual_tile_v<true, true, EDGE>(a, bx, by, bz); break;
     }
+}
+__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
+{
+    const int nzi = a.numz - 2, K0 = by * RES_ROWS + 2;
+    if (by == 0 || K0 + RES_ROWS >= nzi + 2) residual_tile_e<true>(a, bx, by, bz);
+    else residual_tile_e<false>(a, bx, by, bz);
 }
 
 Op residual_op(const ResidualArgs &a, int nplanes)

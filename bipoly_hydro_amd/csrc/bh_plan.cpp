@@ -572,6 +572,19 @@ extern "C" int bhx_dft64(const bh_plan *p, const double *in, double *out, int nc
     return BH_OK;
 }
 
+// forward real DFT in FP32 (the mixed plan's transform of rho, unmasked) of
+// nplanes x ncols device data; test hook.
+extern "C" int bhx_dft32(const bh_plan *p, const double *in, float *out, int ncols)
+{
+    if (!p || !in || !out) return fail(BH_E_INVALID, "null argument");
+    if (!p->ops->dfwd32) return fail(BH_E_STATE, "plan has no FP32 transform");
+    DftArgs d{in, ncols, out, ncols, p->ops->dfwd32, p->g.numphi, ncols, 0, 0};
+    launch_op(dft_op(d, DFT_FWD_F32), 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return BH_OK;
+}
+
 // FP64 / FP32 GEMM with an explicit tile shape (0..4) and transposes (FP32
 // only); test hook.
 extern "C" int bhx_gemm_tile(int fp32, const void *A, long lda, const void *B, long ldb, int R, int M, int Kd, void *C,

@@ -31,11 +31,13 @@ def test_gemm_f64_matches_numpy(R, M, K):
     np.testing.assert_allclose(dC.cpu().numpy(), A @ B, rtol=0, atol=1e-13 * K)
 
 
-@pytest.mark.parametrize("numphi", [4, 16, 64, 512])
-def test_folded_dft_matches_numpy(numphi):
+@pytest.mark.parametrize("numphi,ncols", [(4, 97), (16, 97), (64, 97), (512, 97), (512, 20000)])
+def test_folded_dft_matches_numpy(numphi, ncols):
+    """FP64 forward and inverse transforms against numpy; 20000 columns give
+    every workgroup of the N = 512 kernels several panels (the staging of the
+    next panel runs during the products)."""
     torch = pytest.importorskip("torch")
     L = _lib()
-    ncols = 97
     rng = np.random.default_rng(numphi)
     x = rng.standard_normal((numphi, ncols))
     with P.PotentialPlan(10, 10, numphi) as plan:
@@ -111,3 +113,23 @@ def test_folded_sine_transform_matches_dense(tile, n, nrows):
     ref_inv = np.einsum("zm,brm->bzr", S[:, perm], y)
     np.testing.assert_allclose(dz.cpu().numpy(), ref_inv, rtol=0, atol=tol)
     np.testing.assert_allclose(dz.cpu().numpy(), x, rtol=0, atol=2 * tol)  # S is its own inverse
+
+
+@pytest.mark.parametrize("numphi,ncols", [(64, 97), (512, 97), (512, 20000)])
+def test_fp32_forward_dft_matches_fft(numphi, ncols):
+    """The mixed plan's FP32 forward transform (helmadi's mode order: a_k, then
+    the sine rows as rfft's imaginary parts) against numpy's FFT; 20000
+    columns give every workgroup of the N = 512 kernel several panels."""
+    torch = pytest.importorskip("torch")
+    L = _lib()
+    vp = ctypes.c_void_p
+    L.bhx_dft32.argtypes = [vp, vp, vp, ctypes.c_int]
+    rng = np.random.default_rng(numphi + 1)
+    x = rng.standard_normal((numphi, ncols))
+    with P.PotentialPlan(10, 10, numphi, precision="mixed") as plan:
+        dx = torch.from_numpy(x).cuda()
+        dy = torch.zeros((numphi, ncols), dtype=torch.float32, device="cuda")
+        assert L.bhx_dft32(plan.handle, dx.data_ptr(), dy.data_ptr(), ncols) == 0
+        X = np.fft.rfft(x, axis=0)
+        ref = np.concatenate([X.real[: numphi // 2 + 1], X.imag[1: numphi // 2]], axis=0)
+        np.testing.assert_allclose(dy.cpu().numpy(), ref, rtol=0, atol=3e-6 * numphi)
