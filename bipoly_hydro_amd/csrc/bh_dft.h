@@ -19,6 +19,7 @@ struct DftArgs {
     int mask_numr, mask_numz;  // forward: zero the rhop layers (potential_solver.F90:70-78); 0 = no mask
     const float *xadd;  // inverse at N = 512: input is x + xadd (FP32), ...
     double *xstore;     // ... and x + xadd is also stored here (both null: plain input)
+    int max_blocks;     // single launches at N = 512: cap on the persistent grid (0: one per CU)
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <functional>
 #include <vector>
 
@@ -906,7 +907,9 @@ bool dft_launch(const Op &op, hipStream_t s)
         const int kind = op.variant % 4;
         int nx = op.nx, ny = op.ny, total = (int)op.tiles();
         void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
-        (void)hipLaunchKernel(pipe_fn(kind, false), dim3(pipe_blocks(total)), dim3(512), argv, (int)pipe_bytes(kind), s);
+        const int cap = op.as<DftArgs>().max_blocks;
+        const int nb = cap > 0 ? std::min(pipe_blocks(total), cap) : pipe_blocks(total);
+        (void)hipLaunchKernel(pipe_fn(kind, false), dim3(nb), dim3(512), argv, (int)pipe_bytes(kind), s);
         return true;
     }
     const int lds = (int)dft_lds_bytes(op.variant % 4, 1 << (op.variant / 4));

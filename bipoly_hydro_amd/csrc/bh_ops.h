@@ -107,6 +107,12 @@ enum OpKind {
     OP_NKINDS
 };
 
+// OP_MARK variants past the stage marks (0-7): ops between MARK_FORK and
+// MARK_SIDE_END run on the plan's side stream, ordered after what precedes
+// the fork; MARK_JOIN makes the main stream wait for them.  Grouped
+// (batched) execution runs everything in order on one stream.
+enum { MARK_FORK = 100, MARK_SIDE_END = 101, MARK_JOIN = 102 };
+
 constexpr int OP_ARGS_MAX = 256;
 
 struct Op {

@@ -48,6 +48,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     if (const char *e = std::getenv("BH_GEMM_ORDER_DFT")) p->order_dft = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_ORDER_SPEC")) p->order_spec = std::atoi(e);
     if (const char *e = std::getenv("BH_DFT_KERNEL")) p->use_dft = std::atoi(e) != 0;
+    if (const char *e = std::getenv("BH_OVERLAP")) p->overlap = std::atoi(e) != 0;
+    if (const char *e = std::getenv("BH_SIDE_CUS")) p->side_cus = std::atoi(e);
     bool created;
     int rc;
     if ((rc = get_tables(h, device, &p->tab, &created)) != BH_OK) return rc;
@@ -89,6 +91,10 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     }
 #undef ALLOC
     p->bytes = p->mem.bytes + p->tab->mem.bytes + p->ops->mem.bytes;
+    HIPCHK(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, device));
+    HIPCHK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
     hipStream_t s;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     launch_fill(p->XJ, M * N, 0.0, s);
@@ -109,15 +115,25 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
 // ---------------------------------------------------------------------------
 namespace {
 
+// Workgroups of a DFT that runs on the side stream: half the CUs (a
+// multiple of 8 for the XCD-chunked panel order), the rest left to the
+// main stream's kernels; 0 (one per CU) without the side stream.
+int side_blocks(const bh_plan *p)
+{
+    if (!p->overlap) return 0;
+    return p->side_cus > 0 ? p->side_cus : std::max(8, p->cus / 2 / 8 * 8);
+}
+
 // Forward real DFT along phi of the N rows of B (columns Mcols), written in
 // helmadi's mode order (helmadi.F90:131-156) to C; cosine and sine halves.
 void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, double *C, long ldc, bool mask,
-                   std::vector<Op> *out)
+                   std::vector<Op> *out, int max_blocks = 0)
 {
     const ModeOps &o = *p->ops;
     const int N = p->g.numphi, nh = N / 2 + 1, nsn = N / 2 - 1;
     if (o.dfwd64 && p->use_dft) {
         DftArgs d{B, ldb, C, ldc, o.dfwd64, N, Mcols, mask ? p->g.numr : 0, mask ? p->g.numz : 0};
+        d.max_blocks = max_blocks;
         out->push_back(dft_op(d, DFT_FWD_F64));
         return;
     }
@@ -168,17 +184,23 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
     out->push_back(gemm_op(a, G64_PLAIN));
 }
 
-// Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF
+// Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF.  The
+// side contraction (L2-bound) runs on the side stream, after whatever is
+// queued there, beside the top/bottom contraction (HBM-bound); the faces
+// wait for both (and for the side stream's earlier work).
 void boundary_ops(const bh_plan *p, std::vector<Op> *out)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
+    out->push_back(Op::mark(MARK_FORK));
+    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax));
+    out->push_back(Op::mark(MARK_SIDE_END));
     out->push_back(Op::mark(5));
     out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
     out->push_back(contract_topsym_op(p->tab->T, p->Pt, p->Ptb, numr, numz, mmax));
     out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
-    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax));
+    out->push_back(Op::mark(MARK_JOIN));
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax,
                                  -g.dr * g.dz * g.dphi, 1));
 }
@@ -195,10 +217,15 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     const long PIm = (long)nri * nzi;  // interior values per mode
     const int w = (tstep > 1) ? 1 : 0;
     out->push_back(Op::mark(0));
-    // 1. rho-hat in FP32 (masked as rhop), projections in FP64
+    // 1. rho-hat in FP32 (masked as rhop) on the side stream, on half the
+    //    CUs, while the FP64 projections and the boundary (which do not need
+    //    it) run on the main stream
+    out->push_back(Op::mark(MARK_FORK));
     if (o.dfwd32 && p->use_dft) {
         DftArgs d{d_rho, M, p->RH32, M, o.dfwd32, N, (int)M, numr, numz};
+        d.max_blocks = side_blocks(p);
         out->push_back(dft_op(d, DFT_FWD_F32));
+        out->push_back(Op::mark(MARK_SIDE_END));
         proj64_ops(p, d_rho, out);
     } else {
         const int nh = N / 2 + 1, nsn = N / 2 - 1;
@@ -211,12 +238,13 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         out->push_back(gemm_op(a, G32D_FOLDP_MASK));
         a.A = o.Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * M; a.R = nsn; a.Kd = nsn;
         out->push_back(gemm_op(a, G32D_FOLDM_MASK));
+        out->push_back(Op::mark(MARK_SIDE_END));
         proj64_ops(p, d_rho, out);
     }
     out->push_back(Op::mark(1));
     // 2. boundary faces (FP64), into both potential buffers; the starting
     //    potential is zero (cold) or the one kept from the last solve (warm)
-    boundary_ops(p, out);
+    boundary_ops(p, out);  // joins the side stream
     double *Xa = p->XJ, *Xb = p->XK;
     // a cold start needs no zeroing: the residuals below read only the faces
     // of Xa, and Xa's interior is rewritten by the final update
@@ -319,11 +347,15 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     const bool warm = tstep > 1;
     const int w = warm ? 1 : 0;
     out->push_back(Op::mark(0));
-    // 1. one product gives rho-hat (with rhop's zeroed layers) and the
-    //    boundary projections of rho (bessel.F90:179-185, 228-237)
-    dft_fwd64_ops(p, d_rho, M, (int)M, p->RJ, M, true, out);
-    proj64_ops(p, d_rho, out);
+    // 1. rho-hat (with rhop's zeroed layers, potential_solver.F90:70-78) and
+    //    its K-layout copy on the side stream, on half the CUs, beside the
+    //    boundary projections of rho (bessel.F90:179-185, 228-237) and the
+    //    boundary
+    out->push_back(Op::mark(MARK_FORK));
+    dft_fwd64_ops(p, d_rho, M, (int)M, p->RJ, M, true, out, side_blocks(p));
     out->push_back(transpose_op(p->RJ, p->RK, numr, numz, N));
+    out->push_back(Op::mark(MARK_SIDE_END));
+    proj64_ops(p, d_rho, out);
     out->push_back(Op::mark(1));
     // 2. boundary potential -> Fourier-space faces
     boundary_ops(p, out);
@@ -404,11 +436,25 @@ int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s)
         }
         ts = &p->tsets[p->tcount++];
     }
+    hipStream_t cur = s;
     for (const Op &op : ops) {
         if (op.kind == OP_MARK) {
-            if (ts) HIPCHK(hipEventRecord(ts->ev[op.variant], s));
+            if (op.variant == MARK_FORK) {
+                if (!p->overlap) continue;
+                HIPCHK(hipEventRecord(p->fork_ev, s));
+                HIPCHK(hipStreamWaitEvent(p->side, p->fork_ev, 0));
+                cur = p->side;
+            } else if (op.variant == MARK_SIDE_END) {
+                cur = s;
+            } else if (op.variant == MARK_JOIN) {
+                if (!p->overlap) continue;
+                HIPCHK(hipEventRecord(p->join_ev, p->side));
+                HIPCHK(hipStreamWaitEvent(s, p->join_ev, 0));
+            } else if (ts) {
+                HIPCHK(hipEventRecord(ts->ev[op.variant], s));
+            }
         } else if (!(op.kind == OP_FILL && op.as<FillArgs>().n == 0)) {
-            launch_op(op, s);
+            launch_op(op, cur);
         }
     }
     HIPCHK(hipGetLastError());

@@ -126,6 +126,13 @@ struct bh_plan {
     int tile_dft = 2, tile_spec = 2, tile_dstf = 2, tile_dsti = 2;  // GEMM tile shapes, measured best
     int order_dft = 0, order_spec = 0;             // GEMM tile numbering (GemmArgs::col_fastest)
     bool use_dft = true;                           // panel DFT kernel (bh_dft.hip) when numphi allows
+    // the forward DFT of rho runs on a side stream beside the projections
+    // and the boundary (independent until the faces are scattered)
+    bool overlap = true;
+    int cus = 256;      // compute units of the device
+    int side_cus = 0;   // workgroups of the side-stream DFT (0: half the CUs)
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool has_solution = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;
@@ -133,6 +140,12 @@ struct bh_plan {
     int tcount = 0;
     double last_ms[6] = {0, 0, 0, 0, 0, 0};
     bool have_timing = false;
+    ~bh_plan()
+    {
+        if (side) (void)hipStreamDestroy(side);
+        if (fork_ev) (void)hipEventDestroy(fork_ev);
+        if (join_ev) (void)hipEventDestroy(join_ev);
+    }
 };
 
 namespace bh {
