@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstddef>
+#include <vector>
 
 #include "bh_ops.h"
 
@@ -53,6 +54,12 @@ struct ContractArgs {
     int numr, numz, mmax;
     const double *aux;  // sqrt(rhf) [numr + 1], 1-based J (symmetric top table)
     const double *unused;
+};
+struct ProjArgs {
+    const double *rho;   // [numphi][M]
+    const double *coef;  // [chunk][numphi/2][64] folded mode coefficients
+    double *proj;        // [2*mmax-1][M]
+    int M, numphi, mmax;
 };
 struct RebuildArgs {
     const double *Stb, *Ss, *besc, *bess;
@@ -112,6 +119,12 @@ __host__ __device__ inline int topsym_splits(int numz) { return (numz - 2 + TS_P
 // Ss:  [CT_SPLIT][mmax][2][nzi]
 // symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4),
 // contract (Ptb, mmax*topsym_splits*ceil(NB/2)*NB16*4), reduce into Stb (one slot)
+// proj from rho: the azimuthal mode sums of every (J, K) column, the planes
+// folded in quads (FP64 matrix cores); coef from proj_coefficients
+constexpr int PROJ_MODES = 32;  // modes per chunk (grid x)
+int proj_coef_elems(int numphi, int mmax);
+Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax);
+void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef);
 Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int numr, int numz, int mmax);
 Op contract_topsym_op(const double *TS, const double *Pt, double *Ptb, int numr, int numz, int mmax);
 Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax);

@@ -6,12 +6,15 @@
 //                 half-step reads its lines contiguously (the on-device form
 //                 of swap_r_z / swap_z_r)
 //   gen_*_table   Green's-function tables (tm.F90 / sm.F90 / bm.F90)
+//   proj          azimuthal mode projections of rho (bessel.F90:176-235)
 //   contract_*    boundary contraction (bessel.F90:186-262)
 //   face_*        boundary face rebuild and placement (bessel.F90:396-488)
 #include "bh_internal.h"
 #include "bh_ops.h"
 
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 namespace bh {
 
@@ -493,6 +496,130 @@ void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
 }
 
 // ---------------------------------------------------------------------------
+// Azimuthal projections of rho (bessel.F90:176-184, 226-235): per column
+// (J, K), proj[0] = sum_L rho_L, and for m = 1..mmax-1 proj[m] = sum_L
+// cos(m theta_L) rho_L and proj[mmax-1+m] = sum_L sin(m theta_L) rho_L, with
+// theta_L = (L - 1/2) dphi (potsetup.F90:131-139).
+//
+// The planes fold twice.  theta -> 2 pi - theta pairs plane l (0-based) with
+// N-1-l: cosines take u = rho_l + rho_{N-1-l}, sines v = rho_l - rho_{N-1-l}.
+// theta -> pi - theta pairs l with N/2-1-l and multiplies cos(m theta) by
+// (-1)^m and sin(m theta) by -(-1)^m.  So, with the quad of planes a,
+// N-1-a, N/2-1-a, N/2+a (a < N/4) giving u_a, v_a and u_b, v_b,
+//   even-m cosines  sum_a cos(m theta_a) (u_a + u_b)
+//   odd-m cosines   sum_a cos(m theta_a) (u_a - u_b)
+//   even-m sines    sum_a sin(m theta_a) (v_a - v_b)
+//   odd-m sines     sum_a sin(m theta_a) (v_a + v_b)
+// a quarter of the multiply-adds of the plain sums.  Each is a product
+// (16 modes x N/4 quads) x (N/4 quads x columns) on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64, k = 4 quads per step, exact FP64 products and
+// sums): a wavefront owns 16 columns and the four 16-mode row tiles of one
+// PROJ_MODES chunk (grid x), a workgroup 64 columns (grid y).  The quad
+// planes stream in through a ring of PROJ_RING k-steps of loads, so rho is
+// read once from HBM with the matrix work hidden under it.
+// ---------------------------------------------------------------------------
+constexpr int PROJ_RING = 4;
+
+__host__ __device__ inline int proj_quads(int numphi) { return numphi / 4 < 4 ? 4 : numphi / 4; }
+
+// coef[c][a][i][rt], a < max(N/4, 4): rt 0 cos(m theta_a) with m = c
+// PROJ_MODES + 2i, rt 1 the same with m + 1, rt 2 and 3 the sines; theta_a =
+// (a + 1/2) dphi.  1 and 0 for m = 0; zeros for m >= mmax and for quads
+// past N/4 (N < 16).  The angle is formed as the bessel rows form it (L = a + 1).
+int proj_coef_elems(int numphi, int mmax) { return (mmax + PROJ_MODES - 1) / PROJ_MODES * proj_quads(numphi) * 64; }
+
+void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef)
+{
+    const int Q = numphi / 4, Qt = proj_quads(numphi), nc = (mmax + PROJ_MODES - 1) / PROJ_MODES;
+    coef->assign((size_t)proj_coef_elems(numphi, mmax), 0.0);
+    for (int c = 0; c < nc; ++c)
+        for (int q = 0; q < Q; ++q)
+            for (int i = 0; i < 16; ++i)
+                for (int rt = 0; rt < 4; ++rt) {
+                    const int m = c * PROJ_MODES + 2 * i + (rt & 1);
+                    if (m >= mmax) continue;
+                    const double ang = 0.5 * dphi * (double)m * (2.0 * (double)(q + 1) - 1.0);
+                    const double v = rt < 2 ? (m == 0 ? 1.0 : std::cos(ang)) : (m == 0 ? 0.0 : std::sin(ang));
+                    (*coef)[(((size_t)c * Qt + q) * 16 + i) * 4 + rt] = v;
+                }
+}
+
+__device__ __forceinline__ void proj_tile(const ProjArgs &a, int chunk, int cb, int)
+{
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, r16 = lane & 15, k4 = lane >> 4;
+    const int wave = threadIdx.x >> 6;
+    const int M = a.M, N = a.numphi, Q = N / 4, Qt = proj_quads(N), nks = Qt / 4;
+    const int col = cb * 64 + wave * 16 + r16;
+    const int cc = col < M ? col : M - 1;
+    // this lane's quad in a k-step is 4 ks + k4; past N/4 (N < 16) its
+    // coefficients are zero and its loads repeat the last quad
+    const int kq = k4 < Q ? k4 : Q - 1;
+    const double *pa = a.rho + (long)kq * M + cc;                // plane a
+    const double *pd = a.rho + (long)(N - 1 - kq) * M + cc;      // plane N-1-a
+    const double *pb = a.rho + (long)(N / 2 - 1 - kq) * M + cc;  // plane N/2-1-a
+    const double *pc = a.rho + (long)(N / 2 + kq) * M + cc;      // plane N/2+a
+    const double *tq = a.coef + ((long)chunk * Qt + k4) * 64 + r16 * 4;
+    d4 acc[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] = (d4){0.0, 0.0, 0.0, 0.0};
+    struct Step {
+        double xa, xd, xb, xc;
+        d2 c01, c23;
+    };
+    auto load = [&](int ks, Step &s) {
+        ks = min(ks, nks - 1);  // past the end: the last step again, unused
+        const long o = (long)ks * 4 * M;
+        s.xa = pa[o];
+        s.xd = pd[-o];
+        s.xb = pb[-o];
+        s.xc = pc[o];
+        const d2 *t = (const d2 *)(tq + (long)ks * 4 * 64);
+        s.c01 = t[0];
+        s.c23 = t[1];
+    };
+    auto step = [&](const Step &s) {
+        const double ua = s.xa + s.xd, va = s.xa - s.xd;  // pair a
+        const double ub = s.xb + s.xc, vb = s.xb - s.xc;  // pair N/2-1-a
+        acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c01.x, ua + ub, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c01.y, ua - ub, acc[1], 0, 0, 0);
+        acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c23.x, va - vb, acc[2], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c23.y, va + vb, acc[3], 0, 0, 0);
+    };
+    Step ring[PROJ_RING];
+#pragma unroll
+    for (int u = 0; u < PROJ_RING; ++u) load(u, ring[u]);
+    __builtin_amdgcn_sched_barrier(0);
+    for (int j = 0; j < nks; j += PROJ_RING) {
+#pragma unroll
+        for (int u = 0; u < PROJ_RING; ++u) {
+            if (j + u < nks) step(ring[u]);
+            load(j + PROJ_RING + u, ring[u]);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    // result D[4v + k4][r16] of row tile rt: mode m = chunk PROJ_MODES +
+    // 2 (4v + k4) + (rt & 1), column r16
+    const int mmax = a.mmax;
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = chunk * PROJ_MODES + 2 * (4 * v + k4) + (rt & 1);
+            const bool sine = rt >= 2;
+            const int row = sine ? mmax - 1 + m : m;
+            if (col < M && m < mmax && (!sine || m > 0)) a.proj[(long)row * M + col] = acc[rt][v];
+        }
+}
+
+Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax)
+{
+    ProjArgs a{rho, coef, proj, M, numphi, mmax};
+    return Op::make(OP_PROJ, 0, a, (mmax + PROJ_MODES - 1) / PROJ_MODES, (M + 63) / 64, 1);
+}
+
+// ---------------------------------------------------------------------------
 // Boundary contraction.  Top and bottom share one table: the bottom Green's
 // function at source height z' equals the top one at -z' on the symmetric
 // isym = 1 grid (bm.F90 vs tm.F90), so each table entry is contracted with
@@ -844,6 +971,7 @@ Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int
 BH_KERNEL_PAIR(fill, FillArgs, 256)
 BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
 BH_KERNEL_PAIR(axis, AxisArgs, 128)
+BH_KERNEL_PAIR(proj, ProjArgs, 256)
 BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
@@ -869,6 +997,7 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         BH_REF(OP_FILL, fill, 256)
         BH_REF(OP_TRANSPOSE, transpose, 256)
         BH_REF(OP_AXIS, axis, 128)
+        BH_REF(OP_PROJ, proj, 256)
         BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
         BH_REF(OP_TOPSYM_SCALE, topsym_scale, 256)
         BH_REF(OP_CONTRACT_TOPSYM, contract_topsym, 256)

@@ -92,6 +92,7 @@ enum OpKind {
     OP_TRANSPOSE,
     OP_SWEEP,
     OP_AXIS,
+    OP_PROJ,
     OP_CONTRACT_SIDE,
     OP_TOPSYM_SCALE,
     OP_CONTRACT_TOPSYM,

@@ -171,17 +171,10 @@ void dft_inv64_ops(const bh_plan *p, const double *X, long ldx, int Mcols, doubl
     ops->push_back(gemm_op(a, G64_INV));
 }
 
-// Boundary projections of rho onto the mmax azimuthal modes (bessel.F90:179-185, 228-237)
+// Boundary projections of rho onto the mmax azimuthal modes (bessel.F90:176-184, 226-235)
 void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 {
-    const int N = p->g.numphi;
-    const long M = (long)p->g.numr * p->g.numz;
-    GemmArgs a{};
-    a.A = p->ops->Pm; a.lda = N; a.B = rho; a.ldb = M; a.C = p->proj; a.ldc = M;
-    a.R = 2 * p->g.mmax - 1; a.M = (int)M; a.Kd = N; a.batch = 1;
-    a.tile = M >= 4096 ? p->tile_dft : 2;
-    a.col_fastest = p->order_dft;
-    out->push_back(gemm_op(a, G64_PLAIN));
+    out->push_back(proj_op(rho, p->ops->Pf, p->proj, p->g.numr * p->g.numz, p->g.numphi, p->g.mmax));
 }
 
 // Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF.  The
@@ -626,6 +619,17 @@ extern "C" int bhx_dft32(const bh_plan *p, const double *in, float *out, int nco
     if (!p->ops->dfwd32) return fail(BH_E_STATE, "plan has no FP32 transform");
     DftArgs d{in, ncols, out, ncols, p->ops->dfwd32, p->g.numphi, ncols, 0, 0};
     launch_op(dft_op(d, DFT_FWD_F32), 0);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipDeviceSynchronize());
+    return BH_OK;
+}
+
+// azimuthal mode projections of numphi x numr*numz device data into
+// (2 mmax - 1) x numr*numz; test hook.
+extern "C" int bhx_proj(const bh_plan *p, const double *rho, double *proj)
+{
+    if (!p || !rho || !proj) return fail(BH_E_INVALID, "null argument");
+    launch_op(proj_op(rho, p->ops->Pf, proj, p->g.numr * p->g.numz, p->g.numphi, p->g.mmax), 0);
     HIPCHK(hipGetLastError());
     HIPCHK(hipDeviceSynchronize());
     return BH_OK;

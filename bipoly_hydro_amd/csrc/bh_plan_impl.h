@@ -72,7 +72,7 @@ struct ModeOps {
     DevMem mem;
     Geo g;
     std::vector<double> dt[2];
-    double *Dc = nullptr, *Ds = nullptr, *Ec = nullptr, *Es = nullptr, *Pm = nullptr, *besc = nullptr, *bess = nullptr;
+    double *Dc = nullptr, *Ds = nullptr, *Ec = nullptr, *Es = nullptr, *Pf = nullptr, *besc = nullptr, *bess = nullptr;
     double *alphar = nullptr, *betar = nullptr, *elzb = nullptr;
     // real DFT operands in MFMA fragment order (numphi <= DFT_MAX_N)
     float *dfwd32 = nullptr;

@@ -206,7 +206,7 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
 {
     const size_t numr = d.numr, N = d.numphi, mm = d.mmax, nri = numr - 2, nzi = d.numz - 2;
     const size_t nh = N / 2 + 1, ns = N / 2 - 1;
-    size_t b = 8 * (2 * nh * nh + 2 * ns * ns + (2 * mm - 1) * N + 2 * mm * N + 2 * (numr + 1) + N * (numr + 1));
+    size_t b = 8 * (2 * nh * nh + 2 * ns * ns + (size_t)proj_coef_elems((int)N, (int)mm) + 2 * mm * N + 2 * (numr + 1) + N * (numr + 1));
     if (N <= 512) {
         const size_t kg = (N / 2 + 14) / 16, rt = (N / 2 + 15) / 16 + (N / 2 + 14) / 16;
         b += 8 * 2 * rt * kg * 256 + (precision == BH_PREC_MIXED ? 4 * rt * kg * 256 : 0);
@@ -292,7 +292,7 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     ALLOC(o->Ds, (size_t)ns_ * ns_);
     ALLOC(o->Ec, (size_t)nh * nh);
     ALLOC(o->Es, (size_t)ns_ * ns_);
-    ALLOC(o->Pm, (size_t)(2 * mmax - 1) * N);
+    ALLOC(o->Pf, (size_t)proj_coef_elems(N, mmax));
     ALLOC(o->besc, (size_t)mmax * N);
     ALLOC(o->bess, (size_t)mmax * N);
     ALLOC(o->alphar, numr + 1);
@@ -339,25 +339,19 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
             }
         for (int n = 0; n < ns_; ++n)
             for (int k = 0; k < ns_; ++k) Es[(size_t)n * ns_ + k] = 2.0 * sinr((long)(k + 1) * (n + 1)) / N;
-        // projection rows of bessel (potsetup.F90:131-139): m = 0 (sum),
-        // cos(m (L-1/2) dphi) for m = 1..mmax-1, then sin for m = 1..mmax-1
-        std::vector<double> bc((size_t)mmax * N), bs((size_t)mmax * N), Pm((size_t)(2 * mmax - 1) * N);
+        // bessel's cos(m (L-1/2) dphi) and sin rows (potsetup.F90:131-139),
+        // and the same angles as the quad-folded table of the projection
+        std::vector<double> bc((size_t)mmax * N), bs((size_t)mmax * N), Pf;
         for (int M1 = 1; M1 <= mmax; ++M1)
             for (int L = 1; L <= N; ++L) {
                 double a = 0.5 * g.dphi * ((double)M1 - 1.0) * (2.0 * (double)L - 1.0);
                 bc[(size_t)(M1 - 1) * N + (L - 1)] = std::cos(a);
                 bs[(size_t)(M1 - 1) * N + (L - 1)] = std::sin(a);
             }
-        for (int n = 0; n < N; ++n) {
-            Pm[n] = 1.0;
-            for (int mm = 1; mm < mmax; ++mm) {
-                Pm[(size_t)mm * N + n] = bc[(size_t)mm * N + n];
-                Pm[(size_t)(mmax - 1 + mm) * N + n] = bs[(size_t)mm * N + n];
-            }
-        }
+        proj_coefficients(N, mmax, g.dphi, &Pf);
         if ((rc = up(o->Dc, std::move(Dc))) != BH_OK || (rc = up(o->Ds, std::move(Ds))) != BH_OK ||
             (rc = up(o->Ec, std::move(Ec))) != BH_OK || (rc = up(o->Es, std::move(Es))) != BH_OK ||
-            (rc = up(o->Pm, std::move(Pm))) != BH_OK || (rc = up(o->besc, std::move(bc))) != BH_OK ||
+            (rc = up(o->Pf, std::move(Pf))) != BH_OK || (rc = up(o->besc, std::move(bc))) != BH_OK ||
             (rc = up(o->bess, std::move(bs))) != BH_OK)
             return rc;
     }

@@ -133,3 +133,33 @@ def test_fp32_forward_dft_matches_fft(numphi, ncols):
         X = np.fft.rfft(x, axis=0)
         ref = np.concatenate([X.real[: numphi // 2 + 1], X.imag[1: numphi // 2]], axis=0)
         np.testing.assert_allclose(dy.cpu().numpy(), ref, rtol=0, atol=3e-6 * numphi)
+
+
+@pytest.mark.parametrize("numr,numz,numphi,mmax", [(10, 10, 16, 30), (66, 34, 64, 30), (258, 130, 512, 30),
+                                                   (34, 18, 128, 5), (18, 18, 32, 40), (12, 10, 4, 30),
+                                                   (10, 12, 8, 70), (66, 66, 256, 64)])
+def test_mode_projections_match_direct_sums(numr, numz, numphi, mmax):
+    """The quad-folded projection kernel against the direct sums over all
+    planes (bessel's TMPC/TMPS): one or several 32-mode chunks (mmax 40, 64,
+    70), modes past numphi/2 (aliased), fewer than 16 planes (quads padded
+    with zero coefficients), and column counts that are not multiples of 64."""
+    torch = pytest.importorskip("torch")
+    L = P.load_library()
+    vp = ctypes.c_void_p
+    L.bhx_proj.argtypes = [vp, vp, vp]
+    M = numr * numz
+    rng = np.random.default_rng(numphi + mmax)
+    rho = rng.standard_normal((numphi, M))
+    with P.PotentialPlan(numr, numz, numphi, mmax=mmax) as plan:
+        drho = torch.from_numpy(rho).cuda()
+        dout = torch.full((2 * mmax - 1, M), np.nan, dtype=torch.float64, device="cuda")
+        assert L.bhx_proj(plan.handle, drho.data_ptr(), dout.data_ptr()) == 0
+        got = dout.cpu().numpy()
+    theta = (np.arange(numphi) + 0.5) * (2 * np.pi / numphi)
+    m = np.arange(1, mmax)[:, None]
+    ref = np.concatenate([rho.sum(axis=0)[None], np.cos(m * theta) @ rho, np.sin(m * theta) @ rho], axis=0)
+    # the folds take cos(m theta) of plane N/2-1-a as (-1)^m cos(m theta_a):
+    # equal up to the rounding of angles as large as m pi
+    scale = np.abs(rho).sum(axis=0).max() * max(1.0, mmax / 10)
+    assert np.all(np.isfinite(got))
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-14 * scale)
