@@ -159,14 +159,15 @@ def main():
     lat_p50, lat_p99 = mixed["p50_ms"], mixed["p99_ms"]
 
     # Dominant kernel of the measured (mixed) solve: the top/bottom boundary
-    # contraction, bound by reading the FP64 Green's table once.  The table
-    # is stored in blocked symmetric form (16 x 16 blocks (I, J), J <= I, of
-    # targets x sources per source plane and mode): mmax x NB(NB+1)/2 x
-    # (numz-2) x 256 x 8 B, NB = ceil((numr-2)/16).  The timed span (events
-    # around it) also holds the projection scaling and the final reduction.
+    # contraction, bound by reading the Green's table once (FP32 in the mixed
+    # solve).  The table is stored in blocked symmetric form (16 x 16 blocks
+    # (I, J), J <= I, of targets x sources per source plane and mode): mmax x
+    # NB(NB+1)/2 x (numz-2) x 256 x 4 B, NB = ceil((numr-2)/16).  The timed
+    # span (events around it) also holds the projection scaling and the final
+    # reduction.
     nri, nzi, mmax = numr - 2, numz - 2, 30
     nb = (nri + 15) // 16
-    ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 8
+    ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 4
     ct_ms = mixed["stage_ms"]["contract_topbot"]
     ct_traffic = CONTRACT_TRAFFIC if (numr, numz, numphi) == (258, 130, 512) else None
     # ADI half-step kernel of the FP64 path: 3 streams (read phi, read
@@ -198,8 +199,8 @@ def main():
         "config": {"workload": f"potential_solver {'warm (5 ADI steps)' if args.warm else 'cold (20 ADI steps)'}"
                                f" {numr}x{numz}x{numphi}, isym=1, mmax=30, binary n=1 polytropes, "
                                "inputs resident in HBM",
-                   "grid": [numr, numz, numphi], "precision": "mixed (FP32 MFMA transforms, FP64 "
-                   "tables, residual, 1 refinement pass)", "tstep": tstep},
+                   "grid": [numr, numz, numphi], "precision": "mixed (FP32 MFMA transforms, FP32 "
+                   "top/bottom Green's table, FP64 sums, residual, 1 refinement pass)", "tstep": tstep},
         "p50_ms": lat_p50,
         "p99_ms": lat_p99,
         "relL2_vs_reference_slices": mixed["relL2_vs_reference_slices"],

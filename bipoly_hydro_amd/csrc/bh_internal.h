@@ -52,8 +52,9 @@ struct ContractArgs {
     const double *table, *proj;
     double *out;
     int numr, numz, mmax;
-    const double *aux;  // sqrt(rhf) [numr + 1], 1-based J (symmetric top table)
-    const double *unused;
+    const double *aux;     // sqrt(rhf) [numr + 1], 1-based J (symmetric top table)
+    const float *table32;  // FP32 symmetric top table (in place of table)
+    int nparts;            // partial sums per value (top/bottom reduction)
 };
 struct ProjArgs {
     const double *rho;   // [numphi][M]
@@ -99,7 +100,8 @@ struct TableArgs {
     const double *nu;       // [mmax + 1] 1-based
 };
 // blocked symmetric top table TS[m][pair][kz][16][16] (see bh_kernels.hip)
-void launch_gen_topsym_table(const TableArgs &a, double *T, hipStream_t s);
+// into T (FP64) or, when T is null, T32 (FP32)
+void launch_gen_topsym_table(const TableArgs &a, double *T, float *T32, hipStream_t s);
 inline size_t topsym_table_elems(int numr, int numz, int mmax)
 {
     const size_t NB = (numr - 2 + 15) / 16;
@@ -117,16 +119,33 @@ __host__ __device__ inline int topsym_splits(int numz) { return (numz - 2 + TS_P
 // proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
 // Stb: [mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
 // Ss:  [CT_SPLIT][mmax][2][nzi]
-// symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4),
-// contract (Ptb, mmax*topsym_splits*ceil(NB/2)*NB16*4), reduce into Stb (one slot)
 // proj from rho: the azimuthal mode sums of every (J, K) column, the planes
 // folded in quads (FP64 matrix cores); coef from proj_coefficients
 constexpr int PROJ_MODES = 32;  // modes per chunk (grid x)
 int proj_coef_elems(int numphi, int mmax);
 Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax);
 void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef);
+// Order of the 256 values of one 16 x 16 block plane of the symmetric top
+// table: sixteen 4 x 4 sub-blocks (rows 4ig.., columns 4jg..), stored so
+// that 16-byte piece h of sub-block g = 4ig + jg sits at (h * 16 + g) * 16 B
+// (FP32: piece h = row 4ig + h; FP64: row 4ig + h/2, columns 4jg + 2(h%2)..)
+template <class TT>
+__host__ __device__ inline int tb_off(int i, int j)
+{
+    if (sizeof(TT) == 4) return (((i & 3) * 16 + (i >> 2) * 4 + (j >> 2)) << 2) + (j & 3);
+    return ((((i & 3) * 2 + ((j & 3) >> 1)) * 16 + (i >> 2) * 4 + (j >> 2)) << 1) + (j & 1);
+}
+// the register-blocked contraction takes TB_PLANES source planes per
+// workgroup and grids with nri <= 256 (partials Ptb[m][chunk][NB16][4])
+constexpr int TB_PLANES = 8;
+__host__ __device__ inline int tb_chunks(int numz) { return (numz - 2 + TB_PLANES - 1) / TB_PLANES; }
+__host__ __device__ inline bool tb_blocked(int numr) { return numr - 2 <= 256; }
+// symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4),
+// contract (Ptb, mmax*topsym_splits*ceil(NB/2)*NB16*4), reduce into Stb (one slot)
 Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int numr, int numz, int mmax);
-Op contract_topsym_op(const double *TS, const double *Pt, double *Ptb, int numr, int numz, int mmax);
+// TS is the FP64 table; with TS null, TS32 the FP32 one
+Op contract_topsym_op(const double *TS, const float *TS32, const double *Pt, double *Ptb, int numr, int numz,
+                      int mmax);
 Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax);
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
 // faces in Fourier mode order FF[mode row][fi]: fi in [0,nri) top J=fi+2,

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <type_traits>
 
 namespace bh {
 
@@ -398,8 +399,10 @@ __device__ double d_rd(double x, double y, double z)
                            (ave * sqrt(ave));
 }
 
-// Writes coef * Q_{m-1/2}(xm) to out[m * mstride], m = 0..mmax-1.
-__device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, double *out, long mstride)
+// Writes coef * Q_{m-1/2}(xm) to out[m * mstride], m = 0..mmax-1 (rounded
+// to the table's type).
+template <class TT>
+__device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, TT *out, long mstride)
 {
     const int mmax = a.mmax;
     if (xm < a.hypr_cutoff) {
@@ -409,14 +412,14 @@ __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, double
         double kk = d_rf(0.0, kq, 1.0);
         double ee = kk - mum * mum * d_rd(0.0, kq, 1.0) * (1.0 / 3.0);
         double q0 = kk * mum;
-        out[0] = coef * q0;
+        out[0] = (TT)(coef * q0);
         if (mmax < 2) return;
         double q1 = xm * mum * kk - lam * ee;
-        out[mstride] = coef * q1;
+        out[mstride] = (TT)(coef * q1);
         for (int m = 3; m <= mmax; ++m) {
             double nu = a.nu[m];
             double q2 = (2.0 * nu + 1.0) / (nu + 1.0) * xm * q1 - nu / (nu + 1.0) * q0;
-            out[(long)(m - 1) * mstride] = coef * q2;
+            out[(long)(m - 1) * mstride] = (TT)(coef * q2);
             q0 = q1;
             q1 = q2;
         }
@@ -427,7 +430,7 @@ __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, double
             const double *c = a.cser + (long)m * a.nser;
             double s = c[a.nser - 1];
             for (int ir = a.nser - 2; ir >= 0; --ir) s = s * y + c[ir];
-            out[(long)m * mstride] = coef * (exp(a.coefh[m] - ((double)m + 0.5) * l2x) * s);
+            out[(long)m * mstride] = (TT)(coef * (exp(a.coefh[m] - ((double)m + 0.5) * l2x) * s));
         }
     }
 }
@@ -436,9 +439,11 @@ __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, double
 // source radius R' and the target radius R only through chi, which is
 // symmetric in them, so only the blocks (I, J), J <= I, of 16 x 16 targets
 // x sources are stored: TS[m][pair][kz][i][j], pair = I (I + 1) / 2 + J,
-// target jo = 16 I + i, source jr = 16 J + j, value Q / pi (the sqrt(R'/R)
-// factor of tm.F90:162 is applied to the projections and the result).
-__global__ __launch_bounds__(128) void gen_topsym_table_kernel(TableArgs a, double *T)
+// target jo = 16 I + i, source jr = 16 J + j (block plane order tb_off),
+// value Q / pi (the sqrt(R'/R) factor of tm.F90:162 is applied to the
+// projections and the result).
+template <class TT>
+__global__ __launch_bounds__(128) void gen_topsym_table_kernel(TableArgs a, TT *T)
 {
     const int nri = a.numr - 2, nzi = a.numz - 2, NB = (nri + 15) / 16, NP = NB * (NB + 1) / 2;
     long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -452,9 +457,9 @@ __global__ __launch_bounds__(128) void gen_topsym_table_kernel(TableArgs a, doub
     const int J = pair - I * (I + 1) / 2;
     const int jo = 16 * I + i, jr = 16 * J + j;
     const long mstride = (long)NP * nzi * 256;
-    double *out = T + ((long)pair * nzi + kz) * 256 + e;
+    TT *out = T + ((long)pair * nzi + kz) * 256 + tb_off<TT>(i, j);
     if (jo >= nri || jr >= nri) {
-        for (int m = 0; m < a.mmax; ++m) out[(long)m * mstride] = 0.0;
+        for (int m = 0; m < a.mmax; ++m) out[(long)m * mstride] = (TT)0;
         return;
     }
     const int jl = jr + 2, kl = kz + 2, jg = jo + 2;
@@ -482,11 +487,13 @@ __global__ __launch_bounds__(128) void gen_side_table_kernel(TableArgs a, double
     d_qhalf_store(a, xm, coef, S + (long)d * nri + jr, (long)nd * nri);
 }
 
-void launch_gen_topsym_table(const TableArgs &a, double *T, hipStream_t s)
+void launch_gen_topsym_table(const TableArgs &a, double *T, float *T32, hipStream_t s)
 {
     const long NB = (a.numr - 2 + 15) / 16;
     const long total = NB * (NB + 1) / 2 * (a.numz - 2) * 256;
-    hipLaunchKernelGGL(gen_topsym_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, T);
+    const dim3 grid((unsigned)((total + 127) / 128));
+    if (T) hipLaunchKernelGGL(gen_topsym_table_kernel<double>, grid, dim3(128), 0, s, a, T);
+    else hipLaunchKernelGGL(gen_topsym_table_kernel<float>, grid, dim3(128), 0, s, a, T32);
 }
 
 void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
@@ -670,7 +677,10 @@ Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int n
 // target jo, the transposed part (off the diagonal) T_ij Pt(jo) to target
 // jr.  Per-target partial sums of the workgroup go to
 // Ptb[m][split][w][jo][q], reduced in a fixed order by topsym_reduce.
-__device__ __forceinline__ void contract_topsym_tile(const ContractArgs &a, int bx, int by, int bz)
+// TT: the table's element type (FP32 tables serve the mixed solve; the
+// products and sums are FP64 either way).
+template <class TT>
+__device__ __forceinline__ void contract_topsym_t(const ContractArgs &a, const TT *table, int bx, int by, int bz)
 {
     const int numr = a.numr, numz = a.numz, nri = numr - 2, nzi = numz - 2;
     const int NB = (nri + 15) / 16, NP = NB * (NB + 1) / 2, NB16 = 16 * NB, W = (NB + 1) / 2;
@@ -695,10 +705,10 @@ __device__ __forceinline__ void contract_topsym_tile(const ContractArgs &a, int 
             const bool off = J != I;
             __syncthreads();  // sJ free (and sI written)
             sJ[i][j] = i < nkz ? Pt[(long)i * NB16 + 16 * J + j] : zero4;
-            const double *blk = a.table + (((long)m * NP + I * (I + 1) / 2 + J) * nzi + kz0) * 256 + tid;
+            const TT *blk = table + (((long)m * NP + I * (I + 1) / 2 + J) * nzi + kz0) * 256 + tb_off<TT>(i, j);
             double q[KZC];
 #pragma unroll
-            for (int u = 0; u < KZC; ++u) q[u] = u < nkz ? blk[(long)u * 256] : 0.0;
+            for (int u = 0; u < KZC; ++u) q[u] = u < nkz ? (double)blk[(long)u * 256] : 0.0;
             __syncthreads();  // sJ written
             double4 col = zero4;
 #pragma unroll
@@ -743,11 +753,215 @@ __device__ __forceinline__ void contract_topsym_tile(const ContractArgs &a, int 
     for (int k = tid; k < NB16; k += 256) out[k] = y[k];
 }
 
-Op contract_topsym_op(const double *TS, const double *Pt, double *Ptb, int numr, int numz, int mmax)
+// ---------------------------------------------------------------------------
+// Top/bottom contraction over the symmetric table, register-blocked (grids
+// with nri <= 256).  A workgroup of TB_GROUPS 16-lane groups takes one mode
+// m and TB_PLANES source planes, whose scaled projections (TB_PLANES x NB16
+// double4) sit in LDS.  Lane (ig, jg) of a group owns rows 4ig..4ig+3 and
+// columns 4jg..4jg+3 of the group's 16 x 16 block (I, J), reads them as
+// 16-byte pieces (the table's block plane order), and sums over the planes,
+// in registers, the rows (targets 16I + 4ig + a, from the column-block
+// projections) and the columns (targets 16J + 4jg + b, from the row-block
+// ones).  The groups take the blocks in the rounds of a round-robin
+// tournament among the NB row blocks (every round pairs each block with
+// one other, so the 2 x TB_GROUPS targets of a round are distinct), then
+// the diagonal blocks (rows only), and after each round every group adds
+// its sums into the LDS totals: no two groups touch one total in a round
+// and the rounds come in a fixed order, so the sums are reproducible bit
+// for bit.  Table loads run TB_RING plane steps ahead, across rounds.
+// ---------------------------------------------------------------------------
+constexpr int TB_GROUPS = 8;
+
+// round r (0..n-2) of the circle-method round robin among n (even) teams:
+// match k (0..n/2-1)
+__device__ inline void tb_match(int n, int r, int k, int *x, int *y)
 {
-    ContractArgs a{TS, Pt, Ptb, numr, numz, mmax, nullptr, nullptr};
+    if (k == 0) {
+        *x = n - 1;
+        *y = r;
+    } else {
+        *x = (r + k) % (n - 1);
+        *y = (r - k + n - 1) % (n - 1);
+    }
+}
+
+template <class TT>
+__device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *table, int chunk, int m, int)
+{
+    typedef typename std::conditional<sizeof(TT) == 4, float4, double2>::type Piece;
+    constexpr int V = 16 / sizeof(TT);        // values per 16-byte piece
+    constexpr int NL = 16 / V;                // pieces per lane and plane
+    constexpr int RING = sizeof(TT) == 4 ? 4 : 2;
+    static_assert(TB_PLANES % RING == 0, "ring must divide the planes");
+    const int nri = a.numr - 2, nzi = a.numz - 2;
+    const int NB = (nri + 15) / 16, NP = NB * (NB + 1) / 2, NB16 = 16 * NB;
+    const int kz0 = chunk * TB_PLANES, nkz = min(TB_PLANES, nzi - kz0);
+    const int tid = threadIdx.x, g = tid >> 4, l16 = tid & 15, ig = l16 >> 2, jg = l16 & 3;
+    __shared__ double4 Ps[TB_PLANES][256];
+    __shared__ double4 y[256];
+    const double4 zero4 = {0.0, 0.0, 0.0, 0.0};
+    const double4 *Pt = (const double4 *)a.proj + ((long)m * nzi + kz0) * NB16;
+    for (int e = tid; e < TB_PLANES * NB16; e += 16 * TB_GROUPS) {
+        const int u = e / NB16, p = e - u * NB16;
+        Ps[u][p] = u < nkz ? Pt[(long)u * NB16 + p] : zero4;
+    }
+    for (int e = tid; e < NB16; e += 16 * TB_GROUPS) y[e] = zero4;
+    __syncthreads();
+
+    // schedule: Roff rounds of off-diagonal blocks (sub-rounds when the
+    // tournament has more matches than groups), then the diagonal blocks
+    const int n = NB + (NB & 1), half = n / 2, sub = (half + TB_GROUPS - 1) / TB_GROUPS;
+    const int Roff = (n - 1) * sub, Rtot = Roff + (NB + TB_GROUPS - 1) / TB_GROUPS;
+    struct Blk {
+        int I, J;
+        bool ok;
+    };
+    auto block_of = [&](int R) {
+        Blk b;
+        R = min(R, Rtot - 1);
+        if (R < Roff) {
+            const int k = (R % sub) * TB_GROUPS + g;
+            int x, yy;
+            tb_match(n, R / sub, min(k, half - 1), &x, &yy);
+            b.ok = k < half && x < NB && yy < NB;
+            b.I = b.ok ? max(x, yy) : 0;
+            b.J = b.ok ? min(x, yy) : 0;
+        } else {
+            const int d = (R - Roff) * TB_GROUPS + g;
+            b.ok = d < NB;
+            b.I = b.J = b.ok ? d : 0;
+        }
+        return b;
+    };
+    const TT *tb = table + ((long)m * NP * nzi + kz0) * 256 + l16 * V;
+    auto load = [&](Piece *dst, const Blk &b, int u) {
+        const TT *src = tb + ((long)(b.I * (b.I + 1) / 2 + b.J) * nzi + min(u, nkz - 1)) * 256;
+#pragma unroll
+        for (int h = 0; h < NL; ++h) dst[h] = *(const Piece *)(src + h * 16 * V);
+    };
+    Piece ring[RING][NL];
+    Blk cur = block_of(0), nxt = block_of(1);
+#pragma unroll
+    for (int s = 0; s < RING; ++s) load(ring[s], cur, s);
+    __builtin_amdgcn_sched_barrier(0);
+
+    auto tv = [&](const Piece *t, int r, int c) -> double {
+        if constexpr (sizeof(TT) == 4) {
+            const float4 &q = t[r];
+            return (double)(c == 0 ? q.x : c == 1 ? q.y : c == 2 ? q.z : q.w);
+        } else {
+            const double2 &q = t[2 * r + (c >> 1)];
+            return (c & 1) ? q.y : q.x;
+        }
+    };
+    auto fma4 = [](double4 &acc, double s, const double4 &v) {
+        acc.x = fma(s, v.x, acc.x);
+        acc.y = fma(s, v.y, acc.y);
+        acc.z = fma(s, v.z, acc.z);
+        acc.w = fma(s, v.w, acc.w);
+    };
+    auto sel = [](bool c, const double4 &p, const double4 &q) {
+        return (double4){c ? p.x : q.x, c ? p.y : q.y, c ? p.z : q.z, c ? p.w : q.w};
+    };
+    auto shx = [](const double4 &v, int mask) {
+        return (double4){__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
+                         __shfl_xor(v.w, mask, 64)};
+    };
+    auto add4 = [](const double4 &p, const double4 &q) {
+        return (double4){p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w};
+    };
+    // sum of v[0..3] over the lanes of bits m1, m2 of the lane index; lane
+    // with bits (b2 b1) ends with the total of v[2 b2 + b1]
+    auto reduce_scatter = [&](const double4 *v, int m1, int m2) {
+        const bool b1 = (l16 & m1) != 0, b2 = (l16 & m2) != 0;
+        const double4 k0 = add4(sel(b1, v[1], v[0]), shx(sel(b1, v[0], v[1]), m1));
+        const double4 k1 = add4(sel(b1, v[3], v[2]), shx(sel(b1, v[2], v[3]), m1));
+        return add4(sel(b2, k1, k0), shx(sel(b2, k0, k1), m2));
+    };
+
+    auto round = [&](auto off_tag) {
+        constexpr bool OFF = decltype(off_tag)::value;
+        double4 rowp[4], colp[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) rowp[k] = colp[k] = zero4;
+#pragma unroll
+        for (int u = 0; u < TB_PLANES; ++u) {
+            const int slot = u % RING;
+            Piece t[NL];
+#pragma unroll
+            for (int h = 0; h < NL; ++h) t[h] = ring[slot][h];
+            if (u + RING < TB_PLANES) load(ring[slot], cur, u + RING);
+            else load(ring[slot], nxt, u + RING - TB_PLANES);
+            __builtin_amdgcn_sched_barrier(0);
+            const double4 *pp = Ps[u];
+            double4 pj[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) pj[c] = pp[16 * cur.J + 4 * jg + c];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const double4 pi = pp[16 * cur.I + 4 * ig + r];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const double q = tv(t, r, c);
+                    fma4(rowp[r], q, pj[c]);
+                    if (OFF) fma4(colp[c], q, pi);
+                }
+            }
+        }
+        // rows over the 4 lanes jg, columns over the 4 lanes ig
+        const double4 rs = reduce_scatter(rowp, 1, 2);
+        double4 cs = zero4;
+        if (OFF) cs = reduce_scatter(colp, 4, 8);
+        if (cur.ok) {
+            double4 &yr = y[16 * cur.I + 4 * ig + jg];
+            yr = add4(yr, rs);
+            if (OFF) {
+                double4 &yc = y[16 * cur.J + 4 * jg + ig];
+                yc = add4(yc, cs);
+            }
+        }
+        __syncthreads();
+    };
+    int R = 0;
+    for (; R < Roff; ++R) {
+        round(std::integral_constant<bool, true>());
+        cur = nxt;
+        nxt = block_of(R + 2);
+    }
+    for (; R < Rtot; ++R) {
+        round(std::integral_constant<bool, false>());
+        cur = nxt;
+        nxt = block_of(R + 2);
+    }
+    double4 *out = (double4 *)a.out + ((long)m * tb_chunks(a.numz) + chunk) * NB16;
+    for (int e = tid; e < NB16; e += 16 * TB_GROUPS) out[e] = y[e];
+}
+
+__device__ __forceinline__ void contract_tb64_tile(const ContractArgs &a, int bx, int by, int bz)
+{
+    contract_tb_t<double>(a, a.table, bx, by, bz);
+}
+__device__ __forceinline__ void contract_tb32_tile(const ContractArgs &a, int bx, int by, int bz)
+{
+    contract_tb_t<float>(a, a.table32, bx, by, bz);
+}
+
+__device__ __forceinline__ void contract_topsym_tile(const ContractArgs &a, int bx, int by, int bz)
+{
+    contract_topsym_t<double>(a, a.table, bx, by, bz);
+}
+__device__ __forceinline__ void contract_topsym32_tile(const ContractArgs &a, int bx, int by, int bz)
+{
+    contract_topsym_t<float>(a, a.table32, bx, by, bz);
+}
+
+Op contract_topsym_op(const double *TS, const float *TS32, const double *Pt, double *Ptb, int numr, int numz,
+                      int mmax)
+{
+    ContractArgs a{TS, Pt, Ptb, numr, numz, mmax, nullptr, TS32, 0};
+    if (tb_blocked(numr)) return Op::make(OP_CONTRACT_TOPSYM, TS ? 2 : 3, a, tb_chunks(numz), mmax, 1);
     const int NB = (numr - 2 + 15) / 16;
-    return Op::make(OP_CONTRACT_TOPSYM, 0, a, (NB + 1) / 2, mmax, topsym_splits(numz));
+    return Op::make(OP_CONTRACT_TOPSYM, TS ? 0 : 1, a, (NB + 1) / 2, mmax, topsym_splits(numz));
 }
 
 // Stb[m][q][jo] = sum over (split, w) of Ptb / sqrt(rhf(jo + 2)), in a fixed order
@@ -756,7 +970,7 @@ __device__ __forceinline__ void topsym_reduce_tile(const ContractArgs &a, int bx
     const int nri = a.numr - 2, NB = (nri + 15) / 16, NB16 = 16 * NB, W = (NB + 1) / 2;
     const int m = by, e = bx * 256 + threadIdx.x, q = e / nri, jo = e % nri;
     if (q >= 4) return;
-    const int nk = topsym_splits(a.numz) * W;
+    const int nk = a.nparts ? a.nparts : topsym_splits(a.numz) * W;
     const double *P = a.proj + ((long)m * nk * NB16 + jo) * 4 + q;
     double s = 0.0;
     for (int k0 = 0; k0 < nk; k0 += 8) {
@@ -771,7 +985,7 @@ __device__ __forceinline__ void topsym_reduce_tile(const ContractArgs &a, int bx
 
 Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax)
 {
-    ContractArgs a{nullptr, Ptb, Stb, numr, numz, mmax, sqrt_rhf, nullptr};
+    ContractArgs a{nullptr, Ptb, Stb, numr, numz, mmax, sqrt_rhf, nullptr, tb_blocked(numr) ? tb_chunks(numz) : 0};
     return Op::make(OP_TOPSYM_REDUCE, 0, a, (4 * (numr - 2) + 255) / 256, mmax, 1);
 }
 
@@ -975,6 +1189,9 @@ BH_KERNEL_PAIR(proj, ProjArgs, 256)
 BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_tb64, ContractArgs, 128)
+BH_KERNEL_PAIR(contract_tb32, ContractArgs, 128)
 BH_KERNEL_PAIR(topsym_reduce, ContractArgs, 256)
 BH_KERNEL_PAIR(face_modes, RebuildArgs, 128)
 BH_KERNEL_PAIR(face_scatter, ScatterArgs, 128)
@@ -1000,7 +1217,14 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         BH_REF(OP_PROJ, proj, 256)
         BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
         BH_REF(OP_TOPSYM_SCALE, topsym_scale, 256)
-        BH_REF(OP_CONTRACT_TOPSYM, contract_topsym, 256)
+    case OP_CONTRACT_TOPSYM:
+        switch (variant) {
+        case 0: *k = {(const void *)&contract_topsym_kernel, (const void *)&contract_topsym_grouped, 256}; break;
+        case 1: *k = {(const void *)&contract_topsym32_kernel, (const void *)&contract_topsym32_grouped, 256}; break;
+        case 2: *k = {(const void *)&contract_tb64_kernel, (const void *)&contract_tb64_grouped, 128}; break;
+        default: *k = {(const void *)&contract_tb32_kernel, (const void *)&contract_tb32_grouped, 128}; break;
+        }
+        return true;
         BH_REF(OP_TOPSYM_REDUCE, topsym_reduce, 256)
         BH_REF(OP_FACE_MODES, face_modes, 128)
         BH_REF(OP_SCATTER, face_scatter, 128)

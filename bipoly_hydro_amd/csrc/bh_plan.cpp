@@ -52,7 +52,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     if (const char *e = std::getenv("BH_SIDE_CUS")) p->side_cus = std::atoi(e);
     bool created;
     int rc;
-    if ((rc = get_tables(h, device, &p->tab, &created)) != BH_OK) return rc;
+    if ((rc = get_tables(h, precision, device, &p->tab, &created)) != BH_OK) return rc;
     if ((rc = get_mode_ops(h, precision, device, &p->ops, &created)) != BH_OK) return rc;
 
     const Geo &g = p->g;
@@ -69,7 +69,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     {
         const size_t NB = (nri + 15) / 16;
         ALLOC(p->Pt, (size_t)mmax * nzi * 16 * NB * 4);
-        ALLOC(p->Ptb, (size_t)mmax * topsym_splits(numz) * ((NB + 1) / 2) * 16 * NB * 4);
+        const size_t parts = tb_blocked(numr) ? tb_chunks(numz) : topsym_splits(numz) * ((NB + 1) / 2);
+        ALLOC(p->Ptb, (size_t)mmax * parts * 16 * NB * 4);
     }
     ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
     ALLOC(p->FF, (size_t)N * nf);
@@ -190,7 +191,7 @@ void boundary_ops(const bh_plan *p, std::vector<Op> *out)
     out->push_back(Op::mark(MARK_SIDE_END));
     out->push_back(Op::mark(5));
     out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
-    out->push_back(contract_topsym_op(p->tab->T, p->Pt, p->Ptb, numr, numz, mmax));
+    out->push_back(contract_topsym_op(p->tab->T, p->tab->T32, p->Pt, p->Ptb, numr, numz, mmax));
     out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
     out->push_back(Op::mark(MARK_JOIN));

@@ -62,6 +62,7 @@ void host_geometry(const bh_grid_desc &d, HostGeo *h);
 struct GeoTables {
     DevMem mem;
     double *T = nullptr;         // top (and mirrored bottom) table, blocked symmetric [m][pair][kz][16][16]
+    float *T32 = nullptr;        // ... the same in FP32 (mixed plans; T is then null)
     double *S = nullptr;         // side table, Toeplitz in z [m][d][jr]
     double *sqrt_rhf = nullptr;  // sqrt(rhf) [numr + 1], 1-based J
 };
@@ -91,7 +92,7 @@ struct ModeOps {
 
 // Shared resources from the per-process cache (created on first use; kept
 // while any plan holds them).  `created` reports whether this call built them.
-int get_tables(const HostGeo &h, int device, std::shared_ptr<const GeoTables> *out, bool *created);
+int get_tables(const HostGeo &h, int precision, int device, std::shared_ptr<const GeoTables> *out, bool *created);
 int get_mode_ops(const HostGeo &h, int precision, int device, std::shared_ptr<const ModeOps> *out, bool *created);
 // Bytes of device memory a plan would add: its own buffers, and the shared
 // resources not yet in the cache.

@@ -180,10 +180,17 @@ std::mutex g_cache_mu;
 std::map<std::string, std::weak_ptr<const GeoTables>> g_tables;
 std::map<std::string, std::weak_ptr<const ModeOps>> g_modeops;
 
-std::string tables_key(const bh_grid_desc &d, int device)
+// mixed plans contract an FP32 copy of the top table: its rounding (2^-24
+// relative per entry, independent) moves the potential by about 1e-10
+// relative, far inside the mixed tolerance, and halves the bytes of the
+// HBM-bound contraction
+bool table32(int precision) { return precision == BH_PREC_MIXED; }
+
+std::string tables_key(const bh_grid_desc &d, int precision, int device)
 {
     char b[160];
-    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d", device, d.numr, d.numz, d.dr, d.mmax);
+    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d", device, d.numr, d.numz, d.dr, d.mmax,
+                  table32(precision) ? 32 : 64);
     return b;
 }
 
@@ -196,10 +203,11 @@ std::string modeops_key(const bh_grid_desc &d, int precision, int device)
     return b;
 }
 
-size_t tables_bytes(const bh_grid_desc &d)
+size_t tables_bytes(const bh_grid_desc &d, int precision)
 {
     const size_t nri = d.numr - 2, nzi = d.numz - 2, mm = d.mmax;
-    return 8 * (topsym_table_elems(d.numr, d.numz, d.mmax) + mm * (2 * nzi - 1) * nri + d.numr + 1);
+    return (table32(precision) ? 4 : 8) * topsym_table_elems(d.numr, d.numz, d.mmax) +
+           8 * (mm * (2 * nzi - 1) * nri + d.numr + 1);
 }
 
 size_t modeops_bytes(const bh_grid_desc &d, int precision)
@@ -222,11 +230,13 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     return b;
 }
 
-int build_tables(const HostGeo &h, GeoTables *t)
+int build_tables(const HostGeo &h, bool f32, GeoTables *t)
 {
     const int numr = h.g.numr, numz = h.g.numz, mmax = h.g.mmax, nri = h.g.nri, nzi = h.g.nzi;
     int rc;
-    if ((rc = t->mem.alloc(&t->T, topsym_table_elems(numr, numz, mmax))) != BH_OK) return rc;
+    if ((rc = f32 ? t->mem.alloc(&t->T32, topsym_table_elems(numr, numz, mmax))
+                  : t->mem.alloc(&t->T, topsym_table_elems(numr, numz, mmax))) != BH_OK)
+        return rc;
     if ((rc = t->mem.alloc(&t->S, (size_t)mmax * (2 * nzi - 1) * nri)) != BH_OK) return rc;
     if ((rc = t->mem.alloc(&t->sqrt_rhf, numr + 1)) != BH_OK) return rc;
     // Green's tables on the device (tm.F90:103-200, sm.F90:118-215)
@@ -263,7 +273,7 @@ int build_tables(const HostGeo &h, GeoTables *t)
     ta.piinv = h.g.piinv; ta.dz = h.g.dz;
     ta.rhf_g = d_rhf; ta.rhfinv_g = d_rhfinv; ta.zhf_g = d_zhf;
     ta.coefh = d_coefh; ta.cser = d_cser; ta.nu = d_nu;
-    launch_gen_topsym_table(ta, t->T, ss.s);
+    launch_gen_topsym_table(ta, t->T, t->T32, ss.s);
     launch_gen_side_table(ta, t->S, ss.s);
     std::vector<double> sq(numr + 1, 0.0);
     for (int J = 1; J <= numr; ++J) sq[J] = std::sqrt(h.rhf_g[J]);
@@ -539,10 +549,10 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
 
 }  // namespace
 
-int get_tables(const HostGeo &h, int device, std::shared_ptr<const GeoTables> *out, bool *created)
+int get_tables(const HostGeo &h, int precision, int device, std::shared_ptr<const GeoTables> *out, bool *created)
 {
     std::lock_guard<std::mutex> lk(g_cache_mu);
-    const std::string key = tables_key(h.d, device);
+    const std::string key = tables_key(h.d, precision, device);
     *created = false;
     auto it = g_tables.find(key);
     if (it != g_tables.end()) {
@@ -552,7 +562,7 @@ int get_tables(const HostGeo &h, int device, std::shared_ptr<const GeoTables> *o
         }
     }
     auto t = std::make_shared<GeoTables>();
-    int rc = build_tables(h, t.get());
+    int rc = build_tables(h, table32(precision), t.get());
     if (rc != BH_OK) return rc;
     g_tables[key] = t;
     *out = t;
@@ -587,7 +597,8 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
     const size_t nf = 2 * nri + nzi;
     const size_t NB16 = 16 * ((nri + 15) / 16), W = (nri + 31) / 32;
     size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf +
-                    mm * nzi * NB16 * 4 + mm * topsym_splits(d.numz) * W * NB16 * 4);
+                    mm * nzi * NB16 * 4 +
+                    mm * (tb_blocked(d.numr) ? tb_chunks(d.numz) : topsym_splits(d.numz) * W) * NB16 * 4);
     if (precision == BH_PREC_MIXED) {
         // RH32, r32, D1 + Wf; RT, CT, DT + WT only for the separate-kernel step
         const bool fused = spectral_fused_ok((int)nri, (int)nzi);
@@ -599,15 +610,15 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
 
 size_t plan_shared_bytes(const bh_grid_desc &d, int precision)
 {
-    return tables_bytes(d) + modeops_bytes(d, precision);
+    return tables_bytes(d, precision) + modeops_bytes(d, precision);
 }
 
 size_t plan_new_shared_bytes(const bh_grid_desc &d, int precision, int device)
 {
     std::lock_guard<std::mutex> lk(g_cache_mu);
     size_t b = 0;
-    auto t = g_tables.find(tables_key(d, device));
-    if (t == g_tables.end() || t->second.expired()) b += tables_bytes(d);
+    auto t = g_tables.find(tables_key(d, precision, device));
+    if (t == g_tables.end() || t->second.expired()) b += tables_bytes(d, precision);
     auto o = g_modeops.find(modeops_key(d, precision, device));
     if (o == g_modeops.end() || o->second.expired()) b += modeops_bytes(d, precision);
     return b;
