@@ -755,9 +755,9 @@ __device__ __forceinline__ void contract_topsym_t(const ContractArgs &a, const T
 
 // ---------------------------------------------------------------------------
 // Top/bottom contraction over the symmetric table, register-blocked (grids
-// with nri <= 256).  A workgroup of TB_GROUPS 16-lane groups takes one mode
-// m and TB_PLANES source planes, whose scaled projections (TB_PLANES x NB16
-// double4) sit in LDS.  Lane (ig, jg) of a group owns rows 4ig..4ig+3 and
+// with nri <= 256).  A workgroup takes one mode m and TB_PLANES source
+// planes, whose scaled projections (TB_PLANES x NB16 double4) sit in LDS;
+// its two halves of TB_GROUPS 16-lane groups take half of the planes each.  Lane (ig, jg) of a group owns rows 4ig..4ig+3 and
 // columns 4jg..4jg+3 of the group's 16 x 16 block (I, J), reads them as
 // 16-byte pieces (the table's block plane order), and sums over the planes,
 // in registers, the rows (targets 16I + 4ig + a, from the column-block
@@ -766,9 +766,10 @@ __device__ __forceinline__ void contract_topsym_t(const ContractArgs &a, const T
 // tournament among the NB row blocks (every round pairs each block with
 // one other, so the 2 x TB_GROUPS targets of a round are distinct), then
 // the diagonal blocks (rows only), and after each round every group adds
-// its sums into the LDS totals: no two groups touch one total in a round
-// and the rounds come in a fixed order, so the sums are reproducible bit
-// for bit.  Table loads run TB_RING plane steps ahead, across rounds.
+// its sums into the LDS totals, one half after the other: no two groups
+// touch one total at once and the order is fixed, so the sums are
+// reproducible bit for bit.  Table loads run a ring of plane steps ahead,
+// across rounds.
 // ---------------------------------------------------------------------------
 constexpr int TB_GROUPS = 8;
 
@@ -791,21 +792,24 @@ __device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *t
     typedef typename std::conditional<sizeof(TT) == 4, float4, double2>::type Piece;
     constexpr int V = 16 / sizeof(TT);        // values per 16-byte piece
     constexpr int NL = 16 / V;                // pieces per lane and plane
+    constexpr int PH = TB_PLANES / 2;         // planes per half
     constexpr int RING = sizeof(TT) == 4 ? 4 : 2;
-    static_assert(TB_PLANES % RING == 0, "ring must divide the planes");
+    static_assert(PH % RING == 0, "ring must divide the planes of a half");
     const int nri = a.numr - 2, nzi = a.numz - 2;
     const int NB = (nri + 15) / 16, NP = NB * (NB + 1) / 2, NB16 = 16 * NB;
     const int kz0 = chunk * TB_PLANES, nkz = min(TB_PLANES, nzi - kz0);
-    const int tid = threadIdx.x, g = tid >> 4, l16 = tid & 15, ig = l16 >> 2, jg = l16 & 3;
-    __shared__ double4 Ps[TB_PLANES][256];
-    __shared__ double4 y[256];
-    const double4 zero4 = {0.0, 0.0, 0.0, 0.0};
-    const double4 *Pt = (const double4 *)a.proj + ((long)m * nzi + kz0) * NB16;
-    for (int e = tid; e < TB_PLANES * NB16; e += 16 * TB_GROUPS) {
+    const int tid = threadIdx.x, g = (tid >> 4) & (TB_GROUPS - 1), hf = tid / (16 * TB_GROUPS), l16 = tid & 15;
+    const int ig = l16 >> 2, jg = l16 & 3;
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    __shared__ d4 Ps[TB_PLANES][256];
+    __shared__ d4 y[256];
+    const d4 zero4 = {0.0, 0.0, 0.0, 0.0};
+    const d4 *Pt = (const d4 *)a.proj + ((long)m * nzi + kz0) * NB16;
+    for (int e = tid; e < TB_PLANES * NB16; e += 32 * TB_GROUPS) {
         const int u = e / NB16, p = e - u * NB16;
         Ps[u][p] = u < nkz ? Pt[(long)u * NB16 + p] : zero4;
     }
-    for (int e = tid; e < NB16; e += 16 * TB_GROUPS) y[e] = zero4;
+    for (int e = tid; e < NB16; e += 32 * TB_GROUPS) y[e] = zero4;
     __syncthreads();
 
     // schedule: Roff rounds of off-diagonal blocks (sub-rounds when the
@@ -833,9 +837,10 @@ __device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *t
         }
         return b;
     };
+    const int u0 = hf * PH;  // this half's first plane
     const TT *tb = table + ((long)m * NP * nzi + kz0) * 256 + l16 * V;
     auto load = [&](Piece *dst, const Blk &b, int u) {
-        const TT *src = tb + ((long)(b.I * (b.I + 1) / 2 + b.J) * nzi + min(u, nkz - 1)) * 256;
+        const TT *src = tb + ((long)(b.I * (b.I + 1) / 2 + b.J) * nzi + min(u0 + u, nkz - 1)) * 256;
 #pragma unroll
         for (int h = 0; h < NL; ++h) dst[h] = *(const Piece *)(src + h * 16 * V);
     };
@@ -854,52 +859,48 @@ __device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *t
             return (c & 1) ? q.y : q.x;
         }
     };
-    auto fma4 = [](double4 &acc, double s, const double4 &v) {
+    auto fma4 = [](d4 &acc, double s, const d4 &v) {
         acc.x = fma(s, v.x, acc.x);
         acc.y = fma(s, v.y, acc.y);
         acc.z = fma(s, v.z, acc.z);
         acc.w = fma(s, v.w, acc.w);
     };
-    auto sel = [](bool c, const double4 &p, const double4 &q) {
-        return (double4){c ? p.x : q.x, c ? p.y : q.y, c ? p.z : q.z, c ? p.w : q.w};
+    auto sel = [](bool c, const d4 &p, const d4 &q) { return c ? p : q; };
+    auto shx = [](const d4 &v, int mask) {
+        return (d4){__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
+                    __shfl_xor(v.w, mask, 64)};
     };
-    auto shx = [](const double4 &v, int mask) {
-        return (double4){__shfl_xor(v.x, mask, 64), __shfl_xor(v.y, mask, 64), __shfl_xor(v.z, mask, 64),
-                         __shfl_xor(v.w, mask, 64)};
-    };
-    auto add4 = [](const double4 &p, const double4 &q) {
-        return (double4){p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w};
-    };
+    auto add4 = [](const d4 &p, const d4 &q) { return p + q; };
     // sum of v[0..3] over the lanes of bits m1, m2 of the lane index; lane
     // with bits (b2 b1) ends with the total of v[2 b2 + b1]
-    auto reduce_scatter = [&](const double4 *v, int m1, int m2) {
+    auto reduce_scatter = [&](const d4 *v, int m1, int m2) {
         const bool b1 = (l16 & m1) != 0, b2 = (l16 & m2) != 0;
-        const double4 k0 = add4(sel(b1, v[1], v[0]), shx(sel(b1, v[0], v[1]), m1));
-        const double4 k1 = add4(sel(b1, v[3], v[2]), shx(sel(b1, v[2], v[3]), m1));
+        const d4 k0 = add4(sel(b1, v[1], v[0]), shx(sel(b1, v[0], v[1]), m1));
+        const d4 k1 = add4(sel(b1, v[3], v[2]), shx(sel(b1, v[2], v[3]), m1));
         return add4(sel(b2, k1, k0), shx(sel(b2, k0, k1), m2));
     };
 
     auto round = [&](auto off_tag) {
         constexpr bool OFF = decltype(off_tag)::value;
-        double4 rowp[4], colp[4];
+        d4 rowp[4], colp[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) rowp[k] = colp[k] = zero4;
 #pragma unroll
-        for (int u = 0; u < TB_PLANES; ++u) {
+        for (int u = 0; u < PH; ++u) {
             const int slot = u % RING;
             Piece t[NL];
 #pragma unroll
             for (int h = 0; h < NL; ++h) t[h] = ring[slot][h];
-            if (u + RING < TB_PLANES) load(ring[slot], cur, u + RING);
-            else load(ring[slot], nxt, u + RING - TB_PLANES);
+            if (u + RING < PH) load(ring[slot], cur, u + RING);
+            else load(ring[slot], nxt, u + RING - PH);
             __builtin_amdgcn_sched_barrier(0);
-            const double4 *pp = Ps[u];
-            double4 pj[4];
+            const d4 *pp = Ps[u0 + u];
+            d4 pj[4];
 #pragma unroll
             for (int c = 0; c < 4; ++c) pj[c] = pp[16 * cur.J + 4 * jg + c];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const double4 pi = pp[16 * cur.I + 4 * ig + r];
+                const d4 pi = pp[16 * cur.I + 4 * ig + r];
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const double q = tv(t, r, c);
@@ -909,18 +910,21 @@ __device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *t
             }
         }
         // rows over the 4 lanes jg, columns over the 4 lanes ig
-        const double4 rs = reduce_scatter(rowp, 1, 2);
-        double4 cs = zero4;
+        const d4 rs = reduce_scatter(rowp, 1, 2);
+        d4 cs = zero4;
         if (OFF) cs = reduce_scatter(colp, 4, 8);
-        if (cur.ok) {
-            double4 &yr = y[16 * cur.I + 4 * ig + jg];
-            yr = add4(yr, rs);
-            if (OFF) {
-                double4 &yc = y[16 * cur.J + 4 * jg + ig];
-                yc = add4(yc, cs);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            if (hf == h && cur.ok) {
+                d4 &yr = y[16 * cur.I + 4 * ig + jg];
+                yr = add4(yr, rs);
+                if (OFF) {
+                    d4 &yc = y[16 * cur.J + 4 * jg + ig];
+                    yc = add4(yc, cs);
+                }
             }
+            __syncthreads();
         }
-        __syncthreads();
     };
     int R = 0;
     for (; R < Roff; ++R) {
@@ -933,8 +937,8 @@ __device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *t
         cur = nxt;
         nxt = block_of(R + 2);
     }
-    double4 *out = (double4 *)a.out + ((long)m * tb_chunks(a.numz) + chunk) * NB16;
-    for (int e = tid; e < NB16; e += 16 * TB_GROUPS) out[e] = y[e];
+    d4 *out = (d4 *)a.out + ((long)m * tb_chunks(a.numz) + chunk) * NB16;
+    for (int e = tid; e < NB16; e += 32 * TB_GROUPS) out[e] = y[e];
 }
 
 __device__ __forceinline__ void contract_tb64_tile(const ContractArgs &a, int bx, int by, int bz)
@@ -989,76 +993,104 @@ Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int 
     return Op::make(OP_TOPSYM_REDUCE, 0, a, (4 * (numr - 2) + 255) / 256, mmax, 1);
 }
 
-// Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
+// Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr].
+// A workgroup takes SD_TGT targets ko, one mode and the kz range of one
+// split; thread jr sums over its source column, SD_TGT source planes at a
+// time: the table values of the SD_TGT x SD_TGT (ko, kz) tile lie on 2
+// SD_TGT - 1 diagonals, so one window of table loads serves all of its
+// products (8 multiply-adds per load).  The workgroup then adds its
+// columns in a fixed order (wavefront butterflies, then the 4 wavefronts).
+constexpr int SD_TGT = 16;
+
 __device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx, int by, int bz)
 {
     const double *__restrict__ S = a.table;
     const double *__restrict__ proj = a.proj;
-    double *__restrict__ Ss = a.out;
     const int numr = a.numr, numz = a.numz, mmax = a.mmax;
-    const int nri = numr - 2, nzi = numz - 2;
+    const int nri = numr - 2, nzi = numz - 2, nd = 2 * nzi - 1;
     const long plane = (long)numr * numz;
-    const int ko0 = bx * CT_TGT, m = by, sp = bz;
-    const int nt = min(CT_TGT, nzi - ko0);
-    const int nd = 2 * nzi - 1;
+    const int k0 = bx * SD_TGT, m = by, sp = bz;
+    const int ch = ((nzi + CT_SPLIT - 1) / CT_SPLIT + SD_TGT - 1) / SD_TGT * SD_TGT;
+    const int kzs = min(nzi, sp * ch), kze = min(nzi, kzs + ch);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double *sm = S + (long)m * nd * nri;
     const double *pc = proj + (long)m * plane;
-    const double *ps = proj + (long)(m > 0 ? (mmax - 1 + m) : 0) * plane;
-    double v[2 * CT_TGT];
+    const double *ps = proj + (long)(m > 0 ? mmax - 1 + m : 0) * plane;
+    double acc[2][SD_TGT];
 #pragma unroll
-    for (int k = 0; k < 2 * CT_TGT; ++k) v[k] = 0.0;
-    const long npts = (long)nri * nzi;
-    const long chunk = (npts + CT_SPLIT - 1) / CT_SPLIT;
-    const long i0 = sp * chunk, i1 = min(npts, i0 + chunk);
-    constexpr int U = 4;
-    for (long base = i0 + threadIdx.x; base < i1; base += 256 * U) {
-        double c0[U], s0[U];
-        long gi[U];
-        bool ok[U];
+    for (int t = 0; t < SD_TGT; ++t) acc[0][t] = acc[1][t] = 0.0;
+    for (int jr0 = 0; jr0 < nri; jr0 += 256) {
+        const int jr = jr0 + tid;
+        const bool jok = jr < nri;
+        const int jc = jok ? jr : nri - 1;
+        for (int kb = kzs; kb < kze; kb += SD_TGT) {
+            // window: diagonals d0 - (SD_TGT-1) .. d0 + (SD_TGT-1), d0 = k0 - kb + nzi - 1
+            const int d0 = k0 - kb + nzi - 1;
+            double w[2 * SD_TGT - 1], xc[SD_TGT], xs[SD_TGT];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            long idx = base + (long)u * 256;
-            ok[u] = idx < i1;
-            long id = ok[u] ? idx : i0;
-            int kz = (int)(id / nri), jr = (int)(id % nri);
-            long pa = (long)(kz + 1) * numr + jr + 1;
-            c0[u] = pc[pa];
-            s0[u] = (m > 0) ? ps[pa] : 0.0;
-            gi[u] = (long)(ko0 - kz + nzi - 1) * nri + jr;
-        }
+            for (int i = 0; i < 2 * SD_TGT - 1; ++i) {
+                const int d = min(max(d0 - (SD_TGT - 1) + i, 0), nd - 1);
+                w[i] = sm[(long)d * nri + jc];
+            }
 #pragma unroll
-        for (int q = 0; q < CT_TGT; ++q) {
-            if (q < nt) {
+            for (int k = 0; k < SD_TGT; ++k) {
+                const int kz = kb + k;
+                const bool ok = jok && kz < kze;
+                const long o = (long)(min(kz, nzi - 1) + 1) * numr + jc + 1;
+                const double c = pc[o], sv = ps[o];
+                xc[k] = ok ? c : 0.0;
+                xs[k] = ok && m > 0 ? sv : 0.0;
+            }
 #pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    double g = ok[u] ? sm[gi[u] + (long)q * nri] : 0.0;
-                    v[2 * q + 0] += g * c0[u];
-                    v[2 * q + 1] += g * s0[u];
+            for (int k = 0; k < SD_TGT; ++k)
+#pragma unroll
+                for (int t = 0; t < SD_TGT; ++t) {
+                    const double g = w[SD_TGT - 1 + t - k];
+                    acc[0][t] = fma(g, xc[k], acc[0][t]);
+                    acc[1][t] = fma(g, xs[k], acc[1][t]);
                 }
+        }
+    }
+    // sum over the 256 threads: butterflies in each wavefront, then the 4
+    // wavefronts in order
+    double v[2 * SD_TGT];
+#pragma unroll
+    for (int t = 0; t < SD_TGT; ++t) {
+        v[t] = acc[0][t];
+        v[SD_TGT + t] = acc[1][t];
+    }
+    // reduce-scatter over lane bits 5..1: after the step on bit b the lane
+    // keeps the half of its values selected by that bit
+    int n = 2 * SD_TGT;
+#pragma unroll
+    for (int b = 32; b >= 2; b >>= 1) {
+        const bool hi = (lane & b) != 0;
+        n >>= 1;
+#pragma unroll
+        for (int i = 0; i < SD_TGT; ++i) {
+            if (i < n) {
+                const double keep = hi ? v[n + i] : v[i], give = hi ? v[i] : v[n + i];
+                v[i] = keep + __shfl_xor(give, b, 64);
             }
         }
     }
-    __shared__ double red[4][2 * CT_TGT];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < 2 * CT_TGT; ++k) {
-        double x = v[k];
-#pragma unroll
-        for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
-        if (lane == 0) red[wave][k] = x;
-    }
+    // now n == 1: lane holds value lane >> 1 summed over the 32 lanes that
+    // share lane bit 0; add the pair
+    const double tot = v[0] + __shfl_xor(v[0], 1, 64);
+    __shared__ double red[4][2 * SD_TGT];
+    if ((lane & 1) == 0) red[wave][lane >> 1] = tot;
     __syncthreads();
-    if (threadIdx.x < 2 * nt) {
-        int q = threadIdx.x / 2, c = threadIdx.x % 2;
-        int k = threadIdx.x;
-        Ss[(((long)sp * mmax + m) * 2 + c) * nzi + ko0 + q] = (red[0][k] + red[1][k]) + (red[2][k] + red[3][k]);
+    if (tid < 2 * SD_TGT) {
+        const int q = tid / SD_TGT, t = tid % SD_TGT, ko = k0 + t;
+        const double r = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+        if (ko < nzi) a.out[(((long)sp * mmax + m) * 2 + q) * nzi + ko] = r;
     }
 }
 
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax)
 {
-    ContractArgs a{S, proj, Ss, numr, numz, mmax};
-    return Op::make(OP_CONTRACT_SIDE, 0, a, (numz - 2 + CT_TGT - 1) / CT_TGT, mmax, CT_SPLIT);
+    ContractArgs a{S, proj, Ss, numr, numz, mmax, nullptr, nullptr, 0};
+    return Op::make(OP_CONTRACT_SIDE, 0, a, (numz - 2 + SD_TGT - 1) / SD_TGT, mmax, CT_SPLIT);
 }
 
 // Boundary faces in helmadi's Fourier mode order.  bessel rebuilds each
@@ -1190,8 +1222,8 @@ BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
-BH_KERNEL_PAIR(contract_tb64, ContractArgs, 128)
-BH_KERNEL_PAIR(contract_tb32, ContractArgs, 128)
+BH_KERNEL_PAIR(contract_tb64, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_tb32, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_reduce, ContractArgs, 256)
 BH_KERNEL_PAIR(face_modes, RebuildArgs, 128)
 BH_KERNEL_PAIR(face_scatter, ScatterArgs, 128)
@@ -1221,8 +1253,8 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         switch (variant) {
         case 0: *k = {(const void *)&contract_topsym_kernel, (const void *)&contract_topsym_grouped, 256}; break;
         case 1: *k = {(const void *)&contract_topsym32_kernel, (const void *)&contract_topsym32_grouped, 256}; break;
-        case 2: *k = {(const void *)&contract_tb64_kernel, (const void *)&contract_tb64_grouped, 128}; break;
-        default: *k = {(const void *)&contract_tb32_kernel, (const void *)&contract_tb32_grouped, 128}; break;
+        case 2: *k = {(const void *)&contract_tb64_kernel, (const void *)&contract_tb64_grouped, 256}; break;
+        default: *k = {(const void *)&contract_tb32_kernel, (const void *)&contract_tb32_grouped, 256}; break;
         }
         return true;
         BH_REF(OP_TOPSYM_REDUCE, topsym_reduce, 256)
