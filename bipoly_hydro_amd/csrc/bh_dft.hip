@@ -805,6 +805,163 @@ constexpr size_t pipe_lds_bytes()
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Inverse at N = 512 by one radix-4 step, decimating the output index:
+//   x_{4m+r} = (1/N) sum_{k<N} X_k w^{k(4m+r)},   w = e^{2 pi i/N}
+//            = (1/4) IDFT_n(Y^(r))_m,   n = N/4,
+//   Y^(r)_k  = w^{kr} sum_{q<4} i^{qr} X_{k+qn}     (k < n; X_{N-k} = conj X_k)
+// with X_k = a_k + i s_k (s_k the mode rows N/2+1..N-1).  Each Y^(r) is
+// Hermitian, so output quarter r is the size-n real inverse of (Re, Im)
+// of Y^(r)_0..n/2: the same P/Q products as above on four virtual columns
+// per column, at a quarter of the matrix work of the direct size-N form.
+// A workgroup takes a panel of R4CB columns: it forms the four quarter
+// spectra (the X + xadd update and its store fused in) straight from HBM
+// into LDS, runs the products (wave w: n-tile w % 4 of P and Q, quarters
+// 2 (w / 4) and 2 (w / 4) + 1) and writes the rows 4m + r.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int R4N = PN / 4, R4KG = 4, R4KPp = 16 * R4KG + 2, R4CB = 16, R4VC = 4 * R4CB;
+constexpr size_t R4_LDS = (size_t)2 * R4VC * R4KPp * 8 + 2 * R4VC * 8;
+constexpr int R4_FRAGS = 8 * R4KG * 64 * 4;  // doubles of the size-n P and Q fragments
+
+__device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned char *lds)
+{
+    typedef Mma<double>::acc_t acc_t;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    double *Ca = (double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col0 = bx * R4CB;
+    const double *x = a.x;
+    const long ldx = a.ldx;
+    const double *tw = (const double *)a.A + R4_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N)
+    // 1. quarter spectra Y^(r)_k, k = 0..n/2, times 1/4
+#pragma unroll 1
+    for (int e = tid; e < (R4N / 2 + 1) * R4CB; e += 512) {
+        const int k = e / R4CB, c = e - k * R4CB, gc = col0 + c;
+        const bool ok = gc < a.M;
+        const int gcc = ok ? gc : a.M - 1;
+        const int ra[4] = {k, R4N + k, 2 * R4N - k, R4N - k};  // mode rows of X_k, X_{k+n}, X_{2n-k}, X_{n-k}
+        double av[4], sv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const bool hs = ra[q] > 0 && ra[q] < PN / 2;  // s_0 = s_{N/2} = 0
+            av[q] = x[(long)ra[q] * ldx + gcc];
+            sv[q] = hs ? x[(long)(PN / 2 + ra[q]) * ldx + gcc] : 0.0;
+        }
+        if (a.xadd) {
+            // x + xadd, stored once per row: q = 0, 1 always, q = 2 for k < n/2,
+            // q = 3 for 0 < k < n/2 (the four ranges tile rows 0..N/2)
+            float fa[4], fs[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool hs = ra[q] > 0 && ra[q] < PN / 2;
+                fa[q] = a.xadd[(long)ra[q] * ldx + gcc];
+                fs[q] = hs ? a.xadd[(long)(PN / 2 + ra[q]) * ldx + gcc] : 0.0f;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool hs = ra[q] > 0 && ra[q] < PN / 2;
+                av[q] += (double)fa[q];
+                sv[q] += (double)fs[q];
+                const bool own = ok && (q < 2 || (q == 2 && k < R4N / 2) || (q == 3 && k > 0 && k < R4N / 2));
+                if (own) {
+                    a.xstore[(long)ra[q] * ldx + gc] = av[q];
+                    if (hs) a.xstore[(long)(PN / 2 + ra[q]) * ldx + gc] = sv[q];
+                }
+            }
+        }
+        // X_k, X_{k+n}, X_{k+2n} = conj X_{2n-k}, X_{k+3n} = conj X_{n-k}
+        const double x0r = av[0], x0i = sv[0], x1r = av[1], x1i = sv[1];
+        const double x2r = av[2], x2i = -sv[2], x3r = av[3], x3i = -sv[3];
+        const double pr = x0r + x2r, pi = x0i + x2i, mr = x0r - x2r, mi = x0i - x2i;  // X0 +- X2
+        const double ur = x1r + x3r, ui = x1i + x3i, dr = x1r - x3r, di = x1i - x3i;  // X1 +- X3
+        // r = 0: (X0+X2) + (X1+X3); 1: (X0-X2) + i (X1-X3); 2: (X0+X2) - (X1+X3); 3: (X0-X2) - i (X1-X3)
+        const double zr[4] = {pr + ur, mr - di, pr - ur, mr + di};
+        const double zi[4] = {pi + ui, mi + dr, pi - ui, mi - dr};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const double wc = tw[(k * 4 + r) * 2], ws = tw[(k * 4 + r) * 2 + 1];
+            const double yr = 0.25 * (wc * zr[r] - ws * zi[r]), yi = 0.25 * (wc * zi[r] + ws * zr[r]);
+            const int vc = r * R4CB + c;
+            if (k == 0) {
+                E0[vc] = yr;
+            } else if (k == R4N / 2) {
+                EH[vc] = yr;
+                Ca[vc * R4KPp + R4N / 2 - 1] = 0.0;  // K padding (k = 64 of the 16 R4KG slots)
+                Sb[vc * R4KPp + R4N / 2 - 1] = 0.0;
+            } else {
+                Ca[vc * R4KPp + k - 1] = yr;
+                Sb[vc * R4KPp + k - 1] = yi;
+            }
+        }
+    }
+    __syncthreads();
+
+    // 2. size-n products: P rows m = 16 mt.., Q the same, for quarters t0, t0 + 1
+    const v4 *Af = (const v4 *)a.A;
+    const int mt = wave & 3, t0 = 2 * (wave >> 2), jl = 4 * (lane >> 4);
+    acc_t accP[2], accQ[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) accP[i] = accQ[i] = (acc_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int grp = 0; grp < R4KG; ++grp) {
+        const v4 fp = Af[((long)mt * R4KG + grp) * 64 + lane];
+        const v4 fq = Af[((long)(4 + mt) * R4KG + grp) * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int off = (16 * (t0 + i) + (lane & 15)) * R4KPp + 16 * grp + jl;
+            const v4 bc = *(const v4 *)(Ca + off), bs = *(const v4 *)(Sb + off);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                accP[i] = Mma<double>::mma(fp[g], bc[g], accP[i]);
+                accQ[i] = Mma<double>::mma(fq[g], bs[g], accQ[i]);
+            }
+        }
+    }
+    // 3. rows 4m + r (m < n/2) and 4(n - m) + r
+    const double invn = 1.0 / (double)R4N;
+    double *out = (double *)a.out;
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int r = t0 + i, c = lane & 15, gc = col0 + c, vc = r * R4CB + c;
+        const double e0 = E0[vc] * invn, eh = EH[vc] * invn;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = 16 * mt + Mma<double>::row(lane, v);
+            if (gc >= a.M) continue;
+            const double p = (e0 + ((m & 1) ? -eh : eh)) + accP[i][v];
+            const double q = accQ[i][v];
+            out[(long)(4 * m + r) * a.ldo + gc] = p - q;
+            if (m > 0) out[(long)(4 * (R4N - m) + r) * a.ldo + gc] = p + q;
+        }
+    }
+    // row 4 (n/2) + r: e0 + eh + sum_k (2/n) (-1)^k a'_k, eight lanes per column
+    {
+        const int vc = tid >> 3, j = tid & 7;
+        double sm = 0.0;
+        for (int k = 1 + j; k < R4N / 2; k += 8) sm += (k & 1) ? -Ca[vc * R4KPp + k - 1] : Ca[vc * R4KPp + k - 1];
+        sm += __shfl_xor(sm, 1, 64);
+        sm += __shfl_xor(sm, 2, 64);
+        sm += __shfl_xor(sm, 4, 64);
+        const int r = vc / R4CB, c = vc - r * R4CB, gc = col0 + c;
+        if (j == 0 && gc < a.M)
+            out[(long)(4 * (R4N / 2) + r) * a.ldo + gc] = (E0[vc] + EH[vc]) * invn + 2.0 * invn * sm;
+    }
+}
+
+__global__ __launch_bounds__(512) void dft_inv4_kernel(DftArgs a, int nx, int ny, int total)
+{
+    extern __shared__ __align__(16) unsigned char inv4_lds[];
+    BH_SINGLE_LOOP(a, nx, ny, total, (dft_inv4_tile(a, bx, inv4_lds)));
+}
+
+__global__ __launch_bounds__(512) void dft_inv4_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+{
+    extern __shared__ __align__(16) unsigned char inv4_lds[];
+    BH_GROUP_LOOP(DftArgs, args, gs, (dft_inv4_tile(ga, bx, inv4_lds)));
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // entry points.  variant = kind + 4 log2(N); kind 0: forward FP32 (32
 // columns per panel), 1: forward FP64 (16 columns), 2: inverse FP64 (16)
 // ---------------------------------------------------------------------------
@@ -907,6 +1064,11 @@ bool dft_launch(const Op &op, hipStream_t s)
         const int kind = op.variant % 4;
         int nx = op.nx, ny = op.ny, total = (int)op.tiles();
         void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
+        if (kind == DFT_INV_F64) {
+            (void)hipLaunchKernel((const void *)&dft_inv4_kernel, dim3(persistent_blocks(total, 0)), dim3(512), argv,
+                                  (int)R4_LDS, s);
+            return true;
+        }
         const int cap = op.as<DftArgs>().max_blocks;
         const int nb = cap > 0 ? std::min(pipe_blocks(total), cap) : pipe_blocks(total);
         (void)hipLaunchKernel(pipe_fn(kind, false), dim3(nb), dim3(512), argv, (int)pipe_bytes(kind), s);
@@ -927,6 +1089,10 @@ bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSc
         GroupSched g = gs;
         const void *ap = d_args;
         void *argv[] = {(void *)&ap, (void *)&g};
+        if (variant % 4 == DFT_INV_F64) {
+            (void)hipLaunchKernel((const void *)&dft_inv4_grouped, dim3(nblocks), dim3(512), argv, (int)R4_LDS, s);
+            return true;
+        }
         (void)hipLaunchKernel(pipe_fn(variant % 4, true), dim3(pipe_blocks(gs.total)), dim3(512), argv,
                               (int)pipe_bytes(variant % 4), s);
         return true;
@@ -942,6 +1108,10 @@ bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSc
 bool dft_occupancy(int kind, int variant, int *blocks)
 {
     if (kind != OP_DFT) return false;
+    if ((1 << (variant / 4)) == DFT_MAX_N && variant % 4 == DFT_INV_F64) {
+        *blocks = kernel_occupancy((const void *)&dft_inv4_grouped, 512, (int)R4_LDS);
+        return true;
+    }
     *blocks = kernel_occupancy(dft_fn(variant, true), 512, (int)dft_lds_bytes(variant % 4, 1 << (variant / 4)));
     return true;
 }
@@ -984,6 +1154,20 @@ void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *
 
 void dft_inverse_fragments(int N, std::vector<double> *f64)
 {
+    if (N == DFT_MAX_N) {
+        // radix-4 form: the size-N/4 operand, then the twiddles w^{kr}
+        // (cos, sin) for k = 0..N/8, r = 0..3
+        std::vector<double> tw;
+        dft_inverse_fragments(N / 4, f64);
+        const double pi = std::acos(-1.0);
+        for (int k = 0; k <= N / 8; ++k)
+            for (int r = 0; r < 4; ++r) {
+                const double ang = 2.0 * pi * (double)((long)k * r % N) / (double)N;
+                f64->push_back(std::cos(ang));
+                f64->push_back(std::sin(ang));
+            }
+        return;
+    }
     const int H = N / 2 - 1, KS = (H + 15) / 16;
     const int NT = (N / 2 + 15) / 16;
     const double pi = std::acos(-1.0);
