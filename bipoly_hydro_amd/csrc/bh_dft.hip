@@ -828,10 +828,15 @@ __device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned
     typedef Mma<double>::acc_t acc_t;
     typedef double v4 __attribute__((ext_vector_type(4)));
     double *Ca = (double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // (as ldx below: lane-dependent offsets recomputed per tile)
+    const int lane = tid & 63, wave = tid >> 6;
     const int col0 = bx * R4CB;
     const double *x = a.x;
-    const long ldx = a.ldx;
+    long ldx = a.ldx;
+    // opaque per tile: keeps the row offsets below from being hoisted out of
+    // the persistent loop into registers held across tiles
+    asm volatile("" : "+s"(ldx));
     const double *tw = (const double *)a.A + R4_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N)
     // 1. quarter spectra Y^(r)_k, k = 0..n/2, times 1/4
 #pragma unroll 1
@@ -902,7 +907,7 @@ __device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned
     acc_t accP[2], accQ[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) accP[i] = accQ[i] = (acc_t){0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+#pragma unroll 1
     for (int grp = 0; grp < R4KG; ++grp) {
         const v4 fp = Af[((long)mt * R4KG + grp) * 64 + lane];
         const v4 fq = Af[((long)(4 + mt) * R4KG + grp) * 64 + lane];
@@ -962,6 +967,185 @@ __global__ __launch_bounds__(512) void dft_inv4_grouped(const DftArgs *__restric
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Forward at N = 512 by one radix-4 step, decimating the input index:
+//   X_k = sum_{r<4} w^{-kr} F^(r)_{k mod n},  F^(r) = DFT_n(x_{4j+r}),  n = N/4,
+// so with G^(r)_k = w^{-kr} F^(r)_k (k <= n/2, F^(r)_{n-k} = conj F^(r)_k),
+//   X_k = sum_r G^(r)_k,            X_{n+k}  = sum_r (-i)^r G^(r)_k,
+//   X_{n-k} = sum_r G'^(r)_k,       X_{2n-k} = sum_r (-i)^r G'^(r)_k,
+//   G'^(r)_k = w^{-(n-k) r} conj F^(r)_k.
+// The four length-n real transforms F^(r) run as the folded cosine/sine
+// products of the direct form at size n on four virtual columns per
+// column (a quarter of its matrix work); a workgroup takes R4CB columns,
+// folds the quarter sequences into LDS, runs the products (wave w: row
+// tile w % 4 of the cosine and of the sine half, quarters 2 (w / 4) and
+// 2 (w / 4) + 1), puts F in LDS and combines the rows k, n + k, n - k,
+// 2n - k of X in FP64, masked as the direct form is.
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T>
+struct R4Fwd {
+    static constexpr int KPp = 16 * R4KG + Pad<T>::v;
+    static constexpr size_t uv = (size_t)2 * R4VC * KPp * sizeof(T);          // U, V (then Fa, Fs)
+    static constexpr size_t f = (size_t)2 * (R4N / 2 + 1) * R4VC * sizeof(T);  // Fa, Fs
+    static constexpr size_t bytes = (uv > f ? uv : f) + 3 * R4VC * sizeof(double);
+    static constexpr int frags = 8 * R4KG * 64 * 4;  // size-n cosine and sine fragments
+};
+
+template <typename T>
+__device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned char *lds)
+{
+    typedef typename Mma<T>::acc_t acc_t;
+    typedef T v4 __attribute__((ext_vector_type(4)));
+    typedef R4Fwd<T> Ly;
+    constexpr int KPp = Ly::KPp;
+    T *U = (T *)lds, *V = U + R4VC * KPp;
+    T *Fa = (T *)lds, *Fs = Fa + (R4N / 2 + 1) * R4VC;  // after the products, over U and V
+    double *X0 = (double *)(lds + (Ly::uv > Ly::f ? Ly::uv : Ly::f)), *XH = X0 + R4VC, *FM = XH + R4VC;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));  // lane-dependent offsets recomputed per tile, not held across tiles
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col0 = bx * R4CB;
+    const double *x = a.x;
+    long ldx = a.ldx;
+    asm volatile("" : "+s"(ldx));
+    // 1. fold the quarter sequences x^(r)_j = x_{4j+r}: u_j, v_j (j = 1..n/2-1), x^(r)_0, x^(r)_{n/2}
+#pragma unroll 1
+    for (int e = tid; e < (R4N / 2 + 1) * R4CB; e += 512) {
+        const int j = e / R4CB, c = e - j * R4CB, gc = col0 + c;
+        const int gcc = gc < a.M ? gc : a.M - 1;
+        const bool fold = j > 0 && j < R4N / 2;
+        double p[4], q[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            p[r] = x[(long)(4 * j + r) * ldx + gcc];
+            q[r] = fold ? x[(long)(PN - 4 * j + r) * ldx + gcc] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int vc = r * R4CB + c;
+            if (j == 0) {
+                X0[vc] = p[r];
+            } else if (j == R4N / 2) {
+                XH[vc] = p[r];
+                U[vc * KPp + R4N / 2 - 1] = (T)0;  // K padding
+                V[vc * KPp + R4N / 2 - 1] = (T)0;
+            } else {
+                U[vc * KPp + j - 1] = (T)(p[r] + q[r]);
+                V[vc * KPp + j - 1] = (T)(p[r] - q[r]);
+            }
+        }
+    }
+    __syncthreads();
+    // middle row of each quarter: F_{n/2} = x_0 + x_{n/2} + sum_j (-1)^j u_j (FP64 sum)
+    {
+        const int vc = tid >> 3, jj = tid & 7;
+        double sm = 0.0;
+        for (int j = 1 + jj; j < R4N / 2; j += 8) {
+            const double u = (double)U[vc * KPp + j - 1];
+            sm += (j & 1) ? -u : u;
+        }
+        sm += __shfl_xor(sm, 1, 64);
+        sm += __shfl_xor(sm, 2, 64);
+        sm += __shfl_xor(sm, 4, 64);
+        if (jj == 0) FM[vc] = (X0[vc] + XH[vc]) + sm;
+    }
+    // 2. size-n products: cosine rows k = 16 mt.., sine rows k = 16 mt + 1..
+    const v4 *Af = (const v4 *)a.A;
+    const int mt = wave & 3, t0 = 2 * (wave >> 2), jl = 4 * (lane >> 4);
+    acc_t accC[2], accS[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) accC[i] = accS[i] = (acc_t){(T)0, (T)0, (T)0, (T)0};
+#pragma unroll 1
+    for (int grp = 0; grp < R4KG; ++grp) {
+        const v4 fc = Af[((long)mt * R4KG + grp) * 64 + lane];
+        const v4 fs = Af[((long)(4 + mt) * R4KG + grp) * 64 + lane];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int off = (16 * (t0 + i) + (lane & 15)) * KPp + 16 * grp + jl;
+            const v4 bu = *(const v4 *)(U + off), bv = *(const v4 *)(V + off);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                accC[i] = Mma<T>::mma(fc[g], bu[g], accC[i]);
+                accS[i] = Mma<T>::mma(fs[g], bv[g], accS[i]);
+            }
+        }
+    }
+    __syncthreads();  // U, V read: F goes over them
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int c = lane & 15, vc = (t0 + i) * R4CB + c;
+        const double e0 = X0[vc], eh = XH[vc];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int k = 16 * mt + Mma<T>::row(lane, v);  // cosine row k, sine row k + 1
+            Fa[k * R4VC + vc] = (T)((e0 + ((k & 1) ? -eh : eh)) + (double)accC[i][v]);
+            if (k + 1 < R4N / 2) Fs[(k + 1) * R4VC + vc] = accS[i][v];
+        }
+    }
+    if (tid < R4VC) {
+        Fa[(R4N / 2) * R4VC + tid] = (T)FM[tid];
+        Fs[tid] = (T)0;                      // F_0, F_{n/2} are real
+        Fs[(R4N / 2) * R4VC + tid] = (T)0;
+    }
+    __syncthreads();
+    // 3. combine: rows k, n + k (always), 2n - k (k < n/2), n - k (0 < k < n/2)
+    const T *tw = (const T *)a.A + Ly::frags;  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n
+#pragma unroll 1
+    for (int e = tid; e < (R4N / 2 + 1) * R4CB; e += 512) {
+        const int k = e / R4CB, c = e - k * R4CB, gc = col0 + c;
+        if (gc >= a.M) continue;
+        bool masked = false;
+        if (a.mask_numr) {
+            const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
+            masked = J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1;
+        }
+        double gr[4], gi[4], hr[4], hi[4];  // G^(r)_k and G'^(r)_k
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int vc = r * R4CB + c;
+            const double fr = (double)Fa[k * R4VC + vc], fi = (double)Fs[k * R4VC + vc];
+            const double c1 = (double)tw[(k * 4 + r) * 2], s1 = (double)tw[(k * 4 + r) * 2 + 1];
+            const double c2 = (double)tw[((R4N - k) * 4 + r) * 2], s2 = (double)tw[((R4N - k) * 4 + r) * 2 + 1];
+            // w^{-kr} F = (c1 - i s1)(fr + i fi); w^{-(n-k) r} conj F = (c2 - i s2)(fr - i fi)
+            gr[r] = c1 * fr + s1 * fi;
+            gi[r] = c1 * fi - s1 * fr;
+            hr[r] = c2 * fr - s2 * fi;
+            hi[r] = -c2 * fi - s2 * fr;
+        }
+        // sum_r z_r and sum_r (-i)^r z_r = z0 - i z1 - z2 + i z3
+        const double x0r = (gr[0] + gr[2]) + (gr[1] + gr[3]), x0i = (gi[0] + gi[2]) + (gi[1] + gi[3]);
+        const double x1r = (gr[0] - gr[2]) + (gi[1] - gi[3]), x1i = (gi[0] - gi[2]) - (gr[1] - gr[3]);
+        const double y0r = (hr[0] + hr[2]) + (hr[1] + hr[3]), y0i = (hi[0] + hi[2]) + (hi[1] + hi[3]);
+        const double y1r = (hr[0] - hr[2]) + (hi[1] - hi[3]), y1i = (hi[0] - hi[2]) - (hr[1] - hr[3]);
+        T *out = (T *)a.out;
+        const long ldo = a.ldo;
+        auto put = [&](int row, double re, double im) {  // a_row = Re, s-row N/2 + row = Im
+            out[(long)row * ldo + gc] = (T)(masked ? 0.0 : re);
+            if (row > 0 && row < PN / 2) out[(long)(PN / 2 + row) * ldo + gc] = (T)(masked ? 0.0 : im);
+        };
+        put(k, x0r, x0i);
+        put(R4N + k, x1r, x1i);
+        if (k < R4N / 2) put(2 * R4N - k, y1r, y1i);
+        if (k > 0 && k < R4N / 2) put(R4N - k, y0r, y0i);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void dft_fwd4_kernel(DftArgs a, int nx, int ny, int total)
+{
+    extern __shared__ __align__(16) unsigned char fwd4_lds[];
+    BH_SINGLE_LOOP(a, nx, ny, total, (dft_fwd4_tile<T>(a, bx, fwd4_lds)));
+}
+
+template <typename T>
+__global__ __launch_bounds__(512) void dft_fwd4_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+{
+    extern __shared__ __align__(16) unsigned char fwd4_lds[];
+    BH_GROUP_LOOP(DftArgs, args, gs, (dft_fwd4_tile<T>(ga, bx, fwd4_lds)));
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // entry points.  variant = kind + 4 log2(N); kind 0: forward FP32 (32
 // columns per panel), 1: forward FP64 (16 columns), 2: inverse FP64 (16)
 // ---------------------------------------------------------------------------
@@ -1010,7 +1194,7 @@ __global__ __launch_bounds__(512, 4) void dft_grouped(const DftArgs *__restrict_
 
 Op dft_op(const DftArgs &a, int kind)
 {
-    const int CB = dft_cols(kind);
+    const int CB = a.N == DFT_MAX_N ? R4CB : dft_cols(kind);  // N = 512: the radix-4 kernels' panels
     return Op::make(OP_DFT, kind + 4 * log2i(a.N), a, (a.M + CB - 1) / CB, 1, 1);
 }
 
@@ -1069,6 +1253,15 @@ bool dft_launch(const Op &op, hipStream_t s)
                                   (int)R4_LDS, s);
             return true;
         }
+        if (kind == DFT_FWD_F32 || kind == DFT_FWD_F64) {
+            const bool f32 = kind == DFT_FWD_F32;
+            const int cap = op.as<DftArgs>().max_blocks;
+            int nb = persistent_blocks(total, 0);
+            if (cap > 0 && nb > cap) nb = cap >= 8 ? cap / 8 * 8 : cap;
+            (void)hipLaunchKernel(f32 ? (const void *)&dft_fwd4_kernel<float> : (const void *)&dft_fwd4_kernel<double>,
+                                  dim3(nb), dim3(512), argv, (int)(f32 ? R4Fwd<float>::bytes : R4Fwd<double>::bytes), s);
+            return true;
+        }
         const int cap = op.as<DftArgs>().max_blocks;
         const int nb = cap > 0 ? std::min(pipe_blocks(total), cap) : pipe_blocks(total);
         (void)hipLaunchKernel(pipe_fn(kind, false), dim3(nb), dim3(512), argv, (int)pipe_bytes(kind), s);
@@ -1093,6 +1286,16 @@ bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSc
             (void)hipLaunchKernel((const void *)&dft_inv4_grouped, dim3(nblocks), dim3(512), argv, (int)R4_LDS, s);
             return true;
         }
+        if (variant % 4 == DFT_FWD_F32) {
+            (void)hipLaunchKernel((const void *)&dft_fwd4_grouped<float>, dim3(nblocks), dim3(512), argv,
+                                  (int)R4Fwd<float>::bytes, s);
+            return true;
+        }
+        if (variant % 4 == DFT_FWD_F64) {
+            (void)hipLaunchKernel((const void *)&dft_fwd4_grouped<double>, dim3(nblocks), dim3(512), argv,
+                                  (int)R4Fwd<double>::bytes, s);
+            return true;
+        }
         (void)hipLaunchKernel(pipe_fn(variant % 4, true), dim3(pipe_blocks(gs.total)), dim3(512), argv,
                               (int)pipe_bytes(variant % 4), s);
         return true;
@@ -1108,8 +1311,12 @@ bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSc
 bool dft_occupancy(int kind, int variant, int *blocks)
 {
     if (kind != OP_DFT) return false;
-    if ((1 << (variant / 4)) == DFT_MAX_N && variant % 4 == DFT_INV_F64) {
-        *blocks = kernel_occupancy((const void *)&dft_inv4_grouped, 512, (int)R4_LDS);
+    if ((1 << (variant / 4)) == DFT_MAX_N) {
+        const int k = variant % 4;
+        if (k == DFT_INV_F64) *blocks = kernel_occupancy((const void *)&dft_inv4_grouped, 512, (int)R4_LDS);
+        else if (k == DFT_FWD_F32)
+            *blocks = kernel_occupancy((const void *)&dft_fwd4_grouped<float>, 512, (int)R4Fwd<float>::bytes);
+        else *blocks = kernel_occupancy((const void *)&dft_fwd4_grouped<double>, 512, (int)R4Fwd<double>::bytes);
         return true;
     }
     *blocks = kernel_occupancy(dft_fn(variant, true), 512, (int)dft_lds_bytes(variant % 4, 1 << (variant / 4)));
@@ -1133,6 +1340,25 @@ static void frag_fill(std::vector<T> *out, int RT, int KG, const std::function<d
 
 void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *f64)
 {
+    if (N == DFT_MAX_N) {
+        // radix-4 form: the size-N/4 operand, then the twiddles (cos, sin)(2 pi k r / N)
+        // for k = 0..N/4, r = 0..3, in the operand's precision
+        dft_forward_fragments(N / 4, f32, f64);
+        const double pi = std::acos(-1.0);
+        for (int k = 0; k <= N / 4; ++k)
+            for (int r = 0; r < 4; ++r) {
+                const double ang = 2.0 * pi * (double)((long)k * r % N) / (double)N;
+                if (f32) {
+                    f32->push_back((float)std::cos(ang));
+                    f32->push_back((float)std::sin(ang));
+                }
+                if (f64) {
+                    f64->push_back(std::cos(ang));
+                    f64->push_back(std::sin(ang));
+                }
+            }
+        return;
+    }
     const int H = N / 2 - 1, KS = (H + 15) / 16;
     const int RTc = (N / 2 + 15) / 16, RTs = (H + 15) / 16;
     const double pi = std::acos(-1.0);

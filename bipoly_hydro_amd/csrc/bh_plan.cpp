@@ -116,13 +116,14 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
 // ---------------------------------------------------------------------------
 namespace {
 
-// Workgroups of a DFT that runs on the side stream: half the CUs (a
-// multiple of 8 for the XCD-chunked panel order), the rest left to the
-// main stream's kernels; 0 (one per CU) without the side stream.
+// Workgroups of a DFT that runs on the side stream: two per CU (a multiple
+// of 8 for the XCD-chunked panel order; the kernel fits four per CU, so the
+// main stream's kernels keep room beside it); 0 (one per panel) without the
+// side stream.
 int side_blocks(const bh_plan *p)
 {
     if (!p->overlap) return 0;
-    return p->side_cus > 0 ? p->side_cus : std::max(8, p->cus / 2 / 8 * 8);
+    return p->side_cus > 0 ? p->side_cus : std::max(8, 2 * p->cus / 8 * 8);
 }
 
 // Forward real DFT along phi of the N rows of B (columns Mcols), written in

@@ -131,7 +131,7 @@ struct bh_plan {
     // and the boundary (independent until the faces are scattered)
     bool overlap = true;
     int cus = 256;      // compute units of the device
-    int side_cus = 0;   // workgroups of the side-stream DFT (0: half the CUs)
+    int side_cus = 0;   // workgroups of the side-stream DFT (0: two per CU)
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     bool has_solution = false;
