@@ -26,8 +26,8 @@ HBM_PEAK_GBS = 8000.0
 # reduce) and of one FP64 ADI half-step at 258 x 130 x 512, from rocprofv3
 # FETCH_SIZE (x 2.000, the copy-kernel calibration) + WRITE_SIZE;
 # profiles/potential_solver_258x130x512_hbm_traffic.md
-CONTRACT_TRAFFIC = ((8672.5 + 567109.4 + 9244.0) * 2.0 + (30720.0 + 30720.2 + 240.0)) * 1024
-SWEEP_TRAFFIC = ((162565.4 + 144969.4) * 2.0 + (151260.3 + 139002.8)) * 1024 / 2  # mean of the two half-steps
+CONTRACT_TRAFFIC = ((8677.0 + 276840.0 + 2330.0) * 2.0 + (30721.0 + 3840.0 + 240.0)) * 1024  # FP32 table
+SWEEP_TRAFFIC = ((162531.0 + 144885.0) * 2.0 + (151557.0 + 138717.0)) * 1024 / 2  # mean of the two half-steps
 
 
 def rell2(a, b):

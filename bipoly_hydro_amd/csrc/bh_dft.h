@@ -19,7 +19,7 @@ struct DftArgs {
     int mask_numr, mask_numz;  // forward: zero the rhop layers (potential_solver.F90:70-78); 0 = no mask
     const float *xadd;  // inverse at N = 512: input is x + xadd (FP32), ...
     double *xstore;     // ... and x + xadd is also stored here (both null: plain input)
-    int max_blocks;     // single launches at N = 512: cap on the persistent grid (0: one per CU)
+    int max_blocks;     // single launches at N = 512: cap on the persistent grid (0: one per panel)
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };
@@ -29,7 +29,9 @@ Op dft_op(const DftArgs &a, int kind);
 size_t dft_lds_bytes(int kind, int N);
 // Matrix operands in fragment order: forward [row tile][k step][lane] with
 // cosine rows k = 0..N/2-1 then sine rows k = 1..N/2-1; inverse P rows then
-// Q rows, n = 0..N/2-1.
+// Q rows, n = 0..N/2-1.  At N = 512: the operand of size N/4 followed by the
+// radix-4 twiddles (cos, sin)(2 pi k r / N), r = 0..3 (forward k = 0..N/4,
+// inverse k = 0..N/8).
 void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *f64);
 void dft_inverse_fragments(int N, std::vector<double> *f64);
 

@@ -21,6 +21,11 @@
 // one contiguous 64-lane read from L2.  Row tile t of the forward product
 // belongs to wave t % 8; the inverse pairs the P and Q tiles of the same n
 // in one wave so that x_n and x_{N-n} are formed in registers.
+//
+// At N = 512 (the production grids) one radix-4 step comes first (forward:
+// in the input index, inverse: in the output index; the derivations are at
+// the kernels), so the products run at size N/4 on four virtual columns per
+// column: a quarter of the matrix work of the direct form.
 #include "bh_dft.h"
 #include "bh_ops.h"
 
@@ -79,7 +84,7 @@ struct Pad<double> {
 // ---------------------------------------------------------------------------
 // forward
 // ---------------------------------------------------------------------------
-template <typename T, int CB, bool CONTIG>
+template <typename T, int CB>
 __device__ __forceinline__ void dft_fwd_tile(const DftArgs &a, int bx, unsigned char *lds)
 {
     typedef typename Mma<T>::acc_t acc_t;
@@ -153,14 +158,11 @@ __device__ __forceinline__ void dft_fwd_tile(const DftArgs &a, int bx, unsigned 
         }
     }
 
-    // 3. the MFMA products.  With 32 row tiles (N = 512) wave w owns tiles
-    //    4w..4w+3, so that it reads only u (cosine rows) or only v (sine
-    //    rows); otherwise tiles w, w + 8, w + 16, w + 24.
-    const bool contiguous = CONTIG;  // launched with CONTIG exactly when RT == 32 (N = 512)
+    // 3. the MFMA products: wave w owns row tiles w, w + 8, w + 16, w + 24
     int rt[4], ntile = 0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        rt[i] = contiguous ? 4 * wave + i : wave + 8 * i;
+        rt[i] = wave + 8 * i;
         if (rt[i] < RT) ntile = i + 1;
     }
     bool need_u = false, need_v = false;
@@ -177,38 +179,7 @@ __device__ __forceinline__ void dft_fwd_tile(const DftArgs &a, int bx, unsigned 
         for (int t = 0; t < TN; ++t) acc[i][t] = (acc_t){(T)0, (T)0, (T)0, (T)0};
     const v4 *Af = (const v4 *)a.A;  // [row tile][group][lane] of 4
     const int jl = 4 * (lane >> 4);
-    if (contiguous) {
-        // four tiles of one half: one B source, no per-tile selects; A
-        // fragments alternate between two register sets, one group ahead
-        const T *Bp = (rt[0] < RTc) ? U : Vv;
-        const v4 *A0 = Af + ((long)rt[0] * KG) * 64 + lane;
-        const T *B0 = Bp + (lane & 15) * KPp + jl;
-        v4 p0[4], p1[4];
-        auto loadA = [&](int grp, v4 (&f)[4]) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) f[i] = A0[((long)i * KG + grp) * 64];
-        };
-        auto step = [&](int grp, const v4 (&f)[4]) {
-            v4 b[TN];
-#pragma unroll
-            for (int t = 0; t < TN; ++t) b[t] = *(const v4 *)(B0 + 16 * t * KPp + 16 * grp);
-#pragma unroll
-            for (int g = 0; g < 4; ++g)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int t = 0; t < TN; ++t) acc[i][t] = Mma<T>::mma(f[i][g], b[t][g], acc[i][t]);
-        };
-        loadA(0, p0);
-        int grp = 0;
-        for (; grp + 1 < KG; grp += 2) {
-            loadA(grp + 1, p1);
-            step(grp, p0);
-            if (grp + 2 < KG) loadA(grp + 2, p0);
-            step(grp + 1, p1);
-        }
-        if (grp < KG) step(grp, p0);
-    } else if (ntile > 0) {
+    if (ntile > 0) {
         v4 an[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i)
@@ -421,387 +392,8 @@ __device__ __forceinline__ void dft_inv_tile(const DftArgs &a, int bx, unsigned 
     }
 }
 
-// ---------------------------------------------------------------------------
-// Pipelined form for N = 512 (the production grids): a workgroup per CU
-// loops over its panels with two panel buffers in LDS.  While the MFMA
-// products of panel p run from one buffer, the loads of panel p + 1 are
-// spread over the k-groups (each thread loads one element pair per group,
-// two groups before storing it) into the other buffer, so HBM and the
-// matrix cores stay busy together.
-// ---------------------------------------------------------------------------
 namespace {
-constexpr int PN = 512, PH = PN / 2 - 1, PKG = (PH + 15) / 16, PKP = 16 * PKG;  // 255 -> 16 groups, 256
-
-template <typename T, int CB>
-struct PipeLayout {
-    static constexpr int KPp = PKP + Pad<T>::v;
-    static constexpr size_t half = (size_t)CB * KPp * sizeof(T);        // one of U / V
-    static constexpr size_t buf = 2 * half + 2 * CB * sizeof(double);    // U, V, x0, xh
-    static constexpr size_t bytes = 2 * buf + 16 * CB * sizeof(double);  // two buffers + part
-};
-
-// Loads of one staging slice (e = slice * 512 + tid) of panel bx of a.
-template <int CB>
-__device__ __forceinline__ void slice_load(const DftArgs &a, int bx, int slice, bool inverse, double &v1, double &v2)
-{
-    const int e = slice * 512 + threadIdx.x, j = e / CB, gc = bx * CB + e % CB;
-    v1 = v2 = 0.0;
-    if (e < PKP * CB && j < PH && gc < a.M) {
-        const long i1 = (long)(j + 1) * a.ldx + gc;
-        const long i2 = inverse ? (long)(PN / 2 + 1 + j) * a.ldx + gc : (long)(PN - 1 - j) * a.ldx + gc;
-        v1 = a.x[i1];
-        v2 = a.x[i2];
-        if (inverse && a.xadd) {  // x + xadd, kept as the new x
-            v1 += (double)a.xadd[i1];
-            v2 += (double)a.xadd[i2];
-            a.xstore[i1] = v1;
-            a.xstore[i2] = v2;
-        }
-    }
-}
-
-template <typename T, int CB>
-__device__ __forceinline__ void slice_store(unsigned char *buf, int slice, bool inverse, double v1, double v2)
-{
-    typedef PipeLayout<T, CB> Ly;
-    const int e = slice * 512 + threadIdx.x, j = e / CB, c = e % CB;
-    if (e >= PKP * CB) return;
-    T *P = (T *)buf, *Q = (T *)(buf + Ly::half);
-    P[c * Ly::KPp + j] = inverse ? (T)v1 : (T)(v1 + v2);
-    Q[c * Ly::KPp + j] = inverse ? (T)v2 : (T)(v1 - v2);
-}
-
-template <int CB>
-__device__ __forceinline__ void ends_load(const DftArgs &a, int bx, double &e0, double &eh)
-{
-    const int gc = bx * CB + threadIdx.x;
-    e0 = eh = 0.0;
-    if (threadIdx.x < CB && gc < a.M) {
-        const long ih = (long)(PN / 2) * a.ldx + gc;
-        e0 = a.x[gc];
-        eh = a.x[ih];
-        if (a.xadd) {
-            e0 += (double)a.xadd[gc];
-            eh += (double)a.xadd[ih];
-            a.xstore[gc] = e0;
-            a.xstore[ih] = eh;
-        }
-    }
-}
-
-template <typename T, int CB>
-__device__ __forceinline__ void ends_store(unsigned char *buf, double e0, double eh)
-{
-    typedef PipeLayout<T, CB> Ly;
-    double *d = (double *)(buf + 2 * Ly::half);
-    if (threadIdx.x < CB) {
-        d[threadIdx.x] = e0;
-        d[CB + threadIdx.x] = eh;
-    }
-}
-
-// Stage a whole panel (the first of a workgroup), no overlap.
-template <typename T, int CB>
-__device__ __forceinline__ void stage_all(const DftArgs &a, int bx, bool inverse, unsigned char *buf)
-{
-    constexpr int NS = (PKP * CB + 511) / 512;
-    double v1[NS], v2[NS];
-#pragma unroll
-    for (int sl = 0; sl < NS; ++sl) slice_load<CB>(a, bx, sl, inverse, v1[sl], v2[sl]);
-    double e0, eh;
-    ends_load<CB>(a, bx, e0, eh);
-#pragma unroll
-    for (int sl = 0; sl < NS; ++sl) slice_store<T, CB>(buf, sl, inverse, v1[sl], v2[sl]);
-    ends_store<T, CB>(buf, e0, eh);
-}
-
-// Row N/2 of the panel in buffer `buf` (forward: a_{N/2}; inverse: x_{N/2}).
-template <typename T, int CB>
-__device__ __forceinline__ void middle_row(const DftArgs &a, int bx, bool inverse, const unsigned char *buf,
-                                           double *part)
-{
-    typedef PipeLayout<T, CB> Ly;
-    const T *P = (const T *)buf;
-    const double *d = (const double *)(buf + 2 * Ly::half);
-    const int tid = threadIdx.x, c = tid % CB, q = tid / CB;
-    constexpr int NQ = 512 / CB;
-    double s = 0.0;
-    for (int j = q; j < PH; j += NQ) {
-        const double u = (double)P[c * Ly::KPp + j];
-        s += ((j + 1) & 1) ? -u : u;
-    }
-    if (q < 16) part[q * CB + c] = s;
-    __syncthreads();
-    if (NQ > 16 && q >= 16) part[(q & 15) * CB + c] += s;
-    __syncthreads();
-    if (tid < CB) {
-        double t = 0.0;
-        for (int i = 0; i < 16; ++i) t += part[i * CB + tid];
-        const int gc = bx * CB + tid;
-        if (gc < a.M) {
-            const double e0 = d[tid], eh = d[CB + tid];
-            double val;
-            if (inverse) {
-                const double invN = 1.0 / (double)PN;
-                val = e0 * invN + (((PN / 2) & 1) ? -eh : eh) * invN + 2.0 * invN * t;
-            } else {
-                val = e0 + (((PN / 2) & 1) ? -eh : eh) + t;
-                if (a.mask_numr) {
-                    const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
-                    if (J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1) val = 0.0;
-                }
-            }
-            ((T *)a.out)[(long)(PN / 2) * a.ldo + gc] = (T)val;
-        }
-    }
-}
-
-// The products of one panel from `cur`, staging the next panel (when
-// `nxt` is given) into `nbuf` along the way.
-template <typename T, int CB>
-__device__ __forceinline__ void fwd_panel(const DftArgs &a, int bx, const unsigned char *cur, bool more,
-                                          const DftArgs &nxt, int nbx, unsigned char *nbuf)
-{
-    typedef typename Mma<T>::acc_t acc_t;
-    typedef T v4 __attribute__((ext_vector_type(4)));
-    typedef PipeLayout<T, CB> Ly;
-    constexpr int TN = CB / 16, RTc = PN / 2 / 16;  // 16 cosine row tiles, 16 sine
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rt0 = 4 * wave;  // four tiles of one half
-    const bool cosw = rt0 < RTc;
-    const T *Bp = (const T *)(cur + (cosw ? 0 : Ly::half));
-    const double *d = (const double *)(cur + 2 * Ly::half);
-    const v4 *A0 = (const v4 *)a.A + ((long)rt0 * PKG) * 64 + lane;
-    const T *B0 = Bp + (lane & 15) * Ly::KPp + 4 * (lane >> 4);
-    acc_t acc[4][TN];
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int t = 0; t < TN; ++t) acc[i][t] = (acc_t){(T)0, (T)0, (T)0, (T)0};
-    v4 p0[4], p1[4];
-    double s1[2], s2[2];  // staging loads in flight, two groups deep
-    auto loadA = [&](int grp, v4 (&f)[4]) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) f[i] = A0[((long)i * PKG + grp) * 64];
-    };
-    auto step = [&](int grp, const v4 (&f)[4]) {
-        if (more) {
-            if (grp >= 2) slice_store<T, CB>(nbuf, grp - 2, false, s1[grp & 1], s2[grp & 1]);
-            slice_load<CB>(nxt, nbx, grp, false, s1[grp & 1], s2[grp & 1]);
-        }
-        v4 b[TN];
-#pragma unroll
-        for (int t = 0; t < TN; ++t) b[t] = *(const v4 *)(B0 + 16 * t * Ly::KPp + 16 * grp);
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int t = 0; t < TN; ++t) acc[i][t] = Mma<T>::mma(f[i][g], b[t][g], acc[i][t]);
-    };
-    loadA(0, p0);
-    for (int grp = 0; grp < PKG; grp += 2) {  // PKG = 16: even
-        loadA(grp + 1, p1);
-        step(grp, p0);
-        if (grp + 2 < PKG) loadA(grp + 2, p0);
-        step(grp + 1, p1);
-    }
-    if (more) {
-        slice_store<T, CB>(nbuf, PKG - 2, false, s1[0], s2[0]);
-        slice_store<T, CB>(nbuf, PKG - 1, false, s1[1], s2[1]);
-        double e0, eh;
-        ends_load<CB>(nxt, nbx, e0, eh);
-        ends_store<T, CB>(nbuf, e0, eh);
-    }
-    // epilogue
-    const int col0 = bx * CB;
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int t = 0; t < TN; ++t) {
-            const int c = 16 * t + (lane & 15), gc = col0 + c;
-            bool masked = false;
-            if (a.mask_numr) {
-                const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
-                masked = J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1;
-            }
-            const double e0 = d[c], eh = d[CB + c];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = 16 * ((rt0 + i) % RTc) + Mma<T>::row(lane, v);
-                if (gc >= a.M) continue;
-                double val = (double)acc[i][t][v];
-                long orow;
-                if (cosw) {
-                    orow = r;
-                    val = (e0 + ((r & 1) ? -eh : eh)) + val;
-                } else {
-                    if (r >= PH) continue;
-                    orow = PN / 2 + 1 + r;
-                }
-                ((T *)a.out)[orow * a.ldo + gc] = (T)(masked ? 0.0 : val);
-            }
-        }
-}
-
-template <int CB>
-__device__ __forceinline__ void inv_panel(const DftArgs &a, int bx, const unsigned char *cur, bool more,
-                                          const DftArgs &nxt, int nbx, unsigned char *nbuf)
-{
-    typedef double T;
-    typedef Mma<double>::acc_t acc_t;
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    typedef PipeLayout<T, CB> Ly;
-    constexpr int TN = CB / 16, NT = PN / 2 / 16;  // 16 n tiles of P and of Q
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const T *Ca = (const T *)cur, *Sb = (const T *)(cur + Ly::half);
-    const double *d = (const double *)(cur + 2 * Ly::half);
-    const v4 *Af = (const v4 *)a.A;
-    const int jl = 4 * (lane >> 4);
-    acc_t accP[2][TN], accQ[2][TN];
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int t = 0; t < TN; ++t) {
-            accP[i][t] = (acc_t){0.0, 0.0, 0.0, 0.0};
-            accQ[i][t] = (acc_t){0.0, 0.0, 0.0, 0.0};
-        }
-    double s1[2], s2[2];
-    auto loadA = [&](int grp, v4 (&fp)[2], v4 (&fq)[2]) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            fp[i] = Af[((long)(wave + 8 * i) * PKG + grp) * 64 + lane];
-            fq[i] = Af[((long)(NT + wave + 8 * i) * PKG + grp) * 64 + lane];
-        }
-    };
-    auto step = [&](int grp, const v4 (&fp)[2], const v4 (&fq)[2]) {
-        if (more) {
-            if (grp >= 2) slice_store<T, CB>(nbuf, grp - 2, true, s1[grp & 1], s2[grp & 1]);
-            slice_load<CB>(nxt, nbx, grp, true, s1[grp & 1], s2[grp & 1]);
-        }
-        v4 bc[TN], bs[TN];
-#pragma unroll
-        for (int t = 0; t < TN; ++t) {
-            const int off = (16 * t + (lane & 15)) * Ly::KPp + 16 * grp + jl;
-            bc[t] = *(const v4 *)(Ca + off);
-            bs[t] = *(const v4 *)(Sb + off);
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g)
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int t = 0; t < TN; ++t) {
-                    accP[i][t] = Mma<double>::mma(fp[i][g], bc[t][g], accP[i][t]);
-                    accQ[i][t] = Mma<double>::mma(fq[i][g], bs[t][g], accQ[i][t]);
-                }
-    };
-    v4 p0[2], q0[2], p1[2], q1[2];
-    loadA(0, p0, q0);
-    for (int grp = 0; grp < PKG; grp += 2) {
-        loadA(grp + 1, p1, q1);
-        step(grp, p0, q0);
-        if (grp + 2 < PKG) loadA(grp + 2, p0, q0);
-        step(grp + 1, p1, q1);
-    }
-    if (more) {
-        slice_store<T, CB>(nbuf, PKG - 2, true, s1[0], s2[0]);
-        slice_store<T, CB>(nbuf, PKG - 1, true, s1[1], s2[1]);
-        double e0, eh;
-        ends_load<CB>(nxt, nbx, e0, eh);
-        ends_store<T, CB>(nbuf, e0, eh);
-    }
-    const double invN = 1.0 / (double)PN;
-    const int col0 = bx * CB;
-    double *out = (double *)a.out;
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int t = 0; t < TN; ++t) {
-            const int c = 16 * t + (lane & 15), gc = col0 + c;
-            const double e0 = d[c] * invN, eh = d[CB + c] * invN;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int n = 16 * (wave + 8 * i) + Mma<double>::row(lane, v);
-                if (gc >= a.M) continue;
-                const double p = (e0 + ((n & 1) ? -eh : eh)) + accP[i][t][v];
-                const double q = accQ[i][t][v];
-                out[(long)n * a.ldo + gc] = p - q;
-                if (n > 0) out[(long)(PN - n) * a.ldo + gc] = p + q;
-            }
-        }
-}
-
-// Workgroup loop over panels t0, t0 + st, ... < t1 (XCD-chunked as in
-// BH_TILE_LOOP); `resolve(t, args, bx)` names a tile's arguments and panel.
-template <typename T, int CB, bool INV, class Resolve>
-__device__ __forceinline__ void pipe_loop(int total, Resolve resolve, unsigned char *lds)
-{
-    typedef PipeLayout<T, CB> Ly;
-    const int n_ = gridDim.x, b_ = blockIdx.x;
-    int t0 = b_, t1 = total, st = n_;
-    if (n_ >= 8 && (n_ & 7) == 0) {
-        const int c_ = (total + 7) >> 3, x_ = b_ & 7;
-        t0 = x_ * c_ + (b_ >> 3);
-        t1 = min(total, (x_ + 1) * c_);
-        st = n_ >> 3;
-    }
-    if (t0 >= t1) return;
-    double *part = (double *)(lds + 2 * Ly::buf);
-    DftArgs a, an;
-    int bx, nbx = 0;
-    resolve(t0, a, bx);
-    stage_all<T, CB>(a, bx, INV, lds);
-    __syncthreads();
-    int k = 0;
-    for (int t = t0; t < t1; t += st, k ^= 1) {
-        const bool more = t + st < t1;
-        if (more) resolve(t + st, an, nbx);
-        unsigned char *cur = lds + (size_t)k * Ly::buf, *nb = lds + (size_t)(k ^ 1) * Ly::buf;
-        middle_row<T, CB>(a, bx, INV, cur, part);
-        if (INV) inv_panel<CB>(a, bx, cur, more, an, nbx, nb);
-        else fwd_panel<T, CB>(a, bx, cur, more, an, nbx, nb);
-        __syncthreads();
-        a = an;
-        bx = nbx;
-    }
-}
-
-template <int V>
-struct PipeCfg;
-template <> struct PipeCfg<0> { typedef float T; static constexpr int CB = 32; static constexpr bool INV = false; };
-template <> struct PipeCfg<1> { typedef double T; static constexpr int CB = 16; static constexpr bool INV = false; };
-template <> struct PipeCfg<2> { typedef double T; static constexpr int CB = 16; static constexpr bool INV = true; };
-
-template <int V>
-__global__ __launch_bounds__(512) void dft_pipe_kernel(DftArgs a, int nx, int ny, int total)
-{
-    extern __shared__ __align__(16) unsigned char pipe_lds[];
-    typedef PipeCfg<V> C;
-    pipe_loop<typename C::T, C::CB, C::INV>(
-        total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, pipe_lds);
-}
-
-template <int V>
-__global__ __launch_bounds__(512) void dft_pipe_grouped(const DftArgs *__restrict__ args, GroupSched gs)
-{
-    extern __shared__ __align__(16) unsigned char pipe_lds[];
-    typedef PipeCfg<V> C;
-    pipe_loop<typename C::T, C::CB, C::INV>(
-        gs.total,
-        [&](int t, DftArgs &o, int &bx) {
-            const int g = bh_find_group(gs.grp, gs.ngroups, t);
-            o = args[g];
-            bx = t - gs.grp[g].tile0;
-        },
-        pipe_lds);
-}
-
-template <int V>
-constexpr size_t pipe_lds_bytes()
-{
-    return PipeLayout<typename PipeCfg<V>::T, PipeCfg<V>::CB>::bytes;
-}
+constexpr int PN = DFT_MAX_N;  // the radix-4 kernels' transform length
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1166,16 +758,13 @@ static int log2i(int n)
     return l;
 }
 
+// N < 512: V = the kind (0 forward FP32, 1 forward FP64, 2 inverse FP64)
 template <int V>
 __device__ __forceinline__ void dft_tile_v(const DftArgs &a, int bx, unsigned char *lds)
 {
-    // kinds 0-2 at N = 512 (32 row tiles: contiguous per wave), 4-5: the
-    // forward products at smaller N
-    if (V == 0) dft_fwd_tile<float, 32, true>(a, bx, lds);
-    else if (V == 1) dft_fwd_tile<double, 16, true>(a, bx, lds);
-    else if (V == 2) dft_inv_tile<16>(a, bx, lds);
-    else if (V == 4) dft_fwd_tile<float, 32, false>(a, bx, lds);
-    else dft_fwd_tile<double, 16, false>(a, bx, lds);
+    if (V == 0) dft_fwd_tile<float, 32>(a, bx, lds);
+    else if (V == 1) dft_fwd_tile<double, 16>(a, bx, lds);
+    else dft_inv_tile<16>(a, bx, lds);
 }
 
 template <int V>
@@ -1201,125 +790,79 @@ Op dft_op(const DftArgs &a, int kind)
 namespace {
 const void *dft_fn(int variant, bool grouped)
 {
-    const int kind = variant % 4, N = 1 << (variant / 4);
-    const bool contig = N == DFT_MAX_N;  // 32 row tiles: four per wave
-    if (kind == DFT_INV_F64) return grouped ? (const void *)&dft_grouped<2> : (const void *)&dft_kernel<2>;
-    if (kind == DFT_FWD_F32) {
-        if (contig) return grouped ? (const void *)&dft_grouped<0> : (const void *)&dft_kernel<0>;
-        return grouped ? (const void *)&dft_grouped<4> : (const void *)&dft_kernel<4>;
+    switch (variant % 4) {
+    case DFT_FWD_F32: return grouped ? (const void *)&dft_grouped<0> : (const void *)&dft_kernel<0>;
+    case DFT_FWD_F64: return grouped ? (const void *)&dft_grouped<1> : (const void *)&dft_kernel<1>;
+    default: return grouped ? (const void *)&dft_grouped<2> : (const void *)&dft_kernel<2>;
     }
-    if (contig) return grouped ? (const void *)&dft_grouped<1> : (const void *)&dft_kernel<1>;
-    return grouped ? (const void *)&dft_grouped<5> : (const void *)&dft_kernel<5>;
 }
 }  // namespace
 
-static size_t pipe_bytes(int kind)
-{
-    return kind == 0 ? pipe_lds_bytes<0>() : kind == 1 ? pipe_lds_bytes<1>() : pipe_lds_bytes<2>();
-}
-
-static const void *pipe_fn(int kind, bool grouped)
+// the radix-4 kernels of N = 512: function and dynamic LDS per kind
+static const void *r4_fn(int kind, bool grouped, int *lds)
 {
     switch (kind) {
-    case 0: return grouped ? (const void *)&dft_pipe_grouped<0> : (const void *)&dft_pipe_kernel<0>;
-    case 1: return grouped ? (const void *)&dft_pipe_grouped<1> : (const void *)&dft_pipe_kernel<1>;
-    default: return grouped ? (const void *)&dft_pipe_grouped<2> : (const void *)&dft_pipe_kernel<2>;
+    case DFT_INV_F64:
+        *lds = (int)R4_LDS;
+        return grouped ? (const void *)&dft_inv4_grouped : (const void *)&dft_inv4_kernel;
+    case DFT_FWD_F32:
+        *lds = (int)R4Fwd<float>::bytes;
+        return grouped ? (const void *)&dft_fwd4_grouped<float> : (const void *)&dft_fwd4_kernel<float>;
+    default:
+        *lds = (int)R4Fwd<double>::bytes;
+        return grouped ? (const void *)&dft_fwd4_grouped<double> : (const void *)&dft_fwd4_kernel<double>;
     }
-}
-
-// one workgroup per CU (two panel buffers fill the LDS)
-static int pipe_blocks(int total)
-{
-    static int cus = 0;
-    if (!cus) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-            cus = 256;
-    }
-    const int n = total < cus ? total : cus;
-    return n >= 8 ? n / 8 * 8 : n;
 }
 
 bool dft_launch(const Op &op, hipStream_t s)
 {
     if (op.kind != OP_DFT) return false;
+    int nx = op.nx, ny = op.ny, total = (int)op.tiles();
+    void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
+    int nb = persistent_blocks(total, 0);
     if ((1 << (op.variant / 4)) == DFT_MAX_N) {
-        const int kind = op.variant % 4;
-        int nx = op.nx, ny = op.ny, total = (int)op.tiles();
-        void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
-        if (kind == DFT_INV_F64) {
-            (void)hipLaunchKernel((const void *)&dft_inv4_kernel, dim3(persistent_blocks(total, 0)), dim3(512), argv,
-                                  (int)R4_LDS, s);
-            return true;
-        }
-        if (kind == DFT_FWD_F32 || kind == DFT_FWD_F64) {
-            const bool f32 = kind == DFT_FWD_F32;
-            const int cap = op.as<DftArgs>().max_blocks;
-            int nb = persistent_blocks(total, 0);
-            if (cap > 0 && nb > cap) nb = cap >= 8 ? cap / 8 * 8 : cap;
-            (void)hipLaunchKernel(f32 ? (const void *)&dft_fwd4_kernel<float> : (const void *)&dft_fwd4_kernel<double>,
-                                  dim3(nb), dim3(512), argv, (int)(f32 ? R4Fwd<float>::bytes : R4Fwd<double>::bytes), s);
-            return true;
-        }
-        const int cap = op.as<DftArgs>().max_blocks;
-        const int nb = cap > 0 ? std::min(pipe_blocks(total), cap) : pipe_blocks(total);
-        (void)hipLaunchKernel(pipe_fn(kind, false), dim3(nb), dim3(512), argv, (int)pipe_bytes(kind), s);
+        int lds;
+        const void *fn = r4_fn(op.variant % 4, false, &lds);
+        const int cap = op.as<DftArgs>().max_blocks;  // side stream: leave room for the main stream
+        if (cap > 0 && nb > cap) nb = cap >= 8 ? cap / 8 * 8 : cap;
+        (void)hipLaunchKernel(fn, dim3(nb), dim3(512), argv, lds, s);
         return true;
     }
     const int lds = (int)dft_lds_bytes(op.variant % 4, 1 << (op.variant / 4));
-    int nx = op.nx, ny = op.ny, total = (int)op.tiles();
-    void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
-    const void *fn = dft_fn(op.variant, false);
-    (void)hipLaunchKernel(fn, dim3(persistent_blocks(total, 0)), dim3(512), argv, lds, s);
+    (void)hipLaunchKernel(dft_fn(op.variant, false), dim3(nb), dim3(512), argv, lds, s);
     return true;
 }
 
 bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
 {
     if (kind != OP_DFT) return false;
-    if ((1 << (variant / 4)) == DFT_MAX_N) {
-        GroupSched g = gs;
-        const void *ap = d_args;
-        void *argv[] = {(void *)&ap, (void *)&g};
-        if (variant % 4 == DFT_INV_F64) {
-            (void)hipLaunchKernel((const void *)&dft_inv4_grouped, dim3(nblocks), dim3(512), argv, (int)R4_LDS, s);
-            return true;
-        }
-        if (variant % 4 == DFT_FWD_F32) {
-            (void)hipLaunchKernel((const void *)&dft_fwd4_grouped<float>, dim3(nblocks), dim3(512), argv,
-                                  (int)R4Fwd<float>::bytes, s);
-            return true;
-        }
-        if (variant % 4 == DFT_FWD_F64) {
-            (void)hipLaunchKernel((const void *)&dft_fwd4_grouped<double>, dim3(nblocks), dim3(512), argv,
-                                  (int)R4Fwd<double>::bytes, s);
-            return true;
-        }
-        (void)hipLaunchKernel(pipe_fn(variant % 4, true), dim3(pipe_blocks(gs.total)), dim3(512), argv,
-                              (int)pipe_bytes(variant % 4), s);
-        return true;
-    }
-    const int lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4));
     GroupSched g = gs;
     const void *ap = d_args;
     void *argv[] = {(void *)&ap, (void *)&g};
-    (void)hipLaunchKernel(dft_fn(variant, true), dim3(nblocks), dim3(512), argv, lds, s);
+    int lds;
+    const void *fn;
+    if ((1 << (variant / 4)) == DFT_MAX_N) {
+        fn = r4_fn(variant % 4, true, &lds);
+    } else {
+        lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4));
+        fn = dft_fn(variant, true);
+    }
+    (void)hipLaunchKernel(fn, dim3(nblocks), dim3(512), argv, lds, s);
     return true;
 }
 
 bool dft_occupancy(int kind, int variant, int *blocks)
 {
     if (kind != OP_DFT) return false;
+    int lds;
+    const void *fn;
     if ((1 << (variant / 4)) == DFT_MAX_N) {
-        const int k = variant % 4;
-        if (k == DFT_INV_F64) *blocks = kernel_occupancy((const void *)&dft_inv4_grouped, 512, (int)R4_LDS);
-        else if (k == DFT_FWD_F32)
-            *blocks = kernel_occupancy((const void *)&dft_fwd4_grouped<float>, 512, (int)R4Fwd<float>::bytes);
-        else *blocks = kernel_occupancy((const void *)&dft_fwd4_grouped<double>, 512, (int)R4Fwd<double>::bytes);
-        return true;
+        fn = r4_fn(variant % 4, true, &lds);
+    } else {
+        lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4));
+        fn = dft_fn(variant, true);
     }
-    *blocks = kernel_occupancy(dft_fn(variant, true), 512, (int)dft_lds_bytes(variant % 4, 1 << (variant / 4)));
+    *blocks = kernel_occupancy(fn, 512, lds);
     return true;
 }
 
