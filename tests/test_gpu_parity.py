@@ -35,7 +35,10 @@ def test_fp64_matches_reference_cold_then_warm(golden_dir, name):
     np.testing.assert_array_equal(pot1[:h, :, 0], pot1[h:, :, 1])
 
 
-@pytest.mark.parametrize("shape,seed", [((130, 66, 128), 5), ((98, 34, 64), 11), ((194, 130, 32), 2)])
+# (290, 18, 16): nri = 288 > 256 takes the earlier top/bottom contraction
+# kernel; the others the register-blocked one with 8, 6 and 12 row blocks
+@pytest.mark.parametrize("shape,seed", [((130, 66, 128), 5), ((98, 34, 64), 11), ((194, 130, 32), 2),
+                                        ((290, 18, 16), 3)])
 def test_fp64_matches_oracle_on_other_grids(shape, seed):
     from oracle import oracle as O
     numr, numz, numphi = shape
@@ -143,3 +146,23 @@ def test_mixed_production_grid_against_reference_slices(golden_dir):
             e = rell2(v, g[k])
             print(f"mixed c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
             assert e <= TOL_MIXED, (tag, k, e)
+
+
+@pytest.mark.parametrize("shape,seed", [((98, 34, 64), 11), ((290, 18, 16), 3)])
+def test_mixed_matches_oracle_on_other_grids(shape, seed):
+    """The mixed path (FP32 table copy, FP32 transforms) off the golden
+    grids: an even block count and nri > 256 (the earlier contraction kernel
+    with the FP32 table, and the separate spectral kernels)."""
+    from oracle import oracle as O
+    numr, numz, numphi = shape
+    rho1 = density.scaled_binary(numr, numz, numphi, seed)
+    rho2 = density.scaled_binary(numr, numz, numphi, seed, mass_ratio=0.5, separation=0.6)
+    oplan = O.OraclePlan(numr, numz, numphi)
+    o1 = oplan.solve(rho1, tstep=0)
+    o2 = oplan.solve(rho2, potp=o1, tstep=3)
+    with P.PotentialPlan(numr, numz, numphi, precision="mixed") as plan:
+        g1 = plan.potential_solver(rho1, tstep=0)
+        g2 = plan.potential_solver(rho2, tstep=3)
+    e1, e2 = rell2(g1, o1), rell2(g2, o2)
+    print(f"mixed {shape}: cold {e1:.3e} warm {e2:.3e}", flush=True)
+    assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
