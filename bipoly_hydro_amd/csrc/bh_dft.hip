@@ -415,60 +415,112 @@ constexpr int R4N = PN / 4, R4KG = 4, R4KPp = 16 * R4KG + 2, R4CB = 16, R4VC = 4
 constexpr size_t R4_LDS = (size_t)2 * R4VC * R4KPp * 8 + 2 * R4VC * 8;
 constexpr int R4_FRAGS = 8 * R4KG * 64 * 4;  // doubles of the size-n P and Q fragments
 
-__device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned char *lds)
+// Persistent loop over panels t0, t0 + st, ... < t1 (XCD-chunked as in
+// BH_TILE_LOOP): the loads of panel t + st are issued into registers
+// before the products of panel t run, so HBM and the matrix cores work
+// together.  load(args, bx, regs) issues the loads; form(args, bx, regs)
+// writes the panel's operands to LDS; work(args, bx) runs the products and
+// writes the outputs.
+template <class Regs, class Resolve, class Load, class Form, class Work>
+__device__ __forceinline__ void r4_loop(int total, Resolve resolve, Load load, Form form, Work work)
 {
-    typedef Mma<double>::acc_t acc_t;
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    double *Ca = (double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
+    const int n_ = gridDim.x, b_ = blockIdx.x;
+    int t0 = b_, t1 = total, st = n_;
+    if (n_ >= 8 && (n_ & 7) == 0) {
+        const int c_ = (total + 7) >> 3, x_ = b_ & 7;
+        t0 = x_ * c_ + (b_ >> 3);
+        t1 = min(total, (x_ + 1) * c_);
+        st = n_ >> 3;
+    }
+    if (t0 >= t1) return;
+    DftArgs a, an;
+    int bx, nbx = 0;
+    Regs R;
+    resolve(t0, a, bx);
+    load(a, bx, R);
+    for (int t = t0; t < t1; t += st) {
+        form(a, bx, R);
+        __syncthreads();
+        const bool more = t + st < t1;
+        if (more) {
+            resolve(t + st, an, nbx);
+            load(an, nbx, R);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        work(a, bx);
+        __syncthreads();
+        if (more) {
+            a = an;
+            bx = nbx;
+        }
+    }
+}
+
+// 64 x 16 items per panel, two per thread: item (kk, c) is the butterfly of
+// k = kk for kk = 1..n/2-1, and for kk = 0 both k = 0 and k = n/2 (8 input
+// rows between them, like one regular item)
+constexpr int R4_IPT = 2;
+
+struct Inv4Regs {
+    double av[R4_IPT][4], sv[R4_IPT][4];
+    float fa[R4_IPT][4], fs[R4_IPT][4];
+};
+
+// mode rows of X_k, X_{k+n}, X_{2n-k}, X_{n-k} (q = 0..3)
+__device__ __forceinline__ int inv4_row(int k, int q)
+{
+    return q == 0 ? k : q == 1 ? R4N + k : q == 2 ? 2 * R4N - k : R4N - k;
+}
+// the input row held in slot (q, s) of item kk (s = 0: av, 1: sv), or -1
+// for a zero (s_0 = s_{N/2} = 0).  kk = 0: av = a_0, a_n, a_2n, a_{n/2};
+// sv = s_n, s_{n/2}, a_{3n/2}, s_{3n/2}.
+__device__ __forceinline__ int inv4_slot_row(int kk, int q, int s)
+{
+    if (kk == 0) {
+        const int r0[4] = {0, R4N, 2 * R4N, R4N / 2};
+        const int r1[4] = {PN / 2 + R4N, PN / 2 + R4N / 2, 3 * R4N / 2, PN / 2 + 3 * R4N / 2};
+        return s == 0 ? r0[q] : r1[q];
+    }
+    const int row = inv4_row(kk, q);
+    return s == 0 ? row : PN / 2 + row;  // kk = 1..n/2-1: every s row is 1..N/2-1
+}
+
+__device__ __forceinline__ void inv4_load(const DftArgs &a, int bx, Inv4Regs &R)
+{
     int tid = threadIdx.x;
-    asm volatile("" : "+v"(tid));  // (as ldx below: lane-dependent offsets recomputed per tile)
-    const int lane = tid & 63, wave = tid >> 6;
-    const int col0 = bx * R4CB;
-    const double *x = a.x;
+    asm volatile("" : "+v"(tid));  // lane-dependent offsets recomputed per panel
     long ldx = a.ldx;
-    // opaque per tile: keeps the row offsets below from being hoisted out of
-    // the persistent loop into registers held across tiles
-    asm volatile("" : "+s"(ldx));
-    const double *tw = (const double *)a.A + R4_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N)
-    // 1. quarter spectra Y^(r)_k, k = 0..n/2, times 1/4
-#pragma unroll 1
-    for (int e = tid; e < (R4N / 2 + 1) * R4CB; e += 512) {
-        const int k = e / R4CB, c = e - k * R4CB, gc = col0 + c;
-        const bool ok = gc < a.M;
-        const int gcc = ok ? gc : a.M - 1;
-        const int ra[4] = {k, R4N + k, 2 * R4N - k, R4N - k};  // mode rows of X_k, X_{k+n}, X_{2n-k}, X_{n-k}
-        double av[4], sv[4];
+    asm volatile("" : "+s"(ldx));  // (not hoisted into registers held across panels)
+    const double *x = a.x;
+#pragma unroll
+    for (int i = 0; i < R4_IPT; ++i) {
+        const int e = tid + 512 * i, kk = e / R4CB, c = e - kk * R4CB, gc = min(bx * R4CB + c, a.M - 1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const bool hs = ra[q] > 0 && ra[q] < PN / 2;  // s_0 = s_{N/2} = 0
-            av[q] = x[(long)ra[q] * ldx + gcc];
-            sv[q] = hs ? x[(long)(PN / 2 + ra[q]) * ldx + gcc] : 0.0;
-        }
-        if (a.xadd) {
-            // x + xadd, stored once per row: q = 0, 1 always, q = 2 for k < n/2,
-            // q = 3 for 0 < k < n/2 (the four ranges tile rows 0..N/2)
-            float fa[4], fs[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool hs = ra[q] > 0 && ra[q] < PN / 2;
-                fa[q] = a.xadd[(long)ra[q] * ldx + gcc];
-                fs[q] = hs ? a.xadd[(long)(PN / 2 + ra[q]) * ldx + gcc] : 0.0f;
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool hs = ra[q] > 0 && ra[q] < PN / 2;
-                av[q] += (double)fa[q];
-                sv[q] += (double)fs[q];
-                const bool own = ok && (q < 2 || (q == 2 && k < R4N / 2) || (q == 3 && k > 0 && k < R4N / 2));
-                if (own) {
-                    a.xstore[(long)ra[q] * ldx + gc] = av[q];
-                    if (hs) a.xstore[(long)(PN / 2 + ra[q]) * ldx + gc] = sv[q];
-                }
+            const long o0 = (long)inv4_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv4_slot_row(kk, q, 1) * ldx + gc;
+            R.av[i][q] = x[o0];
+            R.sv[i][q] = x[o1];
+            if (a.xadd) {
+                R.fa[i][q] = a.xadd[o0];
+                R.fs[i][q] = a.xadd[o1];
             }
         }
-        // X_k, X_{k+n}, X_{k+2n} = conj X_{2n-k}, X_{k+3n} = conj X_{n-k}
-        const double x0r = av[0], x0i = sv[0], x1r = av[1], x1i = sv[1];
-        const double x2r = av[2], x2i = -sv[2], x3r = av[3], x3i = -sv[3];
+    }
+}
+
+// quarter spectra Y^(r)_k, k = 0..n/2, times 1/4, into LDS; with xadd, x +
+// xadd is also stored (every input row belongs to exactly one item slot)
+__device__ __forceinline__ void inv4_form(const DftArgs &a, int bx, const Inv4Regs &R, unsigned char *lds)
+{
+    double *Ca = (double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    long ldx = a.ldx;
+    asm volatile("" : "+s"(ldx));
+    const double *tw = (const double *)a.A + R4_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N)
+    // Y^(r) of one k from X_k, X_{k+n}, conj X_{2n-k}, conj X_{n-k}
+    auto quarters = [&](int k, double x0r, double x0i, double x1r, double x1i, double x2r, double x2i, double x3r,
+                        double x3i, double *yr, double *yi) {
         const double pr = x0r + x2r, pi = x0i + x2i, mr = x0r - x2r, mi = x0i - x2i;  // X0 +- X2
         const double ur = x1r + x3r, ui = x1i + x3i, dr = x1r - x3r, di = x1i - x3i;  // X1 +- X3
         // r = 0: (X0+X2) + (X1+X3); 1: (X0-X2) + i (X1-X3); 2: (X0+X2) - (X1+X3); 3: (X0-X2) - i (X1-X3)
@@ -477,23 +529,69 @@ __device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const double wc = tw[(k * 4 + r) * 2], ws = tw[(k * 4 + r) * 2 + 1];
-            const double yr = 0.25 * (wc * zr[r] - ws * zi[r]), yi = 0.25 * (wc * zi[r] + ws * zr[r]);
-            const int vc = r * R4CB + c;
-            if (k == 0) {
-                E0[vc] = yr;
-            } else if (k == R4N / 2) {
-                EH[vc] = yr;
+            yr[r] = 0.25 * (wc * zr[r] - ws * zi[r]);
+            yi[r] = 0.25 * (wc * zi[r] + ws * zr[r]);
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < R4_IPT; ++i) {
+        const int e = tid + 512 * i, kk = e / R4CB, c = e - kk * R4CB, gc = bx * R4CB + c;
+        double av[4], sv[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            av[q] = R.av[i][q];
+            sv[q] = R.sv[i][q];
+        }
+        if (a.xadd) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                av[q] += (double)R.fa[i][q];
+                sv[q] += (double)R.fs[i][q];
+                if (gc < a.M) {
+                    a.xstore[(long)inv4_slot_row(kk, q, 0) * ldx + gc] = av[q];
+                    a.xstore[(long)inv4_slot_row(kk, q, 1) * ldx + gc] = sv[q];
+                }
+            }
+        }
+        double yr[4], yi[4];
+        if (kk == 0) {
+            // k = 0: X_0 = a_0, X_n = a_n + i s_n, X_2n = a_2n, conj X_n
+            quarters(0, av[0], 0.0, av[1], sv[0], av[2], 0.0, av[1], -sv[0], yr, yi);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) E0[r * R4CB + c] = yr[r];
+            // k = n/2: X_{n/2}, X_{3n/2}, conj X_{3n/2}, conj X_{n/2}
+            quarters(R4N / 2, av[3], sv[1], sv[2], sv[3], sv[2], -sv[3], av[3], -sv[1], yr, yi);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int vc = r * R4CB + c;
+                EH[vc] = yr[r];
                 Ca[vc * R4KPp + R4N / 2 - 1] = 0.0;  // K padding (k = 64 of the 16 R4KG slots)
                 Sb[vc * R4KPp + R4N / 2 - 1] = 0.0;
-            } else {
-                Ca[vc * R4KPp + k - 1] = yr;
-                Sb[vc * R4KPp + k - 1] = yi;
+            }
+        } else {
+            // X_k, X_{k+n}, X_{k+2n} = conj X_{2n-k}, X_{k+3n} = conj X_{n-k}
+            quarters(kk, av[0], sv[0], av[1], sv[1], av[2], -sv[2], av[3], -sv[3], yr, yi);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int vc = r * R4CB + c;
+                Ca[vc * R4KPp + kk - 1] = yr[r];
+                Sb[vc * R4KPp + kk - 1] = yi[r];
             }
         }
     }
-    __syncthreads();
+}
 
-    // 2. size-n products: P rows m = 16 mt.., Q the same, for quarters t0, t0 + 1
+// size-n products (wave w: P and Q rows m = 16 (w % 4).., quarters t0 and
+// t0 + 1), then the rows 4m + r, 4(n - m) + r and the middle rows
+__device__ __forceinline__ void inv4_work(const DftArgs &a, int bx, const unsigned char *lds)
+{
+    typedef Mma<double>::acc_t acc_t;
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    const double *Ca = (const double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
+    const int col0 = bx * R4CB;
     const v4 *Af = (const v4 *)a.A;
     const int mt = wave & 3, t0 = 2 * (wave >> 2), jl = 4 * (lane >> 4);
     acc_t accP[2], accQ[2];
@@ -514,7 +612,6 @@ __device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned
             }
         }
     }
-    // 3. rows 4m + r (m < n/2) and 4(n - m) + r
     const double invn = 1.0 / (double)R4N;
     double *out = (double *)a.out;
 #pragma unroll
@@ -545,16 +642,32 @@ __device__ __forceinline__ void dft_inv4_tile(const DftArgs &a, int bx, unsigned
     }
 }
 
-__global__ __launch_bounds__(512) void dft_inv4_kernel(DftArgs a, int nx, int ny, int total)
+template <class Resolve>
+__device__ __forceinline__ void inv4_loop(int total, Resolve resolve, unsigned char *lds)
 {
-    extern __shared__ __align__(16) unsigned char inv4_lds[];
-    BH_SINGLE_LOOP(a, nx, ny, total, (dft_inv4_tile(a, bx, inv4_lds)));
+    r4_loop<Inv4Regs>(
+        total, resolve, [&](const DftArgs &a, int bx, Inv4Regs &R) { inv4_load(a, bx, R); },
+        [&](const DftArgs &a, int bx, const Inv4Regs &R) { inv4_form(a, bx, R, lds); },
+        [&](const DftArgs &a, int bx) { inv4_work(a, bx, lds); });
 }
 
-__global__ __launch_bounds__(512) void dft_inv4_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv4_kernel(DftArgs a, int nx, int ny, int total)
 {
     extern __shared__ __align__(16) unsigned char inv4_lds[];
-    BH_GROUP_LOOP(DftArgs, args, gs, (dft_inv4_tile(ga, bx, inv4_lds)));
+    inv4_loop(total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, inv4_lds);
+}
+
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv4_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+{
+    extern __shared__ __align__(16) unsigned char inv4_lds[];
+    inv4_loop(
+        gs.total,
+        [&](int t, DftArgs &o, int &bx) {
+            const int g = bh_find_group(gs.grp, gs.ngroups, t);
+            o = args[g];
+            bx = t - gs.grp[g].tile0;
+        },
+        inv4_lds);
 }
 }  // namespace
 
