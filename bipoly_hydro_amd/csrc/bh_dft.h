@@ -20,6 +20,10 @@ struct DftArgs {
     const float *xadd;  // inverse at N = 512: input is x + xadd (FP32), ...
     double *xstore;     // ... and x + xadd is also stored here (both null: plain input)
     int max_blocks;     // single launches at N = 512: cap on the persistent grid (0: one per panel)
+    // forward FP32 at N = 512 with the mask: also rout[L][K-2][J-2] =
+    // rscale * out for the interior points (null: not written)
+    float *rout;
+    double rscale;
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };

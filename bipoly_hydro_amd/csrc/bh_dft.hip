@@ -800,10 +800,14 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
         const int k = e / R4CB, c = e - k * R4CB, gc = col0 + c;
         if (gc >= a.M) continue;
         bool masked = false;
+        long ri = -1;  // offset of (J, K) in a plane of rout, if interior
         if (a.mask_numr) {
             const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
             masked = J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1;
+            if (a.rout && J >= 2 && J <= a.mask_numr - 1 && K >= 2 && K <= a.mask_numz - 1)
+                ri = (long)(K - 2) * (a.mask_numr - 2) + (J - 2);
         }
+        const long rplane = (long)(a.mask_numr - 2) * (a.mask_numz - 2);
         double gr[4], gi[4], hr[4], hi[4];  // G^(r)_k and G'^(r)_k
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -825,8 +829,13 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
         T *out = (T *)a.out;
         const long ldo = a.ldo;
         auto put = [&](int row, double re, double im) {  // a_row = Re, s-row N/2 + row = Im
-            out[(long)row * ldo + gc] = (T)(masked ? 0.0 : re);
-            if (row > 0 && row < PN / 2) out[(long)(PN / 2 + row) * ldo + gc] = (T)(masked ? 0.0 : im);
+            const T vr = (T)(masked ? 0.0 : re), vi = (T)(masked ? 0.0 : im);
+            out[(long)row * ldo + gc] = vr;
+            if (ri >= 0) a.rout[(long)row * rplane + ri] = (float)(a.rscale * (double)vr);
+            if (row > 0 && row < PN / 2) {
+                out[(long)(PN / 2 + row) * ldo + gc] = vi;
+                if (ri >= 0) a.rout[(long)(PN / 2 + row) * rplane + ri] = (float)(a.rscale * (double)vi);
+            }
         };
         put(k, x0r, x0i);
         put(R4N + k, x1r, x1i);

@@ -68,11 +68,8 @@ struct RebuildArgs {
     int numr, numz, numphi, mmax;
     double factor;
     int nsplit_tb;  // partial sums per top/bottom value in Stb
-};
-struct ScatterArgs {
-    const double *FF;
-    double *X1, *X2;
-    int numr, numz, x2_klayout;
+    double *X1, *X2;  // when X1 is given: each face value also placed at its (J, K) in X1 and X2
+    int x2_klayout;
 };
 
 // grid-stride fill for plan set-up
@@ -149,11 +146,11 @@ Op contract_topsym_op(const double *TS, const float *TS32, const double *Pt, dou
 Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax);
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
 // faces in Fourier mode order FF[mode row][fi]: fi in [0,nri) top J=fi+2,
-// [nri,2nri) bottom, [2nri,2nri+nzi) side, straight from the contraction sums
+// [nri,2nri) bottom, [2nri,2nri+nzi) side, straight from the contraction sums;
+// with X1, each value also placed in X1 (J-layout) and X2 (K-layout when
+// x2_klayout, else J-layout; may be null), the outer corners set to zero
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
-                 int numz, int numphi, int mmax, double factor, int nsplit_tb);
-// scatter Fourier-space faces FF[mode][fi] into X1 (J-layout) and X2
-// (K-layout when x2_klayout, else J-layout; may be null)
-Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int numr, int numz, int numphi);
+                 int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1 = nullptr,
+                 double *X2 = nullptr, int x2_klayout = 0);
 
 }  // namespace bh

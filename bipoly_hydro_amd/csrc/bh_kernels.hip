@@ -8,7 +8,7 @@
 //   gen_*_table   Green's-function tables (tm.F90 / sm.F90 / bm.F90)
 //   proj          azimuthal mode projections of rho (bessel.F90:176-235)
 //   contract_*    boundary contraction (bessel.F90:186-262)
-//   face_*        boundary face rebuild and placement (bessel.F90:396-488)
+//   face_modes    boundary faces in Fourier mode order, placed in the potential (bessel.F90:396-488)
 #include "bh_internal.h"
 #include "bh_ops.h"
 
@@ -1117,6 +1117,18 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     const int nf = 2 * nri + nzi;
     const int fi = bx * 128 + threadIdx.x;
     const int row = by;
+    if (a.X1 && fi == nf) {  // the outer corners of the side face are zero (bessel.F90:428, 456, 487)
+        const long plane = (long)numr * numz;
+        double *x1 = a.X1 + row * plane;
+        x1[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
+        x1[numr - 1] = 0.0;
+        if (a.X2) {
+            double *x2 = a.X2 + row * plane;
+            x2[a.x2_klayout ? (long)(numr - 1) * numz + (numz - 1) : (long)(numz - 1) * numr + (numr - 1)] = 0.0;
+            x2[a.x2_klayout ? (long)(numr - 1) * numz : numr - 1] = 0.0;
+        }
+        return;
+    }
     if (fi >= nf) return;
     const double *sc, *ss;
     long st, sst;  // per-m stride, per-split stride
@@ -1141,66 +1153,39 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     const int k = imag ? row - N / 2 : row;
     double re = 0.0, im = 0.0;
     if (k % N == 0) re = S_c(0);
-    for (int m = 1; m < mmax; ++m) {
-        const bool plus = (m - k) % N == 0, minus = (m + k) % N == 0;
-        if (!plus && !minus) continue;
+    // the m = 1..mmax-1 with m = k or m = -k (mod N), in increasing order
+    // (k = 0..N/2: m = k + jN and m = N - k + jN, m >= 1)
+    int mp = k == 0 ? N : k, mq = k == 0 ? N : N - k;
+    for (;;) {
+        const int m = min(mp, mq);
+        if (m >= mmax) break;
+        const bool plus = m == mp, minus = m == mq;
         const double c = a.besc[(long)m * N], s = a.bess[(long)m * N];
         const double C = S_c(m), S = S_s(m);
         const double r = C * c + S * s, q = C * s - S * c;
-        if (plus) { re += r; im += q; }
-        if (minus) { re += r; im -= q; }
+        if (plus) { re += r; im += q; mp += N; }
+        if (minus) { re += r; im -= q; mq += N; }
     }
-    a.F[(long)row * nf + fi] = a.factor * (double)N * (imag ? im : re);
+    const double v = a.factor * (double)N * (imag ? im : re);
+    a.F[(long)row * nf + fi] = v;
+    if (a.X1) {  // top J = fi + 2, K = numz; bottom K = 1; side J = numr, K = fi - 2 nri + 2
+        int J, K;
+        if (fi < nri) { J = fi + 2; K = numz; }
+        else if (fi < 2 * nri) { J = fi - nri + 2; K = 1; }
+        else { J = numr; K = fi - 2 * nri + 2; }
+        const long plane = (long)numr * numz;
+        a.X1[row * plane + (long)(K - 1) * numr + (J - 1)] = v;
+        if (a.X2)
+            a.X2[row * plane + (a.x2_klayout ? (long)(J - 1) * numz + (K - 1) : (long)(K - 1) * numr + (J - 1))] = v;
+    }
 }
 
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
-                 int numz, int numphi, int mmax, double factor, int nsplit_tb)
+                 int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1, double *X2, int x2_klayout)
 {
-    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor, nsplit_tb};
-    int nf = 2 * (numr - 2) + (numz - 2);
+    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor, nsplit_tb, X1, X2, x2_klayout};
+    const int nf = 2 * (numr - 2) + (numz - 2) + (X1 ? 1 : 0);  // + the corner thread
     return Op::make(OP_FACE_MODES, 0, a, (nf + 127) / 128, numphi, 1);
-}
-
-// Fourier faces into potential planes: X1 in J-layout, X2 (when given) in
-// K-layout or J-layout.  The outer corners of the side face are zero
-// (bessel.F90:428, 456, 487).
-__device__ __forceinline__ void face_scatter_tile(const ScatterArgs &a, int bx, int by, int)
-{
-    const int numr = a.numr, numz = a.numz;
-    const int nri = numr - 2, nzi = numz - 2;
-    const int nf = 2 * nri + nzi;
-    const int L = by;
-    const long plane = (long)numr * numz;
-    double *x1 = a.X1 + L * plane;
-    double *x2 = a.X2 ? a.X2 + L * plane : nullptr;
-    const bool k2 = a.x2_klayout != 0;
-    // position of (J, K) (1-based) in a plane of the second target
-    auto at2 = [&](int J, int K) { return k2 ? (long)(J - 1) * numz + (K - 1) : (long)(K - 1) * numr + (J - 1); };
-    const double *ff = a.FF + (long)L * nf;
-    int t = bx * 128 + threadIdx.x;
-    int J = 0, K = 0;
-    double v = 0.0;
-    if (t < nri) { J = t + 2; K = numz; v = ff[t]; }                      // top
-    else if (t < 2 * nri) { J = t - nri + 2; K = 1; v = ff[t]; }          // bottom
-    else if (t < nf) { J = numr; K = t - 2 * nri + 2; v = ff[t]; }        // side
-    if (J) {
-        x1[(long)(K - 1) * numr + (J - 1)] = v;
-        if (x2) x2[at2(J, K)] = v;
-    } else if (t == nf) {  // corners
-        x1[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
-        x1[numr - 1] = 0.0;
-        if (x2) {
-            x2[at2(numr, numz)] = 0.0;
-            x2[at2(numr, 1)] = 0.0;
-        }
-    }
-}
-
-Op face_scatter_op(const double *FF, double *X1, double *X2, int x2_klayout, int numr, int numz, int numphi)
-{
-    ScatterArgs a{FF, X1, X2, numr, numz, x2_klayout};
-    int nf = 2 * (numr - 2) + (numz - 2) + 1;
-    return Op::make(OP_SCATTER, 0, a, (nf + 127) / 128, numphi, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -1226,7 +1211,6 @@ BH_KERNEL_PAIR(contract_tb64, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_tb32, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_reduce, ContractArgs, 256)
 BH_KERNEL_PAIR(face_modes, RebuildArgs, 128)
-BH_KERNEL_PAIR(face_scatter, ScatterArgs, 128)
 #undef BH_KERNEL_PAIR
 
 namespace {
@@ -1259,7 +1243,6 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         return true;
         BH_REF(OP_TOPSYM_REDUCE, topsym_reduce, 256)
         BH_REF(OP_FACE_MODES, face_modes, 128)
-        BH_REF(OP_SCATTER, face_scatter, 128)
     case OP_SWEEP:
         *lds = sweep_lds(variant);
         switch (variant) {

@@ -98,13 +98,13 @@ enum OpKind {
     OP_CONTRACT_TOPSYM,
     OP_TOPSYM_REDUCE,
     OP_FACE_MODES,
-    OP_SCATTER,
     OP_RESIDUAL,      // bh_mixed.hip
     OP_UPDATE,
     OP_TRIDIAG,
     OP_DENSITY,       // bh_density.hip
     OP_DFT,           // bh_dft.hip
     OP_SPECTRAL,      // bh_spectral.hip
+    OP_RESIDUAL_FACE, // bh_mixed.hip
     OP_NKINDS
 };
 

@@ -179,11 +179,12 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
     out->push_back(proj_op(rho, p->ops->Pf, p->proj, p->g.numr * p->g.numz, p->g.numphi, p->g.mmax));
 }
 
-// Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF.  The
-// side contraction (L2-bound) runs on the side stream, after whatever is
-// queued there, beside the top/bottom contraction (HBM-bound); the faces
+// Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF, also
+// placed in the faces of X1 (J-layout) and X2 (K-layout when x2_klayout,
+// else J-layout).  The side contraction runs on the side stream, after
+// whatever is queued there, beside the top/bottom contraction; the faces
 // wait for both (and for the side stream's earlier work).
-void boundary_ops(const bh_plan *p, std::vector<Op> *out)
+void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
@@ -197,7 +198,7 @@ void boundary_ops(const bh_plan *p, std::vector<Op> *out)
     out->push_back(Op::mark(6));
     out->push_back(Op::mark(MARK_JOIN));
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax,
-                                 -g.dr * g.dz * g.dphi, 1));
+                                 -g.dr * g.dz * g.dphi, 1, X1, X2, x2_klayout));
 }
 
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
@@ -216,9 +217,17 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     //    CUs, while the FP64 projections and the boundary (which do not need
     //    it) run on the main stream
     out->push_back(Op::mark(MARK_FORK));
+    // a cold start at N = 512: the DFT also writes the residual's source
+    // -4 pi rho-hat on the interior, so that after the faces only the points
+    // next to a face need the stencil (residual_face_op)
+    const bool cold_split = tstep <= 1 && o.dfwd32 && p->use_dft && N == DFT_MAX_N;
     if (o.dfwd32 && p->use_dft) {
         DftArgs d{d_rho, M, p->RH32, M, o.dfwd32, N, (int)M, numr, numz};
         d.max_blocks = side_blocks(p);
+        if (cold_split) {
+            d.rout = p->r32;
+            d.rscale = -g.four_pi;
+        }
         out->push_back(dft_op(d, DFT_FWD_F32));
         out->push_back(Op::mark(MARK_SIDE_END));
         proj64_ops(p, d_rho, out);
@@ -238,12 +247,11 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     }
     out->push_back(Op::mark(1));
     // 2. boundary faces (FP64), into both potential buffers; the starting
-    //    potential is zero (cold) or the one kept from the last solve (warm)
-    boundary_ops(p, out);  // joins the side stream
+    //    potential is zero (cold) or the one kept from the last solve (warm).
+    //    A cold start needs no zeroing: the residuals below read only the
+    //    faces of Xa, and Xa's interior is rewritten by the final update.
     double *Xa = p->XJ, *Xb = p->XK;
-    // a cold start needs no zeroing: the residuals below read only the faces
-    // of Xa, and Xa's interior is rewritten by the final update
-    out->push_back(face_scatter_op(p->FF, Xa, Xb, 0, numr, numz, N));
+    boundary_ops(p, Xa, Xb, 0, out);  // joins the side stream
     out->push_back(Op::mark(2));
     // 3. r0 = g - A x0
     ResidualArgs ra{};
@@ -252,7 +260,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.rh = p->RH32;
     ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
     ra.cold = tstep <= 1;
-    out->push_back(residual_op(ra, N));
+    out->push_back(cold_split ? residual_face_op(ra, N) : residual_op(ra, N));
     // folded sine transforms (spectral slots ordered by spec_mode):
     // forward RT[L] (nri x nzi) = r[L]^T S, inverse dst[L] interior
     // (nzi x nri) = S in[L]^T for nb consecutive modes of in and dst
@@ -352,12 +360,11 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     out->push_back(Op::mark(MARK_SIDE_END));
     proj64_ops(p, d_rho, out);
     out->push_back(Op::mark(1));
-    // 2. boundary potential -> Fourier-space faces
-    boundary_ops(p, out);
-    // 3. starting potential: zero (cold, potential_solver.F90:88-93) or the
-    //    previous solution kept in Fourier space (warm)
+    // 2. starting potential: zero (cold, potential_solver.F90:88-93) or the
+    //    previous solution kept in Fourier space (warm); then the boundary
+    //    potential -> Fourier-space faces, placed in both layouts
     out->push_back(fill_op(p->XJ, warm ? 0 : (size_t)M * N, (size_t)M * N, 0.0));
-    out->push_back(face_scatter_op(p->FF, p->XJ, p->XK, 1, numr, numz, N));
+    boundary_ops(p, p->XJ, p->XK, 1, out);
     out->push_back(Op::mark(2));
     // 4. Peaceman-Rachford iterations (helmadi.F90:195-355)
     const std::vector<double> &dt = o.dt[w];
