@@ -33,9 +33,10 @@ Op dft_op(const DftArgs &a, int kind);
 size_t dft_lds_bytes(int kind, int N);
 // Matrix operands in fragment order: forward [row tile][k step][lane] with
 // cosine rows k = 0..N/2-1 then sine rows k = 1..N/2-1; inverse P rows then
-// Q rows, n = 0..N/2-1.  At N = 512: the operand of size N/4 followed by the
-// radix-4 twiddles (cos, sin)(2 pi k r / N), r = 0..3 (forward k = 0..N/4,
-// inverse k = 0..N/8).
+// Q rows, n = 0..N/2-1.  At N = 512: the forward operand of size N/4 followed
+// by the radix-4 twiddles (cos, sin)(2 pi k r / N), r = 0..3, k = 0..N/4;
+// the inverse operand of size N/8 followed by the radix-8 twiddles, r =
+// 0..7, k = 0..N/16.
 void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *f64);
 void dft_inverse_fragments(int N, std::vector<double> *f64);
 

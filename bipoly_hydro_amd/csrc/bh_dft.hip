@@ -22,10 +22,11 @@
 // belongs to wave t % 8; the inverse pairs the P and Q tiles of the same n
 // in one wave so that x_n and x_{N-n} are formed in registers.
 //
-// At N = 512 (the production grids) one radix-4 step comes first (forward:
-// in the input index, inverse: in the output index; the derivations are at
-// the kernels), so the products run at size N/4 on four virtual columns per
-// column: a quarter of the matrix work of the direct form.
+// At N = 512 (the production grids) one radix step comes first (forward:
+// radix 4 in the input index; inverse: radix 8 in the output index; the
+// derivations are at the kernels), so the products run at size N/4 (N/8)
+// on four (eight) virtual columns per column: a quarter (an eighth) of the
+// matrix work of the direct form.
 #include "bh_dft.h"
 #include "bh_ops.h"
 
@@ -397,23 +398,11 @@ constexpr int PN = DFT_MAX_N;  // the radix-4 kernels' transform length
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Inverse at N = 512 by one radix-4 step, decimating the output index:
-//   x_{4m+r} = (1/N) sum_{k<N} X_k w^{k(4m+r)},   w = e^{2 pi i/N}
-//            = (1/4) IDFT_n(Y^(r))_m,   n = N/4,
-//   Y^(r)_k  = w^{kr} sum_{q<4} i^{qr} X_{k+qn}     (k < n; X_{N-k} = conj X_k)
-// with X_k = a_k + i s_k (s_k the mode rows N/2+1..N-1).  Each Y^(r) is
-// Hermitian, so output quarter r is the size-n real inverse of (Re, Im)
-// of Y^(r)_0..n/2: the same P/Q products as above on four virtual columns
-// per column, at a quarter of the matrix work of the direct size-N form.
-// A workgroup takes a panel of R4CB columns: it forms the four quarter
-// spectra (the X + xadd update and its store fused in) straight from HBM
-// into LDS, runs the products (wave w: n-tile w % 4 of P and Q, quarters
-// 2 (w / 4) and 2 (w / 4) + 1) and writes the rows 4m + r.
+// Radix steps at N = 512.  Shared by the forward (radix 4) and the inverse
+// (radix 8): panels of R4CB columns, and the persistent panel loop below.
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int R4N = PN / 4, R4KG = 4, R4KPp = 16 * R4KG + 2, R4CB = 16, R4VC = 4 * R4CB;
-constexpr size_t R4_LDS = (size_t)2 * R4VC * R4KPp * 8 + 2 * R4VC * 8;
-constexpr int R4_FRAGS = 8 * R4KG * 64 * 4;  // doubles of the size-n P and Q fragments
+constexpr int R4N = PN / 4, R4KG = 4, R4CB = 16, R4VC = 4 * R4CB;
 
 // Persistent loop over panels t0, t0 + st, ... < t1 (XCD-chunked as in
 // BH_TILE_LOOP): the loads of panel t + st are issued into registers
@@ -422,7 +411,7 @@ constexpr int R4_FRAGS = 8 * R4KG * 64 * 4;  // doubles of the size-n P and Q fr
 // writes the panel's operands to LDS; work(args, bx) runs the products and
 // writes the outputs.
 template <class Regs, class Resolve, class Load, class Form, class Work>
-__device__ __forceinline__ void r4_loop(int total, Resolve resolve, Load load, Form form, Work work)
+__device__ __forceinline__ void panel_loop(int total, Resolve resolve, Load load, Form form, Work work)
 {
     const int n_ = gridDim.x, b_ = blockIdx.x;
     int t0 = b_, t1 = total, st = n_;
@@ -456,154 +445,200 @@ __device__ __forceinline__ void r4_loop(int total, Resolve resolve, Load load, F
     }
 }
 
-// 64 x 16 items per panel, two per thread: item (kk, c) is the butterfly of
-// k = kk for kk = 1..n/2-1, and for kk = 0 both k = 0 and k = n/2 (8 input
-// rows between them, like one regular item)
-constexpr int R4_IPT = 2;
+}  // namespace
 
-struct Inv4Regs {
-    double av[R4_IPT][4], sv[R4_IPT][4];
-    float fa[R4_IPT][4], fs[R4_IPT][4];
+// ---------------------------------------------------------------------------
+// Inverse at N = 512 by one radix-8 step, decimating the output index:
+//   x_{8m+r} = (1/N) sum_{k<N} X_k w^{k(8m+r)},   w = e^{2 pi i/N}
+//            = IDFT_n(Y^(r))_m,   n = N/8,
+//   Y^(r)_k  = (1/8) w^{kr} sum_{q<8} e^{2 pi i qr/8} X_{k+qn}   (X_{N-k} = conj X_k)
+// with X_k = a_k + i s_k (s_k the mode rows N/2+1..N-1).  Each Y^(r) is
+// Hermitian, so output residue r is the size-n real inverse of (Re, Im) of
+// Y^(r)_0..n/2: the P/Q products above at size 64 on eight virtual columns
+// per column, an eighth of the matrix work of the direct size-N form.  A
+// workgroup takes a panel of R4CB columns; each thread forms the 8-point
+// sum of one item (k = 1..n/2-1, or k = 0 together with k = n/2: 16 input
+// rows either way; the X + xadd update and its store fused in) into LDS,
+// then the products run (wave w: n-tile w % 2 of P and Q, residues
+// 2 (w / 2) and 2 (w / 2) + 1) and the rows 8m + r are written.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int R8N = PN / 8, R8KG = 2, R8KPp = 16 * R8KG + 2, R8VC = 8 * R4CB;
+constexpr size_t R8_LDS = (size_t)2 * R8VC * R8KPp * 8 + 2 * R8VC * 8;
+constexpr int R8_FRAGS = 4 * R8KG * 64 * 4;  // doubles of the size-n P and Q fragments
+
+struct Inv8Regs {
+    double av[8], sv[8];
+    float fa[8], fs[8];
 };
 
-// mode rows of X_k, X_{k+n}, X_{2n-k}, X_{n-k} (q = 0..3)
-__device__ __forceinline__ int inv4_row(int k, int q)
-{
-    return q == 0 ? k : q == 1 ? R4N + k : q == 2 ? 2 * R4N - k : R4N - k;
-}
-// the input row held in slot (q, s) of item kk (s = 0: av, 1: sv), or -1
-// for a zero (s_0 = s_{N/2} = 0).  kk = 0: av = a_0, a_n, a_2n, a_{n/2};
-// sv = s_n, s_{n/2}, a_{3n/2}, s_{3n/2}.
-__device__ __forceinline__ int inv4_slot_row(int kk, int q, int s)
+// input row of slot (q, s) of item kk (s = 0: av, 1: sv).  kk = 1..n/2-1:
+// X_{kk+qn} is a_d + i s_d for d = kk + qn <= N/2 (q < 4) and the conjugate
+// of X_{N-d} (q >= 4).  kk = 0: av = a_0, a_n, a_2n, a_3n, a_{n/2},
+// a_{3n/2}, a_{5n/2}, a_{7n/2}; sv = a_{N/2}, s_n, s_2n, s_3n, s_{n/2},
+// s_{3n/2}, s_{5n/2}, s_{7n/2}.
+__device__ __forceinline__ int inv8_slot_row(int kk, int q, int s)
 {
     if (kk == 0) {
-        const int r0[4] = {0, R4N, 2 * R4N, R4N / 2};
-        const int r1[4] = {PN / 2 + R4N, PN / 2 + R4N / 2, 3 * R4N / 2, PN / 2 + 3 * R4N / 2};
-        return s == 0 ? r0[q] : r1[q];
+        const int base = q < 4 ? q * R8N : (q - 4) * R8N + R8N / 2;
+        if (s == 0) return base;
+        return q == 0 ? PN / 2 : PN / 2 + base;
     }
-    const int row = inv4_row(kk, q);
-    return s == 0 ? row : PN / 2 + row;  // kk = 1..n/2-1: every s row is 1..N/2-1
+    const int d = q < 4 ? kk + q * R8N : PN - (kk + q * R8N);
+    return s == 0 ? d : PN / 2 + d;
 }
 
-__device__ __forceinline__ void inv4_load(const DftArgs &a, int bx, Inv4Regs &R)
+__device__ __forceinline__ void inv8_load(const DftArgs &a, int bx, Inv8Regs &R)
 {
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));  // lane-dependent offsets recomputed per panel
     long ldx = a.ldx;
     asm volatile("" : "+s"(ldx));  // (not hoisted into registers held across panels)
-    const double *x = a.x;
+    const int kk = tid / R4CB, c = tid - kk * R4CB, gc = min(bx * R4CB + c, a.M - 1);
 #pragma unroll
-    for (int i = 0; i < R4_IPT; ++i) {
-        const int e = tid + 512 * i, kk = e / R4CB, c = e - kk * R4CB, gc = min(bx * R4CB + c, a.M - 1);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const long o0 = (long)inv4_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv4_slot_row(kk, q, 1) * ldx + gc;
-            R.av[i][q] = x[o0];
-            R.sv[i][q] = x[o1];
-            if (a.xadd) {
-                R.fa[i][q] = a.xadd[o0];
-                R.fs[i][q] = a.xadd[o1];
-            }
+    for (int q = 0; q < 8; ++q) {
+        const long o0 = (long)inv8_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv8_slot_row(kk, q, 1) * ldx + gc;
+        R.av[q] = a.x[o0];
+        R.sv[q] = a.x[o1];
+        if (a.xadd) {
+            R.fa[q] = a.xadd[o0];
+            R.fs[q] = a.xadd[o1];
         }
     }
 }
 
-// quarter spectra Y^(r)_k, k = 0..n/2, times 1/4, into LDS; with xadd, x +
-// xadd is also stored (every input row belongs to exactly one item slot)
-__device__ __forceinline__ void inv4_form(const DftArgs &a, int bx, const Inv4Regs &R, unsigned char *lds)
+// z_r = sum_q e^{2 pi i qr/8} x_q, r = 0..7 (even/odd split, two 4-point sums)
+__device__ __forceinline__ void dft8_pos(const double *xr, const double *xi, double *zr, double *zi)
 {
-    double *Ca = (double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
+    double er[4], ei[4], orr[4], oi[4];
+    auto four = [](double a0r, double a0i, double a1r, double a1i, double a2r, double a2i, double a3r, double a3i,
+                   double *yr, double *yi) {
+        const double pr = a0r + a2r, pi = a0i + a2i, mr = a0r - a2r, mi = a0i - a2i;
+        const double ur = a1r + a3r, ui = a1i + a3i, dr = a1r - a3r, di = a1i - a3i;
+        yr[0] = pr + ur; yi[0] = pi + ui;
+        yr[1] = mr - di; yi[1] = mi + dr;
+        yr[2] = pr - ur; yi[2] = pi - ui;
+        yr[3] = mr + di; yi[3] = mi - dr;
+    };
+    four(xr[0], xi[0], xr[2], xi[2], xr[4], xi[4], xr[6], xi[6], er, ei);
+    four(xr[1], xi[1], xr[3], xi[3], xr[5], xi[5], xr[7], xi[7], orr, oi);
+    const double h = 0.70710678118654752440;  // cos(pi/4)
+    const double wr[4] = {1.0, h, 0.0, -h}, wi[4] = {0.0, h, 1.0, h};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const double tr = wr[r] * orr[r] - wi[r] * oi[r], ti = wr[r] * oi[r] + wi[r] * orr[r];
+        zr[r] = er[r] + tr;
+        zi[r] = ei[r] + ti;
+        zr[r + 4] = er[r] - tr;
+        zi[r + 4] = ei[r] - ti;
+    }
+}
+
+// the eight spectra Y^(r)_k into LDS; with xadd, x + xadd is also stored
+// (every input row belongs to exactly one item slot)
+__device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Regs &R, unsigned char *lds)
+{
+    double *Ca = (double *)lds, *Sb = Ca + R8VC * R8KPp, *E0 = Sb + R8VC * R8KPp, *EH = E0 + R8VC;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     long ldx = a.ldx;
     asm volatile("" : "+s"(ldx));
-    const double *tw = (const double *)a.A + R4_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N)
-    // Y^(r) of one k from X_k, X_{k+n}, conj X_{2n-k}, conj X_{n-k}
-    auto quarters = [&](int k, double x0r, double x0i, double x1r, double x1i, double x2r, double x2i, double x3r,
-                        double x3i, double *yr, double *yi) {
-        const double pr = x0r + x2r, pi = x0i + x2i, mr = x0r - x2r, mi = x0i - x2i;  // X0 +- X2
-        const double ur = x1r + x3r, ui = x1i + x3i, dr = x1r - x3r, di = x1i - x3i;  // X1 +- X3
-        // r = 0: (X0+X2) + (X1+X3); 1: (X0-X2) + i (X1-X3); 2: (X0+X2) - (X1+X3); 3: (X0-X2) - i (X1-X3)
-        const double zr[4] = {pr + ur, mr - di, pr - ur, mr + di};
-        const double zi[4] = {pi + ui, mi + dr, pi - ui, mi - dr};
+    const double *tw = (const double *)a.A + R8_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n/2
+    const int kk = tid / R4CB, c = tid - kk * R4CB, gc = bx * R4CB + c;
+    double av[8], sv[8];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const double wc = tw[(k * 4 + r) * 2], ws = tw[(k * 4 + r) * 2 + 1];
-            yr[r] = 0.25 * (wc * zr[r] - ws * zi[r]);
-            yi[r] = 0.25 * (wc * zi[r] + ws * zr[r]);
+    for (int q = 0; q < 8; ++q) {
+        av[q] = R.av[q];
+        sv[q] = R.sv[q];
+    }
+    if (a.xadd) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            av[q] += (double)R.fa[q];
+            sv[q] += (double)R.fs[q];
+            if (gc < a.M) {
+                a.xstore[(long)inv8_slot_row(kk, q, 0) * ldx + gc] = av[q];
+                a.xstore[(long)inv8_slot_row(kk, q, 1) * ldx + gc] = sv[q];
+            }
+        }
+    }
+    auto eighths = [&](int k, const double *xr, const double *xi, double *yr, double *yi) {
+        double zr[8], zi[8];
+        dft8_pos(xr, xi, zr, zi);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+            const double wc = tw[(k * 8 + r) * 2], ws = tw[(k * 8 + r) * 2 + 1];
+            yr[r] = 0.125 * (wc * zr[r] - ws * zi[r]);
+            yi[r] = 0.125 * (wc * zi[r] + ws * zr[r]);
         }
     };
+    double xr[8], xi[8], yr[8], yi[8];
+    if (kk == 0) {
+        // k = 0: X_0 = a_0, X_qn = a_qn + i s_qn (q = 1..3), X_{N/2} = a_{N/2}, then conjugates
+        xr[0] = av[0]; xi[0] = 0.0;
+        xr[4] = sv[0]; xi[4] = 0.0;
 #pragma unroll
-    for (int i = 0; i < R4_IPT; ++i) {
-        const int e = tid + 512 * i, kk = e / R4CB, c = e - kk * R4CB, gc = bx * R4CB + c;
-        double av[4], sv[4];
+        for (int q = 1; q < 4; ++q) {
+            xr[q] = av[q]; xi[q] = sv[q];
+            xr[8 - q] = av[q]; xi[8 - q] = -sv[q];
+        }
+        eighths(0, xr, xi, yr, yi);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) E0[r * R4CB + c] = yr[r];
+        // k = n/2: X_{n/2 + qn} (q = 0..3) direct, q = 4..7 the conjugates of q = 3..0
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            av[q] = R.av[i][q];
-            sv[q] = R.sv[i][q];
+            xr[q] = av[4 + q]; xi[q] = sv[4 + q];
+            xr[7 - q] = av[4 + q]; xi[7 - q] = -sv[4 + q];
         }
-        if (a.xadd) {
+        eighths(R8N / 2, xr, xi, yr, yi);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                av[q] += (double)R.fa[i][q];
-                sv[q] += (double)R.fs[i][q];
-                if (gc < a.M) {
-                    a.xstore[(long)inv4_slot_row(kk, q, 0) * ldx + gc] = av[q];
-                    a.xstore[(long)inv4_slot_row(kk, q, 1) * ldx + gc] = sv[q];
-                }
-            }
+        for (int r = 0; r < 8; ++r) {
+            const int vc = r * R4CB + c;
+            EH[vc] = yr[r];
+            Ca[vc * R8KPp + R8N / 2 - 1] = 0.0;  // K padding
+            Sb[vc * R8KPp + R8N / 2 - 1] = 0.0;
         }
-        double yr[4], yi[4];
-        if (kk == 0) {
-            // k = 0: X_0 = a_0, X_n = a_n + i s_n, X_2n = a_2n, conj X_n
-            quarters(0, av[0], 0.0, av[1], sv[0], av[2], 0.0, av[1], -sv[0], yr, yi);
+    } else {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) E0[r * R4CB + c] = yr[r];
-            // k = n/2: X_{n/2}, X_{3n/2}, conj X_{3n/2}, conj X_{n/2}
-            quarters(R4N / 2, av[3], sv[1], sv[2], sv[3], sv[2], -sv[3], av[3], -sv[1], yr, yi);
+        for (int q = 0; q < 8; ++q) {
+            xr[q] = av[q];
+            xi[q] = q < 4 ? sv[q] : -sv[q];
+        }
+        eighths(kk, xr, xi, yr, yi);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int vc = r * R4CB + c;
-                EH[vc] = yr[r];
-                Ca[vc * R4KPp + R4N / 2 - 1] = 0.0;  // K padding (k = 64 of the 16 R4KG slots)
-                Sb[vc * R4KPp + R4N / 2 - 1] = 0.0;
-            }
-        } else {
-            // X_k, X_{k+n}, X_{k+2n} = conj X_{2n-k}, X_{k+3n} = conj X_{n-k}
-            quarters(kk, av[0], sv[0], av[1], sv[1], av[2], -sv[2], av[3], -sv[3], yr, yi);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int vc = r * R4CB + c;
-                Ca[vc * R4KPp + kk - 1] = yr[r];
-                Sb[vc * R4KPp + kk - 1] = yi[r];
-            }
+        for (int r = 0; r < 8; ++r) {
+            const int vc = r * R4CB + c;
+            Ca[vc * R8KPp + kk - 1] = yr[r];
+            Sb[vc * R8KPp + kk - 1] = yi[r];
         }
     }
 }
 
-// size-n products (wave w: P and Q rows m = 16 (w % 4).., quarters t0 and
-// t0 + 1), then the rows 4m + r, 4(n - m) + r and the middle rows
-__device__ __forceinline__ void inv4_work(const DftArgs &a, int bx, const unsigned char *lds)
+// size-n products (wave w: P and Q rows m = 16 (w % 2).., virtual column
+// tiles 2 (w / 2) and 2 (w / 2) + 1, i.e. residues r), the rows 8m + r and
+// 8(n - m) + r, and the middle rows 8 (n/2) + r
+__device__ __forceinline__ void inv8_work(const DftArgs &a, int bx, const unsigned char *lds)
 {
     typedef Mma<double>::acc_t acc_t;
     typedef double v4 __attribute__((ext_vector_type(4)));
-    const double *Ca = (const double *)lds, *Sb = Ca + R4VC * R4KPp, *E0 = Sb + R4VC * R4KPp, *EH = E0 + R4VC;
+    const double *Ca = (const double *)lds, *Sb = Ca + R8VC * R8KPp, *E0 = Sb + R8VC * R8KPp, *EH = E0 + R8VC;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
     const int lane = tid & 63, wave = tid >> 6;
     const int col0 = bx * R4CB;
     const v4 *Af = (const v4 *)a.A;
-    const int mt = wave & 3, t0 = 2 * (wave >> 2), jl = 4 * (lane >> 4);
+    const int mt = wave & 1, t0 = 2 * (wave >> 1), jl = 4 * (lane >> 4);
     acc_t accP[2], accQ[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) accP[i] = accQ[i] = (acc_t){0.0, 0.0, 0.0, 0.0};
-#pragma unroll 1
-    for (int grp = 0; grp < R4KG; ++grp) {
-        const v4 fp = Af[((long)mt * R4KG + grp) * 64 + lane];
-        const v4 fq = Af[((long)(4 + mt) * R4KG + grp) * 64 + lane];
+#pragma unroll
+    for (int grp = 0; grp < R8KG; ++grp) {
+        const v4 fp = Af[((long)mt * R8KG + grp) * 64 + lane];
+        const v4 fq = Af[((long)(2 + mt) * R8KG + grp) * 64 + lane];
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            const int off = (16 * (t0 + i) + (lane & 15)) * R4KPp + 16 * grp + jl;
+            const int off = (16 * (t0 + i) + (lane & 15)) * R8KPp + 16 * grp + jl;
             const v4 bc = *(const v4 *)(Ca + off), bs = *(const v4 *)(Sb + off);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
@@ -612,7 +647,7 @@ __device__ __forceinline__ void inv4_work(const DftArgs &a, int bx, const unsign
             }
         }
     }
-    const double invn = 1.0 / (double)R4N;
+    const double invn = 1.0 / (double)R8N;
     double *out = (double *)a.out;
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
@@ -624,50 +659,51 @@ __device__ __forceinline__ void inv4_work(const DftArgs &a, int bx, const unsign
             if (gc >= a.M) continue;
             const double p = (e0 + ((m & 1) ? -eh : eh)) + accP[i][v];
             const double q = accQ[i][v];
-            out[(long)(4 * m + r) * a.ldo + gc] = p - q;
-            if (m > 0) out[(long)(4 * (R4N - m) + r) * a.ldo + gc] = p + q;
+            out[(long)(8 * m + r) * a.ldo + gc] = p - q;
+            if (m > 0) out[(long)(8 * (R8N - m) + r) * a.ldo + gc] = p + q;
         }
     }
-    // row 4 (n/2) + r: e0 + eh + sum_k (2/n) (-1)^k a'_k, eight lanes per column
+    // row 8 (n/2) + r: e0 + eh + sum_k (2/n) (-1)^k a'_k, four lanes per column
     {
-        const int vc = tid >> 3, j = tid & 7;
+        const int vc = tid >> 2, j = tid & 3;
         double sm = 0.0;
-        for (int k = 1 + j; k < R4N / 2; k += 8) sm += (k & 1) ? -Ca[vc * R4KPp + k - 1] : Ca[vc * R4KPp + k - 1];
+        for (int k = 1 + j; k < R8N / 2; k += 4) sm += (k & 1) ? -Ca[vc * R8KPp + k - 1] : Ca[vc * R8KPp + k - 1];
         sm += __shfl_xor(sm, 1, 64);
         sm += __shfl_xor(sm, 2, 64);
-        sm += __shfl_xor(sm, 4, 64);
         const int r = vc / R4CB, c = vc - r * R4CB, gc = col0 + c;
         if (j == 0 && gc < a.M)
-            out[(long)(4 * (R4N / 2) + r) * a.ldo + gc] = (E0[vc] + EH[vc]) * invn + 2.0 * invn * sm;
+            out[(long)(8 * (R8N / 2) + r) * a.ldo + gc] = (E0[vc] + EH[vc]) * invn + 2.0 * invn * sm;
     }
 }
 
 template <class Resolve>
-__device__ __forceinline__ void inv4_loop(int total, Resolve resolve, unsigned char *lds)
+__device__ __forceinline__ void inv8_loop(int total, Resolve resolve, unsigned char *lds)
 {
-    r4_loop<Inv4Regs>(
-        total, resolve, [&](const DftArgs &a, int bx, Inv4Regs &R) { inv4_load(a, bx, R); },
-        [&](const DftArgs &a, int bx, const Inv4Regs &R) { inv4_form(a, bx, R, lds); },
-        [&](const DftArgs &a, int bx) { inv4_work(a, bx, lds); });
+    panel_loop<Inv8Regs>(
+        total, resolve, [&](const DftArgs &a, int bx, Inv8Regs &R) { inv8_load(a, bx, R); },
+        [&](const DftArgs &a, int bx, const Inv8Regs &R) { inv8_form(a, bx, R, lds); },
+        [&](const DftArgs &a, int bx) { inv8_work(a, bx, lds); });
 }
 
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv4_kernel(DftArgs a, int nx, int ny, int total)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv8_kernel(DftArgs a, int nx,
+                                                                                             int ny, int total)
 {
-    extern __shared__ __align__(16) unsigned char inv4_lds[];
-    inv4_loop(total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, inv4_lds);
+    extern __shared__ __align__(16) unsigned char inv8_lds[];
+    inv8_loop(total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, inv8_lds);
 }
 
-__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv4_grouped(const DftArgs *__restrict__ args, GroupSched gs)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv8_grouped(
+    const DftArgs *__restrict__ args, GroupSched gs)
 {
-    extern __shared__ __align__(16) unsigned char inv4_lds[];
-    inv4_loop(
+    extern __shared__ __align__(16) unsigned char inv8_lds[];
+    inv8_loop(
         gs.total,
         [&](int t, DftArgs &o, int &bx) {
             const int g = bh_find_group(gs.grp, gs.ngroups, t);
             o = args[g];
             bx = t - gs.grp[g].tile0;
         },
-        inv4_lds);
+        inv8_lds);
 }
 }  // namespace
 
@@ -925,8 +961,8 @@ static const void *r4_fn(int kind, bool grouped, int *lds)
 {
     switch (kind) {
     case DFT_INV_F64:
-        *lds = (int)R4_LDS;
-        return grouped ? (const void *)&dft_inv4_grouped : (const void *)&dft_inv4_kernel;
+        *lds = (int)R8_LDS;
+        return grouped ? (const void *)&dft_inv8_grouped : (const void *)&dft_inv8_kernel;
     case DFT_FWD_F32:
         *lds = (int)R4Fwd<float>::bytes;
         return grouped ? (const void *)&dft_fwd4_grouped<float> : (const void *)&dft_fwd4_kernel<float>;
@@ -1046,13 +1082,12 @@ void dft_forward_fragments(int N, std::vector<float> *f32, std::vector<double> *
 void dft_inverse_fragments(int N, std::vector<double> *f64)
 {
     if (N == DFT_MAX_N) {
-        // radix-4 form: the size-N/4 operand, then the twiddles w^{kr}
-        // (cos, sin) for k = 0..N/8, r = 0..3
-        std::vector<double> tw;
-        dft_inverse_fragments(N / 4, f64);
+        // radix-8 form: the size-N/8 operand, then the twiddles w^{kr}
+        // (cos, sin) for k = 0..N/16, r = 0..7
+        dft_inverse_fragments(N / 8, f64);
         const double pi = std::acos(-1.0);
-        for (int k = 0; k <= N / 8; ++k)
-            for (int r = 0; r < 4; ++r) {
+        for (int k = 0; k <= N / 16; ++k)
+            for (int r = 0; r < 8; ++r) {
                 const double ang = 2.0 * pi * (double)((long)k * r % N) / (double)N;
                 f64->push_back(std::cos(ang));
                 f64->push_back(std::sin(ang));
