@@ -70,20 +70,23 @@ __device__ __forceinline__ BH_GLOBAL T *gst(void *p)
 // Tile t takes plane t/2 + (t%2) N/2: planes m and m + N/2 (the cosine and
 // sine rows of azimuthal mode m) run side by side on one XCD, and share M_r
 // and the pivots, stored per mode number mi = L or L - N/2.
-template <int NZ>
+template <int NZ, bool MR>
 __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int tile)
 {
     typedef Shape<NZ> S;
     constexpr int P = S::P;
     const int hn = a.nphi / 2;
     const int L = (tile >> 1) + (tile & 1) * hn, mi = L <= hn ? L : L - hn;
-    // numr passes through an empty asm so that the compiler treats the sizes
-    // as new in every tile of the persistent loop: otherwise it hoists
-    // hundreds of tile-invariant address offsets out of the loop and spills
+    // numr and the thread index pass through an empty asm so that the
+    // compiler treats them as new in every tile of the persistent loop:
+    // otherwise it hoists hundreds of tile-invariant address offsets out of
+    // the loop and spills
     int numr = a.numr;
     asm volatile("" : "+s"(numr));
     const int nri = numr - 2;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const int lane = tid & 63, wave = tid >> 6;
     const int t = lane & 15, q = lane >> 4;
     const int j0 = wave * 32;
     const bool active = j0 < nri;
@@ -176,7 +179,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int c = 0; c < CTW; ++c) wacc[i][c] = (f4){0.f, 0.f, 0.f, 0.f};
-    if (a.mr) {
+    if (MR) {
         if (fil) {
             const int nkc = nri / 32;
             // fragment (row tile 4 rg + i, chunk kc, part): 64 lanes x 8 bf16
@@ -246,11 +249,12 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     // ---- 3. (B_r(L) + lambda_m') D = C along j for every column m', as the
     //         separate Thomas kernel's recurrences (forward y_j = (c_j -
     //         a_j y_{j-1}) ib_j, back x_j = y_j - g_{j+1} x_{j+1}), split into
-    //         nri/64 segments of 64 rows: a thread composes its segment's
-    //         affine map, the maps are chained through LDS, and the segment is
-    //         swept again from its true entry value.  The back sweep's
-    //         g_{j+1} = alphar(j + 2) ib_j comes from the pivots already in
-    //         registers, so only ib is read, all 64 loads at once.
+    //         nri/64 segments of 64 rows: a thread holds its segment of one
+    //         column in registers, composes the segment's affine map, the maps
+    //         are chained through LDS, and the segment is swept again from its
+    //         true entry value.  The back sweep's g_{j+1} = alphar(j + 2) ib_j
+    //         comes from the pivots already in registers, so only ib is read,
+    //         all 64 loads at once.
     {
         const int nseg = nri / 64;
         const bool tri = tid < nseg * NZ;  // whole waves: nzi is a multiple of 32
@@ -258,30 +262,33 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         const long fb = (long)mi * nri * NZ + (long)jlo * NZ + m;
         float *col = buf + jlo * P + m;
         const float *al = bl + jlo;
-        float fv[64];
+        float fv[64], cv[64];
         if (tri) {
             const BH_GLOBAL float *ib = gld<float>(a.ibet) + fb;
 #pragma unroll
             for (int u = 0; u < 64; ++u) fv[u] = ib[u * NZ];
-            __builtin_amdgcn_sched_barrier(0);  // all 64 loads in flight before the first use
+#pragma unroll
+            for (int u = 0; u < 64; ++u) cv[u] = col[u * P];
+            __builtin_amdgcn_sched_barrier(0);  // all loads in flight before the first use
             float Pf = 0.f, Qf = 1.f;  // y_{jlo + 63} = Pf + Qf y_{jlo - 1}
 #pragma unroll
             for (int u = 0; u < 64; ++u) {
                 const float aj = al[u];
-                Pf = (col[u * P] - aj * Pf) * fv[u];
+                Pf = (cv[u] - aj * Pf) * fv[u];
                 Qf = -aj * Qf * fv[u];
             }
             pq[sg * NZ + m] = Pf;
             pq[(4 + sg) * NZ + m] = Qf;
         }
         __syncthreads();
+        float Pb = 0.f, Qb = 1.f;  // x_{jlo} = Pb + Qb x_{jlo + 64}
         if (tri) {
             float y = 0.f;  // y_{jlo - 1} (a_0 = 0 in the first segment)
             for (int k = 0; k < sg; ++k) y = pq[k * NZ + m] + pq[(4 + k) * NZ + m] * y;
 #pragma unroll
             for (int u = 0; u < 64; ++u) {
-                y = (col[u * P] - al[u] * y) * fv[u];
-                col[u * P] = y;
+                y = (cv[u] - al[u] * y) * fv[u];
+                cv[u] = y;
             }
             // back over the same rows, high j to low: fv[u] = g_{j+1} for
             // j = jlo + 63 - u (zero past the last row)
@@ -292,15 +299,14 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                 fv[u] = cu[63 - u] * hi;
                 fv[63 - u] = cu[u] * lo;
             }
+#pragma unroll
+            for (int u = 0; u < 64; ++u) {
+                Pb = cv[63 - u] - fv[u] * Pb;
+                Qb = -fv[u] * Qb;
+            }
         }
         __syncthreads();  // the forward maps are read before they are replaced
         if (tri) {
-            float Pb = 0.f, Qb = 1.f;  // x_{jlo} = Pb + Qb x_{jlo + 64}
-#pragma unroll
-            for (int u = 0; u < 64; ++u) {
-                Pb = col[(63 - u) * P] - fv[u] * Pb;
-                Qb = -fv[u] * Qb;
-            }
             pq[sg * NZ + m] = Pb;
             pq[(4 + sg) * NZ + m] = Qb;
         }
@@ -310,7 +316,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
             for (int k = nseg - 1; k > sg; --k) x = pq[k * NZ + m] + pq[(4 + k) * NZ + m] * x;
 #pragma unroll
             for (int u = 0; u < 64; ++u) {
-                x = col[(63 - u) * P] - fv[u] * x;
+                x = cv[63 - u] - fv[u] * x;
                 col[(63 - u) * P] = x;
             }
         }
@@ -365,7 +371,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         }
     };
     inverse(a.dlt);
-    if (a.mr) {
+    if (MR) {
         __syncthreads();  // every wave has read D
         if (fil) {
 #pragma unroll
@@ -382,35 +388,41 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     __syncthreads();  // the block is free for the next mode
 }
 
-template <int NZ>
+template <int NZ, bool MR>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_kernel(SpecArgs a, int nx, int ny, int total)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ>(a, spec_lds, bx)));
+    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR>(a, spec_lds, bx)));
 }
 
-template <int NZ>
+template <int NZ, bool MR>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_grouped(const SpecArgs *__restrict__ args, GroupSched gs)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ>(ga, spec_lds, bx)));
+    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR>(ga, spec_lds, bx)));
 }
 
-// variant v: nzi = 32 << v
+// variant v: nzi = 32 << (v % 3), with the M_r filter when v >= 3
 const void *spec_fn(int variant, bool grouped)
 {
+#define BH_SPEC_FN(NZ, MR) \
+    (grouped ? (const void *)&spectral_grouped<NZ, MR> : (const void *)&spectral_kernel<NZ, MR>)
     switch (variant) {
-    case 0: return grouped ? (const void *)&spectral_grouped<32> : (const void *)&spectral_kernel<32>;
-    case 1: return grouped ? (const void *)&spectral_grouped<64> : (const void *)&spectral_kernel<64>;
-    case 2: return grouped ? (const void *)&spectral_grouped<128> : (const void *)&spectral_kernel<128>;
+    case 0: return BH_SPEC_FN(32, false);
+    case 1: return BH_SPEC_FN(64, false);
+    case 2: return BH_SPEC_FN(128, false);
+    case 3: return BH_SPEC_FN(32, true);
+    case 4: return BH_SPEC_FN(64, true);
+    case 5: return BH_SPEC_FN(128, true);
     default: return nullptr;
     }
+#undef BH_SPEC_FN
 }
 
 // the block, the segment maps of the Thomas sweeps, the sub- and super-diagonal
 int spec_lds_bytes(int variant)
 {
-    const int nz = 32 << variant;
+    const int nz = 32 << (variant % 3);
     return (SPEC_ROWS * (nz + 4) + 8 * nz + 2 * SPEC_ROWS) * (int)sizeof(float);
 }
 
@@ -427,7 +439,7 @@ bool spectral_fused_ok(int nri, int nzi)
 
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi)
 {
-    return Op::make(OP_SPECTRAL, spec_variant(nzi), a, nplanes, 1, 1);
+    return Op::make(OP_SPECTRAL, spec_variant(nzi) + (a.mr ? 3 : 0), a, nplanes, 1, 1);
 }
 
 void spectral_fragments(int n, std::vector<float> *sfb, std::vector<float> *sfa)
