@@ -19,6 +19,10 @@ struct DftArgs {
     int mask_numr, mask_numz;  // forward: zero the rhop layers (potential_solver.F90:70-78); 0 = no mask
     const float *xadd;  // inverse at N = 512: input is x + xadd (FP32), ...
     double *xstore;     // ... and x + xadd is also stored here (both null: plain input)
+    // inverse at N = 512 with xadd, a cold start: on the interior points of
+    // the mask_numr x mask_numz grid the input is xint + xadd (FP32 both,
+    // each widened to FP64), and only the face points of x are read
+    const float *xint;
     int max_blocks;     // single launches at N = 512: cap on the persistent grid (0: one per panel)
     // forward FP32 at N = 512 with the mask: also rout[L][K-2][J-2] =
     // rscale * out for the interior points (null: not written)

@@ -495,11 +495,21 @@ __device__ __forceinline__ void inv8_load(const DftArgs &a, int bx, Inv8Regs &R)
     long ldx = a.ldx;
     asm volatile("" : "+s"(ldx));  // (not hoisted into registers held across panels)
     const int kk = tid / R4CB, c = tid - kk * R4CB, gc = min(bx * R4CB + c, a.M - 1);
+    bool inner = false;  // a cold start's interior column: x is xint there
+    if (a.xint) {
+        const int nr = a.mask_numr, J = gc % nr + 1, K = gc / nr + 1;
+        inner = J >= 2 && J <= nr - 1 && K >= 2 && K <= a.mask_numz - 1;
+    }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const long o0 = (long)inv8_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv8_slot_row(kk, q, 1) * ldx + gc;
-        R.av[q] = a.x[o0];
-        R.sv[q] = a.x[o1];
+        if (inner) {
+            R.av[q] = (double)a.xint[o0];
+            R.sv[q] = (double)a.xint[o1];
+        } else {
+            R.av[q] = a.x[o0];
+            R.sv[q] = a.x[o1];
+        }
         if (a.xadd) {
             R.fa[q] = a.xadd[o0];
             R.fs[q] = a.xadd[o1];

@@ -26,7 +26,8 @@ inline double sine_basis(int a, int k, int n)
     return std::sqrt(2.0 / (n + 1)) * std::sin(3.141592653589793238 * (double)ab / (n + 1));
 }
 
-// dlt, w and Xo are given together (the refinement pass) or all null
+// dlt and w are given together (the refinement pass; Xo = X + dlt is then
+// written unless Xo is null) or both null (the first residual)
 struct ResidualArgs {
     const double *X; const float *dlt; const float *w; const float *rh;
     double *Xo; float *r;

@@ -34,12 +34,14 @@ namespace bh {
 // ---------------------------------------------------------------------------
 // Tile: 256 consecutive J of one plane and RES_ROWS rows of K, the column's
 // rows K0-1 .. K0+RES_ROWS and the J coefficients loaded once.
-// The variant fixes at compile time whether delta, w and Xo are present
-// (UPD: the refinement pass) and whether X is a cold start (COLD), so that
-// the row loop carries no pointer tests.
+// The variant fixes at compile time whether delta and w are present (UPD:
+// the refinement pass), whether Xo is written (WX; a cold start whose x is
+// formed later by the inverse DFT from the two corrections leaves it out)
+// and whether X is a cold start (COLD), so that the row loop carries no
+// pointer tests.
 constexpr int RES_ROWS = 8;
 
-template <bool UPD, bool COLD, bool EDGE>
+template <bool UPD, bool COLD, bool EDGE, bool WX>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz)
 {
     const int numr = a.numr, nri = numr - 2, nzi = a.numz - 2;
@@ -53,7 +55,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
     const float *__restrict__ rh = a.rh + L * plane;
     const float *__restrict__ w = UPD ? a.w + L * plane : nullptr;
-    double *__restrict__ Xo = UPD ? a.Xo + L * plane : nullptr;
+    double *__restrict__ Xo = WX ? a.Xo + L * plane : nullptr;
     float *__restrict__ r32 = a.r + (long)L * nri * nzi;
     const double brbJ = a.brb[(long)L * (numr + 1) + J], aJ = a.alphar[J], bJ = a.betar[J];
     // cold start: the interior of X is zero and dlt is zero on the faces, so
@@ -114,25 +116,26 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
             Ax += has_west ? bJ * xwv[i] : 0.0;
             Ax += 2.0 * a.gamma * xc - a.gamma * (xcv[i] + xcv[i + 2]);
             double r = -a.four_pi * (double)rv[i] - Ax;
-            if (UPD) {
-                r -= (double)wv[i];
-                Xo[ci] = xc;
-            }
+            if (UPD) r -= (double)wv[i];
+            if (WX) Xo[ci] = xc;
             r32[(long)(K0 - 2 + i) * nri + jr] = (float)r;
         }
     }
 }
 
-// the refinement form (delta, w and Xo given) or the first residual, cold or
-// warm; the first and last tiles of a plane (face rows K = 1, numz) apart
+// the refinement form (delta and w given, Xo written or not) or the first
+// residual, cold or warm; the first and last tiles of a plane (face rows
+// K = 1, numz) apart
 template <bool EDGE>
 __device__ __forceinline__ void residual_tile_e(const ResidualArgs &a, int bx, int by, int bz)
 {
-    switch ((a.dlt ? 2 : 0) + (a.cold ? 1 : 0)) {
-    case 0: residual_tile_v<false, false, EDGE>(a, bx, by, bz); break;
-    case 1: residual_tile_v<false, true, EDGE>(a, bx, by, bz); break;
-    case 2: residual_tile_v<true, false, EDGE>(a, bx, by, bz); break;
-    default: residual_tile_v<true, true, EDGE>(a, bx, by, bz); break;
+    switch ((a.dlt ? (a.Xo ? 2 : 4) : 0) + (a.cold ? 1 : 0)) {
+    case 0: residual_tile_v<false, false, EDGE, false>(a, bx, by, bz); break;
+    case 1: residual_tile_v<false, true, EDGE, false>(a, bx, by, bz); break;
+    case 2: residual_tile_v<true, false, EDGE, true>(a, bx, by, bz); break;
+    case 3: residual_tile_v<true, true, EDGE, true>(a, bx, by, bz); break;
+    case 4: residual_tile_v<true, false, EDGE, false>(a, bx, by, bz); break;
+    default: residual_tile_v<true, true, EDGE, false>(a, bx, by, bz); break;
     }
 }
 __device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)

@@ -285,6 +285,14 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         out->push_back(gemm_op(a, G32_DST_INV));
     };
     double *Xcur = Xa, *Xnext = Xb;
+    // a cold start with one refinement pass at N = 512: x = faces + delta1
+    // + delta2 is formed by the inverse DFT itself (two FP32 reads in place
+    // of the FP64 x the refinement residual would write and the DFT read);
+    // delta2 goes to the w buffer, free once the refinement residual has
+    // read it
+    const bool cold_fused = tstep <= 1 && p->passes == 1 && o.spec_fused && o.dinv64 && p->use_dft &&
+                            N == DFT_MAX_N;
+    float *dlt2 = cold_fused ? p->Wf : p->D1;
     if (o.spec_fused) {
         // 4-5 in one kernel per mode, the block kept in LDS
         SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.betar, o.alphar, p->D1, p->Wf,
@@ -311,11 +319,11 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     }
     // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
     for (int it = 0; it < p->passes; ++it) {
-        ra.X = Xcur; ra.dlt = p->D1; ra.w = p->Wf; ra.Xo = Xnext; ra.r = p->r32;
+        ra.X = Xcur; ra.dlt = p->D1; ra.w = p->Wf; ra.Xo = cold_fused ? nullptr : Xnext; ra.r = p->r32;
         ra.cold = (tstep <= 1 && it == 0);  // Xcur = Xa until the first pass writes Xnext
         out->push_back(residual_op(ra, N));
         if (o.spec_fused) {
-            SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, p->D1, nullptr,
+            SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, dlt2, nullptr,
                         numr, numz, N};
             out->push_back(spectral_op(sa, N, nzi));
         } else {
@@ -327,7 +335,14 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     }
     // 7. x = Xcur + D1 into Xa, the warm-start state, then the inverse DFT
     //    (FP64) and the axis ghost; the panel DFT does both in one pass
-    if (o.dinv64 && p->use_dft && N == DFT_MAX_N) {
+    if (cold_fused) {
+        out->push_back(Op::mark(3));
+        // Xa holds the faces (its interior is not read); in place is safe:
+        // every element is read and written by the same thread
+        DftArgs d{Xa, M, d_pot, M, o.dinv64, N, (int)M, numr, numz, dlt2, Xa};
+        d.xint = p->D1;
+        out->push_back(dft_op(d, DFT_INV_F64));
+    } else if (o.dinv64 && p->use_dft && N == DFT_MAX_N) {
         out->push_back(Op::mark(3));
         DftArgs d{Xcur, M, d_pot, M, o.dinv64, N, (int)M, 0, 0, p->D1, Xa};
         out->push_back(dft_op(d, DFT_INV_F64));
