@@ -467,10 +467,28 @@ constexpr int R8N = PN / 8, R8KG = 2, R8KPp = 16 * R8KG + 2, R8VC = 8 * R4CB;
 constexpr size_t R8_LDS = (size_t)2 * R8VC * R8KPp * 8 + 2 * R8VC * 8;
 constexpr int R8_FRAGS = 4 * R8KG * 64 * 4;  // doubles of the size-n P and Q fragments
 
+// Input modes of the inverse: plain x; x + xadd (the FP32 correction
+// loaded beside x); a cold start, where an interior column's register pair
+// holds the two FP32 values xint, xadd (bit-packed) and a face column's the
+// FP64 x (xadd is zero on the faces).  Each mode is its own kernel, so that
+// the registers held across a panel are only those the mode needs.
+enum { INV_PLAIN = 0, INV_ADD = 1, INV_COLD = 2 };
+template <int MODE>
 struct Inv8Regs {
+    double av[8], sv[8];
+};
+template <>
+struct Inv8Regs<INV_ADD> {
     double av[8], sv[8];
     float fa[8], fs[8];
 };
+typedef float inv_f2 __attribute__((ext_vector_type(2)));
+// a cold start's interior column (2 <= J <= numr - 1, 2 <= K <= numz - 1)
+__device__ __forceinline__ bool inv8_inner(const DftArgs &a, int gc)
+{
+    const int nr = a.mask_numr, J = gc % nr + 1, K = gc / nr + 1;
+    return J >= 2 && J <= nr - 1 && K >= 2 && K <= a.mask_numz - 1;
+}
 
 // input row of slot (q, s) of item kk (s = 0: av, 1: sv).  kk = 1..n/2-1:
 // X_{kk+qn} is a_d + i s_d for d = kk + qn <= N/2 (q < 4) and the conjugate
@@ -488,31 +506,35 @@ __device__ __forceinline__ int inv8_slot_row(int kk, int q, int s)
     return s == 0 ? d : PN / 2 + d;
 }
 
-__device__ __forceinline__ void inv8_load(const DftArgs &a, int bx, Inv8Regs &R)
+template <int MODE>
+__device__ __forceinline__ void inv8_load(const DftArgs &a, int bx, Inv8Regs<MODE> &R)
 {
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));  // lane-dependent offsets recomputed per panel
     long ldx = a.ldx;
     asm volatile("" : "+s"(ldx));  // (not hoisted into registers held across panels)
     const int kk = tid / R4CB, c = tid - kk * R4CB, gc = min(bx * R4CB + c, a.M - 1);
-    bool inner = false;  // a cold start's interior column: x is xint there
-    if (a.xint) {
-        const int nr = a.mask_numr, J = gc % nr + 1, K = gc / nr + 1;
-        inner = J >= 2 && J <= nr - 1 && K >= 2 && K <= a.mask_numz - 1;
-    }
+    // one branch around all the loads of a kind (a branch per load makes
+    // the compiler wait where the paths join)
+    if (MODE == INV_COLD && inv8_inner(a, gc)) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const long o0 = (long)inv8_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv8_slot_row(kk, q, 1) * ldx + gc;
-        if (inner) {
-            R.av[q] = (double)a.xint[o0];
-            R.sv[q] = (double)a.xint[o1];
-        } else {
-            R.av[q] = a.x[o0];
-            R.sv[q] = a.x[o1];
+        for (int q = 0; q < 8; ++q) {
+            const long o0 = (long)inv8_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv8_slot_row(kk, q, 1) * ldx + gc;
+            R.av[q] = __builtin_bit_cast(double, (inv_f2){a.xint[o0], a.xadd[o0]});
+            R.sv[q] = __builtin_bit_cast(double, (inv_f2){a.xint[o1], a.xadd[o1]});
         }
-        if (a.xadd) {
-            R.fa[q] = a.xadd[o0];
-            R.fs[q] = a.xadd[o1];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            R.av[q] = a.x[(long)inv8_slot_row(kk, q, 0) * ldx + gc];
+            R.sv[q] = a.x[(long)inv8_slot_row(kk, q, 1) * ldx + gc];
+        }
+    }
+    if constexpr (MODE == INV_ADD) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            R.fa[q] = a.xadd[(long)inv8_slot_row(kk, q, 0) * ldx + gc];
+            R.fs[q] = a.xadd[(long)inv8_slot_row(kk, q, 1) * ldx + gc];
         }
     }
 }
@@ -546,7 +568,8 @@ __device__ __forceinline__ void dft8_pos(const double *xr, const double *xi, dou
 
 // the eight spectra Y^(r)_k into LDS; with xadd, x + xadd is also stored
 // (every input row belongs to exactly one item slot)
-__device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Regs &R, unsigned char *lds)
+template <int MODE>
+__device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Regs<MODE> &R, unsigned char *lds)
 {
     double *Ca = (double *)lds, *Sb = Ca + R8VC * R8KPp, *E0 = Sb + R8VC * R8KPp, *EH = E0 + R8VC;
     int tid = threadIdx.x;
@@ -556,16 +579,26 @@ __device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Re
     const double *tw = (const double *)a.A + R8_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n/2
     const int kk = tid / R4CB, c = tid - kk * R4CB, gc = bx * R4CB + c;
     double av[8], sv[8];
+    const bool inner = MODE == INV_COLD && inv8_inner(a, min(gc, a.M - 1));
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         av[q] = R.av[q];
         sv[q] = R.sv[q];
+        if constexpr (MODE == INV_COLD) {
+            if (inner) {
+                const inv_f2 pa = __builtin_bit_cast(inv_f2, R.av[q]), ps = __builtin_bit_cast(inv_f2, R.sv[q]);
+                av[q] = (double)pa.x + (double)pa.y;
+                sv[q] = (double)ps.x + (double)ps.y;
+            }
+        }
     }
-    if (a.xadd) {
+    if constexpr (MODE != INV_PLAIN) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
-            av[q] += (double)R.fa[q];
-            sv[q] += (double)R.fs[q];
+            if constexpr (MODE == INV_ADD) {
+                av[q] += (double)R.fa[q];
+                sv[q] += (double)R.fs[q];
+            }
             if (gc < a.M) {
                 a.xstore[(long)inv8_slot_row(kk, q, 0) * ldx + gc] = av[q];
                 a.xstore[(long)inv8_slot_row(kk, q, 1) * ldx + gc] = sv[q];
@@ -686,27 +719,29 @@ __device__ __forceinline__ void inv8_work(const DftArgs &a, int bx, const unsign
     }
 }
 
-template <class Resolve>
+template <int MODE, class Resolve>
 __device__ __forceinline__ void inv8_loop(int total, Resolve resolve, unsigned char *lds)
 {
-    panel_loop<Inv8Regs>(
-        total, resolve, [&](const DftArgs &a, int bx, Inv8Regs &R) { inv8_load(a, bx, R); },
-        [&](const DftArgs &a, int bx, const Inv8Regs &R) { inv8_form(a, bx, R, lds); },
+    panel_loop<Inv8Regs<MODE>>(
+        total, resolve, [&](const DftArgs &a, int bx, Inv8Regs<MODE> &R) { inv8_load<MODE>(a, bx, R); },
+        [&](const DftArgs &a, int bx, const Inv8Regs<MODE> &R) { inv8_form<MODE>(a, bx, R, lds); },
         [&](const DftArgs &a, int bx) { inv8_work(a, bx, lds); });
 }
 
+template <int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv8_kernel(DftArgs a, int nx,
                                                                                              int ny, int total)
 {
     extern __shared__ __align__(16) unsigned char inv8_lds[];
-    inv8_loop(total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, inv8_lds);
+    inv8_loop<MODE>(total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, inv8_lds);
 }
 
+template <int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv8_grouped(
     const DftArgs *__restrict__ args, GroupSched gs)
 {
     extern __shared__ __align__(16) unsigned char inv8_lds[];
-    inv8_loop(
+    inv8_loop<MODE>(
         gs.total,
         [&](int t, DftArgs &o, int &bx) {
             const int g = bh_find_group(gs.grp, gs.ngroups, t);
@@ -952,7 +987,9 @@ __global__ __launch_bounds__(512, 4) void dft_grouped(const DftArgs *__restrict_
 Op dft_op(const DftArgs &a, int kind)
 {
     const int CB = a.N == DFT_MAX_N ? R4CB : dft_cols(kind);  // N = 512: the radix-4 kernels' panels
-    return Op::make(OP_DFT, kind + 4 * log2i(a.N), a, (a.M + CB - 1) / CB, 1, 1);
+    // variant: kind + 4 log2 N + 64 (input mode of the inverse at N = 512)
+    const int mode = kind == DFT_INV_F64 && a.N == DFT_MAX_N ? (a.xint ? INV_COLD : a.xadd ? INV_ADD : INV_PLAIN) : 0;
+    return Op::make(OP_DFT, kind + 4 * log2i(a.N) + 64 * mode, a, (a.M + CB - 1) / CB, 1, 1);
 }
 
 namespace {
@@ -966,13 +1003,17 @@ const void *dft_fn(int variant, bool grouped)
 }
 }  // namespace
 
-// the radix-4 kernels of N = 512: function and dynamic LDS per kind
-static const void *r4_fn(int kind, bool grouped, int *lds)
+// the radix-4 kernels of N = 512: function and dynamic LDS per variant
+static const void *r4_fn(int variant, bool grouped, int *lds)
 {
-    switch (kind) {
+    switch (variant % 4) {
     case DFT_INV_F64:
         *lds = (int)R8_LDS;
-        return grouped ? (const void *)&dft_inv8_grouped : (const void *)&dft_inv8_kernel;
+        switch (variant / 64) {
+        case INV_PLAIN: return grouped ? (const void *)&dft_inv8_grouped<0> : (const void *)&dft_inv8_kernel<0>;
+        case INV_ADD: return grouped ? (const void *)&dft_inv8_grouped<1> : (const void *)&dft_inv8_kernel<1>;
+        default: return grouped ? (const void *)&dft_inv8_grouped<2> : (const void *)&dft_inv8_kernel<2>;
+        }
     case DFT_FWD_F32:
         *lds = (int)R4Fwd<float>::bytes;
         return grouped ? (const void *)&dft_fwd4_grouped<float> : (const void *)&dft_fwd4_kernel<float>;
@@ -988,15 +1029,15 @@ bool dft_launch(const Op &op, hipStream_t s)
     int nx = op.nx, ny = op.ny, total = (int)op.tiles();
     void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
     int nb = persistent_blocks(total, 0);
-    if ((1 << (op.variant / 4)) == DFT_MAX_N) {
+    if ((1 << (op.variant / 4 % 16)) == DFT_MAX_N) {
         int lds;
-        const void *fn = r4_fn(op.variant % 4, false, &lds);
+        const void *fn = r4_fn(op.variant, false, &lds);
         const int cap = op.as<DftArgs>().max_blocks;  // side stream: leave room for the main stream
         if (cap > 0 && nb > cap) nb = cap >= 8 ? cap / 8 * 8 : cap;
         (void)hipLaunchKernel(fn, dim3(nb), dim3(512), argv, lds, s);
         return true;
     }
-    const int lds = (int)dft_lds_bytes(op.variant % 4, 1 << (op.variant / 4));
+    const int lds = (int)dft_lds_bytes(op.variant % 4, 1 << (op.variant / 4 % 16));
     (void)hipLaunchKernel(dft_fn(op.variant, false), dim3(nb), dim3(512), argv, lds, s);
     return true;
 }
@@ -1009,10 +1050,10 @@ bool dft_launch_grouped(int kind, int variant, const void *d_args, const GroupSc
     void *argv[] = {(void *)&ap, (void *)&g};
     int lds;
     const void *fn;
-    if ((1 << (variant / 4)) == DFT_MAX_N) {
-        fn = r4_fn(variant % 4, true, &lds);
+    if ((1 << (variant / 4 % 16)) == DFT_MAX_N) {
+        fn = r4_fn(variant, true, &lds);
     } else {
-        lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4));
+        lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4 % 16));
         fn = dft_fn(variant, true);
     }
     (void)hipLaunchKernel(fn, dim3(nblocks), dim3(512), argv, lds, s);
@@ -1024,10 +1065,10 @@ bool dft_occupancy(int kind, int variant, int *blocks)
     if (kind != OP_DFT) return false;
     int lds;
     const void *fn;
-    if ((1 << (variant / 4)) == DFT_MAX_N) {
-        fn = r4_fn(variant % 4, true, &lds);
+    if ((1 << (variant / 4 % 16)) == DFT_MAX_N) {
+        fn = r4_fn(variant, true, &lds);
     } else {
-        lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4));
+        lds = (int)dft_lds_bytes(variant % 4, 1 << (variant / 4 % 16));
         fn = dft_fn(variant, true);
     }
     *blocks = kernel_occupancy(fn, 512, lds);
