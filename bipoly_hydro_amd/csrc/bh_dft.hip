@@ -486,7 +486,9 @@ typedef float inv_f2 __attribute__((ext_vector_type(2)));
 // a cold start's interior column (2 <= J <= numr - 1, 2 <= K <= numz - 1)
 __device__ __forceinline__ bool inv8_inner(const DftArgs &a, int gc)
 {
-    const int nr = a.mask_numr, J = gc % nr + 1, K = gc / nr + 1;
+    int nr = a.mask_numr;
+    asm volatile("" : "+s"(nr));  // the division's constants recomputed per use, not held across panels
+    const int J = gc % nr + 1, K = gc / nr + 1;
     return J >= 2 && J <= nr - 1 && K >= 2 && K <= a.mask_numz - 1;
 }
 
