@@ -763,19 +763,23 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void d
 //   G'^(r)_k = w^{-(n-k) r} conj F^(r)_k.
 // The four length-n real transforms F^(r) run as the folded cosine/sine
 // products of the direct form at size n on four virtual columns per
-// column (a quarter of its matrix work); a workgroup takes R4CB columns,
+// column (a quarter of its matrix work); a workgroup takes R4Fwd::CB columns,
 // folds the quarter sequences into LDS, runs the products (wave w: row
-// tile w % 4 of the cosine and of the sine half, quarters 2 (w / 4) and
-// 2 (w / 4) + 1), puts F in LDS and combines the rows k, n + k, n - k,
+// tile w % 4 of the cosine and of the sine half, the half w / 4 of the
+// 16-wide virtual-column tiles), puts F in LDS and combines the rows k, n + k, n - k,
 // 2n - k of X in FP64, masked as the direct form is.
 // ---------------------------------------------------------------------------
 namespace {
 template <typename T>
 struct R4Fwd {
+    // panel columns: an FP32 output row of a panel is then 128 bytes, a
+    // whole cache line (64-byte rows, each line written half by one panel
+    // and half by the next, ran the stores about 30 % slower)
+    static constexpr int CB = sizeof(T) == 4 ? 2 * R4CB : R4CB, VC = 4 * CB;
     static constexpr int KPp = 16 * R4KG + Pad<T>::v;
-    static constexpr size_t uv = (size_t)2 * R4VC * KPp * sizeof(T);          // U, V (then Fa, Fs)
-    static constexpr size_t f = (size_t)2 * (R4N / 2 + 1) * R4VC * sizeof(T);  // Fa, Fs
-    static constexpr size_t bytes = (uv > f ? uv : f) + 3 * R4VC * sizeof(double);
+    static constexpr size_t uv = (size_t)2 * VC * KPp * sizeof(T);          // U, V (then Fa, Fs)
+    static constexpr size_t f = (size_t)2 * (R4N / 2 + 1) * VC * sizeof(T);  // Fa, Fs
+    static constexpr size_t bytes = (uv > f ? uv : f) + 3 * VC * sizeof(double);
     static constexpr int frags = 8 * R4KG * 64 * 4;  // size-n cosine and sine fragments
 };
 
@@ -785,21 +789,21 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     typedef typename Mma<T>::acc_t acc_t;
     typedef T v4 __attribute__((ext_vector_type(4)));
     typedef R4Fwd<T> Ly;
-    constexpr int KPp = Ly::KPp;
-    T *U = (T *)lds, *V = U + R4VC * KPp;
-    T *Fa = (T *)lds, *Fs = Fa + (R4N / 2 + 1) * R4VC;  // after the products, over U and V
-    double *X0 = (double *)(lds + (Ly::uv > Ly::f ? Ly::uv : Ly::f)), *XH = X0 + R4VC, *FM = XH + R4VC;
+    constexpr int KPp = Ly::KPp, CB = Ly::CB, VC = Ly::VC, CT = VC / 16;  // CT: 16-wide column tiles
+    T *U = (T *)lds, *V = U + VC * KPp;
+    T *Fa = (T *)lds, *Fs = Fa + (R4N / 2 + 1) * VC;  // after the products, over U and V
+    double *X0 = (double *)(lds + (Ly::uv > Ly::f ? Ly::uv : Ly::f)), *XH = X0 + VC, *FM = XH + VC;
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));  // lane-dependent offsets recomputed per tile, not held across tiles
     const int lane = tid & 63, wave = tid >> 6;
-    const int col0 = bx * R4CB;
+    const int col0 = bx * CB;
     const double *x = a.x;
     long ldx = a.ldx;
     asm volatile("" : "+s"(ldx));
     // 1. fold the quarter sequences x^(r)_j = x_{4j+r}: u_j, v_j (j = 1..n/2-1), x^(r)_0, x^(r)_{n/2}
 #pragma unroll 1
-    for (int e = tid; e < (R4N / 2 + 1) * R4CB; e += 512) {
-        const int j = e / R4CB, c = e - j * R4CB, gc = col0 + c;
+    for (int e = tid; e < (R4N / 2 + 1) * CB; e += 512) {
+        const int j = e / CB, c = e - j * CB, gc = col0 + c;
         const int gcc = gc < a.M ? gc : a.M - 1;
         const bool fold = j > 0 && j < R4N / 2;
         double p[4], q[4];
@@ -810,7 +814,7 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int vc = r * R4CB + c;
+            const int vc = r * CB + c;
             if (j == 0) {
                 X0[vc] = p[r];
             } else if (j == R4N / 2) {
@@ -825,8 +829,9 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     }
     __syncthreads();
     // middle row of each quarter: F_{n/2} = x_0 + x_{n/2} + sum_j (-1)^j u_j (FP64 sum)
-    {
-        const int vc = tid >> 3, jj = tid & 7;
+#pragma unroll
+    for (int vc0 = 0; vc0 < VC; vc0 += 64) {
+        const int vc = vc0 + (tid >> 3), jj = tid & 7;
         double sm = 0.0;
         for (int j = 1 + jj; j < R4N / 2; j += 8) {
             const double u = (double)U[vc * KPp + j - 1];
@@ -839,16 +844,18 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     }
     // 2. size-n products: cosine rows k = 16 mt.., sine rows k = 16 mt + 1..
     const v4 *Af = (const v4 *)a.A;
-    const int mt = wave & 3, t0 = 2 * (wave >> 2), jl = 4 * (lane >> 4);
-    acc_t accC[2], accS[2];
+    // wave w: row tile w % 4, column tiles TW (w / 4) .. + TW - 1
+    constexpr int TW = CT / 2;
+    const int mt = wave & 3, t0 = TW * (wave >> 2), jl = 4 * (lane >> 4);
+    acc_t accC[TW], accS[TW];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) accC[i] = accS[i] = (acc_t){(T)0, (T)0, (T)0, (T)0};
+    for (int i = 0; i < TW; ++i) accC[i] = accS[i] = (acc_t){(T)0, (T)0, (T)0, (T)0};
 #pragma unroll 1
     for (int grp = 0; grp < R4KG; ++grp) {
         const v4 fc = Af[((long)mt * R4KG + grp) * 64 + lane];
         const v4 fs = Af[((long)(4 + mt) * R4KG + grp) * 64 + lane];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < TW; ++i) {
             const int off = (16 * (t0 + i) + (lane & 15)) * KPp + 16 * grp + jl;
             const v4 bu = *(const v4 *)(U + off), bv = *(const v4 *)(V + off);
 #pragma unroll
@@ -860,27 +867,27 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     }
     __syncthreads();  // U, V read: F goes over them
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int c = lane & 15, vc = (t0 + i) * R4CB + c;
+    for (int i = 0; i < TW; ++i) {
+        const int vc = 16 * (t0 + i) + (lane & 15);
         const double e0 = X0[vc], eh = XH[vc];
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int k = 16 * mt + Mma<T>::row(lane, v);  // cosine row k, sine row k + 1
-            Fa[k * R4VC + vc] = (T)((e0 + ((k & 1) ? -eh : eh)) + (double)accC[i][v]);
-            if (k + 1 < R4N / 2) Fs[(k + 1) * R4VC + vc] = accS[i][v];
+            Fa[k * VC + vc] = (T)((e0 + ((k & 1) ? -eh : eh)) + (double)accC[i][v]);
+            if (k + 1 < R4N / 2) Fs[(k + 1) * VC + vc] = accS[i][v];
         }
     }
-    if (tid < R4VC) {
-        Fa[(R4N / 2) * R4VC + tid] = (T)FM[tid];
+    if (tid < VC) {
+        Fa[(R4N / 2) * VC + tid] = (T)FM[tid];
         Fs[tid] = (T)0;                      // F_0, F_{n/2} are real
-        Fs[(R4N / 2) * R4VC + tid] = (T)0;
+        Fs[(R4N / 2) * VC + tid] = (T)0;
     }
     __syncthreads();
     // 3. combine: rows k, n + k (always), 2n - k (k < n/2), n - k (0 < k < n/2)
     const T *tw = (const T *)a.A + Ly::frags;  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n
 #pragma unroll 1
-    for (int e = tid; e < (R4N / 2 + 1) * R4CB; e += 512) {
-        const int k = e / R4CB, c = e - k * R4CB, gc = col0 + c;
+    for (int e = tid; e < (R4N / 2 + 1) * CB; e += 512) {
+        const int k = e / CB, c = e - k * CB, gc = col0 + c;
         if (gc >= a.M) continue;
         bool masked = false;
         long ri = -1;  // offset of (J, K) in a plane of rout, if interior
@@ -894,8 +901,8 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
         double gr[4], gi[4], hr[4], hi[4];  // G^(r)_k and G'^(r)_k
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int vc = r * R4CB + c;
-            const double fr = (double)Fa[k * R4VC + vc], fi = (double)Fs[k * R4VC + vc];
+            const int vc = r * CB + c;
+            const double fr = (double)Fa[k * VC + vc], fi = (double)Fs[k * VC + vc];
             const double c1 = (double)tw[(k * 4 + r) * 2], s1 = (double)tw[(k * 4 + r) * 2 + 1];
             const double c2 = (double)tw[((R4N - k) * 4 + r) * 2], s2 = (double)tw[((R4N - k) * 4 + r) * 2 + 1];
             // w^{-kr} F = (c1 - i s1)(fr + i fi); w^{-(n-k) r} conj F = (c2 - i s2)(fr - i fi)
@@ -988,7 +995,8 @@ __global__ __launch_bounds__(512, 4) void dft_grouped(const DftArgs *__restrict_
 
 Op dft_op(const DftArgs &a, int kind)
 {
-    const int CB = a.N == DFT_MAX_N ? R4CB : dft_cols(kind);  // N = 512: the radix-4 kernels' panels
+    // N = 512: the radix kernels' panels (the FP32 forward's are twice as wide)
+    const int CB = a.N == DFT_MAX_N ? (kind == DFT_FWD_F32 ? R4Fwd<float>::CB : R4CB) : dft_cols(kind);
     // variant: kind + 4 log2 N + 64 (input mode of the inverse at N = 512)
     const int mode = kind == DFT_INV_F64 && a.N == DFT_MAX_N ? (a.xint ? INV_COLD : a.xadd ? INV_ADD : INV_PLAIN) : 0;
     return Op::make(OP_DFT, kind + 4 * log2i(a.N) + 64 * mode, a, (a.M + CB - 1) / CB, 1, 1);
