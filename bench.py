@@ -22,11 +22,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solves/s @258×130×512 mixed vs FP64 at relL2≤1e-8; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per solve of the boundary contraction sequence (scale, contract,
-# reduce) and of one FP64 ADI half-step at 258 x 130 x 512, from rocprofv3
-# FETCH_SIZE (x 2.000, the copy-kernel calibration) + WRITE_SIZE;
+# HBM bytes per solve of the boundary contraction sequence (contract, with
+# the projections scaled as they are read, then reduce) and of one FP64 ADI
+# half-step at 258 x 130 x 512, from rocprofv3 FETCH_SIZE (x 2.000, the
+# copy-kernel calibration) + WRITE_SIZE;
 # profiles/potential_solver_258x130x512_hbm_traffic.md
-CONTRACT_TRAFFIC = ((8677.0 + 276840.0 + 2330.0) * 2.0 + (30721.0 + 3840.0 + 240.0)) * 1024  # FP32 table
+CONTRACT_TRAFFIC = ((270204.0 + 2332.0) * 2.0 + (3840.0 + 240.0)) * 1024  # FP32 table
 SWEEP_TRAFFIC = ((162531.0 + 144885.0) * 2.0 + (151557.0 + 138717.0)) * 1024 / 2  # mean of the two half-steps
 
 
@@ -205,7 +206,7 @@ def main():
         "p99_ms": lat_p99,
         "relL2_vs_reference_slices": mixed["relL2_vs_reference_slices"],
         "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
-        "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (topsym scale + contract + reduce)",
+        "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (contract with the projections scaled as read, then reduce)",
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,
                      "bytes_per_launch": ct_bytes, "launch_ms": ct_ms},

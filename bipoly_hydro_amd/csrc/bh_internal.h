@@ -137,12 +137,13 @@ __host__ __device__ inline int tb_off(int i, int j)
 constexpr int TB_PLANES = 8;
 __host__ __device__ inline int tb_chunks(int numz) { return (numz - 2 + TB_PLANES - 1) / TB_PLANES; }
 __host__ __device__ inline bool tb_blocked(int numr) { return numr - 2 <= 256; }
-// symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4),
-// contract (Ptb, mmax*topsym_splits*ceil(NB/2)*NB16*4), reduce into Stb (one slot)
+// symmetric-table contraction: scale the projections (Pt, mmax*nzi*NB16*4;
+// the unblocked form only), contract (Ptb), reduce into Stb (one slot)
 Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int numr, int numz, int mmax);
-// TS is the FP64 table; with TS null, TS32 the FP32 one
-Op contract_topsym_op(const double *TS, const float *TS32, const double *Pt, double *Ptb, int numr, int numz,
-                      int mmax);
+// TS is the FP64 table; with TS null, TS32 the FP32 one.  The blocked form
+// (tb_blocked) scales proj by sqrt_rhf itself; the unblocked one reads Pt.
+Op contract_topsym_op(const double *TS, const float *TS32, const double *proj, const double *sqrt_rhf,
+                      const double *Pt, double *Ptb, int numr, int numz, int mmax);
 Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax);
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
 // faces in Fourier mode order FF[mode row][fi]: fi in [0,nri) top J=fi+2,

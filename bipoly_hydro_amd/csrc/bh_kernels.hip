@@ -804,10 +804,27 @@ __device__ __forceinline__ void contract_tb_t(const ContractArgs &a, const TT *t
     __shared__ d4 Ps[TB_PLANES][256];
     __shared__ d4 y[256];
     const d4 zero4 = {0.0, 0.0, 0.0, 0.0};
-    const d4 *Pt = (const d4 *)a.proj + ((long)m * nzi + kz0) * NB16;
-    for (int e = tid; e < TB_PLANES * NB16; e += 32 * TB_GROUPS) {
-        const int u = e / NB16, p = e - u * NB16;
-        Ps[u][p] = u < nkz ? Pt[(long)u * NB16 + p] : zero4;
+    // the scaled projections of the workgroup's planes straight from the
+    // projection rows: sqrt(rhf) x (cos, sin) of source plane kz and of its
+    // mirror nzi - 1 - kz (what topsym_scale_tile stores for the unblocked
+    // contraction, computed here instead of read back)
+    {
+        const long plane = (long)a.numr * a.numz;
+        const double *pc = a.proj + (long)m * plane, *ps = a.proj + (long)(m > 0 ? (a.mmax - 1 + m) : 0) * plane;
+        for (int e = tid; e < TB_PLANES * NB16; e += 32 * TB_GROUPS) {
+            const int u = e / NB16, p = e - u * NB16;
+            d4 v = zero4;
+            if (u < nkz && p < nri) {
+                const int kz = kz0 + u;
+                const long pa = (long)(kz + 1) * a.numr + p + 1, pm = (long)(nzi - kz) * a.numr + p + 1;
+                const double sq = a.aux[p + 2];
+                v.x = sq * pc[pa];
+                v.y = m > 0 ? sq * ps[pa] : 0.0;
+                v.z = sq * pc[pm];
+                v.w = m > 0 ? sq * ps[pm] : 0.0;
+            }
+            Ps[u][p] = v;
+        }
     }
     for (int e = tid; e < NB16; e += 32 * TB_GROUPS) y[e] = zero4;
     __syncthreads();
@@ -959,11 +976,14 @@ __device__ __forceinline__ void contract_topsym32_tile(const ContractArgs &a, in
     contract_topsym_t<float>(a, a.table32, bx, by, bz);
 }
 
-Op contract_topsym_op(const double *TS, const float *TS32, const double *Pt, double *Ptb, int numr, int numz,
-                      int mmax)
+Op contract_topsym_op(const double *TS, const float *TS32, const double *proj, const double *sqrt_rhf,
+                      const double *Pt, double *Ptb, int numr, int numz, int mmax)
 {
+    if (tb_blocked(numr)) {
+        ContractArgs a{TS, proj, Ptb, numr, numz, mmax, sqrt_rhf, TS32, 0};
+        return Op::make(OP_CONTRACT_TOPSYM, TS ? 2 : 3, a, tb_chunks(numz), mmax, 1);
+    }
     ContractArgs a{TS, Pt, Ptb, numr, numz, mmax, nullptr, TS32, 0};
-    if (tb_blocked(numr)) return Op::make(OP_CONTRACT_TOPSYM, TS ? 2 : 3, a, tb_chunks(numz), mmax, 1);
     const int NB = (numr - 2 + 15) / 16;
     return Op::make(OP_CONTRACT_TOPSYM, TS ? 0 : 1, a, (NB + 1) / 2, mmax, topsym_splits(numz));
 }

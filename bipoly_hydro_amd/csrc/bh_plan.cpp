@@ -68,7 +68,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     ALLOC(p->Stb, (size_t)mmax * 4 * nri);
     {
         const size_t NB = (nri + 15) / 16;
-        ALLOC(p->Pt, (size_t)mmax * nzi * 16 * NB * 4);
+        if (!tb_blocked(numr)) ALLOC(p->Pt, (size_t)mmax * nzi * 16 * NB * 4);  // the blocked form scales itself
         const size_t parts = tb_blocked(numr) ? tb_chunks(numz) : topsym_splits(numz) * ((NB + 1) / 2);
         ALLOC(p->Ptb, (size_t)mmax * parts * 16 * NB * 4);
     }
@@ -192,8 +192,9 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax));
     out->push_back(Op::mark(MARK_SIDE_END));
     out->push_back(Op::mark(5));
-    out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
-    out->push_back(contract_topsym_op(p->tab->T, p->tab->T32, p->Pt, p->Ptb, numr, numz, mmax));
+    if (!tb_blocked(numr)) out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
+    out->push_back(contract_topsym_op(p->tab->T, p->tab->T32, p->proj, p->tab->sqrt_rhf, p->Pt, p->Ptb, numr, numz,
+                                      mmax));
     out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
     out->push_back(Op::mark(MARK_JOIN));
