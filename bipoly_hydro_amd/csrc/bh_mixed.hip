@@ -39,7 +39,7 @@ namespace bh {
 // formed later by the inverse DFT from the two corrections leaves it out)
 // and whether X is a cold start (COLD), so that the row loop carries no
 // pointer tests.
-constexpr int RES_ROWS = 8;
+constexpr int RES_ROWS = 4;
 
 template <bool UPD, bool COLD, bool EDGE, bool WX>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz)
