@@ -23,6 +23,10 @@ struct DftArgs {
     // the mask_numr x mask_numz grid the input is xint + xadd (FP32 both,
     // each widened to FP64), and only the face points of x are read
     const float *xint;
+    // inverse at N = 512: when nonzero, the grid's numr; the axis ghost
+    // column J = 1 is then written from J = 2 of the plane rotated by pi
+    // (helmadi.F90:396-405) in place of its own transform
+    int axis_numr;
     int max_blocks;     // single launches at N = 512: cap on the persistent grid (0: one per panel)
     // forward FP32 at N = 512 with the mask: also rout[L][K-2][J-2] =
     // rscale * out for the interior points (null: not written)

@@ -694,18 +694,28 @@ __device__ __forceinline__ void inv8_work(const DftArgs &a, int bx, const unsign
     }
     const double invn = 1.0 / (double)R8N;
     double *out = (double *)a.out;
+    // with the axis ghost: a J = 2 value also goes to J = 1 of the plane
+    // rotated by pi; the J = 1 column's own transform is not written
+    const int anr = a.axis_numr;
+    auto put = [&](int row, int gc, int jm, double v) {
+        if (jm != 0) out[(long)row * a.ldo + gc] = v;
+        if (jm == 1) out[(long)((row + PN / 2) & (PN - 1)) * a.ldo + gc - 1] = v;
+    };
+    {
+        const int c = lane & 15, gc = col0 + c, jm = anr ? gc % anr : -1;
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const int r = t0 + i, c = lane & 15, gc = col0 + c, vc = r * R4CB + c;
-        const double e0 = E0[vc] * invn, eh = EH[vc] * invn;
+        for (int i = 0; i < 2; ++i) {
+            const int r = t0 + i, vc = r * R4CB + c;
+            const double e0 = E0[vc] * invn, eh = EH[vc] * invn;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int m = 16 * mt + Mma<double>::row(lane, v);
-            if (gc >= a.M) continue;
-            const double p = (e0 + ((m & 1) ? -eh : eh)) + accP[i][v];
-            const double q = accQ[i][v];
-            out[(long)(8 * m + r) * a.ldo + gc] = p - q;
-            if (m > 0) out[(long)(8 * (R8N - m) + r) * a.ldo + gc] = p + q;
+            for (int v = 0; v < 4; ++v) {
+                const int m = 16 * mt + Mma<double>::row(lane, v);
+                if (gc >= a.M) continue;
+                const double p = (e0 + ((m & 1) ? -eh : eh)) + accP[i][v];
+                const double q = accQ[i][v];
+                put(8 * m + r, gc, jm, p - q);
+                if (m > 0) put(8 * (R8N - m) + r, gc, jm, p + q);
+            }
         }
     }
     // row 8 (n/2) + r: e0 + eh + sum_k (2/n) (-1)^k a'_k, four lanes per column
@@ -717,7 +727,7 @@ __device__ __forceinline__ void inv8_work(const DftArgs &a, int bx, const unsign
         sm += __shfl_xor(sm, 2, 64);
         const int r = vc / R4CB, c = vc - r * R4CB, gc = col0 + c;
         if (j == 0 && gc < a.M)
-            out[(long)(8 * (R8N / 2) + r) * a.ldo + gc] = (E0[vc] + EH[vc]) * invn + 2.0 * invn * sm;
+            put(8 * (R8N / 2) + r, gc, anr ? gc % anr : -1, (E0[vc] + EH[vc]) * invn + 2.0 * invn * sm);
     }
 }
 

@@ -342,17 +342,20 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         // every element is read and written by the same thread
         DftArgs d{Xa, M, d_pot, M, o.dinv64, N, (int)M, numr, numz, dlt2, Xa};
         d.xint = p->D1;
+        d.axis_numr = numr;
         out->push_back(dft_op(d, DFT_INV_F64));
     } else if (o.dinv64 && p->use_dft && N == DFT_MAX_N) {
         out->push_back(Op::mark(3));
         DftArgs d{Xcur, M, d_pot, M, o.dinv64, N, (int)M, 0, 0, p->D1, Xa};
+        d.axis_numr = numr;
         out->push_back(dft_op(d, DFT_INV_F64));
     } else {
         out->push_back(add_update_op(Xcur, p->D1, Xa, numr, numz, N));  // pointwise: in place is safe
         out->push_back(Op::mark(3));
         dft_inv64_ops(p, Xa, M, (int)M, d_pot, M, out);
     }
-    out->push_back(axis_op(d_pot, numr, numz, N));
+    // the N = 512 inverse writes the axis ghost itself
+    if (!(o.dinv64 && p->use_dft && N == DFT_MAX_N)) out->push_back(axis_op(d_pot, numr, numz, N));
     out->push_back(Op::mark(4));
 }
 
