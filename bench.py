@@ -27,7 +27,7 @@ HBM_PEAK_GBS = 8000.0
 # half-step at 258 x 130 x 512, from rocprofv3 FETCH_SIZE (x 2.000, the
 # copy-kernel calibration) + WRITE_SIZE;
 # profiles/potential_solver_258x130x512_hbm_traffic.md
-CONTRACT_TRAFFIC = ((270204.0 + 2332.0) * 2.0 + (3840.0 + 240.0)) * 1024  # FP32 table
+CONTRACT_TRAFFIC = None  # FP64 table: not yet measured
 SWEEP_TRAFFIC = ((162531.0 + 144885.0) * 2.0 + (151557.0 + 138717.0)) * 1024 / 2  # mean of the two half-steps
 
 
@@ -51,17 +51,25 @@ def cpu_baseline(numr, numz, numphi, rho, threads):
                       f"tables zero-filled), {dt:.2f} s"}
 
 
-def reference_slices_error(pot):
+def reference_errors(pot, tag="1"):
+    """relL2 of a 258 x 130 x 512 potential against the reference's own
+    output kept in tests/golden/c2_258x130x512.npz: over the kept slices,
+    and over the whole-field digests (plane / K / J norms and weighted row
+    sums); tag "1" is the cold solve, "2" the warm one that follows it."""
     path = os.path.join(ROOT, "tests", "golden", "c2_258x130x512.npz")
     if not os.path.exists(path) or pot.shape != (512, 130, 258):
         return None
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_cases import c2_digests, c2_slices
     g = np.load(path)
-    # the cold-solve slices kept by tests/golden/make_golden.py
-    parts = [(pot[:, 64, :], g["mid1"]), (pot[0], g["mer1"]), (pot[::8, -1, :], g["top1"]),
-             (pot[::8, 0, :], g["bot1"]), (pot[::8, :, -1], g["side1"])]
+    parts = [(v, g[k]) for k, v in c2_slices(pot, tag).items()]
     num = sum(float(np.sum((a - b) ** 2)) for a, b in parts)
     den = sum(float(np.sum(b ** 2)) for _, b in parts)
-    return (num / den) ** 0.5
+    out = {"slices": (num / den) ** 0.5}
+    if "row_dot" + tag in g:
+        for k, v in c2_digests(pot, g["digest_weights"]).items():
+            out[k] = rell2(v, g[k + tag])
+    return out
 
 
 def main():
@@ -71,6 +79,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--grid", type=int, nargs=3, default=[258, 130, 512])
     ap.add_argument("--warm", action="store_true", help="time warm solves (tstep > 1, 5 ADI steps)")
+    ap.add_argument("--passes", type=int, default=2, help="refinement passes of the headline mixed solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batched", action="store_true",
@@ -119,16 +128,8 @@ def main():
     sp = stream.cuda_stream
     tstep = 2 if args.warm else 0
 
-    def run(precision):
-        plan = P.PotentialPlan(numr, numz, numphi, precision=precision)
-        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
-        torch.cuda.synchronize()
-        pot_cold = d_pot.cpu().numpy()
-        for _ in range(args.warmup):
-            plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
-        torch.cuda.synchronize()
-        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
+    def timed(plan, tstep, steps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for a, b in ev:
@@ -137,6 +138,25 @@ def main():
             b.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        lat = np.array([a.elapsed_time(b) for a, b in ev])
+        return {"ms_per_step": wall * 1e3 / steps, "solves_per_s": steps / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+
+    def run(precision, passes=2, warm_steps=0):
+        plan = P.PotentialPlan(numr, numz, numphi, precision=precision, refine_passes=passes)
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
+        torch.cuda.synchronize()
+        pot_cold = d_pot.cpu().numpy()
+        # the warm solve that follows a cold one (tstep 2, the reference's
+        # 5 ADI steps from the kept potential), on a perturbed density
+        d_rho.mul_(1.01).add_(d_rho2, alpha=1e-3)
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 2, sp)
+        torch.cuda.synchronize()
+        pot_warm = d_pot.cpu().numpy()
+        d_rho.copy_(d_rho1)
+        for _ in range(args.warmup):
+            plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
+        res = timed(plan, tstep, args.steps)
         # stage and kernel times: a second timed region of the same solves in
         # which the library records HIP events on the solve's stream at the
         # stage boundaries and around the boundary contraction (kept out of
@@ -145,30 +165,52 @@ def main():
         for _ in range(args.steps):
             plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep, sp)
         torch.cuda.synchronize()
-        stages = plan.last_timings()
+        res["stage_ms"] = plan.last_timings()
         plan.set_timing(False)
-        lat = np.array([a.elapsed_time(b) for a, b in ev])
+        if warm_steps:
+            w = timed(plan, 2, warm_steps)
+            res["warm"] = {k: w[k] for k in ("ms_per_step", "p50_ms", "p99_ms")}
+        res["error_vs_reference_cold"] = reference_errors(pot_cold, "1")
+        res["error_vs_reference_warm"] = reference_errors(pot_warm, "2")
         plan.close()
-        return {"ms_per_step": wall * 1e3 / args.steps, "solves_per_s": args.steps / wall,
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "stage_ms": stages, "relL2_vs_reference_slices": reference_slices_error(pot_cold)}, pot_cold
+        return res, pot_cold
 
-    fp64, pot64 = run("fp64")
-    mixed, potmx = run("mixed")
+    d_rho1 = d_rho.clone()
+    if (numr, numz, numphi) == (258, 130, 512):
+        d_rho2 = torch.from_numpy(density.config2_density(seed=99)).to(dev)
+    else:
+        d_rho2 = torch.from_numpy(density.scaled_binary(numr, numz, numphi, seed=99)).to(dev)
+    fp64, pot64 = run("fp64", warm_steps=args.steps)
+    mixed, potmx = run("mixed", passes=args.passes, warm_steps=args.steps)
     mixed["relL2_vs_fp64_gpu"] = rell2(potmx, pot64)
     ms_per_step = mixed["ms_per_step"]
     lat_p50, lat_p99 = mixed["p50_ms"], mixed["p99_ms"]
+    # the error after each refinement pass (the solve with that many passes),
+    # its time, and the opt-in FP32-table variant at one pass
+    per_pass = []
+    for npass in (1, 2, 3, 4):
+        plan = P.PotentialPlan(numr, numz, numphi, precision="mixed", refine_passes=npass)
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
+        torch.cuda.synchronize()
+        pot = d_pot.cpu().numpy()
+        t = timed(plan, 0, max(10, args.steps // 5))
+        plan.close()
+        per_pass.append({"passes": npass, "ms_per_solve": t["ms_per_step"],
+                         "error_vs_reference": reference_errors(pot, "1"), "relL2_vs_fp64_gpu": rell2(pot, pot64)})
+    t32, pot32 = run("mixed_t32", passes=1)
+    t32 = {"ms_per_step": t32["ms_per_step"], "error_vs_reference_cold": t32["error_vs_reference_cold"],
+           "precision": "FP32 top/bottom Green's table (opt-in), 1 refinement pass"}
 
     # Dominant kernel of the measured (mixed) solve: the top/bottom boundary
-    # contraction, bound by reading the Green's table once (FP32 in the mixed
-    # solve).  The table is stored in blocked symmetric form (16 x 16 blocks
-    # (I, J), J <= I, of targets x sources per source plane and mode): mmax x
-    # NB(NB+1)/2 x (numz-2) x 256 x 4 B, NB = ceil((numr-2)/16).  The timed
+    # contraction, bound by reading the FP64 Green's table once.  The table
+    # is stored in blocked symmetric form (16 x 16 blocks (I, J), J <= I, of
+    # targets x sources per source plane and mode): mmax x NB(NB+1)/2 x
+    # (numz-2) x 256 x 8 B, NB = ceil((numr-2)/16).  The timed
     # span (events around it) also holds the projection scaling and the final
     # reduction.
     nri, nzi, mmax = numr - 2, numz - 2, 30
     nb = (nri + 15) // 16
-    ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 4
+    ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 8
     ct_ms = mixed["stage_ms"]["contract_topbot"]
     ct_traffic = CONTRACT_TRAFFIC if (numr, numz, numphi) == (258, 130, 512) else None
     # ADI half-step kernel of the FP64 path: 3 streams (read phi, read
@@ -200,12 +242,18 @@ def main():
         "config": {"workload": f"potential_solver {'warm (5 ADI steps)' if args.warm else 'cold (20 ADI steps)'}"
                                f" {numr}x{numz}x{numphi}, isym=1, mmax=30, binary n=1 polytropes, "
                                "inputs resident in HBM",
-                   "grid": [numr, numz, numphi], "precision": "mixed (FP32 MFMA transforms, FP32 "
-                   "top/bottom Green's table, FP64 sums, residual, 1 refinement pass)", "tstep": tstep},
+                   "grid": [numr, numz, numphi],
+                   "precision": f"mixed: FP32 MFMA DFT of rho, FP32 spectral ADI (sine transforms, split-bf16 "
+                                f"M_r filter, Thomas sweeps), FP64 Green's tables, FP64 residual and "
+                                f"{args.passes} refinement passes, FP64 inverse DFT", "tstep": tstep},
         "p50_ms": lat_p50,
         "p99_ms": lat_p99,
-        "relL2_vs_reference_slices": mixed["relL2_vs_reference_slices"],
+        "relL2_vs_reference_slices": (mixed["error_vs_reference_cold"] or {}).get("slices"),
+        "error_per_pass": per_pass,
         "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+        "warm": {"mixed_ms": mixed["warm"]["ms_per_step"], "fp64_ms": fp64["warm"]["ms_per_step"],
+                 "mixed_over_fp64": fp64["warm"]["ms_per_step"] / mixed["warm"]["ms_per_step"]},
+        "mixed_table32": t32,
         "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (contract with the projections scaled as read, then reduce)",
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,

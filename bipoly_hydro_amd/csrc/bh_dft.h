@@ -32,6 +32,10 @@ struct DftArgs {
     // rscale * out for the interior points (null: not written)
     float *rout;
     double rscale;
+    // inverse at N = 512 with xint and xadd and warm != 0: the input is
+    // x + (xint + xadd) at every point (the FP32 corrections are zero on
+    // the faces), and that sum is also stored to xstore
+    int warm;
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };

@@ -470,9 +470,11 @@ constexpr int R8_FRAGS = 4 * R8KG * 64 * 4;  // doubles of the size-n P and Q fr
 // Input modes of the inverse: plain x; x + xadd (the FP32 correction
 // loaded beside x); a cold start, where an interior column's register pair
 // holds the two FP32 values xint, xadd (bit-packed) and a face column's the
-// FP64 x (xadd is zero on the faces).  Each mode is its own kernel, so that
-// the registers held across a panel are only those the mode needs.
-enum { INV_PLAIN = 0, INV_ADD = 1, INV_COLD = 2 };
+// FP64 x (xadd is zero on the faces); a warm start with two corrections,
+// x + (xint + xadd), the pair bit-packed beside x.  Each mode is its own
+// kernel, so that the registers held across a panel are only those the
+// mode needs.
+enum { INV_PLAIN = 0, INV_ADD = 1, INV_COLD = 2, INV_ADD2 = 3 };
 template <int MODE>
 struct Inv8Regs {
     double av[8], sv[8];
@@ -482,6 +484,7 @@ struct Inv8Regs<INV_ADD> {
     double av[8], sv[8];
     float fa[8], fs[8];
 };
+
 typedef float inv_f2 __attribute__((ext_vector_type(2)));
 // a cold start's interior column (2 <= J <= numr - 1, 2 <= K <= numz - 1)
 __device__ __forceinline__ bool inv8_inner(const DftArgs &a, int gc)
@@ -594,12 +597,31 @@ __device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Re
             }
         }
     }
+    // the warm start's two corrections are loaded here, not with the
+    // panel's prefetch: held across the previous panel's products they do
+    // not fit in the registers beside x
+    float pa[8], pb[8], sa[8], sb[8];
+    if constexpr (MODE == INV_ADD2) {
+        const int gcl = min(gc, a.M - 1);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const long o0 = (long)inv8_slot_row(kk, q, 0) * ldx + gcl, o1 = (long)inv8_slot_row(kk, q, 1) * ldx + gcl;
+            pa[q] = a.xint[o0];
+            pb[q] = a.xadd[o0];
+            sa[q] = a.xint[o1];
+            sb[q] = a.xadd[o1];
+        }
+    }
     if constexpr (MODE != INV_PLAIN) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             if constexpr (MODE == INV_ADD) {
                 av[q] += (double)R.fa[q];
                 sv[q] += (double)R.fs[q];
+            }
+            if constexpr (MODE == INV_ADD2) {
+                av[q] += (double)pa[q] + (double)pb[q];
+                sv[q] += (double)sa[q] + (double)sb[q];
             }
             if (gc < a.M) {
                 a.xstore[(long)inv8_slot_row(kk, q, 0) * ldx + gc] = av[q];
@@ -1008,7 +1030,9 @@ Op dft_op(const DftArgs &a, int kind)
     // N = 512: the radix kernels' panels (the FP32 forward's are twice as wide)
     const int CB = a.N == DFT_MAX_N ? (kind == DFT_FWD_F32 ? R4Fwd<float>::CB : R4CB) : dft_cols(kind);
     // variant: kind + 4 log2 N + 64 (input mode of the inverse at N = 512)
-    const int mode = kind == DFT_INV_F64 && a.N == DFT_MAX_N ? (a.xint ? INV_COLD : a.xadd ? INV_ADD : INV_PLAIN) : 0;
+    const int mode = kind == DFT_INV_F64 && a.N == DFT_MAX_N
+                         ? (a.xint ? (a.warm ? INV_ADD2 : INV_COLD) : a.xadd ? INV_ADD : INV_PLAIN)
+                         : 0;
     return Op::make(OP_DFT, kind + 4 * log2i(a.N) + 64 * mode, a, (a.M + CB - 1) / CB, 1, 1);
 }
 
@@ -1032,7 +1056,8 @@ static const void *r4_fn(int variant, bool grouped, int *lds)
         switch (variant / 64) {
         case INV_PLAIN: return grouped ? (const void *)&dft_inv8_grouped<0> : (const void *)&dft_inv8_kernel<0>;
         case INV_ADD: return grouped ? (const void *)&dft_inv8_grouped<1> : (const void *)&dft_inv8_kernel<1>;
-        default: return grouped ? (const void *)&dft_inv8_grouped<2> : (const void *)&dft_inv8_kernel<2>;
+        case INV_COLD: return grouped ? (const void *)&dft_inv8_grouped<2> : (const void *)&dft_inv8_kernel<2>;
+        default: return grouped ? (const void *)&dft_inv8_grouped<3> : (const void *)&dft_inv8_kernel<3>;
         }
     case DFT_FWD_F32:
         *lds = (int)R4Fwd<float>::bytes;

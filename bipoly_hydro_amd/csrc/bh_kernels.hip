@@ -353,50 +353,27 @@ Op axis_op(double *pot, int numr, int numz, int numphi)
 // hypergeometric series truncated after nser terms, here summed by Horner's
 // rule in 1/chi^2 with coefficients exp(dcoefh) prepared on the host.
 // ---------------------------------------------------------------------------
-__device__ double d_rf(double x, double y, double z)
+// Complete elliptic integrals of the first and second kind from the
+// arithmetic-geometric mean (Gauss; Abramowitz & Stegun 17.6.1-17.6.4):
+// a0 = 1, b0 = k' = sqrt(1 - k^2), c0 = k; a_{n+1} = (a_n + b_n) / 2,
+// b_{n+1} = sqrt(a_n b_n), c_{n+1} = (a_n - b_n) / 2; K = pi / (2 a_inf),
+// E = K (1 - sum_n 2^(n-1) c_n^2).  kp2 = 1 - k^2 is given separately so
+// that it keeps its digits as k -> 1.  Convergence is quadratic: once
+// c <= 1e-8 a, a is within ~c^2 / (4a) <= 3e-17 a of the limit and the
+// terms left of the sum are below 1e-32.
+__device__ void d_ellke(double k2, double kp2, double *K, double *E)
 {
-    const double errtol = 0.0025;
-    double xt = x, yt = y, zt = z, ave, dx, dy, dz;
-    for (int it = 0; it < 64; ++it) {
-        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
-        double lam = sx * (sy + sz) + sy * sz;
-        xt = 0.25 * (xt + lam);
-        yt = 0.25 * (yt + lam);
-        zt = 0.25 * (zt + lam);
-        ave = (1.0 / 3.0) * (xt + yt + zt);
-        dx = (ave - xt) / ave;
-        dy = (ave - yt) / ave;
-        dz = (ave - zt) / ave;
-        if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) <= errtol) break;
+    double a = 1.0, b = sqrt(kp2), s = 0.5 * k2, w = 0.5;
+    for (int n = 0; n < 16; ++n) {
+        const double c = 0.5 * (a - b), an = 0.5 * (a + b);
+        b = sqrt(a * b);
+        a = an;
+        w += w;
+        s += w * c * c;
+        if (c <= 1e-8 * a) break;
     }
-    double e2 = dx * dy - dz * dz, e3 = dx * dy * dz;
-    return (1.0 + ((1.0 / 24.0) * e2 - 0.1 - (3.0 / 44.0) * e3) * e2 + (1.0 / 14.0) * e3) / sqrt(ave);
-}
-
-__device__ double d_rd(double x, double y, double z)
-{
-    const double errtol = 0.0015;
-    double xt = x, yt = y, zt = z, sum = 0.0, fac = 1.0, ave, dx, dy, dz;
-    for (int it = 0; it < 64; ++it) {
-        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
-        double lam = sx * (sy + sz) + sy * sz;
-        sum += fac / (sz * (zt + lam));
-        fac *= 0.25;
-        xt = 0.25 * (xt + lam);
-        yt = 0.25 * (yt + lam);
-        zt = 0.25 * (zt + lam);
-        ave = 0.2 * (xt + yt + 3.0 * zt);
-        dx = (ave - xt) / ave;
-        dy = (ave - yt) / ave;
-        dz = (ave - zt) / ave;
-        if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) <= errtol) break;
-    }
-    const double c1 = 3.0 / 14.0, c2 = 1.0 / 6.0, c3 = 9.0 / 22.0, c4 = 3.0 / 26.0;
-    double ea = dx * dy, eb = dz * dz, ec = ea - eb, ed = ea - 6.0 * eb, ee = ed + ec + ec;
-    double c5 = 0.25 * c3, c6 = 1.5 * c4;
-    return 3.0 * sum + fac * (1.0 + ed * (-c1 + c5 * ed - c6 * dz * ee) +
-                              dz * (c2 * ee + dz * (-c3 * ec + dz * c4 * ea))) /
-                           (ave * sqrt(ave));
+    *K = 1.5707963267948966192 / a;
+    *E = *K * (1.0 - s);
 }
 
 // Writes coef * Q_{m-1/2}(xm) to out[m * mstride], m = 0..mmax-1 (rounded
@@ -408,9 +385,9 @@ __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, TT *ou
     if (xm < a.hypr_cutoff) {
         double mum = sqrt(2.0 / (1.0 + xm));
         double lam = sqrt(2.0 * (1.0 + xm));
-        double kq = (1.0 - mum) * (1.0 + mum);
-        double kk = d_rf(0.0, kq, 1.0);
-        double ee = kk - mum * mum * d_rd(0.0, kq, 1.0) * (1.0 / 3.0);
+        // K(k), E(k) with k = mum (tm.F90:170-171 takes them from ellf, elle)
+        double kk, ee;
+        d_ellke(mum * mum, (1.0 - mum) * (1.0 + mum), &kk, &ee);
         double q0 = kk * mum;
         out[0] = (TT)(coef * q0);
         if (mmax < 2) return;

@@ -27,20 +27,26 @@ inline double sine_basis(int a, int k, int n)
 }
 
 // dlt and w are given together (the refinement pass; Xo = X + dlt is then
-// written unless Xo is null) or both null (the first residual)
+// written unless Xo is null) or both null (the first residual).  dlt2 (with
+// dlt, and Xo null) is a second FP32 correction: x = X + (dlt + dlt2).
 struct ResidualArgs {
     const double *X; const float *dlt; const float *w; const float *rh;
     double *Xo; float *r;
+    const float *dlt2;
     const double *brb, *alphar, *betar;
     double gamma, four_pi;
     int numr, numz;
     int cold;  // X is zero on the interior (a cold start): read it only on the faces
 };
+// Xo = X + (dlt + dlt2) on the interior (dlt2 may be null); cold: X is zero
+// on the interior and is not read
 struct UpdateArgs {
     const double *X;
     const float *dlt;
     double *Xo;
     int numr, numz;
+    const float *dlt2;
+    int cold;
 };
 struct TridiagArgs {
     const float *cin;
@@ -55,7 +61,8 @@ Op residual_op(const ResidualArgs &a, int nplanes);
 // next to a face (rows K = 2 and numz - 1, column J = numr - 1), the only
 // ones where A x0 is not zero.  Same arithmetic as residual_op, cold.
 Op residual_face_op(const ResidualArgs &a, int nplanes);
-Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes);
+Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes,
+                 const float *dlt2 = nullptr, int cold = 0);
 Op tridiag_shift_op(const float *cin, float *yout, const float *ibet, const float *gam, const double *betar, int nri,
                     int nzi, int nplanes);
 

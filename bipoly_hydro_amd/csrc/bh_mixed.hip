@@ -35,13 +35,13 @@ namespace bh {
 // Tile: 256 consecutive J of one plane and RES_ROWS rows of K, the column's
 // rows K0-1 .. K0+RES_ROWS and the J coefficients loaded once.
 // The variant fixes at compile time whether delta and w are present (UPD:
-// the refinement pass), whether Xo is written (WX; a cold start whose x is
-// formed later by the inverse DFT from the two corrections leaves it out)
-// and whether X is a cold start (COLD), so that the row loop carries no
-// pointer tests.
+// the refinement pass), whether Xo is written (WX; a solve whose x is
+// formed later by the inverse DFT from the corrections leaves it out),
+// whether a second correction is added (ADD2) and whether X is a cold start
+// (COLD), so that the row loop carries no pointer tests.
 constexpr int RES_ROWS = 4;
 
-template <bool UPD, bool COLD, bool EDGE, bool WX>
+template <bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz)
 {
     const int numr = a.numr, nri = numr - 2, nzi = a.numz - 2;
@@ -53,15 +53,19 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     const int K0 = by * RES_ROWS + 2, K1 = min(K0 + RES_ROWS, nzi + 2);
     const double *__restrict__ X = a.X + L * plane;
     const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
+    const float *__restrict__ dl2 = ADD2 ? a.dlt2 + L * plane : nullptr;
     const float *__restrict__ rh = a.rh + L * plane;
     const float *__restrict__ w = UPD ? a.w + L * plane : nullptr;
     double *__restrict__ Xo = WX ? a.Xo + L * plane : nullptr;
     float *__restrict__ r32 = a.r + (long)L * nri * nzi;
     const double brbJ = a.brb[(long)L * (numr + 1) + J], aJ = a.alphar[J], bJ = a.betar[J];
-    // cold start: the interior of X is zero and dlt is zero on the faces, so
-    // interior values are dlt alone and face values X alone
-    auto val = [&](long i) { return UPD ? X[i] + (double)dl[i] : X[i]; };
-    auto inner = [&](long i) { return COLD ? (UPD ? (double)dl[i] : 0.0) : val(i); };
+    // cold start: the interior of X is zero and dlt (dlt2) is zero on the
+    // faces, so interior values are the corrections alone and face values X
+    // alone.  Two corrections are summed first (exact in FP64 for FP32
+    // values this close), as the inverse DFT sums them.
+    auto corr = [&](long i) { return ADD2 ? (double)dl[i] + (double)dl2[i] : (double)dl[i]; };
+    auto val = [&](long i) { return UPD ? X[i] + corr(i) : X[i]; };
+    auto inner = [&](long i) { return COLD ? (UPD ? corr(i) : 0.0) : val(i); };
     auto face = [&](long i) { return COLD ? X[i] : val(i); };
     // the lane-dependent cases (the east face at J = numr - 1, no west term
     // at J = 2) are selects, not branches; every address read is inside the
@@ -129,13 +133,15 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
 template <bool EDGE>
 __device__ __forceinline__ void residual_tile_e(const ResidualArgs &a, int bx, int by, int bz)
 {
-    switch ((a.dlt ? (a.Xo ? 2 : 4) : 0) + (a.cold ? 1 : 0)) {
+    switch ((a.dlt ? (a.Xo ? 2 : a.dlt2 ? 6 : 4) : 0) + (a.cold ? 1 : 0)) {
     case 0: residual_tile_v<false, false, EDGE, false>(a, bx, by, bz); break;
     case 1: residual_tile_v<false, true, EDGE, false>(a, bx, by, bz); break;
     case 2: residual_tile_v<true, false, EDGE, true>(a, bx, by, bz); break;
     case 3: residual_tile_v<true, true, EDGE, true>(a, bx, by, bz); break;
     case 4: residual_tile_v<true, false, EDGE, false>(a, bx, by, bz); break;
-    default: residual_tile_v<true, true, EDGE, false>(a, bx, by, bz); break;
+    case 5: residual_tile_v<true, true, EDGE, false>(a, bx, by, bz); break;
+    case 6: residual_tile_v<true, false, EDGE, false, true>(a, bx, by, bz); break;
+    default: residual_tile_v<true, true, EDGE, false, true>(a, bx, by, bz); break;
     }
 }
 __device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
@@ -185,7 +191,8 @@ Op residual_face_op(const ResidualArgs &a, int nplanes)
     return Op::make(OP_RESIDUAL_FACE, 0, a, (np + 255) / 256, nplanes, 1);
 }
 
-// Xo = X + dlt on the interior (faces of Xo are kept)
+// Xo = X + (dlt + dlt2) on the interior (faces of Xo are kept); a cold X is
+// zero on the interior and is not read
 __device__ __forceinline__ void add_update_tile(const UpdateArgs &a, int bx, int by, int)
 {
     const int numr = a.numr, numz = a.numz;
@@ -196,12 +203,14 @@ __device__ __forceinline__ void add_update_tile(const UpdateArgs &a, int bx, int
     if (t >= nri * nzi) return;
     int jr = t % nri, kz = t / nri;
     long c = L * plane + (long)(kz + 1) * numr + (jr + 1);
-    a.Xo[c] = a.X[c] + (double)a.dlt[c];
+    const double d = a.dlt2 ? (double)a.dlt[c] + (double)a.dlt2[c] : (double)a.dlt[c];
+    a.Xo[c] = a.cold ? d : a.X[c] + d;
 }
 
-Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes)
+Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes, const float *dlt2,
+                 int cold)
 {
-    UpdateArgs a{X, dlt, Xo, numr, numz};
+    UpdateArgs a{X, dlt, Xo, numr, numz, dlt2, cold};
     int n = (numr - 2) * (numz - 2);
     return Op::make(OP_UPDATE, 0, a, (n + 255) / 256, nplanes, 1);
 }

@@ -40,6 +40,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     p->budget = budget;
     p->g = h.g;
     p->device = device;
+    p->passes = d.refine_passes;
     // GEMM tile shapes, overridable for tuning sweeps
     if (const char *e = std::getenv("BH_GEMM_TILE_DFT")) p->tile_dft = std::atoi(e);
     if (const char *e = std::getenv("BH_GEMM_TILE_SPEC")) p->tile_spec = std::atoi(e);
@@ -74,7 +75,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     }
     ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
     ALLOC(p->FF, (size_t)N * nf);
-    if (precision == BH_PREC_MIXED) {
+    const bool with_d2 = is_mixed(precision) && p->ops->spec_fused && p->passes >= 2;
+    if (is_mixed(precision)) {
         const size_t PI = (size_t)N * nri * nzi;
         ALLOC(p->RH32, M * N);
         ALLOC(p->r32, PI);
@@ -84,8 +86,10 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
             ALLOC(p->DT, 2 * PI);  // DT then WT, so that one product maps both back
             p->WT = p->DT + PI;
         }
-        ALLOC(p->D1, 2 * M * N);  // D1 then Wf
+        // D1 then Wf (then D2, the summed refinement corrections); faces zero
+        ALLOC(p->D1, (with_d2 ? 3 : 2) * M * N);
         p->Wf = p->D1 + M * N;
+        if (with_d2) p->D2 = p->Wf + M * N;
     } else {
         ALLOC(p->RJ, M * N);
         ALLOC(p->RK, M * N);
@@ -100,8 +104,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     launch_fill(p->XJ, M * N, 0.0, s);
     launch_fill(p->XK, M * N, 0.0, s);
-    if (precision == BH_PREC_MIXED) {
-        launch_fillf(p->D1, 2 * M * N, 0.f, s);
+    if (is_mixed(precision)) {
+        launch_fillf(p->D1, (with_d2 ? 3 : 2) * M * N, 0.f, s);
     }
     hipError_t e = hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
@@ -285,21 +289,51 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         a.col_fastest = p->order_spec;
         out->push_back(gemm_op(a, G32_DST_INV));
     };
-    double *Xcur = Xa, *Xnext = Xb;
-    // a cold start with one refinement pass at N = 512: x = faces + delta1
-    // + delta2 is formed by the inverse DFT itself (two FP32 reads in place
-    // of the FP64 x the refinement residual would write and the DFT read);
-    // delta2 goes to the w buffer, free once the refinement residual has
-    // read it
-    const bool cold_fused = tstep <= 1 && p->passes == 1 && o.spec_fused && o.dinv64 && p->use_dft &&
-                            N == DFT_MAX_N;
-    float *dlt2 = cold_fused ? p->Wf : p->D1;
+    const bool inv512 = o.dinv64 && p->use_dft && N == DFT_MAX_N;
+    const bool cold = tstep <= 1;
     if (o.spec_fused) {
-        // 4-5 in one kernel per mode, the block kept in LDS
+        // 4-5 in one kernel per mode, the block kept in LDS: delta1 into D1,
+        // w into Wf
         SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.betar, o.alphar, p->D1, p->Wf,
-                    numr, numz, N};
+                    numr, numz, N, 0};
         out->push_back(spectral_op(sa, N, nzi));
+        // 6. refinement passes.  The iterate is never written: pass k forms
+        //    x_k = X0 + (delta1 + Dacc) in its residual (X0's faces alone for
+        //    a cold start, whose X0 is zero inside), r_k = g - w - A x_k in
+        //    FP64, and adds the next correction to Dacc in FP32.  Every
+        //    correction after the first is about 1e-7 of x, so their FP32 sum
+        //    holds them to ~1e-14 of x.  With one pass Dacc is the w buffer,
+        //    free once the one refinement residual has read w.
+        float *Dacc = p->passes >= 2 ? p->D2 : p->Wf;
+        for (int it = 0; it < p->passes; ++it) {
+            ra.X = Xa; ra.dlt = p->D1; ra.dlt2 = it > 0 ? Dacc : nullptr; ra.w = p->Wf; ra.Xo = nullptr;
+            ra.r = p->r32; ra.cold = cold;
+            out->push_back(residual_op(ra, N));
+            SpecArgs sr{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, Dacc, nullptr,
+                        numr, numz, N, it > 0 ? 1 : 0};
+            out->push_back(spectral_op(sr, N, nzi));
+        }
+        // 7. x = X0 + (delta1 + Dacc) into Xa, the warm-start state, then the
+        //    inverse DFT (FP64) and the axis ghost; at N = 512 the panel DFT
+        //    does all three in one pass (in place is safe: every element of
+        //    Xa is read and written by the same thread)
+        if (inv512) {
+            out->push_back(Op::mark(3));
+            DftArgs d{Xa, M, d_pot, M, o.dinv64, N, (int)M, numr, numz, Dacc, Xa};
+            d.xint = p->D1;
+            d.warm = cold ? 0 : 1;
+            d.axis_numr = numr;
+            out->push_back(dft_op(d, DFT_INV_F64));
+        } else {
+            out->push_back(add_update_op(Xa, p->D1, Xa, numr, numz, N, Dacc, cold ? 1 : 0));
+            out->push_back(Op::mark(3));
+            dft_inv64_ops(p, Xa, M, (int)M, d_pot, M, out);
+            out->push_back(axis_op(d_pot, numr, numz, N));
+        }
     } else {
+        // the separate transform, filter and Thomas kernels (grid sizes the
+        // fused step does not take): each pass writes its iterate
+        double *Xcur = Xa, *Xnext = Xb;
         dst_fwd(p->r32, p->RT);
         // 4. W = mu_z . (M_r RT), C = RT - W
         {
@@ -317,45 +351,30 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         //    and w = S W in one launch: WT follows DT and Wf follows D1
         out->push_back(tridiag_shift_op(p->CT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
         dst_inv(p->DT, p->D1, 2 * N);
-    }
-    // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
-    for (int it = 0; it < p->passes; ++it) {
-        ra.X = Xcur; ra.dlt = p->D1; ra.w = p->Wf; ra.Xo = cold_fused ? nullptr : Xnext; ra.r = p->r32;
-        ra.cold = (tstep <= 1 && it == 0);  // Xcur = Xa until the first pass writes Xnext
-        out->push_back(residual_op(ra, N));
-        if (o.spec_fused) {
-            SpecArgs sa{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, dlt2, nullptr,
-                        numr, numz, N};
-            out->push_back(spectral_op(sa, N, nzi));
-        } else {
+        // 6. refinement: x += delta, r = g - w - A x (FP64), solve again
+        for (int it = 0; it < p->passes; ++it) {
+            ra.X = Xcur; ra.dlt = p->D1; ra.dlt2 = nullptr; ra.w = p->Wf; ra.Xo = Xnext; ra.r = p->r32;
+            ra.cold = (cold && it == 0);  // Xcur = Xa until the first pass writes Xnext
+            out->push_back(residual_op(ra, N));
             dst_fwd(p->r32, p->RT);
             out->push_back(tridiag_shift_op(p->RT, p->DT, o.tri_ibet, o.tri_gam, o.betar, nri, nzi, N));
             dst_inv(p->DT, p->D1, N);
+            std::swap(Xcur, Xnext);
         }
-        std::swap(Xcur, Xnext);
+        // 7. x = Xcur + D1 into Xa, the warm-start state, then the inverse DFT
+        //    (FP64) and the axis ghost; the panel DFT does both in one pass
+        if (inv512) {
+            out->push_back(Op::mark(3));
+            DftArgs d{Xcur, M, d_pot, M, o.dinv64, N, (int)M, 0, 0, p->D1, Xa};
+            d.axis_numr = numr;
+            out->push_back(dft_op(d, DFT_INV_F64));
+        } else {
+            out->push_back(add_update_op(Xcur, p->D1, Xa, numr, numz, N));  // pointwise: in place is safe
+            out->push_back(Op::mark(3));
+            dft_inv64_ops(p, Xa, M, (int)M, d_pot, M, out);
+            out->push_back(axis_op(d_pot, numr, numz, N));
+        }
     }
-    // 7. x = Xcur + D1 into Xa, the warm-start state, then the inverse DFT
-    //    (FP64) and the axis ghost; the panel DFT does both in one pass
-    if (cold_fused) {
-        out->push_back(Op::mark(3));
-        // Xa holds the faces (its interior is not read); in place is safe:
-        // every element is read and written by the same thread
-        DftArgs d{Xa, M, d_pot, M, o.dinv64, N, (int)M, numr, numz, dlt2, Xa};
-        d.xint = p->D1;
-        d.axis_numr = numr;
-        out->push_back(dft_op(d, DFT_INV_F64));
-    } else if (o.dinv64 && p->use_dft && N == DFT_MAX_N) {
-        out->push_back(Op::mark(3));
-        DftArgs d{Xcur, M, d_pot, M, o.dinv64, N, (int)M, 0, 0, p->D1, Xa};
-        d.axis_numr = numr;
-        out->push_back(dft_op(d, DFT_INV_F64));
-    } else {
-        out->push_back(add_update_op(Xcur, p->D1, Xa, numr, numz, N));  // pointwise: in place is safe
-        out->push_back(Op::mark(3));
-        dft_inv64_ops(p, Xa, M, (int)M, d_pot, M, out);
-    }
-    // the N = 512 inverse writes the axis ghost itself
-    if (!(o.dinv64 && p->use_dft && N == DFT_MAX_N)) out->push_back(axis_op(d_pot, numr, numz, N));
     out->push_back(Op::mark(4));
 }
 
@@ -442,7 +461,7 @@ int collect_timing(bh_plan *p)
 void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out)
 {
     out->clear();
-    if (p->precision == BH_PREC_MIXED) mixed_ops(p, d_rho, d_pot, tstep, out);
+    if (is_mixed(p->precision)) mixed_ops(p, d_rho, d_pot, tstep, out);
     else fp64_ops(p, d_rho, d_pot, tstep, out);
 }
 

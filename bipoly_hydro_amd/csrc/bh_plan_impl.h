@@ -55,6 +55,8 @@ struct HostGeo {
 };
 // Fills defaults (zeros take the reference's values) and validates.
 int normalize_desc(const bh_grid_desc *in, int precision, bh_grid_desc *out);
+// The mixed-precision solve (BH_PREC_MIXED or its FP32-table variant).
+inline bool is_mixed(int precision) { return precision == BH_PREC_MIXED || precision == BH_PREC_MIXED_TABLE32; }
 void host_geometry(const bh_grid_desc &d, HostGeo *h);
 
 // Green's tables of one (numr, numz, dr, mmax) geometry on one device
@@ -62,7 +64,7 @@ void host_geometry(const bh_grid_desc &d, HostGeo *h);
 struct GeoTables {
     DevMem mem;
     double *T = nullptr;         // top (and mirrored bottom) table, blocked symmetric [m][pair][kz][16][16]
-    float *T32 = nullptr;        // ... the same in FP32 (mixed plans; T is then null)
+    float *T32 = nullptr;        // ... the same in FP32 (BH_PREC_MIXED_TABLE32 plans; T is then null)
     double *S = nullptr;         // side table, Toeplitz in z [m][d][jr]
     double *sqrt_rhf = nullptr;  // sqrt(rhf) [numr + 1], 1-based J
 };
@@ -122,8 +124,9 @@ struct bh_plan {
     double *Pt = nullptr, *Ptb = nullptr;  // scaled projections, contraction partials (top/bottom)
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
     float *D1 = nullptr, *Wf = nullptr;
+    float *D2 = nullptr;  // sum of the refinement corrections (mixed, fused spectral step, passes >= 2)
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
-    int passes = 1;                                // FP64 refinement passes (mixed)
+    int passes = 2;                                // FP64 refinement passes (mixed; desc.refine_passes)
     int tile_dft = 2, tile_spec = 2, tile_dstf = 2, tile_dsti = 2;  // GEMM tile shapes, measured best
     int order_dft = 0, order_spec = 0;             // GEMM tile numbering (GemmArgs::col_fastest)
     bool use_dft = true;                           // panel DFT kernel (bh_dft.hip) when numphi allows

@@ -57,6 +57,7 @@ int normalize_desc(const bh_grid_desc *desc, int precision, bh_grid_desc *out)
     if (d.dr == 0.0) d.dr = 1.0 / (130.0 - 3.0);      // oldnumr = 130 (runhydro.h:4)
     if (d.nsteps_cold == 0) d.nsteps_cold = 20;       // potential_solver.F90:88-93
     if (d.nsteps_warm == 0) d.nsteps_warm = 5;
+    if (d.refine_passes == 0) d.refine_passes = 2;
     if (d.isym != 1) return fail(BH_E_INVALID, "isym: only 1 (no symmetry) is supported");
     if (d.numr < 5 || d.numz < 5) return fail(BH_E_INVALID, "numr and numz must be >= 5");
     if (d.numphi < 4 || (d.numphi & (d.numphi - 1))) return fail(BH_E_INVALID, "numphi must be a power of two >= 4");
@@ -65,8 +66,9 @@ int normalize_desc(const bh_grid_desc *desc, int precision, bh_grid_desc *out)
     if (d.numr - 2 > 512 || d.numz - 2 > 512) return fail(BH_E_INVALID, "numr-2 and numz-2 must be <= 512");
     if (d.nsteps_cold < 2 || d.nsteps_warm < 2) return fail(BH_E_INVALID, "nsteps must be >= 2");
     if (!(d.dr > 0.0) || !std::isfinite(d.dr)) return fail(BH_E_INVALID, "dr must be positive");
-    if (precision != BH_PREC_FP64 && precision != BH_PREC_MIXED)
-        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64 or BH_PREC_MIXED");
+    if (d.refine_passes < 1 || d.refine_passes > 4) return fail(BH_E_INVALID, "refine_passes must be in [1, 4]");
+    if (precision != BH_PREC_FP64 && !is_mixed(precision))
+        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64, BH_PREC_MIXED or BH_PREC_MIXED_TABLE32");
     *out = d;
     return BH_OK;
 }
@@ -180,11 +182,13 @@ std::mutex g_cache_mu;
 std::map<std::string, std::weak_ptr<const GeoTables>> g_tables;
 std::map<std::string, std::weak_ptr<const ModeOps>> g_modeops;
 
-// mixed plans contract an FP32 copy of the top table: its rounding (2^-24
-// relative per entry, independent) moves the potential by about 1e-10
-// relative, far inside the mixed tolerance, and halves the bytes of the
-// HBM-bound contraction
-bool table32(int precision) { return precision == BH_PREC_MIXED; }
+// the opt-in variant contracts an FP32 copy of the top table: its rounding
+// (2^-24 relative per entry, independent) moves the potential by about
+// 1e-10 relative and halves the bytes of the HBM-bound contraction; every
+// other plan keeps the table in FP64
+bool table32(int precision) { return precision == BH_PREC_MIXED_TABLE32; }
+// the transform and operator data do not depend on the table precision
+int ops_precision(int precision) { return is_mixed(precision) ? BH_PREC_MIXED : BH_PREC_FP64; }
 
 std::string tables_key(const bh_grid_desc &d, int precision, int device)
 {
@@ -198,8 +202,8 @@ std::string modeops_key(const bh_grid_desc &d, int precision, int device)
 {
     char b[200];
     std::snprintf(b, sizeof b, "%d|%d|%d|%d|%.17g|%d|%d|%d|%d|%d", device, d.numr, d.numz, d.numphi, d.dr, d.mmax,
-                  d.nsteps_cold, d.nsteps_warm, precision,
-                  precision == BH_PREC_MIXED && spectral_fused_ok(d.numr - 2, d.numz - 2) ? 1 : 0);
+                  d.nsteps_cold, d.nsteps_warm, ops_precision(precision),
+                  is_mixed(precision) && spectral_fused_ok(d.numr - 2, d.numz - 2) ? 1 : 0);
     return b;
 }
 
@@ -217,9 +221,9 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     size_t b = 8 * (2 * nh * nh + 2 * ns * ns + (size_t)proj_coef_elems((int)N, (int)mm) + 2 * mm * N + 2 * (numr + 1) + N * (numr + 1));
     if (N <= 512) {
         const size_t kg = (N / 2 + 14) / 16, rt = (N / 2 + 15) / 16 + (N / 2 + 14) / 16;
-        b += 8 * 2 * rt * kg * 256 + (precision == BH_PREC_MIXED ? 4 * rt * kg * 256 : 0);
+        b += 8 * 2 * rt * kg * 256 + (is_mixed(precision) ? 4 * rt * kg * 256 : 0);
     }
-    if (precision == BH_PREC_MIXED) {
+    if (is_mixed(precision)) {
         b += 4 * (nh * nh + ns * ns + (nzi + 1) / 2 * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
         if (spectral_fused_ok((int)nri, (int)nzi))  // tables; no g; M_r and pivots per mode number
             b += 4 * 2 * (nzi / 2) * nzi - 4 * N * nri * nzi - 4 * (N / 2 - 1) * (2 * nri * nri + nri * nzi);
@@ -291,7 +295,7 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     o->dt[1] = h.dt[1];
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax, nri = g.nri, nzi = g.nzi;
     (void)numz;
-    const bool mixed = precision == BH_PREC_MIXED;
+    const bool mixed = is_mixed(precision);
     const int nh = N / 2 + 1, ns_ = N / 2 - 1;  // sizes of the cosine and sine halves
     const double pi = std::acos(-1.0);
     DevMem &m = o->mem;
@@ -599,10 +603,11 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
     size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf +
                     (tb_blocked(d.numr) ? 0 : mm * nzi * NB16 * 4) +
                     mm * (tb_blocked(d.numr) ? tb_chunks(d.numz) : topsym_splits(d.numz) * W) * NB16 * 4);
-    if (precision == BH_PREC_MIXED) {
-        // RH32, r32, D1 + Wf; RT, CT, DT + WT only for the separate-kernel step
+    if (is_mixed(precision)) {
+        // RH32, r32, D1 + Wf (+ D2 with two or more passes of the fused
+        // step); RT, CT, DT + WT only for the separate-kernel step
         const bool fused = spectral_fused_ok((int)nri, (int)nzi);
-        b += 4 * (M * N + (fused ? 1 : 5) * N * nri * nzi + 2 * M * N);
+        b += 4 * (M * N + (fused ? 1 : 5) * N * nri * nzi + (fused && d.refine_passes >= 2 ? 3 : 2) * M * N);
     }
     else b += 8 * 2 * M * N;
     return b;

@@ -245,8 +245,8 @@ struct Iteration {
 std::string grid_key(const bh_grid_desc &d)
 {
     char b[200];
-    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d|%d", d.numr, d.numz, d.numphi, d.dr, d.mmax, d.nsteps_cold,
-                  d.nsteps_warm);
+    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d|%d|%d", d.numr, d.numz, d.numphi, d.dr, d.mmax, d.nsteps_cold,
+                  d.nsteps_warm, d.refine_passes);
     return b;
 }
 
@@ -575,8 +575,8 @@ extern "C" int bh_server_create(const bh_server_config *cfg, bh_server **out)
 {
     if (!cfg || !out) return fail(BH_E_INVALID, "null argument");
     *out = nullptr;
-    if (cfg->precision != BH_PREC_FP64 && cfg->precision != BH_PREC_MIXED)
-        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64 or BH_PREC_MIXED");
+    if (cfg->precision != BH_PREC_FP64 && !is_mixed(cfg->precision))
+        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64, BH_PREC_MIXED or BH_PREC_MIXED_TABLE32");
     std::unique_ptr<bh_server> S(new bh_server());
     S->cfg = *cfg;
     if (S->cfg.max_active <= 0) S->cfg.max_active = 64;

@@ -14,6 +14,8 @@ namespace bh {
 //   W = mu . (M_r RT), C = RT - W  when mr is given (FP32 MFMA)
 //   (B_r + lambda) D = C           Thomas sweeps per spectral column (FP32)
 //   delta = S D^T, w = S W^T       folded inverse transforms into planes
+//                                  (or dlt += S D^T, the refinement passes
+//                                  that accumulate their corrections)
 // r is compact [L][nzi][nri]; delta and w are full numr x numz planes whose
 // interior is written (faces untouched).
 struct SpecArgs {
@@ -25,6 +27,7 @@ struct SpecArgs {
     const double *betar, *alphar;
     float *dlt, *w;
     int numr, numz, nphi;         // mode number of plane L: L, or L - nphi/2 past nphi/2
+    int acc;                      // without mr: add delta to dlt's interior instead of storing it
 };
 
 // Sizes the fused kernel takes: nri a multiple of 64 up to 256 and nzi one

@@ -22,6 +22,7 @@
 
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 
 namespace bh {
 
@@ -70,7 +71,7 @@ __device__ __forceinline__ BH_GLOBAL T *gst(void *p)
 // Tile t takes plane t/2 + (t%2) N/2: planes m and m + N/2 (the cosine and
 // sine rows of azimuthal mode m) run side by side on one XCD, and share M_r
 // and the pivots, stored per mode number mi = L or L - N/2.
-template <int NZ, bool MR>
+template <int NZ, bool MR, bool ACC>
 __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int tile)
 {
     typedef Shape<NZ> S;
@@ -327,7 +328,8 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     //         over the even spectral slots of the LDS block and O the odd.
     //         Wave w takes z tile w % ZT and 32 ZT columns j from (w / ZT) 32 ZT,
     //         so that it reads 1/ZT of the table (the block comes from LDS).
-    auto inverse = [&](float *outp) {
+    auto inverse = [&](float *outp, auto acc_c) {
+        constexpr bool ACCUM = decltype(acc_c)::value;
         constexpr int JW = 32 * S::ZT, JT = JW / 16;
         const int zt = wave % S::ZT, jb = (wave / S::ZT) * JW;
         if (jb >= nri) return;
@@ -358,6 +360,32 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
             afc = afn;
         }
         BH_GLOBAL float *pl = gst<float>(outp) + (long)L * numr * a.numz + numr + 1 + jb + t;
+        if constexpr (ACCUM) {
+            // every previous value is loaded before the first store
+#pragma unroll
+            for (int i = 0; i < JT; ++i) {
+                if (jb + 16 * i < nri) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int z = 16 * zt + 4 * q + e;
+                        const float x = ev[i][e], y = od[i][e];
+                        ev[i][e] = (x + y) + pl[(long)z * numr + 16 * i];
+                        od[i][e] = (x - y) + pl[(long)(NZ - 1 - z) * numr + 16 * i];
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < JT; ++i) {
+                if (jb + 16 * i >= nri) break;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int z = 16 * zt + 4 * q + e;
+                    pl[(long)z * numr + 16 * i] = ev[i][e];
+                    pl[(long)(NZ - 1 - z) * numr + 16 * i] = od[i][e];
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int i = 0; i < JT; ++i) {
             if (jb + 16 * i >= nri) break;
@@ -370,7 +398,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
             }
         }
     };
-    inverse(a.dlt);
+    inverse(a.dlt, std::integral_constant<bool, ACC>{});
     if (MR) {
         __syncthreads();  // every wave has read D
         if (fil) {
@@ -383,37 +411,41 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                         buf[(64 * rg + 16 * i + 4 * q + e) * P + (NZ / 2) * cg + 16 * c + t] = wacc[i][c][e];
         }
         __syncthreads();
-        inverse(a.w);
+        inverse(a.w, std::false_type{});
     }
     __syncthreads();  // the block is free for the next mode
 }
 
-template <int NZ, bool MR>
+template <int NZ, bool MR, bool ACC>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_kernel(SpecArgs a, int nx, int ny, int total)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR>(a, spec_lds, bx)));
+    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR, ACC>(a, spec_lds, bx)));
 }
 
-template <int NZ, bool MR>
+template <int NZ, bool MR, bool ACC>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_grouped(const SpecArgs *__restrict__ args, GroupSched gs)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR>(ga, spec_lds, bx)));
+    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR, ACC>(ga, spec_lds, bx)));
 }
 
-// variant v: nzi = 32 << (v % 3), with the M_r filter when v >= 3
+// variant v: nzi = 32 << (v % 3); v / 3 = 0 plain, 1 with the M_r filter,
+// 2 accumulating delta into dlt
 const void *spec_fn(int variant, bool grouped)
 {
-#define BH_SPEC_FN(NZ, MR) \
-    (grouped ? (const void *)&spectral_grouped<NZ, MR> : (const void *)&spectral_kernel<NZ, MR>)
+#define BH_SPEC_FN(NZ, MR, ACC) \
+    (grouped ? (const void *)&spectral_grouped<NZ, MR, ACC> : (const void *)&spectral_kernel<NZ, MR, ACC>)
     switch (variant) {
-    case 0: return BH_SPEC_FN(32, false);
-    case 1: return BH_SPEC_FN(64, false);
-    case 2: return BH_SPEC_FN(128, false);
-    case 3: return BH_SPEC_FN(32, true);
-    case 4: return BH_SPEC_FN(64, true);
-    case 5: return BH_SPEC_FN(128, true);
+    case 0: return BH_SPEC_FN(32, false, false);
+    case 1: return BH_SPEC_FN(64, false, false);
+    case 2: return BH_SPEC_FN(128, false, false);
+    case 3: return BH_SPEC_FN(32, true, false);
+    case 4: return BH_SPEC_FN(64, true, false);
+    case 5: return BH_SPEC_FN(128, true, false);
+    case 6: return BH_SPEC_FN(32, false, true);
+    case 7: return BH_SPEC_FN(64, false, true);
+    case 8: return BH_SPEC_FN(128, false, true);
     default: return nullptr;
     }
 #undef BH_SPEC_FN
@@ -439,7 +471,7 @@ bool spectral_fused_ok(int nri, int nzi)
 
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi)
 {
-    return Op::make(OP_SPECTRAL, spec_variant(nzi) + (a.mr ? 3 : 0), a, nplanes, 1, 1);
+    return Op::make(OP_SPECTRAL, spec_variant(nzi) + (a.mr ? 3 : a.acc ? 6 : 0), a, nplanes, 1, 1);
 }
 
 void spectral_fragments(int n, std::vector<float> *sfb, std::vector<float> *sfa)

@@ -7,12 +7,13 @@ module bh_potential
   integer(c_int), parameter :: BH_OK = 0
   integer(c_int), parameter :: BH_PREC_FP64 = 0
   integer(c_int), parameter :: BH_PREC_MIXED = 1
+  integer(c_int), parameter :: BH_PREC_MIXED_TABLE32 = 2
 
   ! mirrors struct bh_grid_desc
   type, bind(C) :: bh_grid_desc
      integer(c_int) :: numr, numz, numphi, isym
      real(c_double) :: dr
-     integer(c_int) :: mmax, nsteps_cold, nsteps_warm
+     integer(c_int) :: mmax, nsteps_cold, nsteps_warm, refine_passes
   end type bh_grid_desc
 
   interface

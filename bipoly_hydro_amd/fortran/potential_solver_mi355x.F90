@@ -53,10 +53,15 @@ subroutine potsetup
      desc%mmax = 30
      desc%nsteps_cold = 20
      desc%nsteps_warm = 5
+     desc%refine_passes = 0
      env = ' '
      call get_environment_variable('BH_PRECISION', env)
      prec = BH_PREC_FP64
      if (trim(env) == 'mixed') prec = BH_PREC_MIXED
+     if (trim(env) == 'mixed_t32') prec = BH_PREC_MIXED_TABLE32
+     env = ' '
+     call get_environment_variable('BH_REFINE_PASSES', env)
+     if (len_trim(env) > 0) read(env, *) desc%refine_passes
      rc = bh_plan_create(desc, prec, 0_c_size_t, plan)
      if (rc /= BH_OK) then
         write(6,*) 'potsetup: bh_plan_create failed: ', trim(bh_last_error())

@@ -27,7 +27,11 @@ LIB_PATH = os.path.join(_HERE, "libbh_potential.so")
 BH_OK = 0
 BH_PREC_FP64 = 0
 BH_PREC_MIXED = 1
-_PREC = {"fp64": BH_PREC_FP64, "mixed": BH_PREC_MIXED}
+BH_PREC_MIXED_TABLE32 = 2
+# "mixed": FP32 transforms and sweeps, every Green's table in FP64, FP64
+# residual and refinement; "mixed_t32": the same with the top/bottom table
+# stored in FP32 (an opt-in variant)
+_PREC = {"fp64": BH_PREC_FP64, "mixed": BH_PREC_MIXED, "mixed_t32": BH_PREC_MIXED_TABLE32}
 
 EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_set_potential", "bh_plan_device_bytes", "bh_plan_set_timing",
@@ -50,7 +54,7 @@ class BHError(RuntimeError):
 class GridDesc(ctypes.Structure):
     _fields_ = [("numr", ctypes.c_int), ("numz", ctypes.c_int), ("numphi", ctypes.c_int),
                 ("isym", ctypes.c_int), ("dr", ctypes.c_double), ("mmax", ctypes.c_int),
-                ("nsteps_cold", ctypes.c_int), ("nsteps_warm", ctypes.c_int)]
+                ("nsteps_cold", ctypes.c_int), ("nsteps_warm", ctypes.c_int), ("refine_passes", ctypes.c_int)]
 
 
 class BatchStats(ctypes.Structure):
@@ -151,10 +155,12 @@ class PotentialPlan:
 
     def __init__(self, numr: int, numz: int, numphi: int, *, precision: str = "fp64",
                  hbm_budget: int = 0, dr: float = 1.0 / 127.0, mmax: int = 30,
-                 nsteps_cold: int = 20, nsteps_warm: int = 5, isym: int = 1):
+                 nsteps_cold: int = 20, nsteps_warm: int = 5, isym: int = 1, refine_passes: int = 2):
+        """precision "fp64", "mixed" or "mixed_t32"; refine_passes (1..4) is
+        the number of FP64 residual + refinement passes of a mixed solve."""
         L = load_library()
         self.shape = (numphi, numz, numr)
-        self.desc = GridDesc(numr, numz, numphi, isym, dr, mmax, nsteps_cold, nsteps_warm)
+        self.desc = GridDesc(numr, numz, numphi, isym, dr, mmax, nsteps_cold, nsteps_warm, refine_passes)
         h = ctypes.c_void_p()
         _check(L.bh_plan_create(ctypes.byref(self.desc), _PREC[precision], hbm_budget, ctypes.byref(h)))
         self._h = h
@@ -227,11 +233,12 @@ class PotentialBatch:
 
     def __init__(self, grids, *, precision: str = "mixed", hbm_budget: int = 0, graph: bool = False,
                  autotune: bool = False, dr: float = 1.0 / 127.0, mmax: int = 30,
-                 nsteps_cold: int = 20, nsteps_warm: int = 5):
+                 nsteps_cold: int = 20, nsteps_warm: int = 5, refine_passes: int = 2):
         L = load_library()
         self.shapes = [(p, z, r) for (r, z, p) in grids]
         n = len(grids)
-        descs = (GridDesc * n)(*[GridDesc(r, z, p, 1, dr, mmax, nsteps_cold, nsteps_warm) for (r, z, p) in grids])
+        descs = (GridDesc * n)(*[GridDesc(r, z, p, 1, dr, mmax, nsteps_cold, nsteps_warm, refine_passes)
+                                 for (r, z, p) in grids])
         flags = (BH_BATCH_GRAPH if graph else 0) | (BH_BATCH_AUTOTUNE if autotune else 0)
         h = ctypes.c_void_p()
         _check(L.bh_batch_create(descs, n, _PREC[precision], hbm_budget, flags, ctypes.byref(h)))
@@ -330,7 +337,7 @@ class PotentialServer:
         self.close()
 
     def submit(self, grid, nsolves: int, model: dict, *, want_potential: bool = False,
-               want_latencies: bool = True, dr: float = 1.0 / 127.0) -> int:
+               want_latencies: bool = True, dr: float = 1.0 / 127.0, refine_passes: int = 2) -> int:
         """grid = (numr, numz, numphi); model keys as bh_binary_model."""
         numr, numz, numphi = grid
         m = BinaryModel()
@@ -343,7 +350,7 @@ class PotentialServer:
         m.seed = int(model.get("seed", 0))
         pot = np.empty((numphi, numz, numr)) if want_potential else None
         lat = np.zeros(nsolves) if want_latencies else None
-        r = Request(GridDesc(numr, numz, numphi, 1, dr, 30, 20, 5), nsolves, m,
+        r = Request(GridDesc(numr, numz, numphi, 1, dr, 30, 20, 5, refine_passes), nsolves, m,
                     pot.ctypes.data if pot is not None else None, lat.ctypes.data if lat is not None else None)
         rid = ctypes.c_longlong()
         _check(load_library().bh_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))

@@ -33,6 +33,11 @@ extern "C" {
 
 #define BH_PREC_FP64 0  /* plain path: FP64 throughout */
 #define BH_PREC_MIXED 1 /* FP32 sweeps / DFT, FP64 tables, residual, refinement */
+/* As BH_PREC_MIXED, but the top/bottom Green's table is stored in FP32 (its
+ * products and sums stay FP64): half the bytes of the boundary contraction,
+ * about 1e-10 more error.  An opt-in variant; BH_PREC_MIXED keeps every
+ * Green's table in FP64. */
+#define BH_PREC_MIXED_TABLE32 2
 
 /* Geometry and solver constants.  Replaces the compile-time parameters of
  * runhydro.h:1-30 and pot.h:3-7 (numr, numz, numphi; mmax = 30,
@@ -47,6 +52,9 @@ typedef struct bh_grid_desc {
     int mmax;          /* boundary azimuthal modes; 0 -> 30                  */
     int nsteps_cold;   /* ADI steps when tstep <= 1; 0 -> 20                 */
     int nsteps_warm;   /* ADI steps when tstep > 1;  0 -> 5                  */
+    int refine_passes; /* mixed plans: FP64 residual + refinement passes after
+                          the first FP32 solve, 1..4; 0 -> 2.  FP64 plans
+                          ignore it.                                         */
 } bh_grid_desc;
 
 typedef struct bh_plan bh_plan;

@@ -22,8 +22,9 @@
  *   tm.F90:101-200, bm.F90:101-172, sm.F90:115-215
  *                                 Green's-function tables (isym = 1)
  *   gammln.F90:14-26              ln Gamma, 6-term Lanczos form
- *   rf.F90 / rd.F90 / ellf.F90 / elle.F90
- *                                 complete elliptic K, E via Carlson RF, RD
+ *   ellf.F90 / elle.F90           complete elliptic K, E via Carlson's RF,
+ *                                 RD (rf.F90, rd.F90: semantics only; the
+ *                                 series and stopping rule of Carlson 1995)
  *   setup_frac.F90:374-463        grid coordinates
  *
  * Only isym = 1 (no symmetry, full z range) is restated; it is the only
@@ -101,58 +102,66 @@ double bho_gammln(double xx)
 }
 
 /* ------------------------------------------------------------------ */
-/* Carlson's symmetric elliptic integrals by the duplication theorem    */
-/* (B. C. Carlson, Numer. Algorithms 10 (1995) 13-26).  Stopping rule   */
-/* and Taylor truncation order as in rf.F90:104-122 / rd.F90:158-184:  */
-/* relative deviation <= 0.0025 (RF) / 0.0015 (RD), 5th-order series. */
+/* Carlson's symmetric elliptic integrals (B. C. Carlson, Numer.        */
+/* Algorithms 10 (1995) 13-26; DLMF 19.36.1-2).  The duplication        */
+/* v <- (v + lam) / 4 (lam = sqrt(xy) + sqrt(xz) + sqrt(yz)) shrinks    */
+/* the spread of the arguments about their mean A by 4 per step.  It    */
+/* stops by Carlson's a-priori rule 4^-m Q < |A_m|, Q = (3 r)^(-1/6)    */
+/* (RF) or (r / 4)^(-1/6) (RD) times the initial spread, r = 1e-17; the */
+/* deviations X, Y, Z, formed from the initial arguments, enter the     */
+/* series in the elementary symmetric functions E2..E5.  The reference  */
+/* (rf.F90, rd.F90) stops on the running deviations instead; both are   */
+/* accurate to a few units of the last place.                           */
 /* ------------------------------------------------------------------ */
+static double spread3(double a, double x, double y, double z)
+{
+    return fmax(fabs(a - x), fmax(fabs(a - y), fabs(a - z)));
+}
+
 double bho_rf(double x, double y, double z)
 {
-    const double errtol = 0.0025;
-    double xt = x, yt = y, zt = z, ave, dx, dy, dz;
-    for (;;) {
-        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
-        double lam = sx * (sy + sz) + sy * sz;
-        xt = 0.25 * (xt + lam);
-        yt = 0.25 * (yt + lam);
-        zt = 0.25 * (zt + lam);
-        ave = (1.0 / 3.0) * (xt + yt + zt);
-        dx = (ave - xt) / ave;
-        dy = (ave - yt) / ave;
-        dz = (ave - zt) / ave;
-        if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) <= errtol) break;
+    const double x0 = x, y0 = y, a0 = (x + y + z) / 3.0;
+    const double q = pow(3.0e-17, -1.0 / 6.0) * spread3(a0, x, y, z);
+    double a = a0, f = 1.0; /* f = 4^-m */
+    while (f * q >= fabs(a)) {
+        const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+        const double lam = sx * sy + sx * sz + sy * sz;
+        x = 0.25 * (x + lam);
+        y = 0.25 * (y + lam);
+        z = 0.25 * (z + lam);
+        a = 0.25 * (a + lam);
+        f *= 0.25;
     }
-    double e2 = dx * dy - dz * dz;
-    double e3 = dx * dy * dz;
-    const double c1 = 1.0 / 24.0, c2 = 0.1, c3 = 3.0 / 44.0, c4 = 1.0 / 14.0;
-    return (1.0 + (c1 * e2 - c2 - c3 * e3) * e2 + c4 * e3) / sqrt(ave);
+    const double X = f * (a0 - x0) / a, Y = f * (a0 - y0) / a, Z = -(X + Y);
+    const double e2 = X * Y - Z * Z, e3 = X * Y * Z;
+    /* DLMF 19.36.1 through seventh order */
+    const double t = 1.0 - e2 / 10.0 + e3 / 14.0 + e2 * e2 / 24.0 - 3.0 * e2 * e3 / 44.0 -
+                     5.0 * e2 * e2 * e2 / 208.0 + 3.0 * e3 * e3 / 104.0 + e2 * e2 * e3 / 16.0;
+    return t / sqrt(a);
 }
 
 double bho_rd(double x, double y, double z)
 {
-    const double errtol = 0.0015;
-    double xt = x, yt = y, zt = z, sum = 0.0, fac = 1.0, ave, dx, dy, dz;
-    for (;;) {
-        double sx = sqrt(xt), sy = sqrt(yt), sz = sqrt(zt);
-        double lam = sx * (sy + sz) + sy * sz;
-        sum += fac / (sz * (zt + lam));
-        fac *= 0.25;
-        xt = 0.25 * (xt + lam);
-        yt = 0.25 * (yt + lam);
-        zt = 0.25 * (zt + lam);
-        ave = 0.2 * (xt + yt + 3.0 * zt);
-        dx = (ave - xt) / ave;
-        dy = (ave - yt) / ave;
-        dz = (ave - zt) / ave;
-        if (fmax(fabs(dx), fmax(fabs(dy), fabs(dz))) <= errtol) break;
+    const double x0 = x, y0 = y, a0 = (x + y + 3.0 * z) / 5.0;
+    const double q = pow(0.25e-17, -1.0 / 6.0) * spread3(a0, x, y, z);
+    double a = a0, f = 1.0, tail = 0.0; /* tail = sum_m 4^-m / (sqrt(z_m) (z_m + lam_m)) */
+    while (f * q >= fabs(a)) {
+        const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+        const double lam = sx * sy + sx * sz + sy * sz;
+        tail += f / (sz * (z + lam));
+        x = 0.25 * (x + lam);
+        y = 0.25 * (y + lam);
+        z = 0.25 * (z + lam);
+        a = 0.25 * (a + lam);
+        f *= 0.25;
     }
-    /* fifth-order Taylor expansion of R_D about the mean (Carlson 1995) */
-    const double c1 = 3.0 / 14.0, c2 = 1.0 / 6.0, c3 = 9.0 / 22.0, c4 = 3.0 / 26.0;
-    double ea = dx * dy, eb = dz * dz, ec = ea - eb, ed = ea - 6.0 * eb, ee = ed + ec + ec;
-    double c5 = 0.25 * c3, c6 = 1.5 * c4;
-    return 3.0 * sum + fac * (1.0 + ed * (-c1 + c5 * ed - c6 * dz * ee) +
-                              dz * (c2 * ee + dz * (-c3 * ec + dz * c4 * ea))) /
-                           (ave * sqrt(ave));
+    const double X = f * (a0 - x0) / a, Y = f * (a0 - y0) / a, Z = -(X + Y) / 3.0;
+    const double xy = X * Y, z2 = Z * Z;
+    const double e2 = xy - 6.0 * z2, e3 = (3.0 * xy - 8.0 * z2) * Z, e4 = 3.0 * (xy - z2) * z2, e5 = xy * z2 * Z;
+    /* DLMF 19.36.2 */
+    const double t = 1.0 - 3.0 * e2 / 14.0 + e3 / 6.0 + 9.0 * e2 * e2 / 88.0 - 3.0 * e4 / 22.0 -
+                     9.0 * e2 * e3 / 52.0 + 3.0 * e5 / 26.0;
+    return f * t / (a * sqrt(a)) + 3.0 * tail;
 }
 
 /* complete elliptic integrals at modulus k (ellf.F90:16, elle.F90:16-18) */

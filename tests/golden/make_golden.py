@@ -19,7 +19,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 from bipoly_hydro_amd import density  # noqa: E402
+from golden_cases import c2_digests  # noqa: E402
 
 # name, numr, numz, numphi, numr_procs, numz_procs, density builder
 CASES = [
@@ -48,15 +50,21 @@ def run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, workdir):
 
 
 def c2_case(wd):
-    """258 x 130 x 512 on 4 x 2 ranks: slices and norms (the full field is
-    137 MB).  Cold solve: the mid-plane row K = 65 (all J, all L), the
-    meridional plane L = 1, and the top, bottom and side faces at every 8th
-    L.  Warm solve: row K = 65 at every 4th L and the plane L = 1."""
+    """258 x 130 x 512 on 4 x 2 ranks: slices, norms and digests (the full
+    field is 137 MB).  Cold solve: the mid-plane row K = 65 (all J, all L),
+    the meridional plane L = 1, and the top, bottom and side faces at every
+    8th L.  Warm solve: row K = 65 at every 4th L and the plane L = 1.  Both
+    solves: c2_digests of the whole field."""
     name, numr, numz, numphi, nrp, nzp = "c2_258x130x512", 258, 130, 512, 4, 2
     rho1 = density.config2_density(seed=1234)
     rho2 = rho1 * 1.01 + 1e-3 * density.config2_density(seed=99)
     pot1, pot2, log = run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, wd)
-    np.savez(os.path.join(HERE, f"{name}.npz"),
+    weights = np.random.default_rng(2024).standard_normal(numr)
+    dig = {}
+    for tag, pot in (("1", pot1), ("2", pot2)):
+        for k, v in c2_digests(pot, weights).items():
+            dig[f"{k}{tag}"] = v
+    np.savez(os.path.join(HERE, f"{name}.npz"), digest_weights=weights, **dig,
              rho_sums=np.array([rho1.sum(), rho2.sum(), np.abs(rho1).max()]),
              seeds=np.array([1234, 99]), procs=np.array([nrp, nzp]),
              mid1=pot1[:, 64, :], mer1=pot1[0], top1=pot1[::8, -1, :], bot1=pot1[::8, 0, :],

@@ -36,3 +36,14 @@ def c2_slices(pot, tag):
         return {"mid1": pot[:, 64, :], "mer1": pot[0], "top1": pot[::8, -1, :], "bot1": pot[::8, 0, :],
                 "side1": pot[::8, :, -1]}
     return {"mid2": pot[::4, 64, :], "mer2": pot[0]}
+
+
+def c2_digests(pot, weights):
+    """Digests of a whole 258 x 130 x 512 potential (numpy (L, K, J)): the L2
+    norm of every plane L, of every K slab and of every J column, and the
+    weighted sum over J of every (L, K) row, sum_J w_J pot[L, K, J] (first
+    order in any error, so a fault confined to one row moves its entry)."""
+    return {"plane_norm": np.sqrt(np.einsum("lkj,lkj->l", pot, pot)),
+            "k_norm": np.sqrt(np.einsum("lkj,lkj->k", pot, pot)),
+            "j_norm": np.sqrt(np.einsum("lkj,lkj->j", pot, pot)),
+            "row_dot": pot @ weights}

@@ -39,13 +39,16 @@ def test_library_exports_every_declared_function():
     (dict(numr=66, numz=66, numphi=64, isym=2), "isym"),
     (dict(numr=66, numz=66, numphi=64, mmax=-1), "mmax"),
     (dict(numr=600, numz=66, numphi=64), "<= 512"),
+    (dict(numr=66, numz=66, numphi=64, passes=5), "refine_passes"),
+    (dict(numr=66, numz=66, numphi=64, passes=-1), "refine_passes"),
+    (dict(numr=66, numz=66, numphi=64, precision=3), "precision"),
 ])
 def test_invalid_descriptors_are_rejected_without_a_gpu(desc, msg):
     lib = P.load_library()
     d = P.GridDesc(desc["numr"], desc["numz"], desc["numphi"], desc.get("isym", 1), 1.0 / 127.0,
-                   desc.get("mmax", 30), 20, 5)
+                   desc.get("mmax", 30), 20, 5, desc.get("passes", 0))
     h = ctypes.c_void_p()
-    rc = lib.bh_plan_create(ctypes.byref(d), 0, 0, ctypes.byref(h))
+    rc = lib.bh_plan_create(ctypes.byref(d), desc.get("precision", 0), 0, ctypes.byref(h))
     assert rc == -1
     assert msg in lib.bh_last_error().decode()
     assert not h.value
@@ -55,3 +58,12 @@ def test_null_arguments_are_errors():
     lib = P.load_library()
     assert lib.bh_solve(None, None, None, 0) == -1
     assert lib.bh_plan_destroy(None) == 0
+
+
+def test_grid_desc_layout_matches_the_header():
+    """The ctypes mirror has the header's fields in the header's order."""
+    text = open(os.path.join(ROOT, "include", "bh_potential.h")).read()
+    body = re.search(r"typedef struct bh_grid_desc \{(.*?)\} bh_grid_desc;", text, flags=re.S).group(1)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    fields = re.findall(r"\b(?:int|double)\s+([a-z_]+)\s*;", body)
+    assert fields == [f for f, _ in P.GridDesc._fields_]

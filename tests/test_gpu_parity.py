@@ -9,7 +9,7 @@ import pytest
 
 from bipoly_hydro_amd import density
 from bipoly_hydro_amd import potential as P
-from golden_cases import c2_slices, load_case
+from golden_cases import c2_digests, c2_slices, load_case
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +71,8 @@ def test_fp64_production_grid_against_reference_slices(golden_dir):
             print(f"c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
             assert e <= TOL_FP64, (tag, k, e)
         assert abs(np.linalg.norm(pot) / g["norm" + tag][0] - 1.0) < TOL_FP64
+        if "plane_norm" + tag in g:
+            check_c2_digests(pot, g, tag, TOL_FP64)
 
 
 def test_solves_are_deterministic_and_warm_start_matches_import():
@@ -122,30 +124,53 @@ def test_error_paths():
 TOL_MIXED = 1e-8
 
 
-@pytest.mark.parametrize("name", ["g42x26x16", "g34x34x32", "c1_66x66x64"])
-def test_mixed_matches_reference_cold_then_warm(golden_dir, name):
+# the configuration-3 recipe (FP64 tables, 2 refinement passes) on every
+# small golden grid; 1 and 4 passes and the FP32-table variant on one each
+@pytest.mark.parametrize("name,precision,passes", [
+    ("g42x26x16", "mixed", 2), ("g34x34x32", "mixed", 2), ("c1_66x66x64", "mixed", 2),
+    ("c1_66x66x64", "mixed", 1), ("c1_66x66x64", "mixed", 4), ("g34x34x32", "mixed", 3),
+    ("g42x26x16", "mixed_t32", 2)])
+def test_mixed_matches_reference_cold_then_warm(golden_dir, name, precision, passes):
     (numr, numz, numphi), rho1, rho2, ref1, ref2 = load_case(golden_dir, name)
-    with P.PotentialPlan(numr, numz, numphi, precision="mixed") as plan:
+    with P.PotentialPlan(numr, numz, numphi, precision=precision, refine_passes=passes) as plan:
         pot1 = plan.potential_solver(rho1, tstep=0)
         pot2 = plan.potential_solver(rho2, tstep=2)
     e1, e2 = rell2(pot1, ref1), rell2(pot2, ref2)
-    print(f"mixed {name}: cold relL2 {e1:.3e} warm relL2 {e2:.3e}", flush=True)
+    print(f"{precision} passes {passes} {name}: cold relL2 {e1:.3e} warm relL2 {e2:.3e}", flush=True)
     assert e1 <= TOL_MIXED and e2 <= TOL_MIXED
     assert np.all(pot1[:, 0, -1] == 0.0) and np.all(pot1[:, -1, -1] == 0.0)
+    h = numphi // 2
+    np.testing.assert_array_equal(pot1[:h, :, 0], pot1[h:, :, 1])
 
 
-def test_mixed_production_grid_against_reference_slices(golden_dir):
+@pytest.mark.parametrize("precision,passes", [("mixed", 2), ("mixed", 4), ("mixed_t32", 1)])
+def test_mixed_production_grid_against_reference_slices(golden_dir, precision, passes):
     path = os.path.join(golden_dir, "c2_258x130x512.npz")
     g = np.load(path)
     rho1 = density.config2_density(seed=int(g["seeds"][0]))
     rho2 = rho1 * 1.01 + 1e-3 * density.config2_density(seed=int(g["seeds"][1]))
-    with P.PotentialPlan(258, 130, 512, precision="mixed") as plan:
+    with P.PotentialPlan(258, 130, 512, precision=precision, refine_passes=passes) as plan:
         pots = [plan.potential_solver(rho1, tstep=0), plan.potential_solver(rho2, tstep=2)]
     for tag, pot in zip(("1", "2"), pots):
         for k, v in c2_slices(pot, tag).items():
             e = rell2(v, g[k])
-            print(f"mixed c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
+            print(f"{precision} passes {passes} c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
             assert e <= TOL_MIXED, (tag, k, e)
+        if "plane_norm" + tag in g:
+            check_c2_digests(pot, g, tag, TOL_MIXED)
+
+
+def check_c2_digests(pot, g, tag, tol):
+    """The whole field against the reference's digests: every plane, K slab
+    and J column norm, and the weighted sum of every (L, K) row."""
+    dig = c2_digests(pot, g["digest_weights"])
+    for k, v in dig.items():
+        ref = g[k + tag]
+        e = rell2(v, ref)
+        # per entry too: each row's weighted sum against the row's own scale
+        worst = float(np.max(np.abs(v - ref) / (np.abs(ref).max())))
+        print(f"c2 solve {tag} digest {k}: relL2 {e:.3e}, worst entry {worst:.3e}", flush=True)
+        assert e <= tol and worst <= tol, (tag, k, e, worst)
 
 
 @pytest.mark.parametrize("shape,seed", [((98, 34, 64), 11), ((290, 18, 16), 3)])

@@ -33,10 +33,17 @@ def test_carlson_known_values_and_complete_integrals():
     from scipy import special
     # Carlson (1995) test values
     assert abs(O.rf(1.0, 2.0, 0.0) - 1.3110287771461) < 1e-12
+    assert abs(O.rf(2.0, 3.0, 4.0) - 0.58408284167715) < 1e-13
     assert abs(O.rd(0.0, 2.0, 1.0) - 1.7972103521034) < 1e-12
+    assert abs(O.rd(2.0, 3.0, 4.0) - 0.16510527294261) < 1e-13
     for k in np.linspace(0.05, 0.999, 40):
         assert abs(O.ellk(k) / special.ellipk(k * k) - 1.0) < 1e-14
         assert abs(O.elle(k) / special.ellipe(k * k) - 1.0) < 1e-14
+    # Legendre's relation E K' + E' K - K K' = pi / 2 ties the pair together
+    for k in (0.1, 0.5, 0.9, 0.99, 0.9999):
+        kp = math.sqrt((1.0 - k) * (1.0 + k))
+        K, E, Kp, Ep = O.ellk(k), O.elle(k), O.ellk(kp), O.elle(kp)
+        assert abs(E * Kp + Ep * K - K * Kp - math.pi / 2) < 2e-14
 
 
 def test_dft_against_numpy_and_roundtrip():
