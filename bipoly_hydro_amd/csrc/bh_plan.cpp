@@ -100,6 +100,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     HIPCHK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming));
     hipStream_t s;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     launch_fill(p->XJ, M * N, 0.0, s);
@@ -130,10 +131,12 @@ int side_blocks(const bh_plan *p)
     return p->side_cus > 0 ? p->side_cus : std::max(8, 2 * p->cus / 8 * 8);
 }
 
+}  // namespace
+
 // Forward real DFT along phi of the N rows of B (columns Mcols), written in
 // helmadi's mode order (helmadi.F90:131-156) to C; cosine and sine halves.
 void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, double *C, long ldc, bool mask,
-                   std::vector<Op> *out, int max_blocks = 0)
+                   std::vector<Op> *out, int max_blocks)
 {
     const ModeOps &o = *p->ops;
     const int N = p->g.numphi, nh = N / 2 + 1, nsn = N / 2 - 1;
@@ -153,6 +156,8 @@ void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, doubl
     a.A = o.Ds; a.lda = nsn; a.C = C + (long)nh * ldc; a.R = nsn; a.Kd = nsn;
     out->push_back(gemm_op(a, gemm_inst64(B_FOLD_MINUS, mask ? E_MASK : E_PLAIN)));
 }
+
+namespace {
 
 // Inverse real DFT with the 2/N normalisation (helmadi.F90:357-392) of the
 // N mode rows of X into N phi rows of out.
@@ -476,6 +481,8 @@ int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s)
         }
         ts = &p->tsets[p->tcount++];
     }
+    // after the plan's previous solve, whatever stream it was enqueued on
+    if (p->done_recorded) HIPCHK(hipStreamWaitEvent(s, p->done_ev, 0));
     hipStream_t cur = s;
     for (const Op &op : ops) {
         if (op.kind == OP_MARK) {
@@ -498,7 +505,15 @@ int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s)
         }
     }
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(p->done_ev, s));
+    p->done_recorded = true;
     p->has_solution = true;
+    return BH_OK;
+}
+
+int wait_plan_idle(bh_plan *p)
+{
+    if (p->done_recorded) HIPCHK(hipEventSynchronize(p->done_ev));
     return BH_OK;
 }
 
@@ -598,8 +613,10 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
     const long M = (long)g.numr * g.numz;
     const size_t n = (size_t)M * g.numphi;
     const double *src = pot;
+    int rc = wait_plan_idle(p);  // a solve still running reads and writes XJ
+    if (rc != BH_OK) return rc;
     if (!on_device) {
-        int rc = ensure_staging(p, n);
+        rc = ensure_staging(p, n);
         if (rc != BH_OK) return rc;
         HIPCHK(hipMemcpy(p->pot_out, pot, n * sizeof(double), hipMemcpyHostToDevice));
         src = p->pot_out;

@@ -137,6 +137,11 @@ struct bh_plan {
     int side_cus = 0;   // workgroups of the side-stream DFT (0: two per CU)
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    // recorded at the end of every solve enqueued on the plan: the next
+    // solve (on any stream) waits for it, and bh_set_potential synchronises
+    // with it, so that calls on one plan never overlap on the device
+    hipEvent_t done_ev = nullptr;
+    bool done_recorded = false;
     bool has_solution = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;
@@ -149,6 +154,7 @@ struct bh_plan {
         if (side) (void)hipStreamDestroy(side);
         if (fork_ev) (void)hipEventDestroy(fork_ev);
         if (join_ev) (void)hipEventDestroy(join_ev);
+        if (done_ev) (void)hipEventDestroy(done_ev);
     }
 };
 
@@ -161,4 +167,11 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
 void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out);
 // Launch the Ops of one solve on s (records timing events when enabled).
 int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s);
+// Forward real DFT along phi (FP64) of the numphi rows of B (Mcols columns,
+// row pitch ldb) into C (row pitch ldc), in helmadi's mode order; mask
+// zeroes the rhop layers; max_blocks caps the persistent grid (0: none).
+void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, double *C, long ldc, bool mask,
+                   std::vector<Op> *out, int max_blocks = 0);
+// Wait until the plan's last enqueued solve has finished.
+int wait_plan_idle(bh_plan *p);
 }  // namespace bh
