@@ -39,7 +39,7 @@ EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_batch_create", "bh_batch_destroy", "bh_batch_solve_device", "bh_batch_solve",
            "bh_batch_device_bytes", "bh_batch_get_stats", "bh_batch_set_timing", "bh_batch_tuning",
            "bh_batch_plan", "bh_server_create", "bh_server_submit", "bh_server_wait", "bh_server_drain",
-           "bh_server_get_stats", "bh_server_destroy")
+           "bh_server_get_stats", "bh_server_destroy", "bh_set_potential_noalloc")
 
 BH_BATCH_GRAPH = 1
 BH_BATCH_AUTOTUNE = 2
@@ -116,6 +116,7 @@ def load_library(path: str = LIB_PATH):
     L.bh_solve.argtypes = [vp, dp, dp, ctypes.c_int]
     L.bh_solve_device.argtypes = [vp, vp, vp, ctypes.c_int, vp]
     L.bh_set_potential.argtypes = [vp, vp, ctypes.c_int]
+    L.bh_set_potential_noalloc.argtypes = [vp, vp]
     L.bh_plan_device_bytes.argtypes = [vp]
     L.bh_plan_device_bytes.restype = ctypes.c_size_t
     L.bh_plan_set_timing.argtypes = [vp, ctypes.c_int]
@@ -210,6 +211,14 @@ class PotentialPlan:
         else:
             pot = np.ascontiguousarray(pot, dtype=np.float64)
             _check(load_library().bh_set_potential(self._h, ctypes.c_void_p(pot.ctypes.data), 0))
+
+    def set_potential_noalloc(self, pot):
+        """As set_potential from a host array, staged through the library's
+        fixed device array (no device allocation)."""
+        pot = np.ascontiguousarray(pot, dtype=np.float64)
+        if pot.shape != self.shape:
+            raise ValueError(f"pot shape {pot.shape} != {self.shape}")
+        _check(load_library().bh_set_potential_noalloc(self._h, ctypes.c_void_p(pot.ctypes.data)))
 
     def set_timing(self, on: bool = True):
         _check(load_library().bh_plan_set_timing(self._h, 1 if on else 0))

@@ -89,6 +89,18 @@ int bh_solve_device(bh_plan *plan, const double *d_rho, double *d_pot, int tstep
  * is a device or a host pointer.  Synchronous. */
 int bh_set_potential(bh_plan *plan, const double *pot, int on_device);
 
+/* As bh_set_potential from a host array, but allocating nothing: the field
+ * is streamed to the device in column chunks through a fixed 32 MiB staging
+ * array that the library holds on every device it runs on until the
+ * process ends, where bh_set_potential allocates a staging buffer of the
+ * whole field (numr*numz*numphi doubles) on first use.  No counterpart in
+ * the reference; for callers that cannot allocate device memory (tight
+ * restart or serving loops, harnesses that forbid hipMalloc).  Plans of
+ * any grid size are accepted up to numphi = 65536 (one 64-column chunk of
+ * all planes must fit); calls that overlap on one GPU share the array and
+ * race.  The plan's device must be current.  Synchronous. */
+int bh_set_potential_noalloc(bh_plan *plan, const double *pot);
+
 /* Device memory the plan holds, in bytes. */
 size_t bh_plan_device_bytes(const bh_plan *plan);
 
