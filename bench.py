@@ -27,7 +27,14 @@ HBM_PEAK_GBS = 8000.0
 # half-step at 258 x 130 x 512, from rocprofv3 FETCH_SIZE (x 2.000, the
 # copy-kernel calibration) + WRITE_SIZE;
 # profiles/potential_solver_258x130x512_hbm_traffic.md
-CONTRACT_TRAFFIC = None  # FP64 table: not yet measured
+CONTRACT_TRAFFIC = ((533848.0 + 2332.0) * 2.0 + (3840.0 + 240.0)) * 1024  # FP64 table
+# the mixed cold inverse DFT (radix 8, x = faces + delta1 + correction sum
+# in, phi and the warm-start state out): HBM bytes per launch (FETCH x 2.000
+# + WRITE) and the FP64 MFMA operations it executes
+# (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512); profiles/potential_solver_258x130x512_{hbm_traffic,mfma}.md
+INV_TRAFFIC = (77127.0 * 2.0 + 293835.0) * 1024
+INV_MFMA_FLOP = 2147328.0 * 512
+FP64_PEAK_TFLOPS = 78.6
 SWEEP_TRAFFIC = ((162531.0 + 144885.0) * 2.0 + (151557.0 + 138717.0)) * 1024 / 2  # mean of the two half-steps
 
 
@@ -35,20 +42,36 @@ def rell2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def cpu_baseline(numr, numz, numphi, rho, threads):
-    """The oracle (C restatement, OpenMP) timed on one cold solve of the same
-    grid; Green's tables zero-filled because their generation is one-time
-    setup (potsetup), which the reference's timer excludes too."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(numr, numz, numphi, rho, threads, samples=5):
+    """The oracle (C restatement, OpenMP) timed on cold solves of the same
+    grid, the median of `samples`.  Its Green's tables are allocated and
+    every page is written with a nonzero constant, so that the 5 GB table
+    stream reaches DRAM; their generation is one-time setup (potsetup),
+    which the reference's timer excludes too."""
     from oracle import oracle as O
     O.set_threads(threads)
-    plan = O.OraclePlan(numr, numz, numphi, with_tables="zero")
-    t0 = time.perf_counter()
-    plan.solve(rho, tstep=0)
-    dt = time.perf_counter() - t0
+    plan = O.OraclePlan(numr, numz, numphi, with_tables="filled")
+    ts = []
+    for _ in range(samples):
+        t0 = time.perf_counter()
+        plan.solve(rho, tstep=0)
+        ts.append(time.perf_counter() - t0)
     del plan
+    dt = float(np.median(ts))
     return {"value": 1.0 / dt, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"1 cold solve {numr}x{numz}x{numphi} (oracle/bh_oracle.c, OpenMP, "
-                      f"tables zero-filled), {dt:.2f} s"}
+            "sample": f"median of {samples} cold solves {numr}x{numz}x{numphi} (oracle/bh_oracle.c, OpenMP, "
+                      f"{threads} threads on {cpu_model()}; Green's tables filled, every page written): "
+                      f"{', '.join(f'{t:.2f}' for t in ts)} s"}
 
 
 def reference_errors(pot, tag="1"):
@@ -80,6 +103,8 @@ def main():
     ap.add_argument("--grid", type=int, nargs=3, default=[258, 130, 512])
     ap.add_argument("--warm", action="store_true", help="time warm solves (tstep > 1, 5 ADI steps)")
     ap.add_argument("--passes", type=int, default=2, help="refinement passes of the headline mixed solve")
+    ap.add_argument("--only", choices=["mixed", "fp64", "mixed_t32"],
+                    help="time one precision alone (profiling runs); prints a reduced line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batched", action="store_true",
@@ -180,6 +205,20 @@ def main():
         d_rho2 = torch.from_numpy(density.config2_density(seed=99)).to(dev)
     else:
         d_rho2 = torch.from_numpy(density.scaled_binary(numr, numz, numphi, seed=99)).to(dev)
+    if args.only:
+        # one precision alone (for profiling runs): the headline fields of
+        # that solve, nothing else
+        prec = {"mixed": "mixed", "fp64": "fp64", "mixed_t32": "mixed_t32"}[args.only]
+        res, _ = run(prec, passes=args.passes)
+        print(json.dumps({"metric": METRIC, "value": res["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+                          "config": {"workload": f"potential_solver {numr}x{numz}x{numphi} tstep={tstep}, "
+                                                 f"{prec} only, {args.passes} refinement passes"},
+                          "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "stage_ms": res["stage_ms"],
+                          "error_vs_reference_cold": res["error_vs_reference_cold"]}), flush=True)
+        return
     fp64, pot64 = run("fp64", warm_steps=args.steps)
     mixed, potmx = run("mixed", passes=args.passes, warm_steps=args.steps)
     mixed["relL2_vs_fp64_gpu"] = rell2(potmx, pot64)
@@ -213,6 +252,15 @@ def main():
     ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 8
     ct_ms = mixed["stage_ms"]["contract_topbot"]
     ct_traffic = CONTRACT_TRAFFIC if (numr, numz, numphi) == (258, 130, 512) else None
+    # the inverse DFT of the mixed cold solve (the stage from mark 3 to 4 is
+    # this one kernel at N = 512): x's faces, delta1 and the correction sum
+    # in (FP32 interior), phi and the warm-start state out
+    c2 = (numr, numz, numphi) == (258, 130, 512)
+    inv_bytes = 2 * 4 * (numr - 2) * (numz - 2) * numphi + (numr * numz - (numr - 2) * (numz - 2)) * numphi * 8 + \
+        2 * 8 * numr * numz * numphi
+    inv_ms = mixed["stage_ms"]["inverse_dft"]
+    inv_traffic = INV_TRAFFIC if c2 else None
+    inv_flop = INV_MFMA_FLOP if c2 else None
     # ADI half-step kernel of the FP64 path: 3 streams (read phi, read
     # rho-hat, write phi) x 8 B x interior points x modes per launch
     nsteps = 5 if args.warm else 20
@@ -258,6 +306,12 @@ def main():
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,
                      "bytes_per_launch": ct_bytes, "launch_ms": ct_ms},
+        "roofline_dft_inverse": {"bound": "hbm", "kernel": "dft_inv8 (mixed cold inverse real DFT, radix 8 on FP64 MFMA)",
+                                 "achieved": inv_bytes / (inv_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": inv_bytes / (inv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": inv_traffic,
+                                 "bytes_per_launch": inv_bytes, "launch_ms": inv_ms,
+                                 "mfma_fp64_tflops": inv_flop / (inv_ms * 1e-3) / 1e12 if inv_flop else None,
+                                 "mfma_fp64_frac": inv_flop / (inv_ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS if inv_flop else None},
         "roofline_fp64_sweep": {"bound": "hbm", "kernel": "line_sweep (FP64 ADI half-step)",
                                 "achieved": sw_bytes / (sw_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                 "unit": "GB/s", "frac": sw_bytes / (sw_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,

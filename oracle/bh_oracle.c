@@ -232,8 +232,10 @@ static void bho_build_tables(bho_plan *p);
 
 /* Create a plan: grid, ADI coefficients, trig and Green's tables.
    with_tables = 1 computes the Green's tables; 0 skips them (bessel must
-   not be called); 2 allocates them zero-filled, for timing the solve
-   arithmetic only (table generation is one-time setup, potsetup).
+   not be called); 2 allocates them filled with a nonzero constant, every
+   page written, for timing the solve's arithmetic and table stream only
+   (table generation is one-time setup, potsetup; the results are then
+   meaningless).
    Returns NULL on invalid arguments. */
 bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, int mmax,
                           int with_tables)
@@ -335,9 +337,19 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
     if (with_tables == 1) bho_build_tables(p);
     if (with_tables == 2) {
         size_t nri = numr - 2, nzi = numz - 2;
-        p->tmr = (double *)xcalloc((size_t)mmax * numr * nzi * nri, sizeof(double));
-        p->bmr = (double *)xcalloc((size_t)mmax * numr * nzi * nri, sizeof(double));
-        p->smz = (double *)xcalloc((size_t)mmax * numz * nzi * nri, sizeof(double));
+        const size_t nt = (size_t)mmax * numr * nzi * nri, ns = (size_t)mmax * numz * nzi * nri;
+        p->tmr = (double *)malloc(nt * sizeof(double));
+        p->bmr = (double *)malloc(nt * sizeof(double));
+        p->smz = (double *)malloc(ns * sizeof(double));
+        if (p->tmr && p->bmr && p->smz) {
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < nt; ++i) {
+                p->tmr[i] = 1e-3;
+                p->bmr[i] = 1e-3;
+            }
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < ns; ++i) p->smz[i] = 1e-3;
+        }
     }
     return p;
 }

@@ -71,10 +71,12 @@ class OraclePlan:
 
     def __init__(self, numr, numz, numphi, dr=1.0 / 127.0, mmax=30, with_tables=True):
         """with_tables: True computes the Green's tables, False skips them,
-        "zero" allocates them zero-filled (timing of the solve arithmetic)."""
+        "filled" allocates them filled with a nonzero constant, every page
+        written (timing of the solve's arithmetic and table stream; the
+        potential is then meaningless)."""
         self.shape = (numphi, numz, numr)
         self.numr, self.numz, self.numphi, self.mmax = numr, numz, numphi, mmax
-        mode = 2 if with_tables == "zero" else (1 if with_tables else 0)
+        mode = 2 if with_tables == "filled" else (1 if with_tables else 0)
         self._h = lib().bho_plan_create(numr, numz, numphi, 1, dr, mmax, mode)
         if not self._h:
             raise ValueError("oracle: invalid geometry")
