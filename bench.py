@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--grid", type=int, nargs=3, default=[258, 130, 512])
     ap.add_argument("--warm", action="store_true", help="time warm solves (tstep > 1, 5 ADI steps)")
     ap.add_argument("--passes", type=int, default=2, help="refinement passes of the headline mixed solve")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the short batched and served runs of the default line")
     ap.add_argument("--only", choices=["mixed", "fp64", "mixed_t32"],
                     help="time one precision alone (profiling runs); prints a reduced line")
     ap.add_argument("--no-cpu", action="store_true")
@@ -320,10 +322,27 @@ def main():
         "mixed": mixed,
         "fp64": fp64,
     }
+    if not args.no_workloads:
+        # the batched (64 uneven grids) and served (Poisson SCF stream)
+        # workloads, short runs
+        import copy
+        ab = copy.copy(args)
+        ab.steps, ab.warmup = 10, 2
+        bt = run_batched(ab, emit=False)
+        out["batched"] = {k: bt[k] for k in ("value", "unit", "ms_per_step", "p50_ms", "p99_ms",
+                                             "grouped_launches_per_step", "one_plan_at_a_time_ms", "batch_speedup",
+                                             "batch_equals_single_plans_bitwise", "tuned_tiles")}
+        out["batched"]["workload"] = bt["config"]["workload"]
+        asv = copy.copy(args)
+        asv.requests, asv.rate = 32, 40.0
+        sv = run_serve(asv, emit=False)
+        out["served"] = {k: sv[k] for k in ("value", "unit", "p50_ms", "p99_ms", "mean_ms", "solves", "wall_s",
+                                            "peak_active", "tail")}
+        out["served"]["workload"] = sv["config"]["workload"]
     print(json.dumps(out), flush=True)
 
 
-def run_batched(args):
+def run_batched(args, emit=True):
     """64 independent grids of seeded uneven sizes, mixed precision, solved
     as one batch; a step is one solve of every grid (cold, 20 ADI steps)."""
     import ctypes
@@ -394,10 +413,12 @@ def run_batched(args):
         "device_bytes": b.device_bytes(),
     }
     b.close()
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
 
 
-def run_serve(args):
+def run_serve(args, emit=True):
     """Poisson arrivals at a fixed rate of SCF requests: grid sizes drawn as in
     the batched workload, 20-50 dependent solves each (cold, then warm) on a
     slowly changing binary of bipolytropes (core n = 3, envelope n = 1.5, core
@@ -436,16 +457,20 @@ def run_serve(args):
         if d > 0:
             time.sleep(d)
         ids.append(srv.submit(g, n, m))
-    lat = []
+    lat, first, later = [], [], []
     for i in ids:
         res, _, l = srv.wait(i)
         if res["status"] != 0:
             raise RuntimeError(f"request {i} failed: {res}")
         lat.extend(l.tolist())
+        first.append(l[0])
+        later.extend(l[1:].tolist())
     wall = time.perf_counter() - t0
     st = srv.stats()
+    log = srv.iterations()
     srv.close()
     lat = np.array(lat)
+    tail = serve_tail(log, lat, len(sizes), (first, later))
     nsolves = int(st["solves"] - base["solves"])
     out = {
         "metric": "served SCF stream: Poisson solves/s with per-solve p50/p99 latency, mixed precision",
@@ -460,9 +485,56 @@ def run_serve(args):
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()), "solves": nsolves, "wall_s": wall,
         "iterations": int(st["iterations"] - base["iterations"]), "peak_active": st["peak_active"],
-        "peak_device_bytes": st["peak_bytes"], "warmup_s": setup_s,
+        "peak_device_bytes": st["peak_bytes"], "warmup_s": setup_s, "tail": tail,
     }
-    print(json.dumps(out), flush=True)
+    if emit:
+        print(json.dumps(out), flush=True)
+    return out
+
+
+def serve_tail(log, lat, skip, first=None):
+    """Where the served latency tail comes from, from the loop's iteration
+    log (rows: t_admit, t_enq0, t_enq1, t_done, solves, plans created,
+    points; the first `skip` iterations are the warm-up).  A solve's latency
+    is its iteration's completion minus the previous one's, so the interval
+    between completions, weighted by the solves in the iteration, stands for
+    the per-solve latency; it is split by whether the admission before the
+    iteration created a plan, and related to the points solved."""
+    later = None
+    if first is not None:
+        first, later = np.asarray(first[0]), np.asarray(first[1])
+    log = log[skip:]
+    if len(log) < 3:
+        return None
+    t_done = log[:, 3]
+    # from the later of the previous completion and this iteration's enqueue
+    # (an iteration after an idle gap starts when it is enqueued)
+    iv = (t_done[1:] - np.maximum(t_done[:-1], log[1:, 1])) * 1e3
+    rows = log[1:]
+    w = rows[:, 4]
+    p99 = float(np.percentile(lat, 99))
+    tail_it = iv >= p99
+    created = rows[:, 5] > 0
+    admit_ms = (rows[:, 1] - rows[:, 0]) * 1e3
+    return {
+        "iterations": int(len(rows)),
+        "interval_ms_p50_p99": [float(np.percentile(iv, 50)), float(np.percentile(iv, 99))],
+        "solves_per_iteration_p50_max": [float(np.median(w)), float(w.max())],
+        "mpoints_per_iteration_p50_p99": [float(np.percentile(rows[:, 6], 50)) / 1e6,
+                                          float(np.percentile(rows[:, 6], 99)) / 1e6],
+        "tail_iterations": int(tail_it.sum()),
+        "tail_iterations_with_plan_creation": int((tail_it & created).sum()),
+        "iterations_with_plan_creation": int(created.sum()),
+        "admission_ms_p50_max": [float(np.median(admit_ms)), float(admit_ms.max())],
+        "mean_mpoints_tail_vs_all": [float(rows[tail_it, 6].mean()) / 1e6 if tail_it.any() else None,
+                                     float(rows[:, 6].mean()) / 1e6],
+        # first solves of a request count from the submission (admission at
+        # the next iteration boundary, behind the iteration in flight)
+        "first_solve_ms_p50_p99": ([float(np.percentile(first, 50)), float(np.percentile(first, 99))]
+                                   if first is not None and len(first) else None),
+        "later_solves_ms_p50_p99": ([float(np.percentile(later, 50)), float(np.percentile(later, 99))]
+                                    if later is not None and len(later) else None),
+    }
 
 
 if __name__ == "__main__":

@@ -240,6 +240,15 @@ struct Slot {
 struct Iteration {
     hipEvent_t ev;
     std::vector<std::pair<Slot *, int>> solves;  // (slot, solve index)
+    size_t rec = 0;                              // its entry in bh_server::iter_log
+};
+
+// Host-side record of one iteration (seconds since the server started):
+// the admission that preceded it, the enqueue, and the completion.
+struct IterRec {
+    double t_admit, t_enq0, t_enq1, t_done;
+    int nsolves, new_shells;
+    double points;  // grid points solved in the iteration
 };
 
 std::string grid_key(const bh_grid_desc &d)
@@ -263,6 +272,7 @@ struct bh_server {
     long long next_id = 1;
     std::deque<Req *> incoming;
     std::map<long long, Req *> finished;
+    std::set<long long> waited;  // ids a bh_server_wait has taken (or is waiting for)
     // worker-owned
     std::vector<Slot *> active;
     std::map<std::string, std::vector<Shell *>> pool;
@@ -280,6 +290,10 @@ struct bh_server {
     std::vector<double> lat_ms;
     double t0 = 0, t_first = -1, t_last = 0;
     std::string worker_error;
+    // iteration log (worker-written, read under mu)
+    std::vector<IterRec> iter_log;
+    int new_shells = 0;    // shells created since the last iteration
+    double t_admit = 0;    // start of the last admission pass
 };
 
 namespace {
@@ -322,6 +336,7 @@ Shell *take_shell(bh_server *S, const bh_grid_desc &d, int *rc)
     }
     std::unique_ptr<Shell> sh(new Shell());
     sh->key = key;
+    S->new_shells++;
     if ((*rc = plan_create(d, S->cfg.precision, 0, &sh->plan)) != BH_OK) return nullptr;
     const size_t n = (size_t)d.numr * d.numz * d.numphi;
     if ((*rc = sh->mem.alloc(&sh->rho, n)) != BH_OK || (*rc = sh->mem.alloc(&sh->pot, n)) != BH_OK) return nullptr;
@@ -461,6 +476,12 @@ int enqueue_iteration(bh_server *S, bool *any)
         sl->enq++;
     }
     if (it.solves.empty()) return BH_OK;
+    IterRec rec{S->t_admit - S->t0, now_s() - S->t0, 0, 0, (int)it.solves.size(), S->new_shells, 0};
+    S->new_shells = 0;
+    for (auto &sv : it.solves) {
+        const Geo &g = sv.first->sh->plan->g;
+        rec.points += (double)g.numr * g.numz * g.numphi;
+    }
     bool same;
     int rc;
     if ((rc = S->ex.prepare(S->lists, &same)) != BH_OK) return rc;
@@ -474,6 +495,12 @@ int enqueue_iteration(bh_server *S, bool *any)
     it.ev = S->free_events.back();
     S->free_events.pop_back();
     HIPCHK(hipEventRecord(it.ev, S->stream));
+    rec.t_enq1 = now_s() - S->t0;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        it.rec = S->iter_log.size();
+        S->iter_log.push_back(rec);
+    }
     S->inflight.push_back(std::move(it));
     *any = true;
     return BH_OK;
@@ -491,6 +518,7 @@ int complete_oldest(bh_server *S)
     {
         std::lock_guard<std::mutex> lk(S->mu);
         S->iterations++;
+        if (it.rec < S->iter_log.size()) S->iter_log[it.rec].t_done = t - S->t0;
         for (auto &sv : it.solves) {
             Slot *sl = sv.first;
             const double ms = (t - sl->t_ready) * 1e3;
@@ -535,6 +563,7 @@ void worker_main(bh_server *S)
             });
             if (S->stop && S->incoming.empty() && S->active.empty() && S->inflight.empty()) return;
         }
+        S->t_admit = now_s();
         admit(S);
         bool any = false;
         int rc = enqueue_iteration(S, &any);
@@ -636,6 +665,10 @@ extern "C" int bh_server_wait(bh_server *S, long long id, bh_request_result *res
     if (!S) return fail(BH_E_INVALID, "null server");
     std::unique_lock<std::mutex> lk(S->mu);
     if (id <= 0 || id >= S->next_id) return fail(BH_E_INVALID, "unknown request id");
+    // each id is waited for once: a second wait (or a concurrent one) would
+    // never see it finish
+    if (!S->waited.insert(id).second)
+        return fail(BH_E_STATE, "request " + std::to_string(id) + " is already being or has been waited for");
     S->done_cv.wait(lk, [&] { return S->finished.count(id) > 0; });
     Req *q = S->finished[id];
     S->finished.erase(id);
@@ -734,4 +767,21 @@ extern "C" int bhx_density(const bh_grid_desc *desc, const bh_binary_model *mode
     launch_op(density_op(model_args(m, h.g, solve_index, d_rho, p)), 0);
     HIPCHK(hipDeviceSynchronize());
     return BH_OK;
+}
+
+// Iteration log (test and bench hook, not part of the public ABI): up to n
+// records of 8 doubles (t_admit, t_enq0, t_enq1, t_done, nsolves,
+// new_shells, points, 0) into out; returns the number of iterations logged.
+extern "C" int bhx_server_iterations(bh_server *S, double *out, int n)
+{
+    if (!S) return fail(BH_E_INVALID, "null server");
+    std::lock_guard<std::mutex> lk(S->mu);
+    const int m = (int)S->iter_log.size();
+    for (int i = 0; i < m && i < n && out; ++i) {
+        const IterRec &r = S->iter_log[i];
+        const double v[8] = {r.t_admit, r.t_enq0, r.t_enq1, r.t_done, (double)r.nsolves, (double)r.new_shells,
+                             r.points, 0.0};
+        for (int k = 0; k < 8; ++k) out[8 * i + k] = v[k];
+    }
+    return m;
 }

@@ -378,6 +378,19 @@ class PotentialServer:
     def drain(self):
         _check(load_library().bh_server_drain(self._h))
 
+    def iterations(self) -> np.ndarray:
+        """Host-side log of the serving loop's iterations, one row each:
+        t_admit, t_enq0, t_enq1, t_done (seconds since the server started;
+        the admission pass before the iteration, its enqueue, its completion),
+        solves in it, plans created for it, grid points solved."""
+        L = load_library()
+        f = L.bhx_server_iterations
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        n = f(self._h, None, 0)
+        out = np.zeros((max(n, 1), 8))
+        n = f(self._h, out.ctypes.data, n)
+        return out[:n, :7]
+
     def stats(self) -> dict:
         st = ServerStats()
         _check(load_library().bh_server_get_stats(self._h, ctypes.byref(st)))

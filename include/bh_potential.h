@@ -231,7 +231,8 @@ int bh_server_create(const bh_server_config *config, bh_server **out);
  * not fit in hbm_budget even alone are refused with BH_E_NOMEM. */
 int bh_server_submit(bh_server *server, const bh_request *request, long long *id);
 /* Wait for request id to finish; fills *result (may be NULL).  Each id can
- * be waited for once. */
+ * be waited for once: a second call for it, or one while another thread
+ * waits for it, returns BH_E_STATE at once. */
 int bh_server_wait(bh_server *server, long long id, bh_request_result *result);
 /* Wait until every submitted request has finished. */
 int bh_server_drain(bh_server *server);

@@ -105,3 +105,15 @@ def test_admission_by_device_memory_budget():
         with pytest.raises(P.BHError) as e:
             srv.submit((258, 130, 512), 2, model)
         assert e.value.code == -2
+
+
+def test_a_request_id_is_waited_for_once():
+    with P.PotentialServer(precision="mixed") as srv:
+        rid = srv.submit((42, 26, 16), 2, dict(radius=(5.0, 4.0), separation=14.0, seed=1))
+        res, _, _ = srv.wait(rid)
+        assert res["status"] == 0
+        with pytest.raises(P.BHError) as e:
+            srv.wait(rid)
+        assert e.value.code == -5
+        log = srv.iterations()
+        assert log.shape[0] >= 2 and np.all(log[:, 3] >= log[:, 2]) and np.all(log[:, 2] >= log[:, 1])
