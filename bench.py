@@ -331,7 +331,7 @@ def main():
         bt = run_batched(ab, emit=False)
         out["batched"] = {k: bt[k] for k in ("value", "unit", "ms_per_step", "p50_ms", "p99_ms",
                                              "grouped_launches_per_step", "one_plan_at_a_time_ms", "batch_speedup",
-                                             "batch_equals_single_plans_bitwise", "tuned_tiles")}
+                                             "batch_equals_single_plans_bitwise", "autotune")}
         out["batched"]["workload"] = bt["config"]["workload"]
         asv = copy.copy(args)
         asv.requests, asv.rate = 32, 40.0
@@ -362,7 +362,12 @@ def run_batched(args, emit=True):
     t0 = time.perf_counter()
     b = P.PotentialBatch([(r, z, p) for r, z, p, _ in grids], precision="mixed", graph=True, autotune=True)
     setup_s = time.perf_counter() - t0
-    for _ in range(5 + args.warmup):  # five autotuning solves, then warm-up of the tuned graph
+    b.solve_device(rp, pp, 0, sp)  # the first autotuning solve
+    ntune = 1
+    while b.stats()["tuning_left"] > 0:  # the others, one candidate each
+        b.solve_device(rp, pp, 0, sp)
+        ntune += 1
+    for _ in range(args.warmup):  # then warm-up of the tuned graph
         b.solve_device(rp, pp, 0, sp)
     torch.cuda.synchronize()
     ref = [x.cpu() for x in d_pot]
@@ -396,6 +401,27 @@ def run_batched(args, emit=True):
     torch.cuda.synchronize()
     seq_ms = (time.perf_counter() - t0) * 1e3 / nseq
     ms = wall * 1e3 / args.steps
+    # what the tuning buys: the same batch with every launch at its default
+    # shape (graph on, no autotuning), timed the same way
+    tune = b.tuning()
+    b.close()
+    bd = P.PotentialBatch([(r, z, p) for r, z, p, _ in grids], precision="mixed", graph=True, autotune=False)
+    for _ in range(1 + args.warmup):
+        bd.solve_device(rp, pp, 0, sp)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        bd.solve_device(rp, pp, 0, sp)
+    torch.cuda.synchronize()
+    default_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    same_default = all(torch.equal(x.cpu(), y) for x, y in zip(d_pot, ref))
+    bd.close()
+    tuning = {}
+    for ax, v in tune.items():
+        ch = [c for c in v["choice"] if c >= 0]
+        if ch:
+            tuning[ax] = {"steps_tuned": len(ch), "choices": {str(c): ch.count(c) for c in sorted(set(ch))},
+                          "gain_ms_in_tuning": v["gain_ms"]}
     out = {
         "metric": "batched uneven Poisson solves/s (grids solved per second), mixed precision",
         "value": len(grids) * 1e3 / ms, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -407,12 +433,12 @@ def run_batched(args, emit=True):
                                "radius; cold solves", "grids": [g[:3] for g in grids]},
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "grouped_launches_per_step": st["launches"], "graph": st["graph"],
-        "tuned_tiles": b.tuning()[0], "setup_s": setup_s,
+        "autotune": {"tuning_solves": ntune, "axes": tuning, "default_shapes_ms": default_ms,
+                     "tuned_ms": ms, "speedup_over_default": default_ms / ms},
+        "setup_s": setup_s,
         "one_plan_at_a_time_ms": seq_ms, "batch_speedup": seq_ms / ms,
-        "batch_equals_single_plans_bitwise": bool(same and same_single),
-        "device_bytes": b.device_bytes(),
+        "batch_equals_single_plans_bitwise": bool(same and same_single and same_default),
     }
-    b.close()
     if emit:
         print(json.dumps(out), flush=True)
     return out

@@ -14,11 +14,12 @@
 using namespace bh;
 
 namespace {
-// candidates timed by the autotuner, one per solve: GEMM tile shape, and
-// line-sweep lines per workgroup
-constexpr int TUNE_TILES[] = {0, 1, 2, 3, 4};
-constexpr int TUNE_NL[] = {16, 8, 16, 8, 16};
-constexpr int TUNE_ROUNDS = 5;
+// Candidates of each tuning axis (bh_exec.h), the default first, and the Op
+// kind each applies to (-1: every kernel launch).
+const std::vector<int> TUNE_CANDS[TUNE_AXES] = {{2, 0, 1, 3, 4}, {16, 8}, {4, 2, 8}, {0, 1, 2, 4}, {0, 1}};
+constexpr int TUNE_KIND[TUNE_AXES] = {OP_GEMM, OP_SWEEP, OP_RESIDUAL, -1, -1};
+
+bool axis_applies(int axis, int kind) { return kind != OP_MARK && (TUNE_KIND[axis] < 0 || TUNE_KIND[axis] == kind); }
 }  // namespace
 
 struct bh_batch {
@@ -26,9 +27,16 @@ struct bh_batch {
     size_t budget = 0, bytes = 0;
     std::vector<bh_plan *> plans;
     GroupExec ex;
-    Tuning tune;
-    int tune_round = 0;  // next autotuning round (TUNE_ROUNDS: done)
-    std::vector<std::vector<double>> tune_ms;  // [round][position]
+    // autotuning: the axes in use, searched one after another, every
+    // candidate of the current axis timed at every position it applies to
+    // (one solve per candidate) with the other axes at their best so far
+    Tuning tune;                    // best choices so far (the result, once done)
+    bool tune_init = false;
+    std::vector<int> tune_axes;     // axes with at least one position
+    int tune_ax = 0, tune_cand = 0; // next round: axis tune_axes[tune_ax], candidate tune_cand
+    int tune_total = 0, tune_done = 0;
+    std::vector<std::vector<double>> tune_ms;  // [candidate][position] of the current axis
+    std::vector<double> tune_gain;  // per axis searched: default ms - chosen ms (summed over positions)
     hipStream_t cap_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -66,52 +74,78 @@ int ensure_events(bh_batch *b, int n)
     return BH_OK;
 }
 
-// Tuning of autotuning round r: candidate r at every tunable position.
-Tuning round_tuning(const bh_batch *b, int r)
+size_t max_len(const bh_batch *b)
 {
-    Tuning t;
-    size_t maxlen = 0;
-    for (const auto &l : b->lists) maxlen = std::max(maxlen, l.size());
-    t.tile.assign(maxlen, -1);
-    t.nl.assign(maxlen, -1);
+    size_t m = 0;
+    for (const auto &l : b->lists) m = std::max(m, l.size());
+    return m;
+}
+
+// positions where `axis` applies
+std::vector<bool> axis_positions(const bh_batch *b, int axis)
+{
+    std::vector<bool> on(max_len(b), false);
     for (const auto &l : b->lists)
-        for (size_t k = 0; k < l.size(); ++k) {
-            if (l[k].kind == OP_GEMM) t.tile[k] = TUNE_TILES[r];
-            else if (l[k].kind == OP_SWEEP) t.nl[k] = TUNE_NL[r];
-        }
+        for (size_t k = 0; k < l.size(); ++k)
+            if (axis_applies(axis, l[k].kind)) on[k] = true;
+    return on;
+}
+
+// First tuning solve: the axes in use and the number of rounds.
+void init_tuning(bh_batch *b)
+{
+    const size_t m = max_len(b);
+    for (int ax = 0; ax < TUNE_AXES; ++ax) b->tune.v[ax].assign(m, -1);
+    b->tune_axes.clear();
+    b->tune_total = 0;
+    for (int ax = 0; ax < TUNE_AXES; ++ax) {
+        const std::vector<bool> on = axis_positions(b, ax);
+        if (std::find(on.begin(), on.end(), true) == on.end()) continue;
+        b->tune_axes.push_back(ax);
+        b->tune_total += (int)TUNE_CANDS[ax].size();
+    }
+    b->tune_ax = b->tune_cand = b->tune_done = 0;
+    b->tune_ms.clear();
+    b->tune_gain.assign(TUNE_AXES, 0.0);
+    b->tune_init = true;
+}
+
+// Tuning of the next round: the best so far, the current axis at its
+// current candidate wherever it applies.
+Tuning round_tuning(const bh_batch *b)
+{
+    Tuning t = b->tune;
+    const int ax = b->tune_axes[b->tune_ax];
+    const std::vector<bool> on = axis_positions(b, ax);
+    t.v[ax].assign(on.size(), -1);
+    for (size_t k = 0; k < on.size(); ++k)
+        if (on[k]) t.v[ax][k] = TUNE_CANDS[ax][b->tune_cand];
     return t;
 }
 
-// After the last round: the fastest candidate per position.
-void finish_tuning(bh_batch *b)
+// After the last candidate of an axis: the fastest per position.
+void finish_axis(bh_batch *b)
 {
-    size_t maxlen = 0;
-    for (const auto &l : b->lists) maxlen = std::max(maxlen, l.size());
-    Tuning best;
-    best.tile.assign(maxlen, -1);
-    best.nl.assign(maxlen, -1);
-    for (size_t k = 0; k < maxlen; ++k) {
-        double bt = 1e300;
-        int br = -1;
-        for (int r = 0; r < TUNE_ROUNDS; ++r) {
-            if (k >= b->tune_ms[r].size()) continue;
-            const double t = b->tune_ms[r][k];
-            if (t > 0.0 && t < bt) {
-                bt = t;
-                br = r;
+    const int ax = b->tune_axes[b->tune_ax];
+    const std::vector<bool> on = axis_positions(b, ax);
+    b->tune.v[ax].assign(on.size(), -1);
+    double gain = 0.0;
+    for (size_t k = 0; k < on.size(); ++k) {
+        if (!on[k]) continue;
+        double best = 1e300;
+        int bc = 0;
+        for (size_t c = 0; c < b->tune_ms.size(); ++c) {
+            const double t = k < b->tune_ms[c].size() ? b->tune_ms[c][k] : 0.0;
+            if (t > 0.0 && t < best) {
+                best = t;
+                bc = (int)c;
             }
         }
-        if (br < 0) continue;
-        bool gemm = false, sweep = false;
-        for (const auto &l : b->lists)
-            if (k < l.size()) {
-                gemm |= l[k].kind == OP_GEMM;
-                sweep |= l[k].kind == OP_SWEEP;
-            }
-        if (gemm) best.tile[k] = TUNE_TILES[br];
-        if (sweep) best.nl[k] = TUNE_NL[br];
+        b->tune.v[ax][k] = TUNE_CANDS[ax][bc];
+        if (k < b->tune_ms[0].size() && best < 1e300) gain += b->tune_ms[0][k] - best;
     }
-    b->tune = best;
+    b->tune_gain[ax] = gain;
+    b->tune_ms.clear();
 }
 
 int batch_enqueue(bh_batch *b, const double *const *d_rho, double *const *d_pot, const int *tstep, hipStream_t s)
@@ -119,11 +153,12 @@ int batch_enqueue(bh_batch *b, const double *const *d_rho, double *const *d_pot,
     const int n = (int)b->plans.size();
     b->lists.resize(n);
     for (int i = 0; i < n; ++i) build_ops(b->plans[i], d_rho[i], d_pot[i], tstep ? tstep[i] : 0, &b->lists[i]);
-    const bool tuning = (b->flags & BH_BATCH_AUTOTUNE) && b->tune_round < TUNE_ROUNDS;
-    const Tuning t = tuning ? round_tuning(b, b->tune_round) : b->tune;
+    if ((b->flags & BH_BATCH_AUTOTUNE) && !b->tune_init) init_tuning(b);
+    const bool tuning = (b->flags & BH_BATCH_AUTOTUNE) && b->tune_ax < (int)b->tune_axes.size();
+    const Tuning t = tuning ? round_tuning(b) : b->tune;
     for (auto &l : b->lists) apply_tuning(t, &l);
     bool same = false;
-    int rc = b->ex.prepare(b->lists, &same);
+    int rc = b->ex.prepare(b->lists, &same, &t);
     if (rc != BH_OK) return rc;
     if ((rc = b->ex.upload(s)) != BH_OK) return rc;
     const int nl = b->ex.nlaunches();
@@ -144,7 +179,12 @@ int batch_enqueue(bh_batch *b, const double *const *d_rho, double *const *d_pot,
         float tot;
         HIPCHK(hipEventElapsedTime(&tot, b->ev[0], b->ev[nl]));
         b->last_ms = tot;
-        if (++b->tune_round == TUNE_ROUNDS) finish_tuning(b);
+        b->tune_done++;
+        if (++b->tune_cand == (int)TUNE_CANDS[b->tune_axes[b->tune_ax]].size()) {
+            finish_axis(b);
+            b->tune_ax++;
+            b->tune_cand = 0;
+        }
         return BH_OK;
     }
     if (b->flags & BH_BATCH_GRAPH) {
@@ -305,20 +345,20 @@ extern "C" int bh_batch_get_stats(bh_batch *b, bh_batch_stats *st)
     st->ngrids = (int)b->plans.size();
     st->launches = b->ex.nlaunches();
     st->graph = b->gexec != nullptr;
-    st->tuning_left = (b->flags & BH_BATCH_AUTOTUNE) ? std::max(0, TUNE_ROUNDS - b->tune_round) : 0;
+    st->tuning_left = (b->flags & BH_BATCH_AUTOTUNE) ? (b->tune_init ? b->tune_total - b->tune_done : -1) : 0;
     st->last_ms = b->last_ms;
     return BH_OK;
 }
 
-extern "C" int bh_batch_tuning(const bh_batch *b, int *tile, int *nl, int n)
+extern "C" int bh_batch_tuning(const bh_batch *b, int axis, int *choice, int n, double *gain_ms)
 {
     if (!b) return fail(BH_E_INVALID, "null batch");
-    const int m = (int)std::max(b->tune.tile.size(), b->tune.nl.size());
-    for (int k = 0; k < n && k < m; ++k) {
-        if (tile) tile[k] = b->tune.at(b->tune.tile, k);
-        if (nl) nl[k] = b->tune.at(b->tune.nl, k);
-    }
-    return m;
+    if (axis < 0 || axis >= TUNE_AXES) return fail(BH_E_INVALID, "bh_batch_tuning: axis out of range");
+    const std::vector<int> &v = b->tune.v[axis];
+    for (int k = 0; k < n && k < (int)v.size(); ++k)
+        if (choice) choice[k] = v[k];
+    if (gain_ms) *gain_ms = axis < (int)b->tune_gain.size() ? b->tune_gain[axis] : 0.0;
+    return (int)v.size();
 }
 
 extern "C" int bh_batch_plan(bh_batch *b, int i, bh_plan **plan)

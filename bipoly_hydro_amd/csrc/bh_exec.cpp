@@ -1,6 +1,7 @@
 // bh_exec.cpp -- grouped execution of several grids' Op lists (see bh_exec.h).
 #include "bh_exec.h"
 #include "bh_gemm.h"
+#include "bh_mixed.h"
 #include "bh_internal.h"
 #include "bh_plan_impl.h"
 
@@ -17,15 +18,18 @@ void apply_tuning(const Tuning &t, std::vector<Op> *ops)
     for (size_t k = 0; k < ops->size(); ++k) {
         Op &op = (*ops)[k];
         if (op.kind == OP_GEMM) {
-            const int tile = t.at(t.tile, k);
+            const int tile = t.at(TUNE_TILE, k);
             if (tile >= 0) {
                 GemmArgs a = op.as<GemmArgs>();
                 a.tile = tile;
                 op = gemm_op(a, op.variant / 8);
             }
         } else if (op.kind == OP_SWEEP) {
-            const int nl = t.at(t.nl, k);
+            const int nl = t.at(TUNE_NL, k);
             if (nl > 0) op = sweep_op(op.as<LineArgs>(), op.ny, nl);
+        } else if (op.kind == OP_RESIDUAL) {
+            const int rows = t.at(TUNE_ROWS, k);
+            if (rows > 0) op = residual_op(op.as<ResidualArgs>(), op.nz, rows);
         }
     }
 }
@@ -60,7 +64,7 @@ GroupExec::~GroupExec()
     if (dev_) (void)hipFree(dev_);
 }
 
-int GroupExec::prepare(const std::vector<std::vector<Op>> &lists, bool *same_layout)
+int GroupExec::prepare(const std::vector<std::vector<Op>> &lists, bool *same_layout, const Tuning *tune)
 {
     if (!cus_) {
         int dev = 0;
@@ -122,6 +126,19 @@ int GroupExec::prepare(const std::vector<std::vector<Op>> &lists, bool *same_lay
             if (total > 0x7fffffffL) return fail(BH_E_INVALID, "grouped launch with more than 2^31 tiles");
             L.total = (int)total;
             L.nblocks = persistent_blocks(total, occupancy(L.kind, L.variant));
+            if (tune) {
+                // a cap of g workgroups per CU (a multiple of 8 keeps the
+                // XCD-chunked tile order), and one more than a multiple of
+                // 8 for the linear order (BH_TILE_LOOP chunks only grids
+                // that are multiples of 8)
+                const int g = tune->at(TUNE_GRID, k), lin = tune->at(TUNE_ORDER, k);
+                int nb = L.nblocks & ~7;
+                if (g > 0) nb = std::max(8, std::min(nb, g * cus_ / 8 * 8));
+                L.nblocks = (lin == 1 && nb < 0x7ffffff8) ? nb + 1 : nb;
+            }
+            char sigb[32];
+            std::snprintf(sigb, sizeof sigb, "#%d", L.nblocks);
+            layout += sigb;
             launches_.push_back(L);
         }
     }

@@ -16,12 +16,20 @@
 
 namespace bh {
 
-// Per-position overrides chosen by the autotuner: GEMM tile shape and
-// line-sweep lines per workgroup (-1: each grid's default).
+// Per-position overrides chosen by the autotuner (-1: the default), one
+// vector per axis, indexed by Op position:
+//   tile   GEMM tile shape (0..4)
+//   nl     line-sweep lines per workgroup (8, 16)
+//   rows   residual K rows per tile (2, 4, 8)
+//   grid   persistent workgroups per CU of the grouped launch (0: one per
+//          tile, as many as the hardware keeps resident)
+//   order  tile order of the grouped launch: 0 XCD-chunked, 1 linear
+// None of them changes any result: every choice computes each value with
+// the same operations in the same order.
+enum { TUNE_TILE = 0, TUNE_NL, TUNE_ROWS, TUNE_GRID, TUNE_ORDER, TUNE_AXES };
 struct Tuning {
-    std::vector<int> tile;  // indexed by Op position
-    std::vector<int> nl;
-    int at(const std::vector<int> &v, size_t k) const { return k < v.size() ? v[k] : -1; }
+    std::vector<int> v[TUNE_AXES];
+    int at(int axis, size_t k) const { return k < v[axis].size() ? v[axis][k] : -1; }
 };
 
 // Apply the overrides of `t` to one grid's Op list.
@@ -37,7 +45,7 @@ class GroupExec {
     // Lay out the grouped launches for these Op lists.  Returns BH_OK; sets
     // *same_layout when the launch sequence and arena layout equal the
     // previous call's (a captured graph of them is still valid).
-    int prepare(const std::vector<std::vector<Op>> &lists, bool *same_layout);
+    int prepare(const std::vector<std::vector<Op>> &lists, bool *same_layout, const Tuning *tune = nullptr);
     // Copy the argument arena to the device, stream-ordered on s.
     int upload(hipStream_t s);
     // Enqueue the grouped launches on s.  With `events` (size nlaunches()+1)

@@ -55,7 +55,8 @@ struct TridiagArgs {
     const double *betar;
     int nri, nzi;
 };
-Op residual_op(const ResidualArgs &a, int nplanes);
+// rows: K rows per tile, 2, 4 or 8 (0: the default); the same results for any
+Op residual_op(const ResidualArgs &a, int nplanes, int rows = 0);
 // The cold first residual where the interior of r already holds -4 pi rh
 // (written by the forward FP32 DFT): recompute r at the interior points
 // next to a face (rows K = 2 and numz - 1, column J = numr - 1), the only

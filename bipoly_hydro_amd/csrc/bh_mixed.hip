@@ -32,8 +32,10 @@ namespace bh {
 // (numr x numz) planes in J-layout; dlt and w have zero faces.  r is written
 // compact: r[(K-2)*nri + (J-2)].
 // ---------------------------------------------------------------------------
-// Tile: 256 consecutive J of one plane and RES_ROWS rows of K, the column's
-// rows K0-1 .. K0+RES_ROWS and the J coefficients loaded once.
+// Tile: 256 consecutive J of one plane and ROWS rows of K (2, 4 or 8: the
+// op variant, an autotuning choice; the default RES_ROWS), the column's rows
+// K0-1 .. K0+ROWS and the J coefficients loaded once.  Every point's
+// arithmetic is the same for any ROWS.
 // The variant fixes at compile time whether delta and w are present (UPD:
 // the refinement pass), whether Xo is written (WX; a solve whose x is
 // formed later by the inverse DFT from the corrections leaves it out),
@@ -41,7 +43,7 @@ namespace bh {
 // (COLD), so that the row loop carries no pointer tests.
 constexpr int RES_ROWS = 4;
 
-template <bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false>
+template <int ROWS, bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz)
 {
     const int numr = a.numr, nri = numr - 2, nzi = a.numz - 2;
@@ -50,7 +52,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     const int jr = bx * 256 + threadIdx.x;
     if (jr >= nri) return;
     const int J = jr + 2;
-    const int K0 = by * RES_ROWS + 2, K1 = min(K0 + RES_ROWS, nzi + 2);
+    const int K0 = by * ROWS + 2, K1 = min(K0 + ROWS, nzi + 2);
     const double *__restrict__ X = a.X + L * plane;
     const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
     const float *__restrict__ dl2 = ADD2 ? a.dlt2 + L * plane : nullptr;
@@ -73,15 +75,15 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     // the stores follow the arithmetic (stores share the load counter, so
     // interleaving them would serialise the rows).
     const bool east_face = J + 1 == numr, has_west = J > 2;
-    // interior tiles (not EDGE) have RES_ROWS rows and no face row: their
+    // interior tiles (not EDGE) have ROWS rows and no face row: their
     // loads are straight-line code (a branch around a load makes the
     // compiler wait for every load in flight where the paths join)
-    const int nk = EDGE ? K1 - K0 : RES_ROWS;
+    const int nk = EDGE ? K1 - K0 : ROWS;
     const long c0 = (long)(K0 - 1) * numr + (J - 1);  // (J, K0)
-    double xcv[RES_ROWS + 2], xev[RES_ROWS], xwv[RES_ROWS];  // xcv[i]: row K0 - 1 + i
-    float rv[RES_ROWS], wv[RES_ROWS];
+    double xcv[ROWS + 2], xev[ROWS], xwv[ROWS];  // xcv[i]: row K0 - 1 + i
+    float rv[ROWS], wv[ROWS];
 #pragma unroll
-    for (int i = 0; i < RES_ROWS + 2; ++i) {
+    for (int i = 0; i < ROWS + 2; ++i) {
         const int K = K0 - 1 + i;
         const long ci = c0 + (long)(i - 1) * numr;
         if (!EDGE) {
@@ -92,7 +94,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
         }
     }
 #pragma unroll
-    for (int i = 0; i < RES_ROWS; ++i) {
+    for (int i = 0; i < ROWS; ++i) {
         const long ci = c0 + (long)i * numr;
         xev[i] = 0.0;
         xwv[i] = 0.0;
@@ -112,7 +114,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
         }
     }
 #pragma unroll
-    for (int i = 0; i < RES_ROWS; ++i) {
+    for (int i = 0; i < ROWS; ++i) {
         if (i < nk) {
             const long ci = c0 + (long)i * numr;
             const double xc = xcv[i + 1];
@@ -130,30 +132,34 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
 // the refinement form (delta and w given, Xo written or not) or the first
 // residual, cold or warm; the first and last tiles of a plane (face rows
 // K = 1, numz) apart
-template <bool EDGE>
+template <int ROWS, bool EDGE>
 __device__ __forceinline__ void residual_tile_e(const ResidualArgs &a, int bx, int by, int bz)
 {
     switch ((a.dlt ? (a.Xo ? 2 : a.dlt2 ? 6 : 4) : 0) + (a.cold ? 1 : 0)) {
-    case 0: residual_tile_v<false, false, EDGE, false>(a, bx, by, bz); break;
-    case 1: residual_tile_v<false, true, EDGE, false>(a, bx, by, bz); break;
-    case 2: residual_tile_v<true, false, EDGE, true>(a, bx, by, bz); break;
-    case 3: residual_tile_v<true, true, EDGE, true>(a, bx, by, bz); break;
-    case 4: residual_tile_v<true, false, EDGE, false>(a, bx, by, bz); break;
-    case 5: residual_tile_v<true, true, EDGE, false>(a, bx, by, bz); break;
-    case 6: residual_tile_v<true, false, EDGE, false, true>(a, bx, by, bz); break;
-    default: residual_tile_v<true, true, EDGE, false, true>(a, bx, by, bz); break;
+    case 0: residual_tile_v<ROWS, false, false, EDGE, false>(a, bx, by, bz); break;
+    case 1: residual_tile_v<ROWS, false, true, EDGE, false>(a, bx, by, bz); break;
+    case 2: residual_tile_v<ROWS, true, false, EDGE, true>(a, bx, by, bz); break;
+    case 3: residual_tile_v<ROWS, true, true, EDGE, true>(a, bx, by, bz); break;
+    case 4: residual_tile_v<ROWS, true, false, EDGE, false>(a, bx, by, bz); break;
+    case 5: residual_tile_v<ROWS, true, true, EDGE, false>(a, bx, by, bz); break;
+    case 6: residual_tile_v<ROWS, true, false, EDGE, false, true>(a, bx, by, bz); break;
+    default: residual_tile_v<ROWS, true, true, EDGE, false, true>(a, bx, by, bz); break;
     }
 }
-__device__ __forceinline__ void residual_tile(const ResidualArgs &a, int bx, int by, int bz)
+template <int ROWS>
+__device__ __forceinline__ void residual_tile_r(const ResidualArgs &a, int bx, int by, int bz)
 {
-    const int nzi = a.numz - 2, K0 = by * RES_ROWS + 2;
-    if (by == 0 || K0 + RES_ROWS >= nzi + 2) residual_tile_e<true>(a, bx, by, bz);
-    else residual_tile_e<false>(a, bx, by, bz);
+    const int nzi = a.numz - 2, K0 = by * ROWS + 2;
+    if (by == 0 || K0 + ROWS >= nzi + 2) residual_tile_e<ROWS, true>(a, bx, by, bz);
+    else residual_tile_e<ROWS, false>(a, bx, by, bz);
 }
 
-Op residual_op(const ResidualArgs &a, int nplanes)
+int residual_rows(int variant) { return variant == 1 ? 2 : variant == 2 ? 8 : RES_ROWS; }
+
+Op residual_op(const ResidualArgs &a, int nplanes, int rows)
 {
-    return Op::make(OP_RESIDUAL, 0, a, (a.numr - 2 + 255) / 256, (a.numz - 2 + RES_ROWS - 1) / RES_ROWS, nplanes);
+    const int v = rows == 2 ? 1 : rows == 8 ? 2 : 0, r = residual_rows(v);
+    return Op::make(OP_RESIDUAL, v, a, (a.numr - 2 + 255) / 256, (a.numz - 2 + r - 1) / r, nplanes);
 }
 
 // Cold first residual at the interior points next to a face: t < nri row K
@@ -386,17 +392,34 @@ void launch_fillf(float *p, size_t n, float v, hipStream_t s)
     {                                                                                             \
         BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
     }
-BH_KERNEL_PAIR(residual, ResidualArgs, 256)
 BH_KERNEL_PAIR(residual_face, ResidualArgs, 256)
 BH_KERNEL_PAIR(add_update, UpdateArgs, 256)
 BH_KERNEL_PAIR(tridiag_shift, TridiagArgs, 256)
 #undef BH_KERNEL_PAIR
 
+// the residual, one kernel pair per rows-per-tile variant
+template <int ROWS>
+__global__ __launch_bounds__(256) void residual_kernel(ResidualArgs a, int nx, int ny, int total)
+{
+    BH_SINGLE_LOOP(a, nx, ny, total, (residual_tile_r<ROWS>(a, bx, by, bz)));
+}
+template <int ROWS>
+__global__ __launch_bounds__(256) void residual_grouped(const ResidualArgs *__restrict__ args, GroupSched gs)
+{
+    BH_GROUP_LOOP(ResidualArgs, args, gs, (residual_tile_r<ROWS>(ga, bx, by, bz)));
+}
+
 namespace {
-bool mixed_ref(int kind, const void **single, const void **grouped)
+bool mixed_ref(int kind, int variant, const void **single, const void **grouped)
 {
     switch (kind) {
-    case OP_RESIDUAL: *single = (const void *)&residual_kernel; *grouped = (const void *)&residual_grouped; return true;
+    case OP_RESIDUAL:
+        switch (variant) {
+        case 1: *single = (const void *)&residual_kernel<2>; *grouped = (const void *)&residual_grouped<2>; break;
+        case 2: *single = (const void *)&residual_kernel<8>; *grouped = (const void *)&residual_grouped<8>; break;
+        default: *single = (const void *)&residual_kernel<4>; *grouped = (const void *)&residual_grouped<4>; break;
+        }
+        return true;
     case OP_RESIDUAL_FACE:
         *single = (const void *)&residual_face_kernel;
         *grouped = (const void *)&residual_face_grouped;
@@ -415,7 +438,7 @@ bool mixed_ref(int kind, const void **single, const void **grouped)
 bool mixed_launch(const Op &op, hipStream_t s)
 {
     const void *k1, *k2;
-    if (!mixed_ref(op.kind, &k1, &k2)) return false;
+    if (!mixed_ref(op.kind, op.variant, &k1, &k2)) return false;
     int nx = op.nx, ny = op.ny, total = (int)op.tiles();
     void *argv[] = {(void *)op.args, (void *)&nx, (void *)&ny, (void *)&total};
     const int nb = persistent_blocks(total, kernel_occupancy(k1, 256, 0));
@@ -423,10 +446,11 @@ bool mixed_launch(const Op &op, hipStream_t s)
     return true;
 }
 
-bool mixed_launch_grouped(int kind, int, const void *d_args, const GroupSched &gs, int nblocks, hipStream_t s)
+bool mixed_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
+                          hipStream_t s)
 {
     const void *k1, *k2;
-    if (!mixed_ref(kind, &k1, &k2)) return false;
+    if (!mixed_ref(kind, variant, &k1, &k2)) return false;
     GroupSched g = gs;
     const void *a = d_args;
     void *argv[] = {(void *)&a, (void *)&g};
@@ -434,10 +458,10 @@ bool mixed_launch_grouped(int kind, int, const void *d_args, const GroupSched &g
     return true;
 }
 
-bool mixed_occupancy(int kind, int, int *blocks)
+bool mixed_occupancy(int kind, int variant, int *blocks)
 {
     const void *k1, *k2;
-    if (!mixed_ref(kind, &k1, &k2)) return false;
+    if (!mixed_ref(kind, variant, &k1, &k2)) return false;
     *blocks = kernel_occupancy(k2, 256, 0);
     return true;
 }

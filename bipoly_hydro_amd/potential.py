@@ -134,7 +134,7 @@ def load_library(path: str = LIB_PATH):
     L.bh_batch_device_bytes.restype = ctypes.c_size_t
     L.bh_batch_get_stats.argtypes = [vp, ctypes.POINTER(BatchStats)]
     L.bh_batch_set_timing.argtypes = [vp, ctypes.c_int]
-    L.bh_batch_tuning.argtypes = [vp, ip, ip, ctypes.c_int]
+    L.bh_batch_tuning.argtypes = [vp, ctypes.c_int, ip, ctypes.c_int, dp]
     L.bh_batch_plan.argtypes = [vp, ctypes.c_int, pp]
     L.bh_server_create.argtypes = [ctypes.POINTER(ServerConfig), pp]
     L.bh_server_submit.argtypes = [vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)]
@@ -306,15 +306,22 @@ class PotentialBatch:
         _check(load_library().bh_batch_get_stats(self._h, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in BatchStats._fields_}
 
-    def tuning(self):
+    TUNE_AXES = ("tile", "nl", "rows", "grid", "order")
+
+    def tuning(self) -> dict:
+        """Per tuning axis: the choice at each step of the solve (-1: not
+        applicable) and the ms the axis's choices saved over its default."""
         L = load_library()
-        m = L.bh_batch_tuning(self._h, None, None, 0)
-        if m < 0:
-            _check(m)
-        tile = (ctypes.c_int * max(m, 1))()
-        nl = (ctypes.c_int * max(m, 1))()
-        L.bh_batch_tuning(self._h, tile, nl, m)
-        return list(tile)[:m], list(nl)[:m]
+        out = {}
+        for ax, name in enumerate(self.TUNE_AXES):
+            m = L.bh_batch_tuning(self._h, ax, None, 0, None)
+            if m < 0:
+                _check(m)
+            ch = (ctypes.c_int * max(m, 1))()
+            gain = ctypes.c_double(0.0)
+            L.bh_batch_tuning(self._h, ax, ch, m, ctypes.byref(gain))
+            out[name] = {"choice": list(ch)[:m], "gain_ms": gain.value}
+        return out
 
 
 class PotentialServer:

@@ -127,9 +127,10 @@ const char *bh_last_error(void);
 typedef struct bh_batch bh_batch;
 
 #define BH_BATCH_GRAPH 1    /* capture the launch sequence as a hipGraph and replay it */
-#define BH_BATCH_AUTOTUNE 2 /* time the tile and pencil shape candidates over the first
-                               5 solves (one candidate per solve), then keep the fastest
-                               per step */
+#define BH_BATCH_AUTOTUNE 2 /* search the launch shapes over the first solves: one tuning
+                               axis after another (below), each candidate of the axis
+                               timed in one solve at every step it applies to, the
+                               fastest kept per step; no choice changes any result */
 
 /* Create a batch of ngrids grids (descs[i] as for bh_plan_create).
  * hbm_budget = 0 means no limit; otherwise creation fails with BH_E_NOMEM
@@ -153,15 +154,27 @@ typedef struct bh_batch_stats {
     int ngrids;
     int launches;    /* grouped launches per solve */
     int graph;       /* 1 when solves replay a captured hipGraph */
-    int tuning_left; /* autotuning solves still to run */
+    int tuning_left; /* autotuning solves still to run (-1 before the first solve) */
     double last_ms;  /* device time of the last timed solve (ms) */
 } bh_batch_stats;
 /* Statistics; waits for the last timed solve. */
 int bh_batch_get_stats(bh_batch *batch, bh_batch_stats *stats);
 int bh_batch_set_timing(bh_batch *batch, int on);
-/* Tile shape and sweep lines per workgroup chosen for each step of the
- * solve (-1: default); writes up to n entries, returns the step count. */
-int bh_batch_tuning(const bh_batch *batch, int *tile, int *nl, int n);
+/* Autotuning axes: the FP64 and separate-kernel GEMM tile shape (0..4),
+ * line-sweep lines per workgroup (8, 16), residual rows per tile (2, 4, 8),
+ * persistent workgroups per CU of a grouped launch (0: one per tile) and
+ * its tile order (0: XCD-chunked, 1: linear). */
+#define BH_TUNE_TILE 0
+#define BH_TUNE_NL 1
+#define BH_TUNE_ROWS 2
+#define BH_TUNE_GRID 3
+#define BH_TUNE_ORDER 4
+/* The choice made on `axis` for each step of the solve (-1: the axis does
+ * not apply there, or was not tuned); writes up to n entries into choice,
+ * and into *gain_ms (may be NULL) the time the axis's choices saved over
+ * its first (default) candidate, summed over the steps, in the solve that
+ * timed them.  Returns the step count. */
+int bh_batch_tuning(const bh_batch *batch, int axis, int *choice, int n, double *gain_ms);
 /* Grid i's plan (owned by the batch: do not destroy), e.g. for
  * bh_set_potential. */
 int bh_batch_plan(bh_batch *batch, int i, bh_plan **plan);

@@ -73,15 +73,20 @@ def test_batch_autotune_keeps_results_and_reports_choices():
     with P.PotentialBatch(GRIDS, precision="mixed") as eager:
         ref = eager.solve(rhos, 0)
     with P.PotentialBatch(GRIDS, precision="mixed", autotune=True, graph=True) as b:
-        for _ in range(6):  # five tuning solves, then the tuned sequence
+        b.solve(rhos, 0)
+        left = b.stats()["tuning_left"]
+        assert left > 0
+        for _ in range(left + 1):  # the remaining tuning solves, then the tuned sequence
             got = b.solve(rhos, 0)
             for i in range(len(GRIDS)):
                 assert np.array_equal(got[i], ref[i])
         st = b.stats()
         assert st["tuning_left"] == 0 and st["graph"] == 1
-        tile, nl = b.tuning()
-    chosen = [t for t in tile if t >= 0]
-    assert chosen and all(0 <= t <= 4 for t in chosen)
+        tune = b.tuning()
+    # the mixed solve's residuals and every launch's grid and order were searched
+    for ax, allowed in (("rows", {2, 4, 8}), ("grid", {0, 1, 2, 4}), ("order", {0, 1})):
+        chosen = [c for c in tune[ax]["choice"] if c >= 0]
+        assert chosen and set(chosen) <= allowed, (ax, tune[ax])
 
 
 def test_batch_over_budget_is_refused():
