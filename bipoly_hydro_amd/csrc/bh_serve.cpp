@@ -226,12 +226,14 @@ struct Req {
     double t_submit = 0, t_admit = 0, t_done = 0;
     std::vector<double> lat_ms;
     int status = BH_OK;
+    std::vector<double> start;  // the potential to resume from (copied at submission), or empty
 };
 
 struct Slot {
     Shell *sh = nullptr;
     Req *req = nullptr;
     int enq = 0, done = 0;  // solves enqueued / completed
+    bool resume = false;    // the first solve is warm (from the request's start potential)
     double t_ready = 0;     // completion time of the last completed solve
     std::shared_ptr<const DevMem> prof;
     const double *prof_ptr = nullptr;
@@ -442,9 +444,31 @@ void admit(bh_server *S)
             finish_request(S, q, rc != BH_OK ? rc : BH_E_DEVICE);
             continue;
         }
+        if (!q->start.empty()) {
+            // resume: the potential into the shell's buffer, its transform
+            // into the plan's warm-start state, ahead of the first solve on
+            // the server's stream
+            hipError_t e = hipMemcpyAsync(sh->pot, q->start.data(), q->start.size() * sizeof(double),
+                                          hipMemcpyHostToDevice, S->stream);
+            if (e == hipSuccess) {
+                bh_plan *p = sh->plan;
+                const long M = (long)p->g.numr * p->g.numz;
+                std::vector<Op> ops;
+                dft_fwd64_ops(p, sh->pot, M, (int)M, p->XJ, M, false, &ops);
+                for (const Op &op : ops) launch_op(op, S->stream);
+                e = hipGetLastError();
+            }
+            // (q->start lives until the request is collected, after the copy)
+            if (e != hipSuccess) {
+                S->pool[sh->key].push_back(sh);
+                finish_request(S, q, fail(BH_E_DEVICE, std::string("resume: ") + hipGetErrorString(e)));
+                continue;
+            }
+        }
         Slot *sl = new Slot();
         sl->sh = sh;
         sl->req = q;
+        sl->resume = !q->start.empty();
         sl->prof = prof;
         sl->prof_ptr = prof_ptr;
         q->t_admit = now_s();
@@ -469,7 +493,7 @@ int enqueue_iteration(bh_server *S, bool *any)
         const int i = sl->enq;
         bh_plan *p = sl->sh->plan;
         std::vector<Op> ops;
-        build_ops(p, sl->sh->rho, sl->sh->pot, i == 0 ? 0 : 2, &ops);
+        build_ops(p, sl->sh->rho, sl->sh->pot, (i == 0 && !sl->resume) ? 0 : 2, &ops);
         ops.insert(ops.begin(), density_op(model_args(sl->req->r.model, p->g, i, sl->sh->rho, sl->prof_ptr)));
         S->lists.push_back(std::move(ops));
         it.solves.push_back({sl, i});
@@ -638,6 +662,8 @@ extern "C" int bh_server_submit(bh_server *S, const bh_request *req, long long *
     Req *q = new Req();
     q->r = *req;
     q->r.grid = d;
+    if (req->start_pot) q->start.assign(req->start_pot, req->start_pot + (size_t)d.numr * d.numz * d.numphi);
+    q->r.start_pot = nullptr;
     fill_model_defaults(&q->r.model);
     if (q->r.model.radius[0] <= 0.0 || q->r.model.radius[1] <= 0.0) {
         delete q;

@@ -39,7 +39,9 @@ EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_batch_create", "bh_batch_destroy", "bh_batch_solve_device", "bh_batch_solve",
            "bh_batch_device_bytes", "bh_batch_get_stats", "bh_batch_set_timing", "bh_batch_tuning",
            "bh_batch_plan", "bh_server_create", "bh_server_submit", "bh_server_wait", "bh_server_drain",
-           "bh_server_get_stats", "bh_server_destroy", "bh_set_potential_noalloc")
+           "bh_server_get_stats", "bh_server_destroy", "bh_set_potential_noalloc",
+           "bh_cont_write", "bh_cont_read", "bh_cont_write_scalars", "bh_cont_read_scalars",
+           "bh_set_potential_from_cont")
 
 BH_BATCH_GRAPH = 1
 BH_BATCH_AUTOTUNE = 2
@@ -70,7 +72,7 @@ class BinaryModel(ctypes.Structure):
 
 class Request(ctypes.Structure):
     _fields_ = [("grid", GridDesc), ("nsolves", ctypes.c_int), ("model", BinaryModel),
-                ("pot_out", ctypes.c_void_p), ("solve_ms", ctypes.c_void_p)]
+                ("pot_out", ctypes.c_void_p), ("solve_ms", ctypes.c_void_p), ("start_pot", ctypes.c_void_p)]
 
 
 class RequestResult(ctypes.Structure):
@@ -89,6 +91,20 @@ class ServerStats(ctypes.Structure):
                 ("elapsed_s", ctypes.c_double), ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
                 ("mean_ms", ctypes.c_double)]
 
+
+class ContLayout(ctypes.Structure):
+    _fields_ = [("numr", ctypes.c_int), ("numz", ctypes.c_int), ("numphi", ctypes.c_int),
+                ("numr_procs", ctypes.c_int), ("numz_procs", ctypes.c_int)]
+
+
+class ContScalars(ctypes.Structure):
+    _fields_ = [("time", ctypes.c_double), ("frnum", ctypes.c_int), ("phi_com", ctypes.c_double),
+                ("R_com", ctypes.c_double), ("com", ctypes.c_double * 9), ("v_com", ctypes.c_double * 3),
+                ("a_com", ctypes.c_double * 3), ("cylin_a_com", ctypes.c_void_p), ("cylin_v_com", ctypes.c_void_p),
+                ("delt", ctypes.c_double * 3)]
+
+
+CONT_FIELDS = ("s", "t", "a", "rho", "tau", "pot", "tempa", "frac1", "frac2")
 
 _lib = None
 
@@ -117,6 +133,12 @@ def load_library(path: str = LIB_PATH):
     L.bh_solve_device.argtypes = [vp, vp, vp, ctypes.c_int, vp]
     L.bh_set_potential.argtypes = [vp, vp, ctypes.c_int]
     L.bh_set_potential_noalloc.argtypes = [vp, vp]
+    cp = ctypes.c_char_p
+    L.bh_cont_write.argtypes = [cp, ctypes.c_int, ctypes.POINTER(ContLayout), ctypes.POINTER(vp)]
+    L.bh_cont_read.argtypes = [cp, ctypes.POINTER(ContLayout), ctypes.POINTER(vp)]
+    L.bh_cont_write_scalars.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ContScalars)]
+    L.bh_cont_read_scalars.argtypes = [cp, ctypes.c_int, ctypes.POINTER(ContScalars)]
+    L.bh_set_potential_from_cont.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int]
     L.bh_plan_device_bytes.argtypes = [vp]
     L.bh_plan_device_bytes.restype = ctypes.c_size_t
     L.bh_plan_set_timing.argtypes = [vp, ctypes.c_int]
@@ -219,6 +241,11 @@ class PotentialPlan:
         if pot.shape != self.shape:
             raise ValueError(f"pot shape {pot.shape} != {self.shape}")
         _check(load_library().bh_set_potential_noalloc(self._h, ctypes.c_void_p(pot.ctypes.data)))
+
+    def set_potential_from_cont(self, prefix: str, numr_procs: int, numz_procs: int):
+        """Warm-start potential from a run's fort.12 continuation files
+        (prefix: the path up to the rank number)."""
+        _check(load_library().bh_set_potential_from_cont(self._h, prefix.encode(), numr_procs, numz_procs))
 
     def set_timing(self, on: bool = True):
         _check(load_library().bh_plan_set_timing(self._h, 1 if on else 0))
@@ -353,8 +380,11 @@ class PotentialServer:
         self.close()
 
     def submit(self, grid, nsolves: int, model: dict, *, want_potential: bool = False,
-               want_latencies: bool = True, dr: float = 1.0 / 127.0, refine_passes: int = 2) -> int:
-        """grid = (numr, numz, numphi); model keys as bh_binary_model."""
+               want_latencies: bool = True, dr: float = 1.0 / 127.0, refine_passes: int = 2,
+               start_pot=None) -> int:
+        """grid = (numr, numz, numphi); model keys as bh_binary_model;
+        start_pot: a (numphi, numz, numr) potential to resume from (e.g. from
+        cont_read), the first solve then warm."""
         numr, numz, numphi = grid
         m = BinaryModel()
         m.rho_c[:] = list(model.get("rho_c", (0.0, 0.0)))
@@ -366,8 +396,14 @@ class PotentialServer:
         m.seed = int(model.get("seed", 0))
         pot = np.empty((numphi, numz, numr)) if want_potential else None
         lat = np.zeros(nsolves) if want_latencies else None
+        sp = None
+        if start_pot is not None:
+            sp = np.ascontiguousarray(start_pot, dtype=np.float64)
+            if sp.shape != (numphi, numz, numr):
+                raise ValueError(f"start_pot shape {sp.shape} != {(numphi, numz, numr)}")
         r = Request(GridDesc(numr, numz, numphi, 1, dr, 30, 20, 5, refine_passes), nsolves, m,
-                    pot.ctypes.data if pot is not None else None, lat.ctypes.data if lat is not None else None)
+                    pot.ctypes.data if pot is not None else None, lat.ctypes.data if lat is not None else None,
+                    sp.ctypes.data if sp is not None else None)
         rid = ctypes.c_longlong()
         _check(load_library().bh_server_submit(self._h, ctypes.byref(r), ctypes.byref(rid)))
         self._keep[rid.value] = (pot, lat)
@@ -415,3 +451,47 @@ def bipolytrope_profile(n_core: float = 3.0, n_env: float = 1.5, core_frac: floa
     info = np.zeros(2)
     _check(f(n_core, n_env, core_frac, t.ctypes.data, ntab, info.ctypes.data))
     return t, float(info[0]), float(info[1])
+
+
+# ---- continuation files (fort.12.<tstep>.<rank>, fort.13.<tstep>) ---------
+
+def cont_write(directory: str, tstep: int, grid, procs, fields: dict):
+    """Write fort.12.<tstep>.<rank> files in the reference's format; fields
+    maps names of CONT_FIELDS to (numphi, numz, numr) arrays (missing: zeros)."""
+    numr, numz, numphi = grid
+    lay = ContLayout(numr, numz, numphi, procs[0], procs[1])
+    keep = [np.ascontiguousarray(fields[n], dtype=np.float64) if n in fields else None for n in CONT_FIELDS]
+    ptrs = (ctypes.c_void_p * len(CONT_FIELDS))(*[a.ctypes.data if a is not None else None for a in keep])
+    _check(load_library().bh_cont_write(directory.encode(), int(tstep), ctypes.byref(lay), ptrs))
+
+
+def cont_read(prefix: str, grid, procs, names=CONT_FIELDS) -> dict:
+    """Read the named fields from fort.12 files (prefix up to the rank)."""
+    numr, numz, numphi = grid
+    lay = ContLayout(numr, numz, numphi, procs[0], procs[1])
+    out = {n: np.zeros((numphi, numz, numr)) for n in names}
+    ptrs = (ctypes.c_void_p * len(CONT_FIELDS))(*[out[n].ctypes.data if n in out else None for n in CONT_FIELDS])
+    _check(load_library().bh_cont_read(prefix.encode(), ctypes.byref(lay), ptrs))
+    return out
+
+
+def cont_read_scalars(path: str, numphi: int) -> dict:
+    ca, cv = np.zeros((numphi, 3)), np.zeros((numphi, 3))
+    sc = ContScalars()
+    sc.cylin_a_com, sc.cylin_v_com = ca.ctypes.data, cv.ctypes.data
+    _check(load_library().bh_cont_read_scalars(path.encode(), numphi, ctypes.byref(sc)))
+    return {"time": sc.time, "frnum": sc.frnum, "phi_com": sc.phi_com, "R_com": sc.R_com,
+            "com": np.array(sc.com[:]).reshape(3, 3, order="F"), "v_com": np.array(sc.v_com[:]),
+            "a_com": np.array(sc.a_com[:]), "cylin_a_com": ca.T.copy(), "cylin_v_com": cv.T.copy(),
+            "delt": np.array(sc.delt[:])}
+
+
+def cont_write_scalars(directory: str, tstep: int, numphi: int, sc: dict):
+    ca = np.ascontiguousarray(np.asarray(sc["cylin_a_com"], dtype=np.float64).T)
+    cv = np.ascontiguousarray(np.asarray(sc["cylin_v_com"], dtype=np.float64).T)
+    c = ContScalars()
+    c.time, c.frnum, c.phi_com, c.R_com = sc["time"], int(sc["frnum"]), sc["phi_com"], sc["R_com"]
+    c.com[:] = list(np.asarray(sc["com"], dtype=np.float64).reshape(-1, order="F"))
+    c.v_com[:], c.a_com[:], c.delt[:] = list(sc["v_com"]), list(sc["a_com"]), list(sc["delt"])
+    c.cylin_a_com, c.cylin_v_com = ca.ctypes.data, cv.ctypes.data
+    _check(load_library().bh_cont_write_scalars(directory.encode(), int(tstep), numphi, ctypes.byref(c)))

@@ -89,6 +89,46 @@ int bh_solve_device(bh_plan *plan, const double *d_rho, double *d_pot, int tstep
  * is a device or a host pointer.  Synchronous. */
 int bh_set_potential(bh_plan *plan, const double *pot, int on_device);
 
+/* ---- Continuation files -------------------------------------------------
+ * The hydro code's restart files, read and written on the host in the
+ * reference's own format: fort.12.<tstep>.<rank>, one per MPI rank, each one
+ * unformatted sequential record (4-byte length markers) of that rank's
+ * (numr_dd, numz_dd, numphi) blocks, ghost layers included, of the nine
+ * fields s, t, a, rho, tau, pot, tempa, frac1, frac2; and fort.13.<tstep>,
+ * the root rank's scalars.  Written by ritecont_frac_new.F90:117-158, read
+ * by setup_frac.F90:625-653.  Fields here are global (numr, numz, numphi)
+ * arrays, Fortran order; rank = column + numr_procs * row. */
+#define BH_CONT_NFIELDS 9
+typedef struct bh_cont_layout {
+    int numr, numz, numphi;
+    int numr_procs, numz_procs;  /* the writer's process grid */
+} bh_cont_layout;
+/* Writes <dir>/fort.12.<tstep>.<rank> for every rank, the step formatted as
+ * the reference does (i5, i6 or i7: a blank-padded width below 10000);
+ * fields[i] NULL writes zeros.  Replaces ritecont_frac_new.F90:117-152. */
+int bh_cont_write(const char *dir, int tstep, const bh_cont_layout *layout, const double *const *fields);
+/* Reads <prefix><rank> for every rank (prefix as setup_frac.F90:283's
+ * conts_template, e.g. "input/conts/fort.12.12345.") into the global
+ * fields; fields[i] NULL skips that field.  Replaces setup_frac.F90:625-641. */
+int bh_cont_read(const char *prefix, const bh_cont_layout *layout, double *const *fields);
+typedef struct bh_cont_scalars {
+    double time;
+    int frnum;
+    double phi_com, R_com;
+    double com[9], v_com[3], a_com[3];
+    double *cylin_a_com, *cylin_v_com;  /* caller's arrays of 3 * numphi */
+    double delt[3];
+} bh_cont_scalars;
+/* fort.13.<tstep> in <dir> (ritecont_frac_new.F90:154-158) and its reader
+ * (setup_frac.F90:645-653). */
+int bh_cont_write_scalars(const char *dir, int tstep, int numphi, const bh_cont_scalars *scalars);
+int bh_cont_read_scalars(const char *path, int numphi, bh_cont_scalars *scalars);
+/* Reads pot from the fort.12 files of a numr_procs x numz_procs run and
+ * makes it the plan's warm-start potential (the restart path of
+ * setup_frac.F90:625-641 followed by the next potential_solver call's use
+ * of potp).  Synchronous. */
+int bh_set_potential_from_cont(bh_plan *plan, const char *prefix, int numr_procs, int numz_procs);
+
 /* As bh_set_potential from a host array, but allocating nothing: the field
  * is streamed to the device in column chunks through a fixed 32 MiB staging
  * array that the library holds on every device it runs on until the
@@ -214,6 +254,9 @@ typedef struct bh_request {
     bh_binary_model model;
     double *pot_out;        /* optional host (numr, numz, numphi) array: the last potential */
     double *solve_ms;       /* optional host array [nsolves]: per-solve latencies (ms) */
+    const double *start_pot; /* optional host (numr, numz, numphi) potential to resume from
+                                (e.g. read by bh_cont_read): the first solve is then warm
+                                from it; read during bh_server_submit */
 } bh_request;
 
 typedef struct bh_request_result {

@@ -2,12 +2,14 @@
 Fortran/MPI potential_solver, built by oracle/ref/Makefile from the sources
 under /root/reference (build container only).
 
-    python tests/golden/make_golden.py [--c2 | --c2-only]
+    python tests/golden/make_golden.py [--c2 | --c2-only] [--cont]
 
-Each case: a seeded density (bipoly_hydro_amd.density), a cold solve
-(tstep = 0) and a warm solve (tstep = 2) of a perturbed density starting
-from the cold result.  Output: <case>.npz with rho1, pot1, rho2, pot2 (small
-grids) or slices and norms (258 x 130 x 512).
+Each solver case: a seeded density (bipoly_hydro_amd.density), a cold
+solve (tstep = 0) and a warm solve (tstep = 2) of a perturbed density
+starting from the cold result.  Output: <case>.npz with rho1, pot1, rho2,
+pot2 (small grids) or slices and norms (258 x 130 x 512).  --cont: the
+continuation files the reference's writer makes of known fields
+(cont_<grid>_<ranks>/).
 """
 import os
 import subprocess
@@ -74,6 +76,42 @@ def c2_case(wd):
     print(name, log.strip().replace("\n", " | "))
 
 
+CONT_CASES = [("cont_42x26x16_2x2", 42, 26, 16, 2, 2), ("cont_34x18x8_4x2", 34, 18, 8, 4, 2)]
+CONT_NAMES = ["s", "t", "a", "rho", "tau", "pot", "tempa", "frac1", "frac2"]
+CONT_TSTEP = 12345
+
+
+def cont_fields(numr, numz, numphi):
+    """The nine fields written to the continuation-file fixtures: field f at
+    (J, K, L) (1-based) is (f+1) 1e9 + J 1e6 + K 1e3 + L, exact in FP64, so
+    every value names its own field and position."""
+    J = np.arange(1, numr + 1)[None, None, :]
+    K = np.arange(1, numz + 1)[None, :, None]
+    L = np.arange(1, numphi + 1)[:, None, None]
+    return [((f + 1) * 1e9 + J * 1e6 + K * 1e3 + L).astype(np.float64) for f in range(len(CONT_NAMES))]
+
+
+def cont_case(wd, name, numr, numz, numphi, nrp, nzp):
+    """The reference's own continuation writer (ritecont_frac_new) on a
+    process grid of nrp x nzp ranks; its files become the fixture."""
+    import shutil
+    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle/ref/Makefile"), f"NAME=h_{name}",
+                    f"NUMR={numr}", f"NUMZ={numz}", f"NUMPHI={numphi}", f"NRP={nrp}", f"NZP={nzp}", "hydro", "-j8"],
+                   check=True)
+    exe = os.path.join(ROOT, "oracle/_ref", f"h_{name}", "bh_hydro")
+    run = os.path.join(wd, name)
+    os.makedirs(os.path.join(run, "output", "conts"), exist_ok=True)
+    for n, v in zip(CONT_NAMES, cont_fields(numr, numz, numphi)):
+        v.tofile(os.path.join(run, f"in.{n}.bin"))
+    cmd = f"ulimit -s unlimited; cd {run} && /opt/conda/bin/mpiexec -n {nrp * nzp} {exe} cont in {CONT_TSTEP}"
+    subprocess.run(cmd, shell=True, check=True, capture_output=True, text=True)
+    dst = os.path.join(HERE, name)
+    os.makedirs(dst, exist_ok=True)
+    for f in sorted(os.listdir(os.path.join(run, "output", "conts"))):
+        shutil.copy(os.path.join(run, "output", "conts", f), os.path.join(dst, f))
+    print(name, sorted(os.listdir(dst)))
+
+
 def main():
     os.makedirs(os.path.join(ROOT, "oracle", "_ref"), exist_ok=True)
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as wd:
@@ -87,6 +125,9 @@ def main():
             print(name, log.strip().replace("\n", " | "))
         if "--c2" in sys.argv or "--c2-only" in sys.argv:
             c2_case(wd)
+        if "--cont" in sys.argv:
+            for c in CONT_CASES:
+                cont_case(wd, *c)
 
 
 if __name__ == "__main__":

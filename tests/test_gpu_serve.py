@@ -117,3 +117,26 @@ def test_a_request_id_is_waited_for_once():
         assert e.value.code == -5
         log = srv.iterations()
         assert log.shape[0] >= 2 and np.all(log[:, 3] >= log[:, 2]) and np.all(log[:, 2] >= log[:, 1])
+
+
+def test_served_sequence_resumes_from_continuation_files(tmp_path):
+    """A request resumed from a potential written as continuation files (the
+    reference's fort.12 format) gives the single-plan warm solves from that
+    potential."""
+    grid, n = (42, 26, 16), 3
+    model = dict(radius=(5.0, 4.0), separation=14.0, drift=0.01, noise=1e-5, seed=9)
+    prof, _, _ = P.bipolytrope_profile()
+    with P.PotentialPlan(*grid, precision="fp64") as plan:
+        start = plan.potential_solver(host_density(grid, dict(model, seed=3), 0, prof), tstep=0)
+    P.cont_write(str(tmp_path), 77, grid, (2, 2), {"pot": start})
+    back = P.cont_read(str(tmp_path / "fort.12.   77."), grid, (2, 2), names=("pot",))["pot"]
+    np.testing.assert_array_equal(back, start)
+    with P.PotentialServer(precision="fp64") as srv:
+        rid = srv.submit(grid, n, model, want_potential=True, start_pot=back)
+        res, pot, _ = srv.wait(rid)
+    assert res["status"] == 0
+    with P.PotentialPlan(*grid, precision="fp64") as plan:
+        plan.set_potential_from_cont(str(tmp_path / "fort.12.   77."), 2, 2)
+        for i in range(n):
+            ref = plan.potential_solver(host_density(grid, model, i, prof), tstep=2)
+    assert np.linalg.norm(pot - ref) <= 1e-12 * np.linalg.norm(ref)
