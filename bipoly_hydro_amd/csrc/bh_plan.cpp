@@ -75,6 +75,8 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     }
     ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
     ALLOC(p->FF, (size_t)N * nf);
+    ALLOC(p->src_grid, 4 * (size_t)(numr + 1));
+    ALLOC(p->src_com, 6 * (size_t)N);
     const bool with_d2 = is_mixed(precision) && p->ops->spec_fused && p->passes >= 2;
     if (is_mixed(precision)) {
         const size_t PI = (size_t)N * nri * nzi;
@@ -103,6 +105,15 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     HIPCHK(hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming));
     hipStream_t s;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<double> sg(4 * (size_t)(numr + 1), 0.0);
+    for (int J = 1; J <= numr; ++J) {
+        sg[J] = h.r_g[J];
+        sg[(numr + 1) + J] = h.rhf_g[J];
+        sg[2 * (numr + 1) + J] = h.rinv_g[J];
+        sg[3 * (numr + 1) + J] = h.rhfinv_g[J];
+    }
+    HIPCHK(hipMemcpyAsync(p->src_grid, sg.data(), sg.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_fill(p->src_com, 6 * (size_t)N, 0.0, s);
     launch_fill(p->XJ, M * N, 0.0, s);
     launch_fill(p->XK, M * N, 0.0, s);
     if (is_mixed(precision)) {

@@ -126,6 +126,10 @@ struct bh_plan {
     float *D1 = nullptr, *Wf = nullptr;
     float *D2 = nullptr;  // sum of the refinement corrections (mixed, fused spectral step, passes >= 2)
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
+    // the source step (bh_hydro.hip): r, rhf, rinv, rhfinv [4][numr + 1]
+    // (1-based J), and the centre-of-mass arrays cylin_a_com, cylin_v_com
+    // [2][numphi][3]
+    double *src_grid = nullptr, *src_com = nullptr;
     int passes = 2;                                // FP64 refinement passes (mixed; desc.refine_passes)
     int tile_dft = 2, tile_spec = 2, tile_dstf = 2, tile_dsti = 2;  // GEMM tile shapes, measured best
     int order_dft = 0, order_spec = 0;             // GEMM tile numbering (GemmArgs::col_fastest)

@@ -41,7 +41,7 @@ EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_batch_plan", "bh_server_create", "bh_server_submit", "bh_server_wait", "bh_server_drain",
            "bh_server_get_stats", "bh_server_destroy", "bh_set_potential_noalloc",
            "bh_cont_write", "bh_cont_read", "bh_cont_write_scalars", "bh_cont_read_scalars",
-           "bh_set_potential_from_cont")
+           "bh_set_potential_from_cont", "bh_source_device", "bh_make_source_temp_device")
 
 BH_BATCH_GRAPH = 1
 BH_BATCH_AUTOTUNE = 2
@@ -106,6 +106,16 @@ class ContScalars(ctypes.Structure):
 
 CONT_FIELDS = ("s", "t", "a", "rho", "tau", "pot", "tempa", "frac1", "frac2")
 
+
+class HydroFields(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("s", "t", "a", "temps", "tempa", "rho", "p", "pot")]
+
+
+class SourceParams(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("omega_frame", ctypes.c_double), ("tstep", ctypes.c_int),
+                ("boundary_condition", ctypes.c_int * 3), ("R_com", ctypes.c_double),
+                ("cylin_a_com", ctypes.c_void_p), ("cylin_v_com", ctypes.c_void_p)]
+
 _lib = None
 
 
@@ -139,6 +149,8 @@ def load_library(path: str = LIB_PATH):
     L.bh_cont_write_scalars.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ContScalars)]
     L.bh_cont_read_scalars.argtypes = [cp, ctypes.c_int, ctypes.POINTER(ContScalars)]
     L.bh_set_potential_from_cont.argtypes = [vp, cp, ctypes.c_int, ctypes.c_int]
+    L.bh_source_device.argtypes = [vp, ctypes.c_int, ctypes.POINTER(HydroFields), ctypes.POINTER(SourceParams), vp]
+    L.bh_make_source_temp_device.argtypes = [vp, ctypes.POINTER(HydroFields), ctypes.POINTER(SourceParams), vp]
     L.bh_plan_device_bytes.argtypes = [vp]
     L.bh_plan_device_bytes.restype = ctypes.c_size_t
     L.bh_plan_set_timing.argtypes = [vp, ctypes.c_int]
@@ -246,6 +258,37 @@ class PotentialPlan:
         """Warm-start potential from a run's fort.12 continuation files
         (prefix: the path up to the rank number)."""
         _check(load_library().bh_set_potential_from_cont(self._h, prefix.encode(), numr_procs, numz_procs))
+
+    @staticmethod
+    def _source_args(fields: dict, params: dict):
+        f = HydroFields(**{k: int(v) for k, v in fields.items()})
+        keep = []
+        prm = SourceParams()
+        prm.dt, prm.omega_frame, prm.tstep = params["dt"], params["omega"], int(params["tstep"])
+        prm.boundary_condition[:] = list(params["bc"])
+        prm.R_com = params.get("R_com", 0.0)
+        for k in ("cylin_a_com", "cylin_v_com"):
+            if params.get(k) is not None:
+                arr = np.ascontiguousarray(np.asarray(params[k], dtype=np.float64).T)  # (numphi, 3): Fortran (3, numphi)
+                keep.append(arr)
+                setattr(prm, k, arr.ctypes.data)
+        return f, prm, keep
+
+    def source_device(self, source_time: int, fields: dict, params: dict, stream: int = 0):
+        """The hydro's momentum source step on device fields (device
+        pointers: s, t, a, temps, tempa, rho, p, pot); params: dt, omega,
+        tstep, bc (3), and R_com, cylin_a_com, cylin_v_com (3, numphi) when
+        tstep > 1000."""
+        f, prm, keep = self._source_args(fields, params)
+        _check(load_library().bh_source_device(self._h, int(source_time), ctypes.byref(f), ctypes.byref(prm),
+                                               ctypes.c_void_p(stream)))
+        del keep
+
+    def make_source_temp_device(self, fields: dict, params: dict, stream: int = 0):
+        f, prm, keep = self._source_args(fields, params)
+        _check(load_library().bh_make_source_temp_device(self._h, ctypes.byref(f), ctypes.byref(prm),
+                                                         ctypes.c_void_p(stream)))
+        del keep
 
     def set_timing(self, on: bool = True):
         _check(load_library().bh_plan_set_timing(self._h, 1 if on else 0))

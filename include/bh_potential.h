@@ -89,6 +89,40 @@ int bh_solve_device(bh_plan *plan, const double *d_rho, double *d_pot, int tstep
  * is a device or a host pointer.  Synchronous. */
 int bh_set_potential(bh_plan *plan, const double *pot, int on_device);
 
+/* ---- The source step: the consumer of the potential ----------------------
+ * The hydro code's momentum source step, which reads the potential the
+ * solve left in HBM, so that it need not be downloaded each step: replaces
+ * source(source_time) (source.F90:6-382, with make_source_temp and the
+ * boundary routines source_bc.F90 and make_source_temp_bc.F90; comm is the
+ * identity on one global block) and the start-up make_source_temp
+ * (make_source_temp.F90:6-306, main.F90:315).  isym = 1.  All fields are
+ * device (numr, numz, numphi) arrays of the plan's grid, ghost layers
+ * included (the COMMONs /kinematic/, /source_temp/, /thermo/, /poisson/). */
+typedef struct bh_hydro_fields {
+    double *s, *t, *a;       /* radial, vertical, angular momentum densities: updated */
+    double *temps, *tempa;   /* written by make_source_temp (source_time 2), read */
+    const double *rho, *p;   /* density, pressure */
+    const double *pot;       /* the potential, e.g. bh_solve_device's d_pot */
+} bh_hydro_fields;
+typedef struct bh_source_params {
+    double dt;                   /* the (half) step, /timestep/ dt */
+    double omega_frame;          /* frame rotation, /rotframe/ */
+    int tstep;                   /* > 1000 adds the centre-of-mass acceleration terms */
+    int boundary_condition[3];   /* bottom, edge, top: 1 wall, 2 free, other Dirichlet */
+    double R_com;                /* /centerofmass/, read when tstep > 1000 */
+    const double *cylin_a_com;   /* host (3, numphi), read when tstep > 1000 */
+    const double *cylin_v_com;   /* host (3, numphi), read when tstep > 1000 */
+} bh_source_params;
+/* source(source_time), source_time 1 or 2 (2 first forms temps, tempa).
+ * Enqueued on stream; returns once enqueued (with tstep > 1000 it first
+ * waits for the stream's earlier work, then copies the centre-of-mass
+ * arrays). */
+int bh_source_device(bh_plan *plan, int source_time, const bh_hydro_fields *fields,
+                     const bh_source_params *params, void *stream);
+/* make_source_temp alone (t is not used). */
+int bh_make_source_temp_device(bh_plan *plan, const bh_hydro_fields *fields, const bh_source_params *params,
+                               void *stream);
+
 /* ---- Continuation files -------------------------------------------------
  * The hydro code's restart files, read and written on the host in the
  * reference's own format: fort.12.<tstep>.<rank>, one per MPI rank, each one

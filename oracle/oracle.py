@@ -55,6 +55,11 @@ def lib():
         L.bho_dft_forward.argtypes = [ctypes.c_int, _dp, _dp]
         L.bho_dft_inverse.argtypes = [ctypes.c_int, _dp, _dp]
         L.bho_dt_schedule.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.bho_source.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ctypes.c_int, ip] +
+                                 [ctypes.c_double] * 3 + [_dp] * 10)
+        L.bho_make_source_temp.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_double, ctypes.c_int, ip] +
+                                           [ctypes.c_double] * 3 + [_dp] * 9)
         L.bho_max_threads.restype = ctypes.c_int
         L.bho_set_threads.argtypes = [ctypes.c_int]
         _lib = L
@@ -175,3 +180,34 @@ def set_threads(n: int) -> None:
 
 def max_threads() -> int:
     return lib().bho_max_threads()
+
+
+def source(fields: dict, *, source_time: int, tstep: int, bc, dt: float, omega: float, R_com: float = 0.0,
+           cylin_a_com=None, cylin_v_com=None, dr: float = 1.0 / 127.0) -> dict:
+    """The momentum source step (source.F90 with make_source_temp,
+    source_bc, make_source_temp_bc) on one global block.  fields: rho, p,
+    pot, s, t, a, temps, tempa as (numphi, numz, numr) arrays; returns the
+    updated s, t, a, temps, tempa.  cylin_*_com: (3, numphi)."""
+    numphi, numz, numr = fields["rho"].shape
+    f = {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in fields.items()}
+    cac = np.zeros((numphi, 3)) if cylin_a_com is None else np.ascontiguousarray(np.asarray(cylin_a_com).T)
+    cvc = np.zeros((numphi, 3)) if cylin_v_com is None else np.ascontiguousarray(np.asarray(cylin_v_com).T)
+    bcv = (ctypes.c_int * 3)(*bc)
+    lib().bho_source(numr, numz, numphi, dr, source_time, tstep, bcv, dt, omega, R_com, _ptr(cac), _ptr(cvc),
+                     _ptr(f["s"]), _ptr(f["t"]), _ptr(f["a"]), _ptr(f["temps"]), _ptr(f["tempa"]),
+                     _ptr(f["rho"]), _ptr(f["p"]), _ptr(f["pot"]))
+    return {k: f[k] for k in ("s", "t", "a", "temps", "tempa")}
+
+
+def make_source_temp(fields: dict, *, tstep: int, bc, dt: float, omega: float, R_com: float = 0.0,
+                     cylin_a_com=None, cylin_v_com=None, dr: float = 1.0 / 127.0) -> dict:
+    """make_source_temp alone (the start-up call, main.F90:315): temps, tempa."""
+    numphi, numz, numr = fields["rho"].shape
+    f = {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in fields.items()}
+    cac = np.zeros((numphi, 3)) if cylin_a_com is None else np.ascontiguousarray(np.asarray(cylin_a_com).T)
+    cvc = np.zeros((numphi, 3)) if cylin_v_com is None else np.ascontiguousarray(np.asarray(cylin_v_com).T)
+    bcv = (ctypes.c_int * 3)(*bc)
+    lib().bho_make_source_temp(numr, numz, numphi, dr, tstep, bcv, dt, omega, R_com, _ptr(cac), _ptr(cvc),
+                               _ptr(f["s"]), _ptr(f["a"]), _ptr(f["temps"]), _ptr(f["tempa"]),
+                               _ptr(f["rho"]), _ptr(f["p"]), _ptr(f["pot"]))
+    return {k: f[k] for k in ("temps", "tempa")}

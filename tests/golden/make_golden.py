@@ -2,14 +2,15 @@
 Fortran/MPI potential_solver, built by oracle/ref/Makefile from the sources
 under /root/reference (build container only).
 
-    python tests/golden/make_golden.py [--c2 | --c2-only] [--cont]
+    python tests/golden/make_golden.py [--c2 | --c2-only] [--cont] [--source]
 
 Each solver case: a seeded density (bipoly_hydro_amd.density), a cold
 solve (tstep = 0) and a warm solve (tstep = 2) of a perturbed density
 starting from the cold result.  Output: <case>.npz with rho1, pot1, rho2,
 pot2 (small grids) or slices and norms (258 x 130 x 512).  --cont: the
 continuation files the reference's writer makes of known fields
-(cont_<grid>_<ranks>/).
+(cont_<grid>_<ranks>/).  --source: the reference's momentum source step
+on seeded fields (source_<grid>.npz).
 """
 import os
 import subprocess
@@ -23,7 +24,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 from bipoly_hydro_amd import density  # noqa: E402
-from golden_cases import c2_digests  # noqa: E402
+from golden_cases import SOURCE_CASES, SOURCE_GRID, SOURCE_PROCS, c2_digests, source_inputs  # noqa: E402
 
 # name, numr, numz, numphi, numr_procs, numz_procs, density builder
 CASES = [
@@ -112,6 +113,37 @@ def cont_case(wd, name, numr, numz, numphi, nrp, nzp):
     print(name, sorted(os.listdir(dst)))
 
 
+def source_case(wd):
+    """The reference's own momentum source step (source.F90 with
+    make_source_temp, source_bc, make_source_temp_bc and comm) on a 2 x 2
+    process grid, for each of SOURCE_CASES, on the seeded inputs of
+    golden_cases.source_inputs; source_<grid>.npz holds the outputs."""
+    numr, numz, numphi = SOURCE_GRID
+    nrp, nzp = SOURCE_PROCS
+    name = f"h_src_{numr}x{numz}x{numphi}"
+    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle/ref/Makefile"), f"NAME={name}",
+                    f"NUMR={numr}", f"NUMZ={numz}", f"NUMPHI={numphi}", f"NRP={nrp}", f"NZP={nzp}", "hydro", "-j8"],
+                   check=True)
+    exe = os.path.join(ROOT, "oracle/_ref", name, "bh_hydro")
+    f, sc = source_inputs()
+    run = os.path.join(wd, "source")
+    os.makedirs(run, exist_ok=True)
+    for k, v in f.items():
+        v.tofile(os.path.join(run, f"in.{k}.bin"))
+    com = np.concatenate([[sc["dt"], sc["omega"], sc["R_com"]], sc["cylin_a_com"].T.reshape(-1),
+                          sc["cylin_v_com"].T.reshape(-1)])
+    com.tofile(os.path.join(run, "in.com.bin"))
+    out = {}
+    for tag, st, tstep, bc in SOURCE_CASES:
+        cmd = (f"ulimit -s unlimited; cd {run} && /opt/conda/bin/mpiexec -n {nrp * nzp} {exe} source in out_{tag} "
+               f"{st} {tstep} {bc[0]} {bc[1]} {bc[2]}")
+        subprocess.run(cmd, shell=True, check=True, capture_output=True, text=True)
+        for k in ("s", "t", "a", "temps", "tempa"):
+            out[f"{tag}_{k}"] = np.fromfile(os.path.join(run, f"out_{tag}.{k}.bin")).reshape(numphi, numz, numr)
+    np.savez(os.path.join(HERE, f"source_{numr}x{numz}x{numphi}.npz"), **out)
+    print("source", sorted(out))
+
+
 def main():
     os.makedirs(os.path.join(ROOT, "oracle", "_ref"), exist_ok=True)
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as wd:
@@ -125,6 +157,8 @@ def main():
             print(name, log.strip().replace("\n", " | "))
         if "--c2" in sys.argv or "--c2-only" in sys.argv:
             c2_case(wd)
+        if "--source" in sys.argv:
+            source_case(wd)
         if "--cont" in sys.argv:
             for c in CONT_CASES:
                 cont_case(wd, *c)

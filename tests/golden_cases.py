@@ -47,3 +47,44 @@ def c2_digests(pot, weights):
             "k_norm": np.sqrt(np.einsum("lkj,lkj->k", pot, pot)),
             "j_norm": np.sqrt(np.einsum("lkj,lkj->j", pot, pot)),
             "row_dot": pot @ weights}
+
+
+SOURCE_GRID, SOURCE_PROCS, SOURCE_SEED = (34, 18, 16), (2, 2), 2718
+# (source_time, tstep, boundary_condition): the main branch with wall
+# boundaries, the centre-of-mass branch (tstep > 1000) with free boundaries
+# and with Dirichlet ones, and the start-up make_source_temp alone (tag "mst")
+SOURCE_CASES = [("st2_t500_wall", 2, 500, (1, 1, 1)), ("st1_t2000_free", 1, 2000, (2, 2, 2)),
+                ("st2_t2000_dirichlet", 2, 2000, (0, 0, 0))]
+
+
+def source_inputs(grid=SOURCE_GRID, seed=SOURCE_SEED):
+    """Seeded inputs of the source-step fixtures: fields rho, p, pot, s, t, a,
+    temps, tempa as (numphi, numz, numr) arrays (rho zero over part of the
+    grid, so both branches of the radial update run), and the scalars dt,
+    omega, R_com and the centre-of-mass arrays cylin_a_com, cylin_v_com
+    (3, numphi)."""
+    numr, numz, numphi = grid
+    n = numr * numz * numphi
+    u = density.splitmix64_uniform(seed, 8 * n + 6 * numphi).reshape(-1)
+    shape = (numphi, numz, numr)
+    dr = 1.0 / 127.0
+    J = np.arange(1, numr + 1)[None, None, :]
+    K = np.arange(1, numz + 1)[None, :, None]
+    L = np.arange(1, numphi + 1)[:, None, None]
+    R = (J - 1.5) * dr
+    Z = (K - 0.5 - numz / 2) * dr
+    ph = (L - 1) * 2 * np.pi / numphi
+    x, y = R * np.cos(ph), R * np.sin(ph)
+    w = 0.55 * (numr - 3) * dr
+    blob = np.maximum(0.0, 1.0 - ((x - 0.3 * w) ** 2 + y ** 2 + Z ** 2) / w ** 2)
+    f = {}
+    f["rho"] = blob * (1.0 + 0.1 * u[0:n].reshape(shape))
+    f["p"] = 0.3 * f["rho"] ** (5.0 / 3.0) + 1e-4 * u[n:2 * n].reshape(shape)
+    f["pot"] = -1.0 / np.sqrt(1.0 + (x / w) ** 2 + (y / w) ** 2 + (Z / w) ** 2) + 1e-3 * u[2 * n:3 * n].reshape(shape)
+    for i, k in enumerate(("s", "t", "a", "temps", "tempa")):
+        f[k] = 2.0 * u[(3 + i) * n:(4 + i) * n].reshape(shape) - 1.0
+    c = u[8 * n:]
+    sc = dict(dt=1.0e-3, omega=0.7, R_com=0.05,
+              cylin_a_com=(1e-2 * (2.0 * c[:3 * numphi] - 1.0)).reshape(numphi, 3).T.copy(),
+              cylin_v_com=(1e-2 * (2.0 * c[3 * numphi:] - 1.0)).reshape(numphi, 3).T.copy())
+    return f, sc
