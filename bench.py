@@ -322,6 +322,8 @@ def main():
         "mixed": mixed,
         "fp64": fp64,
     }
+    # the hydro's source step on the device, fed by the potential in HBM
+    out["source_step"] = run_source_step(numr, numz, numphi, args)
     if not args.no_workloads:
         # the batched (64 uneven grids) and served (Poisson SCF stream)
         # workloads, short runs
@@ -340,6 +342,48 @@ def main():
                                             "peak_active", "tail")}
         out["served"]["workload"] = sv["config"]["workload"]
     print(json.dumps(out), flush=True)
+
+
+def run_source_step(numr, numz, numphi, args):
+    """bh_source_device(source_time = 2) on seeded fields of the grid: its
+    time per call (HIP events on the stream), and the HBM bytes it moves
+    (make_source_temp: s, a, p, rho, pot in, temps, tempa out; source: t, s,
+    a, p, rho, pot, temps, tempa in, t, s, a out: 18 doubles per point)
+    against the 8 TB/s peak; beside it the PCIe download of the potential
+    that keeping the step on the device avoids."""
+    import torch
+    from bipoly_hydro_amd import potential as P
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from golden_cases import source_inputs
+    f, sc = source_inputs((numr, numz, numphi), seed=5)
+    d = {k: torch.from_numpy(v).cuda() for k, v in f.items()}
+    ptrs = {k: v.data_ptr() for k, v in d.items()}
+    prm = dict(dt=sc["dt"], omega=sc["omega"], tstep=20, bc=(1, 1, 1))
+    stream = torch.cuda.current_stream()
+    plan = P.PotentialPlan(numr, numz, numphi, precision="mixed")
+    for _ in range(3):
+        plan.source_device(2, ptrs, prm, stream.cuda_stream)
+    steps = max(5, args.steps // 5)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        plan.source_device(2, ptrs, prm, stream.cuda_stream)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    host = torch.empty(numphi, numz, numr, dtype=torch.float64, pin_memory=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    host.copy_(d["pot"])
+    torch.cuda.synchronize()
+    d2h_ms = (time.perf_counter() - t0) * 1e3
+    plan.close()
+    nbytes = 18 * 8 * numr * numz * numphi
+    return {"ms_per_call": ms, "bytes_per_call": nbytes, "achieved_GBs": nbytes / (ms * 1e-3) / 1e9,
+            "frac_of_hbm_peak": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "potential_download_ms_pinned": d2h_ms,
+            "workload": f"source(2) with make_source_temp and boundary conditions, {numr}x{numz}x{numphi}, "
+                        "wall boundaries, device fields"}
 
 
 def run_batched(args, emit=True):
