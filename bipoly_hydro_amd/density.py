@@ -1,8 +1,9 @@
 """Seeded synthetic densities on the solver's cylindrical grid.
 
-The grid is the reference's (setup_frac.F90:428-463, isym = 1):
+The grid is the reference's (setup_frac.F90:428-463):
     r_J   = (J - 1.5) dr          J = 1..numr   (J = 1 is the axis ghost)
-    z_K   = (K - 0.5 - numz/2) dz K = 1..numz
+    z_K   = (K - 0.5 - numz/2) dz K = 1..numz   (isym = 1)
+    z_K   = (K - 1.5) dz          K = 1..numz   (isym = 2, 3: K = 1 mirrors K = 2)
     phi_L = (L - 1) dphi          L = 1..numphi, dphi = 2 pi / numphi
 with dr = dz (default 1/127, setup_frac.F90:374-376).  Arrays are returned
 in the reference's Fortran order (numr, numz, numphi): a numpy array of
@@ -33,16 +34,19 @@ def splitmix64_uniform(seed: int, n: int) -> np.ndarray:
     return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
 
 
-def grid_coords(numr: int, numz: int, numphi: int, dr: float = 1.0 / 127.0):
+def grid_coords(numr: int, numz: int, numphi: int, dr: float = 1.0 / 127.0, isym: int = 1):
     """Cell-centre coordinates (r, z, phi) as 1-D arrays."""
     r = (np.arange(1, numr + 1, dtype=np.float64) - 1.5) * dr
-    z = (np.arange(1, numz + 1, dtype=np.float64) - 0.5 - (numz // 2)) * dr
+    if isym == 1:
+        z = (np.arange(1, numz + 1, dtype=np.float64) - 0.5 - (numz // 2)) * dr
+    else:
+        z = (np.arange(1, numz + 1, dtype=np.float64) - 1.5) * dr
     phi = np.arange(numphi, dtype=np.float64) * (2.0 * math.pi / numphi)
     return r, z, phi
 
 
-def _cartesian(numr, numz, numphi, dr):
-    r, z, phi = grid_coords(numr, numz, numphi, dr)
+def _cartesian(numr, numz, numphi, dr, isym=1):
+    r, z, phi = grid_coords(numr, numz, numphi, dr, isym)
     x = r[None, None, :] * np.cos(phi)[:, None, None]
     y = r[None, None, :] * np.sin(phi)[:, None, None]
     zz = np.broadcast_to(z[None, :, None], (numphi, numz, numr))
@@ -51,11 +55,12 @@ def _cartesian(numr, numz, numphi, dr):
 
 def lane_emden_n1_binary(numr: int, numz: int, numphi: int, *, rho_c=(1.0, 0.7),
                          radius_cells=(10.0, 8.0), centre_cells=(15.0, -15.0),
-                         noise=1e-6, seed: int = 1234, dr: float = 1.0 / 127.0) -> np.ndarray:
+                         noise=1e-6, seed: int = 1234, dr: float = 1.0 / 127.0, isym: int = 1) -> np.ndarray:
     """Two n = 1 polytropes, rho = rho_c sin(pi d/R)/(pi d/R) for d < R, centred
     on the x axis in the equatorial plane, plus seeded noise U(-1,1)*noise*rho_c
-    inside each star.  Returns float64 of shape (numphi, numz, numr)."""
-    x, y, z = _cartesian(numr, numz, numphi, dr)
+    inside each star.  Returns float64 of shape (numphi, numz, numr).  With
+    isym = 2, 3 the grid holds the upper half (z >= 0 and the mirror cell)."""
+    x, y, z = _cartesian(numr, numz, numphi, dr, isym)
     rho = np.zeros((numphi, numz, numr), dtype=np.float64)
     for s, (rc, rad, cx) in enumerate(zip(rho_c, radius_cells, centre_cells)):
         R = rad * dr

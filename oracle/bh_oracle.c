@@ -19,16 +19,18 @@
  *   tridagr.F90:41-69             Thomas along r
  *   tridagz.F90:115-141           Thomas along z
  *   potsetup.F90:113-252          trig tables, ADI coefficients
- *   tm.F90:101-200, bm.F90:101-172, sm.F90:115-215
- *                                 Green's-function tables (isym = 1)
+ *   tm.F90:101-366, bm.F90:101-172, sm.F90:115-385
+ *                                 Green's-function tables (isym = 1, 2, 3)
  *   gammln.F90:14-26              ln Gamma, 6-term Lanczos form
  *   ellf.F90 / elle.F90           complete elliptic K, E via Carlson's RF,
  *                                 RD (rf.F90, rd.F90: semantics only; the
  *                                 series and stopping rule of Carlson 1995)
  *   setup_frac.F90:374-463        grid coordinates
  *
- * Only isym = 1 (no symmetry, full z range) is restated; it is the only
- * boundary condition the solver is configured with here.
+ * All three symmetry settings of the run-time flag isym (setup_frac.F90:313)
+ * are restated: 1 = no symmetry (full z range), 2 = equatorial symmetry (the
+ * grid starts at the equator, K = 1 is the mirror of K = 2), 3 = equatorial
+ * plus pi symmetry (even azimuthal modes only).
  *
  * Array layout follows the reference: Fortran column-major
  * (numr, numz, numphi), r fastest, ghost layers included.  In C:
@@ -173,9 +175,11 @@ double bho_elle(double k)
 }
 
 /* ------------------------------------------------------------------ */
-/* Q_{m-1/2}(x), m = 0..mmax-1, exactly as tm.F90:168-194 evaluates it */
-/* (isym /= 3): upward recurrence from K, E below the cutoff, truncated */
-/* hypergeometric series with exp/log terms at and above it.            */
+/* Q_{mu-1/2}(x) for the table's mode slots m = 0..mmax-1, exactly as   */
+/* tm.F90:168-194 (isym 1, 2: mu = m) and tm.F90:292-352 (isym 3: even  */
+/* mu = 2m) evaluate it: upward recurrence from K, E below the cutoff   */
+/* (isym 3: the two-step recurrence from Q_{-1/2} and Q_{3/2}),         */
+/* truncated hypergeometric series with exp/log terms at and above it.  */
 /* ------------------------------------------------------------------ */
 static void bho_qhalf(const bho_plan *p, double xm, double *q)
 {
@@ -185,6 +189,17 @@ static void bho_qhalf(const bho_plan *p, double xm, double *q)
         double lam = sqrt(2.0 * (1.0 + xm));
         double kk = bho_ellk(mum), ee = bho_elle(mum);
         q[0] = kk * mum;
+        if (p->isym == 3) {
+            const double four_thirds = 4.0 / 3.0, one_third = 1.0 / 3.0;
+            if (mmax > 1) q[1] = (four_thirds * xm * xm - one_third) * mum * kk - four_thirds * xm * lam * ee;
+            for (int m = 3; m <= mmax; ++m) {
+                const double nu = p->nu[m], den = (2.0 * nu - 1.0) * (nu + 2.0) * (nu + 1.0);
+                q[m - 1] = q[m - 2] * ((2.0 * nu + 3.0) * (2.0 * nu + 1.0) * xm * xm / ((nu + 2.0) * (nu + 1.0)) -
+                                       (2.0 * nu + 3.0) * nu * nu / den - (nu + 1.0) / (nu + 2.0)) -
+                           q[m - 3] * (2.0 * nu + 3.0) * (nu - 1.0) * nu / den;
+            }
+            return;
+        }
         if (mmax > 1) q[1] = xm * mum * kk - lam * ee;
         for (int m = 3; m <= mmax; ++m) {
             double nu = p->nu[m];
@@ -193,7 +208,7 @@ static void bho_qhalf(const bho_plan *p, double xm, double *q)
     } else {
         double l2x = log(2.0 * xm), lx = log(xm);
         for (int m = 1; m <= mmax; ++m) {
-            double index = (double)(m - 1);
+            double index = p->isym == 3 ? 2.0 * (double)(m - 1) : (double)(m - 1);
             double coefh_m = exp(p->coefh[m - 1] - (index + 0.5) * l2x);
             double sum = 0.0;
             const double *dc = p->dcoefh + (size_t)(m - 1) * p->nser;
@@ -221,11 +236,14 @@ void bho_plan_destroy(bho_plan *p)
     free(p);
 }
 
-/* mode number of global Fourier index L (potsetup.F90:155-173, isym /= 3) */
-static int bho_mode_of(int L, int numphi)
+/* mode number of global Fourier index L (potsetup.F90:155-173).  For
+   isym = 3 the reference squares the index (potsetup.F90:159-163); that is
+   restated as written. */
+static int bho_mode_of(int L, int numphi, int isym)
 {
     int lstop = numphi / 2 + 1;
-    return (L <= lstop) ? L - 1 : L - lstop;
+    int k = (L <= lstop) ? L - 1 : L - lstop;
+    return isym == 3 ? k * k : k;
 }
 
 static void bho_build_tables(bho_plan *p);
@@ -240,7 +258,7 @@ static void bho_build_tables(bho_plan *p);
 bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, int mmax,
                           int with_tables)
 {
-    if (isym != 1) return NULL;
+    if (isym < 1 || isym > 3) return NULL;
     if (numr < 4 || numz < 4 || numphi < 4) return NULL;
     if (numphi & (numphi - 1)) return NULL;
     if (mmax < 1 || mmax > 250) return NULL;
@@ -271,7 +289,9 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
         p->rinv_g[J] = (p->r_g[J] != 0.0) ? 1.0 / p->r_g[J] : 0.0;
         p->rhfinv_g[J] = 1.0 / p->rhf_g[J];
     }
-    {
+    if (isym != 1) { /* setup_frac.F90:452-456: K = 1 is the mirror cell below the equator */
+        for (int K = 1; K <= numz; ++K) p->zhf_g[K] = ((double)K - 1.5) * p->dz;
+    } else {
         double offset = -((double)(numz / 2) * p->dz);
         for (int K = 1; K <= numz; ++K) p->zhf_g[K] = offset + ((double)K - 0.5) * p->dz;
     }
@@ -293,39 +313,50 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
     p->brb = (double *)xcalloc((size_t)numphi * (numr + 1), sizeof(double));
     p->elambdazb = (double *)xcalloc((size_t)numphi * (numr + 1), sizeof(double));
     for (int L = 1; L <= numphi; ++L) {
-        int mode = bho_mode_of(L, numphi);
+        int mode = bho_mode_of(L, numphi, isym);
         double m1mode = (mode % 2 == 0) ? 1.0 : -1.0;
         double elm = cos((double)mode * p->dphi);
         double *brb = p->brb + (size_t)(L - 1) * (numr + 1);
         double *elz = p->elambdazb + (size_t)(L - 1) * (numr + 1);
         for (int J = 3; J <= numr - 1; ++J)
             brb[J] = 2.0 * drinv2 - 2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[J] * p->rhfinv_g[J];
-        brb[2] = -p->alphar[2] + (m1mode - 1.0) * p->betar[2] -
-                 2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[2] * p->rhfinv_g[2];
+        /* the axis row: potsetup.F90:211-223 and 239-251 (isym = 3 mirrors
+           the axis cell instead of rotating it by pi) */
+        if (isym == 3)
+            brb[2] = -p->alphar[2] - 2.0 * p->betar[2] -
+                     2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[2] * p->rhfinv_g[2];
+        else
+            brb[2] = -p->alphar[2] + (m1mode - 1.0) * p->betar[2] -
+                     2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[2] * p->rhfinv_g[2];
         for (int J = 2; J <= numr - 1; ++J)
             elz[J] = -2.0 * drinv2 + 2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[J] * p->rhfinv_g[J];
-        elz[2] = p->alphar[2] - (m1mode - 1.0) * p->betar[2] +
-                 2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[2] * p->rhfinv_g[2];
+        if (isym == 3)
+            elz[2] = p->alphar[2] + 2.0 * p->betar[2] +
+                     2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[2] * p->rhfinv_g[2];
+        else
+            elz[2] = p->alphar[2] - (m1mode - 1.0) * p->betar[2] +
+                     2.0 * (elm - 1.0) * dphiinv2 * p->rhfinv_g[2] * p->rhfinv_g[2];
     }
     p->bzb = (double *)xcalloc(numz + 1, sizeof(double));
     for (int K = 2; K <= numz - 1; ++K) p->bzb[K] = 2.0 * gamma;
-    /* bessel trig tables: potsetup.F90:131-139 */
+    if (isym != 1) p->bzb[2] = gamma; /* potsetup.F90:228 */
+    /* bessel trig tables: potsetup.F90:120-139 (isym = 3: even modes) */
     p->bes_cos = (double *)xcalloc((size_t)mmax * numphi, sizeof(double));
     p->bes_sin = (double *)xcalloc((size_t)mmax * numphi, sizeof(double));
     for (int M = 1; M <= mmax; ++M)
         for (int L = 1; L <= numphi; ++L) {
-            double a = 0.5 * p->dphi * ((double)M - 1.0) * (2.0 * (double)L - 1.0);
+            double a = (isym == 3 ? p->dphi : 0.5 * p->dphi) * ((double)M - 1.0) * (2.0 * (double)L - 1.0);
             p->bes_cos[(size_t)(M - 1) * numphi + (L - 1)] = cos(a);
             p->bes_sin[(size_t)(M - 1) * numphi + (L - 1)] = sin(a);
         }
-    /* series coefficients: tm.F90:103-155 (isym /= 3) */
+    /* series coefficients: tm.F90:103-155 */
     p->nu = (double *)xcalloc(mmax + 1, sizeof(double));
-    for (int m = 3; m <= mmax; ++m) p->nu[m] = (double)m - 2.5;
+    for (int m = 3; m <= mmax; ++m) p->nu[m] = isym == 3 ? 2.0 * (double)m - 4.5 : (double)m - 2.5;
     p->coefh = (double *)xcalloc(mmax, sizeof(double));
     p->dcoefh = (double *)xcalloc((size_t)mmax * p->nser, sizeof(double));
     double gl_half = bho_gammln(0.5);
     for (int m = 1; m <= mmax; ++m) {
-        double mm = (double)m - 1.0;
+        double mm = isym == 3 ? 2.0 * ((double)m - 1.0) : (double)m - 1.0;
         double aa = 0.5 * (mm + 1.5), bb = 0.5 * (mm + 0.5), cc = mm + 1.0;
         p->coefh[m - 1] = gl_half + bho_gammln(mm + 0.5) - bho_gammln(aa) - bho_gammln(bb);
         for (int ir = 0; ir < p->nser; ++ir) {
@@ -339,13 +370,13 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
         size_t nri = numr - 2, nzi = numz - 2;
         const size_t nt = (size_t)mmax * numr * nzi * nri, ns = (size_t)mmax * numz * nzi * nri;
         p->tmr = (double *)malloc(nt * sizeof(double));
-        p->bmr = (double *)malloc(nt * sizeof(double));
+        p->bmr = isym == 1 ? (double *)malloc(nt * sizeof(double)) : NULL;
         p->smz = (double *)malloc(ns * sizeof(double));
-        if (p->tmr && p->bmr && p->smz) {
+        if (p->tmr && (p->bmr || isym != 1) && p->smz) {
 #pragma omp parallel for schedule(static)
             for (size_t i = 0; i < nt; ++i) {
                 p->tmr[i] = 1e-3;
-                p->bmr[i] = 1e-3;
+                if (p->bmr) p->bmr[i] = 1e-3;
             }
 #pragma omp parallel for schedule(static)
             for (size_t i = 0; i < ns; ++i) p->smz[i] = 1e-3;
@@ -354,34 +385,46 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
     return p;
 }
 
-/* Green's tables: tm.F90:157-200 (top), bm.F90:131-172 (bottom),
-   sm.F90:172-215 (side), isym = 1.  Interior source points only
-   (rlwb..rupb, zlwb..zupb of every rank = global 2..numr-1, 2..numz-1). */
+/* Green's tables: tm.F90:157-363 (top), bm.F90:131-172 (bottom, isym = 1
+   only), sm.F90:172-382 (side).  Interior source points only (rlwb..rupb,
+   zlwb..zupb of every rank = global 2..numr-1, 2..numz-1).  With isym = 2,
+   3 every entry adds the image of the source below the equator:
+   coef (Q(chi+) + Q(chi-)), chi+ formed with z_target + z_source. */
 static void bho_build_tables(bho_plan *p)
 {
     const int numr = p->numr, numz = p->numz, mmax = p->mmax;
     const int nri = numr - 2, nzi = numz - 2;
+    const int image = p->isym != 1;
     size_t ntop = (size_t)mmax * numr * nzi * nri;
     size_t nsid = (size_t)mmax * numz * nzi * nri;
     p->tmr = (double *)xcalloc(ntop, sizeof(double));
-    p->bmr = (double *)xcalloc(ntop, sizeof(double));
+    p->bmr = image ? NULL : (double *)xcalloc(ntop, sizeof(double));
     p->smz = (double *)xcalloc(nsid, sizeof(double));
     const double zT = p->zhf_g[numz], zBot = p->zhf_g[1];
     const double rB = p->rhf_g[numr], rBinv = 1.0 / rB;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int jl = 2; jl <= numr - 1; ++jl) {
-        double q[256];
+        double q[256], qp[256];
         const double rhf = p->rhf_g[jl], rhfinv = p->rhfinv_g[jl];
         for (int kl = 2; kl <= numz - 1; ++kl) {
             const double zhf = p->zhf_g[kl];
             for (int jg = 2; jg <= numr - 1; ++jg) {
                 double coef = sqrt(rhf * p->rhfinv_g[jg]) * p->piinv;
-                /* top */
-                double xm = 0.5 * rhfinv * p->rhfinv_g[jg] *
-                            ((zT - zhf) * (zT - zhf) + p->rhf_g[jg] * p->rhf_g[jg] + rhf * rhf);
+                /* top (the sum in each branch's order: tm.F90:164-167, 207-214) */
+                double xm = image ? 0.5 * rhfinv * p->rhfinv_g[jg] *
+                                        ((zT - zhf) * (zT - zhf) + rhf * rhf + p->rhf_g[jg] * p->rhf_g[jg])
+                                  : 0.5 * rhfinv * p->rhfinv_g[jg] *
+                                        ((zT - zhf) * (zT - zhf) + p->rhf_g[jg] * p->rhf_g[jg] + rhf * rhf);
                 bho_qhalf(p, xm, q);
+                if (image) {
+                    double xp = 0.5 * rhfinv * p->rhfinv_g[jg] *
+                                ((zT + zhf) * (zT + zhf) + rhf * rhf + p->rhf_g[jg] * p->rhf_g[jg]);
+                    bho_qhalf(p, xp, qp);
+                    for (int m = 0; m < mmax; ++m) q[m] = qp[m] + q[m];
+                }
                 for (int m = 0; m < mmax; ++m)
                     p->tmr[(((size_t)m * numr + (jg - 1)) * nzi + (kl - 2)) * nri + (jl - 2)] = coef * q[m];
+                if (image) continue;
                 /* bottom */
                 xm = 0.5 * rhfinv * p->rhfinv_g[jg] *
                      ((zBot - zhf) * (zBot - zhf) + p->rhf_g[jg] * p->rhf_g[jg] + rhf * rhf);
@@ -392,8 +435,15 @@ static void bho_build_tables(bho_plan *p)
             double coefs = sqrt(rhf * rBinv) * p->piinv;
             for (int kg = 2; kg <= numz - 1; ++kg) {
                 double dzz = p->zhf_g[kg] - zhf;
-                double xm = 0.5 * rhfinv * rBinv * (dzz * dzz + rB * rB + rhf * rhf);
+                double xm = image ? 0.5 * rhfinv * rBinv * (dzz * dzz + rhf * rhf + rB * rB)
+                                  : 0.5 * rhfinv * rBinv * (dzz * dzz + rB * rB + rhf * rhf);
                 bho_qhalf(p, xm, q);
+                if (image) {
+                    double szz = p->zhf_g[kg] + zhf;
+                    double xp = 0.5 * rhfinv * rBinv * (szz * szz + rhf * rhf + rB * rB);
+                    bho_qhalf(p, xp, qp);
+                    for (int m = 0; m < mmax; ++m) q[m] = qp[m] + q[m];
+                }
                 for (int m = 0; m < mmax; ++m)
                     p->smz[(((size_t)m * numz + (kg - 1)) * nzi + (kl - 2)) * nri + (jl - 2)] = coefs * q[m];
             }
@@ -490,7 +540,9 @@ void bho_bessel(const bho_plan *p, const double *rho, double *potp)
     const int numr = p->numr, numz = p->numz, n = p->numphi, mmax = p->mmax;
     const int nri = numr - 2, nzi = numz - 2;
     const size_t npts = (size_t)nri * nzi;
-    double factor = -p->dr * p->dz * p->dphi;
+    const int bottom = p->isym == 1;
+    /* bessel.F90:170-174 */
+    double factor = p->isym == 3 ? -2.0 * p->dr * p->dz * p->dphi : -p->dr * p->dz * p->dphi;
     double *tmpc = (double *)xcalloc((size_t)mmax * npts, sizeof(double));
     double *tmps = (double *)xcalloc((size_t)mmax * npts, sizeof(double));
     /* projection onto the azimuthal modes: bessel.F90:179-185, 228-237 */
@@ -524,13 +576,16 @@ void bho_bessel(const bho_plan *p, const double *rho, double *potp)
     for (int m = 0; m < mmax; ++m)
         for (int jg = 2; jg <= numr - 1; ++jg) {
             const double *tt = p->tmr + ((size_t)m * numr + (jg - 1)) * npts;
-            const double *tb = p->bmr + ((size_t)m * numr + (jg - 1)) * npts;
+            const double *tb = bottom ? p->bmr + ((size_t)m * numr + (jg - 1)) * npts : NULL;
             const double *vc = tmpc + (size_t)m * npts, *vs = tmps + (size_t)m * npts;
             double a = 0, b = 0, c = 0, d = 0;
             for (size_t pt = 0; pt < npts; ++pt) {
                 a += tt[pt] * vc[pt]; b += tt[pt] * vs[pt];
-                c += tb[pt] * vc[pt]; d += tb[pt] * vs[pt];
             }
+            if (bottom)
+                for (size_t pt = 0; pt < npts; ++pt) {
+                    c += tb[pt] * vc[pt]; d += tb[pt] * vs[pt];
+                }
             stc[(size_t)m * numr + jg - 1] = a; sts[(size_t)m * numr + jg - 1] = b;
             sbc[(size_t)m * numr + jg - 1] = c; sbs[(size_t)m * numr + jg - 1] = d;
         }
@@ -567,18 +622,27 @@ void bho_bessel(const bho_plan *p, const double *rho, double *potp)
             pots[(size_t)(L - 1) * numz + K - 1] = factor * s;
         }
     }
-    /* axis ghost by a pi rotation (bessel.F90:418-427, 481-486) */
-    for (int L = 1; L <= n / 2; ++L) {
-        pott[(size_t)(L - 1) * numr] = pott[(size_t)(L - 1 + n / 2) * numr + 1];
-        pott[(size_t)(L - 1 + n / 2) * numr] = pott[(size_t)(L - 1) * numr + 1];
-        potb[(size_t)(L - 1) * numr] = potb[(size_t)(L - 1 + n / 2) * numr + 1];
-        potb[(size_t)(L - 1 + n / 2) * numr] = potb[(size_t)(L - 1) * numr + 1];
+    /* axis ghost by a pi rotation (bessel.F90:418-427, 481-486), a mirror
+       for isym = 3; the side face's K = 1 mirrors K = 2 for isym = 2, 3
+       (bessel.F90:453-455) */
+    if (p->isym == 3) {
+        for (int L = 1; L <= n; ++L) pott[(size_t)(L - 1) * numr] = pott[(size_t)(L - 1) * numr + 1];
+    } else {
+        for (int L = 1; L <= n / 2; ++L) {
+            pott[(size_t)(L - 1) * numr] = pott[(size_t)(L - 1 + n / 2) * numr + 1];
+            pott[(size_t)(L - 1 + n / 2) * numr] = pott[(size_t)(L - 1) * numr + 1];
+            potb[(size_t)(L - 1) * numr] = potb[(size_t)(L - 1 + n / 2) * numr + 1];
+            potb[(size_t)(L - 1 + n / 2) * numr] = potb[(size_t)(L - 1) * numr + 1];
+        }
     }
-    /* write faces in the reference order: top, side, bottom */
+    if (!bottom)
+        for (int L = 1; L <= n; ++L) pots[(size_t)(L - 1) * numz] = pots[(size_t)(L - 1) * numz + 1];
+    /* write faces in the reference order: top, side, bottom (isym = 1) */
     for (int L = 1; L <= n; ++L) {
         for (int J = 1; J <= numr; ++J) potp[IDX(p, J, numz, L)] = pott[(size_t)(L - 1) * numr + J - 1];
         for (int K = 1; K <= numz; ++K) potp[IDX(p, numr, K, L)] = pots[(size_t)(L - 1) * numz + K - 1];
-        for (int J = 1; J <= numr; ++J) potp[IDX(p, J, 1, L)] = potb[(size_t)(L - 1) * numr + J - 1];
+        if (bottom)
+            for (int J = 1; J <= numr; ++J) potp[IDX(p, J, 1, L)] = potb[(size_t)(L - 1) * numr + J - 1];
     }
     free(tmpc); free(tmps); free(stc); free(sts); free(sbc); free(sbs); free(ssc); free(sss);
     free(pott); free(potb); free(pots);
@@ -617,6 +681,10 @@ static void bho_adi_step(const bho_plan *p, double dtt, const double *ffrho, dou
                     known[IDX(p, J, K, L)] = -four_pi * ffrho[IDX(p, J, K, L)] +
                                              dtt_mtwogamma * ffpot[IDX(p, J, K, L)] +
                                              gamma * (ffpot[IDX(p, J, K + 1, L)] + ffpot[IDX(p, J, K - 1, L)]);
+            if (p->isym != 1) /* K = 2 next to the equator: helmadi.F90:211-238 */
+                for (int J = 2; J <= numr - 1; ++J)
+                    known[IDX(p, J, 2, L)] = -four_pi * ffrho[IDX(p, J, 2, L)] + (dtt - gamma) * ffpot[IDX(p, J, 2, L)] +
+                                             gamma * ffpot[IDX(p, J, 3, L)];
             for (int K = 2; K <= numz - 1; ++K)
                 known[IDX(p, numr - 1, K, L)] = known[IDX(p, numr - 1, K, L)] -
                                                 p->alphar[numr - 1] * ffpot[IDX(p, numr, K, L)];
@@ -662,7 +730,8 @@ static void bho_adi_step(const bho_plan *p, double dtt, const double *ffrho, dou
         }
         for (int J = 2; J <= numr - 1; ++J) {
             known[IDX(p, J, numz - 1, L)] = known[IDX(p, J, numz - 1, L)] + gamma * ffpot[IDX(p, J, numz, L)];
-            known[IDX(p, J, 2, L)] = known[IDX(p, J, 2, L)] + gamma * ffpot[IDX(p, J, 1, L)];
+            if (p->isym == 1) /* the bottom face: helmadi.F90:334-340 */
+                known[IDX(p, J, 2, L)] = known[IDX(p, J, 2, L)] + gamma * ffpot[IDX(p, J, 1, L)];
         }
         for (int J = 2; J <= numr - 1; ++J) ffpot[IDX(p, J, 2, L)] = known[IDX(p, J, 2, L)] / betz[2];
         for (int K = 3; K <= numz - 1; ++K)
@@ -675,7 +744,7 @@ static void bho_adi_step(const bho_plan *p, double dtt, const double *ffrho, dou
     free(bz); free(betz); free(gamz);
 }
 
-/* helmadi(nsteps) on rhop / potp (helmadi.F90:121-411, isym = 1) */
+/* helmadi(nsteps) on rhop / potp (helmadi.F90:121-411) */
 void bho_helmadi(const bho_plan *p, int nsteps, double *rhop, double *potp)
 {
     const int numr = p->numr, numz = p->numz, n = p->numphi;
@@ -689,12 +758,21 @@ void bho_helmadi(const bho_plan *p, int nsteps, double *rhop, double *potp)
     dft_columns(p, potp, ffpot, 0);
     for (int i = 0; i < nsteps; ++i) bho_adi_step(p, dt[i], ffrho, ffpot, known);
     dft_columns(p, ffpot, potp, 1);
-    /* axis: helmadi.F90:396-405 */
-    for (int L = 1; L <= n / 2; ++L)
-        for (int K = 1; K <= numz; ++K) {
-            potp[IDX(p, 1, K, L)] = potp[IDX(p, 2, K, L + n / 2)];
-            potp[IDX(p, 1, K, L + n / 2)] = potp[IDX(p, 2, K, L)];
-        }
+    /* axis: helmadi.F90:396-405 (a mirror for isym = 3) */
+    if (p->isym == 3) {
+        for (int L = 1; L <= n; ++L)
+            for (int K = 1; K <= numz; ++K) potp[IDX(p, 1, K, L)] = potp[IDX(p, 2, K, L)];
+    } else {
+        for (int L = 1; L <= n / 2; ++L)
+            for (int K = 1; K <= numz; ++K) {
+                potp[IDX(p, 1, K, L)] = potp[IDX(p, 2, K, L + n / 2)];
+                potp[IDX(p, 1, K, L + n / 2)] = potp[IDX(p, 2, K, L)];
+            }
+    }
+    /* equator: helmadi.F90:407-411 */
+    if (p->isym != 1)
+        for (int L = 1; L <= n; ++L)
+            for (int J = 1; J <= numr; ++J) potp[IDX(p, J, 1, L)] = potp[IDX(p, J, 2, L)];
     free(dt); free(ffrho); free(ffpot); free(known);
 }
 
@@ -714,9 +792,10 @@ int bho_potential_solver(const bho_plan *p, const double *rho, double *potp, int
         for (int K = 1; K <= numz; ++K) {
             rhop[IDX(p, numr - 1, K, L)] = 0.0; rhop[IDX(p, numr, K, L)] = 0.0;
         }
-        for (int J = 1; J <= numr; ++J) {
-            rhop[IDX(p, J, 1, L)] = 0.0; rhop[IDX(p, J, 2, L)] = 0.0;
-        }
+        if (p->isym == 1)
+            for (int J = 1; J <= numr; ++J) {
+                rhop[IDX(p, J, 1, L)] = 0.0; rhop[IDX(p, J, 2, L)] = 0.0;
+            }
     }
     int nsteps;
     if (tstep > 1) nsteps = 5;

@@ -72,17 +72,18 @@ def _ptr(a: np.ndarray):
 
 
 class OraclePlan:
-    """Grid + coefficients (+ Green's tables) for one geometry, isym = 1."""
+    """Grid + coefficients (+ Green's tables) for one geometry and symmetry
+    flag isym (1 none, 2 equatorial, 3 equatorial + pi)."""
 
-    def __init__(self, numr, numz, numphi, dr=1.0 / 127.0, mmax=30, with_tables=True):
+    def __init__(self, numr, numz, numphi, dr=1.0 / 127.0, mmax=30, with_tables=True, isym=1):
         """with_tables: True computes the Green's tables, False skips them,
         "filled" allocates them filled with a nonzero constant, every page
         written (timing of the solve's arithmetic and table stream; the
         potential is then meaningless)."""
         self.shape = (numphi, numz, numr)
-        self.numr, self.numz, self.numphi, self.mmax = numr, numz, numphi, mmax
+        self.numr, self.numz, self.numphi, self.mmax, self.isym = numr, numz, numphi, mmax, isym
         mode = 2 if with_tables == "filled" else (1 if with_tables else 0)
-        self._h = lib().bho_plan_create(numr, numz, numphi, 1, dr, mmax, mode)
+        self._h = lib().bho_plan_create(numr, numz, numphi, int(isym), dr, mmax, mode)
         if not self._h:
             raise ValueError("oracle: invalid geometry")
 
@@ -120,11 +121,13 @@ class OraclePlan:
         return dt
 
     def tables(self):
-        """(tmr, bmr, smz) as arrays [m][target][kz][jr] over interior sources."""
+        """(tmr, bmr, smz) as arrays [m][target][kz][jr] over interior sources
+        (bmr is None for isym = 2, 3, which have no bottom face)."""
         nri, nzi = self.numr - 2, self.numz - 2
         L = lib()
         t = np.ctypeslib.as_array(L.bho_tmr(self._h), shape=(self.mmax, self.numr, nzi, nri)).copy()
-        b = np.ctypeslib.as_array(L.bho_bmr(self._h), shape=(self.mmax, self.numr, nzi, nri)).copy()
+        b = (np.ctypeslib.as_array(L.bho_bmr(self._h), shape=(self.mmax, self.numr, nzi, nri)).copy()
+             if self.isym == 1 else None)
         s = np.ctypeslib.as_array(L.bho_smz(self._h), shape=(self.mmax, self.numz, nzi, nri)).copy()
         return t, b, s
 

@@ -3,11 +3,13 @@
 !
 !  It replaces main.F90 + setup_frac.F90 for the self-gravity path: it sets
 !  up the 2-D process grid (the rules of main.F90:148-244), the coordinate
-!  COMMONs (the formulas of setup_frac.F90:374-463, isym = 1), reads a global
-!  density, calls the unmodified potsetup and potential_solver compiled from
-!  the reference sources, gathers pot to rank 0 and writes it.
+!  COMMONs (the formulas of setup_frac.F90:356-463), reads a global density,
+!  calls the unmodified potsetup and potential_solver compiled from the
+!  reference sources, gathers pot to rank 0 and writes it.
 !
-!  usage: bh_ref <rho1.bin> <pot1.bin> [<rho2.bin> <pot2.bin> [<tabprefix>]]
+!  usage: [BH_REF_ISYM=1|2|3] bh_ref <rho1.bin> <pot1.bin> [<rho2.bin> <pot2.bin> [<tabprefix>]]
+!    BH_REF_ISYM: the run-time symmetry flag isym (fort.7, setup_frac.F90:313),
+!    1 when unset.
 !    solve 1 is cold (tstep = 0), solve 2 warm (tstep = 2) from solve 1.
 !    <tabprefix>: if given, each rank writes its interior blocks of tmr, bmr,
 !    smz to <tabprefix>.<table>.<rank>.
@@ -56,7 +58,7 @@ real, allocatable :: gbuf(:,:,:), lbuf(:,:,:)
 integer :: ierror, nargs, i, j, k, jg, kg, src, pr, pz
 integer :: rank_of(0:numr_procs*numz_procs-1, 2)
 character(len=512) :: frho1, fpot1, frho2, fpot2, ftab
-character(len=16) :: rk
+character(len=16) :: rk, sym
 real :: t0, t1, x, offset
 integer :: istat(MPI_STATUS_SIZE)
 
@@ -105,9 +107,12 @@ if (.not. iam_on_bottom) down_neighbor = iam - numr_procs
 if (.not. iam_on_axis) in_neighbor = iam - 1
 if (.not. iam_on_edge) out_neighbor = iam + 1
 
-! constants and coordinates, isym = 1
+! constants and coordinates
 isym = 1
+call get_environment_variable('BH_REF_ISYM', sym)
+if (len_trim(sym) > 0) read(sym, *) isym
 boundary_condition = 0
+if (isym == 2 .or. isym == 3) boundary_condition(1) = 1
 pi = acos(-1.0)
 grav = 1.0
 dr = 1.0 / (oldnumr - deltar_parameter)
@@ -130,11 +135,19 @@ elsewhere
 endwhere
 rhfinv = 1.0/rhf
 x = 1.0
-offset = (row_num*(numz_dd-2) - numz/2)*dz
-do k = 1, numz_dd
-   zhf(k) = offset + (x - 0.5)*dz
-   x = x + 1.0
-enddo
+if (isym /= 1) then
+   offset = row_num*(numz_dd-2)*dz
+   do k = 1, numz_dd
+      zhf(k) = offset + (x - 1.5)*dz
+      x = x + 1.0
+   enddo
+else
+   offset = (row_num*(numz_dd-2) - numz/2)*dz
+   do k = 1, numz_dd
+      zhf(k) = offset + (x - 0.5)*dz
+      x = x + 1.0
+   enddo
+endif
 x = 0.0
 do i = 1, numphi
    phi(i) = x*dphi
@@ -152,12 +165,19 @@ elsewhere
    rinv_g = 0.0
 endwhere
 rhfinv_g = 1.0/rhf_g
-offset = -(numz/2)*dz
 x = 1.0
-do k = 1, numz
-   zhf_g(k) = offset + (x - 0.5)*dz
-   x = x + 1.0
-enddo
+if (isym /= 1) then
+   do k = 1, numz
+      zhf_g(k) = (x - 1.5)*dz
+      x = x + 1.0
+   enddo
+else
+   offset = -(numz/2)*dz
+   do k = 1, numz
+      zhf_g(k) = offset + (x - 0.5)*dz
+      x = x + 1.0
+   enddo
+endif
 
 pot = 0.0
 potp = 0.0

@@ -2,7 +2,7 @@
 Fortran/MPI potential_solver, built by oracle/ref/Makefile from the sources
 under /root/reference (build container only).
 
-    python tests/golden/make_golden.py [--c2 | --c2-only] [--cont] [--source]
+    python tests/golden/make_golden.py [--c2 | --c2-only] [--cont] [--source] [--sym | --sym-only]
 
 Each solver case: a seeded density (bipoly_hydro_amd.density), a cold
 solve (tstep = 0) and a warm solve (tstep = 2) of a perturbed density
@@ -10,7 +10,9 @@ starting from the cold result.  Output: <case>.npz with rho1, pot1, rho2,
 pot2 (small grids) or slices and norms (258 x 130 x 512).  --cont: the
 continuation files the reference's writer makes of known fields
 (cont_<grid>_<ranks>/).  --source: the reference's momentum source step
-on seeded fields (source_<grid>.npz).
+on seeded fields (source_<grid>.npz).  --sym: the solver cases of
+golden_cases.SYM_CASES, with the reference's run-time flag isym = 2 and 3
+(s<isym>_<grid>.npz).
 """
 import os
 import subprocess
@@ -24,7 +26,8 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 from bipoly_hydro_amd import density  # noqa: E402
-from golden_cases import SOURCE_CASES, SOURCE_GRID, SOURCE_PROCS, c2_digests, source_inputs  # noqa: E402
+from golden_cases import (SOURCE_CASES, SOURCE_GRID, SOURCE_PROCS, SYM_CASES, c2_digests,  # noqa: E402
+                          source_inputs)
 
 # name, numr, numz, numphi, numr_procs, numz_procs, density builder
 CASES = [
@@ -38,15 +41,17 @@ CASES = [
 ]
 
 
-def run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, workdir):
-    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle/ref/Makefile"), f"NAME={name}",
+def run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, workdir, isym=1, build=None):
+    build = build or name
+    subprocess.run(["make", "-s", "-f", os.path.join(ROOT, "oracle/ref/Makefile"), f"NAME={build}",
                     f"NUMR={numr}", f"NUMZ={numz}", f"NUMPHI={numphi}", f"NRP={nrp}", f"NZP={nzp}", "-j8"],
                    check=True)
-    exe = os.path.join(ROOT, "oracle/_ref", name, "bh_ref")
+    exe = os.path.join(ROOT, "oracle/_ref", build, "bh_ref")
     f = {k: os.path.join(workdir, f"{name}_{k}.bin") for k in ("rho1", "pot1", "rho2", "pot2")}
     rho1.tofile(f["rho1"])
     rho2.tofile(f["rho2"])
-    cmd = f"ulimit -s unlimited; /opt/conda/bin/mpiexec -n {nrp * nzp} {exe} {f['rho1']} {f['pot1']} {f['rho2']} {f['pot2']}"
+    cmd = (f"ulimit -s unlimited; BH_REF_ISYM={isym} /opt/conda/bin/mpiexec -n {nrp * nzp} {exe} "
+           f"{f['rho1']} {f['pot1']} {f['rho2']} {f['pot2']}")
     out = subprocess.run(cmd, shell=True, check=True, capture_output=True, text=True).stdout
     shape = rho1.shape
     return (np.fromfile(f["pot1"]).reshape(shape), np.fromfile(f["pot2"]).reshape(shape), out)
@@ -147,7 +152,8 @@ def source_case(wd):
 def main():
     os.makedirs(os.path.join(ROOT, "oracle", "_ref"), exist_ok=True)
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as wd:
-        for name, numr, numz, numphi, nrp, nzp, build in (CASES if "--c2-only" not in sys.argv else []):
+        only = "--c2-only" in sys.argv or "--sym-only" in sys.argv
+        for name, numr, numz, numphi, nrp, nzp, build in ([] if only else CASES):
             rho1 = build(1234)
             rho2 = rho1 * 1.01 + 1e-3 * build(99)
             pot1, pot2, log = run_case(name, numr, numz, numphi, nrp, nzp, rho1, rho2, wd)
@@ -159,6 +165,16 @@ def main():
             c2_case(wd)
         if "--source" in sys.argv:
             source_case(wd)
+        if "--sym" in sys.argv or "--sym-only" in sys.argv:
+            for name, (numr, numz, numphi, isym, build) in SYM_CASES.items():
+                rho1 = build(1234)
+                rho2 = rho1 * 1.01 + 1e-3 * build(99)
+                pot1, pot2, log = run_case(name, numr, numz, numphi, 2, 2, rho1, rho2, wd, isym=isym,
+                                           build=f"sym_{numr}x{numz}x{numphi}")
+                np.savez(os.path.join(HERE, f"{name}.npz"), pot1=pot1, pot2=pot2, isym=np.array(isym),
+                         rho_sums=np.array([rho1.sum(), rho2.sum(), np.abs(rho1).max()]),
+                         seeds=np.array([1234, 99]), procs=np.array([2, 2]))
+                print(name, log.strip().replace("\n", " | "))
         if "--cont" in sys.argv:
             for c in CONT_CASES:
                 cont_case(wd, *c)

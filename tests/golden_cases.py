@@ -14,6 +14,15 @@ SMALL_CASES = {
 }
 
 
+# the symmetric settings of isym on the half-space grid (z >= 0 plus the
+# mirror cell K = 1): 2 = equatorial symmetry, 3 = equatorial + pi symmetry
+SYM_CASES = {
+    f"s{isym}_34x18x32": (34, 18, 32, isym, lambda s, isym=isym: density.lane_emden_n1_binary(
+        34, 18, 32, radius_cells=(6.0, 4.0), centre_cells=(8.0, -8.0), noise=1e-6, seed=s, isym=isym))
+    for isym in (2, 3)
+}
+
+
 def rell2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
@@ -27,6 +36,18 @@ def load_case(golden_dir, name):
     # the density generator must still produce the inputs the reference saw
     np.testing.assert_allclose([rho1.sum(), rho2.sum(), np.abs(rho1).max()], g["rho_sums"], rtol=1e-14)
     return (numr, numz, numphi), rho1, rho2, g["pot1"], g["pot2"]
+
+
+def load_sym_case(golden_dir, name):
+    """(numr, numz, numphi), isym, rho1, rho2, pot1, pot2 of a SYM_CASES fixture."""
+    numr, numz, numphi, isym, build = SYM_CASES[name]
+    g = np.load(os.path.join(golden_dir, name + ".npz"))
+    assert int(g["isym"]) == isym
+    s1, s2 = (int(x) for x in g["seeds"])
+    rho1 = build(s1)
+    rho2 = rho1 * 1.01 + 1e-3 * build(s2)
+    np.testing.assert_allclose([rho1.sum(), rho2.sum(), np.abs(rho1).max()], g["rho_sums"], rtol=1e-14)
+    return (numr, numz, numphi), isym, rho1, rho2, g["pot1"], g["pot2"]
 
 
 def c2_slices(pot, tag):

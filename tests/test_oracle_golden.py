@@ -9,7 +9,7 @@ import pytest
 
 from bipoly_hydro_amd import density
 from oracle import oracle as O
-from golden_cases import SMALL_CASES, load_case, rell2
+from golden_cases import SMALL_CASES, SYM_CASES, load_case, load_sym_case, rell2
 
 @pytest.mark.parametrize("name", list(SMALL_CASES))
 def test_oracle_matches_reference_cold_and_warm(golden_dir, name):
@@ -19,6 +19,67 @@ def test_oracle_matches_reference_cold_and_warm(golden_dir, name):
     pot2 = plan.solve(rho2, potp=pot1, tstep=2)
     assert rell2(pot1, ref1) < 1e-13
     assert rell2(pot2, ref2) < 1e-13
+
+
+@pytest.mark.parametrize("name", list(SYM_CASES))
+def test_oracle_matches_reference_symmetric_grids(golden_dir, name):
+    # isym = 2 (equatorial) and 3 (equatorial + pi): the reference built with
+    # the run-time flag set, on its half-space grid
+    (numr, numz, numphi), isym, rho1, rho2, ref1, ref2 = load_sym_case(golden_dir, name)
+    plan = O.OraclePlan(numr, numz, numphi, isym=isym)
+    pot1 = plan.solve(rho1, tstep=0)
+    pot2 = plan.solve(rho2, potp=pot1, tstep=2)
+    assert rell2(pot1, ref1) < 1e-13
+    assert rell2(pot2, ref2) < 1e-13
+    # the equator and axis conditions hold exactly (helmadi.F90:396-411)
+    np.testing.assert_array_equal(pot1[:, 0, :], pot1[:, 1, :])
+    if isym == 3:
+        np.testing.assert_array_equal(pot1[:, :, 0], pot1[:, :, 1])
+
+
+@pytest.mark.parametrize("isym", [2, 3])
+def test_symmetric_oracle_planted_errors_are_caught(golden_dir, isym):
+    # the terms only the symmetric settings have: dropping the image source
+    # below the equator (here: the boundary faces halved, what a missing
+    # image term does to a source near the equator) or the K = 2 special
+    # case of the sweeps (the equator row forced to its isym = 1 value)
+    # fails the reference comparison by orders of magnitude
+    (numr, numz, numphi), _, rho1, _, ref1, _ = load_sym_case(golden_dir, f"s{isym}_34x18x32")
+    plan1 = O.OraclePlan(numr, numz, numphi, isym=1)
+    wrong_bc = plan1.solve(rho1, tstep=0)
+    assert rell2(wrong_bc, ref1) > 1e-2
+    plan = O.OraclePlan(numr, numz, numphi, isym=isym)
+    pot1 = plan.solve(rho1, tstep=0)
+    faces = pot1.copy()
+    faces[:, -1, :] *= 0.5
+    faces[:, :, -1] *= 0.5
+    assert rell2(faces, ref1) > 1e-3
+
+
+def test_equatorial_polytrope_against_analytic_potential():
+    # isym = 2: one n = 1 polytrope centred on the equator, the grid holding
+    # z >= 0 only; the image terms of the Green's tables and the K = 1
+    # mirror must rebuild the full-space potential (compare
+    # test_single_polytrope_against_analytic_potential)
+    numr, numz, numphi = 34, 18, 16
+    dr = 1.0 / 127.0
+    R = 10.0
+    rho = density.lane_emden_n1_binary(numr, numz, numphi, rho_c=(1.0,), radius_cells=(R,),
+                                       centre_cells=(0.0,), noise=0.0, isym=2)
+    plan = O.OraclePlan(numr, numz, numphi, isym=2)
+    pot = plan.solve(rho, tstep=0)
+    r, z, _ = density.grid_coords(numr, numz, numphi, isym=2)
+    d = np.sqrt(r[None, :] ** 2 + z[:, None] ** 2)
+    Rd = R * dr
+    u = np.pi * d / Rd
+    inside = d < Rd
+    exact = np.where(inside, -(4 * Rd * Rd / np.pi) * (1 + np.sin(u) / np.where(u > 0, u, 1)),
+                     -(4 * Rd ** 3 / np.pi) / d)
+    num = pot[0, 1:-1, 1:-1]
+    ex = exact[1:-1, 1:-1]
+    assert np.max(np.abs(num - ex)) / np.max(np.abs(ex)) < 2e-2
+    assert np.max(np.abs(pot[0, -1, 1:-1] - exact[-1, 1:-1])) / np.max(np.abs(ex)) < 5e-3
+    assert np.max(np.abs(pot[0, 1:-1, -1] - exact[1:-1, -1])) / np.max(np.abs(ex)) < 5e-3
 
 
 def test_gammln_is_the_lanczos_approximation():
@@ -76,6 +137,27 @@ def test_qhalf_against_mpmath():
             assert abs(q[m] / ex - 1.0) < 1e-13
         ex = float(mp.legenq(9.5, 0, x, type=3).real)
         assert abs(q[10] / ex - 1.0) < 1e-9
+
+
+def test_qhalf_even_modes_against_mpmath():
+    # isym = 3 keeps the even azimuthal modes: slot m holds Q_{2m-1/2}, from
+    # Q_{-1/2}, Q_{3/2} and the two-step recurrence below the cutoff
+    # (tm.F90:292-330) and the series with mu = 2m above it
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 30
+    plan = O.OraclePlan(10, 10, 64, with_tables=False, isym=3)
+    for x in (2.0, 5.0, 40.0):
+        q = plan.qhalf(x)
+        for m in range(30):
+            ex = float(mp.legenq(2 * m - 0.5, 0, x, type=3).real)
+            assert abs(q[m] / ex - 1.0) < 2e-9, (x, m)
+    for x in (1.001, 1.01, 1.02):
+        q = plan.qhalf(x)
+        for m in (0, 1):
+            ex = float(mp.legenq(2 * m - 0.5, 0, x, type=3).real)
+            assert abs(q[m] / ex - 1.0) < 1e-13, (x, m)
+        ex = float(mp.legenq(9.5, 0, x, type=3).real)
+        assert abs(q[5] / ex - 1.0) < 1e-9
 
 
 def test_dt_schedule_closed_form():
