@@ -36,6 +36,10 @@ struct DftArgs {
     // x + (xint + xadd) at every point (the FP32 corrections are zero on
     // the faces), and that sum is also stored to xstore
     int warm;
+    // forward with the mask, isym = 2, 3: the layers K = 1, 2 next to the
+    // equator keep their density (potential_solver.F90:76-78 zeroes them
+    // for isym = 1 only)
+    int mask_keep_bottom;
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };

@@ -152,7 +152,7 @@ __device__ __forceinline__ void dft_fwd_tile(const DftArgs &a, int bx, unsigned 
                 double val = x0[tid] + (((N / 2) & 1) ? -xh[tid] : xh[tid]) + t;
                 if (a.mask_numr) {
                     const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
-                    if (J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1) val = 0.0;
+                    if (J >= a.mask_numr - 1 || (K <= 2 && !a.mask_keep_bottom) || K >= a.mask_numz - 1) val = 0.0;
                 }
                 ((T *)a.out)[(long)(N / 2) * a.ldo + gc] = (T)val;
             }
@@ -239,7 +239,7 @@ __device__ __forceinline__ void dft_fwd_tile(const DftArgs &a, int bx, unsigned 
                 }
                 if (a.mask_numr) {
                     const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
-                    if (J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1) val = 0.0;
+                    if (J >= a.mask_numr - 1 || (K <= 2 && !a.mask_keep_bottom) || K >= a.mask_numz - 1) val = 0.0;
                 }
                 ((T *)a.out)[orow * a.ldo + gc] = (T)val;
             }
@@ -925,7 +925,7 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
         long ri = -1;  // offset of (J, K) in a plane of rout, if interior
         if (a.mask_numr) {
             const int J = gc % a.mask_numr + 1, K = gc / a.mask_numr + 1;
-            masked = J >= a.mask_numr - 1 || K <= 2 || K >= a.mask_numz - 1;
+            masked = J >= a.mask_numr - 1 || (K <= 2 && !a.mask_keep_bottom) || K >= a.mask_numz - 1;
             if (a.rout && J >= 2 && J <= a.mask_numr - 1 && K >= 2 && K <= a.mask_numz - 1)
                 ri = (long)(K - 2) * (a.mask_numr - 2) + (J - 2);
         }

@@ -33,6 +33,7 @@ struct GemmArgs {
     const void *Rin; void *C2;    // E_FILTER: C2 = Rin - mu * acc, same indexing as C
     int tile;                     // 0: 64x256, 1: 128x128, 2: 64x64, 3: 64x128, 4: 128x64 (rows x cols)
     int col_fastest;              // tile numbering: 0 row tiles fastest, 1 column tiles fastest
+    int mask_keep_bottom;         // E_MASK: keep the layers K = 1, 2 (isym = 2, 3)
 };
 
 // The instantiated (element types, B mode, epilogue, transposes) combinations.

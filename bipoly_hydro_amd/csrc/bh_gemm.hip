@@ -225,8 +225,9 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs &g, int rt, int ct, int
                 const long o = (long)r * g.ldc + c;
                 if (EPI == E_MASK) {
                     // c = (K-1)*numr + (J-1): zero K in {1,2,numz-1,numz}, J in {numr-1,numr}
+                    // (K in {1,2} kept for isym = 2, 3)
                     int J = c % g.mask_numr + 1, K = c / g.mask_numr + 1;
-                    if (J >= g.mask_numr - 1 || K <= 2 || K >= g.mask_numz - 1) val = (T)0;
+                    if (J >= g.mask_numr - 1 || (K <= 2 && !g.mask_keep_bottom) || K >= g.mask_numz - 1) val = (T)0;
                     C[o] = val;
                 } else if (EPI == E_FILTER) {
                     T w = (T)g.mu[c] * val;

@@ -248,6 +248,8 @@ int make_args(bh_plan *p, const bh_hydro_fields *f, const bh_source_params *prm,
     if (!f->temps || !f->tempa || !f->rho || !f->p || !f->pot || !f->s || !f->a || (need_momenta && !f->t))
         return fail(BH_E_INVALID, "a field pointer is null");
     const Geo &G = p->g;
+    if (G.isym != 1)
+        return fail(BH_E_INVALID, "the source step is restated for isym = 1 only (its isym = 2, 3 walls are not)");
     *g = SrcArgs{};
     g->s = f->s; g->t = f->t; g->a = f->a; g->temps = f->temps; g->tempa = f->tempa;
     g->rho = f->rho; g->p = f->p; g->pot = f->pot;

@@ -15,6 +15,7 @@ struct Geo {
     int numr, numz, numphi, mmax;
     int nri, nzi;  // interior counts numr-2, numz-2
     double dr, dz, dphi, gamma, four_pi, piinv;
+    int isym;      // 1 no symmetry, 2 equatorial, 3 equatorial + pi (setup_frac.F90:313)
 };
 
 // Line-sweep arguments for one ADI half-step (helmadi.F90:195-353).
@@ -32,6 +33,9 @@ struct LineArgs {
     const double *alphar, *betar;  // [numr + 1], 1-based
     const double *elzb;            // [numphi][numr + 1], 1-based J
     int numr;
+    // isym = 2, 3: K = 2 borders the equator, whose mirror cell K = 1 takes
+    // K = 2's value (helmadi.F90:211-238, 334-340)
+    int eqsym;
 };
 
 struct FillArgs {
@@ -47,7 +51,9 @@ struct TransposeArgs {
 struct AxisArgs {
     double *pot;
     int numr, numz, numphi;
+    int mode;  // AXIS_ROTATE, AXIS_MIRROR or EQUATOR
 };
+enum { AXIS_ROTATE = 0, AXIS_MIRROR = 1, EQUATOR = 2 };
 struct ContractArgs {
     const double *table, *proj;
     double *out;
@@ -55,12 +61,14 @@ struct ContractArgs {
     const double *aux;     // sqrt(rhf) [numr + 1], 1-based J (symmetric top table)
     const float *table32;  // FP32 symmetric top table (in place of table)
     int nparts;            // partial sums per value (top/bottom reduction)
+    const double *hankel;  // side, isym = 2, 3: image table H[m][kg + kl - 4][jr] (null: none)
 };
 struct ProjArgs {
     const double *rho;   // [numphi][M]
     const double *coef;  // [chunk][numphi/2][64] folded mode coefficients
     double *proj;        // [2*mmax-1][M]
     int M, numphi, mmax;
+    int even;            // isym = 3: every slot's frequency 2m is even
 };
 struct RebuildArgs {
     const double *Stb, *Ss, *besc, *bess;
@@ -70,6 +78,8 @@ struct RebuildArgs {
     int nsplit_tb;  // partial sums per top/bottom value in Stb
     double *X1, *X2;  // when X1 is given: each face value also placed at its (J, K) in X1 and X2
     int x2_klayout;
+    int fstep;   // azimuthal frequency of mode slot m is fstep * m (2 for isym = 3)
+    int bottom;  // the bottom face exists (isym = 1)
 };
 
 // grid-stride fill for plan set-up
@@ -82,12 +92,15 @@ Op transpose_op(const double *src, double *dst, int numr, int numz, int nplanes)
 // one ADI half-step; nl = lines per workgroup (8 or 16, 0: default)
 Op sweep_op(const LineArgs &a, int nplanes, int nl);
 void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s);
-// axis ghost by a pi rotation (helmadi.F90:396-405)
-Op axis_op(double *pot, int numr, int numz, int numphi);
+// axis ghost by a pi rotation (helmadi.F90:396-405; mode AXIS_ROTATE), or
+// a mirror (isym = 3, AXIS_MIRROR); EQUATOR: pot(J, 1, L) = pot(J, 2, L)
+// (helmadi.F90:407-411)
+Op axis_op(double *pot, int numr, int numz, int numphi, int mode = AXIS_ROTATE);
 
 // Green's tables ------------------------------------------------------------
 struct TableArgs {
     int numr, numz, mmax, nser;
+    int isym;               // 2, 3: add the image below the equator; 3: even modes
     double hypr_cutoff, piinv, dz;
     const double *rhf_g;    // [numr + 1] 1-based
     const double *rhfinv_g; // [numr + 1]
@@ -104,8 +117,9 @@ inline size_t topsym_table_elems(int numr, int numz, int mmax)
     const size_t NB = (numr - 2 + 15) / 16;
     return (size_t)mmax * NB * (NB + 1) / 2 * (numz - 2) * 256;
 }
-// side table in Toeplitz form S[m][d][jr], d = kg - kl + nzi - 1
-void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s);
+// side table in Toeplitz form S[m][d][jr], d = kg - kl + nzi - 1; with
+// isym = 2, 3 also the image part H[m][s][jr], s = kg + kl - 4 (H may be null)
+void launch_gen_side_table(const TableArgs &a, double *S, double *H, hipStream_t s);
 
 // boundary contraction and face rebuild (bessel.F90:176-296, 396-488);
 // the side contraction writes CT_SPLIT partial sums that the face kernel adds
@@ -120,8 +134,8 @@ __host__ __device__ inline int topsym_splits(int numz) { return (numz - 2 + TS_P
 // folded in quads (FP64 matrix cores); coef from proj_coefficients
 constexpr int PROJ_MODES = 32;  // modes per chunk (grid x)
 int proj_coef_elems(int numphi, int mmax);
-Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax);
-void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef);
+Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax, int isym = 1);
+void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef, int isym = 1);
 // Order of the 256 values of one 16 x 16 block plane of the symmetric top
 // table: sixteen 4 x 4 sub-blocks (rows 4ig.., columns 4jg..), stored so
 // that 16-byte piece h of sub-block g = 4ig + jg sits at (h * 16 + g) * 16 B
@@ -145,13 +159,16 @@ Op topsym_scale_op(const double *proj, const double *sqrt_rhf, double *Pt, int n
 Op contract_topsym_op(const double *TS, const float *TS32, const double *proj, const double *sqrt_rhf,
                       const double *Pt, double *Ptb, int numr, int numz, int mmax);
 Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int numr, int numz, int mmax);
-Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax);
+Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax,
+                    const double *H = nullptr);
 // faces in Fourier mode order FF[mode row][fi]: fi in [0,nri) top J=fi+2,
 // [nri,2nri) bottom, [2nri,2nri+nzi) side, straight from the contraction sums;
 // with X1, each value also placed in X1 (J-layout) and X2 (K-layout when
-// x2_klayout, else J-layout; may be null), the outer corners set to zero
+// x2_klayout, else J-layout; may be null), the outer corners set to zero.
+// isym = 2, 3 have no bottom face (those values are not written) and
+// isym = 3 mode slot m has frequency 2m.
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
                  int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1 = nullptr,
-                 double *X2 = nullptr, int x2_klayout = 0);
+                 double *X2 = nullptr, int x2_klayout = 0, int isym = 1);
 
 }  // namespace bh

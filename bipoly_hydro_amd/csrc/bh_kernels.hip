@@ -208,6 +208,7 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
             load_row(rh + (long)(i + 2) * a.PA, rn);
         }
         double cd, cp, cn;
+        const bool eqrow = a.dir == 0 && a.eqsym && i == 0;  // K = 2
         if (a.dir == 0) {
             cd = dtt_mtwogamma; cp = a.gamma; cn = a.gamma;
         } else {
@@ -223,15 +224,19 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
             double t = 0.0;
             if (e < len) {
                 if (a.dir == 0) {
-                    // knownr (helmadi.F90:240-257)
-                    t = -a.four_pi * rc[v] + cd * xc[v] + cp * (xn[v] + xm[v]);
+                    // knownr (helmadi.F90:240-257); the row next to the
+                    // equator (isym = 2, 3) leaves out its mirror K = 1
+                    // and weights K = 2 by dt - gamma (helmadi.F90:228-236)
+                    if (eqrow) t = -a.four_pi * rc[v] + (a.dt - a.gamma) * xc[v] + a.gamma * xn[v];
+                    else t = -a.four_pi * rc[v] + cd * xc[v] + cp * (xn[v] + xm[v]);
                     if (e == len - 1) t = t - a.alphar[a.numr - 1] * b1;
                 } else {
-                    // knownz (helmadi.F90:284-340)
+                    // knownz (helmadi.F90:284-340); the bottom face enters
+                    // for isym = 1 only
                     t = -a.four_pi * rc[v] + cd * xc[v] + cn * xn[v];
                     if (cp != 0.0) t = t + cp * xm[v];
                     if (e == len - 1) t = t + a.gamma * b1;
-                    if (e == 0) t = t + a.gamma * b0;
+                    if (e == 0 && !a.eqsym) t = t + a.gamma * b0;
                 }
             }
             val[v] = t;
@@ -324,25 +329,31 @@ Op sweep_op(const LineArgs &a, int nplanes, int nl)
 
 void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s) { launch_op(sweep_op(a, nplanes, 0), s); }
 
-// axis ghost by a pi rotation: pot(1,K,L) = pot(2,K,L +- numphi/2)
-// (helmadi.F90:396-405)
+// axis ghost by a pi rotation: pot(1,K,L) = pot(2,K,L +- numphi/2), or by
+// a mirror for isym = 3: pot(1,K,L) = pot(2,K,L) (helmadi.F90:396-405);
+// the equator for isym = 2, 3: pot(J,1,L) = pot(J,2,L) (helmadi.F90:407-411)
 __device__ __forceinline__ void axis_tile(const AxisArgs &a, int bx, int by, int)
 {
     const int numr = a.numr, numz = a.numz, numphi = a.numphi;
     double *pot = a.pot;
+    const int L = by;
+    const long plane = (long)numr * numz;
+    if (a.mode == EQUATOR) {
+        const int J = bx * 128 + threadIdx.x;
+        if (J < numr) pot[(long)L * plane + J] = pot[(long)L * plane + numr + J];
+        return;
+    }
     int K = bx * 128 + threadIdx.x;
-    int L = by;
     if (K >= numz) return;
     int h = numphi / 2;
-    int Ls = (L < h) ? L + h : L - h;
-    long plane = (long)numr * numz;
+    int Ls = a.mode == AXIS_MIRROR ? L : (L < h) ? L + h : L - h;
     pot[(long)L * plane + (long)K * numr] = pot[(long)Ls * plane + (long)K * numr + 1];
 }
 
-Op axis_op(double *pot, int numr, int numz, int numphi)
+Op axis_op(double *pot, int numr, int numz, int numphi, int mode)
 {
-    AxisArgs a{pot, numr, numz, numphi};
-    return Op::make(OP_AXIS, 0, a, (numz + 127) / 128, numphi, 1);
+    AxisArgs a{pot, numr, numz, numphi, mode};
+    return Op::make(OP_AXIS, 0, a, ((mode == EQUATOR ? numr : numz) + 127) / 128, numphi, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -376,9 +387,21 @@ __device__ void d_ellke(double k2, double kp2, double *K, double *E)
     *E = *K * (1.0 - s);
 }
 
-// Writes coef * Q_{m-1/2}(xm) to out[m * mstride], m = 0..mmax-1 (rounded
-// to the table's type).
-template <class TT>
+// Q_{mu-1/2}(xm) for the mode slots m = 0..mmax-1 (mu = m; isym = 3: the
+// even mu = 2m, from Q_{-1/2}, Q_{3/2} and the two-step recurrence of
+// tm.F90:292-330), stored to out[m * mstride] by STORE: Q_PLAIN coef * Q,
+// Q_RAW Q alone, Q_IMAGE coef * (Q + the raw value already there) -- the
+// image sum coef (Q(chi+) + Q(chi-)) of tm.F90:282-285 in two passes
+// (rounded to the table's type).
+enum { Q_PLAIN = 0, Q_RAW = 1, Q_IMAGE = 2 };
+template <int STORE, class TT>
+__device__ __forceinline__ void d_qstore(TT *o, double coef, double q)
+{
+    if (STORE == Q_PLAIN) *o = (TT)(coef * q);
+    else if (STORE == Q_RAW) *o = (TT)q;
+    else *o = (TT)(coef * (q + (double)*o));
+}
+template <int STORE, class TT>
 __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, TT *out, long mstride)
 {
     const int mmax = a.mmax;
@@ -389,25 +412,41 @@ __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, TT *ou
         double kk, ee;
         d_ellke(mum * mum, (1.0 - mum) * (1.0 + mum), &kk, &ee);
         double q0 = kk * mum;
-        out[0] = (TT)(coef * q0);
+        d_qstore<STORE>(out, coef, q0);
         if (mmax < 2) return;
+        if (a.isym == 3) {
+            const double four_thirds = 4.0 / 3.0, one_third = 1.0 / 3.0;
+            double q1 = (four_thirds * xm * xm - one_third) * mum * kk - four_thirds * xm * lam * ee;
+            d_qstore<STORE>(out + mstride, coef, q1);
+            for (int m = 3; m <= mmax; ++m) {
+                const double nu = a.nu[m], den = (2.0 * nu - 1.0) * (nu + 2.0) * (nu + 1.0);
+                const double q2 = q1 * ((2.0 * nu + 3.0) * (2.0 * nu + 1.0) * xm * xm / ((nu + 2.0) * (nu + 1.0)) -
+                                        (2.0 * nu + 3.0) * nu * nu / den - (nu + 1.0) / (nu + 2.0)) -
+                                  q0 * (2.0 * nu + 3.0) * (nu - 1.0) * nu / den;
+                d_qstore<STORE>(out + (long)(m - 1) * mstride, coef, q2);
+                q0 = q1;
+                q1 = q2;
+            }
+            return;
+        }
         double q1 = xm * mum * kk - lam * ee;
-        out[mstride] = (TT)(coef * q1);
+        d_qstore<STORE>(out + mstride, coef, q1);
         for (int m = 3; m <= mmax; ++m) {
             double nu = a.nu[m];
             double q2 = (2.0 * nu + 1.0) / (nu + 1.0) * xm * q1 - nu / (nu + 1.0) * q0;
-            out[(long)(m - 1) * mstride] = (TT)(coef * q2);
+            d_qstore<STORE>(out + (long)(m - 1) * mstride, coef, q2);
             q0 = q1;
             q1 = q2;
         }
     } else {
         double l2x = log(2.0 * xm);
         double y = 1.0 / (xm * xm);
+        const double mstep = a.isym == 3 ? 2.0 : 1.0;
         for (int m = 0; m < mmax; ++m) {
             const double *c = a.cser + (long)m * a.nser;
             double s = c[a.nser - 1];
             for (int ir = a.nser - 2; ir >= 0; --ir) s = s * y + c[ir];
-            out[(long)m * mstride] = (TT)(coef * (exp(a.coefh[m] - ((double)m + 0.5) * l2x) * s));
+            d_qstore<STORE>(out + (long)m * mstride, coef, exp(a.coefh[m] - (mstep * (double)m + 0.5) * l2x) * s);
         }
     }
 }
@@ -442,9 +481,17 @@ __global__ __launch_bounds__(128) void gen_topsym_table_kernel(TableArgs a, TT *
     const int jl = jr + 2, kl = kz + 2, jg = jo + 2;
     const double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
     const double zT = a.zhf_g[a.numz], zhf = a.zhf_g[kl];
-    const double xm = 0.5 * rhfinv * a.rhfinv_g[jg] *
-                      ((zT - zhf) * (zT - zhf) + a.rhf_g[jg] * a.rhf_g[jg] + rhf * rhf);
-    d_qhalf_store(a, xm, a.piinv, out, mstride);
+    if (a.isym == 1) {
+        const double xm = 0.5 * rhfinv * a.rhfinv_g[jg] *
+                          ((zT - zhf) * (zT - zhf) + a.rhf_g[jg] * a.rhf_g[jg] + rhf * rhf);
+        d_qhalf_store<Q_PLAIN>(a, xm, a.piinv, out, mstride);
+        return;
+    }
+    // isym = 2, 3: the source and its image below the equator (tm.F90:205-214)
+    const double xm = 0.5 * rhfinv * a.rhfinv_g[jg] * ((zT - zhf) * (zT - zhf) + rhf * rhf + a.rhf_g[jg] * a.rhf_g[jg]);
+    const double xp = 0.5 * rhfinv * a.rhfinv_g[jg] * ((zT + zhf) * (zT + zhf) + rhf * rhf + a.rhf_g[jg] * a.rhf_g[jg]);
+    d_qhalf_store<Q_RAW>(a, xm, a.piinv, out, mstride);
+    d_qhalf_store<Q_IMAGE>(a, xp, a.piinv, out, mstride);
 }
 
 __global__ __launch_bounds__(128) void gen_side_table_kernel(TableArgs a, double *S)
@@ -460,8 +507,28 @@ __global__ __launch_bounds__(128) void gen_side_table_kernel(TableArgs a, double
     double rB = a.rhf_g[a.numr], rBinv = 1.0 / rB;
     double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
     double coef = sqrt(rhf * rBinv) * a.piinv;
-    double xm = 0.5 * rhfinv * rBinv * (dz * dz + rB * rB + rhf * rhf);
-    d_qhalf_store(a, xm, coef, S + (long)d * nri + jr, (long)nd * nri);
+    double xm = a.isym == 1 ? 0.5 * rhfinv * rBinv * (dz * dz + rB * rB + rhf * rhf)
+                            : 0.5 * rhfinv * rBinv * (dz * dz + rhf * rhf + rB * rB);
+    d_qhalf_store<Q_PLAIN>(a, xm, coef, S + (long)d * nri + jr, (long)nd * nri);
+}
+
+// isym = 2, 3: the side table's image part (sm.F90:222-226), Hankel in z:
+// H[m][s][jr], s = kg + kl - 4, from z_target + z_source = (s + 1) dz
+__global__ __launch_bounds__(128) void gen_hankel_table_kernel(TableArgs a, double *H)
+{
+    const int nri = a.numr - 2, nzi = a.numz - 2;
+    const int nd = 2 * nzi - 1;
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nd * nri) return;
+    int jr = (int)(t % nri);
+    int sidx = (int)(t / nri);
+    int jl = jr + 2;
+    double sz = (double)(sidx + 1) * a.dz;
+    double rB = a.rhf_g[a.numr], rBinv = 1.0 / rB;
+    double rhf = a.rhf_g[jl], rhfinv = a.rhfinv_g[jl];
+    double coef = sqrt(rhf * rBinv) * a.piinv;
+    double xp = 0.5 * rhfinv * rBinv * (sz * sz + rhf * rhf + rB * rB);
+    d_qhalf_store<Q_PLAIN>(a, xp, coef, H + (long)sidx * nri + jr, (long)nd * nri);
 }
 
 void launch_gen_topsym_table(const TableArgs &a, double *T, float *T32, hipStream_t s)
@@ -473,10 +540,11 @@ void launch_gen_topsym_table(const TableArgs &a, double *T, float *T32, hipStrea
     else hipLaunchKernelGGL(gen_topsym_table_kernel<float>, grid, dim3(128), 0, s, a, T32);
 }
 
-void launch_gen_side_table(const TableArgs &a, double *S, hipStream_t s)
+void launch_gen_side_table(const TableArgs &a, double *S, double *H, hipStream_t s)
 {
     long total = (long)(2 * (a.numz - 2) - 1) * (a.numr - 2);
     hipLaunchKernelGGL(gen_side_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, S);
+    if (H) hipLaunchKernelGGL(gen_hankel_table_kernel, dim3((unsigned)((total + 127) / 128)), dim3(128), 0, s, a, H);
 }
 
 // ---------------------------------------------------------------------------
@@ -509,10 +577,12 @@ __host__ __device__ inline int proj_quads(int numphi) { return numphi / 4 < 4 ? 
 // coef[c][a][i][rt], a < max(N/4, 4): rt 0 cos(m theta_a) with m = c
 // PROJ_MODES + 2i, rt 1 the same with m + 1, rt 2 and 3 the sines; theta_a =
 // (a + 1/2) dphi.  1 and 0 for m = 0; zeros for m >= mmax and for quads
-// past N/4 (N < 16).  The angle is formed as the bessel rows form it (L = a + 1).
+// past N/4 (N < 16).  The angle is formed as the bessel rows form it (L =
+// a + 1); for isym = 3 slot m has frequency 2m (potsetup.F90:120-129), all
+// even, and every row tile then folds as the even ones do (ProjArgs::even).
 int proj_coef_elems(int numphi, int mmax) { return (mmax + PROJ_MODES - 1) / PROJ_MODES * proj_quads(numphi) * 64; }
 
-void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef)
+void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *coef, int isym)
 {
     const int Q = numphi / 4, Qt = proj_quads(numphi), nc = (mmax + PROJ_MODES - 1) / PROJ_MODES;
     coef->assign((size_t)proj_coef_elems(numphi, mmax), 0.0);
@@ -522,7 +592,7 @@ void proj_coefficients(int numphi, int mmax, double dphi, std::vector<double> *c
                 for (int rt = 0; rt < 4; ++rt) {
                     const int m = c * PROJ_MODES + 2 * i + (rt & 1);
                     if (m >= mmax) continue;
-                    const double ang = 0.5 * dphi * (double)m * (2.0 * (double)(q + 1) - 1.0);
+                    const double ang = (isym == 3 ? dphi : 0.5 * dphi) * (double)m * (2.0 * (double)(q + 1) - 1.0);
                     const double v = rt < 2 ? (m == 0 ? 1.0 : std::cos(ang)) : (m == 0 ? 0.0 : std::sin(ang));
                     (*coef)[(((size_t)c * Qt + q) * 16 + i) * 4 + rt] = v;
                 }
@@ -563,13 +633,16 @@ __device__ __forceinline__ void proj_tile(const ProjArgs &a, int chunk, int cb, 
         s.c01 = t[0];
         s.c23 = t[1];
     };
+    // the odd-slot row tiles fold as odd frequencies, or (isym = 3: slot m
+    // has frequency 2m) as even ones: the sign of pair b in their sums
+    const double ob = a.even ? 1.0 : -1.0;
     auto step = [&](const Step &s) {
         const double ua = s.xa + s.xd, va = s.xa - s.xd;  // pair a
         const double ub = s.xb + s.xc, vb = s.xb - s.xc;  // pair N/2-1-a
         acc[0] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c01.x, ua + ub, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c01.y, ua - ub, acc[1], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c01.y, fma(ob, ub, ua), acc[1], 0, 0, 0);
         acc[2] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c23.x, va - vb, acc[2], 0, 0, 0);
-        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c23.y, va + vb, acc[3], 0, 0, 0);
+        acc[3] = __builtin_amdgcn_mfma_f64_16x16x4f64(s.c23.y, fma(-ob, vb, va), acc[3], 0, 0, 0);
     };
     Step ring[PROJ_RING];
 #pragma unroll
@@ -597,9 +670,9 @@ __device__ __forceinline__ void proj_tile(const ProjArgs &a, int chunk, int cb, 
         }
 }
 
-Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax)
+Op proj_op(const double *rho, const double *coef, double *proj, int M, int numphi, int mmax, int isym)
 {
-    ProjArgs a{rho, coef, proj, M, numphi, mmax};
+    ProjArgs a{rho, coef, proj, M, numphi, mmax, isym == 3 ? 1 : 0};
     return Op::make(OP_PROJ, 0, a, (mmax + PROJ_MODES - 1) / PROJ_MODES, (M + 63) / 64, 1);
 }
 
@@ -999,7 +1072,10 @@ Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int 
 // columns in a fixed order (wavefront butterflies, then the 4 wavefronts).
 constexpr int SD_TGT = 16;
 
-__device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx, int by, int bz)
+// IMG (isym = 2, 3): the image part H[m][ko + kz][jr] is added, from a
+// second window of 2 SD_TGT - 1 values along its anti-diagonals.
+template <bool IMG>
+__device__ __forceinline__ void contract_side_t(const ContractArgs &a, int bx, int by, int bz)
 {
     const double *__restrict__ S = a.table;
     const double *__restrict__ proj = a.proj;
@@ -1023,11 +1099,17 @@ __device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx
         for (int kb = kzs; kb < kze; kb += SD_TGT) {
             // window: diagonals d0 - (SD_TGT-1) .. d0 + (SD_TGT-1), d0 = k0 - kb + nzi - 1
             const int d0 = k0 - kb + nzi - 1;
-            double w[2 * SD_TGT - 1], xc[SD_TGT], xs[SD_TGT];
+            double w[2 * SD_TGT - 1], wh[IMG ? 2 * SD_TGT - 1 : 1], xc[SD_TGT], xs[SD_TGT];
 #pragma unroll
             for (int i = 0; i < 2 * SD_TGT - 1; ++i) {
                 const int d = min(max(d0 - (SD_TGT - 1) + i, 0), nd - 1);
                 w[i] = sm[(long)d * nri + jc];
+            }
+            if constexpr (IMG) {
+                // anti-diagonals k0 + kb .. k0 + kb + 2 SD_TGT - 2
+                const double *hm = a.hankel + (long)m * nd * nri;
+#pragma unroll
+                for (int i = 0; i < 2 * SD_TGT - 1; ++i) wh[i] = hm[(long)min(k0 + kb + i, nd - 1) * nri + jc];
             }
 #pragma unroll
             for (int k = 0; k < SD_TGT; ++k) {
@@ -1042,7 +1124,7 @@ __device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx
             for (int k = 0; k < SD_TGT; ++k)
 #pragma unroll
                 for (int t = 0; t < SD_TGT; ++t) {
-                    const double g = w[SD_TGT - 1 + t - k];
+                    const double g = IMG ? w[SD_TGT - 1 + t - k] + wh[t + k] : w[SD_TGT - 1 + t - k];
                     acc[0][t] = fma(g, xc[k], acc[0][t]);
                     acc[1][t] = fma(g, xs[k], acc[1][t]);
                 }
@@ -1084,10 +1166,19 @@ __device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx
     }
 }
 
-Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax)
+__device__ __forceinline__ void contract_side_tile(const ContractArgs &a, int bx, int by, int bz)
 {
-    ContractArgs a{S, proj, Ss, numr, numz, mmax, nullptr, nullptr, 0};
-    return Op::make(OP_CONTRACT_SIDE, 0, a, (numz - 2 + SD_TGT - 1) / SD_TGT, mmax, CT_SPLIT);
+    contract_side_t<false>(a, bx, by, bz);
+}
+__device__ __forceinline__ void contract_side_img_tile(const ContractArgs &a, int bx, int by, int bz)
+{
+    contract_side_t<true>(a, bx, by, bz);
+}
+
+Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax, const double *H)
+{
+    ContractArgs a{S, proj, Ss, numr, numz, mmax, nullptr, nullptr, 0, H};
+    return Op::make(OP_CONTRACT_SIDE, H ? 1 : 0, a, (numz - 2 + SD_TGT - 1) / SD_TGT, mmax, CT_SPLIT);
 }
 
 // Boundary faces in helmadi's Fourier mode order.  bessel rebuilds each
@@ -1104,7 +1195,9 @@ Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, i
 // c = cos alpha, s = sin alpha, and the mode rows hold a_k = Re X_k (rows
 // 0..N/2) and -b_k = Im X_k (rows N/2+1..N-1, k = 1..N/2-1).  So the faces
 // come straight from the contraction sums, with no phi-space face array and
-// no transform (aliasing included when mmax > N/2).
+// no transform (aliasing included when mmax > N/2).  With isym = 3 slot m
+// carries frequency p = 2m (its cos, sin rows have angle 2m theta) and
+// counts where p = +-k (mod N); f then doubles (bessel.F90:170-174).
 __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, int by, int)
 {
     const double *__restrict__ Stb = a.Stb;
@@ -1127,6 +1220,7 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
         return;
     }
     if (fi >= nf) return;
+    if (!a.bottom && fi >= nri && fi < 2 * nri) return;  // isym = 2, 3: no bottom face
     const double *sc, *ss;
     long st, sst;  // per-m stride, per-split stride
     int idx;
@@ -1150,9 +1244,12 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     const int k = imag ? row - N / 2 : row;
     double re = 0.0, im = 0.0;
     if (k % N == 0) re = S_c(0);
-    // the m = 1..mmax-1 with m = k or m = -k (mod N), in increasing order
-    // (k = 0..N/2: m = k + jN and m = N - k + jN, m >= 1)
-    int mp = k == 0 ? N : k, mq = k == 0 ? N : N - k;
+    // the m = 1..mmax-1 with fstep m = k or -k (mod N), in increasing order
+    // (k = 0..N/2: fstep m = k + jN and N - k + jN, m >= 1; none when
+    // fstep = 2 and k is odd)
+    const int fs = a.fstep, stp = N / fs;
+    int mp = (k == 0 ? N : k) / fs, mq = (k == 0 ? N : N - k) / fs;
+    if (k % fs) mp = mq = mmax;
     for (;;) {
         const int m = min(mp, mq);
         if (m >= mmax) break;
@@ -1160,8 +1257,8 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
         const double c = a.besc[(long)m * N], s = a.bess[(long)m * N];
         const double C = S_c(m), S = S_s(m);
         const double r = C * c + S * s, q = C * s - S * c;
-        if (plus) { re += r; im += q; mp += N; }
-        if (minus) { re += r; im -= q; mq += N; }
+        if (plus) { re += r; im += q; mp += stp; }
+        if (minus) { re += r; im -= q; mq += stp; }
     }
     const double v = a.factor * (double)N * (imag ? im : re);
     a.F[(long)row * nf + fi] = v;
@@ -1178,9 +1275,11 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
 }
 
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
-                 int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1, double *X2, int x2_klayout)
+                 int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1, double *X2, int x2_klayout,
+                 int isym)
 {
-    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor, nsplit_tb, X1, X2, x2_klayout};
+    RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor, nsplit_tb, X1, X2, x2_klayout,
+                  isym == 3 ? 2 : 1, isym == 1 ? 1 : 0};
     const int nf = 2 * (numr - 2) + (numz - 2) + (X1 ? 1 : 0);  // + the corner thread
     return Op::make(OP_FACE_MODES, 0, a, (nf + 127) / 128, numphi, 1);
 }
@@ -1201,6 +1300,7 @@ BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
 BH_KERNEL_PAIR(axis, AxisArgs, 128)
 BH_KERNEL_PAIR(proj, ProjArgs, 256)
 BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_side_img, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
@@ -1228,7 +1328,10 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
         BH_REF(OP_TRANSPOSE, transpose, 256)
         BH_REF(OP_AXIS, axis, 128)
         BH_REF(OP_PROJ, proj, 256)
-        BH_REF(OP_CONTRACT_SIDE, contract_side, 256)
+    case OP_CONTRACT_SIDE:
+        if (variant == 1) *k = {(const void *)&contract_side_img_kernel, (const void *)&contract_side_img_grouped, 256};
+        else *k = {(const void *)&contract_side_kernel, (const void *)&contract_side_grouped, 256};
+        return true;
         BH_REF(OP_TOPSYM_SCALE, topsym_scale, 256)
     case OP_CONTRACT_TOPSYM:
         switch (variant) {

@@ -154,6 +154,7 @@ void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, doubl
     if (o.dfwd64 && p->use_dft) {
         DftArgs d{B, ldb, C, ldc, o.dfwd64, N, Mcols, mask ? p->g.numr : 0, mask ? p->g.numz : 0};
         d.max_blocks = max_blocks;
+        d.mask_keep_bottom = p->g.isym != 1;
         out->push_back(dft_op(d, DFT_FWD_F64));
         return;
     }
@@ -161,6 +162,7 @@ void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, doubl
     a.A = o.Dc; a.lda = nh; a.B = B; a.ldb = ldb; a.C = C; a.ldc = ldc;
     a.R = nh; a.M = Mcols; a.Kd = nh; a.batch = 1; a.nfold = N;
     a.mask_numr = p->g.numr; a.mask_numz = p->g.numz;
+    a.mask_keep_bottom = p->g.isym != 1;
     a.tile = Mcols >= 4096 ? p->tile_dft : 2;
     a.col_fastest = p->order_dft;
     out->push_back(gemm_op(a, gemm_inst64(B_FOLD_PLUS, mask ? E_MASK : E_PLAIN)));
@@ -196,7 +198,7 @@ void dft_inv64_ops(const bh_plan *p, const double *X, long ldx, int Mcols, doubl
 // Boundary projections of rho onto the mmax azimuthal modes (bessel.F90:176-184, 226-235)
 void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 {
-    out->push_back(proj_op(rho, p->ops->Pf, p->proj, p->g.numr * p->g.numz, p->g.numphi, p->g.mmax));
+    out->push_back(proj_op(rho, p->ops->Pf, p->proj, p->g.numr * p->g.numz, p->g.numphi, p->g.mmax, p->g.isym));
 }
 
 // Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF, also
@@ -209,7 +211,7 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
     out->push_back(Op::mark(MARK_FORK));
-    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax));
+    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax, p->tab->H));
     out->push_back(Op::mark(MARK_SIDE_END));
     out->push_back(Op::mark(5));
     if (!tb_blocked(numr)) out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
@@ -218,8 +220,10 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
     out->push_back(Op::mark(MARK_JOIN));
-    out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax,
-                                 -g.dr * g.dz * g.dphi, 1, X1, X2, x2_klayout));
+    // factor of bessel.F90:170-174 (doubled for isym = 3)
+    const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
+    out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax, factor, 1,
+                                 X1, X2, x2_klayout, g.isym));
 }
 
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
@@ -433,6 +437,7 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
         r.coef_stride = nri;
         r.dir = 0; r.dt = dt[st]; r.gamma = g.gamma; r.four_pi = g.four_pi;
         r.alphar = o.alphar; r.betar = o.betar; r.elzb = o.elzb; r.numr = numr;
+        r.eqsym = g.isym != 1;
         out->push_back(sweep_op(r, N, 0));
         LineArgs z = r;
         z.xin = p->XK; z.xout = p->XJ; z.rh = p->RK;
@@ -446,9 +451,12 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
         out->push_back(sweep_op(z, N, 0));
     }
     out->push_back(Op::mark(3));
-    // 5. inverse DFT with the 2/N normalisation, then the axis ghost
+    // 5. inverse DFT with the 2/N normalisation, then the axis ghost (a
+    //    mirror for isym = 3) and, for isym = 2, 3, the equator
+    //    (helmadi.F90:396-411)
     dft_inv64_ops(p, p->XJ, M, (int)M, d_pot, M, out);
-    out->push_back(axis_op(d_pot, numr, numz, N));
+    out->push_back(axis_op(d_pot, numr, numz, N, g.isym == 3 ? AXIS_MIRROR : AXIS_ROTATE));
+    if (g.isym != 1) out->push_back(axis_op(d_pot, numr, numz, N, EQUATOR));
     out->push_back(Op::mark(4));
 }
 

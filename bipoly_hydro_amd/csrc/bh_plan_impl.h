@@ -66,6 +66,7 @@ struct GeoTables {
     double *T = nullptr;         // top (and mirrored bottom) table, blocked symmetric [m][pair][kz][16][16]
     float *T32 = nullptr;        // ... the same in FP32 (BH_PREC_MIXED_TABLE32 plans; T is then null)
     double *S = nullptr;         // side table, Toeplitz in z [m][d][jr]
+    double *H = nullptr;         // isym = 2, 3: side image table, Hankel in z [m][kg + kl - 4][jr]
     double *sqrt_rhf = nullptr;  // sqrt(rhf) [numr + 1], 1-based J
 };
 

@@ -58,10 +58,13 @@ int normalize_desc(const bh_grid_desc *desc, int precision, bh_grid_desc *out)
     if (d.nsteps_cold == 0) d.nsteps_cold = 20;       // potential_solver.F90:88-93
     if (d.nsteps_warm == 0) d.nsteps_warm = 5;
     if (d.refine_passes == 0) d.refine_passes = 2;
-    if (d.isym != 1) return fail(BH_E_INVALID, "isym: only 1 (no symmetry) is supported");
+    if (d.isym < 1 || d.isym > 3) return fail(BH_E_INVALID, "isym must be 1, 2 or 3");
+    if (d.isym != 1 && is_mixed(precision))
+        return fail(BH_E_INVALID, "isym = 2, 3 need precision BH_PREC_FP64 (the mixed solve's sine basis in z "
+                                  "is the one of two Dirichlet faces)");
     if (d.numr < 5 || d.numz < 5) return fail(BH_E_INVALID, "numr and numz must be >= 5");
     if (d.numphi < 4 || (d.numphi & (d.numphi - 1))) return fail(BH_E_INVALID, "numphi must be a power of two >= 4");
-    if (d.numz % 2) return fail(BH_E_INVALID, "numz must be even for isym = 1 (symmetric z grid)");
+    if (d.isym == 1 && d.numz % 2) return fail(BH_E_INVALID, "numz must be even for isym = 1 (symmetric z grid)");
     if (d.mmax < 1 || d.mmax > 250) return fail(BH_E_INVALID, "mmax must be in [1, 250]");
     if (d.numr - 2 > 512 || d.numz - 2 > 512) return fail(BH_E_INVALID, "numr-2 and numz-2 must be <= 512");
     if (d.nsteps_cold < 2 || d.nsteps_warm < 2) return fail(BH_E_INVALID, "nsteps must be >= 2");
@@ -79,6 +82,7 @@ void host_geometry(const bh_grid_desc &d, HostGeo *h)
     const int numr = d.numr, numz = d.numz, N = d.numphi, mmax = d.mmax;
     Geo &g = h->g;
     g.numr = numr; g.numz = numz; g.numphi = N; g.mmax = mmax; g.nri = numr - 2; g.nzi = numz - 2;
+    g.isym = d.isym;
     // constants and grid: setup_frac.F90:345, 374-379, 428-463; potsetup.F90:113-114, 151-153
     const double pi = std::acos(-1.0);
     g.dr = d.dr; g.dz = d.dr;
@@ -97,7 +101,9 @@ void host_geometry(const bh_grid_desc &d, HostGeo *h)
         h->rinv_g[J] = (h->r_g[J] != 0.0) ? 1.0 / h->r_g[J] : 0.0;
         h->rhfinv_g[J] = 1.0 / h->rhf_g[J];
     }
-    {
+    if (d.isym != 1) {  // the grid starts at the equator; K = 1 mirrors K = 2 (setup_frac.F90:452-456)
+        for (int K = 1; K <= numz; ++K) h->zhf_g[K] = ((double)K - 1.5) * g.dz;
+    } else {
         double offset = -((double)(numz / 2) * g.dz);
         for (int K = 1; K <= numz; ++K) h->zhf_g[K] = offset + ((double)K - 0.5) * g.dz;
     }
@@ -114,16 +120,21 @@ void host_geometry(const bh_grid_desc &d, HostGeo *h)
     for (int L = 1; L <= N; ++L) {
         int lstop = N / 2 + 1;
         int mode = (L <= lstop) ? L - 1 : L - lstop;
+        if (d.isym == 3) mode = mode * mode;  // potsetup.F90:158-163 squares the index for isym = 3
         double m1mode = (mode % 2 == 0) ? 1.0 : -1.0;
         double elm = std::cos((double)mode * g.dphi);
         double *b = &h->brb[(size_t)(L - 1) * (numr + 1)], *e = &h->elzb[(size_t)(L - 1) * (numr + 1)];
         const std::vector<double> &rhfinv = h->rhfinv_g;
         for (int J = 3; J <= numr - 1; ++J)
             b[J] = 2.0 * drinv2 - 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[J] * rhfinv[J];
-        b[2] = -alphar[2] + (m1mode - 1.0) * betar[2] - 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[2] * rhfinv[2];
+        // the axis row (potsetup.F90:211-223, 239-251): the ghost J = 1 is
+        // the cell rotated by pi, or for isym = 3 the cell itself
+        if (d.isym == 3) b[2] = -alphar[2] - 2.0 * betar[2] - 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[2] * rhfinv[2];
+        else b[2] = -alphar[2] + (m1mode - 1.0) * betar[2] - 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[2] * rhfinv[2];
         for (int J = 2; J <= numr - 1; ++J)
             e[J] = -2.0 * drinv2 + 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[J] * rhfinv[J];
-        e[2] = alphar[2] - (m1mode - 1.0) * betar[2] + 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[2] * rhfinv[2];
+        if (d.isym == 3) e[2] = alphar[2] + 2.0 * betar[2] + 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[2] * rhfinv[2];
+        else e[2] = alphar[2] - (m1mode - 1.0) * betar[2] + 2.0 * (elm - 1.0) * dphiinv2 * rhfinv[2] * rhfinv[2];
     }
     // pseudo-timesteps: helmadi.F90:124-129
     for (int w = 0; w < 2; ++w) {
@@ -193,17 +204,17 @@ int ops_precision(int precision) { return is_mixed(precision) ? BH_PREC_MIXED : 
 std::string tables_key(const bh_grid_desc &d, int precision, int device)
 {
     char b[160];
-    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d", device, d.numr, d.numz, d.dr, d.mmax,
-                  table32(precision) ? 32 : 64);
+    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d|%d", device, d.numr, d.numz, d.dr, d.mmax,
+                  table32(precision) ? 32 : 64, d.isym);
     return b;
 }
 
 std::string modeops_key(const bh_grid_desc &d, int precision, int device)
 {
     char b[200];
-    std::snprintf(b, sizeof b, "%d|%d|%d|%d|%.17g|%d|%d|%d|%d|%d", device, d.numr, d.numz, d.numphi, d.dr, d.mmax,
+    std::snprintf(b, sizeof b, "%d|%d|%d|%d|%.17g|%d|%d|%d|%d|%d|%d", device, d.numr, d.numz, d.numphi, d.dr, d.mmax,
                   d.nsteps_cold, d.nsteps_warm, ops_precision(precision),
-                  is_mixed(precision) && spectral_fused_ok(d.numr - 2, d.numz - 2) ? 1 : 0);
+                  is_mixed(precision) && spectral_fused_ok(d.numr - 2, d.numz - 2) ? 1 : 0, d.isym);
     return b;
 }
 
@@ -211,7 +222,7 @@ size_t tables_bytes(const bh_grid_desc &d, int precision)
 {
     const size_t nri = d.numr - 2, nzi = d.numz - 2, mm = d.mmax;
     return (table32(precision) ? 4 : 8) * topsym_table_elems(d.numr, d.numz, d.mmax) +
-           8 * (mm * (2 * nzi - 1) * nri + d.numr + 1);
+           8 * ((d.isym == 1 ? 1 : 2) * mm * (2 * nzi - 1) * nri + d.numr + 1);
 }
 
 size_t modeops_bytes(const bh_grid_desc &d, int precision)
@@ -242,14 +253,17 @@ int build_tables(const HostGeo &h, bool f32, GeoTables *t)
                   : t->mem.alloc(&t->T, topsym_table_elems(numr, numz, mmax))) != BH_OK)
         return rc;
     if ((rc = t->mem.alloc(&t->S, (size_t)mmax * (2 * nzi - 1) * nri)) != BH_OK) return rc;
+    if (h.g.isym != 1 && (rc = t->mem.alloc(&t->H, (size_t)mmax * (2 * nzi - 1) * nri)) != BH_OK) return rc;
     if ((rc = t->mem.alloc(&t->sqrt_rhf, numr + 1)) != BH_OK) return rc;
-    // Green's tables on the device (tm.F90:103-200, sm.F90:118-215)
+    // Green's tables on the device (tm.F90:103-363, sm.F90:118-382); for
+    // isym = 3 slot m holds the even mode 2m (tm.F90:105-133)
     const int nser = 111;  // hypr_upr_bnd + 1 (pot.h:7)
+    const bool even = h.g.isym == 3;
     std::vector<double> nu(mmax + 1, 0.0), coefh(mmax), cser((size_t)mmax * nser);
-    for (int m = 3; m <= mmax; ++m) nu[m] = (double)m - 2.5;
+    for (int m = 3; m <= mmax; ++m) nu[m] = even ? 2.0 * (double)m - 4.5 : (double)m - 2.5;
     double glh = lanczos_lngamma(0.5);
     for (int m = 1; m <= mmax; ++m) {
-        double mm = (double)m - 1.0;
+        double mm = even ? 2.0 * ((double)m - 1.0) : (double)m - 1.0;
         double aa = 0.5 * (mm + 1.5), bb = 0.5 * (mm + 0.5), cc = mm + 1.0;
         coefh[m - 1] = glh + lanczos_lngamma(mm + 0.5) - lanczos_lngamma(aa) - lanczos_lngamma(bb);
         for (int ir = 0; ir < nser; ++ir) {
@@ -272,13 +286,13 @@ int build_tables(const HostGeo &h, bool f32, GeoTables *t)
         (rc = upload(d_cser, cser, ss.s)) != BH_OK || (rc = upload(d_nu, nu, ss.s)) != BH_OK)
         return rc;
     TableArgs ta;
-    ta.numr = numr; ta.numz = numz; ta.mmax = mmax; ta.nser = nser;
+    ta.numr = numr; ta.numz = numz; ta.mmax = mmax; ta.nser = nser; ta.isym = h.g.isym;
     ta.hypr_cutoff = 1.025;  // pot.h:5
     ta.piinv = h.g.piinv; ta.dz = h.g.dz;
     ta.rhf_g = d_rhf; ta.rhfinv_g = d_rhfinv; ta.zhf_g = d_zhf;
     ta.coefh = d_coefh; ta.cser = d_cser; ta.nu = d_nu;
     launch_gen_topsym_table(ta, t->T, t->T32, ss.s);
-    launch_gen_side_table(ta, t->S, ss.s);
+    launch_gen_side_table(ta, t->S, t->H, ss.s);
     std::vector<double> sq(numr + 1, 0.0);
     for (int J = 1; J <= numr; ++J) sq[J] = std::sqrt(h.rhf_g[J]);
     if ((rc = upload(t->sqrt_rhf, sq, ss.s)) != BH_OK) return rc;
@@ -358,11 +372,11 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
         std::vector<double> bc((size_t)mmax * N), bs((size_t)mmax * N), Pf;
         for (int M1 = 1; M1 <= mmax; ++M1)
             for (int L = 1; L <= N; ++L) {
-                double a = 0.5 * g.dphi * ((double)M1 - 1.0) * (2.0 * (double)L - 1.0);
+                double a = (g.isym == 3 ? g.dphi : 0.5 * g.dphi) * ((double)M1 - 1.0) * (2.0 * (double)L - 1.0);
                 bc[(size_t)(M1 - 1) * N + (L - 1)] = std::cos(a);
                 bs[(size_t)(M1 - 1) * N + (L - 1)] = std::sin(a);
             }
-        proj_coefficients(N, mmax, g.dphi, &Pf);
+        proj_coefficients(N, mmax, g.dphi, &Pf, g.isym);
         if ((rc = up(o->Dc, std::move(Dc))) != BH_OK || (rc = up(o->Ds, std::move(Ds))) != BH_OK ||
             (rc = up(o->Ec, std::move(Ec))) != BH_OK || (rc = up(o->Es, std::move(Es))) != BH_OK ||
             (rc = up(o->Pf, std::move(Pf))) != BH_OK || (rc = up(o->besc, std::move(bc))) != BH_OK ||
@@ -420,6 +434,7 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
                 std::vector<double> bz(g.numz + 1), betz(g.numz + 1), gamz(g.numz + 2);
                 const double az = -g.gamma, cz = -g.gamma;
                 for (int K = 2; K <= g.numz - 1; ++K) bz[K] = 2.0 * g.gamma + dt[st];
+                if (g.isym != 1) bz[2] = g.gamma + dt[st];  // bzb(2) = gamma (potsetup.F90:228)
                 betz[2] = bz[2];
                 for (int K = 3; K <= g.numz - 1; ++K) {
                     gamz[K] = cz / betz[K - 1];

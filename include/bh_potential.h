@@ -47,7 +47,9 @@ typedef struct bh_grid_desc {
     int numr;          /* radial extent incl. axis ghost and outer boundary  */
     int numz;          /* vertical extent incl. bottom and top boundaries    */
     int numphi;        /* azimuthal zones, a power of two                    */
-    int isym;          /* 1 = no symmetry (the only value accepted)          */
+    int isym;          /* 1 = no symmetry; 2 = equatorial, 3 = equatorial + pi
+                          (the grid starts at the equator: K = 1 mirrors
+                          K = 2); 2 and 3 need BH_PREC_FP64; 0 -> 1        */
     double dr;         /* dr = dz; 0 -> 1/(130 - 3) (setup_frac.F90:374-376) */
     int mmax;          /* boundary azimuthal modes; 0 -> 30                  */
     int nsteps_cold;   /* ADI steps when tstep <= 1; 0 -> 20                 */

@@ -36,7 +36,9 @@ def test_library_exports_every_declared_function():
     (dict(numr=66, numz=66, numphi=48), "power of two"),
     (dict(numr=66, numz=65, numphi=64), "numz must be even"),
     (dict(numr=3, numz=66, numphi=64), ">= 5"),
-    (dict(numr=66, numz=66, numphi=64, isym=2), "isym"),
+    (dict(numr=66, numz=66, numphi=64, isym=4), "isym must be 1, 2 or 3"),
+    (dict(numr=66, numz=66, numphi=64, isym=2, precision=1), "need precision BH_PREC_FP64"),
+    (dict(numr=66, numz=66, numphi=64, isym=3, precision=2), "need precision BH_PREC_FP64"),
     (dict(numr=66, numz=66, numphi=64, mmax=-1), "mmax"),
     (dict(numr=600, numz=66, numphi=64), "<= 512"),
     (dict(numr=66, numz=66, numphi=64, passes=5), "refine_passes"),

@@ -9,7 +9,7 @@ import pytest
 
 from bipoly_hydro_amd import density
 from bipoly_hydro_amd import potential as P
-from golden_cases import c2_digests, c2_slices, load_case
+from golden_cases import SYM_CASES, c2_digests, c2_slices, load_case, load_sym_case
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +33,44 @@ def test_fp64_matches_reference_cold_then_warm(golden_dir, name):
     assert np.all(pot1[:, 0, -1] == 0.0) and np.all(pot1[:, -1, -1] == 0.0)
     h = numphi // 2
     np.testing.assert_array_equal(pot1[:h, :, 0], pot1[h:, :, 1])
+
+
+@pytest.mark.parametrize("name", list(SYM_CASES))
+def test_fp64_symmetric_grids_match_reference(golden_dir, name):
+    # isym = 2 (equatorial) and 3 (equatorial + pi): image Green's tables,
+    # no bottom face, the K = 2 sweep rows, axis and equator conditions
+    (numr, numz, numphi), isym, rho1, rho2, ref1, ref2 = load_sym_case(golden_dir, name)
+    with P.PotentialPlan(numr, numz, numphi, isym=isym) as plan:
+        pot1 = plan.potential_solver(rho1, tstep=0)
+        pot2 = plan.potential_solver(rho2, tstep=2)
+    e1, e2 = rell2(pot1, ref1), rell2(pot2, ref2)
+    print(f"{name}: cold relL2 {e1:.3e} warm relL2 {e2:.3e}", flush=True)
+    assert e1 <= TOL_FP64 and e2 <= TOL_FP64
+    np.testing.assert_array_equal(pot1[:, 0, :], pot1[:, 1, :])  # K = 1 mirrors K = 2
+    if isym == 3:
+        np.testing.assert_array_equal(pot1[:, 1:, 0], pot1[:, 1:, 1])  # the axis ghost mirrors J = 2
+
+
+# symmetric settings on grids the fixtures do not cover: numphi = 512 (the
+# panel DFT), nri = 288 > 256 (the unblocked top contraction), odd numz
+@pytest.mark.parametrize("shape,isym,seed", [((130, 66, 128), 2, 5), ((34, 18, 512), 2, 7), ((290, 18, 16), 3, 3),
+                                             ((66, 33, 64), 3, 11), ((66, 33, 64), 2, 13)])
+def test_fp64_symmetric_grids_match_oracle(shape, isym, seed):
+    from oracle import oracle as O
+    numr, numz, numphi = shape
+    rng = np.random.default_rng(seed)
+    rho1 = density.lane_emden_n1_binary(numr, numz, numphi, radius_cells=(0.2 * numr, 0.15 * numr),
+                                        centre_cells=(0.3 * numr, -0.3 * numr), noise=1e-3, seed=seed, isym=isym)
+    rho2 = rho1 * (1.0 + 0.01 * rng.standard_normal(rho1.shape))
+    oplan = O.OraclePlan(numr, numz, numphi, isym=isym)
+    o1 = oplan.solve(rho1, tstep=0)
+    o2 = oplan.solve(rho2, potp=o1, tstep=3)
+    with P.PotentialPlan(numr, numz, numphi, isym=isym) as plan:
+        g1 = plan.potential_solver(rho1, tstep=0)
+        g2 = plan.potential_solver(rho2, tstep=3)
+    e1, e2 = rell2(g1, o1), rell2(g2, o2)
+    print(f"{shape} isym={isym}: cold {e1:.3e} warm {e2:.3e}", flush=True)
+    assert e1 <= TOL_FP64 and e2 <= TOL_FP64
 
 
 # (290, 18, 16): nri = 288 > 256 takes the earlier top/bottom contraction
