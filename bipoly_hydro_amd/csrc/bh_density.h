@@ -18,6 +18,7 @@ struct DensityArgs {
     double rho_c[2], R[2], cx[2];  // per star: central density, radius, centre x (physical units)
     double noise;
     unsigned long long seed;
+    int isym;  // 2, 3: the grid starts at the equator, z_K = (K - 1.5) dz
 };
 Op density_op(const DensityArgs &a);
 

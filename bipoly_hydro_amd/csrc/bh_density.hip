@@ -3,8 +3,9 @@
 // index) on the solver's cylindrical grid, so that a self-consistent-field
 // sequence of solves keeps its density on the device.
 //
-// Grid (setup_frac.F90:428-463, isym = 1): r_J = (J - 1.5) dr, z_K =
-// (K - 0.5 - numz/2) dz, phi_L = (L - 1) 2 pi / numphi, dz = dr.  Each star
+// Grid (setup_frac.F90:407-463): r_J = (J - 1.5) dr, z_K = (K - 0.5 -
+// numz/2) dz (isym = 1) or (K - 1.5) dz (isym = 2, 3), phi_L = (L - 1) 2 pi /
+// numphi, dz = dr.  Each star
 // adds rho_c f(d / R) inside its radius, f tabulated on [0, 1] and
 // interpolated linearly, plus noise * rho_c * (2 h - 1), h a splitmix64
 // hash of (seed, star, point index) in [0, 1).
@@ -34,7 +35,7 @@ __device__ __forceinline__ void density_tile(const DensityArgs &a, int bx, int b
     const int L = by;
     const int J = (int)(t % numr) + 1, K = (int)(t / numr) + 1;
     const double r = ((double)J - 1.5) * a.dr;
-    const double z = ((double)K - 0.5 - (double)(numz / 2)) * a.dr;
+    const double z = a.isym == 1 ? ((double)K - 0.5 - (double)(numz / 2)) * a.dr : ((double)K - 1.5) * a.dr;
     const double phi = (double)L * a.dphi;
     const double x = r * cos(phi), y = r * sin(phi);
     const long idx = (long)L * plane + t;

@@ -194,6 +194,7 @@ DensityArgs model_args(const bh_binary_model &m, const Geo &g, int i, double *rh
     a.numphi = g.numphi;
     a.dr = g.dr;
     a.dphi = g.dphi;
+    a.isym = g.isym;
     const double sep = m.separation * (1.0 + m.drift * (double)i);
     const double m1 = m.rho_c[0] * m.radius[0] * m.radius[0] * m.radius[0];
     const double m2 = m.rho_c[1] * m.radius[1] * m.radius[1] * m.radius[1];
@@ -256,8 +257,8 @@ struct IterRec {
 std::string grid_key(const bh_grid_desc &d)
 {
     char b[200];
-    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d|%d|%d", d.numr, d.numz, d.numphi, d.dr, d.mmax, d.nsteps_cold,
-                  d.nsteps_warm, d.refine_passes);
+    std::snprintf(b, sizeof b, "%d|%d|%d|%.17g|%d|%d|%d|%d|%d", d.numr, d.numz, d.numphi, d.dr, d.mmax, d.nsteps_cold,
+                  d.nsteps_warm, d.refine_passes, d.isym);
     return b;
 }
 

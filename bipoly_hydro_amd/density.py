@@ -134,7 +134,7 @@ def _splitmix_hash_uniform(x: np.ndarray) -> np.ndarray:
 
 
 def bipolytrope_binary(numr: int, numz: int, numphi: int, model: dict, solve_index: int, profile: np.ndarray,
-                       dr: float = 1.0 / 127.0) -> np.ndarray:
+                       dr: float = 1.0 / 127.0, isym: int = 1) -> np.ndarray:
     """Host statement of the served workload's density (the device generator
     of bh_density.hip): two stars with rho_c f(d/R), f the tabulated
     bipolytrope profile interpolated linearly, centred about the centre of
@@ -149,7 +149,7 @@ def bipolytrope_binary(numr: int, numz: int, numphi: int, model: dict, solve_ind
     J = np.arange(1, numr + 1, dtype=np.float64)
     K = np.arange(1, numz + 1, dtype=np.float64)
     r = (J - 1.5) * dr
-    z = (K - 0.5 - float(numz // 2)) * dr
+    z = (K - 0.5 - float(numz // 2)) * dr if isym == 1 else (K - 1.5) * dr
     phi = np.arange(numphi, dtype=np.float64) * (2.0 * math.pi * (1.0 / numphi))
     x = r[None, None, :] * np.cos(phi)[:, None, None]
     y = r[None, None, :] * np.sin(phi)[:, None, None]

@@ -306,7 +306,7 @@ class PotentialPlan:
 class PotentialBatch:
     """Independent grids of different sizes solved together (one plan per
     grid; step k of every grid's solve runs as one grouped launch per kernel
-    kind).  ``graph`` replays the launch sequence from a captured hipGraph;
+    kind).  A grid is (numr, numz, numphi) or (numr, numz, numphi, isym).  ``graph`` replays the launch sequence from a captured hipGraph;
     ``autotune`` times tile and pencil candidates over the first five solves
     and keeps the fastest per step."""
 
@@ -314,10 +314,12 @@ class PotentialBatch:
                  autotune: bool = False, dr: float = 1.0 / 127.0, mmax: int = 30,
                  nsteps_cold: int = 20, nsteps_warm: int = 5, refine_passes: int = 2):
         L = load_library()
-        self.shapes = [(p, z, r) for (r, z, p) in grids]
+        # (numr, numz, numphi) or (numr, numz, numphi, isym)
+        grids = [tuple(g) + (1,) * (4 - len(g)) for g in grids]
+        self.shapes = [(p, z, r) for (r, z, p, _) in grids]
         n = len(grids)
-        descs = (GridDesc * n)(*[GridDesc(r, z, p, 1, dr, mmax, nsteps_cold, nsteps_warm, refine_passes)
-                                 for (r, z, p) in grids])
+        descs = (GridDesc * n)(*[GridDesc(r, z, p, sym, dr, mmax, nsteps_cold, nsteps_warm, refine_passes)
+                                 for (r, z, p, sym) in grids])
         flags = (BH_BATCH_GRAPH if graph else 0) | (BH_BATCH_AUTOTUNE if autotune else 0)
         h = ctypes.c_void_p()
         _check(L.bh_batch_create(descs, n, _PREC[precision], hbm_budget, flags, ctypes.byref(h)))
@@ -425,10 +427,10 @@ class PotentialServer:
     def submit(self, grid, nsolves: int, model: dict, *, want_potential: bool = False,
                want_latencies: bool = True, dr: float = 1.0 / 127.0, refine_passes: int = 2,
                start_pot=None) -> int:
-        """grid = (numr, numz, numphi); model keys as bh_binary_model;
+        """grid = (numr, numz, numphi) or (numr, numz, numphi, isym); model keys as bh_binary_model;
         start_pot: a (numphi, numz, numr) potential to resume from (e.g. from
         cont_read), the first solve then warm."""
-        numr, numz, numphi = grid
+        numr, numz, numphi, isym = tuple(grid) + (1,) * (4 - len(grid))
         m = BinaryModel()
         m.rho_c[:] = list(model.get("rho_c", (0.0, 0.0)))
         m.radius[:] = list(model["radius"])
@@ -444,7 +446,7 @@ class PotentialServer:
             sp = np.ascontiguousarray(start_pot, dtype=np.float64)
             if sp.shape != (numphi, numz, numr):
                 raise ValueError(f"start_pot shape {sp.shape} != {(numphi, numz, numr)}")
-        r = Request(GridDesc(numr, numz, numphi, 1, dr, 30, 20, 5, refine_passes), nsolves, m,
+        r = Request(GridDesc(numr, numz, numphi, isym, dr, 30, 20, 5, refine_passes), nsolves, m,
                     pot.ctypes.data if pot is not None else None, lat.ctypes.data if lat is not None else None,
                     sp.ctypes.data if sp is not None else None)
         rid = ctypes.c_longlong()

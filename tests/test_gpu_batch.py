@@ -45,6 +45,26 @@ def test_batch_equals_single_plans_bitwise(precision):
         assert np.array_equal(got2[i], ref[i][1]), f"grid {GRIDS[i]} second solve"
 
 
+def test_fp64_batch_of_mixed_symmetries_equals_single_plans_bitwise():
+    # isym 1, 2 and 3 grids in one batch: the symmetric grids' kernels
+    # (side contraction with the image table, equator rows) group beside the
+    # plain ones
+    grids = [(34, 18, 16, 2), (42, 26, 16, 1), (66, 33, 32, 3), (50, 22, 64, 2)]
+    rhos = [[density.lane_emden_n1_binary(r, z, p, radius_cells=(0.2 * r, 0.15 * r), centre_cells=(0.3 * r, -0.3 * r),
+                                          noise=1e-3, seed=7 + i, isym=sym) * sc
+             for i, (r, z, p, sym) in enumerate(grids)] for sc in (1.0, 1.01)]
+    ts = [[0, 0, 0, 0], [2, 0, 2, 2]]
+    ref = []
+    for i, (r, z, p, sym) in enumerate(grids):
+        with P.PotentialPlan(r, z, p, precision="fp64", isym=sym) as plan:
+            ref.append([plan.potential_solver(rhos[k][i], tstep=ts[k][i]) for k in range(2)])
+    with P.PotentialBatch(grids, precision="fp64") as b:
+        got = [b.solve(rhos[k], ts[k]) for k in range(2)]
+    for i in range(len(grids)):
+        for k in range(2):
+            assert np.array_equal(got[k][i], ref[i][k]), f"grid {grids[i]} solve {k}"
+
+
 def test_batch_graph_replays_match_eager():
     import torch
     dev = torch.device("cuda:0")

@@ -16,11 +16,11 @@ pytestmark = pytest.mark.gpu
 MODEL_A = dict(radius=(8.0, 6.0), separation=20.0, drift=-0.01, noise=1e-4, seed=5)
 
 
-def host_density(grid, model, i, prof):
+def host_density(grid, model, i, prof, isym=1):
     r, z, p = grid
     m = dict(model)
     m.setdefault("rho_c", (1.0, 1.0))
-    return density.bipolytrope_binary(r, z, p, m, i, prof)
+    return density.bipolytrope_binary(r, z, p, m, i, prof, isym=isym)
 
 
 def test_device_density_matches_host_statement():
@@ -83,6 +83,26 @@ def test_served_sequence_matches_single_plan_solves():
         for i in range(n):
             ref = plan.potential_solver(host_density(grid, model, i, prof), tstep=0 if i == 0 else 2)
     assert np.linalg.norm(pot - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+@pytest.mark.parametrize("isym", [2, 3])
+def test_served_symmetric_grid_matches_single_plan_solves(isym):
+    # an isym = 2 or 3 request beside an isym = 1 one of the same shape:
+    # separate plan pools, the device density on the half-space grid
+    grid, n = (42, 26, 16), 3
+    model = dict(radius=(5.0, 4.0), separation=14.0, drift=0.01, noise=1e-5, seed=9)
+    with P.PotentialServer(precision="fp64") as srv:
+        r1 = srv.submit(grid, n, model, want_potential=True)
+        rs = srv.submit(grid + (isym,), n, model, want_potential=True)
+        res1, pot1, _ = srv.wait(r1)
+        res, pot, _ = srv.wait(rs)
+    assert res["status"] == 0 and res1["status"] == 0
+    prof, _, _ = P.bipolytrope_profile()
+    with P.PotentialPlan(*grid, precision="fp64", isym=isym) as plan:
+        for i in range(n):
+            ref = plan.potential_solver(host_density(grid, model, i, prof, isym), tstep=0 if i == 0 else 2)
+    assert np.linalg.norm(pot - ref) <= 1e-12 * np.linalg.norm(ref)
+    assert np.linalg.norm(pot1 - ref) > 1e-3 * np.linalg.norm(ref)
 
 
 def test_admission_by_device_memory_budget():
