@@ -1304,8 +1304,22 @@ BH_KERNEL_PAIR(contract_side_img, ContractArgs, 256)
 BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
-BH_KERNEL_PAIR(contract_tb64, ContractArgs, 256)
+// two workgroups per CU (the LDS of the register-blocked contraction
+// allows two): the register budget is held to 256 per lane
+#define BH_KERNEL_PAIR2(NAME, ARGS_T, NT)                                                         \
+    __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void NAME##_kernel(   \
+        ARGS_T a, int nx, int ny, int total)                                                      \
+    {                                                                                             \
+        BH_SINGLE_LOOP(a, nx, ny, total, (NAME##_tile(a, bx, by, bz)));                           \
+    }                                                                                             \
+    __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void NAME##_grouped(  \
+        const ARGS_T *__restrict__ args, GroupSched gs)                                           \
+    {                                                                                             \
+        BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
+    }
+BH_KERNEL_PAIR2(contract_tb64, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_tb32, ContractArgs, 256)
+#undef BH_KERNEL_PAIR2
 BH_KERNEL_PAIR(topsym_reduce, ContractArgs, 256)
 BH_KERNEL_PAIR(face_modes, RebuildArgs, 128)
 #undef BH_KERNEL_PAIR

@@ -315,7 +315,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         // 4-5 in one kernel per mode, the block kept in LDS: delta1 into D1,
         // w into Wf
         SpecArgs sa{p->r32, o.sfb, o.sfa, o.Mr[w], o.mu32[w], o.tri_ibet, o.betar, o.alphar, p->D1, p->Wf,
-                    numr, numz, N, 0};
+                    numr, numz, N, 0, o.sfb16, o.sfa16};
         out->push_back(spectral_op(sa, N, nzi));
         // 6. refinement passes.  The iterate is never written: pass k forms
         //    x_k = X0 + (delta1 + Dacc) in its residual (X0's faces alone for
@@ -330,7 +330,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
             ra.r = p->r32; ra.cold = cold;
             out->push_back(residual_op(ra, N));
             SpecArgs sr{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, Dacc, nullptr,
-                        numr, numz, N, it > 0 ? 1 : 0};
+                        numr, numz, N, it > 0 ? 1 : 0, o.sfb16, o.sfa16, p->passes == 1 ? 1 : 0};
             out->push_back(spectral_op(sr, N, nzi));
         }
         // 7. x = X0 + (delta1 + Dacc) into Xa, the warm-start state, then the

@@ -91,6 +91,7 @@ struct ModeOps {
     // fused per-mode spectral step (then Mr holds split bf16 fragments)
     bool spec_fused = false;
     float *sfb = nullptr, *sfa = nullptr;
+    float *sfb16 = nullptr, *sfa16 = nullptr;  // nzi >= 64: split-bf16 fragments (same sizes)
 };
 
 // Shared resources from the per-process cache (created on first use; kept

@@ -237,7 +237,8 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
     if (is_mixed(precision)) {
         b += 4 * (nh * nh + ns * ns + (nzi + 1) / 2 * nzi + 2 * nzi + 2 * N * nri * nri + 2 * N * nri * nzi);
         if (spectral_fused_ok((int)nri, (int)nzi))  // tables; no g; M_r and pivots per mode number
-            b += 4 * 2 * (nzi / 2) * nzi - 4 * N * nri * nzi - 4 * (N / 2 - 1) * (2 * nri * nri + nri * nzi);
+            b += 4 * (nzi >= 64 ? 4 : 2) * (nzi / 2) * nzi - 4 * N * nri * nzi -
+                 4 * (N / 2 - 1) * (2 * nri * nri + nri * nzi);
         b += 8 * N * (numr + 1);
     } else {
         b += 8 * 3 * ((size_t)d.nsteps_cold + d.nsteps_warm) * (N * nri + nzi);
@@ -476,13 +477,20 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
     if (o->spec_fused) {
         ALLOC(o->sfb, (size_t)(nzi / 2) * nzi);
         ALLOC(o->sfa, (size_t)(nzi / 2) * nzi);
+        if (nzi >= 64) {
+            ALLOC(o->sfb16, (size_t)(nzi / 2) * nzi);
+            ALLOC(o->sfa16, (size_t)(nzi / 2) * nzi);
+        }
     }
 #undef ALLOC
     if ((rc = up(o->brb_d, std::vector<double>(h.brb))) != BH_OK) return rc;
     if (o->spec_fused) {
         std::vector<float> fb, fa;
-        spectral_fragments(nzi, &fb, &fa);
+        std::vector<float> fb16, fa16;
+        spectral_fragments(nzi, &fb, &fa, &fb16, &fa16);
         if ((rc = upf(o->sfb, std::move(fb))) != BH_OK || (rc = upf(o->sfa, std::move(fa))) != BH_OK) return rc;
+        if (o->sfb16 && ((rc = upf(o->sfb16, std::move(fb16))) != BH_OK || (rc = upf(o->sfa16, std::move(fa16))) != BH_OK))
+            return rc;
     }
     {
         std::vector<float> Dc32((size_t)nh * nh), Ds32((size_t)ns_ * ns_), S((size_t)((nzi + 1) / 2) * nzi);

@@ -28,6 +28,10 @@ struct SpecArgs {
     float *dlt, *w;
     int numr, numz, nphi;         // mode number of plane L: L, or L - nphi/2 past nphi/2
     int acc;                      // without mr: add delta to dlt's interior instead of storing it
+    const float *sfb16, *sfa16;   // nzi >= 64: the same tables split into bf16 hi, lo fragments
+    // a correction (no mr) with FP32 transforms instead of split bf16: the
+    // solve's only refinement pass, whose error no later pass contracts
+    int fp32t;
 };
 
 // Sizes the fused kernel takes: nri a multiple of 64 up to 256 and nzi one
@@ -35,7 +39,9 @@ struct SpecArgs {
 bool spectral_fused_ok(int nri, int nzi);
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi);
 // The folded sine table (ceil(n/2) x n, spectral slot columns) in the two
-// fragment orders the kernel reads.
-void spectral_fragments(int nzi, std::vector<float> *sfb, std::vector<float> *sfa);
+// fragment orders the kernel reads, in FP32 and (nzi >= 64; else left
+// empty) split into bf16 hi and lo parts.
+void spectral_fragments(int nzi, std::vector<float> *sfb, std::vector<float> *sfa, std::vector<float> *sfb16,
+                        std::vector<float> *sfa16);
 
 }  // namespace bh

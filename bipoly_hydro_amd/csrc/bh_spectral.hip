@@ -11,10 +11,15 @@
 // in the forward transform and the M_r product, and columns j of the same
 // range in the inverse transforms.  The LDS block holds [j][m'] with a row
 // pitch of nzi + 4 floats (16-byte aligned rows for the b128 reads of the
-// inverse).  MFMA v_mfma_f32_16x16x4f32, operand layouts A[l%16][l/16],
-// B[l/16][l%16], result element v of lane l at D[4(l/16) + v][l%16]; the
-// fragment tables and M_r are stored so that lane l reads its four k values
-// 4(l/16) + g of a 16-wide k group as one 16-byte load.
+// inverse).  The sine transforms of nzi >= 64 and the M_r product are
+// split-bf16 products on v_mfma_f32_16x16x32_bf16 (x = hi + lo, three
+// products hi.hi + hi.lo + lo.hi summed in FP32, about 2^-17 relative):
+// lane l holds row (or column) l % 16 and the eight k values 8 (l / 16) + e,
+// result element v of lane l at D[4(l/16) + v][l%16].  The tables are stored
+// split, as [tile][32-wide k group][hi, lo][lane][8]; the data operand is
+// split as it is read.  nzi = 32 keeps FP32 MFMA (v_mfma_f32_16x16x4f32, A
+// lane l: row l%16, k l/16; four k values 4(l/16) + g of a 16-wide k group
+// as one 16-byte load).
 #include "bh_mixed.h"
 #include "bh_spectral.h"
 
@@ -51,7 +56,24 @@ struct Shape {
     static constexpr int KG = HE / 16;    // 16-wide k groups of one half
     static constexpr int ZT = HE / 16;    // 16-row z tiles of the folded inverse
     static constexpr int P = NZ + 4;      // LDS row pitch (floats)
+    static constexpr bool BF = NZ >= 64;  // split-bf16 sine transforms possible
+    static constexpr int KB = HE / 32;    // 32-wide k groups of one half (BF)
 };
+
+// element e of h and l: x = hi + lo in bf16, each rounded to nearest
+__device__ __forceinline__ void split_bf16(float x, bf8 &h, bf8 &l, int e)
+{
+    const __bf16 hi = (__bf16)x;
+    h[e] = hi;
+    l[e] = (__bf16)(x - (float)hi);
+}
+// the three products of a split-bf16 multiply-add, small terms first
+__device__ __forceinline__ f4 mma3(const bf8 &ah, const bf8 &al, const bf8 &bh, const bf8 &bl, f4 c)
+{
+    c = mma_bf16(al, bh, c);
+    c = mma_bf16(ah, bl, c);
+    return mma_bf16(ah, bh, c);
+}
 
 // Global-memory views of the argument pointers: loads through them are
 // vector-memory loads with their own wait counter (generic pointers become
@@ -71,7 +93,8 @@ __device__ __forceinline__ BH_GLOBAL T *gst(void *p)
 // Tile t takes plane t/2 + (t%2) N/2: planes m and m + N/2 (the cosine and
 // sine rows of azimuthal mode m) run side by side on one XCD, and share M_r
 // and the pivots, stored per mode number mi = L or L - N/2.
-template <int NZ, bool MR, bool ACC>
+// BFT: split-bf16 sine transforms for delta (and, without MR, the forward one)
+template <int NZ, bool MR, bool ACC, bool BFT>
 __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int tile)
 {
     typedef Shape<NZ> S;
@@ -106,6 +129,57 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     //         per wave.  The r planes of k group kg + 1 load while group kg is
     //         multiplied.
     const BH_GLOBAL float *r = gld<float>(a.r) + (long)L * NZ * nri + j0 + 2 * t;
+    f4 acc[2][S::CT];
+    // The first pass (MR) keeps its forward transform in FP32: it feeds w,
+    // whose error no refinement pass removes; a correction's transforms are
+    // split-bf16, their error contracted by the next pass (or, in the last,
+    // below 1e-5 of a correction of about 1e-7 of x)
+    if constexpr (BFT && !MR) {
+        // every plane this lane needs, loaded at once: k = 32 kb + 8 q + e
+        f2 xr[S::KB][8], yr[S::KB][8];
+        if (active) {
+#pragma unroll
+            for (int kb = 0; kb < S::KB; ++kb)
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int pa = 32 * kb + 8 * q + e;
+                    xr[kb][e] = *(const BH_GLOBAL f2 *)(r + (long)pa * nri);
+                    yr[kb][e] = *(const BH_GLOBAL f2 *)(r + (long)(NZ - 1 - pa) * nri);
+                }
+        }
+        {
+            const BH_GLOBAL f4 *sbg = gld<f4>(a.sfb16);
+            f4 *sbl = (f4 *)buf;
+            for (int i = tid; i < S::HE * NZ / 4; i += SPEC_THREADS) sbl[i] = sbg[i];
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int c = 0; c < S::CT; ++c) acc[b][c] = (f4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < S::KB; ++kb) {
+                bf8 uh[2], ul[2], vh[2], vl[2];
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b) {
+                        split_bf16(xr[kb][e][b] + yr[kb][e][b], uh[b], ul[b], e);
+                        split_bf16(xr[kb][e][b] - yr[kb][e][b], vh[b], vl[b], e);
+                    }
+                const bf8 *sb = (const bf8 *)buf + lane;
+#pragma unroll
+                for (int c = 0; c < S::CT; ++c) {
+                    const bf8 bh = sb[((c * S::KB + kb) * 2) * 64], bl = sb[((c * S::KB + kb) * 2 + 1) * 64];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[b][c] = c < S::CT / 2 ? mma3(uh[b], ul[b], bh, bl, acc[b][c])
+                                                  : mma3(vh[b], vl[b], bh, bl, acc[b][c]);
+                }
+            }
+        }
+    } else {
     f2 xr[2][4], yr[2][4];
     auto load = [&](int kg, int slot) {
 #pragma unroll
@@ -122,7 +196,6 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
         for (int i = tid; i < S::HE * NZ / 4; i += SPEC_THREADS) sbl[i] = sbg[i];
     }
     __syncthreads();
-    f4 acc[2][S::CT];
     if (active) {
         const f4 *sb = (const f4 *)buf + lane;
 #pragma unroll
@@ -151,6 +224,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                     for (int b = 0; b < 2; ++b) acc[b][c] = mma(c < S::CT / 2 ? u[g][b] : v[g][b], bc[g], acc[b][c]);
             }
         }
+    }
     }
     __syncthreads();  // every wave is done with the table
     if (active) {
@@ -328,8 +402,9 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     //         over the even spectral slots of the LDS block and O the odd.
     //         Wave w takes z tile w % ZT and 32 ZT columns j from (w / ZT) 32 ZT,
     //         so that it reads 1/ZT of the table (the block comes from LDS).
-    auto inverse = [&](float *outp, auto acc_c) {
+    auto inverse = [&](float *outp, auto acc_c, auto bf_c) {
         constexpr bool ACCUM = decltype(acc_c)::value;
+        constexpr bool BFI = decltype(bf_c)::value;
         constexpr int JW = 32 * S::ZT, JT = JW / 16;
         const int zt = wave % S::ZT, jb = (wave / S::ZT) * JW;
         if (jb >= nri) return;
@@ -349,6 +424,38 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                 for (int g = 0; g < 4; ++g) acc[i] = mma(af[g], bf[g], acc[i]);
             }
         };
+        if constexpr (BFI) {
+            // k groups [0, CB/2) of 32 slots are the even slots, the rest odd
+            constexpr int CB = NZ / 32;
+            const BH_GLOBAL bf8 *sa = gld<bf8>(a.sfa16) + (long)zt * CB * 2 * 64 + lane;
+            auto kstep2 = [&](const bf8 &ah, const bf8 &al, int kb, f4(&acc)[JT]) {
+#pragma unroll
+                for (int i = 0; i < JT; ++i) {
+                    const float *bp = buf + (jb + 16 * i + t) * P + 32 * kb + 8 * q;
+                    const f4 b0 = *(const f4 *)bp, b1 = *(const f4 *)(bp + 4);
+                    bf8 bh, bl;
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        split_bf16(b0[e], bh, bl, e);
+                        split_bf16(b1[e], bh, bl, 4 + e);
+                    }
+                    acc[i] = mma3(ah, al, bh, bl, acc[i]);
+                }
+            };
+            bf8 ahc = sa[0], alc = sa[64], ahn = ahc, aln = alc;
+#pragma unroll 1
+            for (int kb = 0; kb < CB; ++kb) {
+                if (kb + 1 < CB) {
+                    ahn = sa[(kb + 1) * 2 * 64];
+                    aln = sa[((kb + 1) * 2 + 1) * 64];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                if (kb < CB / 2) kstep2(ahc, alc, kb, ev);
+                else kstep2(ahc, alc, kb, od);
+                ahc = ahn;
+                alc = aln;
+            }
+        } else {
         const BH_GLOBAL f4 *sa = gld<f4>(a.sfa) + (long)zt * S::CT * 64 + lane;
         f4 afc = sa[0], afn = afc;
 #pragma unroll 1
@@ -358,6 +465,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
             if (kg < S::CT / 2) kstep(afc, kg, ev);
             else kstep(afc, kg, od);
             afc = afn;
+        }
         }
         BH_GLOBAL float *pl = gst<float>(outp) + (long)L * numr * a.numz + numr + 1 + jb + t;
         if constexpr (ACCUM) {
@@ -398,7 +506,8 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
             }
         }
     };
-    inverse(a.dlt, std::integral_constant<bool, ACC>{});
+    // delta (split-bf16: refined later), w (FP32: never refined)
+    inverse(a.dlt, std::integral_constant<bool, ACC>{}, std::integral_constant<bool, BFT>{});
     if (MR) {
         __syncthreads();  // every wave has read D
         if (fil) {
@@ -411,41 +520,46 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
                         buf[(64 * rg + 16 * i + 4 * q + e) * P + (NZ / 2) * cg + 16 * c + t] = wacc[i][c][e];
         }
         __syncthreads();
-        inverse(a.w, std::false_type{});
+        inverse(a.w, std::false_type{}, std::false_type{});
     }
     __syncthreads();  // the block is free for the next mode
 }
 
-template <int NZ, bool MR, bool ACC>
+template <int NZ, bool MR, bool ACC, bool BFT>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_kernel(SpecArgs a, int nx, int ny, int total)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR, ACC>(a, spec_lds, bx)));
+    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR, ACC, BFT>(a, spec_lds, bx)));
 }
 
-template <int NZ, bool MR, bool ACC>
+template <int NZ, bool MR, bool ACC, bool BFT>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_grouped(const SpecArgs *__restrict__ args, GroupSched gs)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR, ACC>(ga, spec_lds, bx)));
+    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR, ACC, BFT>(ga, spec_lds, bx)));
 }
 
 // variant v: nzi = 32 << (v % 3); v / 3 = 0 plain, 1 with the M_r filter,
-// 2 accumulating delta into dlt
+// 2 accumulating delta into dlt, 3 plain with FP32 transforms (the split
+// bf16 ones wherever nzi >= 64 otherwise)
 const void *spec_fn(int variant, bool grouped)
 {
-#define BH_SPEC_FN(NZ, MR, ACC) \
-    (grouped ? (const void *)&spectral_grouped<NZ, MR, ACC> : (const void *)&spectral_kernel<NZ, MR, ACC>)
+#define BH_SPEC_FN(NZ, MR, ACC, BFT)                                                               \
+    (grouped ? (const void *)&spectral_grouped<NZ, MR, ACC, BFT>                                  \
+             : (const void *)&spectral_kernel<NZ, MR, ACC, BFT>)
     switch (variant) {
-    case 0: return BH_SPEC_FN(32, false, false);
-    case 1: return BH_SPEC_FN(64, false, false);
-    case 2: return BH_SPEC_FN(128, false, false);
-    case 3: return BH_SPEC_FN(32, true, false);
-    case 4: return BH_SPEC_FN(64, true, false);
-    case 5: return BH_SPEC_FN(128, true, false);
-    case 6: return BH_SPEC_FN(32, false, true);
-    case 7: return BH_SPEC_FN(64, false, true);
-    case 8: return BH_SPEC_FN(128, false, true);
+    case 0: return BH_SPEC_FN(32, false, false, false);
+    case 1: return BH_SPEC_FN(64, false, false, true);
+    case 2: return BH_SPEC_FN(128, false, false, true);
+    case 3: return BH_SPEC_FN(32, true, false, false);
+    case 4: return BH_SPEC_FN(64, true, false, true);
+    case 5: return BH_SPEC_FN(128, true, false, true);
+    case 6: return BH_SPEC_FN(32, false, true, false);
+    case 7: return BH_SPEC_FN(64, false, true, true);
+    case 8: return BH_SPEC_FN(128, false, true, true);
+    case 9: return BH_SPEC_FN(32, false, false, false);
+    case 10: return BH_SPEC_FN(64, false, false, false);
+    case 11: return BH_SPEC_FN(128, false, false, false);
     default: return nullptr;
     }
 #undef BH_SPEC_FN
@@ -471,12 +585,61 @@ bool spectral_fused_ok(int nri, int nzi)
 
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi)
 {
-    return Op::make(OP_SPECTRAL, spec_variant(nzi) + (a.mr ? 3 : a.acc ? 6 : 0), a, nplanes, 1, 1);
+    return Op::make(OP_SPECTRAL, spec_variant(nzi) + (a.mr ? 3 : a.acc ? 6 : a.fp32t ? 9 : 0), a, nplanes, 1, 1);
 }
 
-void spectral_fragments(int n, std::vector<float> *sfb, std::vector<float> *sfa)
+namespace {
+// bf16 bits of x, rounded to nearest even (finite x)
+unsigned short bf16_bits(float x)
+{
+    unsigned u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+float bf16_value(unsigned short h)
+{
+    const unsigned u = (unsigned)h << 16;
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+}
+// fragment tables of nzi >= 64: [tile][32-wide k group][hi, lo][lane][8]
+// bf16, packed two to a float slot (the same bytes as the FP32 tables);
+// value(tile, k, lane) = S entry of lane l's element e, k = 8 (l / 16) + e
+template <class F>
+void split_fragments(int ntile, int nkb, F value, std::vector<float> *out)
+{
+    std::vector<unsigned short> h((size_t)ntile * nkb * 2 * 64 * 8);
+    for (int tl = 0; tl < ntile; ++tl)
+        for (int kb = 0; kb < nkb; ++kb)
+            for (int l = 0; l < 64; ++l)
+                for (int e = 0; e < 8; ++e) {
+                    const double x = value(tl, 32 * kb + 8 * (l >> 4) + e, l & 15);
+                    const unsigned short hi = bf16_bits((float)x);
+                    const unsigned short lo = bf16_bits((float)(x - (double)bf16_value(hi)));
+                    const size_t base = (((size_t)tl * nkb + kb) * 2) * 512 + (size_t)l * 8 + e;
+                    h[base] = hi;
+                    h[base + 512] = lo;
+                }
+    out->assign(h.size() / 2, 0.f);
+    std::memcpy(out->data(), h.data(), h.size() * sizeof(unsigned short));
+}
+}  // namespace
+
+void spectral_fragments(int n, std::vector<float> *sfb, std::vector<float> *sfa, std::vector<float> *sfb16,
+                        std::vector<float> *sfa16)
 {
     const int he = n / 2, ct = n / 16, kg = he / 16, zt = he / 16;
+    sfb16->clear();
+    sfa16->clear();
+    if (n >= 64) {
+        auto sd = [&](int a, int m) { return sine_basis(a, spec_mode(m, n), n); };
+        // forward B: tile = 16-column spectral tile c, k = plane a (< n/2)
+        split_fragments(ct, he / 32, [&](int c, int k, int col) { return sd(k, 16 * c + col); }, sfb16);
+        // inverse A: tile = 16-row z tile, k = spectral slot
+        split_fragments(zt, n / 32, [&](int i, int k, int row) { return sd(16 * i + row, k); }, sfa16);
+    }
     auto sf = [&](int a, int m) { return (float)sine_basis(a, spec_mode(m, n), n); };
     // B order for the forward: [c][k group][lane][4], k = plane a, n = slot m'
     sfb->assign((size_t)ct * kg * 256, 0.f);
