@@ -122,14 +122,16 @@ inline size_t topsym_table_elems(int numr, int numz, int mmax)
 void launch_gen_side_table(const TableArgs &a, double *S, double *H, hipStream_t s);
 
 // boundary contraction and face rebuild (bessel.F90:176-296, 396-488);
-// the side contraction writes CT_SPLIT partial sums that the face kernel adds
-constexpr int CT_SPLIT = 4;
+// the side contraction writes one partial sum per split of SD_PLANES source
+// planes, which the face kernel adds in order
+constexpr int SD_PLANES = 16;
+__host__ __device__ inline int side_splits(int numz) { return (numz - 2 + SD_PLANES - 1) / SD_PLANES; }
 // the symmetric top/bottom contraction takes 16 source planes per workgroup
 constexpr int TS_PLANES = 16;
 __host__ __device__ inline int topsym_splits(int numz) { return (numz - 2 + TS_PLANES - 1) / TS_PLANES; }
 // proj: [2*mmax-1][numz*numr] projections (row 0 = m0, 1..mmax-1 cos, mmax.. sin)
 // Stb: [mmax][4][nri] (top cos, top sin, bottom cos, bottom sin)
-// Ss:  [CT_SPLIT][mmax][2][nzi]
+// Ss:  [side_splits][mmax][2][nzi]
 // proj from rho: the azimuthal mode sums of every (J, K) column, the planes
 // folded in quads (FP64 matrix cores); coef from proj_coefficients
 constexpr int PROJ_MODES = 32;  // modes per chunk (grid x)

@@ -1063,17 +1063,23 @@ Op topsym_reduce_op(const double *Ptb, const double *sqrt_rhf, double *Stb, int 
     return Op::make(OP_TOPSYM_REDUCE, 0, a, (4 * (numr - 2) + 255) / 256, mmax, 1);
 }
 
-// Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr].
-// A workgroup takes SD_TGT targets ko, one mode and the kz range of one
-// split; thread jr sums over its source column, SD_TGT source planes at a
-// time: the table values of the SD_TGT x SD_TGT (ko, kz) tile lie on 2
-// SD_TGT - 1 diagonals, so one window of table loads serves all of its
-// products (8 multiply-adds per load).  The workgroup then adds its
-// columns in a fixed order (wavefront butterflies, then the 4 wavefronts).
-constexpr int SD_TGT = 16;
+// Side: Ss[split][m][q][ko], q = cos, sin; Toeplitz table S[m][ko - kz + nzi - 1][jr]
+// (isym = 2, 3: plus the image table H[m][ko + kz][jr]).  A workgroup takes
+// one mode m and the SD_PLANES source planes of one split, for every
+// target ko.  Thread jr reads its column's projections of those planes
+// once, into registers; the targets follow in tiles of SD_TGT, and the
+// table values of a tile (its SD_TGT x SD_PLANES (ko, kz) pairs lie on
+// SD_TGT + SD_PLANES - 1 diagonals) sit in a register window that slides
+// by SD_TGT from tile to tile, so each table value of the split's range of
+// diagonals is loaded once per column.  After each tile the workgroup adds
+// its columns in a fixed order (wavefront butterflies, then the 4
+// wavefronts) into LDS totals, column chunks of 256 in order, and the
+// totals are written at the end: the sums are reproducible bit for bit.
+// SD_GROUPS > 1 would split the targets between that many workgroups (grid
+// z), reading the projections once per group; one group measured fastest.
+constexpr int SD_TGT = 16, SD_GROUPS = 1;
+static_assert(SD_PLANES == SD_TGT, "the window holds 2 SD_TGT - 1 diagonals");
 
-// IMG (isym = 2, 3): the image part H[m][ko + kz][jr] is added, from a
-// second window of 2 SD_TGT - 1 values along its anti-diagonals.
 template <bool IMG>
 __device__ __forceinline__ void contract_side_t(const ContractArgs &a, int bx, int by, int bz)
 {
@@ -1082,87 +1088,107 @@ __device__ __forceinline__ void contract_side_t(const ContractArgs &a, int bx, i
     const int numr = a.numr, numz = a.numz, mmax = a.mmax;
     const int nri = numr - 2, nzi = numz - 2, nd = 2 * nzi - 1;
     const long plane = (long)numr * numz;
-    const int k0 = bx * SD_TGT, m = by, sp = bz;
-    const int ch = ((nzi + CT_SPLIT - 1) / CT_SPLIT + SD_TGT - 1) / SD_TGT * SD_TGT;
-    const int kzs = min(nzi, sp * ch), kze = min(nzi, kzs + ch);
+    const int sp = bx, m = by;
+    const int kz0 = sp * SD_PLANES, nk = min(SD_PLANES, nzi - kz0);
+    const int ntall = (nzi + SD_TGT - 1) / SD_TGT, tpg = (ntall + SD_GROUPS - 1) / SD_GROUPS;
+    const int T0 = bz * tpg, ntile = min(ntall, T0 + tpg);  // this workgroup's tiles [T0, ntile)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double *sm = S + (long)m * nd * nri;
+    const double *hm = IMG ? a.hankel + (long)m * nd * nri : nullptr;
     const double *pc = proj + (long)m * plane;
     const double *ps = proj + (long)(m > 0 ? mmax - 1 + m : 0) * plane;
-    double acc[2][SD_TGT];
-#pragma unroll
-    for (int t = 0; t < SD_TGT; ++t) acc[0][t] = acc[1][t] = 0.0;
+    __shared__ double tot[2][512];  // per-target totals (nzi <= 512)
+    __shared__ double red[4][2 * SD_TGT];
+    for (int k = tid; k < 2 * nzi; k += 256) tot[k / nzi][k % nzi] = 0.0;
     for (int jr0 = 0; jr0 < nri; jr0 += 256) {
         const int jr = jr0 + tid;
         const bool jok = jr < nri;
         const int jc = jok ? jr : nri - 1;
-        for (int kb = kzs; kb < kze; kb += SD_TGT) {
-            // window: diagonals d0 - (SD_TGT-1) .. d0 + (SD_TGT-1), d0 = k0 - kb + nzi - 1
-            const int d0 = k0 - kb + nzi - 1;
-            double w[2 * SD_TGT - 1], wh[IMG ? 2 * SD_TGT - 1 : 1], xc[SD_TGT], xs[SD_TGT];
+        double xc[SD_PLANES], xs[SD_PLANES];
 #pragma unroll
-            for (int i = 0; i < 2 * SD_TGT - 1; ++i) {
-                const int d = min(max(d0 - (SD_TGT - 1) + i, 0), nd - 1);
-                w[i] = sm[(long)d * nri + jc];
-            }
-            if constexpr (IMG) {
-                // anti-diagonals k0 + kb .. k0 + kb + 2 SD_TGT - 2
-                const double *hm = a.hankel + (long)m * nd * nri;
+        for (int k = 0; k < SD_PLANES; ++k) {
+            const bool ok = jok && k < nk;
+            const long o = (long)(min(kz0 + k, nzi - 1) + 1) * numr + jc + 1;
+            const double c = pc[o], sv = ps[o];
+            xc[k] = ok ? c : 0.0;
+            xs[k] = ok && m > 0 ? sv : 0.0;
+        }
+        // tile T: d = ko - kz + nzi - 1 = dlo + i, i = t - k + SD_TGT - 1; the
+        // image index s = ko + kz = slo + t + k
+        auto ldw = [&](int d) { return sm[(long)min(max(d, 0), nd - 1) * nri + jc]; };
+        auto ldh = [&](int sx) { return hm[(long)min(sx, nd - 1) * nri + jc]; };
+        const int dlo0 = nzi - 1 - kz0 - (SD_TGT - 1);
+        double w[2 * SD_TGT - 1], wh[IMG ? 2 * SD_TGT - 1 : 1];
 #pragma unroll
-                for (int i = 0; i < 2 * SD_TGT - 1; ++i) wh[i] = hm[(long)min(k0 + kb + i, nd - 1) * nri + jc];
-            }
+        for (int i = 0; i < 2 * SD_TGT - 1; ++i) {
+            w[i] = ldw(dlo0 + SD_TGT * T0 + i);
+            if constexpr (IMG) wh[i] = ldh(kz0 + SD_TGT * T0 + i);
+        }
+#pragma unroll 1
+        for (int T = T0; T < ntile; ++T) {
+            double acc[2][SD_TGT];
 #pragma unroll
-            for (int k = 0; k < SD_TGT; ++k) {
-                const int kz = kb + k;
-                const bool ok = jok && kz < kze;
-                const long o = (long)(min(kz, nzi - 1) + 1) * numr + jc + 1;
-                const double c = pc[o], sv = ps[o];
-                xc[k] = ok ? c : 0.0;
-                xs[k] = ok && m > 0 ? sv : 0.0;
-            }
+            for (int t = 0; t < SD_TGT; ++t) acc[0][t] = acc[1][t] = 0.0;
 #pragma unroll
-            for (int k = 0; k < SD_TGT; ++k)
+            for (int k = 0; k < SD_PLANES; ++k)
 #pragma unroll
                 for (int t = 0; t < SD_TGT; ++t) {
                     const double g = IMG ? w[SD_TGT - 1 + t - k] + wh[t + k] : w[SD_TGT - 1 + t - k];
                     acc[0][t] = fma(g, xc[k], acc[0][t]);
                     acc[1][t] = fma(g, xs[k], acc[1][t]);
                 }
-        }
-    }
-    // sum over the 256 threads: butterflies in each wavefront, then the 4
-    // wavefronts in order
-    double v[2 * SD_TGT];
+            // the next tile's window: SD_TGT - 1 values move down, SD_TGT load
+            if (T + 1 < ntile) {
 #pragma unroll
-    for (int t = 0; t < SD_TGT; ++t) {
-        v[t] = acc[0][t];
-        v[SD_TGT + t] = acc[1][t];
-    }
-    // reduce-scatter over lane bits 5..1: after the step on bit b the lane
-    // keeps the half of its values selected by that bit
-    int n = 2 * SD_TGT;
+                for (int i = 0; i < SD_TGT - 1; ++i) {
+                    w[i] = w[i + SD_TGT];
+                    if constexpr (IMG) wh[i] = wh[i + SD_TGT];
+                }
 #pragma unroll
-    for (int b = 32; b >= 2; b >>= 1) {
-        const bool hi = (lane & b) != 0;
-        n >>= 1;
-#pragma unroll
-        for (int i = 0; i < SD_TGT; ++i) {
-            if (i < n) {
-                const double keep = hi ? v[n + i] : v[i], give = hi ? v[i] : v[n + i];
-                v[i] = keep + __shfl_xor(give, b, 64);
+                for (int i = SD_TGT - 1; i < 2 * SD_TGT - 1; ++i) {
+                    w[i] = ldw(dlo0 + SD_TGT * (T + 1) + i);
+                    if constexpr (IMG) wh[i] = ldh(kz0 + SD_TGT * (T + 1) + i);
+                }
             }
+            // sum over the 256 threads: reduce-scatter over lane bits 5..1
+            // (after the step on bit b the lane keeps the half of its values
+            // selected by that bit), the pair, then the 4 wavefronts in order
+            double v[2 * SD_TGT];
+#pragma unroll
+            for (int t = 0; t < SD_TGT; ++t) {
+                v[t] = acc[0][t];
+                v[SD_TGT + t] = acc[1][t];
+            }
+            int n = 2 * SD_TGT;
+#pragma unroll
+            for (int b = 32; b >= 2; b >>= 1) {
+                const bool hi = (lane & b) != 0;
+                n >>= 1;
+#pragma unroll
+                for (int i = 0; i < SD_TGT; ++i) {
+                    if (i < n) {
+                        const double keep = hi ? v[n + i] : v[i], give = hi ? v[i] : v[n + i];
+                        v[i] = keep + __shfl_xor(give, b, 64);
+                    }
+                }
+            }
+            // lane holds value lane >> 1 summed over the 32 lanes that share
+            // lane bit 0; add the pair
+            const double pr = v[0] + __shfl_xor(v[0], 1, 64);
+            if ((lane & 1) == 0) red[wave][lane >> 1] = pr;
+            __syncthreads();
+            if (tid < 2 * SD_TGT) {
+                const int q = tid / SD_TGT, ko = SD_TGT * T + tid % SD_TGT;
+                const double r = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+                if (ko < nzi) tot[q][ko] += r;
+            }
+            __syncthreads();  // red is free for the next tile
         }
     }
-    // now n == 1: lane holds value lane >> 1 summed over the 32 lanes that
-    // share lane bit 0; add the pair
-    const double tot = v[0] + __shfl_xor(v[0], 1, 64);
-    __shared__ double red[4][2 * SD_TGT];
-    if ((lane & 1) == 0) red[wave][lane >> 1] = tot;
-    __syncthreads();
-    if (tid < 2 * SD_TGT) {
-        const int q = tid / SD_TGT, t = tid % SD_TGT, ko = k0 + t;
-        const double r = (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
-        if (ko < nzi) a.out[(((long)sp * mmax + m) * 2 + q) * nzi + ko] = r;
+    const int k0 = SD_TGT * T0, nt = min(nzi, SD_TGT * ntile) - k0;  // this workgroup's targets
+    for (int k = tid; k < 2 * nt; k += 256) {
+        const int q = k / nt, ko = k0 + k % nt;
+        a.out[(((long)sp * mmax + m) * 2 + q) * nzi + ko] = tot[q][ko];
     }
 }
 
@@ -1178,7 +1204,7 @@ __device__ __forceinline__ void contract_side_img_tile(const ContractArgs &a, in
 Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, int numz, int mmax, const double *H)
 {
     ContractArgs a{S, proj, Ss, numr, numz, mmax, nullptr, nullptr, 0, H};
-    return Op::make(OP_CONTRACT_SIDE, H ? 1 : 0, a, (numz - 2 + SD_TGT - 1) / SD_TGT, mmax, CT_SPLIT);
+    return Op::make(OP_CONTRACT_SIDE, H ? 1 : 0, a, side_splits(numz), mmax, SD_GROUPS);
 }
 
 // Boundary faces in helmadi's Fourier mode order.  bessel rebuilds each
@@ -1227,7 +1253,7 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     int nsp;  // partial sums stored per value
     if (fi < nri) { idx = fi; sc = Stb; ss = Stb + nri; st = 4 * (long)nri; nsp = a.nsplit_tb; }
     else if (fi < 2 * nri) { idx = fi - nri; sc = Stb + 2 * nri; ss = Stb + 3 * nri; st = 4 * (long)nri; nsp = a.nsplit_tb; }
-    else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; nsp = CT_SPLIT; }
+    else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; nsp = side_splits(numz); }
     sst = st * mmax;
     // partial sums of the source ranges of the contraction, in a fixed order
     auto S_c = [&](int m) {
@@ -1295,17 +1321,8 @@ Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const 
     {                                                                                             \
         BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
     }
-BH_KERNEL_PAIR(fill, FillArgs, 256)
-BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
-BH_KERNEL_PAIR(axis, AxisArgs, 128)
-BH_KERNEL_PAIR(proj, ProjArgs, 256)
-BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
-BH_KERNEL_PAIR(contract_side_img, ContractArgs, 256)
-BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
-BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
-BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
-// two workgroups per CU (the LDS of the register-blocked contraction
-// allows two): the register budget is held to 256 per lane
+// two waves per SIMD: the register budget is held to 256 per lane (the
+// register-blocked top contraction, whose LDS allows two workgroups per CU)
 #define BH_KERNEL_PAIR2(NAME, ARGS_T, NT)                                                         \
     __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2))) void NAME##_kernel(   \
         ARGS_T a, int nx, int ny, int total)                                                      \
@@ -1317,6 +1334,15 @@ BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
     {                                                                                             \
         BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
     }
+BH_KERNEL_PAIR(fill, FillArgs, 256)
+BH_KERNEL_PAIR(transpose, TransposeArgs, 256)
+BH_KERNEL_PAIR(axis, AxisArgs, 128)
+BH_KERNEL_PAIR(proj, ProjArgs, 256)
+BH_KERNEL_PAIR(contract_side, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_side_img, ContractArgs, 256)
+BH_KERNEL_PAIR(topsym_scale, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_topsym, ContractArgs, 256)
+BH_KERNEL_PAIR(contract_topsym32, ContractArgs, 256)
 BH_KERNEL_PAIR2(contract_tb64, ContractArgs, 256)
 BH_KERNEL_PAIR(contract_tb32, ContractArgs, 256)
 #undef BH_KERNEL_PAIR2

@@ -73,7 +73,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
         const size_t parts = tb_blocked(numr) ? tb_chunks(numz) : topsym_splits(numz) * ((NB + 1) / 2);
         ALLOC(p->Ptb, (size_t)mmax * parts * 16 * NB * 4);
     }
-    ALLOC(p->Ss, (size_t)CT_SPLIT * mmax * 2 * nzi);
+    ALLOC(p->Ss, (size_t)side_splits(numz) * mmax * 2 * nzi);
     ALLOC(p->FF, (size_t)N * nf);
     ALLOC(p->src_grid, 4 * (size_t)(numr + 1));
     ALLOC(p->src_com, 6 * (size_t)N);
@@ -672,7 +672,7 @@ extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
     const size_t nf = 2 * nri + nzi;
     const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->tab->T, p->tab->S, p->Stb, p->Ss, nullptr, p->FF};
     const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), topsym_table_elems(g.numr, g.numz, g.mmax),
-                          mm * (2 * nzi - 1) * nri, mm * 4 * nri, CT_SPLIT * mm * 2 * nzi, N * nf, N * nf};
+                          mm * (2 * nzi - 1) * nri, mm * 4 * nri, side_splits(g.numz) * mm * 2 * nzi, N * nf, N * nf};
     if (which < 0 || which > 10) return fail(BH_E_INVALID, "which");
     if (!src[which]) return fail(BH_E_STATE, "buffer not used by this plan's precision");
     if (n < cnt[which]) return fail(BH_E_INVALID, "buffer too small: need " + std::to_string(cnt[which]));

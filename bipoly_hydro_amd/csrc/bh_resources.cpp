@@ -623,7 +623,7 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
     const size_t M = (size_t)d.numr * d.numz, N = d.numphi, mm = d.mmax, nri = d.numr - 2, nzi = d.numz - 2;
     const size_t nf = 2 * nri + nzi;
     const size_t NB16 = 16 * ((nri + 15) / 16), W = (nri + 31) / 32;
-    size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + mm * 4 * nri + CT_SPLIT * mm * 2 * nzi + N * nf +
+    size_t b = 8 * (2 * M * N + M * (2 * mm - 1) + mm * 4 * nri + side_splits(d.numz) * mm * 2 * nzi + N * nf +
                     4 * (d.numr + 1) + 6 * N +
                     (tb_blocked(d.numr) ? 0 : mm * nzi * NB16 * 4) +
                     mm * (tb_blocked(d.numr) ? tb_chunks(d.numz) : topsym_splits(d.numz) * W) * NB16 * 4);
