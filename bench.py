@@ -528,13 +528,15 @@ def run_serve(args, emit=True):
             time.sleep(d)
         ids.append(srv.submit(g, n, m))
     lat, first, later = [], [], []
-    for i in ids:
+    by_lane = {0: [], 1: [], 2: []}
+    for i, (g, _, _) in zip(ids, reqs):
         res, _, l = srv.wait(i)
         if res["status"] != 0:
             raise RuntimeError(f"request {i} failed: {res}")
         lat.extend(l.tolist())
         first.append(l[0])
         later.extend(l[1:].tolist())
+        by_lane[serve_lane(g)].extend(l.tolist())
     wall = time.perf_counter() - t0
     st = srv.stats()
     log = srv.iterations()
@@ -556,31 +558,51 @@ def run_serve(args, emit=True):
         "mean_ms": float(lat.mean()), "solves": nsolves, "wall_s": wall,
         "iterations": int(st["iterations"] - base["iterations"]), "peak_active": st["peak_active"],
         "peak_device_bytes": st["peak_bytes"], "warmup_s": setup_s, "tail": tail,
+        # per lane of the serving loop (grid points < 2^20, < 2^22, larger)
+        "lanes_p50_p99_ms": {str(k): ([float(np.percentile(v, 50)), float(np.percentile(v, 99)), len(v)]
+                                      if v else None) for k, v in by_lane.items()},
     }
     if emit:
         print(json.dumps(out), flush=True)
     return out
 
 
+def serve_lane(grid):
+    """The serving loop's lane of a grid: its points below 2^20, below 2^22,
+    or larger (each lane's iterations run on a stream of their own)."""
+    pts = grid[0] * grid[1] * grid[2]
+    return 0 if pts < 1 << 20 else 1 if pts < 1 << 22 else 2
+
+
 def serve_tail(log, lat, skip, first=None):
     """Where the served latency tail comes from, from the loop's iteration
     log (rows: t_admit, t_enq0, t_enq1, t_done, solves, plans created,
-    points; the first `skip` iterations are the warm-up).  A solve's latency
-    is its iteration's completion minus the previous one's, so the interval
-    between completions, weighted by the solves in the iteration, stands for
-    the per-solve latency; it is split by whether the admission before the
-    iteration created a plan, and related to the points solved."""
+    points, lane; the first `skip` iterations are the warm-up).  A solve's
+    latency is its iteration's completion minus the previous one's in the
+    same lane, so the interval between a lane's completions, weighted by the
+    solves in the iteration, stands for the per-solve latency; it is split by
+    whether the admission before the iteration created a plan, and related
+    to the points solved."""
     later = None
     if first is not None:
         first, later = np.asarray(first[0]), np.asarray(first[1])
     log = log[skip:]
     if len(log) < 3:
         return None
-    t_done = log[:, 3]
-    # from the later of the previous completion and this iteration's enqueue
-    # (an iteration after an idle gap starts when it is enqueued)
-    iv = (t_done[1:] - np.maximum(t_done[:-1], log[1:, 1])) * 1e3
-    rows = log[1:]
+    # from the later of the lane's previous completion and this iteration's
+    # enqueue (an iteration after an idle gap starts when it is enqueued)
+    ivs, rows = [], []
+    for lane in np.unique(log[:, 7]):
+        lg = log[log[:, 7] == lane]
+        lg = lg[np.argsort(lg[:, 3])]
+        if len(lg) < 2:
+            continue
+        ivs.append((lg[1:, 3] - np.maximum(lg[:-1, 3], lg[1:, 1])) * 1e3)
+        rows.append(lg[1:])
+    if not rows:
+        return None
+    iv = np.concatenate(ivs)
+    rows = np.concatenate(rows)
     w = rows[:, 4]
     p99 = float(np.percentile(lat, 99))
     tail_it = iv >= p99

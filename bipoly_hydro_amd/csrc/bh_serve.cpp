@@ -2,16 +2,20 @@
 // self-consistent-field sequence of dependent solves, admitted by device
 // memory budget and run with continuous batching.
 //
-// A worker thread owns one non-blocking HIP stream.  Each loop iteration
-// takes the next solve of every active request (its density generated on the
-// device, then a cold first solve or a warm one from the potential the
-// request's plan kept) and runs them together as grouped launches
-// (bh_exec.h).  Requests that arrive while iterations are on the GPU are
-// admitted at the next iteration as soon as a slot and device memory are
-// free, so new work joins work already running; iterations are enqueued one
-// ahead of the one being waited for.  Plans (with their density and
-// potential buffers) of finished requests are kept per grid size and reused;
-// idle ones are released when memory is needed.
+// Requests run in lanes by grid size (grid points below 2^20, below 2^22,
+// and larger), each lane with its own non-blocking HIP stream.  A lane's
+// iteration takes the next solve of every active request of that lane (its
+// density generated on the device, then a cold first solve or a warm one
+// from the potential the request's plan kept) and runs them together as
+// grouped launches (bh_exec.h); the lanes' iterations run side by side on
+// the GPU, so a small grid's solve does not wait for the large grids of an
+// iteration it would otherwise share.  One worker thread admits requests,
+// keeps up to two iterations in flight per lane and polls their completion.
+// Requests that arrive while iterations are on the GPU are admitted as soon
+// as a slot and device memory are free, so new work joins work already
+// running.  Plans (with their density and potential buffers) of finished
+// requests are kept per grid size and reused; idle ones are released when
+// memory is needed.
 #include "bh_density.h"
 #include "bh_exec.h"
 #include "bh_plan_impl.h"
@@ -233,6 +237,7 @@ struct Req {
 struct Slot {
     Shell *sh = nullptr;
     Req *req = nullptr;
+    int lane = 0;
     int enq = 0, done = 0;  // solves enqueued / completed
     bool resume = false;    // the first solve is warm (from the request's start potential)
     double t_ready = 0;     // completion time of the last completed solve
@@ -252,6 +257,24 @@ struct IterRec {
     double t_admit, t_enq0, t_enq1, t_done;
     int nsolves, new_shells;
     double points;  // grid points solved in the iteration
+    int lane;
+};
+
+// lanes by grid points: [0, 2^20), [2^20, 2^22), [2^22, ...)
+constexpr int NLANES = 3;
+int lane_of(const bh_grid_desc &d)
+{
+    const double pts = (double)d.numr * d.numz * d.numphi;
+    return pts < 1048576.0 ? 0 : pts < 4194304.0 ? 1 : 2;
+}
+
+// One lane: its stream, its iterations in flight (oldest first) and the
+// grouped launches of the next one.
+struct Lane {
+    hipStream_t stream = nullptr;
+    std::deque<Iteration> inflight;
+    GroupExec ex;
+    std::vector<std::vector<Op>> lists;
 };
 
 std::string grid_key(const bh_grid_desc &d)
@@ -267,7 +290,7 @@ std::string grid_key(const bh_grid_desc &d)
 struct bh_server {
     bh_server_config cfg{};
     int device = 0;
-    hipStream_t stream = nullptr;
+    Lane lanes[NLANES];
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -282,10 +305,7 @@ struct bh_server {
     std::vector<Shell *> shells;  // every shell (active and pooled)
     std::map<std::string, std::shared_ptr<const DevMem>> profiles;
     std::map<std::string, const double *> profile_ptr;
-    std::deque<Iteration> inflight;
     std::vector<hipEvent_t> free_events;
-    GroupExec ex;
-    std::vector<std::vector<Op>> lists;
     // statistics (under mu)
     long long submitted = 0, completed = 0, solves = 0, iterations = 0;
     int nactive = 0, peak_active = 0;
@@ -311,7 +331,8 @@ size_t resident_bytes(bh_server *S)
         if (seen.insert(sh->plan->ops.get()).second) b += sh->plan->ops->mem.bytes;
     }
     for (auto &kv : S->profiles) b += kv.second->bytes;
-    return b + S->ex.arena_bytes();
+    for (const Lane &L : S->lanes) b += L.ex.arena_bytes();
+    return b;
 }
 
 size_t shell_new_bytes(const bh_grid_desc &d, int precision, int device)
@@ -389,6 +410,13 @@ void finish_request(bh_server *S, Req *q, int status)
 
 // Admit waiting requests (first come, first served) while slots and device
 // memory allow; idle pooled plans are released to make room.
+bool any_inflight(const bh_server *S)
+{
+    for (const Lane &L : S->lanes)
+        if (!L.inflight.empty()) return true;
+    return false;
+}
+
 void admit(bh_server *S)
 {
     for (;;) {
@@ -420,7 +448,7 @@ void admit(bh_server *S)
             used = resident_bytes(S);
         }
         if (S->cfg.hbm_budget && used + need > S->cfg.hbm_budget) {
-            if (S->active.empty() && S->inflight.empty()) {
+            if (S->active.empty() && !any_inflight(S)) {
                 // cannot fit even alone
                 {
                     std::lock_guard<std::mutex> lk(S->mu);
@@ -445,18 +473,20 @@ void admit(bh_server *S)
             finish_request(S, q, rc != BH_OK ? rc : BH_E_DEVICE);
             continue;
         }
+        const int lane = lane_of(d);
         if (!q->start.empty()) {
             // resume: the potential into the shell's buffer, its transform
             // into the plan's warm-start state, ahead of the first solve on
-            // the server's stream
+            // the lane's stream
+            const hipStream_t ls = S->lanes[lane].stream;
             hipError_t e = hipMemcpyAsync(sh->pot, q->start.data(), q->start.size() * sizeof(double),
-                                          hipMemcpyHostToDevice, S->stream);
+                                          hipMemcpyHostToDevice, ls);
             if (e == hipSuccess) {
                 bh_plan *p = sh->plan;
                 const long M = (long)p->g.numr * p->g.numz;
                 std::vector<Op> ops;
                 dft_fwd64_ops(p, sh->pot, M, (int)M, p->XJ, M, false, &ops);
-                for (const Op &op : ops) launch_op(op, S->stream);
+                for (const Op &op : ops) launch_op(op, ls);
                 e = hipGetLastError();
             }
             // (q->start lives until the request is collected, after the copy)
@@ -469,6 +499,7 @@ void admit(bh_server *S)
         Slot *sl = new Slot();
         sl->sh = sh;
         sl->req = q;
+        sl->lane = lane;
         sl->resume = !q->start.empty();
         sl->prof = prof;
         sl->prof_ptr = prof_ptr;
@@ -483,25 +514,27 @@ void admit(bh_server *S)
     }
 }
 
-// Enqueue the next solve of every active request that has one left.
-int enqueue_iteration(bh_server *S, bool *any)
+// Enqueue the next solve of every active request of the lane that has one
+// left.
+int enqueue_iteration(bh_server *S, int lane, bool *any)
 {
     *any = false;
+    Lane &Ln = S->lanes[lane];
     Iteration it;
-    S->lists.clear();
+    Ln.lists.clear();
     for (Slot *sl : S->active) {
-        if (sl->enq >= sl->req->r.nsolves) continue;
+        if (sl->lane != lane || sl->enq >= sl->req->r.nsolves) continue;
         const int i = sl->enq;
         bh_plan *p = sl->sh->plan;
         std::vector<Op> ops;
         build_ops(p, sl->sh->rho, sl->sh->pot, (i == 0 && !sl->resume) ? 0 : 2, &ops);
         ops.insert(ops.begin(), density_op(model_args(sl->req->r.model, p->g, i, sl->sh->rho, sl->prof_ptr)));
-        S->lists.push_back(std::move(ops));
+        Ln.lists.push_back(std::move(ops));
         it.solves.push_back({sl, i});
         sl->enq++;
     }
     if (it.solves.empty()) return BH_OK;
-    IterRec rec{S->t_admit - S->t0, now_s() - S->t0, 0, 0, (int)it.solves.size(), S->new_shells, 0};
+    IterRec rec{S->t_admit - S->t0, now_s() - S->t0, 0, 0, (int)it.solves.size(), S->new_shells, 0, lane};
     S->new_shells = 0;
     for (auto &sv : it.solves) {
         const Geo &g = sv.first->sh->plan->g;
@@ -509,9 +542,9 @@ int enqueue_iteration(bh_server *S, bool *any)
     }
     bool same;
     int rc;
-    if ((rc = S->ex.prepare(S->lists, &same)) != BH_OK) return rc;
-    if ((rc = S->ex.upload(S->stream)) != BH_OK) return rc;
-    if ((rc = S->ex.launch(S->stream, nullptr)) != BH_OK) return rc;
+    if ((rc = Ln.ex.prepare(Ln.lists, &same)) != BH_OK) return rc;
+    if ((rc = Ln.ex.upload(Ln.stream)) != BH_OK) return rc;
+    if ((rc = Ln.ex.launch(Ln.stream, nullptr)) != BH_OK) return rc;
     if (S->free_events.empty()) {
         hipEvent_t e;
         HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -519,23 +552,24 @@ int enqueue_iteration(bh_server *S, bool *any)
     }
     it.ev = S->free_events.back();
     S->free_events.pop_back();
-    HIPCHK(hipEventRecord(it.ev, S->stream));
+    HIPCHK(hipEventRecord(it.ev, Ln.stream));
     rec.t_enq1 = now_s() - S->t0;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         it.rec = S->iter_log.size();
         S->iter_log.push_back(rec);
     }
-    S->inflight.push_back(std::move(it));
+    Ln.inflight.push_back(std::move(it));
     *any = true;
     return BH_OK;
 }
 
-// Wait for the oldest iteration and account for its solves.
-int complete_oldest(bh_server *S)
+// Wait for the lane's oldest iteration and account for its solves.
+int complete_oldest(bh_server *S, int lane)
 {
-    Iteration it = std::move(S->inflight.front());
-    S->inflight.pop_front();
+    Lane &Ln = S->lanes[lane];
+    Iteration it = std::move(Ln.inflight.front());
+    Ln.inflight.pop_front();
     hipError_t e = hipEventSynchronize(it.ev);
     const double t = now_s();
     S->free_events.push_back(it.ev);
@@ -584,19 +618,37 @@ void worker_main(bh_server *S)
         {
             std::unique_lock<std::mutex> lk(S->mu);
             S->cv.wait(lk, [&] {
-                return S->stop || !S->incoming.empty() || !S->active.empty() || !S->inflight.empty();
+                return S->stop || !S->incoming.empty() || !S->active.empty() || any_inflight(S);
             });
-            if (S->stop && S->incoming.empty() && S->active.empty() && S->inflight.empty()) return;
+            if (S->stop && S->incoming.empty() && S->active.empty() && !any_inflight(S)) return;
         }
         S->t_admit = now_s();
         admit(S);
-        bool any = false;
-        int rc = enqueue_iteration(S, &any);
-        if (rc == BH_OK && (S->inflight.size() >= 2 || (!any && !S->inflight.empty()))) rc = complete_oldest(S);
+        // per lane: account for the finished iterations, then keep two in
+        // flight
+        bool progress = false;
+        int rc = BH_OK;
+        for (int l = 0; l < NLANES && rc == BH_OK; ++l) {
+            Lane &Ln = S->lanes[l];
+            while (rc == BH_OK && !Ln.inflight.empty()) {
+                const hipError_t q = hipEventQuery(Ln.inflight.front().ev);
+                if (q == hipErrorNotReady) break;
+                rc = complete_oldest(S, l);  // done, or failed: its error is reported there
+                progress = true;
+            }
+            if (rc == BH_OK && Ln.inflight.size() < 2) {
+                bool any = false;
+                rc = enqueue_iteration(S, l, &any);
+                progress = progress || any;
+            }
+        }
+        if (rc == BH_OK && !progress && any_inflight(S))
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
         if (rc != BH_OK) {
             // fail every request in the loop and keep serving new ones
             std::string msg = last_error();
-            while (!S->inflight.empty()) complete_oldest(S);
+            for (int l = 0; l < NLANES; ++l)
+                while (!S->lanes[l].inflight.empty()) complete_oldest(S, l);
             for (Slot *sl : S->active) {
                 S->pool[sl->sh->key].push_back(sl->sh);
                 finish_request(S, sl->req, rc);
@@ -640,7 +692,7 @@ extern "C" int bh_server_create(const bh_server_config *cfg, bh_server **out)
         HIPCHK(hipMemGetInfo(&fr, &tot));
         S->cfg.hbm_budget = fr / 10 * 9;
     }
-    HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    for (Lane &L : S->lanes) HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
     S->t0 = now_s();
     bh_server *raw = S.get();
     S->worker = std::thread(worker_main, raw);
@@ -751,11 +803,11 @@ extern "C" int bh_server_destroy(bh_server *S)
     }
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
-    (void)hipStreamSynchronize(S->stream);
+    for (Lane &L : S->lanes) (void)hipStreamSynchronize(L.stream);
     for (auto &kv : S->finished) delete kv.second;
     for (Shell *sh : std::vector<Shell *>(S->shells)) destroy_shell(S, sh);
     for (hipEvent_t e : S->free_events) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(S->stream);
+    for (Lane &L : S->lanes) (void)hipStreamDestroy(L.stream);
     delete S;
     return BH_OK;
 }
@@ -798,7 +850,7 @@ extern "C" int bhx_density(const bh_grid_desc *desc, const bh_binary_model *mode
 
 // Iteration log (test and bench hook, not part of the public ABI): up to n
 // records of 8 doubles (t_admit, t_enq0, t_enq1, t_done, nsolves,
-// new_shells, points, 0) into out; returns the number of iterations logged.
+// new_shells, points, lane) into out; returns the number of iterations logged.
 extern "C" int bhx_server_iterations(bh_server *S, double *out, int n)
 {
     if (!S) return fail(BH_E_INVALID, "null server");
@@ -807,7 +859,7 @@ extern "C" int bhx_server_iterations(bh_server *S, double *out, int n)
     for (int i = 0; i < m && i < n && out; ++i) {
         const IterRec &r = S->iter_log[i];
         const double v[8] = {r.t_admit, r.t_enq0, r.t_enq1, r.t_done, (double)r.nsolves, (double)r.new_shells,
-                             r.points, 0.0};
+                             r.points, (double)r.lane};
         for (int k = 0; k < 8; ++k) out[8 * i + k] = v[k];
     }
     return m;

@@ -470,14 +470,15 @@ class PotentialServer:
         """Host-side log of the serving loop's iterations, one row each:
         t_admit, t_enq0, t_enq1, t_done (seconds since the server started;
         the admission pass before the iteration, its enqueue, its completion),
-        solves in it, plans created for it, grid points solved."""
+        solves in it, plans created for it, grid points solved, and its lane
+        (grid points below 2^20, below 2^22, larger: lanes run side by side)."""
         L = load_library()
         f = L.bhx_server_iterations
         f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
         n = f(self._h, None, 0)
         out = np.zeros((max(n, 1), 8))
         n = f(self._h, out.ctypes.data, n)
-        return out[:n, :7]
+        return out[:n]
 
     def stats(self) -> dict:
         st = ServerStats()

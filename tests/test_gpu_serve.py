@@ -160,3 +160,29 @@ def test_served_sequence_resumes_from_continuation_files(tmp_path):
         for i in range(n):
             ref = plan.potential_solver(host_density(grid, model, i, prof), tstep=2)
     assert np.linalg.norm(pot - ref) <= 1e-12 * np.linalg.norm(ref)
+
+
+def test_lanes_by_grid_size_run_side_by_side():
+    """A small grid (lane 0, < 2^20 points) and a large one (lane 2, >= 2^22
+    points) served together: each lane has its own iterations, and the small
+    request's results equal those it gets alone."""
+    small, big = (42, 26, 16), (258, 130, 128)
+    ms = dict(radius=(5.0, 4.0), separation=14.0, drift=0.005, seed=1)
+    mb = dict(radius=(20.0, 15.0), separation=60.0, drift=0.002, seed=2)
+    with P.PotentialServer(precision="mixed") as srv:
+        r0 = srv.submit(small, 6, ms, want_potential=True)
+        alone = srv.wait(r0)[1]
+    with P.PotentialServer(precision="mixed") as srv:
+        rb = srv.submit(big, 3, mb, want_potential=True)
+        rs = srv.submit(small, 6, ms, want_potential=True)
+        res_s, pot_s, _ = srv.wait(rs)
+        res_b, _, _ = srv.wait(rb)
+        log = srv.iterations()
+    assert res_s["status"] == 0 and res_b["status"] == 0
+    assert np.array_equal(pot_s, alone)
+    lanes = set(int(x) for x in log[:, 7])
+    assert {0, 2} <= lanes
+    # no iteration mixes the two: each solved only its own lane's points
+    for row in log:
+        pts = row[6]
+        assert (pts < (1 << 20) * 6) if row[7] == 0 else (pts >= (1 << 22))
