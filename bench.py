@@ -105,7 +105,7 @@ def main():
     ap.add_argument("--passes", type=int, default=2, help="refinement passes of the headline mixed solve")
     ap.add_argument("--no-workloads", action="store_true",
                     help="skip the short batched and served runs of the default line")
-    ap.add_argument("--only", choices=["mixed", "fp64", "mixed_t32"],
+    ap.add_argument("--only", choices=["mixed", "fp64", "mixed_t32", "spectral64"],
                     help="time one precision alone (profiling runs); prints a reduced line")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -210,14 +210,15 @@ def main():
     if args.only:
         # one precision alone (for profiling runs): the headline fields of
         # that solve, nothing else
-        prec = {"mixed": "mixed", "fp64": "fp64", "mixed_t32": "mixed_t32"}[args.only]
+        prec = args.only
         res, _ = run(prec, passes=args.passes)
         print(json.dumps({"metric": METRIC, "value": res["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-                          "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+                          "dtype": "f64" if prec in ("fp64", "spectral64") else "f32+f64", "data": "synthetic",
                           "config": {"workload": f"potential_solver {numr}x{numz}x{numphi} tstep={tstep}, "
-                                                 f"{prec} only, {args.passes} refinement passes"},
+                                                 f"{prec} only" + (f", {args.passes} refinement passes"
+                                                                   if prec.startswith("mixed") else "")},
                           "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "stage_ms": res["stage_ms"],
                           "error_vs_reference_cold": res["error_vs_reference_cold"]}), flush=True)
         return
@@ -238,6 +239,13 @@ def main():
         plan.close()
         per_pass.append({"passes": npass, "ms_per_solve": t["ms_per_step"],
                          "error_vs_reference": reference_errors(pot, "1"), "relL2_vs_fp64_gpu": rell2(pot, pot64)})
+    # the same spectral form of the iteration evaluated once in FP64: what
+    # the FP32/bf16 transforms and the refinement passes buy
+    s64, pots64 = run("spectral64", warm_steps=max(10, args.steps // 5))
+    s64 = {"ms_per_step": s64["ms_per_step"], "warm_ms": s64["warm"]["ms_per_step"],
+           "stage_ms": s64["stage_ms"], "error_vs_reference_cold": s64["error_vs_reference_cold"],
+           "relL2_vs_fp64_gpu": rell2(pots64, pot64),
+           "precision": "FP64 spectral solve: FP64 MFMA sine transforms and M_r products, FP64 Thomas, one pass"}
     t32, pot32 = run("mixed_t32", passes=1)
     t32 = {"ms_per_step": t32["ms_per_step"], "error_vs_reference_cold": t32["error_vs_reference_cold"],
            "precision": "FP32 top/bottom Green's table (opt-in), 1 refinement pass"}
@@ -304,6 +312,8 @@ def main():
         "warm": {"mixed_ms": mixed["warm"]["ms_per_step"], "fp64_ms": fp64["warm"]["ms_per_step"],
                  "mixed_over_fp64": fp64["warm"]["ms_per_step"] / mixed["warm"]["ms_per_step"]},
         "mixed_table32": t32,
+        "fp64_spectral": s64,
+        "mixed_over_fp64_spectral": s64["ms_per_step"] / mixed["ms_per_step"],
         "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (contract with the projections scaled as read, then reduce)",
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,

@@ -73,6 +73,7 @@ struct MrArgs {
     double *scratch_v, *scratch_u, *scratch_g;  // nmodes*nri*nri each
     float *Mr;                                   // [numphi][nri][nri]
     int frag;  // store M_r(L) as bf16 hi/lo pairs in the bf16 MFMA A-fragment order
+    double *Mr64;  // when given (in place of Mr): M_r(L)^T in FP64, [mode][j'][j] = M_r[j][j']
 };
 void launch_build_mr(const MrArgs &a, hipStream_t s);
 void launch_fillf(float *p, size_t n, float v, hipStream_t s);

@@ -341,6 +341,12 @@ __global__ void build_mr_kernel(MrArgs a)
         }
         for (int j = nri - 2; j >= 0; --j) v[j * st] = v[j * st] - gm[(j + 1) * st] * v[(j + 1) * st];
     }
+    if (a.Mr64) {
+        // the transpose in FP64: column jc of M_r is row jc of the output
+        double *o64 = a.Mr64 + ((long)L * nri + jc) * nri;
+        for (int j = 0; j < nri; ++j) o64[j] = v[j * st];
+        return;
+    }
     float *out = a.Mr + (long)L * nri * nri;
     if (a.frag) {
         // bf16 pairs x = hi + lo (each rounded to nearest) in the A-fragment

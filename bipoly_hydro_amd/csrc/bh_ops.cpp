@@ -50,6 +50,7 @@ void launch_op(const Op &op, hipStream_t s)
     if (mixed_launch(op, s)) return;
     if (density_launch(op, s)) return;
     if (spectral_launch(op, s)) return;
+    if (spec64_launch(op, s)) return;
     (void)dft_launch(op, s);
 }
 
@@ -61,6 +62,7 @@ void launch_op_grouped(int kind, int variant, const void *d_args, const GroupSch
     if (mixed_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (density_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     if (spectral_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
+    if (spec64_launch_grouped(kind, variant, d_args, gs, nblocks, s)) return;
     (void)dft_launch_grouped(kind, variant, d_args, gs, nblocks, s);
 }
 
@@ -73,6 +75,7 @@ int op_blocks_per_cu(int kind, int variant)
     if (density_occupancy(kind, variant, &b)) return b;
     if (dft_occupancy(kind, variant, &b)) return b;
     if (spectral_occupancy(kind, variant, &b)) return b;
+    if (spec64_occupancy(kind, variant, &b)) return b;
     return 0;
 }
 

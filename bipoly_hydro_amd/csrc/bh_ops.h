@@ -105,6 +105,9 @@ enum OpKind {
     OP_DFT,           // bh_dft.hip
     OP_SPECTRAL,      // bh_spectral.hip
     OP_RESIDUAL_FACE, // bh_mixed.hip
+    OP_RES64,         // bh_spec64.hip
+    OP_THOMAS64,
+    OP_UPDATE64,
     OP_NKINDS
 };
 
@@ -185,5 +188,9 @@ bool spectral_launch(const Op &op, hipStream_t s);
 bool spectral_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
                              hipStream_t s);
 bool spectral_occupancy(int kind, int variant, int *blocks);
+bool spec64_launch(const Op &op, hipStream_t s);
+bool spec64_launch_grouped(int kind, int variant, const void *d_args, const GroupSched &gs, int nblocks,
+                           hipStream_t s);
+bool spec64_occupancy(int kind, int variant, int *blocks);
 
 }  // namespace bh

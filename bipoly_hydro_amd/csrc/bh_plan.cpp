@@ -5,6 +5,7 @@
 #include "bh_dft.h"
 #include "bh_gemm.h"
 #include "bh_mixed.h"
+#include "bh_spec64.h"
 #include "bh_spectral.h"
 
 #include <hip/hip_runtime.h>
@@ -92,6 +93,12 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
         ALLOC(p->D1, (with_d2 ? 3 : 2) * M * N);
         p->Wf = p->D1 + M * N;
         if (with_d2) p->D2 = p->Wf + M * N;
+    } else if (is_spec64(precision)) {
+        const size_t PI = (size_t)N * nri * nzi;
+        ALLOC(p->RJ, M * N);
+        ALLOC(p->R64, PI);
+        ALLOC(p->T64, PI);
+        ALLOC(p->U64, PI);
     } else {
         ALLOC(p->RJ, M * N);
         ALLOC(p->RK, M * N);
@@ -460,6 +467,66 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     out->push_back(Op::mark(4));
 }
 
+// The FP64 spectral solve: the truncated Peaceman-Rachford iteration of
+// helmadi.F90:195-355 as x_N = X0 + delta, A delta = (I - M) r0 (the mixed
+// solve's form), evaluated once in FP64 (bh_spec64.hip).
+void spec64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out)
+{
+    const Geo &g = p->g;
+    const ModeOps &o = *p->ops;
+    const int numr = g.numr, numz = g.numz, N = g.numphi, nri = g.nri, nzi = g.nzi, nh = N / 2 + 1;
+    const long M = (long)numr * numz, PIm = (long)nri * nzi;
+    const int w = tstep > 1 ? 1 : 0, cold = tstep <= 1 ? 1 : 0;
+    out->push_back(Op::mark(0));
+    // 1-2. rho-hat (masked as rhop) on the side stream beside the
+    //      projections; the boundary faces into XJ (XK takes a copy)
+    out->push_back(Op::mark(MARK_FORK));
+    dft_fwd64_ops(p, d_rho, M, (int)M, p->RJ, M, true, out, side_blocks(p));
+    out->push_back(Op::mark(MARK_SIDE_END));
+    proj64_ops(p, d_rho, out);
+    out->push_back(Op::mark(1));
+    boundary_ops(p, p->XJ, p->XK, 0, out);
+    out->push_back(Op::mark(2));
+    // 3. r0 = g - A X0 (X0 zero inside for a cold start)
+    Res64Args ra{p->XJ, p->RJ, p->R64, o.brb_d, o.alphar, o.betar, g.gamma, g.four_pi, numr, numz, cold};
+    out->push_back(res64_op(ra, N));
+    // 4. T = S r0 per plane
+    GemmArgs a{};
+    a.A = o.S64; a.lda = nzi; a.sA = 0;
+    a.B = p->R64; a.ldb = nri; a.sB = PIm;
+    a.C = p->T64; a.ldc = nri; a.sC = PIm;
+    a.R = nzi; a.M = nri; a.Kd = nzi; a.batch = N;
+    a.tile = 2;
+    out->push_back(gemm_op(a, G64_PLAIN));
+    // 5. U = T M_r^T: planes 0..N/2 take mode numbers 0..N/2, planes
+    //    N/2+1..N-1 mode numbers 1..N/2-1
+    GemmArgs b{};
+    b.A = p->T64; b.lda = nri; b.sA = PIm;
+    b.B = o.MrT64[w]; b.ldb = nri; b.sB = (long)nri * nri;
+    b.C = p->U64; b.ldc = nri; b.sC = PIm;
+    b.R = nzi; b.M = nri; b.Kd = nri; b.batch = nh;
+    b.tile = 2;
+    out->push_back(gemm_op(b, G64_PLAIN));
+    if (N - nh > 0) {
+        b.A = p->T64 + nh * PIm; b.B = o.MrT64[w] + (long)nri * nri; b.C = p->U64 + nh * PIm;
+        b.batch = N - nh;
+        out->push_back(gemm_op(b, G64_PLAIN));
+    }
+    // 6. (B_r + lambda_k) D = T - mu_k U along J, D into U
+    Thomas64Args ta{p->T64, p->U64, o.mu64[w], o.lam64, o.brb_d, o.alphar, o.betar, numr, nri, nzi};
+    out->push_back(thomas64_op(ta, N));
+    // 7. delta = S D, then X = X0 + delta
+    a.B = p->U64; a.C = p->R64;
+    out->push_back(gemm_op(a, G64_PLAIN));
+    Update64Args ua{p->XJ, p->R64, numr, numz, cold};
+    out->push_back(update64_op(ua, N));
+    out->push_back(Op::mark(3));
+    // 8. inverse DFT and the axis ghost (helmadi.F90:357-411)
+    dft_inv64_ops(p, p->XJ, M, (int)M, d_pot, M, out);
+    out->push_back(axis_op(d_pot, numr, numz, N));
+    out->push_back(Op::mark(4));
+}
+
 int collect_timing(bh_plan *p)
 {
     if (!p->timing || p->tcount == 0) return BH_OK;
@@ -486,6 +553,7 @@ void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
 {
     out->clear();
     if (is_mixed(p->precision)) mixed_ops(p, d_rho, d_pot, tstep, out);
+    else if (is_spec64(p->precision)) spec64_ops(p, d_rho, d_pot, tstep, out);
     else fp64_ops(p, d_rho, d_pot, tstep, out);
 }
 

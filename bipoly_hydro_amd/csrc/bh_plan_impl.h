@@ -57,6 +57,12 @@ struct HostGeo {
 int normalize_desc(const bh_grid_desc *in, int precision, bh_grid_desc *out);
 // The mixed-precision solve (BH_PREC_MIXED or its FP32-table variant).
 inline bool is_mixed(int precision) { return precision == BH_PREC_MIXED || precision == BH_PREC_MIXED_TABLE32; }
+// The FP64 spectral solve (BH_PREC_SPECTRAL64).
+inline bool is_spec64(int precision) { return precision == BH_PREC_SPECTRAL64; }
+inline bool valid_precision(int precision)
+{
+    return precision == BH_PREC_FP64 || is_mixed(precision) || is_spec64(precision);
+}
 void host_geometry(const bh_grid_desc &d, HostGeo *h);
 
 // Green's tables of one (numr, numz, dr, mmax) geometry on one device
@@ -92,6 +98,10 @@ struct ModeOps {
     bool spec_fused = false;
     float *sfb = nullptr, *sfa = nullptr;
     float *sfb16 = nullptr, *sfa16 = nullptr;  // nzi >= 64: split-bf16 fragments (same sizes)
+    // FP64 spectral solve: sine basis S [a][k] (symmetric), lambda_k and
+    // mu_k per sine mode k, M_r(m)^T per azimuthal mode number m = 0..N/2
+    // ([m][j'][j]) for both schedules
+    double *S64 = nullptr, *lam64 = nullptr, *mu64[2] = {}, *MrT64[2] = {};
 };
 
 // Shared resources from the per-process cache (created on first use; kept
@@ -127,6 +137,9 @@ struct bh_plan {
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
     float *D1 = nullptr, *Wf = nullptr;
     float *D2 = nullptr;  // sum of the refinement corrections (mixed, fused spectral step, passes >= 2)
+    // FP64 spectral solve: r0 then delta, S r0, its product with M_r^T, each
+    // compact [L][k or K][J-2]
+    double *R64 = nullptr, *T64 = nullptr, *U64 = nullptr;
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
     // the source step (bh_hydro.hip): r, rhf, rinv, rhfinv [4][numr + 1]
     // (1-based J), and the centre-of-mass arrays cylin_a_com, cylin_v_com

@@ -59,8 +59,8 @@ int normalize_desc(const bh_grid_desc *desc, int precision, bh_grid_desc *out)
     if (d.nsteps_warm == 0) d.nsteps_warm = 5;
     if (d.refine_passes == 0) d.refine_passes = 2;
     if (d.isym < 1 || d.isym > 3) return fail(BH_E_INVALID, "isym must be 1, 2 or 3");
-    if (d.isym != 1 && is_mixed(precision))
-        return fail(BH_E_INVALID, "isym = 2, 3 need precision BH_PREC_FP64 (the mixed solve's sine basis in z "
+    if (d.isym != 1 && (is_mixed(precision) || is_spec64(precision)))
+        return fail(BH_E_INVALID, "isym = 2, 3 need precision BH_PREC_FP64 (the spectral solves' sine basis in z "
                                   "is the one of two Dirichlet faces)");
     if (d.numr < 5 || d.numz < 5) return fail(BH_E_INVALID, "numr and numz must be >= 5");
     if (d.numphi < 4 || (d.numphi & (d.numphi - 1))) return fail(BH_E_INVALID, "numphi must be a power of two >= 4");
@@ -70,8 +70,9 @@ int normalize_desc(const bh_grid_desc *desc, int precision, bh_grid_desc *out)
     if (d.nsteps_cold < 2 || d.nsteps_warm < 2) return fail(BH_E_INVALID, "nsteps must be >= 2");
     if (!(d.dr > 0.0) || !std::isfinite(d.dr)) return fail(BH_E_INVALID, "dr must be positive");
     if (d.refine_passes < 1 || d.refine_passes > 4) return fail(BH_E_INVALID, "refine_passes must be in [1, 4]");
-    if (precision != BH_PREC_FP64 && !is_mixed(precision))
-        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64, BH_PREC_MIXED or BH_PREC_MIXED_TABLE32");
+    if (!valid_precision(precision))
+        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64, BH_PREC_MIXED, BH_PREC_MIXED_TABLE32 or "
+                                  "BH_PREC_SPECTRAL64");
     *out = d;
     return BH_OK;
 }
@@ -199,7 +200,7 @@ std::map<std::string, std::weak_ptr<const ModeOps>> g_modeops;
 // other plan keeps the table in FP64
 bool table32(int precision) { return precision == BH_PREC_MIXED_TABLE32; }
 // the transform and operator data do not depend on the table precision
-int ops_precision(int precision) { return is_mixed(precision) ? BH_PREC_MIXED : BH_PREC_FP64; }
+int ops_precision(int precision) { return is_mixed(precision) ? BH_PREC_MIXED : precision; }
 
 std::string tables_key(const bh_grid_desc &d, int precision, int device)
 {
@@ -240,6 +241,8 @@ size_t modeops_bytes(const bh_grid_desc &d, int precision)
             b += 4 * (nzi >= 64 ? 4 : 2) * (nzi / 2) * nzi - 4 * N * nri * nzi -
                  4 * (N / 2 - 1) * (2 * nri * nri + nri * nzi);
         b += 8 * N * (numr + 1);
+    } else if (is_spec64(precision)) {
+        b += 8 * (nzi * nzi + 3 * nzi + 2 * nh * nri * nri + N * (numr + 1));
     } else {
         b += 8 * 3 * ((size_t)d.nsteps_cold + d.nsteps_warm) * (N * nri + nzi);
     }
@@ -402,6 +405,59 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
         (rc = up(o->elzb, std::vector<double>(h.elzb))) != BH_OK)
         return rc;
     const std::vector<double> &ar = h.betar, &cr = h.alphar;
+    if (is_spec64(precision)) {
+        // FP64 spectral solve: the sine basis of B_z, lambda_k, mu_k of both
+        // schedules, and M_r(m)^T for the N/2 + 1 azimuthal mode numbers
+        // (planes m and m + N/2 share one)
+        const int NM = N / 2 + 1;
+        ALLOC(o->S64, (size_t)nzi * nzi);
+        ALLOC(o->lam64, nzi);
+        ALLOC(o->mu64[0], nzi);
+        ALLOC(o->mu64[1], nzi);
+        ALLOC(o->MrT64[0], (size_t)NM * nri * nri);
+        ALLOC(o->MrT64[1], (size_t)NM * nri * nri);
+        ALLOC(o->brb_d, (size_t)N * (numr + 1));
+        std::vector<double> S((size_t)nzi * nzi), lam(nzi);
+        for (int a = 0; a < nzi; ++a)
+            for (int k = 0; k < nzi; ++k) S[(size_t)a * nzi + k] = sine_basis(a, k, nzi);
+        for (int k = 0; k < nzi; ++k) lam[k] = 2.0 * g.gamma * (1.0 - std::cos(pi * (k + 1) / (nzi + 1)));
+        std::vector<double> mu[2];
+        for (int w = 0; w < 2; ++w) {
+            mu[w].resize(nzi);
+            for (int k = 0; k < nzi; ++k) {
+                double mm = 1.0;
+                for (double d : h.dt[w]) mm *= (d - lam[k]) / (d + lam[k]);
+                mu[w][k] = mm;
+            }
+        }
+        if ((rc = up(o->brb_d, std::vector<double>(h.brb))) != BH_OK || (rc = up(o->S64, std::move(S))) != BH_OK ||
+            (rc = up(o->lam64, std::move(lam))) != BH_OK || (rc = up(o->mu64[0], std::move(mu[0]))) != BH_OK ||
+            (rc = up(o->mu64[1], std::move(mu[1]))) != BH_OK)
+            return rc;
+        const int chunk = 64;
+        DevMem scratch;
+        double *sv, *su, *sg, *ddt[2];
+        const size_t nsc = (size_t)chunk * nri * nri;
+        if ((rc = scratch.alloc(&sv, nsc)) != BH_OK || (rc = scratch.alloc(&su, nsc)) != BH_OK ||
+            (rc = scratch.alloc(&sg, nsc)) != BH_OK || (rc = scratch.alloc(&ddt[0], h.dt[0].size())) != BH_OK ||
+            (rc = scratch.alloc(&ddt[1], h.dt[1].size())) != BH_OK)
+            return rc;
+        for (int w = 0; w < 2; ++w) {
+            if ((rc = upload(ddt[w], h.dt[w], s)) != BH_OK) return rc;
+            for (int m0 = 0; m0 < NM; m0 += chunk) {
+                MrArgs ma{};
+                ma.brb = o->brb_d; ma.alphar = o->alphar; ma.betar = o->betar; ma.dt = ddt[w];
+                ma.numr = numr; ma.nri = nri; ma.nsteps = (int)h.dt[w].size();
+                ma.mode0 = m0; ma.nmodes = std::min(chunk, NM - m0);
+                ma.scratch_v = sv; ma.scratch_u = su; ma.scratch_g = sg;
+                ma.Mr64 = o->MrT64[w];
+                launch_build_mr(ma, s);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(s));
+        return BH_OK;
+    }
     if (!mixed) {
         // Thomas factors per pseudo-timestep (tridagr.F90:41-60, tridagz.F90:115-132)
         for (int w = 0; w < 2; ++w) {
@@ -558,7 +614,7 @@ int build_modeops(const HostGeo &h, int precision, ModeOps *o)
         for (int w = 0; w < 2; ++w) {
             if ((rc = upload(ddt[w], h.dt[w], s)) != BH_OK) return rc;
             for (int m0 = 0; m0 < NM; m0 += chunk) {
-                MrArgs ma;
+                MrArgs ma{};
                 ma.brb = o->brb_d; ma.alphar = o->alphar; ma.betar = o->betar; ma.dt = ddt[w];
                 ma.numr = numr; ma.nri = nri; ma.nsteps = (int)h.dt[w].size();
                 ma.mode0 = m0; ma.nmodes = std::min(chunk, NM - m0);
@@ -633,6 +689,7 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
         const bool fused = spectral_fused_ok((int)nri, (int)nzi);
         b += 4 * (M * N + (fused ? 1 : 5) * N * nri * nzi + (fused && d.refine_passes >= 2 ? 3 : 2) * M * N);
     }
+    else if (is_spec64(precision)) b += 8 * (M * N + 3 * N * nri * nzi);
     else b += 8 * 2 * M * N;
     return b;
 }

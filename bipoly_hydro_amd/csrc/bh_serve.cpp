@@ -681,8 +681,9 @@ extern "C" int bh_server_create(const bh_server_config *cfg, bh_server **out)
 {
     if (!cfg || !out) return fail(BH_E_INVALID, "null argument");
     *out = nullptr;
-    if (cfg->precision != BH_PREC_FP64 && !is_mixed(cfg->precision))
-        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64, BH_PREC_MIXED or BH_PREC_MIXED_TABLE32");
+    if (!valid_precision(cfg->precision))
+        return fail(BH_E_INVALID, "precision must be BH_PREC_FP64, BH_PREC_MIXED, BH_PREC_MIXED_TABLE32 or "
+                                  "BH_PREC_SPECTRAL64");
     std::unique_ptr<bh_server> S(new bh_server());
     S->cfg = *cfg;
     if (S->cfg.max_active <= 0) S->cfg.max_active = 64;

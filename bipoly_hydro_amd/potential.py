@@ -28,10 +28,13 @@ BH_OK = 0
 BH_PREC_FP64 = 0
 BH_PREC_MIXED = 1
 BH_PREC_MIXED_TABLE32 = 2
+BH_PREC_SPECTRAL64 = 3
 # "mixed": FP32 transforms and sweeps, every Green's table in FP64, FP64
 # residual and refinement; "mixed_t32": the same with the top/bottom table
-# stored in FP32 (an opt-in variant)
-_PREC = {"fp64": BH_PREC_FP64, "mixed": BH_PREC_MIXED, "mixed_t32": BH_PREC_MIXED_TABLE32}
+# stored in FP32 (an opt-in variant); "spectral64": the mixed solve's
+# spectral form of the iteration evaluated once in FP64 (isym = 1)
+_PREC = {"fp64": BH_PREC_FP64, "mixed": BH_PREC_MIXED, "mixed_t32": BH_PREC_MIXED_TABLE32,
+         "spectral64": BH_PREC_SPECTRAL64}
 
 EXPORTS = ("bh_plan_create", "bh_plan_destroy", "bh_solve", "bh_solve_device",
            "bh_set_potential", "bh_plan_device_bytes", "bh_plan_set_timing",
@@ -191,8 +194,9 @@ class PotentialPlan:
     def __init__(self, numr: int, numz: int, numphi: int, *, precision: str = "fp64",
                  hbm_budget: int = 0, dr: float = 1.0 / 127.0, mmax: int = 30,
                  nsteps_cold: int = 20, nsteps_warm: int = 5, isym: int = 1, refine_passes: int = 2):
-        """precision "fp64", "mixed" or "mixed_t32"; refine_passes (1..4) is
-        the number of FP64 residual + refinement passes of a mixed solve."""
+        """precision "fp64", "mixed", "mixed_t32" or "spectral64";
+        refine_passes (1..4) is the number of FP64 residual + refinement
+        passes of a mixed solve."""
         L = load_library()
         self.shape = (numphi, numz, numr)
         self.desc = GridDesc(numr, numz, numphi, isym, dr, mmax, nsteps_cold, nsteps_warm, refine_passes)

@@ -38,6 +38,12 @@ extern "C" {
  * about 1e-10 more error.  An opt-in variant; BH_PREC_MIXED keeps every
  * Green's table in FP64. */
 #define BH_PREC_MIXED_TABLE32 2
+/* The truncated Peaceman-Rachford iteration in its exact spectral form (the
+ * mixed solve's), every product and sweep in FP64 with no refinement: the
+ * same potential as BH_PREC_FP64 to rounding, by another sequence of
+ * kernels (sine transforms along z, one dense product per azimuthal mode
+ * along r).  isym = 1 only. */
+#define BH_PREC_SPECTRAL64 3
 
 /* Geometry and solver constants.  Replaces the compile-time parameters of
  * runhydro.h:1-30 and pot.h:3-7 (numr, numz, numphi; mmax = 30,
@@ -305,7 +311,7 @@ typedef struct bh_request_result {
 typedef struct bh_server_config {
     size_t hbm_budget;  /* device memory the server may hold; 0 -> 90% of what is free */
     int max_active;     /* requests on the device at once; 0 -> 64 */
-    int precision;      /* BH_PREC_MIXED or BH_PREC_FP64 */
+    int precision;      /* a BH_PREC_* value */
 } bh_server_config;
 
 typedef struct bh_server_stats {

@@ -43,7 +43,9 @@ def test_library_exports_every_declared_function():
     (dict(numr=600, numz=66, numphi=64), "<= 512"),
     (dict(numr=66, numz=66, numphi=64, passes=5), "refine_passes"),
     (dict(numr=66, numz=66, numphi=64, passes=-1), "refine_passes"),
-    (dict(numr=66, numz=66, numphi=64, precision=3), "precision"),
+    (dict(numr=66, numz=66, numphi=64, isym=2, precision=3), "need precision BH_PREC_FP64"),
+    (dict(numr=66, numz=66, numphi=64, precision=4), "precision"),
+    (dict(numr=66, numz=66, numphi=64, precision=-1), "precision"),
 ])
 def test_invalid_descriptors_are_rejected_without_a_gpu(desc, msg):
     lib = P.load_library()

@@ -157,6 +157,54 @@ def test_error_paths():
 
 
 # ---------------------------------------------------------------------------
+# The FP64 spectral solve (sine transforms along z, a dense product per mode
+# along r, one pass): the same truncated iteration as the FP64 path, so the
+# same bar against the reference.
+@pytest.mark.parametrize("name", ["g42x26x16", "g34x34x32", "c1_66x66x64"])
+def test_spectral64_matches_reference_cold_then_warm(golden_dir, name):
+    (numr, numz, numphi), rho1, rho2, ref1, ref2 = load_case(golden_dir, name)
+    with P.PotentialPlan(numr, numz, numphi, precision="spectral64") as plan:
+        pot1 = plan.potential_solver(rho1, tstep=0)
+        pot2 = plan.potential_solver(rho2, tstep=2)
+    e1, e2 = rell2(pot1, ref1), rell2(pot2, ref2)
+    print(f"spectral64 {name}: cold relL2 {e1:.3e} warm relL2 {e2:.3e}", flush=True)
+    assert e1 <= TOL_FP64 and e2 <= TOL_FP64
+    h = numphi // 2
+    np.testing.assert_array_equal(pot1[:h, :, 0], pot1[h:, :, 1])
+
+
+# grids whose nri and nzi are not multiples of the 16-wide MFMA tiles, and
+# a numphi = 512 grid (the panel DFTs)
+@pytest.mark.parametrize("shape,seed", [((98, 34, 64), 11), ((45, 28, 16), 4), ((34, 18, 512), 6)])
+def test_spectral64_matches_fp64_plan(shape, seed):
+    numr, numz, numphi = shape
+    rho1 = density.scaled_binary(numr, numz, numphi, seed)
+    rho2 = density.scaled_binary(numr, numz, numphi, seed, mass_ratio=0.5, separation=0.6)
+    out = {}
+    for prec in ("fp64", "spectral64"):
+        with P.PotentialPlan(numr, numz, numphi, precision=prec) as plan:
+            out[prec] = (plan.potential_solver(rho1, tstep=0), plan.potential_solver(rho2, tstep=3))
+    e1, e2 = (rell2(a, b) for a, b in zip(out["spectral64"], out["fp64"]))
+    print(f"spectral64 vs fp64 {shape}: cold {e1:.3e} warm {e2:.3e}", flush=True)
+    assert e1 <= 1e-12 and e2 <= 1e-12
+
+
+def test_spectral64_production_grid_against_reference_slices(golden_dir):
+    g = np.load(os.path.join(golden_dir, "c2_258x130x512.npz"))
+    rho1 = density.config2_density(seed=int(g["seeds"][0]))
+    rho2 = rho1 * 1.01 + 1e-3 * density.config2_density(seed=int(g["seeds"][1]))
+    with P.PotentialPlan(258, 130, 512, precision="spectral64") as plan:
+        pots = [plan.potential_solver(rho1, tstep=0), plan.potential_solver(rho2, tstep=2)]
+    for tag, pot in zip(("1", "2"), pots):
+        for k, v in c2_slices(pot, tag).items():
+            e = rell2(v, g[k])
+            print(f"spectral64 c2 solve {tag} {k}: relL2 {e:.3e}", flush=True)
+            assert e <= TOL_FP64, (tag, k, e)
+        if "plane_norm" + tag in g:
+            check_c2_digests(pot, g, tag, TOL_FP64)
+
+
+# ---------------------------------------------------------------------------
 # Mixed precision: FP32 MFMA transforms, FP64 residual and refinement.  Bar:
 # relative L2 <= 1e-8 against the reference's FP64 output.
 TOL_MIXED = 1e-8
