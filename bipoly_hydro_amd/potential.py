@@ -299,12 +299,17 @@ class PotentialPlan:
 
     def last_timings(self) -> dict:
         """Mean stage times (ms, HIP events on the solve stream) over the
-        solves timed since timing was switched on or last read."""
+        solves timed since timing was switched on or last read.  The
+        forward DFT of rho runs on a side stream beside the projections:
+        "projections" is the main stream's projection kernel, and
+        "boundary_and_forward_dft" the boundary contraction up to the join
+        with the side stream (its DFT and side contraction overlapped)."""
         L = load_library()
         _check(L.bh_sync_timing(self._h))
         ms = (ctypes.c_double * 6)()
         _check(L.bh_last_timings(self._h, ms, 6))
-        return dict(zip(("boundary", "forward_dft", "adi", "inverse_dft", "total", "contract_topbot"), list(ms)))
+        return dict(zip(("boundary_and_forward_dft", "projections", "adi", "inverse_dft", "total",
+                         "contract_topbot"), list(ms)))
 
 
 class PotentialBatch:
