@@ -21,7 +21,7 @@ struct Geo {
 // Line-sweep arguments for one ADI half-step (helmadi.F90:195-353).
 struct LineArgs {
     const double *xin;   // potential, input layout
-    double *xout;        // potential, output (transposed) layout
+    double *xout;        // potential, output (transposed) layout; its face rows give the lines' end values
     const double *rh;    // -> rho-hat, input layout
     long SA, PA;         // input: plane stride, row pitch
     long SB, PB;         // output: plane stride, row pitch
@@ -36,7 +36,27 @@ struct LineArgs {
     // isym = 2, 3: K = 2 borders the equator, whose mirror cell K = 1 takes
     // K = 2's value (helmadi.F90:211-238, 334-340)
     int eqsym;
+    // Compact interiors between the half-steps: element e of interior line
+    // i read at xt[i * P(len) + e] (plane stride STi) and written at
+    // yt[e * P(nlines) + i] (plane stride STo), P = compact_pitch, so that a
+    // workgroup's 16 lines write whole aligned 128-byte lines.  The faces (and, unless in_tiled, the interior) stay
+    // in xin's layout; out_tiled = 0 writes xout.  cold: xin's interior is
+    // zero (a cold start) and is not read.
+    const double *xt;
+    double *yt;
+    long STi, STo;
+    int in_tiled, out_tiled, cold;
+    // rh in the same compact form as xt (plane stride STr)
+    int rh_compact;
+    long STr;
 };
+// Row pitch (doubles) of a compact interior with rows of n elements: n
+// rounded up to 16 (aligned 128-byte lines) plus 16, so that the rows of a
+// plane do not sit a power of two apart (rows 1 and 2 KB apart, as the 128
+// and 256 of the 258 x 130 grid would be, ran its half-steps 4-7 % slower).
+__host__ __device__ inline long compact_pitch(int n) { return (long)(n + 15) / 16 * 16 + 16; }
+// Plane stride of a compact interior read as nlines lines of len elements.
+__host__ __device__ inline long tiled_stride(int nlines, int len) { return (long)nlines * compact_pitch(len); }
 
 struct FillArgs {
     double *p;
@@ -88,6 +108,7 @@ void launch_fill(double *p, size_t n, double v, hipStream_t s);
 // covers `capacity` (n may be 0: same grid, no work).
 Op fill_op(double *p, size_t n, size_t capacity, double v);
 // [plane][K][J] -> [plane][J][K]
+// J layout -> compact K-layout interior (rows of compact_pitch(numz - 2))
 Op transpose_op(const double *src, double *dst, int numr, int numz, int nplanes);
 // one ADI half-step; nl = lines per workgroup (8 or 16, 0: default)
 Op sweep_op(const LineArgs &a, int nplanes, int nl);

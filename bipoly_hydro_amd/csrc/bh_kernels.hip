@@ -48,30 +48,33 @@ __device__ __forceinline__ void fill_tile(const FillArgs &a, int bx, int, int)
 }
 
 // [plane][K][J] -> [plane][J][K]
+// J layout -> the compact K-layout interior of LineArgs (rows J' = J - 2 of
+// compact_pitch(numz - 2) doubles, element K' = K - 2): 32 x 32 tiles of
+// interior points through LDS, each output row segment 256 aligned bytes
 __device__ __forceinline__ void transpose_tile(const TransposeArgs &a, int bx, int by, int bz)
 {
     __shared__ double tile[32][33];
-    const int numr = a.numr, numz = a.numz;
-    const long plane = (long)numr * numz;
-    const double *s = a.src + (long)bz * plane;
-    double *d = a.dst + (long)bz * plane;
+    const int numr = a.numr, numz = a.numz, nri = numr - 2, nzi = numz - 2;
+    const long pk = compact_pitch(nzi);
+    const double *s = a.src + (long)bz * numr * numz;
+    double *d = a.dst + (long)bz * tiled_stride(nri, nzi);
     int j0 = bx * 32, k0 = by * 32;
     int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (int r = ty; r < 32; r += 8) {
         int k = k0 + r, j = j0 + tx;
-        if (k < numz && j < numr) tile[r][tx] = s[(long)k * numr + j];
+        if (k < nzi && j < nri) tile[r][tx] = s[(long)(k + 1) * numr + j + 1];
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         int j = j0 + r, k = k0 + tx;
-        if (k < numz && j < numr) d[(long)j * numz + k] = tile[tx][r];
+        if (k < nzi && j < nri) d[(long)j * pk + k] = tile[tx][r];
     }
 }
 
 Op transpose_op(const double *src, double *dst, int numr, int numz, int nplanes)
 {
     TransposeArgs a{src, dst, numr, numz};
-    return Op::make(OP_TRANSPOSE, 0, a, (numr + 31) / 32, (numz + 31) / 32, nplanes);
+    return Op::make(OP_TRANSPOSE, 0, a, (numr - 2 + 31) / 32, (numz - 2 + 31) / 32, nplanes);
 }
 
 Op fill_op(double *p, size_t n, size_t capacity, double v)
@@ -81,15 +84,16 @@ Op fill_op(double *p, size_t n, size_t capacity, double v)
 }
 
 // ---------------------------------------------------------------------------
-// ADI half-step, one wavefront per tridiagonal line.
+// ADI half-step, one wavefront (or half of one) per tridiagonal line.
 // The Thomas forward sweep y_e = alpha_e y_{e-1} + ibet_e rhs_e and back
 // substitution x_e = y_e + gback_e x_{e+1} are first-order linear
 // recurrences: each lane composes the affine maps of its own consecutive
-// elements, an inclusive wavefront scan of those maps (DPP row shifts inside
-// rows of 16 lanes, readlane across rows) joins the lanes, and each lane then
-// finishes its elements from the value entering it.  A wave owns consecutive
-// lines and rolls the neighbour rows in registers, loading the row two lines
-// ahead while it solves.
+// elements, an inclusive scan of those maps over the line's lanes (DPP row
+// shifts inside rows of 16 lanes, row broadcasts upwards, readlane
+// downwards) joins the lanes, and each lane then finishes its elements from
+// the value entering it (DPP wave shifts).  A wave owns consecutive lines
+// and rolls the neighbour rows in registers, loading the next line's rows
+// while it solves.
 // ---------------------------------------------------------------------------
 template <int CTRL>
 __device__ inline double dpp_d(double old, double v)
@@ -115,23 +119,36 @@ __device__ inline void compose(double &A, double &B, double Ap, double Bp)
     A = A * Ap;
 }
 
-// inclusive scan from lane 0 upwards
-__device__ inline void scan_up(double &A, double &B, int lane)
+// dpp_d with a row mask: rows outside ROWS keep `old`
+template <int CTRL, int ROWS>
+__device__ inline double dppm_d(double old, double v)
+{
+    long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROWS, 0xf, false);
+    int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROWS, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Scans over the lanes of a line: the whole wave (H = 1) or each half of 32
+// lanes (H = 2, two lines side by side).
+// Inclusive scan upwards: row shifts inside the rows of 16 lanes, then lane
+// 15 broadcast to row 1 and lane 47 to row 3 (row_bcast:15), and for H = 1
+// lane 31 to rows 2 and 3 (row_bcast:31); rows left out compose with the
+// identity.
+template <int H>
+__device__ inline void scan_up(double &A, double &B)
 {
     compose(A, B, dpp_d<0x111>(1.0, A), dpp_d<0x111>(0.0, B));  // row_shr:1
     compose(A, B, dpp_d<0x112>(1.0, A), dpp_d<0x112>(0.0, B));  // row_shr:2
     compose(A, B, dpp_d<0x114>(1.0, A), dpp_d<0x114>(0.0, B));  // row_shr:4
     compose(A, B, dpp_d<0x118>(1.0, A), dpp_d<0x118>(0.0, B));  // row_shr:8
-    const int row = lane >> 4;
-    double A15 = readlane_d(A, 15), B15 = readlane_d(B, 15);
-    double A47 = readlane_d(A, 47), B47 = readlane_d(B, 47);
-    if (row == 1) compose(A, B, A15, B15);
-    if (row == 3) compose(A, B, A47, B47);
-    double A31 = readlane_d(A, 31), B31 = readlane_d(B, 31);
-    if (row >= 2) compose(A, B, A31, B31);
+    compose(A, B, dppm_d<0x142, 0xa>(1.0, A), dppm_d<0x142, 0xa>(0.0, B));  // row_bcast:15
+    if (H == 1) compose(A, B, dppm_d<0x143, 0xc>(1.0, A), dppm_d<0x143, 0xc>(0.0, B));  // row_bcast:31
 }
 
-// inclusive scan from lane 63 downwards
+// inclusive scan downwards: row shifts, then the first lane of row 1 (of
+// row 3) into row 0 (row 2), and for H = 1 lane 32 into rows 0 and 1
+template <int H>
 __device__ inline void scan_down(double &A, double &B, int lane)
 {
     compose(A, B, dpp_d<0x101>(1.0, A), dpp_d<0x101>(0.0, B));  // row_shl:1
@@ -143,27 +160,48 @@ __device__ inline void scan_down(double &A, double &B, int lane)
     double A48 = readlane_d(A, 48), B48 = readlane_d(B, 48);
     if (row == 0) compose(A, B, A16, B16);
     if (row == 2) compose(A, B, A48, B48);
-    double A32 = readlane_d(A, 32), B32 = readlane_d(B, 32);
-    if (row <= 1) compose(A, B, A32, B32);
+    if (H == 1) {
+        double A32 = readlane_d(A, 32), B32 = readlane_d(B, 32);
+        if (row <= 1) compose(A, B, A32, B32);
+    }
 }
 
-// LS_LINES lines per workgroup (4 waves, LS_LINES/4 consecutive lines each);
-// LDS tile [LS_LINES][64*V + 1] doubles for the transposed write.  Blocked
-// layout: lane l owns the V consecutive elements e = l*V .. l*V+V-1, composes
-// their affine maps locally, and one wavefront scan per direction joins the
-// lanes.
-template <int V, int LS_LINES>
+// the value of lane - 1 (wave_shr:1) and of lane + 1 (wave_shl:1), 0 for
+// the first (last) lane of a line
+template <int H>
+__device__ inline double from_lane_below(double v, int ll)
+{
+    const double r = dpp_d<0x138>(0.0, v);
+    return (H == 2 && ll == 0) ? 0.0 : r;
+}
+template <int H>
+__device__ inline double from_lane_above(double v, int ll)
+{
+    const double r = dpp_d<0x130>(0.0, v);
+    return (H == 2 && ll == 31) ? 0.0 : r;
+}
+
+// LS_LINES lines per workgroup (4 waves, LS_LINES/4 consecutive lines
+// each), a line on all 64 lanes of a wave (H = 1) or two lines side by side
+// on its halves (H = 2, each half LS_LINES/8 consecutive lines); LDS tile
+// [LS_LINES][(64/H)*V + 1] doubles for the transposed write.  Blocked
+// layout: lane ll of a line owns the V consecutive elements e = ll*V ..
+// ll*V+V-1, composes their affine maps locally, and one scan per direction
+// joins the lanes.
+template <int V, int LS_LINES, int DIR, int H>
 __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int by, double *sres)
 {
-    constexpr int LPW = LS_LINES / 4;
+    constexpr int LPW = LS_LINES / 4, LPH = LPW / H, W = 64 / H;
+    static_assert(LPH * H == LPW, "lines per wave must split over the halves");
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int hl = lane / W, ll = lane % W;  // half of the wave, lane within the line
     const int L = by;  // mode plane
     const int line0 = bx * LS_LINES;
-    const int lenp = 64 * V + 1;
+    const int lenp = W * V + 1;
     const double *xin = a.xin + (long)L * a.SA;
     const double *rh = a.rh + (long)L * a.SA;
     const int len = a.len;
-    const int e0 = lane * V;  // first element of this lane
+    const int e0 = ll * V;  // first element of this lane
     const double dtt_mtwogamma = a.dt - 2.0 * a.gamma;
 
     // Thomas factors: the same for every line of this mode plane
@@ -181,9 +219,11 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
             fg[v] = ok ? gback[e] : 0.0;
         }
     }
-    const int first = line0 + wave * LPW;          // first line of this wave
-    const int nmine = min(LPW, a.nlines - first);  // may be <= 0
-    double xm[V], xc[V], xn[V], rc[V], rn[V];
+    // this half's lines first .. first + nmine - 1; lines past the end are
+    // solved on clamped rows (every lane takes part in the scans) and not
+    // stored
+    const int first = line0 + wave * LPW + hl * LPH;
+    const int nmine = min(LPH, a.nlines - first);  // may be <= 0
     auto load_row = [&](const double *row, double *dst) {
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -191,54 +231,87 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
             dst[v] = (e < len) ? row[e + 1] : 0.0;
         }
     };
-    if (nmine > 0) {
-        load_row(xin + (long)first * a.PA, xm);
-        load_row(xin + (long)(first + 1) * a.PA, xc);
-        load_row(xin + (long)(first + 2) * a.PA, xn);
-        load_row(rh + (long)(first + 1) * a.PA, rc);
-    }
-    for (int k = 0; k < nmine; ++k) {
-        const int i = first + k;  // interior line, row i+1 of the input layout
-        const double *x0 = xin + (long)(i + 1) * a.PA;
-        const double b0 = x0[0], b1 = x0[len + 1];  // boundary values of this line
-        double xnn[V];
-        const bool more = (k + 1 < nmine);
-        if (more) {
-            load_row(xin + (long)(i + 3) * a.PA, xnn);
-            load_row(rh + (long)(i + 2) * a.PA, rn);
-        }
-        double cd, cp, cn;
-        const bool eqrow = a.dir == 0 && a.eqsym && i == 0;  // K = 2
-        if (a.dir == 0) {
-            cd = dtt_mtwogamma; cp = a.gamma; cn = a.gamma;
+    // row q of the potential: q = 0 and nlines + 1 are the faces, the
+    // interior rows come from the compact input or (zero when cold) from xin
+    const double *xt = a.xt + (long)L * a.STi;
+    const double *xf = a.xout + (long)L * a.SB;
+    const long pin = compact_pitch(len);
+    auto load_x = [&](int q, double *dst) {
+        q = min(q, a.nlines + 1);
+        if (q >= 1 && q <= a.nlines && (a.in_tiled || a.cold)) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const int e = e0 + v;
+                dst[v] = (e < len && a.in_tiled) ? xt[(q - 1) * pin + e] : 0.0;
+            }
         } else {
+            load_row(xin + (long)q * a.PA, dst);
+        }
+    };
+    // rho-hat row q (an interior line): the input layout, or the compact
+    // interior (rh_compact; rows of compact_pitch(len), plane stride STr)
+    auto load_rh = [&](int q, double *dst) {
+        q = min(q, a.nlines);
+        if (a.rh_compact) {
+            const double *row = a.rh + (long)L * a.STr + (long)(q - 1) * pin;
+#pragma unroll
+            for (int v = 0; v < V; ++v) dst[v] = (e0 + v < len) ? row[e0 + v] : 0.0;
+        } else {
+            load_row(rh + (long)q * a.PA, dst);
+        }
+    };
+    double xm[V], xc[V], xn[V], rc[V], rn[V];
+    load_x(first, xm);
+    load_x(first + 1, xc);
+    load_x(first + 2, xn);
+    load_rh(first + 1, rc);
+    // the right-hand side per element: t = -4 pi rc + cc xc + cn xn + cm xm,
+    // plus wl * eb1 at the line's last element and wf * eb0 at its first
+    // (lane weights 0 or 1, formed once; padding elements load as 0)
+    double wl[V], wf[DIR == 1 ? V : 1];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        wl[v] = (e0 + v == len - 1) ? 1.0 : 0.0;
+        if (DIR == 1) wf[v] = (e0 + v == 0 && !a.eqsym) ? 1.0 : 0.0;
+    }
+    for (int k = 0; k < LPH; ++k) {
+        const int i = min(first + k, a.nlines - 1);  // interior line, row i+1 of the input layout
+        // the boundary values at the ends of this line, from the face rows
+        // of the other layout (xout), where the lines' ends are contiguous
+        const double b0 = DIR == 1 ? xf[i + 1] : 0.0, b1 = xf[(long)(len + 1) * a.PB + i + 1];
+        double xnn[V];
+        const bool more = k + 1 < LPH;
+        if (more) {
+            load_x(first + k + 3, xnn);
+            load_rh(first + k + 2, rn);
+        }
+        double cc, cm, cn, eb0 = 0.0, eb1;
+        if (DIR == 0) {
+            // knownr (helmadi.F90:240-257); the row next to the equator
+            // (isym = 2, 3) leaves out its mirror K = 1 and weights K = 2
+            // by dt - gamma (helmadi.F90:228-236)
+            const bool eqrow = a.eqsym && i == 0;  // K = 2
+            cc = eqrow ? a.dt - a.gamma : dtt_mtwogamma;
+            cm = eqrow ? 0.0 : a.gamma;
+            cn = a.gamma;
+            eb1 = -a.alphar[a.numr - 1] * b1;
+        } else {
+            // knownz (helmadi.F90:284-340); the bottom face enters for
+            // isym = 1 only
             const int J = i + 2;
-            cd = a.elzb[(long)L * (a.numr + 1) + J] + a.dt;
+            cc = a.elzb[(long)L * (a.numr + 1) + J] + a.dt;
             cn = -a.alphar[J];
-            cp = (J > 2) ? -a.betar[J] : 0.0;
+            cm = (J > 2) ? -a.betar[J] : 0.0;
+            eb0 = a.gamma * b0;
+            eb1 = a.gamma * b1;
         }
         double val[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            const int e = e0 + v;
-            double t = 0.0;
-            if (e < len) {
-                if (a.dir == 0) {
-                    // knownr (helmadi.F90:240-257); the row next to the
-                    // equator (isym = 2, 3) leaves out its mirror K = 1
-                    // and weights K = 2 by dt - gamma (helmadi.F90:228-236)
-                    if (eqrow) t = -a.four_pi * rc[v] + (a.dt - a.gamma) * xc[v] + a.gamma * xn[v];
-                    else t = -a.four_pi * rc[v] + cd * xc[v] + cp * (xn[v] + xm[v]);
-                    if (e == len - 1) t = t - a.alphar[a.numr - 1] * b1;
-                } else {
-                    // knownz (helmadi.F90:284-340); the bottom face enters
-                    // for isym = 1 only
-                    t = -a.four_pi * rc[v] + cd * xc[v] + cn * xn[v];
-                    if (cp != 0.0) t = t + cp * xm[v];
-                    if (e == len - 1) t = t + a.gamma * b1;
-                    if (e == 0 && !a.eqsym) t = t + a.gamma * b0;
-                }
-            }
+            double t = -a.four_pi * rc[v] + cc * xc[v] + cn * xn[v];
+            t = fma(cm, xm[v], t);
+            t = fma(wl[v], eb1, t);
+            if (DIR == 1) t = fma(wf[v], eb0, t);
             val[v] = t;
         }
         // forward sweep y_e = fa_e y_{e-1} + fi_e rhs_e: local composition,
@@ -251,9 +324,8 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
                 B = fa[v] * B + b;
                 A = fa[v] * A;
             }
-            scan_up(A, B, lane);
-            double yin = __shfl_up(B, 1, 64);  // y of the element before this lane's first
-            if (lane == 0) yin = 0.0;
+            scan_up<H>(A, B);
+            double yin = from_lane_below<H>(B, ll);  // y of the element before this lane's first
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 yin = fa[v] * yin + fi[v] * val[v];
@@ -268,18 +340,18 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
                 B = fg[v] * B + val[v];
                 A = fg[v] * A;
             }
-            scan_down(A, B, lane);
-            double xin_ = __shfl_down(B, 1, 64);  // x of the element after this lane's last
-            if (lane == 63) xin_ = 0.0;
+            scan_down<H>(A, B, lane);
+            double xin_ = from_lane_above<H>(B, ll);  // x of the element after this lane's last
 #pragma unroll
             for (int v = V - 1; v >= 0; --v) {
                 xin_ = fg[v] * xin_ + val[v];
                 val[v] = xin_;
             }
         }
-        const int jl = wave * LPW + k;
+        const int jl = first - line0 + k;
+        if (k < nmine)
 #pragma unroll
-        for (int v = 0; v < V; ++v) sres[jl * lenp + e0 + v] = val[v];
+            for (int v = 0; v < V; ++v) sres[jl * lenp + e0 + v] = val[v];
         // roll the neighbour rows
 #pragma unroll
         for (int v = 0; v < V; ++v) {
@@ -292,8 +364,19 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
         }
     }
     __syncthreads();
-    // transposed, coalesced write: element e of line i -> out[(e+1)*PB + i+1]
     const int nl = min(LS_LINES, a.nlines - line0);
+    if (a.out_tiled) {
+        // element e of line line0 + j: 16 lines fill one aligned 128-byte
+        // line of each output row
+        const long pout = compact_pitch(a.nlines);
+        double *yt = a.yt + (long)L * a.STo + line0;
+        for (int t = threadIdx.x; t < LS_LINES * len; t += 256) {
+            int j = t % LS_LINES, e = t / LS_LINES;
+            if (j < nl) yt[e * pout + j] = sres[j * lenp + e];
+        }
+        return;
+    }
+    // transposed, coalesced write: element e of line i -> out[(e+1)*PB + i+1]
     double *xout = a.xout + (long)L * a.SB;
     for (int t = threadIdx.x; t < LS_LINES * len; t += 256) {
         int j = t % LS_LINES, e = t / LS_LINES;
@@ -301,30 +384,41 @@ __device__ __forceinline__ void line_sweep_tile(const LineArgs &a, int bx, int b
     }
 }
 
-template <int V, int NL>
+template <int V, int NL, int DIR, int H>
 __global__ __launch_bounds__(256) void line_sweep_kernel(LineArgs a, int nx, int ny, int total)
 {
     extern __shared__ double sres[];
-    BH_SINGLE_LOOP(a, nx, ny, total, (line_sweep_tile<V, NL>(a, bx, by, sres)));
+    BH_SINGLE_LOOP(a, nx, ny, total, (line_sweep_tile<V, NL, DIR, H>(a, bx, by, sres)));
 }
 
-template <int V, int NL>
+template <int V, int NL, int DIR, int H>
 __global__ __launch_bounds__(256) void line_sweep_grouped(const LineArgs *__restrict__ args, GroupSched gs)
 {
     extern __shared__ double sres[];
-    BH_GROUP_LOOP(LineArgs, args, gs, (line_sweep_tile<V, NL>(ga, bx, by, sres)));
+    BH_GROUP_LOOP(LineArgs, args, gs, (line_sweep_tile<V, NL, DIR, H>(ga, bx, by, sres)));
 }
 
 // variant = V * 100 + NL: V elements per lane (line length <= 64 V), NL
 // lines per workgroup
-static int sweep_lds(int variant) { return (variant % 100) * (64 * (variant / 100) + 1) * (int)sizeof(double); }
+// variant = DIR * 10000 + H * 1000 + V * 100 + NL
+static int sweep_lds(int variant)
+{
+    const int nl = variant % 100, v = (variant / 100) % 10, h = (variant / 1000) % 10;
+    return nl * ((64 / h) * v + 1) * (int)sizeof(double);
+}
 
 Op sweep_op(const LineArgs &a, int nplanes, int nl)
 {
-    const int c = (a.len + 63) / 64;
+    // lines of up to 128 elements two to a wave, longer ones one to a wave
+    const int H = a.len <= 128 ? 2 : 1;
+    const int c = (a.len + 64 / H - 1) / (64 / H);
     const int V = c <= 1 ? 1 : c <= 2 ? 2 : c <= 4 ? 4 : 8;
     if (nl != 8 && nl != 16) nl = (V <= 4) ? 16 : 8;
-    return Op::make(OP_SWEEP, V * 100 + nl, a, (a.nlines + nl - 1) / nl, nplanes, 1);
+    // the compact layouts take whole 128-byte segments of 16 lines: 16 lines
+    // per workgroup, 32 for lines two to a wave
+    if (a.in_tiled || a.out_tiled) nl = H == 2 ? 32 : 16;
+    return Op::make(OP_SWEEP, (a.dir ? 10000 : 0) + H * 1000 + V * 100 + nl, a, (a.nlines + nl - 1) / nl, nplanes,
+                    1);
 }
 
 void launch_line_sweep(const LineArgs &a, int nplanes, hipStream_t s) { launch_op(sweep_op(a, nplanes, 0), s); }
@@ -1386,11 +1480,15 @@ bool kernel_ref(int kind, int variant, KernelRef *k, int *lds)
     case OP_SWEEP:
         *lds = sweep_lds(variant);
         switch (variant) {
-#define BH_SW(V, NL)                                                                              \
-    case V * 100 + NL:                                                                            \
-        *k = {(const void *)&line_sweep_kernel<V, NL>, (const void *)&line_sweep_grouped<V, NL>, 256}; \
+#define BH_SW(V, NL, D, H)                                                                       \
+    case D * 10000 + H * 1000 + V * 100 + NL:                                                    \
+        *k = {(const void *)&line_sweep_kernel<V, NL, D, H>, (const void *)&line_sweep_grouped<V, NL, D, H>, 256}; \
         return true;
-            BH_SW(1, 8) BH_SW(1, 16) BH_SW(2, 8) BH_SW(2, 16) BH_SW(4, 8) BH_SW(4, 16) BH_SW(8, 8) BH_SW(8, 16)
+            BH_SW(4, 8, 0, 1) BH_SW(4, 16, 0, 1) BH_SW(8, 8, 0, 1) BH_SW(8, 16, 0, 1)
+            BH_SW(1, 8, 0, 2) BH_SW(1, 16, 0, 2) BH_SW(2, 8, 0, 2) BH_SW(2, 16, 0, 2) BH_SW(4, 8, 0, 2) BH_SW(4, 16, 0, 2)
+            BH_SW(4, 8, 1, 1) BH_SW(4, 16, 1, 1) BH_SW(8, 8, 1, 1) BH_SW(8, 16, 1, 1)
+            BH_SW(1, 8, 1, 2) BH_SW(1, 16, 1, 2) BH_SW(2, 8, 1, 2) BH_SW(2, 16, 1, 2) BH_SW(4, 8, 1, 2) BH_SW(4, 16, 1, 2)
+            BH_SW(1, 32, 0, 2) BH_SW(2, 32, 0, 2) BH_SW(4, 32, 0, 2) BH_SW(1, 32, 1, 2) BH_SW(2, 32, 1, 2) BH_SW(4, 32, 1, 2)
 #undef BH_SW
         default:
             return false;

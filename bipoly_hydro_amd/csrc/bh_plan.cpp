@@ -101,7 +101,9 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
         ALLOC(p->U64, PI);
     } else {
         ALLOC(p->RJ, M * N);
-        ALLOC(p->RK, M * N);
+        ALLOC(p->RK, (size_t)N * tiled_stride(nri, nzi));
+        ALLOC(p->TJ, (size_t)N * tiled_stride(nzi, nri));
+        ALLOC(p->TK, (size_t)N * tiled_stride(nri, nzi));
     }
 #undef ALLOC
     p->bytes = p->mem.bytes + p->tab->mem.bytes + p->ops->mem.bytes;
@@ -425,18 +427,24 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     out->push_back(Op::mark(MARK_SIDE_END));
     proj64_ops(p, d_rho, out);
     out->push_back(Op::mark(1));
-    // 2. starting potential: zero (cold, potential_solver.F90:88-93) or the
-    //    previous solution kept in Fourier space (warm); then the boundary
-    //    potential -> Fourier-space faces, placed in both layouts
-    out->push_back(fill_op(p->XJ, warm ? 0 : (size_t)M * N, (size_t)M * N, 0.0));
+    // 2. starting potential: zero (cold, potential_solver.F90:88-93: the
+    //    first half-step reads no interior) or the previous solution kept in
+    //    Fourier space (warm); then the boundary potential -> Fourier-space
+    //    faces, placed in both layouts
     boundary_ops(p, p->XJ, p->XK, 1, out);
     out->push_back(Op::mark(2));
-    // 4. Peaceman-Rachford iterations (helmadi.F90:195-355)
+    // 4. Peaceman-Rachford iterations (helmadi.F90:195-355).  Between the
+    //    half-steps the interior lives in the tiled buffers (the r half-step
+    //    writes TK, the z half-step TJ); the first half-step reads XJ, the
+    //    last writes XJ, the warm-start state; the faces stay in XJ and XK.
     const std::vector<double> &dt = o.dt[w];
-    for (int st = 0; st < (int)dt.size(); ++st) {
+    const int nst = (int)dt.size();
+    for (int st = 0; st < nst; ++st) {
         LineArgs r{};
         r.xin = p->XJ; r.xout = p->XK; r.rh = p->RJ;
         r.SA = M; r.PA = numr; r.SB = M; r.PB = numz;
+        r.xt = p->TJ; r.STi = tiled_stride(nzi, nri); r.in_tiled = st > 0; r.cold = st == 0 && !warm;
+        r.yt = p->TK; r.STo = tiled_stride(nri, nzi); r.out_tiled = 1;
         r.nlines = nzi; r.len = nri;
         r.alpha = o.rc_alpha[w] + (size_t)st * N * nri;
         r.ibet = o.rc_ibet[w] + (size_t)st * N * nri;
@@ -449,6 +457,9 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
         LineArgs z = r;
         z.xin = p->XK; z.xout = p->XJ; z.rh = p->RK;
         z.PA = numz; z.PB = numr;
+        z.xt = p->TK; z.STi = tiled_stride(nri, nzi); z.in_tiled = 1; z.cold = 0;
+        z.rh_compact = 1; z.STr = tiled_stride(nri, nzi);
+        z.yt = p->TJ; z.STo = tiled_stride(nzi, nri); z.out_tiled = st + 1 < nst;
         z.nlines = nri; z.len = nzi;
         z.alpha = o.zc_alpha[w] + (size_t)st * nzi;
         z.ibet = o.zc_ibet[w] + (size_t)st * nzi;
@@ -731,7 +742,7 @@ extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb
     return BH_OK;
 }
 
-// which: 0 RJ, 1 RK, 2 XJ, 3 XK, 4 proj, 5 T (blocked symmetric), 6 S, 7 Stb, 8 Ss, 9 (unused), 10 FF
+// which: 0 RJ, 1 RK (compact K-layout interior), 2 XJ, 3 XK, 4 proj, 5 T (blocked symmetric), 6 S, 7 Stb, 8 Ss, 9 (unused), 10 FF
 extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
 {
     if (!p || !host) return fail(BH_E_INVALID, "null argument");
@@ -739,7 +750,7 @@ extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
     const size_t M = (size_t)g.numr * g.numz, N = g.numphi, mm = g.mmax, nri = g.nri, nzi = g.nzi;
     const size_t nf = 2 * nri + nzi;
     const double *src[] = {p->RJ, p->RK, p->XJ, p->XK, p->proj, p->tab->T, p->tab->S, p->Stb, p->Ss, nullptr, p->FF};
-    const size_t cnt[] = {M * N, M * N, M * N, M * N, M * (2 * mm - 1), topsym_table_elems(g.numr, g.numz, g.mmax),
+    const size_t cnt[] = {M * N, p->RK ? N * tiled_stride(g.nri, g.nzi) : 0, M * N, M * N, M * (2 * mm - 1), topsym_table_elems(g.numr, g.numz, g.mmax),
                           mm * (2 * nzi - 1) * nri, mm * 4 * nri, side_splits(g.numz) * mm * 2 * nzi, N * nf, N * nf};
     if (which < 0 || which > 10) return fail(BH_E_INVALID, "which");
     if (!src[which]) return fail(BH_E_STATE, "buffer not used by this plan's precision");

@@ -140,6 +140,9 @@ struct bh_plan {
     // FP64 spectral solve: r0 then delta, S r0, its product with M_r^T, each
     // compact [L][k or K][J-2]
     double *R64 = nullptr, *T64 = nullptr, *U64 = nullptr;
+    // FP64 path: the interior between ADI half-steps in the tiled layouts of
+    // LineArgs (TK after the r half-step, TJ after the z half-step)
+    double *TJ = nullptr, *TK = nullptr;
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
     // the source step (bh_hydro.hip): r, rhf, rinv, rhfinv [4][numr + 1]
     // (1-based J), and the centre-of-mass arrays cylin_a_com, cylin_v_com

@@ -690,7 +690,7 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
         b += 4 * (M * N + (fused ? 1 : 5) * N * nri * nzi + (fused && d.refine_passes >= 2 ? 3 : 2) * M * N);
     }
     else if (is_spec64(precision)) b += 8 * (M * N + 3 * N * nri * nzi);
-    else b += 8 * 2 * M * N;
+    else b += 8 * (M * N + N * (2 * tiled_stride((int)nri, (int)nzi) + tiled_stride((int)nzi, (int)nri)));
     return b;
 }
 
