@@ -417,13 +417,11 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     const bool warm = tstep > 1;
     const int w = warm ? 1 : 0;
     out->push_back(Op::mark(0));
-    // 1. rho-hat (with rhop's zeroed layers, potential_solver.F90:70-78) and
-    //    its K-layout copy on the side stream, on half the CUs, beside the
-    //    boundary projections of rho (bessel.F90:179-185, 228-237) and the
-    //    boundary
+    // 1. rho-hat (with rhop's zeroed layers, potential_solver.F90:70-78) on
+    //    the side stream, on half the CUs, beside the boundary projections
+    //    of rho (bessel.F90:179-185, 228-237) and the boundary
     out->push_back(Op::mark(MARK_FORK));
     dft_fwd64_ops(p, d_rho, M, (int)M, p->RJ, M, true, out, side_blocks(p));
-    out->push_back(transpose_op(p->RJ, p->RK, numr, numz, N));
     out->push_back(Op::mark(MARK_SIDE_END));
     proj64_ops(p, d_rho, out);
     out->push_back(Op::mark(1));
@@ -433,6 +431,10 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     //    faces, placed in both layouts
     boundary_ops(p, p->XJ, p->XK, 1, out);
     out->push_back(Op::mark(2));
+    // 3. rho-hat's compact K-layout interior for the z half-steps, after the
+    //    boundary rather than beside it on the side stream (there it slowed
+    //    the FP64-table contraction by more than it takes alone)
+    out->push_back(transpose_op(p->RJ, p->RK, numr, numz, N));
     // 4. Peaceman-Rachford iterations (helmadi.F90:195-355).  Between the
     //    half-steps the interior lives in the tiled buffers (the r half-step
     //    writes TK, the z half-step TJ); the first half-step reads XJ, the
