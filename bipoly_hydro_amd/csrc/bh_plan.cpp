@@ -430,11 +430,11 @@ void fp64_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, s
     //    Fourier space (warm); then the boundary potential -> Fourier-space
     //    faces, placed in both layouts
     boundary_ops(p, p->XJ, p->XK, 1, out);
-    out->push_back(Op::mark(2));
     // 3. rho-hat's compact K-layout interior for the z half-steps, after the
     //    boundary rather than beside it on the side stream (there it slowed
     //    the FP64-table contraction by more than it takes alone)
     out->push_back(transpose_op(p->RJ, p->RK, numr, numz, N));
+    out->push_back(Op::mark(2));
     // 4. Peaceman-Rachford iterations (helmadi.F90:195-355).  Between the
     //    half-steps the interior lives in the tiled buffers (the r half-step
     //    writes TK, the z half-step TJ); the first half-step reads XJ, the
