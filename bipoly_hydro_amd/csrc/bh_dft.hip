@@ -811,8 +811,12 @@ struct R4Fwd {
     static constexpr int KPp = 16 * R4KG + Pad<T>::v;
     static constexpr size_t uv = (size_t)2 * VC * KPp * sizeof(T);          // U, V (then Fa, Fs)
     static constexpr size_t f = (size_t)2 * (R4N / 2 + 1) * VC * sizeof(T);  // Fa, Fs
-    static constexpr size_t bytes = (uv > f ? uv : f) + 3 * VC * sizeof(double);
+    static constexpr size_t tw = (size_t)(R4N + 1) * 8 * sizeof(T);  // twiddles (cos, sin)(2 pi k r / N)
+    static constexpr size_t bytes = (uv > f ? uv : f) + 3 * VC * sizeof(double) + tw;
     static constexpr int frags = 8 * R4KG * 64 * 4;  // size-n cosine and sine fragments
+    static constexpr int JG = 512 / CB;              // thread groups along j in the fold
+    static constexpr int NJ = (R4N / 2) / JG;        // fold rows j = jg + JG i per thread
+    static_assert(512 % CB == 0 && JG * NJ == R4N / 2, "the fold's thread map covers rows 0..n/2-1");
 };
 
 template <typename T>
@@ -825,6 +829,7 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     T *U = (T *)lds, *V = U + VC * KPp;
     T *Fa = (T *)lds, *Fs = Fa + (R4N / 2 + 1) * VC;  // after the products, over U and V
     double *X0 = (double *)(lds + (Ly::uv > Ly::f ? Ly::uv : Ly::f)), *XH = X0 + VC, *FM = XH + VC;
+    T *twl = (T *)(FM + VC);  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));  // lane-dependent offsets recomputed per tile, not held across tiles
     const int lane = tid & 63, wave = tid >> 6;
@@ -832,33 +837,61 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     const double *x = a.x;
     long ldx = a.ldx;
     asm volatile("" : "+s"(ldx));
-    // 1. fold the quarter sequences x^(r)_j = x_{4j+r}: u_j, v_j (j = 1..n/2-1), x^(r)_0, x^(r)_{n/2}
-#pragma unroll 1
-    for (int e = tid; e < (R4N / 2 + 1) * CB; e += 512) {
-        const int j = e / CB, c = e - j * CB, gc = col0 + c;
-        const int gcc = gc < a.M ? gc : a.M - 1;
-        const bool fold = j > 0 && j < R4N / 2;
-        double p[4], q[4];
+    // 1. fold the quarter sequences x^(r)_j = x_{4j+r}: u_j, v_j (j = 1..n/2-1), x^(r)_0, x^(r)_{n/2}.
+    //    Thread (jg, c) takes rows j = jg + JG i of column c, and group 0 also
+    //    j = n/2; every load of the panel (and of the twiddles, from L2) is
+    //    issued before the first is used, so the panel costs one memory
+    //    latency.  Row j = 0 has no mirror: its second load repeats the first
+    //    row (a select on the address, not a branch around the load).
+    const T *twg = (const T *)a.A + Ly::frags;
+    constexpr int TWN = (R4N + 1) * 8;
+    T twv[(TWN + 511) / 512];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            p[r] = x[(long)(4 * j + r) * ldx + gcc];
-            q[r] = fold ? x[(long)(PN - 4 * j + r) * ldx + gcc] : 0.0;
+    for (int i = 0; i < (TWN + 511) / 512; ++i) twv[i] = tid + 512 * i < TWN ? twg[tid + 512 * i] : (T)0;
+    {
+        constexpr int JG = Ly::JG, NJ = Ly::NJ;
+        const int c = tid % CB, jg = tid / CB, gc = col0 + c;
+        const int gcc = gc < a.M ? gc : a.M - 1;
+        double p[NJ][4], q[NJ][4], ph[4];
+#pragma unroll
+        for (int i = 0; i < NJ; ++i) {
+            const int j = jg + JG * i;
+            const int jm = j > 0 ? PN - 4 * j : 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                p[i][r] = x[(long)(4 * j + r) * ldx + gcc];
+                q[i][r] = x[(long)(jm + r) * ldx + gcc];
+            }
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int vc = r * CB + c;
-            if (j == 0) {
-                X0[vc] = p[r];
-            } else if (j == R4N / 2) {
-                XH[vc] = p[r];
+        for (int r = 0; r < 4; ++r) ph[r] = x[(long)(2 * R4N + r) * ldx + gcc];
+#pragma unroll
+        for (int i = 0; i < NJ; ++i) {
+            const int j = jg + JG * i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int vc = r * CB + c;
+                if (j == 0) {
+                    X0[vc] = p[i][r];
+                } else {
+                    U[vc * KPp + j - 1] = (T)(p[i][r] + q[i][r]);
+                    V[vc * KPp + j - 1] = (T)(p[i][r] - q[i][r]);
+                }
+            }
+        }
+        if (jg == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int vc = r * CB + c;
+                XH[vc] = ph[r];
                 U[vc * KPp + R4N / 2 - 1] = (T)0;  // K padding
                 V[vc * KPp + R4N / 2 - 1] = (T)0;
-            } else {
-                U[vc * KPp + j - 1] = (T)(p[r] + q[r]);
-                V[vc * KPp + j - 1] = (T)(p[r] - q[r]);
             }
         }
     }
+#pragma unroll
+    for (int i = 0; i < (TWN + 511) / 512; ++i)
+        if (tid + 512 * i < TWN) twl[tid + 512 * i] = twv[i];
     __syncthreads();
     // middle row of each quarter: F_{n/2} = x_0 + x_{n/2} + sum_j (-1)^j u_j (FP64 sum)
 #pragma unroll
@@ -916,7 +949,7 @@ __device__ __forceinline__ void dft_fwd4_tile(const DftArgs &a, int bx, unsigned
     }
     __syncthreads();
     // 3. combine: rows k, n + k (always), 2n - k (k < n/2), n - k (0 < k < n/2)
-    const T *tw = (const T *)a.A + Ly::frags;  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n
+    const T *tw = twl;
 #pragma unroll 1
     for (int e = tid; e < (R4N / 2 + 1) * CB; e += 512) {
         const int k = e / CB, c = e - k * CB, gc = col0 + c;

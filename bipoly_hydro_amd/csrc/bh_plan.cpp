@@ -109,8 +109,11 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     p->bytes = p->mem.bytes + p->tab->mem.bytes + p->ops->mem.bytes;
     HIPCHK(hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, device));
     HIPCHK(hipStreamCreateWithFlags(&p->side, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
-    HIPCHK(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming));
+    // the fork and join events order the side stream against the solve's
+    // stream on this device only: no system-scope fence (its cache
+    // writeback and invalidation stalled the recording stream ~20 us)
+    HIPCHK(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming | hipEventDisableSystemFence));
+    HIPCHK(hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming | hipEventDisableSystemFence));
     HIPCHK(hipEventCreateWithFlags(&p->done_ev, hipEventDisableTiming));
     hipStream_t s;
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -212,22 +215,20 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 
 // Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF, also
 // placed in the faces of X1 (J-layout) and X2 (K-layout when x2_klayout,
-// else J-layout).  The side contraction runs on the side stream, after
-// whatever is queued there, beside the top/bottom contraction; the faces
-// wait for both (and for the side stream's earlier work).
+// else J-layout).  The side contraction (a short, latency-bound kernel)
+// runs first on the solve's stream, then the top/bottom contraction; the
+// faces also wait for the side stream's earlier work (the forward DFT).
 void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
-    out->push_back(Op::mark(MARK_FORK));
-    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax, p->tab->H));
-    out->push_back(Op::mark(MARK_SIDE_END));
     out->push_back(Op::mark(5));
     if (!tb_blocked(numr)) out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
     out->push_back(contract_topsym_op(p->tab->T, p->tab->T32, p->proj, p->tab->sqrt_rhf, p->Pt, p->Ptb, numr, numz,
                                       mmax));
     out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
+    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax, p->tab->H));
     out->push_back(Op::mark(MARK_JOIN));
     // factor of bessel.F90:170-174 (doubled for isym = 3)
     const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
