@@ -107,6 +107,8 @@ def main():
                     help="skip the short batched and served runs of the default line")
     ap.add_argument("--only", choices=["mixed", "fp64", "mixed_t32", "spectral64"],
                     help="time one precision alone (profiling runs); prints a reduced line")
+    ap.add_argument("--mmax", type=int, default=30,
+                    help="boundary azimuthal modes of the --only solve (the reference's 30 by default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batched", action="store_true",
@@ -169,8 +171,8 @@ def main():
         return {"ms_per_step": wall * 1e3 / steps, "solves_per_s": steps / wall,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
 
-    def run(precision, passes=2, warm_steps=0):
-        plan = P.PotentialPlan(numr, numz, numphi, precision=precision, refine_passes=passes)
+    def run(precision, passes=2, warm_steps=0, mmax=30):
+        plan = P.PotentialPlan(numr, numz, numphi, precision=precision, refine_passes=passes, mmax=mmax)
         plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
         torch.cuda.synchronize()
         pot_cold = d_pot.cpu().numpy()
@@ -211,14 +213,15 @@ def main():
         # one precision alone (for profiling runs): the headline fields of
         # that solve, nothing else
         prec = args.only
-        res, _ = run(prec, passes=args.passes)
+        res, _ = run(prec, passes=args.passes, mmax=args.mmax)
         print(json.dumps({"metric": METRIC, "value": res["solves_per_s"], "unit": "solves/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": res["ms_per_step"],
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                           "dtype": "f64" if prec in ("fp64", "spectral64") else "f32+f64", "data": "synthetic",
                           "config": {"workload": f"potential_solver {numr}x{numz}x{numphi} tstep={tstep}, "
-                                                 f"{prec} only" + (f", {args.passes} refinement passes"
-                                                                   if prec.startswith("mixed") else "")},
+                                                 f"{prec} only, mmax={args.mmax}" +
+                                                 (f", {args.passes} refinement passes" if prec.startswith("mixed")
+                                                  else "")},
                           "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "stage_ms": res["stage_ms"],
                           "error_vs_reference_cold": res["error_vs_reference_cold"]}), flush=True)
         return
@@ -246,6 +249,30 @@ def main():
            "stage_ms": s64["stage_ms"], "error_vs_reference_cold": s64["error_vs_reference_cold"],
            "relL2_vs_fp64_gpu": rell2(pots64, pot64),
            "precision": "FP64 spectral solve: FP64 MFMA sine transforms and M_r products, FP64 Thomas, one pass"}
+    # boundary modes m = 0..255 (configuration 3's table range; the
+    # reference keeps 30): plan set-up with the device table generators, the
+    # mixed solve against the FP64 solve with the same modes, and how far the
+    # modes past 30 move the potential
+    def run_modes(mm):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan = P.PotentialPlan(numr, numz, numphi, precision="mixed", refine_passes=args.passes, mmax=mm)
+        torch.cuda.synchronize()
+        setup_s = time.perf_counter() - t0
+        plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
+        torch.cuda.synchronize()
+        pot = d_pot.cpu().numpy()
+        t = timed(plan, 0, max(10, args.steps // 5))
+        plan.close()
+        with P.PotentialPlan(numr, numz, numphi, precision="fp64", mmax=mm) as p64:
+            p64.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
+            torch.cuda.synchronize()
+            pot64m = d_pot.cpu().numpy()
+        nb_ = (numr - 2 + 15) // 16
+        return {"mmax": mm, "ms_per_step": t["ms_per_step"], "p50_ms": t["p50_ms"], "plan_setup_s": setup_s,
+                "table_bytes": mm * nb_ * (nb_ + 1) // 2 * (numz - 2) * 256 * 8 + mm * (2 * numz - 5) * (numr - 2) * 8,
+                "relL2_vs_fp64_gpu_same_modes": rell2(pot, pot64m), "relL2_mmax30_vs_this": rell2(potmx, pot64m)}
+    modes = run_modes(min(256, numphi // 2)) if numphi // 2 > 30 else None
     t32, pot32 = run("mixed_t32", passes=1)
     t32 = {"ms_per_step": t32["ms_per_step"], "error_vs_reference_cold": t32["error_vs_reference_cold"],
            "precision": "FP32 top/bottom Green's table (opt-in), 1 refinement pass"}
@@ -313,6 +340,7 @@ def main():
                  "mixed_over_fp64": fp64["warm"]["ms_per_step"] / mixed["warm"]["ms_per_step"]},
         "mixed_table32": t32,
         "fp64_spectral": s64,
+        "boundary_modes": modes,
         "mixed_over_fp64_spectral": s64["ms_per_step"] / mixed["ms_per_step"],
         "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (contract with the projections scaled as read, then reduce)",
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -495,10 +495,75 @@ __device__ __forceinline__ void d_qstore(TT *o, double coef, double q)
     else if (STORE == Q_RAW) *o = (TT)q;
     else *o = (TT)(coef * (q + (double)*o));
 }
+// Mode slots m >= QREF (plans with mmax above the reference's 30,
+// pot.h:3).  The upward recurrence multiplies its rounding error by about
+// e^{2 mu eta} (chi = cosh eta) and the 111-term series does not converge
+// for high orders near chi = 1, so these slots take Q from the ratios
+// R_mu = Q_{mu-1/2} / Q_{mu-3/2} of the recurrence's minimal solution, run
+// downwards (its stable direction; Gautschi, SIAM Rev. 9 (1967)):
+//   R_mu = (mu - 1/2) / (2 mu chi - (mu + 1/2) R_{mu+1}),
+// started at mu = M + 1 with the limit ratio e^{-eta} and M >= mu_top + 16
+// + 20 / eta (the start's error is damped by e^{-2 eta} per step), and
+// normalised by Q_{-1/2} = k K(k), k'^2 = (chi - 1) / (chi + 1) formed
+// directly so that it keeps its digits as chi -> 1.  Slots go in chunks of
+// QCHUNK, each rerunning the downward pass, so that only QCHUNK ratios are
+// held.  isym = 3 (slot m: mu = 2m) multiplies two ratios per slot.
+constexpr int QREF = 30, QCHUNK = 16;
+template <int STORE, class TT>
+__device__ void d_qhigh_store(const TableArgs &a, double xm, double coef, TT *out, long mstride)
+{
+    const int mmax = a.mmax, fs = a.isym == 3 ? 2 : 1;
+    const double e1 = xm - 1.0, sh = sqrt(e1 * (xm + 1.0));  // sinh eta
+    const double eta = log1p(e1 + sh), rinf = 1.0 / (xm + sh);
+    const int mu_top = fs * (mmax - 1);
+    const int M = mu_top + 16 + (int)fmin(ceil(20.0 / eta), 1.0e6);
+    double kk, ee;
+    const double k2 = 2.0 / (1.0 + xm);
+    d_ellke(k2, e1 / (xm + 1.0), &kk, &ee);
+    double qprev = sqrt(k2) * kk;  // Q_{-1/2}, then the slot below the chunk
+    auto step = [&](int mu, double r) {
+        return ((double)mu - 0.5) / (2.0 * (double)mu * xm - ((double)mu + 0.5) * r);
+    };
+    for (int slo = QREF; slo < mmax; slo += QCHUNK) {
+        const int shi = min(slo + QCHUNK, mmax);
+        double sr[QCHUNK];
+        double r = rinf;
+        int mu = M;
+        for (; mu > fs * (shi - 1); --mu) r = step(mu, r);
+#pragma unroll
+        for (int i = QCHUNK - 1; i >= 0; --i) {
+            sr[i] = 1.0;
+            if (slo + i < shi) {
+                for (int k = 0; k < fs; ++k, --mu) {
+                    r = step(mu, r);
+                    sr[i] *= r;
+                }
+            }
+        }
+        if (slo == QREF) {  // slot QREF - 1: Q_{-1/2} R_1 ... R_{fs (QREF - 1)}
+            double prod = 1.0;
+            for (; mu >= 1; --mu) {
+                r = step(mu, r);
+                prod *= r;
+            }
+            qprev *= prod;
+        }
+#pragma unroll
+        for (int i = 0; i < QCHUNK; ++i) {
+            if (slo + i < shi) {
+                qprev *= sr[i];
+                d_qstore<STORE>(out + (long)(slo + i) * mstride, coef, qprev);
+            }
+        }
+    }
+}
+
 template <int STORE, class TT>
 __device__ void d_qhalf_store(const TableArgs &a, double xm, double coef, TT *out, long mstride)
 {
-    const int mmax = a.mmax;
+    // slots below the reference's mmax as the reference evaluates them
+    const int mmax = min(a.mmax, QREF);
+    if (a.mmax > QREF) d_qhigh_store<STORE>(a, xm, coef, out, mstride);
     if (xm < a.hypr_cutoff) {
         double mum = sqrt(2.0 / (1.0 + xm));
         double lam = sqrt(2.0 * (1.0 + xm));

@@ -65,7 +65,7 @@ int normalize_desc(const bh_grid_desc *desc, int precision, bh_grid_desc *out)
     if (d.numr < 5 || d.numz < 5) return fail(BH_E_INVALID, "numr and numz must be >= 5");
     if (d.numphi < 4 || (d.numphi & (d.numphi - 1))) return fail(BH_E_INVALID, "numphi must be a power of two >= 4");
     if (d.isym == 1 && d.numz % 2) return fail(BH_E_INVALID, "numz must be even for isym = 1 (symmetric z grid)");
-    if (d.mmax < 1 || d.mmax > 250) return fail(BH_E_INVALID, "mmax must be in [1, 250]");
+    if (d.mmax < 1 || d.mmax > 512) return fail(BH_E_INVALID, "mmax must be in [1, 512]");
     if (d.numr - 2 > 512 || d.numz - 2 > 512) return fail(BH_E_INVALID, "numr-2 and numz-2 must be <= 512");
     if (d.nsteps_cold < 2 || d.nsteps_warm < 2) return fail(BH_E_INVALID, "nsteps must be >= 2");
     if (!(d.dr > 0.0) || !std::isfinite(d.dr)) return fail(BH_E_INVALID, "dr must be positive");
@@ -263,10 +263,13 @@ int build_tables(const HostGeo &h, bool f32, GeoTables *t)
     // isym = 3 slot m holds the even mode 2m (tm.F90:105-133)
     const int nser = 111;  // hypr_upr_bnd + 1 (pot.h:7)
     const bool even = h.g.isym == 3;
+    // the series and the upward recurrence serve the reference's 30 slots;
+    // slots beyond take the downward ratios (d_qhigh_store)
+    const int mref = std::min(mmax, 30);
     std::vector<double> nu(mmax + 1, 0.0), coefh(mmax), cser((size_t)mmax * nser);
-    for (int m = 3; m <= mmax; ++m) nu[m] = even ? 2.0 * (double)m - 4.5 : (double)m - 2.5;
+    for (int m = 3; m <= mref; ++m) nu[m] = even ? 2.0 * (double)m - 4.5 : (double)m - 2.5;
     double glh = lanczos_lngamma(0.5);
-    for (int m = 1; m <= mmax; ++m) {
+    for (int m = 1; m <= mref; ++m) {
         double mm = even ? 2.0 * ((double)m - 1.0) : (double)m - 1.0;
         double aa = 0.5 * (mm + 1.5), bb = 0.5 * (mm + 0.5), cc = mm + 1.0;
         coefh[m - 1] = glh + lanczos_lngamma(mm + 0.5) - lanczos_lngamma(aa) - lanczos_lngamma(bb);

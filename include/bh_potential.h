@@ -57,7 +57,12 @@ typedef struct bh_grid_desc {
                           (the grid starts at the equator: K = 1 mirrors
                           K = 2); 2 and 3 need BH_PREC_FP64; 0 -> 1        */
     double dr;         /* dr = dz; 0 -> 1/(130 - 3) (setup_frac.F90:374-376) */
-    int mmax;          /* boundary azimuthal modes; 0 -> 30                  */
+    int mmax;          /* boundary azimuthal modes, 1..512; 0 -> 30.  Slots
+                          past the reference's 30 take Q_{m-1/2} from the
+                          downward ratios of the Legendre recurrence (the
+                          reference's own upward recurrence and 111-term
+                          series fail there); slots below 30 are the
+                          reference's                                        */
     int nsteps_cold;   /* ADI steps when tstep <= 1; 0 -> 20                 */
     int nsteps_warm;   /* ADI steps when tstep > 1;  0 -> 5                  */
     int refine_passes; /* mixed plans: FP64 residual + refinement passes after

@@ -50,6 +50,7 @@
 #endif
 
 #define BHO_MMAX_DEFAULT 30
+#define BHO_MMAX_LIMIT 512 /* mode slots beyond the reference's 30: see bho_qhigh */
 #define BHO_HYPR_CUTOFF 1.025
 #define BHO_HYPR_UPR_BND 110
 
@@ -181,9 +182,72 @@ double bho_elle(double k)
 /* (isym 3: the two-step recurrence from Q_{-1/2} and Q_{3/2}),         */
 /* truncated hypergeometric series with exp/log terms at and above it.  */
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* Mode slots m >= 30 (plans with mmax above the reference's pot.h:3     */
+/* value; parity there is with the mathematics, not the reference).      */
+/* The reference's upward recurrence multiplies its rounding error by    */
+/* about e^{2 mu eta} (chi = cosh eta) and its 111-term series does not  */
+/* converge for high orders near chi = 1, so these slots take the        */
+/* toroidal functions from the ratios R_mu = Q_{mu-1/2} / Q_{mu-3/2} of  */
+/* the minimal solution of the Legendre recurrence, run downwards (the   */
+/* stable direction; Gautschi, SIAM Rev. 9 (1967) 24-82):                */
+/*   R_mu = (mu - 1/2) / (2 mu chi - (mu + 1/2) R_{mu+1}),               */
+/* started at mu = M + 1 with the limit ratio e^{-eta}, M far enough     */
+/* above the highest order that the start's error, damped by e^{-2 eta}  */
+/* per step, is gone (M - mu_top >= 20 / eta + 16).  Q_{-1/2} = k K(k),   */
+/* k^2 = 2 / (chi + 1), normalises the products Q_{mu-1/2} =             */
+/* Q_{-1/2} R_1 ... R_mu.  The slots are done in chunks of 16, each      */
+/* rerunning the downward pass from M, so that only 16 ratios are held.  */
+/* isym = 3: slot m holds mu = 2m, the product of two ratios per slot.   */
+/* ------------------------------------------------------------------ */
+#define BHO_MREF 30
+#define BHO_QCHUNK 16
+static void bho_qhigh(const bho_plan *p, double xm, double *q)
+{
+    const int mmax = p->mmax, fs = p->isym == 3 ? 2 : 1, s0 = BHO_MREF;
+    if (mmax <= s0) return;
+    const double e1 = xm - 1.0, sh = sqrt(e1 * (xm + 1.0)); /* sinh eta */
+    const double eta = log1p(e1 + sh), rinf = 1.0 / (xm + sh);
+    const int mu_top = fs * (mmax - 1);
+    const double extra = ceil(20.0 / eta);
+    const int M = mu_top + 16 + (int)(extra < 1.0e6 ? extra : 1.0e6);
+    /* Q_{-1/2} = k K(k) with k'^2 = (chi - 1) / (chi + 1) formed directly
+       (not as 1 - k^2), which keeps its digits as chi -> 1 */
+    double qprev = sqrt(2.0 / (1.0 + xm)) * bho_rf(0.0, e1 / (xm + 1.0), 1.0); /* then the slot below the chunk */
+    for (int slo = s0; slo < mmax; slo += BHO_QCHUNK) {
+        const int shi = slo + BHO_QCHUNK < mmax ? slo + BHO_QCHUNK : mmax;
+        double sr[BHO_QCHUNK]; /* slot ratios Q(slot s) / Q(slot s - 1) */
+        double r = rinf;
+        int mu = M;
+        for (; mu > fs * (shi - 1); --mu) r = ((double)mu - 0.5) / (2.0 * (double)mu * xm - ((double)mu + 0.5) * r);
+        for (int sl = shi - 1; sl >= slo; --sl) {
+            double prod = 1.0;
+            for (int k = 0; k < fs; ++k, --mu) {
+                r = ((double)mu - 0.5) / (2.0 * (double)mu * xm - ((double)mu + 0.5) * r);
+                prod *= r;
+            }
+            sr[sl - slo] = prod;
+        }
+        if (slo == s0) { /* Q of slot s0 - 1: Q_{-1/2} R_1 ... R_{fs (s0 - 1)} */
+            double prod = 1.0;
+            for (; mu >= 1; --mu) {
+                r = ((double)mu - 0.5) / (2.0 * (double)mu * xm - ((double)mu + 0.5) * r);
+                prod *= r;
+            }
+            qprev *= prod;
+        }
+        for (int sl = slo; sl < shi; ++sl) {
+            qprev *= sr[sl - slo];
+            q[sl] = qprev;
+        }
+    }
+}
+
 static void bho_qhalf(const bho_plan *p, double xm, double *q)
 {
-    const int mmax = p->mmax;
+    /* slots below the reference's mmax as the reference evaluates them */
+    const int mmax = p->mmax < BHO_MREF ? p->mmax : BHO_MREF;
+    bho_qhigh(p, xm, q);
     if (xm < p->hypr_cutoff) {
         double mum = sqrt(2.0 / (1.0 + xm));
         double lam = sqrt(2.0 * (1.0 + xm));
@@ -261,7 +325,7 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
     if (isym < 1 || isym > 3) return NULL;
     if (numr < 4 || numz < 4 || numphi < 4) return NULL;
     if (numphi & (numphi - 1)) return NULL;
-    if (mmax < 1 || mmax > 250) return NULL;
+    if (mmax < 1 || mmax > BHO_MMAX_LIMIT) return NULL;
     bho_plan *p = (bho_plan *)xcalloc(1, sizeof(bho_plan));
     p->numr = numr; p->numz = numz; p->numphi = numphi; p->isym = isym; p->mmax = mmax;
     p->nser = BHO_HYPR_UPR_BND + 1;
@@ -355,7 +419,7 @@ bho_plan *bho_plan_create(int numr, int numz, int numphi, int isym, double dr, i
     p->coefh = (double *)xcalloc(mmax, sizeof(double));
     p->dcoefh = (double *)xcalloc((size_t)mmax * p->nser, sizeof(double));
     double gl_half = bho_gammln(0.5);
-    for (int m = 1; m <= mmax; ++m) {
+    for (int m = 1; m <= mmax && m <= BHO_MREF; ++m) {
         double mm = isym == 3 ? 2.0 * ((double)m - 1.0) : (double)m - 1.0;
         double aa = 0.5 * (mm + 1.5), bb = 0.5 * (mm + 0.5), cc = mm + 1.0;
         p->coefh[m - 1] = gl_half + bho_gammln(mm + 0.5) - bho_gammln(aa) - bho_gammln(bb);

@@ -109,3 +109,53 @@ def source_inputs(grid=SOURCE_GRID, seed=SOURCE_SEED):
               cylin_a_com=(1e-2 * (2.0 * c[:3 * numphi] - 1.0)).reshape(numphi, 3).T.copy(),
               cylin_v_com=(1e-2 * (2.0 * c[3 * numphi:] - 1.0)).reshape(numphi, 3).T.copy())
     return f, sc
+
+
+def offaxis_blob(numr, numz, numphi, centre_cells, dr=1.0 / 127.0):
+    """A compact Gaussian source (width one cell, cut at three) centred at x
+    = centre_cells * dr in the equatorial plane, interior cells only; cell
+    (l, K, J) sits at angle (l + 1/2) dphi, the phase the boundary
+    projection and rebuild use."""
+    import math
+    r, z, _ = density.grid_coords(numr, numz, numphi, dr)
+    phi = (np.arange(numphi) + 0.5) * 2.0 * math.pi / numphi
+    shape = (numphi, numz, numr)
+    X = np.broadcast_to(r[None, None, :] * np.cos(phi)[:, None, None], shape)
+    Y = np.broadcast_to(r[None, None, :] * np.sin(phi)[:, None, None], shape)
+    Z = np.broadcast_to(z[None, :, None], shape)
+    d2 = ((X - centre_cells * dr) ** 2 + Y ** 2 + Z ** 2) / dr ** 2
+    rho = np.where(d2 < 9.0, np.exp(-d2), 0.0)
+    rho[:, :, [0, -1]] = 0.0
+    rho[:, [0, -1], :] = 0.0
+    return rho
+
+
+def direct_face_potential(rho, dr=1.0 / 127.0):
+    """The face values of -sum rho dV / |x_B - x'| (G = 1) over the source
+    cells, dV = r' dr dz dphi, at the top (K = numz), bottom (K = 1) and
+    side (J = numr) face points, in the order of face_values()."""
+    import math
+    numphi, numz, numr = rho.shape
+    r, z, _ = density.grid_coords(numr, numz, numphi, dr)
+    phi = (np.arange(numphi) + 0.5) * 2.0 * math.pi / numphi
+    l, K, J = np.nonzero(rho)
+    w = rho[l, K, J] * r[J] * dr * dr * (2.0 * math.pi / numphi)
+    xs, ys, zs = r[J] * np.cos(phi[l]), r[J] * np.sin(phi[l]), z[K]
+    out = []
+    for (lb, Kb, Jb) in face_points(numr, numz, numphi):
+        xb, yb = r[Jb] * math.cos(phi[lb]), r[Jb] * math.sin(phi[lb])
+        out.append(-np.sum(w / np.sqrt((xb - xs) ** 2 + (yb - ys) ** 2 + (z[Kb] - zs) ** 2)))
+    return np.array(out)
+
+
+def face_points(numr, numz, numphi):
+    """(l, K-1, J-1) of the top, bottom and side face points (corners out)."""
+    top = [(l, numz - 1, J) for l in range(numphi) for J in range(1, numr - 1)]
+    bottom = [(l, 0, J) for l in range(numphi) for J in range(1, numr - 1)]
+    side = [(l, K, numr - 1) for l in range(numphi) for K in range(1, numz - 1)]
+    return top + bottom + side
+
+
+def face_values(pot):
+    numphi, numz, numr = pot.shape
+    return np.array([pot[p] for p in face_points(numr, numz, numphi)])

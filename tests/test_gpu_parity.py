@@ -9,7 +9,8 @@ import pytest
 
 from bipoly_hydro_amd import density
 from bipoly_hydro_amd import potential as P
-from golden_cases import SYM_CASES, c2_digests, c2_slices, load_case, load_sym_case
+from golden_cases import (SYM_CASES, c2_digests, c2_slices, direct_face_potential, face_values, load_case,
+                          load_sym_case, offaxis_blob)
 
 pytestmark = pytest.mark.gpu
 
@@ -91,6 +92,38 @@ def test_fp64_matches_oracle_on_other_grids(shape, seed):
     e1, e2 = rell2(g1, o1), rell2(g2, o2)
     print(f"{shape}: cold {e1:.3e} warm {e2:.3e}", flush=True)
     assert e1 <= TOL_FP64 and e2 <= TOL_FP64
+
+
+# boundary modes past the reference's 30 (mmax = numphi / 2 and, for isym
+# = 3, the even frequencies up to 78): the device tables' downward-ratio
+# slots against the oracle's, on cold and warm solves, and the faces of a
+# compact source 14 cells from the side face against the direct point sum
+@pytest.mark.parametrize("shape,isym,mmax,precision", [((34, 34, 128), 1, 64, "fp64"), ((66, 34, 128), 1, 64, "mixed"),
+                                                      ((66, 33, 128), 3, 40, "fp64")])
+def test_high_modes_match_oracle(shape, isym, mmax, precision):
+    from oracle import oracle as O
+    numr, numz, numphi = shape
+    rng = np.random.default_rng(17)
+    rho1 = density.lane_emden_n1_binary(numr, numz, numphi, radius_cells=(0.2 * numr, 0.15 * numr),
+                                        centre_cells=(0.3 * numr, -0.3 * numr), noise=1e-3, seed=17, isym=isym)
+    rho2 = rho1 * (1.0 + 0.01 * rng.standard_normal(rho1.shape))
+    oplan = O.OraclePlan(numr, numz, numphi, isym=isym, mmax=mmax)
+    o1 = oplan.solve(rho1, tstep=0)
+    o2 = oplan.solve(rho2, potp=o1, tstep=3)
+    with P.PotentialPlan(numr, numz, numphi, isym=isym, mmax=mmax, precision=precision) as plan:
+        g1 = plan.potential_solver(rho1, tstep=0)
+        g2 = plan.potential_solver(rho2, tstep=3)
+    tol = TOL_FP64 if precision == "fp64" else 1e-8
+    e1, e2 = rell2(g1, o1), rell2(g2, o2)
+    print(f"{shape} isym={isym} mmax={mmax} {precision}: cold {e1:.3e} warm {e2:.3e}", flush=True)
+    assert e1 <= tol and e2 <= tol
+    if isym == 1 and precision == "fp64":
+        blob = offaxis_blob(numr, numz, numphi, 18.0)
+        with P.PotentialPlan(numr, numz, numphi, mmax=mmax) as plan:
+            faces = face_values(plan.potential_solver(blob, tstep=0))
+        e = rell2(faces, direct_face_potential(blob))
+        print(f"faces against the direct sum: {e:.3e}", flush=True)
+        assert e < 1e-12
 
 
 def test_fp64_production_grid_against_reference_slices(golden_dir):

@@ -9,7 +9,8 @@ import pytest
 
 from bipoly_hydro_amd import density
 from oracle import oracle as O
-from golden_cases import SMALL_CASES, SYM_CASES, load_case, load_sym_case, rell2
+from golden_cases import (SMALL_CASES, SYM_CASES, direct_face_potential, face_values, load_case,
+                          load_sym_case, offaxis_blob, rell2)
 
 @pytest.mark.parametrize("name", list(SMALL_CASES))
 def test_oracle_matches_reference_cold_and_warm(golden_dir, name):
@@ -158,6 +159,50 @@ def test_qhalf_even_modes_against_mpmath():
             assert abs(q[m] / ex - 1.0) < 1e-13, (x, m)
         ex = float(mp.legenq(9.5, 0, x, type=3).real)
         assert abs(q[5] / ex - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("isym", [1, 3])
+def test_qhalf_high_modes_against_mpmath(isym):
+    # mode slots past the reference's 30 (mmax up to numphi/2 and beyond):
+    # downward ratios of the minimal solution normalised by Q_{-1/2}, held
+    # to the exact toroidal functions from chi = 1 + 7.6e-6 (a source one
+    # cell from a face of a 258-wide grid) to chi = 320 (the far axis),
+    # across the recurrence/series cutoff; values below the double range
+    # underflow to zero.  The reference's own 30 slots are unchanged.
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 40
+    plan = O.OraclePlan(10, 10, 512, with_tables=False, mmax=256, isym=isym)
+    plan30 = O.OraclePlan(10, 10, 512, with_tables=False, mmax=30, isym=isym)
+    worst = 0.0
+    for x in (1.0 + 7.6e-6, 1.0 + 1e-4, 1.001, 1.02, 1.0249, 1.0251, 1.3, 2.0, 5.0, 40.0, 320.0):
+        q = plan.qhalf(x)
+        assert np.array_equal(q[:30], plan30.qhalf(x)), x
+        for m in list(range(30, 256, 5)) + [255]:
+            mu = 2 * m if isym == 3 else m
+            ex = float(mp.legenq(mu - 0.5, 0, x, type=3).real)
+            if abs(ex) < 1e-290:
+                assert abs(q[m]) < 1e-280, (x, m)
+                continue
+            worst = max(worst, abs(q[m] / ex - 1.0))
+    assert worst < 3e-13, worst
+
+
+def test_high_modes_complete_the_boundary_sum():
+    # with mmax = numphi / 2 the contraction holds every azimuthal mode the
+    # grid resolves, so for a source 14 cells from the side face the face
+    # potential equals the direct point sum -sum rho dV / |x_B - x'| up to
+    # the modes past numphi / 2 (~ e^{-64 eta}, eta ~ 0.6); the reference's
+    # 30 modes leave ~1e-9.  A wrong sign, scale or order shift in the high
+    # slots fails the first bound.
+    numr, numz, numphi = 34, 34, 128
+    rho = offaxis_blob(numr, numz, numphi, 18.0)
+    ref = direct_face_potential(rho)
+    e = {}
+    for mmax in (30, 64):
+        plan = O.OraclePlan(numr, numz, numphi, mmax=mmax)
+        e[mmax] = rell2(face_values(plan.bessel(rho)), ref)
+    print(e)
+    assert e[64] < 1e-12 and e[30] > 1e-10, e
 
 
 def test_dt_schedule_closed_form():
