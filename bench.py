@@ -27,15 +27,15 @@ HBM_PEAK_GBS = 8000.0
 # half-step at 258 x 130 x 512, from rocprofv3 FETCH_SIZE (x 2.000, the
 # copy-kernel calibration) + WRITE_SIZE;
 # profiles/potential_solver_258x130x512_hbm_traffic.md
-CONTRACT_TRAFFIC = ((533848.0 + 2332.0) * 2.0 + (3840.0 + 240.0)) * 1024  # FP64 table
+CONTRACT_TRAFFIC = ((546326.0 + 2332.0) * 2.0 + (7200.0 + 240.0)) * 1024  # FP64 table
 # the mixed cold inverse DFT (radix 8, x = faces + delta1 + correction sum
 # in, phi and the warm-start state out): HBM bytes per launch (FETCH x 2.000
 # + WRITE) and the FP64 MFMA operations it executes
 # (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512); profiles/potential_solver_258x130x512_{hbm_traffic,mfma}.md
-INV_TRAFFIC = (77127.0 * 2.0 + 293835.0) * 1024
+INV_TRAFFIC = (77004.0 * 2.0 + 293835.0) * 1024
 INV_MFMA_FLOP = 2147328.0 * 512
 FP64_PEAK_TFLOPS = 78.6
-SWEEP_TRAFFIC = ((162531.0 + 144885.0) * 2.0 + (151557.0 + 138717.0)) * 1024 / 2  # mean of the two half-steps
+SWEEP_TRAFFIC = ((159759.0 + 158823.0) * 2.0 + (131072.0 + 131519.0)) * 1024 / 2  # mean of the two half-steps
 
 
 def rell2(a, b):
