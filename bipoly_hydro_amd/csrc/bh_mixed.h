@@ -37,6 +37,7 @@ struct ResidualArgs {
     double gamma, four_pi;
     int numr, numz;
     int cold;  // X is zero on the interior (a cold start): read it only on the faces
+    long rh_pitch;  // floats from one plane of rh to the next (0: numr x numz)
 };
 // Xo = X + (dlt + dlt2) on the interior (dlt2 may be null); cold: X is zero
 // on the interior and is not read

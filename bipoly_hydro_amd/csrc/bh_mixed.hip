@@ -56,7 +56,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     const double *__restrict__ X = a.X + L * plane;
     const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
     const float *__restrict__ dl2 = ADD2 ? a.dlt2 + L * plane : nullptr;
-    const float *__restrict__ rh = a.rh + L * plane;
+    const float *__restrict__ rh = a.rh + L * (a.rh_pitch ? a.rh_pitch : plane);
     const float *__restrict__ w = UPD ? a.w + L * plane : nullptr;
     double *__restrict__ Xo = WX ? a.Xo + L * plane : nullptr;
     float *__restrict__ r32 = a.r + (long)L * nri * nzi;
@@ -187,7 +187,7 @@ __device__ __forceinline__ void residual_face_tile(const ResidualArgs &a, int bx
     double Ax = brbJ * xc + aJ * xe;
     Ax += J > 2 ? bJ * xw : 0.0;
     Ax += 2.0 * a.gamma * xc - a.gamma * (xkm + xkp);
-    const double r = -a.four_pi * (double)a.rh[L * plane + ci] - Ax;
+    const double r = -a.four_pi * (double)a.rh[L * (a.rh_pitch ? a.rh_pitch : plane) + ci] - Ax;
     a.r[(long)L * nri * nzi + (long)(K - 2) * nri + (J - 2)] = (float)r;
 }
 

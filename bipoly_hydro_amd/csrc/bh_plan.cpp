@@ -81,7 +81,10 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     const bool with_d2 = is_mixed(precision) && p->ops->spec_fused && p->passes >= 2;
     if (is_mixed(precision)) {
         const size_t PI = (size_t)N * nri * nzi;
-        ALLOC(p->RH32, M * N);
+        // rho-hat's mode rows on 128-byte boundaries: the forward DFT writes
+        // them in column panels, and aligned rows store ~40 % faster
+        p->rh_pitch = (long)(M + 31) / 32 * 32;
+        ALLOC(p->RH32, (size_t)p->rh_pitch * N);
         ALLOC(p->r32, PI);
         if (!p->ops->spec_fused) {  // spectral blocks between the separate kernels
             ALLOC(p->RT, PI);
@@ -257,7 +260,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     // next to a face need the stencil (residual_face_op)
     const bool cold_split = tstep <= 1 && o.dfwd32 && p->use_dft && N == DFT_MAX_N;
     if (o.dfwd32 && p->use_dft) {
-        DftArgs d{d_rho, M, p->RH32, M, o.dfwd32, N, (int)M, numr, numz};
+        DftArgs d{d_rho, M, p->RH32, p->rh_pitch, o.dfwd32, N, (int)M, numr, numz};
         d.max_blocks = side_blocks(p);
         if (cold_split) {
             d.rout = p->r32;
@@ -269,13 +272,13 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     } else {
         const int nh = N / 2 + 1, nsn = N / 2 - 1;
         GemmArgs a{};
-        a.A = o.Dc32; a.lda = nh; a.B = d_rho; a.ldb = M; a.C = p->RH32; a.ldc = M;
+        a.A = o.Dc32; a.lda = nh; a.B = d_rho; a.ldb = M; a.C = p->RH32; a.ldc = p->rh_pitch;
         a.R = nh; a.M = (int)M; a.Kd = nh; a.batch = 1; a.nfold = N;
         a.mask_numr = numr; a.mask_numz = numz;
         a.tile = M >= 4096 ? p->tile_dft : 2;
         a.col_fastest = p->order_dft;
         out->push_back(gemm_op(a, G32D_FOLDP_MASK));
-        a.A = o.Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * M; a.R = nsn; a.Kd = nsn;
+        a.A = o.Ds32; a.lda = nsn; a.C = p->RH32 + (long)nh * p->rh_pitch; a.R = nsn; a.Kd = nsn;
         out->push_back(gemm_op(a, G32D_FOLDM_MASK));
         out->push_back(Op::mark(MARK_SIDE_END));
         proj64_ops(p, d_rho, out);
@@ -293,6 +296,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.brb = o.brb_d; ra.alphar = o.alphar; ra.betar = o.betar;
     ra.gamma = g.gamma; ra.four_pi = g.four_pi; ra.numr = numr; ra.numz = numz;
     ra.rh = p->RH32;
+    ra.rh_pitch = p->rh_pitch;
     ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
     ra.cold = tstep <= 1;
     out->push_back(cold_split ? residual_face_op(ra, N) : residual_op(ra, N));

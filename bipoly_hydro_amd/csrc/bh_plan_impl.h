@@ -134,6 +134,7 @@ struct bh_plan {
     double *RJ = nullptr, *RK = nullptr, *XJ = nullptr, *XK = nullptr, *proj = nullptr;
     double *Stb = nullptr, *Ss = nullptr, *FF = nullptr;
     double *Pt = nullptr, *Ptb = nullptr;  // scaled projections, contraction partials (top/bottom)
+    long rh_pitch = 0;  // floats per mode row of RH32: numr x numz rounded up to 32 (128-byte rows)
     float *RH32 = nullptr, *r32 = nullptr, *RT = nullptr, *WT = nullptr, *CT = nullptr, *DT = nullptr;
     float *D1 = nullptr, *Wf = nullptr;
     float *D2 = nullptr;  // sum of the refinement corrections (mixed, fused spectral step, passes >= 2)

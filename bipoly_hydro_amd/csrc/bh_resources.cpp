@@ -690,7 +690,8 @@ size_t plan_private_bytes(const bh_grid_desc &d, int precision)
         // RH32, r32, D1 + Wf (+ D2 with two or more passes of the fused
         // step); RT, CT, DT + WT only for the separate-kernel step
         const bool fused = spectral_fused_ok((int)nri, (int)nzi);
-        b += 4 * (M * N + (fused ? 1 : 5) * N * nri * nzi + (fused && d.refine_passes >= 2 ? 3 : 2) * M * N);
+        b += 4 * ((M + 31) / 32 * 32 * N + (fused ? 1 : 5) * N * nri * nzi +
+                  (fused && d.refine_passes >= 2 ? 3 : 2) * M * N);
     }
     else if (is_spec64(precision)) b += 8 * (M * N + 3 * N * nri * nzi);
     else b += 8 * (M * N + N * (2 * tiled_stride((int)nri, (int)nzi) + tiled_stride((int)nzi, (int)nri)));
