@@ -339,12 +339,18 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         //    holds them to ~1e-14 of x.  With one pass Dacc is the w buffer,
         //    free once the one refinement residual has read w.
         float *Dacc = p->passes >= 2 ? p->D2 : p->Wf;
+        // with two or more passes (split-bf16 transforms, nzi >= 64) each
+        // pass forms its residual inside the spectral kernel, in LDS: no r
+        // written and read back, no residual launch
+        const bool fuse = p->passes >= 2 && nzi >= 64;
         for (int it = 0; it < p->passes; ++it) {
             ra.X = Xa; ra.dlt = p->D1; ra.dlt2 = it > 0 ? Dacc : nullptr; ra.w = p->Wf; ra.Xo = nullptr;
             ra.r = p->r32; ra.cold = cold;
-            out->push_back(residual_op(ra, N));
+            if (!fuse) out->push_back(residual_op(ra, N));
             SpecArgs sr{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, Dacc, nullptr,
                         numr, numz, N, it > 0 ? 1 : 0, o.sfb16, o.sfa16, p->passes == 1 ? 1 : 0};
+            sr.fuse_res = fuse ? 1 : 0;
+            sr.ra = ra;
             out->push_back(spectral_op(sr, N, nzi));
         }
         // 7. x = X0 + (delta1 + Dacc) into Xa, the warm-start state, then the

@@ -5,6 +5,7 @@
 
 #include <vector>
 
+#include "bh_mixed.h"
 #include "bh_ops.h"
 
 namespace bh {
@@ -32,6 +33,11 @@ struct SpecArgs {
     // a correction (no mr) with FP32 transforms instead of split bf16: the
     // solve's only refinement pass, whose error no later pass contracts
     int fp32t;
+    // a refinement pass with split-bf16 transforms (nzi >= 64) may form its
+    // own residual r = g - w - A x from ra (the residual kernel's arguments;
+    // ra.r unused) into LDS instead of reading r: fuse_res != 0
+    int fuse_res;
+    ResidualArgs ra;
 };
 
 // Sizes the fused kernel takes: nri a multiple of 64 up to 256 and nzi one

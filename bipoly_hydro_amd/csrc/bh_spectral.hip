@@ -21,6 +21,7 @@
 // lane l: row l%16, k l/16; four k values 4(l/16) + g of a 16-wide k group
 // as one 16-byte load).
 #include "bh_mixed.h"
+#include "bh_residual.h"
 #include "bh_spectral.h"
 
 #include <hip/hip_runtime.h>
@@ -90,6 +91,36 @@ __device__ __forceinline__ BH_GLOBAL T *gst(void *p)
     return (BH_GLOBAL T *)p;
 }
 
+// The refinement residual of plane L (FP64, residual_tile_v's arithmetic)
+// into LDS as r[k][j] with pitch rp, tiles of 8 K rows, 256 threads per
+// tile: thread tid takes column tid % 256 of the tiles tid / 256, + 2, ...
+__device__ __forceinline__ void spec_residual(const ResidualArgs &ra, int L, int nzi, int tid, float *rl, int rp)
+{
+    constexpr int RR = 8;
+    const int ntiles = nzi / RR, tx = tid & 255;
+    const bool add2 = ra.dlt2 != nullptr;
+    for (int kt = tid >> 8; kt < ntiles; kt += SPEC_THREADS / 256) {
+        const bool edge = kt == 0 || kt == ntiles - 1;
+        if (edge) {
+            if (ra.cold) {
+                if (add2) residual_tile_v<RR, true, true, true, false, true>(ra, 0, kt, L, tx, rl, rp);
+                else residual_tile_v<RR, true, true, true, false, false>(ra, 0, kt, L, tx, rl, rp);
+            } else {
+                if (add2) residual_tile_v<RR, true, false, true, false, true>(ra, 0, kt, L, tx, rl, rp);
+                else residual_tile_v<RR, true, false, true, false, false>(ra, 0, kt, L, tx, rl, rp);
+            }
+        } else {
+            if (ra.cold) {
+                if (add2) residual_tile_v<RR, true, true, false, false, true>(ra, 0, kt, L, tx, rl, rp);
+                else residual_tile_v<RR, true, true, false, false, false>(ra, 0, kt, L, tx, rl, rp);
+            } else {
+                if (add2) residual_tile_v<RR, true, false, false, false, true>(ra, 0, kt, L, tx, rl, rp);
+                else residual_tile_v<RR, true, false, false, false, false>(ra, 0, kt, L, tx, rl, rp);
+            }
+        }
+    }
+}
+
 // Tile t takes plane t/2 + (t%2) N/2: planes m and m + N/2 (the cosine and
 // sine rows of azimuthal mode m) run side by side on one XCD, and share M_r
 // and the pivots, stored per mode number mi = L or L - N/2.
@@ -137,7 +168,26 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     if constexpr (BFT && !MR) {
         // every plane this lane needs, loaded at once: k = 32 kb + 8 q + e
         f2 xr[S::KB][8], yr[S::KB][8];
-        if (active) {
+        if (a.fuse_res) {
+            // the plane's residual into LDS (pitch nri + 2: the four 16-lane
+            // rows of an 8-byte read fall on distinct banks), then the
+            // lane's planes from there; the block's space is free again for
+            // the table after the second barrier
+            const int rp = nri + 2;
+            spec_residual(a.ra, L, NZ, tid, buf, rp);
+            __syncthreads();
+            if (active) {
+#pragma unroll
+                for (int kb = 0; kb < S::KB; ++kb)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int pa = 32 * kb + 8 * q + e;
+                        xr[kb][e] = *(const f2 *)(buf + pa * rp + j0 + 2 * t);
+                        yr[kb][e] = *(const f2 *)(buf + (NZ - 1 - pa) * rp + j0 + 2 * t);
+                    }
+            }
+            __syncthreads();
+        } else if (active) {
 #pragma unroll
             for (int kb = 0; kb < S::KB; ++kb)
 #pragma unroll
