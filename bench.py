@@ -271,7 +271,8 @@ def main():
         nb_ = (numr - 2 + 15) // 16
         return {"mmax": mm, "ms_per_step": t["ms_per_step"], "p50_ms": t["p50_ms"], "plan_setup_s": setup_s,
                 "table_bytes": mm * nb_ * (nb_ + 1) // 2 * (numz - 2) * 256 * 8 + mm * (2 * numz - 5) * (numr - 2) * 8,
-                "relL2_vs_fp64_gpu_same_modes": rell2(pot, pot64m), "relL2_mmax30_vs_this": rell2(potmx, pot64m)}
+                "relL2_vs_fp64_gpu_same_modes": rell2(pot, pot64m),
+                "fp64_mmax30_vs_fp64_this_relL2": rell2(pot64, pot64m)}
     modes = run_modes(min(256, numphi // 2)) if numphi // 2 > 30 else None
     t32, pot32 = run("mixed_t32", passes=1)
     t32 = {"ms_per_step": t32["ms_per_step"], "error_vs_reference_cold": t32["error_vs_reference_cold"],
