@@ -100,6 +100,16 @@ struct RebuildArgs {
     int x2_klayout;
     int fstep;   // azimuthal frequency of mode slot m is fstep * m (2 for isym = 3)
     int bottom;  // the bottom face exists (isym = 1)
+    // with r (a cold mixed start, isym = 1): also the first residual at the
+    // interior points next to a face (rows K = 2 and numz - 1, column J =
+    // numr - 1, the only ones where A x0 is not zero), r = -4 pi rh - A x0
+    // with x0 zero inside, compact r[L][K-2][J-2]; the forward DFT has
+    // written -4 pi rh at every other interior point
+    float *r;
+    const float *rh;
+    long rh_pitch;
+    const double *alphar;
+    double gamma, four_pi;
 };
 
 // grid-stride fill for plan set-up
@@ -190,6 +200,10 @@ Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, i
 // x2_klayout, else J-layout; may be null), the outer corners set to zero.
 // isym = 2, 3 have no bottom face (those values are not written) and
 // isym = 3 mode slot m has frequency 2m.
+// a cold mixed start (isym = 1): the face op also writes the first residual
+// at the interior points next to a face (see RebuildArgs::r)
+void face_modes_add_residual(Op *op, float *r, const float *rh, long rh_pitch, const double *alphar, double gamma,
+                             double four_pi);
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
                  int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1 = nullptr,
                  double *X2 = nullptr, int x2_klayout = 0, int isym = 1);

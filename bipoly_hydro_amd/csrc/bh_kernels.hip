@@ -1390,29 +1390,13 @@ Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, i
 // no transform (aliasing included when mmax > N/2).  With isym = 3 slot m
 // carries frequency p = 2m (its cos, sin rows have angle 2m theta) and
 // counts where p = +-k (mod N); f then doubles (bessel.F90:170-174).
-__device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, int by, int)
+// the face value fi (0 <= fi < 2 nri + nzi) of mode row `row`
+__device__ double face_mode_value(const RebuildArgs &a, int fi, int row)
 {
     const double *__restrict__ Stb = a.Stb;
     const double *__restrict__ Ss = a.Ss;
     const int numr = a.numr, numz = a.numz, N = a.numphi, mmax = a.mmax;
     const int nri = numr - 2, nzi = numz - 2;
-    const int nf = 2 * nri + nzi;
-    const int fi = bx * 128 + threadIdx.x;
-    const int row = by;
-    if (a.X1 && fi == nf) {  // the outer corners of the side face are zero (bessel.F90:428, 456, 487)
-        const long plane = (long)numr * numz;
-        double *x1 = a.X1 + row * plane;
-        x1[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
-        x1[numr - 1] = 0.0;
-        if (a.X2) {
-            double *x2 = a.X2 + row * plane;
-            x2[a.x2_klayout ? (long)(numr - 1) * numz + (numz - 1) : (long)(numz - 1) * numr + (numr - 1)] = 0.0;
-            x2[a.x2_klayout ? (long)(numr - 1) * numz : numr - 1] = 0.0;
-        }
-        return;
-    }
-    if (fi >= nf) return;
-    if (!a.bottom && fi >= nri && fi < 2 * nri) return;  // isym = 2, 3: no bottom face
     const double *sc, *ss;
     long st, sst;  // per-m stride, per-split stride
     int idx;
@@ -1452,8 +1436,64 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
         if (plus) { re += r; im += q; mp += stp; }
         if (minus) { re += r; im -= q; mq += stp; }
     }
-    const double v = a.factor * (double)N * (imag ? im : re);
+    return a.factor * (double)N * (imag ? im : re);
+}
+
+__device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, int by, int)
+{
+    const int numr = a.numr, numz = a.numz;
+    const int nri = numr - 2, nzi = numz - 2;
+    const int nf = 2 * nri + nzi;
+    const int fi = bx * 128 + threadIdx.x;
+    const int row = by;
+    if (a.X1 && fi == nf) {  // the outer corners of the side face are zero (bessel.F90:428, 456, 487)
+        const long plane = (long)numr * numz;
+        double *x1 = a.X1 + row * plane;
+        x1[(long)(numz - 1) * numr + (numr - 1)] = 0.0;
+        x1[numr - 1] = 0.0;
+        if (a.X2) {
+            double *x2 = a.X2 + row * plane;
+            x2[a.x2_klayout ? (long)(numr - 1) * numz + (numz - 1) : (long)(numz - 1) * numr + (numr - 1)] = 0.0;
+            x2[a.x2_klayout ? (long)(numr - 1) * numz : numr - 1] = 0.0;
+        }
+        return;
+    }
+    if (fi >= nf) return;
+    if (!a.bottom && fi >= nri && fi < 2 * nri) return;  // isym = 2, 3: no bottom face
+    const double v = face_mode_value(a, fi, row);
     a.F[(long)row * nf + fi] = v;
+    if (a.r) {
+        // the interior point next to this face point, when this thread owns
+        // it: the top face owns row K = numz - 1, the bottom face row K = 2,
+        // the side face J = numr - 1 for K = 3..numz - 2.  A x0 there has
+        // only face terms: alphar(J) x(J+1) (east face) and -gamma (x(K-1) +
+        // x(K+1)) (bottom and top faces); other faces' values at corners
+        // come from face_mode_value.
+        int J, K;
+        double xe = 0.0, xkm = 0.0, xkp = 0.0;
+        bool own = true;
+        if (fi < nri) {
+            J = fi + 2; K = numz - 1;
+            xkp = v;
+            if (K - 1 == 1) xkm = face_mode_value(a, nri + fi, row);
+        } else if (fi < 2 * nri) {
+            J = fi - nri + 2; K = 2;
+            own = K != numz - 1;
+            xkm = v;
+        } else {
+            J = numr - 1; K = fi - 2 * nri + 2;
+            own = K >= 3 && K <= numz - 2;
+            xe = v;
+        }
+        if (own) {
+            if (fi < 2 * nri && J + 1 == numr) xe = face_mode_value(a, 2 * nri + (K - 2), row);
+            const double Ax = a.alphar[J] * xe - a.gamma * (xkm + xkp);
+            const long ci = (long)(K - 1) * numr + (J - 1);
+            const long rp = a.rh_pitch ? a.rh_pitch : (long)numr * numz;
+            const double rr = -a.four_pi * (double)a.rh[row * rp + ci] - Ax;
+            a.r[(long)row * nri * nzi + (long)(K - 2) * nri + (J - 2)] = (float)rr;
+        }
+    }
     if (a.X1) {  // top J = fi + 2, K = numz; bottom K = 1; side J = numr, K = fi - 2 nri + 2
         int J, K;
         if (fi < nri) { J = fi + 2; K = numz; }
@@ -1464,6 +1504,19 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
         if (a.X2)
             a.X2[row * plane + (a.x2_klayout ? (long)(J - 1) * numz + (K - 1) : (long)(K - 1) * numr + (J - 1))] = v;
     }
+}
+
+void face_modes_add_residual(Op *op, float *r, const float *rh, long rh_pitch, const double *alphar, double gamma,
+                             double four_pi)
+{
+    RebuildArgs a = op->as<RebuildArgs>();
+    a.r = r;
+    a.rh = rh;
+    a.rh_pitch = rh_pitch;
+    a.alphar = alphar;
+    a.gamma = gamma;
+    a.four_pi = four_pi;
+    std::memcpy(op->args, &a, sizeof a);
 }
 
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,

@@ -58,11 +58,6 @@ struct TridiagArgs {
 };
 // rows: K rows per tile, 2, 4 or 8 (0: the default); the same results for any
 Op residual_op(const ResidualArgs &a, int nplanes, int rows = 0);
-// The cold first residual where the interior of r already holds -4 pi rh
-// (written by the forward FP32 DFT): recompute r at the interior points
-// next to a face (rows K = 2 and numz - 1, column J = numr - 1), the only
-// ones where A x0 is not zero.  Same arithmetic as residual_op, cold.
-Op residual_face_op(const ResidualArgs &a, int nplanes);
 Op add_update_op(const double *X, const float *dlt, double *Xo, int numr, int numz, int nplanes,
                  const float *dlt2 = nullptr, int cold = 0);
 Op tridiag_shift_op(const float *cin, float *yout, const float *ibet, const float *gam, const double *betar, int nri,

@@ -57,41 +57,6 @@ Op residual_op(const ResidualArgs &a, int nplanes, int rows)
     return Op::make(OP_RESIDUAL, v, a, (a.numr - 2 + 255) / 256, (a.numz - 2 + r - 1) / r, nplanes);
 }
 
-// Cold first residual at the interior points next to a face: t < nri row K
-// = 2, t < 2 nri row K = numz - 1, then column J = numr - 1 for K = 3..numz-2.
-// x0 is zero on the interior, so A x0 has only the face terms; the
-// expressions are those of residual_tile_v<false, true, ...>.
-__device__ __forceinline__ void residual_face_tile(const ResidualArgs &a, int bx, int by, int)
-{
-    const int numr = a.numr, numz = a.numz, nri = numr - 2, nzi = numz - 2;
-    const int np = 2 * nri + max(nzi - 2, 0);
-    const int t = bx * 256 + threadIdx.x, L = by;
-    if (t >= np) return;
-    int J, K;
-    if (t < nri) { J = t + 2; K = 2; }
-    else if (t < 2 * nri) { J = t - nri + 2; K = numz - 1; }
-    else { J = numr - 1; K = t - 2 * nri + 3; }
-    const long plane = (long)numr * numz;
-    const double *__restrict__ X = a.X + L * plane;
-    const long ci = (long)(K - 1) * numr + (J - 1);
-    const double brbJ = a.brb[(long)L * (numr + 1) + J], aJ = a.alphar[J], bJ = a.betar[J];
-    const double xc = 0.0, xw = 0.0;
-    const double xe = J + 1 == numr ? X[(long)(K - 1) * numr + numr - 1] : 0.0;
-    const double xkm = K - 1 == 1 ? X[ci - numr] : 0.0;
-    const double xkp = K + 1 == numz ? X[ci + numr] : 0.0;
-    double Ax = brbJ * xc + aJ * xe;
-    Ax += J > 2 ? bJ * xw : 0.0;
-    Ax += 2.0 * a.gamma * xc - a.gamma * (xkm + xkp);
-    const double r = -a.four_pi * (double)a.rh[L * (a.rh_pitch ? a.rh_pitch : plane) + ci] - Ax;
-    a.r[(long)L * nri * nzi + (long)(K - 2) * nri + (J - 2)] = (float)r;
-}
-
-Op residual_face_op(const ResidualArgs &a, int nplanes)
-{
-    const int nri = a.numr - 2, nzi = a.numz - 2, np = 2 * nri + (nzi > 2 ? nzi - 2 : 0);
-    return Op::make(OP_RESIDUAL_FACE, 0, a, (np + 255) / 256, nplanes, 1);
-}
-
 // Xo = X + (dlt + dlt2) on the interior (faces of Xo are kept); a cold X is
 // zero on the interior and is not read
 __device__ __forceinline__ void add_update_tile(const UpdateArgs &a, int bx, int by, int)
@@ -293,7 +258,6 @@ void launch_fillf(float *p, size_t n, float v, hipStream_t s)
     {                                                                                             \
         BH_GROUP_LOOP(ARGS_T, args, gs, (NAME##_tile(ga, bx, by, bz)));                           \
     }
-BH_KERNEL_PAIR(residual_face, ResidualArgs, 256)
 BH_KERNEL_PAIR(add_update, UpdateArgs, 256)
 BH_KERNEL_PAIR(tridiag_shift, TridiagArgs, 256)
 #undef BH_KERNEL_PAIR
@@ -320,10 +284,6 @@ bool mixed_ref(int kind, int variant, const void **single, const void **grouped)
         case 2: *single = (const void *)&residual_kernel<8>; *grouped = (const void *)&residual_grouped<8>; break;
         default: *single = (const void *)&residual_kernel<4>; *grouped = (const void *)&residual_grouped<4>; break;
         }
-        return true;
-    case OP_RESIDUAL_FACE:
-        *single = (const void *)&residual_face_kernel;
-        *grouped = (const void *)&residual_face_grouped;
         return true;
     case OP_UPDATE: *single = (const void *)&add_update_kernel; *grouped = (const void *)&add_update_grouped; return true;
     case OP_TRIDIAG:

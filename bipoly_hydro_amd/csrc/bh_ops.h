@@ -104,7 +104,6 @@ enum OpKind {
     OP_DENSITY,       // bh_density.hip
     OP_DFT,           // bh_dft.hip
     OP_SPECTRAL,      // bh_spectral.hip
-    OP_RESIDUAL_FACE, // bh_mixed.hip
     OP_RES64,         // bh_spec64.hip
     OP_THOMAS64,
     OP_UPDATE64,

@@ -221,7 +221,8 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 // else J-layout).  The side contraction (a short, latency-bound kernel)
 // runs first on the solve's stream, then the top/bottom contraction; the
 // faces also wait for the side stream's earlier work (the forward DFT).
-void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out)
+void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out,
+                  const ResidualArgs *face_res = nullptr)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
@@ -237,6 +238,9 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax, factor, 1,
                                  X1, X2, x2_klayout, g.isym));
+    if (face_res)
+        face_modes_add_residual(&out->back(), face_res->r, face_res->rh, face_res->rh_pitch, face_res->alphar,
+                                face_res->gamma, face_res->four_pi);
 }
 
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
@@ -257,7 +261,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     out->push_back(Op::mark(MARK_FORK));
     // a cold start at N = 512: the DFT also writes the residual's source
     // -4 pi rho-hat on the interior, so that after the faces only the points
-    // next to a face need the stencil (residual_face_op)
+    // next to a face need the stencil (done by the face kernel)
     const bool cold_split = tstep <= 1 && o.dfwd32 && p->use_dft && N == DFT_MAX_N;
     if (o.dfwd32 && p->use_dft) {
         DftArgs d{d_rho, M, p->RH32, p->rh_pitch, o.dfwd32, N, (int)M, numr, numz};
@@ -289,9 +293,10 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     //    A cold start needs no zeroing: the residuals below read only the
     //    faces of Xa, and Xa's interior is rewritten by the final update.
     double *Xa = p->XJ, *Xb = p->XK;
-    boundary_ops(p, Xa, Xb, 0, out);  // joins the side stream
-    out->push_back(Op::mark(2));
-    // 3. r0 = g - A x0
+    // 3. r0 = g - A x0: a cold start at N = 512 has -4 pi rho-hat in r from
+    //    the DFT already, and the face kernel adds the face terms at the
+    //    points next to a face as it forms the faces; otherwise a residual
+    //    kernel after the faces
     ResidualArgs ra{};
     ra.brb = o.brb_d; ra.alphar = o.alphar; ra.betar = o.betar;
     ra.gamma = g.gamma; ra.four_pi = g.four_pi; ra.numr = numr; ra.numz = numz;
@@ -299,7 +304,9 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.rh_pitch = p->rh_pitch;
     ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
     ra.cold = tstep <= 1;
-    out->push_back(cold_split ? residual_face_op(ra, N) : residual_op(ra, N));
+    boundary_ops(p, Xa, Xb, 0, out, cold_split ? &ra : nullptr);  // joins the side stream
+    out->push_back(Op::mark(2));
+    if (!cold_split) out->push_back(residual_op(ra, N));
     // folded sine transforms (spectral slots ordered by spec_mode):
     // forward RT[L] (nri x nzi) = r[L]^T S, inverse dst[L] interior
     // (nzi x nri) = S in[L]^T for nb consecutive modes of in and dst
