@@ -600,7 +600,10 @@ int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s)
         ts = &p->tsets[p->tcount++];
     }
     // after the plan's previous solve, whatever stream it was enqueued on
-    if (p->done_recorded) HIPCHK(hipStreamWaitEvent(s, p->done_ev, 0));
+    if (p->has_last && p->last_stream != s) {
+        HIPCHK(hipEventRecord(p->done_ev, p->last_stream));
+        HIPCHK(hipStreamWaitEvent(s, p->done_ev, 0));
+    }
     hipStream_t cur = s;
     for (const Op &op : ops) {
         if (op.kind == OP_MARK) {
@@ -623,15 +626,18 @@ int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s)
         }
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(p->done_ev, s));
-    p->done_recorded = true;
+    p->last_stream = s;
+    p->has_last = true;
     p->has_solution = true;
     return BH_OK;
 }
 
 int wait_plan_idle(bh_plan *p)
 {
-    if (p->done_recorded) HIPCHK(hipEventSynchronize(p->done_ev));
+    if (p->has_last) {
+        HIPCHK(hipEventRecord(p->done_ev, p->last_stream));
+        HIPCHK(hipEventSynchronize(p->done_ev));
+    }
     return BH_OK;
 }
 

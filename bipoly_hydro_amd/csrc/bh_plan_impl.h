@@ -160,11 +160,14 @@ struct bh_plan {
     int side_cus = 0;   // workgroups of the side-stream DFT (0: two per CU)
     hipStream_t side = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
-    // recorded at the end of every solve enqueued on the plan: the next
-    // solve (on any stream) waits for it, and bh_set_potential synchronises
-    // with it, so that calls on one plan never overlap on the device
+    // the stream of the plan's last solve (every solve ends there, the side
+    // stream joined): a solve on another stream first waits for an event
+    // recorded on it then, and bh_set_potential synchronises with one, so
+    // that calls on one plan never overlap on the device.  Solves on one
+    // stream are ordered by the stream and record nothing.
     hipEvent_t done_ev = nullptr;
-    bool done_recorded = false;
+    hipStream_t last_stream = nullptr;
+    bool has_last = false;
     bool has_solution = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;
