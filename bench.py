@@ -223,7 +223,8 @@ def main():
                                                  (f", {args.passes} refinement passes" if prec.startswith("mixed")
                                                   else "")},
                           "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "stage_ms": res["stage_ms"],
-                          "error_vs_reference_cold": res["error_vs_reference_cold"]}), flush=True)
+                          "error_vs_reference_cold": res["error_vs_reference_cold"],
+                          "error_vs_reference_warm": res["error_vs_reference_warm"]}), flush=True)
         return
     fp64, pot64 = run("fp64", warm_steps=args.steps)
     mixed, potmx = run("mixed", passes=args.passes, warm_steps=args.steps)

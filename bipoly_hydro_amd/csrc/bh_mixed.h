@@ -38,6 +38,10 @@ struct ResidualArgs {
     int numr, numz;
     int cold;  // X is zero on the interior (a cold start): read it only on the faces
     long rh_pitch;  // floats from one plane of rh to the next (0: numr x numz)
+    // the fused refinement step of a warm solve: the first residual
+    // r0 = g - A x0 (FP32, compact like r), taken in place of X and rh, so
+    // that r = r0 - w - A (dlt + dlt2) with zero faces (cold must be 1)
+    const float *r0;
 };
 // Xo = X + (dlt + dlt2) on the interior (dlt2 may be null); cold: X is zero
 // on the interior and is not read

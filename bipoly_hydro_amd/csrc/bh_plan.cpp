@@ -348,11 +348,18 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         float *Dacc = p->passes >= 2 ? p->D2 : p->Wf;
         // with two or more passes (split-bf16 transforms, nzi >= 64) each
         // pass forms its residual inside the spectral kernel, in LDS: no r
-        // written and read back, no residual launch
+        // written and read back, no residual launch.  r32 then keeps the
+        // first residual r0 = g - A x0, from which a warm pass forms its
+        // residual, r0 - w - A (delta1 + Dacc), without reading x0 (as a
+        // cold one, whose x0 is zero inside, reads only x0's faces)
         const bool fuse = p->passes >= 2 && nzi >= 64;
         for (int it = 0; it < p->passes; ++it) {
             ra.X = Xa; ra.dlt = p->D1; ra.dlt2 = it > 0 ? Dacc : nullptr; ra.w = p->Wf; ra.Xo = nullptr;
             ra.r = p->r32; ra.cold = cold;
+            if (fuse && !cold) {
+                ra.r0 = p->r32;
+                ra.cold = 1;
+            }
             if (!fuse) out->push_back(residual_op(ra, N));
             SpecArgs sr{p->r32, o.sfb, o.sfa, nullptr, nullptr, o.tri_ibet, o.betar, o.alphar, Dacc, nullptr,
                         numr, numz, N, it > 0 ? 1 : 0, o.sfb16, o.sfa16, p->passes == 1 ? 1 : 0};

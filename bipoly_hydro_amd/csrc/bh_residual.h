@@ -25,12 +25,16 @@ namespace bh {
 // the refinement pass), whether Xo is written (WX; a solve whose x is
 // formed later by the inverse DFT from the corrections leaves it out),
 // whether a second correction is added (ADD2) and whether X is a cold start
-// (COLD), so that the row loop carries no pointer tests.
+// (COLD), so that the row loop carries no pointer tests.  R0 (with COLD and
+// UPD) takes the first residual r0 = g - A x0 in place of rh and of X: the
+// corrections are then the whole of x, with zero faces, and
+// r = r0 - w - A (dlt + dlt2), the same r as from X = x0 and rh up to the
+// FP32 rounding of r0 (a warm solve's refinement, which then reads no x0).
 constexpr int RES_ROWS = 4;
 
 // tx is the thread's column in the tile; with lout the tile's r goes to
 // lout[(K - 2) * lpitch + J - 2] (LDS, the fused spectral step) instead of r.
-template <int ROWS, bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false>
+template <int ROWS, bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false, bool R0 = false>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz, int tx = threadIdx.x,
                                                 float *lout = nullptr, int lpitch = 0)
 {
@@ -44,7 +48,9 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     const double *__restrict__ X = a.X + L * plane;
     const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
     const float *__restrict__ dl2 = ADD2 ? a.dlt2 + L * plane : nullptr;
+    static_assert(!R0 || (COLD && UPD && !WX), "R0 forms the corrections' residual only");
     const float *__restrict__ rh = a.rh + L * (a.rh_pitch ? a.rh_pitch : plane);
+    const float *__restrict__ r0 = R0 ? a.r0 + (long)L * nri * nzi : nullptr;
     const float *__restrict__ w = UPD ? a.w + L * plane : nullptr;
     double *__restrict__ Xo = WX ? a.Xo + L * plane : nullptr;
     float *__restrict__ r32 = a.r + (long)L * nri * nzi;
@@ -56,7 +62,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     auto corr = [&](long i) { return ADD2 ? (double)dl[i] + (double)dl2[i] : (double)dl[i]; };
     auto val = [&](long i) { return UPD ? X[i] + corr(i) : X[i]; };
     auto inner = [&](long i) { return COLD ? (UPD ? corr(i) : 0.0) : val(i); };
-    auto face = [&](long i) { return COLD ? X[i] : val(i); };
+    auto face = [&](long i) { return R0 ? 0.0 : COLD ? X[i] : val(i); };
     // the lane-dependent cases (the east face at J = numr - 1, no west term
     // at J = 2) are selects, not branches; every address read is inside the
     // plane.  All of the tile's loads are issued before the first use and
@@ -90,14 +96,14 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
         wv[i] = 0.f;
         if (i < nk) {
             if (COLD) {
-                const double xf = X[ci - (J - 1) + numr - 1];  // the row's face value, one address per row
+                const double xf = R0 ? 0.0 : X[ci - (J - 1) + numr - 1];  // the row's face value, one address per row
                 const double xi = inner(ci + 1);
                 xev[i] = east_face ? xf : xi;
             } else {
                 xev[i] = val(ci + 1);
             }
             xwv[i] = inner(ci - 1);
-            rv[i] = rh[ci];
+            rv[i] = R0 ? r0[(long)(K0 - 2 + i) * nri + jr] : rh[ci];
             if (UPD) wv[i] = w[ci];
         }
     }
@@ -109,7 +115,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
             double Ax = brbJ * xc + aJ * xev[i];
             Ax += has_west ? bJ * xwv[i] : 0.0;
             Ax += 2.0 * a.gamma * xc - a.gamma * (xcv[i] + xcv[i + 2]);
-            double r = -a.four_pi * (double)rv[i] - Ax;
+            double r = R0 ? (double)rv[i] - Ax : -a.four_pi * (double)rv[i] - Ax;
             if (UPD) r -= (double)wv[i];
             if (WX) Xo[ci] = xc;
             if (lout) lout[(K0 - 2 + i) * lpitch + jr] = (float)r;

@@ -94,6 +94,9 @@ __device__ __forceinline__ BH_GLOBAL T *gst(void *p)
 // The refinement residual of plane L (FP64, residual_tile_v's arithmetic)
 // into LDS as r[k][j] with pitch rp, tiles of 8 K rows, 256 threads per
 // tile: thread tid takes column tid % 256 of the tiles tid / 256, + 2, ...
+// A cold start (R0 false) reads the faces of X and rh; a warm one (R0: the
+// kernel variant of ra.r0 set) the first residual r0 in place of both.
+template <bool R0>
 __device__ __forceinline__ void spec_residual(const ResidualArgs &ra, int L, int nzi, int tid, float *rl, int rp)
 {
     constexpr int RR = 8;
@@ -102,21 +105,11 @@ __device__ __forceinline__ void spec_residual(const ResidualArgs &ra, int L, int
     for (int kt = tid >> 8; kt < ntiles; kt += SPEC_THREADS / 256) {
         const bool edge = kt == 0 || kt == ntiles - 1;
         if (edge) {
-            if (ra.cold) {
-                if (add2) residual_tile_v<RR, true, true, true, false, true>(ra, 0, kt, L, tx, rl, rp);
-                else residual_tile_v<RR, true, true, true, false, false>(ra, 0, kt, L, tx, rl, rp);
-            } else {
-                if (add2) residual_tile_v<RR, true, false, true, false, true>(ra, 0, kt, L, tx, rl, rp);
-                else residual_tile_v<RR, true, false, true, false, false>(ra, 0, kt, L, tx, rl, rp);
-            }
+            if (add2) residual_tile_v<RR, true, true, true, false, true, R0>(ra, 0, kt, L, tx, rl, rp);
+            else residual_tile_v<RR, true, true, true, false, false, R0>(ra, 0, kt, L, tx, rl, rp);
         } else {
-            if (ra.cold) {
-                if (add2) residual_tile_v<RR, true, true, false, false, true>(ra, 0, kt, L, tx, rl, rp);
-                else residual_tile_v<RR, true, true, false, false, false>(ra, 0, kt, L, tx, rl, rp);
-            } else {
-                if (add2) residual_tile_v<RR, true, false, false, false, true>(ra, 0, kt, L, tx, rl, rp);
-                else residual_tile_v<RR, true, false, false, false, false>(ra, 0, kt, L, tx, rl, rp);
-            }
+            if (add2) residual_tile_v<RR, true, true, false, false, true, R0>(ra, 0, kt, L, tx, rl, rp);
+            else residual_tile_v<RR, true, true, false, false, false, R0>(ra, 0, kt, L, tx, rl, rp);
         }
     }
 }
@@ -125,7 +118,8 @@ __device__ __forceinline__ void spec_residual(const ResidualArgs &ra, int L, int
 // sine rows of azimuthal mode m) run side by side on one XCD, and share M_r
 // and the pivots, stored per mode number mi = L or L - N/2.
 // BFT: split-bf16 sine transforms for delta (and, without MR, the forward one)
-template <int NZ, bool MR, bool ACC, bool BFT>
+// R0: the fused residual (a.fuse_res) of a warm refinement pass, from r0
+template <int NZ, bool MR, bool ACC, bool BFT, bool R0 = false>
 __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int tile)
 {
     typedef Shape<NZ> S;
@@ -174,7 +168,7 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
             // lane's planes from there; the block's space is free again for
             // the table after the second barrier
             const int rp = nri + 2;
-            spec_residual(a.ra, L, NZ, tid, buf, rp);
+            spec_residual<R0>(a.ra, L, NZ, tid, buf, rp);
             __syncthreads();
             if (active) {
 #pragma unroll
@@ -575,28 +569,32 @@ __device__ __forceinline__ void spectral_tile(const SpecArgs &a, float *buf, int
     __syncthreads();  // the block is free for the next mode
 }
 
-template <int NZ, bool MR, bool ACC, bool BFT>
+template <int NZ, bool MR, bool ACC, bool BFT, bool R0>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_kernel(SpecArgs a, int nx, int ny, int total)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR, ACC, BFT>(a, spec_lds, bx)));
+    BH_SINGLE_LOOP(a, nx, ny, total, (spectral_tile<NZ, MR, ACC, BFT, R0>(a, spec_lds, bx)));
 }
 
-template <int NZ, bool MR, bool ACC, bool BFT>
+template <int NZ, bool MR, bool ACC, bool BFT, bool R0>
 __global__ __launch_bounds__(SPEC_THREADS) void spectral_grouped(const SpecArgs *__restrict__ args, GroupSched gs)
 {
     extern __shared__ __align__(16) float spec_lds[];
-    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR, ACC, BFT>(ga, spec_lds, bx)));
+    BH_GROUP_LOOP(SpecArgs, args, gs, (spectral_tile<NZ, MR, ACC, BFT, R0>(ga, spec_lds, bx)));
 }
 
 // variant v: nzi = 32 << (v % 3); v / 3 = 0 plain, 1 with the M_r filter,
 // 2 accumulating delta into dlt, 3 plain with FP32 transforms (the split
-// bf16 ones wherever nzi >= 64 otherwise)
+// bf16 ones wherever nzi >= 64 otherwise), 4 and 5 as 0 and 2 with a warm
+// pass's fused residual from r0 (nzi >= 64)
 const void *spec_fn(int variant, bool grouped)
 {
 #define BH_SPEC_FN(NZ, MR, ACC, BFT)                                                               \
-    (grouped ? (const void *)&spectral_grouped<NZ, MR, ACC, BFT>                                  \
-             : (const void *)&spectral_kernel<NZ, MR, ACC, BFT>)
+    (grouped ? (const void *)&spectral_grouped<NZ, MR, ACC, BFT, false>                           \
+             : (const void *)&spectral_kernel<NZ, MR, ACC, BFT, false>)
+#define BH_SPEC_R0(NZ, ACC)                                                                        \
+    (grouped ? (const void *)&spectral_grouped<NZ, false, ACC, true, true>                        \
+             : (const void *)&spectral_kernel<NZ, false, ACC, true, true>)
     switch (variant) {
     case 0: return BH_SPEC_FN(32, false, false, false);
     case 1: return BH_SPEC_FN(64, false, false, true);
@@ -610,9 +608,14 @@ const void *spec_fn(int variant, bool grouped)
     case 9: return BH_SPEC_FN(32, false, false, false);
     case 10: return BH_SPEC_FN(64, false, false, false);
     case 11: return BH_SPEC_FN(128, false, false, false);
+    case 13: return BH_SPEC_R0(64, false);
+    case 14: return BH_SPEC_R0(128, false);
+    case 16: return BH_SPEC_R0(64, true);
+    case 17: return BH_SPEC_R0(128, true);
     default: return nullptr;
     }
 #undef BH_SPEC_FN
+#undef BH_SPEC_R0
 }
 
 // the block, the segment maps of the Thomas sweeps, the sub- and super-diagonal
@@ -635,7 +638,10 @@ bool spectral_fused_ok(int nri, int nzi)
 
 Op spectral_op(const SpecArgs &a, int nplanes, int nzi)
 {
-    return Op::make(OP_SPECTRAL, spec_variant(nzi) + (a.mr ? 3 : a.acc ? 6 : a.fp32t ? 9 : 0), a, nplanes, 1, 1);
+    const bool r0 = a.fuse_res && a.ra.r0 && !a.mr;
+    return Op::make(OP_SPECTRAL,
+                    spec_variant(nzi) + (a.mr ? 3 : r0 ? (a.acc ? 15 : 12) : a.acc ? 6 : a.fp32t ? 9 : 0), a,
+                    nplanes, 1, 1);
 }
 
 namespace {
