@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["bh_kernels.hip", "bh_gemm.hip", "bh_mixed.hip", "bh_ops.cpp", "bh_resources.cpp", "bh_plan.cpp",
            "bh_exec.cpp", "bh_batch.cpp", "bh_density.hip", "bh_serve.cpp",
-           "bh_dft.hip", "bh_spectral.hip", "bh_noalloc.hip", "bh_cont.cpp", "bh_hydro.hip", "bh_spec64.hip"]
+           "bh_dft.hip", "bh_spectral.hip", "bh_noalloc.hip", "bh_cont.cpp", "bh_hydro.hip", "bh_spec64.hip",
+           "bh_hostcopy.cpp"]
 
 # no packed FP32 instructions (v_pk_*_f32) anywhere in the device code
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
@@ -46,7 +47,7 @@ def build(verbose: bool = False, jobs: int = 0) -> str:
     with ThreadPoolExecutor(jobs) as ex:
         list(ex.map(run, cmds))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

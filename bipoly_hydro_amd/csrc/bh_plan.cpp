@@ -726,14 +726,20 @@ extern "C" int bh_solve(bh_plan *p, const double *rho, double *pot, int tstep)
 {
     if (!p || !rho || !pot) return fail(BH_E_INVALID, "null argument");
     const size_t n = (size_t)p->g.numr * p->g.numz * p->g.numphi;
-    for (size_t i = 0; i < n; ++i)
-        if (!std::isfinite(rho[i])) return fail(BH_E_NONFINITE, "rho holds a non-finite value at index " + std::to_string(i));
     int rc = ensure_staging(p, n);
     if (rc != BH_OK) return rc;
-    HIPCHK(hipMemcpy(p->rho_in, rho, n * sizeof(double), hipMemcpyHostToDevice));
+    if (!p->stager) p->stager.reset(new HostStager(p->device, host_copy_workers()));
+    // rho in through the pinned ring, checked for finiteness as it is
+    // copied; the solve is enqueued behind the transfers and phi comes back
+    // the same way
+    long bad = -1;
+    rc = p->stager->to_device(p->rho_in, rho, n, 0, true, &bad);
+    if (rc != BH_OK) return fail(rc, "copy of rho to the device failed");
+    if (bad >= 0) return fail(BH_E_NONFINITE, "rho holds a non-finite value at index " + std::to_string(bad));
     rc = enqueue_solve(p, p->rho_in, p->pot_out, tstep, 0);
     if (rc != BH_OK) return rc;
-    HIPCHK(hipMemcpy(pot, p->pot_out, n * sizeof(double), hipMemcpyDeviceToHost));
+    rc = p->stager->to_host(pot, p->pot_out, n, 0);
+    if (rc != BH_OK) return fail(rc, "copy of the potential to the host failed");
     return collect_timing(p);
 }
 

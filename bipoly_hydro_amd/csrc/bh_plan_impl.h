@@ -2,6 +2,7 @@
 // batch executor and the serving loop.
 #pragma once
 #include "../../include/bh_potential.h"
+#include "bh_hostcopy.h"
 #include "bh_internal.h"
 #include "bh_ops.h"
 
@@ -145,6 +146,7 @@ struct bh_plan {
     // LineArgs (TK after the r half-step, TJ after the z half-step)
     double *TJ = nullptr, *TK = nullptr;
     double *rho_in = nullptr, *pot_out = nullptr;  // host-path staging
+    std::unique_ptr<bh::HostStager> stager;        // host-path copies (created on first use)
     // the source step (bh_hydro.hip): r, rhf, rinv, rhfinv [4][numr + 1]
     // (1-based J), and the centre-of-mass arrays cylin_a_com, cylin_v_com
     // [2][numphi][3]

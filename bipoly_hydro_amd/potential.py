@@ -228,12 +228,20 @@ class PotentialPlan:
     def device_bytes(self) -> int:
         return load_library().bh_plan_device_bytes(self._h)
 
-    def potential_solver(self, rho: np.ndarray, tstep: int = 0) -> np.ndarray:
-        """Host arrays in, host potential out (synchronous)."""
+    def potential_solver(self, rho: np.ndarray, tstep: int = 0, out: np.ndarray = None) -> np.ndarray:
+        """Host arrays in, host potential out (synchronous).  `out`, when
+        given, is a C-contiguous float64 array of the grid's shape that
+        receives the potential (a caller-owned array, as the reference's
+        COMMON block is); otherwise a new array is returned."""
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         if rho.shape != self.shape:
             raise ValueError(f"rho shape {rho.shape} != {self.shape}")
-        pot = np.empty(self.shape, dtype=np.float64)
+        if out is None:
+            pot = np.empty(self.shape, dtype=np.float64)
+        else:
+            if out.shape != self.shape or out.dtype != np.float64 or not out.flags["C_CONTIGUOUS"]:
+                raise ValueError("out must be a C-contiguous float64 array of the grid's shape")
+            pot = out
         dp = ctypes.POINTER(ctypes.c_double)
         _check(load_library().bh_solve(self._h, rho.ctypes.data_as(dp), pot.ctypes.data_as(dp), int(tstep)))
         return pot

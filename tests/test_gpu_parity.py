@@ -174,16 +174,34 @@ def test_device_entry_point_matches_host_entry_point():
         plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), tstep=0, stream=stream)
         torch.cuda.synchronize()
         np.testing.assert_array_equal(d_pot.cpu().numpy(), host)
+        # into a caller-owned array (as the reference's COMMON block)
+        out = np.full_like(rho, np.nan)
+        assert plan.potential_solver(rho, tstep=0, out=out) is out
+        np.testing.assert_array_equal(out, host)
+        with pytest.raises(ValueError):
+            plan.potential_solver(rho, tstep=0, out=np.empty(rho.shape, dtype=np.float32))
 
 
 def test_error_paths():
     rho = density.config1_density(seed=3)
     with P.PotentialPlan(66, 66, 64) as plan:
+        good = plan.potential_solver(rho)
         bad = rho.copy()
         bad[3, 4, 5] = np.nan
+        bad[60, 10, 10] = np.nan
+        bad[40, 0, 0] = -np.inf
         with pytest.raises(P.BHError) as ei:
             plan.potential_solver(bad)
         assert ei.value.code == -4
+        # the first non-finite value in memory order, whichever copy thread
+        # met it (the array is split into stripes, one per thread)
+        assert f"index {(3 * 66 + 4) * 66 + 5}" in str(ei.value)
+        bad[3, 4, 5] = 1.0
+        with pytest.raises(P.BHError) as ei:
+            plan.potential_solver(bad)
+        assert f"index {40 * 66 * 66}" in str(ei.value)
+        # a rejected call leaves the plan as it was
+        np.testing.assert_array_equal(plan.potential_solver(rho), good)
     with pytest.raises(P.BHError) as ei:
         P.PotentialPlan(66, 66, 64, hbm_budget=1 << 20)
     assert ei.value.code == -2
