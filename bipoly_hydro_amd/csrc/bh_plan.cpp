@@ -218,22 +218,25 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 
 // Boundary potential (bessel.F90:186-488) -> Fourier-space faces FF, also
 // placed in the faces of X1 (J-layout) and X2 (K-layout when x2_klayout,
-// else J-layout).  The side contraction (a short, latency-bound kernel)
-// runs first on the solve's stream, then the top/bottom contraction; the
-// faces also wait for the side stream's earlier work (the forward DFT).
+// else J-layout).  The side contraction (short and latency-bound) runs on
+// the solve's stream right after the projections, beside the side stream's
+// forward DFT; the solve's stream then joins the side stream, so that the
+// top/bottom contraction streams its table with HBM to itself.  (With the
+// DFT beside the table stream instead the solve takes the same time: the
+// boundary window is bound by its total HBM traffic.)
 void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out,
                   const ResidualArgs *face_res = nullptr)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
+    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax, p->tab->H));
+    out->push_back(Op::mark(MARK_JOIN));
     out->push_back(Op::mark(5));
     if (!tb_blocked(numr)) out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
     out->push_back(contract_topsym_op(p->tab->T, p->tab->T32, p->proj, p->tab->sqrt_rhf, p->Pt, p->Ptb, numr, numz,
                                       mmax));
     out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
     out->push_back(Op::mark(6));
-    out->push_back(contract_side_op(p->tab->S, p->proj, p->Ss, numr, numz, mmax, p->tab->H));
-    out->push_back(Op::mark(MARK_JOIN));
     // factor of bessel.F90:170-174 (doubled for isym = 3)
     const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax, factor, 1,
