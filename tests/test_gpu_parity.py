@@ -310,6 +310,37 @@ def check_c2_digests(pot, g, tag, tol):
         assert e <= tol and worst <= tol, (tag, k, e, worst)
 
 
+def test_mixed_warm_start_from_the_kept_state():
+    """At numphi = 512 (the radix kernels, the cold start's split first
+    residual, the warm refinement passes from r0), two chained warm mixed
+    solves from the state a solve keeps agree with the same solves from the
+    imported potential far below the mixed bar, and meet the bar against
+    the FP64 plan."""
+    numr, numz, numphi = 66, 34, 512
+    rho1 = density.scaled_binary(numr, numz, numphi, seed=21)
+    rho2 = rho1 * 1.003 + 1e-3 * density.scaled_binary(numr, numz, numphi, seed=22)
+    rho3 = rho2 * 0.998
+    with P.PotentialPlan(numr, numz, numphi, precision="mixed") as a, \
+            P.PotentialPlan(numr, numz, numphi, precision="mixed") as b, \
+            P.PotentialPlan(numr, numz, numphi, precision="fp64") as f:
+        p1 = a.potential_solver(rho1, tstep=0)
+        p2 = a.potential_solver(rho2, tstep=2)  # from the kept state
+        p3 = a.potential_solver(rho3, tstep=2)
+        b.potential_solver(rho1, tstep=0)
+        b.set_potential(p1)  # from an import
+        q2 = b.potential_solver(rho2, tstep=2)
+        b.set_potential(p2)
+        q3 = b.potential_solver(rho3, tstep=2)
+        f.potential_solver(rho1, tstep=0)
+        f2 = f.potential_solver(rho2, tstep=2)
+        f3 = f.potential_solver(rho3, tstep=2)
+    e2, e3 = rell2(p2, q2), rell2(p3, q3)
+    g2, g3 = rell2(p2, f2), rell2(p3, f3)
+    print(f"warm from state vs from import: {e2:.3e} {e3:.3e}; vs fp64: {g2:.3e} {g3:.3e}", flush=True)
+    assert e2 < 1e-10 and e3 < 1e-10
+    assert g2 <= TOL_MIXED and g3 <= TOL_MIXED and rell2(q2, f2) <= TOL_MIXED
+
+
 @pytest.mark.parametrize("shape,seed", [((98, 34, 64), 11), ((290, 18, 16), 3)])
 def test_mixed_matches_oracle_on_other_grids(shape, seed):
     """The mixed path (FP32 table copy, FP32 transforms) off the golden
