@@ -152,6 +152,12 @@ int batch_enqueue(bh_batch *b, const double *const *d_rho, double *const *d_pot,
 {
     const int n = (int)b->plans.size();
     b->lists.resize(n);
+    // a state left as a cold solve's corrections (a single-plan solve of a
+    // batch's plan) is written out first, outside any captured graph
+    for (int i = 0; i < n; ++i) {
+        const int rc = materialize_state(b->plans[i], s);
+        if (rc != BH_OK) return rc;
+    }
     for (int i = 0; i < n; ++i) build_ops(b->plans[i], d_rho[i], d_pot[i], tstep ? tstep[i] : 0, &b->lists[i]);
     if ((b->flags & BH_BATCH_AUTOTUNE) && !b->tune_init) init_tuning(b);
     const bool tuning = (b->flags & BH_BATCH_AUTOTUNE) && b->tune_ax < (int)b->tune_axes.size();

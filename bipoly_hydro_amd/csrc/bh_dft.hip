@@ -474,7 +474,9 @@ constexpr int R8_FRAGS = 4 * R8KG * 64 * 4;  // doubles of the size-n P and Q fr
 // x + (xint + xadd), the pair bit-packed beside x.  Each mode is its own
 // kernel, so that the registers held across a panel are only those the
 // mode needs.
-enum { INV_PLAIN = 0, INV_ADD = 1, INV_COLD = 2, INV_ADD2 = 3 };
+// INV_COLD_NS: INV_COLD without the store of x (a cold start whose
+// warm-start state is left as x's faces and the two corrections).
+enum { INV_PLAIN = 0, INV_ADD = 1, INV_COLD = 2, INV_ADD2 = 3, INV_COLD_NS = 4 };
 template <int MODE>
 struct Inv8Regs {
     double av[8], sv[8];
@@ -521,7 +523,7 @@ __device__ __forceinline__ void inv8_load(const DftArgs &a, int bx, Inv8Regs<MOD
     const int kk = tid / R4CB, c = tid - kk * R4CB, gc = min(bx * R4CB + c, a.M - 1);
     // one branch around all the loads of a kind (a branch per load makes
     // the compiler wait where the paths join)
-    if (MODE == INV_COLD && inv8_inner(a, gc)) {
+    if ((MODE == INV_COLD || MODE == INV_COLD_NS) && inv8_inner(a, gc)) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const long o0 = (long)inv8_slot_row(kk, q, 0) * ldx + gc, o1 = (long)inv8_slot_row(kk, q, 1) * ldx + gc;
@@ -584,12 +586,12 @@ __device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Re
     const double *tw = (const double *)a.A + R8_FRAGS;  // [k][r] (cos, sin)(2 pi k r / N), k = 0..n/2
     const int kk = tid / R4CB, c = tid - kk * R4CB, gc = bx * R4CB + c;
     double av[8], sv[8];
-    const bool inner = MODE == INV_COLD && inv8_inner(a, min(gc, a.M - 1));
+    const bool inner = (MODE == INV_COLD || MODE == INV_COLD_NS) && inv8_inner(a, min(gc, a.M - 1));
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         av[q] = R.av[q];
         sv[q] = R.sv[q];
-        if constexpr (MODE == INV_COLD) {
+        if constexpr (MODE == INV_COLD || MODE == INV_COLD_NS) {
             if (inner) {
                 const inv_f2 pa = __builtin_bit_cast(inv_f2, R.av[q]), ps = __builtin_bit_cast(inv_f2, R.sv[q]);
                 av[q] = (double)pa.x + (double)pa.y;
@@ -612,7 +614,7 @@ __device__ __forceinline__ void inv8_form(const DftArgs &a, int bx, const Inv8Re
             sb[q] = a.xadd[o1];
         }
     }
-    if constexpr (MODE != INV_PLAIN) {
+    if constexpr (MODE != INV_PLAIN && MODE != INV_COLD_NS) {
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             if constexpr (MODE == INV_ADD) {
@@ -1064,7 +1066,8 @@ Op dft_op(const DftArgs &a, int kind)
     const int CB = a.N == DFT_MAX_N ? (kind == DFT_FWD_F32 ? R4Fwd<float>::CB : R4CB) : dft_cols(kind);
     // variant: kind + 4 log2 N + 64 (input mode of the inverse at N = 512)
     const int mode = kind == DFT_INV_F64 && a.N == DFT_MAX_N
-                         ? (a.xint ? (a.warm ? INV_ADD2 : INV_COLD) : a.xadd ? INV_ADD : INV_PLAIN)
+                         ? (a.xint ? (a.warm ? INV_ADD2 : a.xstore ? INV_COLD : INV_COLD_NS)
+                                   : a.xadd ? INV_ADD : INV_PLAIN)
                          : 0;
     return Op::make(OP_DFT, kind + 4 * log2i(a.N) + 64 * mode, a, (a.M + CB - 1) / CB, 1, 1);
 }
@@ -1090,6 +1093,7 @@ static const void *r4_fn(int variant, bool grouped, int *lds)
         case INV_PLAIN: return grouped ? (const void *)&dft_inv8_grouped<0> : (const void *)&dft_inv8_kernel<0>;
         case INV_ADD: return grouped ? (const void *)&dft_inv8_grouped<1> : (const void *)&dft_inv8_kernel<1>;
         case INV_COLD: return grouped ? (const void *)&dft_inv8_grouped<2> : (const void *)&dft_inv8_kernel<2>;
+        case INV_COLD_NS: return grouped ? (const void *)&dft_inv8_grouped<4> : (const void *)&dft_inv8_kernel<4>;
         default: return grouped ? (const void *)&dft_inv8_grouped<3> : (const void *)&dft_inv8_kernel<3>;
         }
     case DFT_FWD_F32:

@@ -29,6 +29,9 @@ inline double sine_basis(int a, int k, int n)
 // dlt and w are given together (the refinement pass; Xo = X + dlt is then
 // written unless Xo is null) or both null (the first residual).  dlt2 (with
 // dlt, and Xo null) is a second FP32 correction: x = X + (dlt + dlt2).
+// dlt, dlt2 and Xo with w null and cold set: the first residual of a warm
+// start from a cold solve's kept corrections, x = X's faces + (dlt + dlt2)
+// inside, also written to Xo.
 struct ResidualArgs {
     const double *X; const float *dlt; const float *w; const float *rh;
     double *Xo; float *r;

@@ -30,6 +30,10 @@ namespace bh {
 template <int ROWS, bool EDGE>
 __device__ __forceinline__ void residual_tile_e(const ResidualArgs &a, int bx, int by, int bz)
 {
+    if (a.Xo && a.dlt2) {  // a warm start from a cold solve's two corrections (no w)
+        residual_tile_v<ROWS, true, true, EDGE, true, true, false, true>(a, bx, by, bz);
+        return;
+    }
     switch ((a.dlt ? (a.Xo ? 2 : a.dlt2 ? 6 : 4) : 0) + (a.cold ? 1 : 0)) {
     case 0: residual_tile_v<ROWS, false, false, EDGE, false>(a, bx, by, bz); break;
     case 1: residual_tile_v<ROWS, false, true, EDGE, false>(a, bx, by, bz); break;

@@ -249,7 +249,12 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
 // The mixed-precision solve.  Evaluates the same truncated Peaceman-Rachford
 // iteration as the FP64 path (helmadi.F90:195-355), as A x = g - M r0 (see
 // bh_mixed.hip), with FP32 MFMA transforms and FP64 residuals.
-void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out)
+// the second correction of a mixed solve: D2 with two or more passes, the w
+// buffer at one pass (free once the one refinement residual has read w)
+float *correction_sum(const bh_plan *p) { return p->passes >= 2 ? p->D2 : p->Wf; }
+
+void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out,
+               bool allow_lazy)
 {
     const Geo &g = p->g;
     const ModeOps &o = *p->ops;
@@ -309,7 +314,19 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.cold = tstep <= 1;
     boundary_ops(p, Xa, Xb, 0, out, cold_split ? &ra : nullptr);  // joins the side stream
     out->push_back(Op::mark(2));
-    if (!cold_split) out->push_back(residual_op(ra, N));
+    if (!cold_split) {
+        ResidualArgs r1 = ra;
+        if (tstep > 1 && p->lazy_state) {
+            // x0 inside is the last (cold) solve's delta1 + correction sum:
+            // formed here, written to Xa, the same x0 as if stored whole
+            r1.dlt = p->D1;
+            r1.dlt2 = correction_sum(p);
+            r1.Xo = Xa;
+            r1.w = nullptr;
+            r1.cold = 1;
+        }
+        out->push_back(residual_op(r1, N));
+    }
     // folded sine transforms (spectral slots ordered by spec_mode):
     // forward RT[L] (nri x nzi) = r[L]^T S, inverse dst[L] interior
     // (nzi x nri) = S in[L]^T for nb consecutive modes of in and dst
@@ -348,7 +365,7 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         //    correction after the first is about 1e-7 of x, so their FP32 sum
         //    holds them to ~1e-14 of x.  With one pass Dacc is the w buffer,
         //    free once the one refinement residual has read w.
-        float *Dacc = p->passes >= 2 ? p->D2 : p->Wf;
+        float *Dacc = correction_sum(p);
         // with two or more passes (split-bf16 transforms, nzi >= 64) each
         // pass forms its residual inside the spectral kernel, in LDS: no r
         // written and read back, no residual launch.  r32 then keeps the
@@ -379,6 +396,9 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
             DftArgs d{Xa, M, d_pot, M, o.dinv64, N, (int)M, numr, numz, Dacc, Xa};
             d.xint = p->D1;
             d.warm = cold ? 0 : 1;
+            // a lazy cold start keeps x inside as delta1 and Dacc, which
+            // stay in place (the faces are in Xa already): x is not stored
+            if (allow_lazy && lazy_cold(p, tstep)) d.xstore = nullptr;
             d.axis_numr = numr;
             out->push_back(dft_op(d, DFT_INV_F64));
         } else {
@@ -590,10 +610,17 @@ int collect_timing(bh_plan *p)
 
 }  // namespace
 
-void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out)
+bool lazy_cold(const bh_plan *p, int tstep)
+{
+    return tstep <= 1 && is_mixed(p->precision) && p->ops->spec_fused && p->ops->dinv64 && p->use_dft &&
+           p->g.numphi == DFT_MAX_N;
+}
+
+void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out,
+               bool allow_lazy)
 {
     out->clear();
-    if (is_mixed(p->precision)) mixed_ops(p, d_rho, d_pot, tstep, out);
+    if (is_mixed(p->precision)) mixed_ops(p, d_rho, d_pot, tstep, out, allow_lazy);
     else if (is_spec64(p->precision)) spec64_ops(p, d_rho, d_pot, tstep, out);
     else fp64_ops(p, d_rho, d_pot, tstep, out);
 }
@@ -642,6 +669,19 @@ int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s)
     return BH_OK;
 }
 
+int materialize_state(bh_plan *p, hipStream_t s)
+{
+    if (!p->lazy_state) return BH_OK;
+    const Geo &g = p->g;
+    const std::vector<Op> ops{add_update_op(p->XJ, p->D1, p->XJ, g.numr, g.numz, g.numphi, correction_sum(p), 1)};
+    const bool timing = p->timing;
+    p->timing = false;  // not a solve: no stage events
+    const int rc = enqueue_ops(p, ops, s);
+    p->timing = timing;
+    if (rc == BH_OK) p->lazy_state = false;
+    return rc;
+}
+
 int wait_plan_idle(bh_plan *p)
 {
     if (p->has_last) {
@@ -658,8 +698,10 @@ namespace {
 int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hipStream_t s)
 {
     std::vector<Op> ops;
-    build_ops(p, d_rho, d_pot, tstep, &ops);
-    return enqueue_ops(p, ops, s);
+    build_ops(p, d_rho, d_pot, tstep, &ops, true);
+    const int rc = enqueue_ops(p, ops, s);
+    p->lazy_state = rc == BH_OK && lazy_cold(p, tstep);
+    return rc;
 }
 
 int ensure_staging(bh_plan *p, size_t n)
@@ -755,6 +797,7 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
     const double *src = pot;
     int rc = wait_plan_idle(p);  // a solve still running reads and writes XJ
     if (rc != BH_OK) return rc;
+    p->lazy_state = false;  // XJ is rewritten whole
     if (!on_device) {
         rc = ensure_staging(p, n);
         if (rc != BH_OK) return rc;
@@ -785,9 +828,13 @@ extern "C" int bhx_gemm_f64(const double *A, long lda, const double *B, long ldb
 }
 
 // which: 0 RJ, 1 RK (compact K-layout interior), 2 XJ, 3 XK, 4 proj, 5 T (blocked symmetric), 6 S, 7 Stb, 8 Ss, 9 (unused), 10 FF
-extern "C" int bhx_snapshot(const bh_plan *p, int which, double *host, size_t n)
+extern "C" int bhx_snapshot(bh_plan *p, int which, double *host, size_t n)
 {
     if (!p || !host) return fail(BH_E_INVALID, "null argument");
+    if (which == 2) {  // the warm-start state, whole
+        const int rc = materialize_state(p, 0);
+        if (rc != BH_OK) return rc;
+    }
     const Geo &g = p->g;
     const size_t M = (size_t)g.numr * g.numz, N = g.numphi, mm = g.mmax, nri = g.nri, nzi = g.nzi;
     const size_t nf = 2 * nri + nzi;

@@ -171,6 +171,11 @@ struct bh_plan {
     hipStream_t last_stream = nullptr;
     bool has_last = false;
     bool has_solution = false;
+    // mixed: the interior of the warm-start state XJ is D1 + (D2, or Wf at
+    // one pass) -- a cold solve through the single-plan entry points leaves
+    // it there instead of writing it; XJ's faces are current.  The next warm
+    // solve's first residual writes it out (or materialize_state does).
+    bool lazy_state = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;
     std::vector<bh::TimingSet> tsets;
@@ -192,7 +197,13 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
 // The solve of one grid as Ops, with stage marks 0-4 (start, after the
 // forward DFT, after the boundary, after the ADI, end) and 5-6 around the
 // top/bottom boundary contraction.
-void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out);
+// allow_lazy (the single-plan entry points): a cold mixed solve at N = 512
+// may leave its warm-start state as its two corrections (lazy_cold).  A
+// warm solve of a plan whose state is so left writes it out as it forms
+// its first residual, whatever allow_lazy.
+void build_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out,
+               bool allow_lazy = false);
+bool lazy_cold(const bh_plan *p, int tstep);
 // Launch the Ops of one solve on s (records timing events when enabled).
 int enqueue_ops(bh_plan *p, const std::vector<Op> &ops, hipStream_t s);
 // Forward real DFT along phi (FP64) of the numphi rows of B (Mcols columns,
@@ -202,4 +213,7 @@ void dft_fwd64_ops(const bh_plan *p, const double *B, long ldb, int Mcols, doubl
                    std::vector<Op> *out, int max_blocks = 0);
 // Wait until the plan's last enqueued solve has finished.
 int wait_plan_idle(bh_plan *p);
+// Write out a warm-start state a cold solve left as its two corrections
+// (enqueued on s, after the plan's last solve); nothing when it is whole.
+int materialize_state(bh_plan *p, hipStream_t s);
 }  // namespace bh

@@ -30,11 +30,15 @@ namespace bh {
 // corrections are then the whole of x, with zero faces, and
 // r = r0 - w - A (dlt + dlt2), the same r as from X = x0 and rh up to the
 // FP32 rounding of r0 (a warm solve's refinement, which then reads no x0).
+// NOW (with UPD, COLD, WX and ADD2, no w): the first residual of a warm
+// start whose state was left as its faces (in X) and the two corrections of
+// the cold solve before it: x0 = X's faces + (dlt + dlt2) inside, written
+// to Xo, and r = -4 pi rh - A x0.
 constexpr int RES_ROWS = 4;
 
 // tx is the thread's column in the tile; with lout the tile's r goes to
 // lout[(K - 2) * lpitch + J - 2] (LDS, the fused spectral step) instead of r.
-template <int ROWS, bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false, bool R0 = false>
+template <int ROWS, bool UPD, bool COLD, bool EDGE, bool WX, bool ADD2 = false, bool R0 = false, bool NOW = false>
 __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, int by, int bz, int tx = threadIdx.x,
                                                 float *lout = nullptr, int lpitch = 0)
 {
@@ -49,9 +53,10 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
     const float *__restrict__ dl = UPD ? a.dlt + L * plane : nullptr;
     const float *__restrict__ dl2 = ADD2 ? a.dlt2 + L * plane : nullptr;
     static_assert(!R0 || (COLD && UPD && !WX), "R0 forms the corrections' residual only");
+    static_assert(!NOW || (COLD && UPD && WX && ADD2 && !R0), "NOW forms a warm start's first residual only");
     const float *__restrict__ rh = a.rh + L * (a.rh_pitch ? a.rh_pitch : plane);
     const float *__restrict__ r0 = R0 ? a.r0 + (long)L * nri * nzi : nullptr;
-    const float *__restrict__ w = UPD ? a.w + L * plane : nullptr;
+    const float *__restrict__ w = UPD && !NOW ? a.w + L * plane : nullptr;
     double *__restrict__ Xo = WX ? a.Xo + L * plane : nullptr;
     float *__restrict__ r32 = a.r + (long)L * nri * nzi;
     const double brbJ = a.brb[(long)L * (numr + 1) + J], aJ = a.alphar[J], bJ = a.betar[J];
@@ -104,7 +109,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
             }
             xwv[i] = inner(ci - 1);
             rv[i] = R0 ? r0[(long)(K0 - 2 + i) * nri + jr] : rh[ci];
-            if (UPD) wv[i] = w[ci];
+            if (UPD && !NOW) wv[i] = w[ci];
         }
     }
 #pragma unroll
@@ -116,7 +121,7 @@ __device__ __forceinline__ void residual_tile_v(const ResidualArgs &a, int bx, i
             Ax += has_west ? bJ * xwv[i] : 0.0;
             Ax += 2.0 * a.gamma * xc - a.gamma * (xcv[i] + xcv[i + 2]);
             double r = R0 ? (double)rv[i] - Ax : -a.four_pi * (double)rv[i] - Ax;
-            if (UPD) r -= (double)wv[i];
+            if (UPD && !NOW) r -= (double)wv[i];
             if (WX) Xo[ci] = xc;
             if (lout) lout[(K0 - 2 + i) * lpitch + jr] = (float)r;
             else r32[(long)(K0 - 2 + i) * nri + jr] = (float)r;

@@ -486,6 +486,7 @@ void admit(bh_server *S)
                 const long M = (long)p->g.numr * p->g.numz;
                 std::vector<Op> ops;
                 dft_fwd64_ops(p, sh->pot, M, (int)M, p->XJ, M, false, &ops);
+                p->lazy_state = false;  // XJ is rewritten whole
                 for (const Op &op : ops) launch_op(op, ls);
                 e = hipGetLastError();
             }
