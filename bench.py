@@ -29,10 +29,10 @@ HBM_PEAK_GBS = 8000.0
 # profiles/potential_solver_258x130x512_hbm_traffic.md
 CONTRACT_TRAFFIC = ((546340.0 + 2332.0) * 2.0 + (7200.0 + 240.0)) * 1024  # FP64 table
 # the mixed cold inverse DFT (radix 8, x = faces + delta1 + correction sum
-# in, phi and the warm-start state out): HBM bytes per launch (FETCH x 2.000
-# + WRITE) and the FP64 MFMA operations it executes
+# in, phi out; the warm-start state stays as the two corrections): HBM bytes
+# per launch (FETCH x 2.000 + WRITE) and the FP64 MFMA operations it executes
 # (SQ_INSTS_VALU_MFMA_MOPS_F64 x 512); profiles/potential_solver_258x130x512_{hbm_traffic,mfma}.md
-INV_TRAFFIC = (77654.0 * 2.0 + 294420.0) * 1024
+INV_TRAFFIC = (74626.0 * 2.0 + 138799.0) * 1024
 INV_MFMA_FLOP = 2147328.0 * 512
 FP64_PEAK_TFLOPS = 78.6
 SWEEP_TRAFFIC = ((159728.0 + 158982.0) * 2.0 + (131072.0 + 131512.0)) * 1024 / 2  # mean of the two half-steps
@@ -293,10 +293,11 @@ def main():
     ct_traffic = CONTRACT_TRAFFIC if (numr, numz, numphi) == (258, 130, 512) else None
     # the inverse DFT of the mixed cold solve (the stage from mark 3 to 4 is
     # this one kernel at N = 512): x's faces, delta1 and the correction sum
-    # in (FP32 interior), phi and the warm-start state out
+    # in (FP32 interior), phi out (a cold solve keeps its warm-start state as
+    # the two corrections)
     c2 = (numr, numz, numphi) == (258, 130, 512)
     inv_bytes = 2 * 4 * (numr - 2) * (numz - 2) * numphi + (numr * numz - (numr - 2) * (numz - 2)) * numphi * 8 + \
-        2 * 8 * numr * numz * numphi
+        8 * numr * numz * numphi
     inv_ms = mixed["stage_ms"]["inverse_dft"]
     inv_traffic = INV_TRAFFIC if c2 else None
     inv_flop = INV_MFMA_FLOP if c2 else None
