@@ -199,6 +199,19 @@ def main():
         if warm_steps:
             w = timed(plan, 2, warm_steps)
             res["warm"] = {k: w[k] for k in ("ms_per_step", "p50_ms", "p99_ms")}
+            # a cold solve and the warm one that follows it (the hydro loop's
+            # first two steps): a mixed cold solve leaves its warm-start
+            # state as its corrections, which that warm solve writes out
+            ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(10)]
+            for e in ev:
+                e[0].record(stream)
+                plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 0, sp)
+                e[1].record(stream)
+                plan.potential_solver_device(d_rho.data_ptr(), d_pot.data_ptr(), 2, sp)
+                e[2].record(stream)
+            torch.cuda.synchronize()
+            res["warm"]["cold_then_warm_ms"] = [float(np.median([e[0].elapsed_time(e[1]) for e in ev])),
+                                                float(np.median([e[1].elapsed_time(e[2]) for e in ev]))]
         res["error_vs_reference_cold"] = reference_errors(pot_cold, "1")
         res["error_vs_reference_warm"] = reference_errors(pot_warm, "2")
         plan.close()
@@ -358,6 +371,8 @@ def main():
         "error_per_pass": per_pass,
         "mixed_over_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
         "warm": {"mixed_ms": mixed["warm"]["ms_per_step"], "fp64_ms": fp64["warm"]["ms_per_step"],
+                 "mixed_cold_then_warm_ms": mixed["warm"]["cold_then_warm_ms"],
+                 "fp64_cold_then_warm_ms": fp64["warm"]["cold_then_warm_ms"],
                  "mixed_over_fp64": fp64["warm"]["ms_per_step"] / mixed["warm"]["ms_per_step"]},
         "mixed_table32": t32,
         "fp64_spectral": s64,
