@@ -801,7 +801,9 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
     if (!on_device) {
         rc = ensure_staging(p, n);
         if (rc != BH_OK) return rc;
-        HIPCHK(hipMemcpy(p->pot_out, pot, n * sizeof(double), hipMemcpyHostToDevice));
+        if (!p->stager) p->stager.reset(new HostStager(p->device, host_copy_workers()));
+        rc = p->stager->to_device(p->pot_out, pot, n, 0, false, nullptr);  // through the pinned ring
+        if (rc != BH_OK) return fail(rc, "copy of the potential to the device failed");
         src = p->pot_out;
     }
     std::vector<Op> ops;
