@@ -390,7 +390,8 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
         // 7. x = X0 + (delta1 + Dacc) into Xa, the warm-start state, then the
         //    inverse DFT (FP64) and the axis ghost; at N = 512 the panel DFT
         //    does all three in one pass (in place is safe: every element of
-        //    Xa is read and written by the same thread)
+        //    Xa is read and written by the same thread), and a lazy cold
+        //    start (lazy_cold) leaves x inside as delta1 and Dacc instead
         if (inv512) {
             out->push_back(Op::mark(3));
             DftArgs d{Xa, M, d_pot, M, o.dinv64, N, (int)M, numr, numz, Dacc, Xa};
