@@ -312,7 +312,9 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.rh_pitch = p->rh_pitch;
     ra.X = Xa; ra.dlt = nullptr; ra.w = nullptr; ra.Xo = nullptr; ra.r = p->r32;
     ra.cold = tstep <= 1;
-    boundary_ops(p, Xa, Xb, 0, out, cold_split ? &ra : nullptr);  // joins the side stream
+    // (the fused spectral path never reads Xb: its faces are only needed by
+    // the separate kernels' second iterate)
+    boundary_ops(p, Xa, o.spec_fused ? nullptr : Xb, 0, out, cold_split ? &ra : nullptr);  // joins the side stream
     out->push_back(Op::mark(2));
     if (!cold_split) {
         ResidualArgs r1 = ra;
