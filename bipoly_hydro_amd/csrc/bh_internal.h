@@ -110,6 +110,12 @@ struct RebuildArgs {
     long rh_pitch;
     const double *alphar;
     double gamma, four_pi;
+    // kmap > 0: the launch covers only the mode rows that boundary modes
+    // reach (fstep = 1, mmax <= numphi / 2): cosine rows 0..kmap-1 and sine
+    // rows numphi/2 + 1 .. numphi/2 + kmap - 1, grid row by -> mode row; the
+    // faces of every other row are zero in every solve and already zero in
+    // the potential buffers
+    int kmap;
 };
 
 // grid-stride fill for plan set-up
@@ -204,8 +210,10 @@ Op contract_side_op(const double *S, const double *proj, double *Ss, int numr, i
 // at the interior points next to a face (see RebuildArgs::r)
 void face_modes_add_residual(Op *op, float *r, const float *rh, long rh_pitch, const double *alphar, double gamma,
                              double four_pi);
+// reach_rows: launch over the rows boundary modes reach only (see
+// RebuildArgs::kmap; taken when isym < 3 and mmax <= numphi / 2)
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
                  int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1 = nullptr,
-                 double *X2 = nullptr, int x2_klayout = 0, int isym = 1);
+                 double *X2 = nullptr, int x2_klayout = 0, int isym = 1, bool reach_rows = false);
 
 }  // namespace bh

@@ -1445,7 +1445,7 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     const int nri = numr - 2, nzi = numz - 2;
     const int nf = 2 * nri + nzi;
     const int fi = bx * 128 + threadIdx.x;
-    const int row = by;
+    const int row = a.kmap ? (by < a.kmap ? by : a.numphi / 2 + 1 + (by - a.kmap)) : by;
     if (a.X1 && fi == nf) {  // the outer corners of the side face are zero (bessel.F90:428, 456, 487)
         const long plane = (long)numr * numz;
         double *x1 = a.X1 + row * plane;
@@ -1521,12 +1521,15 @@ void face_modes_add_residual(Op *op, float *r, const float *rh, long rh_pitch, c
 
 Op face_modes_op(const double *Stb, const double *Ss, const double *besc, const double *bess, double *FF, int numr,
                  int numz, int numphi, int mmax, double factor, int nsplit_tb, double *X1, double *X2, int x2_klayout,
-                 int isym)
+                 int isym, bool reach_rows)
 {
     RebuildArgs a{Stb, Ss, besc, bess, FF, numr, numz, numphi, mmax, factor, nsplit_tb, X1, X2, x2_klayout,
                   isym == 3 ? 2 : 1, isym == 1 ? 1 : 0};
     const int nf = 2 * (numr - 2) + (numz - 2) + (X1 ? 1 : 0);  // + the corner thread
-    return Op::make(OP_FACE_MODES, 0, a, (nf + 127) / 128, numphi, 1);
+    // fstep = 1: slot m lands on cosine row m and sine row numphi/2 + m
+    // (m >= 1) alone while m < numphi / 2
+    if (reach_rows && isym != 3 && 2 * mmax <= numphi) a.kmap = mmax;
+    return Op::make(OP_FACE_MODES, 0, a, (nf + 127) / 128, a.kmap ? 2 * mmax - 1 : numphi, 1);
 }
 
 // ---------------------------------------------------------------------------

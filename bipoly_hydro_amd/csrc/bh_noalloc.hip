@@ -67,6 +67,7 @@ extern "C" int bh_set_potential_noalloc(bh_plan *p, const double *pot)
     if (rc != BH_OK) return rc;
     if ((rc = wait_plan_idle(p)) != BH_OK) return rc;  // a solve still running reads and writes XJ
     p->lazy_state = false;  // XJ is rewritten whole
+    p->faces_clean = false;
     std::vector<Op> ops;
     for (long c0 = 0; c0 < M; c0 += cols) {
         const long w = std::min(cols, M - c0);

@@ -131,6 +131,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     launch_fill(p->src_com, 6 * (size_t)N, 0.0, s);
     launch_fill(p->XJ, M * N, 0.0, s);
     launch_fill(p->XK, M * N, 0.0, s);
+    launch_fill(p->FF, (size_t)N * nf, 0.0, s);  // rows the face kernel may skip stay zero
     if (is_mixed(precision)) {
         launch_fillf(p->D1, (with_d2 ? 3 : 2) * M * N, 0.f, s);
     }
@@ -225,7 +226,7 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 // DFT beside the table stream instead the solve takes the same time: the
 // boundary window is bound by its total HBM traffic.)
 void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out,
-                  const ResidualArgs *face_res = nullptr)
+                  const ResidualArgs *face_res = nullptr, bool reach_rows = false)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
@@ -240,7 +241,7 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     // factor of bessel.F90:170-174 (doubled for isym = 3)
     const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax, factor, 1,
-                                 X1, X2, x2_klayout, g.isym));
+                                 X1, X2, x2_klayout, g.isym, reach_rows));
     if (face_res)
         face_modes_add_residual(&out->back(), face_res->r, face_res->rh, face_res->rh_pitch, face_res->alphar,
                                 face_res->gamma, face_res->four_pi);
@@ -314,7 +315,10 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     ra.cold = tstep <= 1;
     // (the fused spectral path never reads Xb: its faces are only needed by
     // the separate kernels' second iterate)
-    boundary_ops(p, Xa, o.spec_fused ? nullptr : Xb, 0, out, cold_split ? &ra : nullptr);  // joins the side stream
+    // (single-plan solves of a plan whose other rows' faces are zero: the
+    // face kernel visits the rows boundary modes reach)
+    boundary_ops(p, Xa, o.spec_fused ? nullptr : Xb, 0, out, cold_split ? &ra : nullptr,
+                 allow_lazy && o.spec_fused && p->faces_clean);  // joins the side stream
     out->push_back(Op::mark(2));
     if (!cold_split) {
         ResidualArgs r1 = ra;
@@ -704,6 +708,7 @@ int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hip
     build_ops(p, d_rho, d_pot, tstep, &ops, true);
     const int rc = enqueue_ops(p, ops, s);
     p->lazy_state = rc == BH_OK && lazy_cold(p, tstep);
+    p->faces_clean = rc == BH_OK;  // every row's faces formed (or kept at zero)
     return rc;
 }
 
@@ -801,6 +806,7 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
     int rc = wait_plan_idle(p);  // a solve still running reads and writes XJ
     if (rc != BH_OK) return rc;
     p->lazy_state = false;  // XJ is rewritten whole
+    p->faces_clean = false;
     if (!on_device) {
         rc = ensure_staging(p, n);
         if (rc != BH_OK) return rc;

@@ -176,6 +176,10 @@ struct bh_plan {
     // it there instead of writing it; XJ's faces are current.  The next warm
     // solve's first residual writes it out (or materialize_state does).
     bool lazy_state = false;
+    // the faces of the mode rows no boundary mode reaches are zero in XJ
+    // (as every solve leaves them; an import of the potential need not):
+    // single-plan mixed solves then form the faces of the other rows only
+    bool faces_clean = true;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;
     std::vector<bh::TimingSet> tsets;

@@ -487,6 +487,7 @@ void admit(bh_server *S)
                 std::vector<Op> ops;
                 dft_fwd64_ops(p, sh->pot, M, (int)M, p->XJ, M, false, &ops);
                 p->lazy_state = false;  // XJ is rewritten whole
+                p->faces_clean = false;
                 for (const Op &op : ops) launch_op(op, ls);
                 e = hipGetLastError();
             }

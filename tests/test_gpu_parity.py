@@ -379,6 +379,28 @@ def test_cold_mixed_state_kept_as_corrections():
         assert e < 1e-13
 
 
+@pytest.mark.parametrize("shape", [(66, 34, 512), (130, 66, 128)])
+def test_face_rows_no_mode_reaches(shape):
+    """Boundary modes m < mmax reach the faces of mode rows 0..mmax-1 and
+    their sine rows alone; a plan whose other rows' faces are zero (as every
+    solve leaves them) forms only those rows.  An import of an arbitrary
+    potential clears that, and the next solve forms every row: cold and warm
+    solves of both plans agree bit for bit."""
+    numr, numz, numphi = shape
+    rho1 = density.scaled_binary(numr, numz, numphi, seed=41)
+    rho2 = rho1 * 0.997
+    noise = np.random.default_rng(5).standard_normal(rho1.shape)
+    with P.PotentialPlan(numr, numz, numphi, precision="mixed") as a, \
+            P.PotentialPlan(numr, numz, numphi, precision="mixed") as b:
+        a1 = a.potential_solver(rho1, tstep=0)
+        a2 = a.potential_solver(rho2, tstep=2)
+        b.set_potential(noise)  # every row's faces nonzero
+        b1 = b.potential_solver(rho1, tstep=0)
+        b2 = b.potential_solver(rho2, tstep=2)
+    np.testing.assert_array_equal(b1, a1)
+    np.testing.assert_array_equal(b2, a2)
+
+
 @pytest.mark.parametrize("shape,seed", [((98, 34, 64), 11), ((290, 18, 16), 3)])
 def test_mixed_matches_oracle_on_other_grids(shape, seed):
     """The mixed path (FP32 table copy, FP32 transforms) off the golden
