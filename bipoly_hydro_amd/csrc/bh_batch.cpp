@@ -152,12 +152,6 @@ int batch_enqueue(bh_batch *b, const double *const *d_rho, double *const *d_pot,
 {
     const int n = (int)b->plans.size();
     b->lists.resize(n);
-    // a state left as a cold solve's corrections (a single-plan solve of a
-    // batch's plan) is written out first, outside any captured graph
-    for (int i = 0; i < n; ++i) {
-        const int rc = materialize_state(b->plans[i], s);
-        if (rc != BH_OK) return rc;
-    }
     for (int i = 0; i < n; ++i) build_ops(b->plans[i], d_rho[i], d_pot[i], tstep ? tstep[i] : 0, &b->lists[i]);
     if ((b->flags & BH_BATCH_AUTOTUNE) && !b->tune_init) init_tuning(b);
     const bool tuning = (b->flags & BH_BATCH_AUTOTUNE) && b->tune_ax < (int)b->tune_axes.size();
@@ -270,6 +264,7 @@ extern "C" int bh_batch_create(const bh_grid_desc *descs, int ngrids, int precis
         bh_plan *p = nullptr;
         int rc = plan_create(d[i], precision, 0, &p);
         if (rc != BH_OK) return fail(rc, "grid " + std::to_string(i) + ": " + last_error());
+        p->in_batch = true;  // its state is always kept whole (batches may replay captured launches)
         b->plans.push_back(p);
         b->bytes += p->mem.bytes;
         if (shared.insert(p->tab.get()).second) b->bytes += p->tab->mem.bytes;

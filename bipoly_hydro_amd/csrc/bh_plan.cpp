@@ -705,9 +705,9 @@ namespace {
 int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hipStream_t s)
 {
     std::vector<Op> ops;
-    build_ops(p, d_rho, d_pot, tstep, &ops, true);
+    build_ops(p, d_rho, d_pot, tstep, &ops, !p->in_batch);
     const int rc = enqueue_ops(p, ops, s);
-    p->lazy_state = rc == BH_OK && lazy_cold(p, tstep);
+    p->lazy_state = rc == BH_OK && !p->in_batch && lazy_cold(p, tstep);
     p->faces_clean = rc == BH_OK;  // every row's faces formed (or kept at zero)
     return rc;
 }

@@ -180,6 +180,10 @@ struct bh_plan {
     // (as every solve leaves them; an import of the potential need not):
     // single-plan mixed solves then form the faces of the other rows only
     bool faces_clean = true;
+    // owned by a batch: single-plan solves of it keep the state whole, so
+    // that the batch's (possibly captured and replayed) launches never meet
+    // a state left as corrections
+    bool in_batch = false;
     // timing (HIP events at the stage marks of each timed solve)
     bool timing = false;
     std::vector<bh::TimingSet> tsets;
