@@ -40,6 +40,9 @@ struct DftArgs {
     // equator keep their density (potential_solver.F90:76-78 zeroes them
     // for isym = 1 only)
     int mask_keep_bottom;
+    // the cold inverse that keeps the state as corrections (x not stored):
+    // *set_flag = 1 (see bh_plan::lazy_flag)
+    float *set_flag;
 };
 
 enum { DFT_FWD_F32 = 0, DFT_FWD_F64 = 1, DFT_INV_F64 = 2 };

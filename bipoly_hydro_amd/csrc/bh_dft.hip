@@ -768,6 +768,8 @@ template <int MODE>
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(4))) void dft_inv8_kernel(DftArgs a, int nx,
                                                                                              int ny, int total)
 {
+    if constexpr (MODE == INV_COLD_NS)
+        if (a.set_flag && blockIdx.x == 0 && threadIdx.x == 0) *a.set_flag = 1.f;
     extern __shared__ __align__(16) unsigned char inv8_lds[];
     inv8_loop<MODE>(total, [&](int t, DftArgs &o, int &bx) { o = a; bx = t; }, inv8_lds);
 }

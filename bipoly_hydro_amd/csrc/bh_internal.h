@@ -116,6 +116,9 @@ struct RebuildArgs {
     // faces of every other row are zero in every solve and already zero in
     // the potential buffers
     int kmap;
+    // flag (bh_plan::lazy_flag): one thread moves flag[0] to flag[1] and
+    // clears flag[0]
+    float *flag;
 };
 
 // grid-stride fill for plan set-up

@@ -1446,6 +1446,10 @@ __device__ __forceinline__ void face_modes_tile(const RebuildArgs &a, int bx, in
     const int nf = 2 * nri + nzi;
     const int fi = bx * 128 + threadIdx.x;
     const int row = a.kmap ? (by < a.kmap ? by : a.numphi / 2 + 1 + (by - a.kmap)) : by;
+    if (a.flag && bx == 0 && by == 0 && threadIdx.x == 0) {
+        a.flag[1] = a.flag[0];
+        a.flag[0] = 0.f;
+    }
     if (a.X1 && fi == nf) {  // the outer corners of the side face are zero (bessel.F90:428, 456, 487)
         const long plane = (long)numr * numz;
         double *x1 = a.X1 + row * plane;

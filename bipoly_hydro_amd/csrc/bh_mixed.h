@@ -45,6 +45,10 @@ struct ResidualArgs {
     // r0 = g - A x0 (FP32, compact like r), taken in place of X and rh, so
     // that r = r0 - w - A (dlt + dlt2) with zero faces (cold must be 1)
     const float *r0;
+    // with dlt, dlt2 and Xo (a warm start of a plan whose cold solves may
+    // keep the state as corrections): write x0 out only when *lazy_flag is
+    // set, else the plain first residual from X
+    const float *lazy_flag;
 };
 // Xo = X + (dlt + dlt2) on the interior (dlt2 may be null); cold: X is zero
 // on the interior and is not read

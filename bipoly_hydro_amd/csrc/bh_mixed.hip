@@ -30,8 +30,9 @@ namespace bh {
 template <int ROWS, bool EDGE>
 __device__ __forceinline__ void residual_tile_e(const ResidualArgs &a, int bx, int by, int bz)
 {
-    if (a.Xo && a.dlt2) {  // a warm start from a cold solve's two corrections (no w)
-        residual_tile_v<ROWS, true, true, EDGE, true, true, false, true>(a, bx, by, bz);
+    if (a.Xo && a.dlt2) {  // a warm start from a cold solve's two corrections (no w), if kept so
+        if (!a.lazy_flag || *a.lazy_flag != 0.f) residual_tile_v<ROWS, true, true, EDGE, true, true, false, true>(a, bx, by, bz);
+        else residual_tile_v<ROWS, false, false, EDGE, false>(a, bx, by, bz);
         return;
     }
     switch ((a.dlt ? (a.Xo ? 2 : a.dlt2 ? 6 : 4) : 0) + (a.cold ? 1 : 0)) {

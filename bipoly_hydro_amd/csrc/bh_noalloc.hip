@@ -6,6 +6,7 @@
 // streams the field through `scratch` in column chunks instead and calls no
 // allocating routine.
 #include "bh_plan_impl.h"
+#include "bh_mixed.h"
 
 #include <hip/hip_runtime.h>
 
@@ -68,6 +69,7 @@ extern "C" int bh_set_potential_noalloc(bh_plan *p, const double *pot)
     if ((rc = wait_plan_idle(p)) != BH_OK) return rc;  // a solve still running reads and writes XJ
     p->lazy_state = false;  // XJ is rewritten whole
     p->faces_clean = false;
+    if (p->lazy_flag) launch_fillf(p->lazy_flag, 1, 0.f, 0);
     std::vector<Op> ops;
     for (long c0 = 0; c0 < M; c0 += cols) {
         const long w = std::min(cols, M - c0);

@@ -76,6 +76,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     }
     ALLOC(p->Ss, (size_t)side_splits(numz) * mmax * 2 * nzi);
     ALLOC(p->FF, (size_t)N * nf);
+    if (is_mixed(precision)) ALLOC(p->lazy_flag, 64);
     ALLOC(p->src_grid, 4 * (size_t)(numr + 1));
     ALLOC(p->src_com, 6 * (size_t)N);
     const bool with_d2 = is_mixed(precision) && p->ops->spec_fused && p->passes >= 2;
@@ -132,6 +133,7 @@ int plan_create(const bh_grid_desc &d, int precision, size_t budget, bh_plan **o
     launch_fill(p->XJ, M * N, 0.0, s);
     launch_fill(p->XK, M * N, 0.0, s);
     launch_fill(p->FF, (size_t)N * nf, 0.0, s);  // rows the face kernel may skip stay zero
+    if (p->lazy_flag) launch_fillf(p->lazy_flag, 64, 0.f, s);
     if (is_mixed(precision)) {
         launch_fillf(p->D1, (with_d2 ? 3 : 2) * M * N, 0.f, s);
     }
@@ -226,7 +228,7 @@ void proj64_ops(const bh_plan *p, const double *rho, std::vector<Op> *out)
 // DFT beside the table stream instead the solve takes the same time: the
 // boundary window is bound by its total HBM traffic.)
 void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std::vector<Op> *out,
-                  const ResidualArgs *face_res = nullptr, bool reach_rows = false)
+                  const ResidualArgs *face_res = nullptr, bool reach_rows = false, float *flag = nullptr)
 {
     const Geo &g = p->g;
     const int numr = g.numr, numz = g.numz, N = g.numphi, mmax = g.mmax;
@@ -242,6 +244,11 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax, factor, 1,
                                  X1, X2, x2_klayout, g.isym, reach_rows));
+    if (flag) {
+        RebuildArgs fa = out->back().as<RebuildArgs>();
+        fa.flag = flag;
+        std::memcpy(out->back().args, &fa, sizeof fa);
+    }
     if (face_res)
         face_modes_add_residual(&out->back(), face_res->r, face_res->rh, face_res->rh_pitch, face_res->alphar,
                                 face_res->gamma, face_res->four_pi);
@@ -253,6 +260,14 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
 // the second correction of a mixed solve: D2 with two or more passes, the w
 // buffer at one pass (free once the one refinement residual has read w)
 float *correction_sum(const bh_plan *p) { return p->passes >= 2 ? p->D2 : p->Wf; }
+
+// a mixed plan whose cold solves at N = 512 may keep the state as their
+// corrections (single-plan solves; not a plan a batch owns)
+bool keeps_corrections(const bh_plan *p)
+{
+    return is_mixed(p->precision) && p->ops->spec_fused && p->ops->dinv64 && p->use_dft &&
+           p->g.numphi == DFT_MAX_N && !p->in_batch && p->lazy_flag;
+}
 
 void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, std::vector<Op> *out,
                bool allow_lazy)
@@ -318,18 +333,22 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
     // (single-plan solves of a plan whose other rows' faces are zero: the
     // face kernel visits the rows boundary modes reach)
     boundary_ops(p, Xa, o.spec_fused ? nullptr : Xb, 0, out, cold_split ? &ra : nullptr,
-                 allow_lazy && o.spec_fused && p->faces_clean);  // joins the side stream
+                 allow_lazy && o.spec_fused && p->faces_clean,
+                 keeps_corrections(p) ? p->lazy_flag : nullptr);  // joins the side stream
     out->push_back(Op::mark(2));
     if (!cold_split) {
         ResidualArgs r1 = ra;
-        if (tstep > 1 && p->lazy_state) {
-            // x0 inside is the last (cold) solve's delta1 + correction sum:
-            // formed here, written to Xa, the same x0 as if stored whole
+        if (tstep > 1 && keeps_corrections(p)) {
+            // when the last solve was a cold one that kept the state as its
+            // delta1 + correction sum (lazy_flag, read on the device, so that
+            // a replayed graph decides afresh): x0 inside is formed from
+            // them and written to Xa, the same x0 as if stored whole
             r1.dlt = p->D1;
             r1.dlt2 = correction_sum(p);
             r1.Xo = Xa;
             r1.w = nullptr;
             r1.cold = 1;
+            r1.lazy_flag = p->lazy_flag + 1;
         }
         out->push_back(residual_op(r1, N));
     }
@@ -405,7 +424,10 @@ void mixed_ops(const bh_plan *p, const double *d_rho, double *d_pot, int tstep, 
             d.warm = cold ? 0 : 1;
             // a lazy cold start keeps x inside as delta1 and Dacc, which
             // stay in place (the faces are in Xa already): x is not stored
-            if (allow_lazy && lazy_cold(p, tstep)) d.xstore = nullptr;
+            if (allow_lazy && lazy_cold(p, tstep) && keeps_corrections(p)) {
+                d.xstore = nullptr;
+                d.set_flag = p->lazy_flag;
+            }
             d.axis_numr = numr;
             out->push_back(dft_op(d, DFT_INV_F64));
         } else {
@@ -685,8 +707,10 @@ int materialize_state(bh_plan *p, hipStream_t s)
     p->timing = false;  // not a solve: no stage events
     const int rc = enqueue_ops(p, ops, s);
     p->timing = timing;
-    if (rc == BH_OK) p->lazy_state = false;
-    return rc;
+    if (rc != BH_OK) return rc;
+    launch_fillf(p->lazy_flag, 1, 0.f, s);  // written out: the next warm start reads it whole
+    p->lazy_state = false;
+    return BH_OK;
 }
 
 int wait_plan_idle(bh_plan *p)
@@ -705,9 +729,17 @@ namespace {
 int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hipStream_t s)
 {
     std::vector<Op> ops;
-    build_ops(p, d_rho, d_pot, tstep, &ops, !p->in_batch);
+    // a solve captured into a graph may be replayed after other calls (an
+    // import among them): it keeps the state whole and forms every row's
+    // faces.  Whether its warm start writes out a state left as corrections
+    // is decided on the device (lazy_flag) at each replay.
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s) HIPCHK(hipStreamIsCapturing(s, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    const bool lazy_ok = !p->in_batch && !capturing;
+    build_ops(p, d_rho, d_pot, tstep, &ops, lazy_ok);
     const int rc = enqueue_ops(p, ops, s);
-    p->lazy_state = rc == BH_OK && !p->in_batch && lazy_cold(p, tstep);
+    p->lazy_state = rc == BH_OK && lazy_ok && lazy_cold(p, tstep) && keeps_corrections(p);
     p->faces_clean = rc == BH_OK;  // every row's faces formed (or kept at zero)
     return rc;
 }
@@ -807,6 +839,7 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
     if (rc != BH_OK) return rc;
     p->lazy_state = false;  // XJ is rewritten whole
     p->faces_clean = false;
+    if (p->lazy_flag) launch_fillf(p->lazy_flag, 1, 0.f, 0);
     if (!on_device) {
         rc = ensure_staging(p, n);
         if (rc != BH_OK) return rc;

@@ -176,6 +176,12 @@ struct bh_plan {
     // it there instead of writing it; XJ's faces are current.  The next warm
     // solve's first residual writes it out (or materialize_state does).
     bool lazy_state = false;
+    // the same on the device, for solves captured into graphs and replayed
+    // (mixed plans that may keep the state so): [0] set by the cold inverse
+    // that keeps it, [1] its value when the current solve began (the face
+    // kernel moves [0] to [1] and clears [0]); a warm first residual writes
+    // the state out when [1] is set
+    float *lazy_flag = nullptr;
     // the faces of the mode rows no boundary mode reaches are zero in XJ
     // (as every solve leaves them; an import of the potential need not):
     // single-plan mixed solves then form the faces of the other rows only

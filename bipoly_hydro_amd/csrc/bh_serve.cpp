@@ -19,6 +19,7 @@
 #include "bh_density.h"
 #include "bh_exec.h"
 #include "bh_plan_impl.h"
+#include "bh_mixed.h"
 
 #include <hip/hip_runtime.h>
 
@@ -488,6 +489,7 @@ void admit(bh_server *S)
                 dft_fwd64_ops(p, sh->pot, M, (int)M, p->XJ, M, false, &ops);
                 p->lazy_state = false;  // XJ is rewritten whole
                 p->faces_clean = false;
+                if (p->lazy_flag) launch_fillf(p->lazy_flag, 1, 0.f, ls);
                 for (const Op &op : ops) launch_op(op, ls);
                 e = hipGetLastError();
             }

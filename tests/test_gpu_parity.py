@@ -379,6 +379,39 @@ def test_cold_mixed_state_kept_as_corrections():
         assert e < 1e-13
 
 
+def test_captured_warm_solve_replays_match_eager():
+    """A warm solve captured into a graph right after a cold solve (which
+    left its state as corrections): the capture writes the state out first
+    and keeps every later state whole, so each replay (eager work and a
+    synchronisation between them) equals, bit for bit, the matching warm
+    solve of an eager cold, warm, warm, warm sequence on another plan."""
+    torch = pytest.importorskip("torch")
+    numr, numz, numphi = 66, 34, 512
+    rho1 = density.scaled_binary(numr, numz, numphi, seed=51)
+    rho2 = rho1 * 1.004
+    d1, d2 = torch.from_numpy(rho1).cuda(), torch.from_numpy(rho2).cuda()
+    st = torch.cuda.Stream()
+    with P.PotentialPlan(numr, numz, numphi, precision="mixed") as a, \
+            P.PotentialPlan(numr, numz, numphi, precision="mixed") as b:
+        ref = []
+        out = torch.empty_like(d1)
+        b.potential_solver(rho1, tstep=0)
+        for _ in range(3):
+            ref.append(b.potential_solver(rho2, tstep=2))
+        pa = torch.empty_like(d1)
+        a.potential_solver_device(d1.data_ptr(), pa.data_ptr(), 0, st.cuda_stream)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=st, capture_error_mode="thread_local"):
+            a.potential_solver_device(d2.data_ptr(), out.data_ptr(), 2, st.cuda_stream)
+        for k in range(3):
+            g.replay()
+            torch.cuda.synchronize()
+            pa.add_(1.0)  # eager work between replays
+            torch.cuda.synchronize()
+            np.testing.assert_array_equal(out.cpu().numpy(), ref[k])
+
+
 @pytest.mark.parametrize("shape", [(66, 34, 512), (130, 66, 128)])
 def test_face_rows_no_mode_reaches(shape):
     """Boundary modes m < mmax reach the faces of mode rows 0..mmax-1 and
