@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -744,6 +745,19 @@ int enqueue_solve(bh_plan *p, const double *d_rho, double *d_pot, int tstep, hip
     return rc;
 }
 
+// the host-path copy threads and their pinned ring (no exception leaves
+// the library: thread or memory exhaustion is an error code)
+int ensure_stager(bh_plan *p)
+{
+    if (p->stager) return BH_OK;
+    try {
+        p->stager.reset(new HostStager(p->device, host_copy_workers()));
+    } catch (const std::exception &e) {
+        return fail(BH_E_DEVICE, std::string("could not start the host copy threads: ") + e.what());
+    }
+    return BH_OK;
+}
+
 int ensure_staging(bh_plan *p, size_t n)
 {
     if (p->rho_in) return BH_OK;
@@ -813,7 +827,7 @@ extern "C" int bh_solve(bh_plan *p, const double *rho, double *pot, int tstep)
     const size_t n = (size_t)p->g.numr * p->g.numz * p->g.numphi;
     int rc = ensure_staging(p, n);
     if (rc != BH_OK) return rc;
-    if (!p->stager) p->stager.reset(new HostStager(p->device, host_copy_workers()));
+    if ((rc = ensure_stager(p)) != BH_OK) return rc;
     // rho in through the pinned ring, checked for finiteness as it is
     // copied; the solve is enqueued behind the transfers and phi comes back
     // the same way
@@ -843,7 +857,7 @@ extern "C" int bh_set_potential(bh_plan *p, const double *pot, int on_device)
     if (!on_device) {
         rc = ensure_staging(p, n);
         if (rc != BH_OK) return rc;
-        if (!p->stager) p->stager.reset(new HostStager(p->device, host_copy_workers()));
+        if ((rc = ensure_stager(p)) != BH_OK) return rc;
         rc = p->stager->to_device(p->pot_out, pot, n, 0, false, nullptr);  // through the pinned ring
         if (rc != BH_OK) return fail(rc, "copy of the potential to the device failed");
         src = p->pot_out;
