@@ -22,12 +22,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Poisson solves/s @258×130×512 mixed vs FP64 at relL2≤1e-8; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per solve of the boundary contraction sequence (contract, with
-# the projections scaled as they are read, then reduce) and of one FP64 ADI
+# HBM bytes per solve of the boundary contraction (with the projections
+# scaled as they are read; its partials are summed by the face kernel) and of one FP64 ADI
 # half-step at 258 x 130 x 512, from rocprofv3 FETCH_SIZE (x 2.000, the
 # copy-kernel calibration) + WRITE_SIZE;
 # profiles/potential_solver_258x130x512_hbm_traffic.md
-CONTRACT_TRAFFIC = ((546340.0 + 2332.0) * 2.0 + (7200.0 + 240.0)) * 1024  # FP64 table
+CONTRACT_TRAFFIC = (546340.0 * 2.0 + 7200.0) * 1024  # FP64 table
 # the mixed cold inverse DFT (radix 8, x = faces + delta1 + correction sum
 # in, phi out; the warm-start state stays as the two corrections): HBM bytes
 # per launch (FETCH x 2.000 + WRITE) and the FP64 MFMA operations it executes
@@ -297,8 +297,8 @@ def main():
     # is stored in blocked symmetric form (16 x 16 blocks (I, J), J <= I, of
     # targets x sources per source plane and mode): mmax x NB(NB+1)/2 x
     # (numz-2) x 256 x 8 B, NB = ceil((numr-2)/16).  The timed
-    # span (events around it) also holds the projection scaling and the final
-    # reduction.
+    # span (events around it) is the contraction alone (the projections are
+    # scaled as they are read; the face kernel sums its partials).
     nri, nzi, mmax = numr - 2, numz - 2, 30
     nb = (nri + 15) // 16
     ct_bytes = mmax * nb * (nb + 1) // 2 * nzi * 256 * 8
@@ -378,7 +378,7 @@ def main():
         "fp64_spectral": s64,
         "boundary_modes": modes,
         "mixed_over_fp64_spectral": s64["ms_per_step"] / mixed["ms_per_step"],
-        "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (contract with the projections scaled as read, then reduce)",
+        "roofline": {"bound": "hbm", "kernel": "boundary Green's-table contraction (contract_tb64, the projections scaled as read)",
                      "achieved": ct_bytes / (ct_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": ct_bytes / (ct_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": ct_traffic,
                      "bytes_per_launch": ct_bytes, "launch_ms": ct_ms},

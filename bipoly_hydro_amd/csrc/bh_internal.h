@@ -119,6 +119,11 @@ struct RebuildArgs {
     // flag (bh_plan::lazy_flag): one thread moves flag[0] to flag[1] and
     // clears flag[0]
     float *flag;
+    // Ptb (with sqrt_rhf): the top and bottom sums are taken straight from
+    // the contraction's nk partials [m][k][16 NB][4] (summed in k order and
+    // divided by sqrt(rhf), as the reduction kernel would), Stb unused
+    const double *Ptb, *sqrt_rhf;
+    int nk;
 };
 
 // grid-stride fill for plan set-up

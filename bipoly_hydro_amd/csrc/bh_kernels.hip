@@ -1406,12 +1406,31 @@ __device__ double face_mode_value(const RebuildArgs &a, int fi, int row)
     else { idx = fi - 2 * nri; sc = Ss; ss = Ss + nzi; st = 2 * (long)nzi; nsp = side_splits(numz); }
     sst = st * mmax;
     // partial sums of the source ranges of the contraction, in a fixed order
+    // top (q = 0, 1) and bottom (q = 2, 3) straight from the contraction's
+    // partials when Ptb is given: the reduction's sum, in its order
+    const bool fromP = a.Ptb && fi < 2 * nri;
+    const int q0 = fi < nri ? 0 : 2;
+    const int NB16 = 16 * ((nri + 15) / 16);
+    auto S_p = [&](int m, int q) {  // eight loads in flight at a time
+        const double *P = a.Ptb + ((long)m * a.nk * NB16 + idx) * 4 + q;
+        double s = 0.0;
+        for (int k0 = 0; k0 < a.nk; k0 += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = k0 + u < a.nk ? P[(long)(k0 + u) * NB16 * 4] : 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) s += v[u];
+        }
+        return s / a.sqrt_rhf[idx + 2];
+    };
     auto S_c = [&](int m) {
+        if (fromP) return S_p(m, q0);
         double v = sc[m * st + idx];
         for (int p = 1; p < nsp; ++p) v += sc[p * sst + m * st + idx];
         return v;
     };
     auto S_s = [&](int m) {
+        if (fromP) return S_p(m, q0 + 1);
         double v = ss[m * st + idx];
         for (int p = 1; p < nsp; ++p) v += ss[p * sst + m * st + idx];
         return v;

@@ -239,15 +239,20 @@ void boundary_ops(const bh_plan *p, double *X1, double *X2, int x2_klayout, std:
     if (!tb_blocked(numr)) out->push_back(topsym_scale_op(p->proj, p->tab->sqrt_rhf, p->Pt, numr, numz, mmax));
     out->push_back(contract_topsym_op(p->tab->T, p->tab->T32, p->proj, p->tab->sqrt_rhf, p->Pt, p->Ptb, numr, numz,
                                       mmax));
-    out->push_back(topsym_reduce_op(p->Ptb, p->tab->sqrt_rhf, p->Stb, numr, numz, mmax));
+    // (no reduction launch: the face kernel sums the contraction's
+    // partials itself, in the reduction's order)
     out->push_back(Op::mark(6));
     // factor of bessel.F90:170-174 (doubled for isym = 3)
     const double factor = g.isym == 3 ? -2.0 * g.dr * g.dz * g.dphi : -g.dr * g.dz * g.dphi;
     out->push_back(face_modes_op(p->Stb, p->Ss, p->ops->besc, p->ops->bess, p->FF, numr, numz, N, mmax, factor, 1,
                                  X1, X2, x2_klayout, g.isym, reach_rows));
-    if (flag) {
+    {
         RebuildArgs fa = out->back().as<RebuildArgs>();
         fa.flag = flag;
+        fa.Ptb = p->Ptb;
+        fa.sqrt_rhf = p->tab->sqrt_rhf;
+        const int NB = (numr - 2 + 15) / 16;
+        fa.nk = tb_blocked(numr) ? tb_chunks(numz) : topsym_splits(numz) * ((NB + 1) / 2);
         std::memcpy(out->back().args, &fa, sizeof fa);
     }
     if (face_res)
